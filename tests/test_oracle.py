@@ -1,0 +1,453 @@
+"""Pins the CPU oracle (oracle/oracle.cpp) before it is trusted as a checker.
+
+Each check is chosen so that a plausible mistake in the restatement (a dropped
+term, a wrong sign or index, a transposed operand) fails: golden tables from
+the reference's own unit tests, brute force on small inputs, conservation
+laws and closed forms.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+EPS = np.finfo(np.float64).eps
+
+
+def _close_fraction(x, ref, tol):
+    # BOOST_CHECK_CLOSE_FRACTION (strong): relative to both operands
+    if ref == 0.0 or x == 0.0:
+        return x == ref
+    d = abs(x - ref)
+    return d <= tol * abs(ref) and d <= tol * abs(x)
+
+
+# ----------------------------------------------------------------------------
+# potentials: reference golden tables
+# ----------------------------------------------------------------------------
+@pytest.mark.parametrize("truncation", ["sharp", "shifted", "force_shifted", "smooth_r4"])
+def test_lennard_jones_tables(oracle, truncation):
+    g = json.load(open(os.path.join(GOLDEN, "potential_tables.json")))["lennard_jones"]
+    pot = oracle.Potential(oracle.POT_LJ, truncation, g["epsilon"], g["sigma"], g["r_cut_sigma"], h=g["h"])
+    tol = g["tolerance_eps"] * EPS
+    for pair, (a, b) in (("aa", (0, 0)), ("ab", (0, 1)), ("bb", (1, 1))):
+        for r, fref, uref in g["tables"][truncation][pair]:
+            f, u = pot(r * r, a, b)
+            assert _close_fraction(f[0], fref, tol), (truncation, pair, r, f[0], fref)
+            assert _close_fraction(u[0], uref, tol), (truncation, pair, r, u[0], uref)
+
+
+def test_morse_tables(oracle):
+    g = json.load(open(os.path.join(GOLDEN, "potential_tables.json")))["morse"]
+    pot = oracle.Potential(oracle.POT_MORSE, g["truncation"], g["epsilon"], g["sigma"], g["r_cut_sigma"],
+                           r_min=g["r_min"], distortion=g["distortion"])
+    for pair, (a, b) in (("aa", (0, 0)), ("ab", (0, 1)), ("bb", (1, 1))):
+        tol = g["tolerance_eps"][pair] * EPS
+        for r, fref, uref in g["tables"][pair]:
+            f, u = pot(r * r, a, b)
+            assert _close_fraction(f[0], fref, 2 * tol), (pair, r, f[0], fref)
+            assert _close_fraction(u[0], uref, tol), (pair, r, u[0], uref)
+
+
+def test_morse_derivative(oracle):
+    # fval = -U'(r)/r by central differences (the check the reference's
+    # compute_potential_reference.py makes), for the Morse parameters of the
+    # batched workload: D=1, a=6/sigma (sigma_M = 1/6), r0 = 1, B = 1
+    pot = oracle.Potential(oracle.POT_MORSE, "shifted", 1.0, 1.0 / 6, 15.0, r_min=1.0, distortion=1.0)
+    for r in (0.8, 0.95, 1.0, 1.1, 1.5, 2.2):
+        h = 1e-6
+        _, up = pot((r + h) ** 2, 0, 0)
+        _, um = pot((r - h) ** 2, 0, 0)
+        f, _ = pot(r * r, 0, 0)
+        assert abs(f[0] - (-(up[0] - um[0]) / (2 * h) / r)) < 1e-6 * max(1.0, abs(f[0]))
+
+
+# ----------------------------------------------------------------------------
+# helper configurations
+# ----------------------------------------------------------------------------
+def _liquid(oracle, n, rho, seed, jitter=0.2, species_frac=None):
+    L = np.full(3, (n / rho) ** (1 / 3))
+    pos, a = oracle.lattice_fcc(n, L)
+    rng = np.random.default_rng(seed)
+    pos = pos + rng.uniform(-jitter, jitter, pos.shape) * a / 4
+    pos = (pos + L / 2) % L - L / 2
+    species = np.zeros(n, dtype=np.int32)
+    if species_frac is not None:
+        nb = int(round(species_frac * n))
+        species[rng.permutation(n)[:nb]] = 1
+    return pos, L, species, rng
+
+
+def _ka_potential(oracle):
+    eps = [[1.0, 1.5], [1.5, 0.5]]
+    sig = [[1.0, 0.8], [0.8, 0.88]]
+    return oracle.Potential(oracle.POT_LJ, "shifted", eps, sig, 2.5)
+
+
+# ----------------------------------------------------------------------------
+# forces
+# ----------------------------------------------------------------------------
+@pytest.mark.parametrize("truncation", ["shifted", "force_shifted", "smooth_r4", "sharp"])
+def test_force_half_list_equals_brute_force(oracle, truncation):
+    pos, L, species, _ = _liquid(oracle, 864, 0.8442, 1)
+    pot = oracle.Potential(oracle.POT_LJ, truncation, 1.0, 1.0, 2.5)
+    f, u, s = oracle.pair_force(pos, species, L, pot, skin=0.3)
+    fb, ub, sb = oracle.pair_force_brute(pos, species, L, pot)
+    scale = np.sqrt(np.mean(np.sum(fb ** 2, axis=1)))
+    assert np.max(np.abs(f - fb)) < 1e-12 * scale
+    assert np.max(np.abs(u - ub)) < 1e-12
+    assert np.max(np.abs(s - sb)) < 1e-11
+    # Newton's third law
+    assert np.max(np.abs(f.sum(axis=0))) < 1e-10 * scale
+
+
+def test_force_binary_mixture_brute(oracle):
+    pos, L, species, _ = _liquid(oracle, 1372, 1.2, 2, species_frac=0.2)
+    pot = _ka_potential(oracle)
+    f, u, s = oracle.pair_force(pos, species, L, pot, skin=0.3)
+    fb, ub, sb = oracle.pair_force_brute(pos, species, L, pot)
+    scale = np.sqrt(np.mean(np.sum(fb ** 2, axis=1)))
+    assert np.max(np.abs(f - fb)) < 1e-12 * scale
+    assert np.max(np.abs(u - ub)) < 1e-12
+
+
+def test_force_is_minus_energy_gradient(oracle):
+    # smooth_r4 makes U C^2, so central differences converge cleanly
+    pos, L, species, rng = _liquid(oracle, 500, 0.8442, 3)
+    pot = oracle.Potential(oracle.POT_LJ, "smooth_r4", 1.0, 1.0, 2.5, h=0.005)
+    f, u, _ = oracle.pair_force_brute(pos, species, L, pot)
+    h = 1e-6
+    for i in rng.choice(len(pos), 5, replace=False):
+        for k in range(3):
+            p1 = pos.copy(); p1[i, k] += h
+            p2 = pos.copy(); p2[i, k] -= h
+            _, u1, _ = oracle.pair_force_brute(p1, species, L, pot)
+            _, u2, _ = oracle.pair_force_brute(p2, species, L, pot)
+            fd = -(u1.sum() - u2.sum()) / (2 * h)
+            assert abs(fd - f[i, k]) < 1e-5 * max(1.0, abs(f[i, k]))
+
+
+def test_force_two_particles_closed_form(oracle):
+    # two particles at distance r along x: F = fval * r, U_i = U/2 each
+    L = np.full(3, 20.0)
+    pot = oracle.Potential(oracle.POT_LJ, "shifted", 1.0, 1.0, 2.5)
+    for r in (0.95, 1.12, 1.7, 2.4):
+        pos = np.array([[0.0, 0.0, 0.0], [r, 0.0, 0.0]])
+        f, u, s = oracle.pair_force_brute(pos, [0, 0], L, pot)
+        fval, uu = pot(r * r, 0, 0)
+        assert np.allclose(f[0], [fval[0] * (-r), 0, 0], rtol=1e-14)
+        assert np.allclose(f[1], [fval[0] * r, 0, 0], rtol=1e-14)
+        assert np.allclose(u, [uu[0] / 2, uu[0] / 2], rtol=1e-14)
+        assert np.isclose(s[0, 0], 0.5 * fval[0] * r * r, rtol=1e-14)
+
+
+# ----------------------------------------------------------------------------
+# binning and neighbour lists
+# ----------------------------------------------------------------------------
+@pytest.mark.parametrize("scale", [0.0, 0.5, 0.9])
+def test_binning_invariant(oracle, scale):
+    # sine-compressed fcc lattice as in test/unit/mdsim/binning.cpp:74-98; every
+    # particle is binned exactly once, into floor((r + L/2) * ncell / L) up to the
+    # periodic offset of the host rule (index = floor(r/len) mod ncell)
+    n = 4000
+    L = np.array([17.0, 19.0, 23.0])
+    pos, _ = oracle.lattice_fcc(n, L)
+    k = 2 * np.pi / L
+    pos = pos + (scale / k) * np.sin(k * pos)
+    ncell = oracle.binning_ncell(L, 2.5, 0.3)
+    assert list(ncell) == [int(L[0] // 2.8), int(L[1] // 2.8), int(L[2] // 2.8)]
+    cell = oracle.binning_cell(pos, L, 2.5, 0.3)
+    idx = np.floor(pos * ncell / L).astype(np.int64) % ncell
+    expect = (idx[:, 2] * ncell[1] + idx[:, 1]) * ncell[0] + idx[:, 0]
+    assert np.array_equal(cell, expect)
+    assert np.bincount(cell, minlength=int(np.prod(ncell))).sum() == n
+
+
+def _brute_pairs(pos, L, species, rr_cut_skin, strict=True):
+    d = pos[:, None, :] - pos[None, :, :]
+    d -= L * np.where(d > L / 2, 1, np.where(d < -L / 2, -1, 0))
+    rr = np.sum(d * d, axis=-1)
+    cut = rr_cut_skin[species[:, None], species[None, :]]
+    m = (rr < cut) if strict else (rr <= cut)
+    np.fill_diagonal(m, False)
+    return m
+
+
+def test_neighbour_half_list_is_brute_force_set(oracle):
+    pos, L, species, _ = _liquid(oracle, 1372, 1.2, 4, species_frac=0.2)
+    pot = _ka_potential(oracle)
+    off, lst = oracle.neighbour_half(pos, species, L, pot, 0.3)
+    rcs = (pot.r_cut + 0.3) ** 2
+    m = _brute_pairs(pos, L, species, rcs)
+    got = np.zeros_like(m)
+    for i in range(len(pos)):
+        js = lst[off[i]:off[i + 1]]
+        assert len(np.unique(js)) == len(js)
+        got[i, js] = True
+    assert not np.any(got & got.T)            # each pair once
+    assert np.array_equal(got | got.T, m)     # and every pair
+
+
+def test_fp32_neighbour_list_rules(oracle):
+    # device rule: full lists; the set equals the FP32 predicate on all pairs,
+    # the lists are symmetric, and entries follow the documented cell order
+    pos, L, species, _ = _liquid(oracle, 2048, 0.8442, 5, species_frac=0.3)
+    pot = _ka_potential(oracle)
+    skin = 0.3
+    rcmax = float(np.max(pot.r_cut))
+    ncell = np.floor(L / (rcmax + skin)).astype(np.uint32)
+    cl = (L / ncell).astype(np.float32)
+    pos4 = np.zeros((len(pos), 4), dtype=np.float32)
+    pos4[:, :3] = pos.astype(np.float32)
+    pos4[:, 3] = species.astype(np.uint32).view(np.float32)
+    rcs = ((pot.r_cut + skin) ** 2).astype(np.float32)
+    counts, lst, m = oracle.fp32_neighbour_list(pos4, ncell, cl, L.astype(np.float32), rcs, 256)
+    assert m <= 256
+    p = pos4[:, :3]
+    Lf = L.astype(np.float32)
+    d = p[:, None, :] - p[None, :, :]
+    d = d - np.rint(d / Lf) * Lf
+    rr = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+    expect = rr <= rcs[species[:, None], species[None, :]]
+    np.fill_diagonal(expect, False)
+    got = np.zeros_like(expect)
+    for i in range(len(p)):
+        got[i, lst[i, :counts[i]]] = True
+        assert len(np.unique(lst[i, :counts[i]])) == counts[i]
+    assert np.array_equal(got, expect)
+    assert np.array_equal(got, got.T)
+    # order: visiting cells j, -j for the 13 offsets preceding 0, then own cell
+    cell = oracle.fp32_cell_index(pos4, ncell, cl)
+    cxyz = np.stack([cell % ncell[0], (cell // ncell[0]) % ncell[1], cell // ncell[0] // ncell[1]], 1).astype(int)
+    order = []
+    for dx in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dz in (-1, 0, 1):
+                if (dx, dy, dz) == (0, 0, 0):
+                    break
+                order += [(dx, dy, dz), (-dx, -dy, -dz)]
+            else:
+                continue
+            break
+        else:
+            continue
+        break
+    order.append((0, 0, 0))
+    assert len(order) == 27
+    rank = {o: k for k, o in enumerate(order)}
+    for i in range(0, len(p), 97):
+        js = lst[i, :counts[i]]
+        off = (cxyz[js] - cxyz[i] + 1) % ncell.astype(int) - 1
+        keys = [(rank[tuple(o)], j) for o, j in zip(off, js)]
+        assert keys == sorted(keys)
+
+
+def test_fp32_cell_index_matches_floor_rule(oracle):
+    n = 4000
+    L = np.array([16.79, 16.79, 16.79])
+    pos, _ = oracle.lattice_fcc(n, L)
+    pos4 = np.zeros((n, 4), np.float32)
+    pos4[:, :3] = pos
+    ncell = np.array([5, 5, 5], np.uint32)
+    cl = (L / ncell).astype(np.float32)
+    c = oracle.fp32_cell_index(pos4, ncell, cl)
+    # numpy float32 restatement of the same operations
+    q = (pos4[:, :3] / cl).astype(np.float32) + ncell.astype(np.float32)
+    idx = q.astype(np.uint32) % ncell
+    assert np.array_equal(c, (idx[:, 2] * ncell[1] + idx[:, 1]) * ncell[0] + idx[:, 0])
+
+
+# ----------------------------------------------------------------------------
+# Hilbert curve
+# ----------------------------------------------------------------------------
+@pytest.mark.parametrize("depth", [1, 2, 3, 4])
+def test_hilbert_curve_is_a_continuous_bijection(oracle, depth):
+    # cells of the 2^depth grid (in fractional coordinates frac(r/L), which the
+    # map takes first) map one-to-one onto codes 0 .. 8^depth-1, and consecutive
+    # codes are face-adjacent cells: the defining property of a Hilbert curve,
+    # which any wrong vertex-table entry breaks
+    n = 1 << depth
+    codes = {}
+    for x in range(n):
+        for y in range(n):
+            for z in range(n):
+                c = oracle.hilbert_map_f64((x + 0.5) / n, (y + 0.5) / n, (z + 0.5) / n, depth)
+                codes[c] = (x, y, z)
+    assert sorted(codes) == list(range(n ** 3))
+    for c in range(n ** 3 - 1):
+        a, b = np.array(codes[c]), np.array(codes[c + 1])
+        assert np.sum(np.abs(a - b)) == 1
+
+
+def test_hilbert_fp32_code_snaps_to_cell(oracle):
+    L = np.array([107.49, 107.49, 107.49])
+    depth = oracle.hilbert_depth(L)
+    assert depth == 7
+    rng = np.random.default_rng(0)
+    pos4 = np.zeros((5000, 4), np.float32)
+    pos4[:, :3] = rng.uniform(-L / 2, L / 2, (5000, 3))
+    code = oracle.fp32_hilbert_code(pos4, L, depth)
+    n = 1 << depth
+    u = (pos4[:, :3] / L.astype(np.float32)).astype(np.float64)
+    cellf = np.floor((u - np.floor(u)) * n)
+    for i in range(0, 5000, 50):
+        x, y, z = cellf[i]
+        ref = oracle.hilbert_map_f64((x + 0.5) / n, (y + 0.5) / n, (z + 0.5) / n, depth)
+        assert code[i] == ref
+
+
+# ----------------------------------------------------------------------------
+# lattice
+# ----------------------------------------------------------------------------
+def test_fcc_lattice(oracle):
+    n = 4000
+    L = np.full(3, (n / 0.8442) ** (1 / 3))
+    pos, a = oracle.lattice_fcc(n, L)
+    assert np.isclose(a, L[0] / 10)
+    assert np.all(pos >= -L / 2) and np.all(pos < L / 2)
+    # fcc: 12 nearest neighbours at a/sqrt(2)
+    d = pos[:50, None, :] - pos[None, :, :]
+    d -= L * np.rint(d / L)
+    r = np.sqrt(np.sum(d * d, -1))
+    r[np.arange(50), np.arange(50)] = np.inf
+    assert np.allclose(np.sort(r, axis=1)[:, :12], a / np.sqrt(2))
+    assert np.all(np.sort(r, axis=1)[:, 12] > a / np.sqrt(2) * 1.2)
+    # first site of each unit cell and its x-fastest cell order
+    assert np.allclose(pos[0], -L / 2 + a * np.array([0.25, 0.25, 0.25]))
+    assert np.allclose(pos[4], -L / 2 + a * np.array([1.25, 0.25, 0.25]))
+
+
+# ----------------------------------------------------------------------------
+# integrators
+# ----------------------------------------------------------------------------
+def test_verlet_ideal_gas(oracle):
+    # test/unit/mdsim/integrators/verlet.cpp:106-140: no interactions, E_kin and
+    # momentum exactly conserved, r advances linearly (mod L)
+    n = 1000
+    L = np.full(3, 40.0)
+    rng = np.random.default_rng(7)
+    pos = rng.uniform(-L / 2, L / 2, (n, 3))
+    vel = rng.normal(0, 1, (n, 3))
+    vel -= vel.mean(0)
+    pot = oracle.Potential(oracle.POT_LJ, "shifted", 1e-30, 1.0, 2.5)   # negligible forces
+    sim = oracle.Simulation(pos, vel, np.zeros(n, np.int32), np.ones(n), L, pot, 0.3, 0.001)
+    sim.run(1000)
+    st = sim.state()
+    expect = pos + vel * 1.0
+    unwrapped = st["position"] + st["image"] * L
+    assert np.allclose(unwrapped, expect, atol=1e-9)
+    assert np.allclose(st["velocity"], vel, atol=1e-12)
+
+
+def test_nve_energy_conservation(oracle):
+    n = 864
+    pos, L, species, rng = _liquid(oracle, n, 0.8442, 8, jitter=0.0)
+    vel = rng.normal(0, np.sqrt(1.2), (n, 3))
+    vel -= vel.mean(0)
+    pot = oracle.Potential(oracle.POT_LJ, "shifted", 1.0, 1.0, 2.5)
+    sim = oracle.Simulation(pos, vel, species, np.ones(n), L, pot, 0.3, 0.002)
+    rec = sim.run(2000, every=50, record0=True)
+    etot = rec[:, 1] + rec[:, 2]
+    assert np.max(np.abs(etot[5:] - etot[5])) < 2e-3
+    assert sim.rebuilds() > 5
+    st = sim.state()
+    assert np.max(np.abs(st["velocity"].sum(0))) < 1e-9
+
+
+def test_nhc_conserved_energy(oracle):
+    # test/unit/mdsim/integrators/verlet_nvt_hoover.cpp:203-206: the extended
+    # energy E_pot + E_kin + E_nhc is conserved; the temperature relaxes to T
+    n = 864
+    pos, L, species, rng = _liquid(oracle, n, 0.8442, 9, jitter=0.0)
+    vel = rng.normal(0, np.sqrt(2.0), (n, 3))
+    vel -= vel.mean(0)
+    pot = oracle.Potential(oracle.POT_LJ, "shifted", 1.0, 1.0, 2.5)
+    sim = oracle.Simulation(pos, vel, species, np.ones(n), L, pot, 0.3, 0.002, integrator=oracle.INT_NHC,
+                            temperature=1.0, nu=5.0)
+    rec = sim.run(4000, every=50, record0=False)
+    econs = rec[:, 1] + rec[:, 2] + rec[:, 4]
+    assert np.max(np.abs(econs - econs[0])) < 5e-3
+    temp = 2 * rec[-20:, 2] / 3
+    assert abs(temp.mean() - 1.0) < 0.1
+
+
+# ----------------------------------------------------------------------------
+# observables
+# ----------------------------------------------------------------------------
+def test_density_modes_perfect_lattice(oracle):
+    # test/unit/observables/ssf.cpp:237-242: on a perfect fcc lattice rho_k = N
+    # (up to a phase) at reciprocal-lattice vectors, ~0 elsewhere
+    n = 4000
+    L = np.full(3, 10.0 * 1.5)
+    pos, a = oracle.lattice_fcc(n, L)
+    b = 2 * np.pi / a
+    q_bragg = np.array([[2 * b, 0, 0], [b, b, b], [2 * b, 2 * b, 0]])
+    q_off = np.array([[2 * np.pi / L[0], 0, 0], [b, 0, 0], [b, b, 0]])   # box vectors, not fcc Bragg
+    rho = oracle.density_mode(pos, q_bragg)
+    assert np.allclose(np.abs(rho), n, rtol=0, atol=50 * n * EPS * 100)
+    rho0 = oracle.density_mode(pos, q_off)
+    assert np.all(np.abs(rho0) < 1e-8 * n)
+
+
+def test_wavevector_shells(oracle):
+    L = np.full(3, 37.35)
+    k = np.array([2.0, 4.0, 7.25, 10.0])
+    q, shell = oracle.wavevector_shell(k, L, 0.01, 7)
+    u = 2 * np.pi / L[0]
+    for s, kk in enumerate(k):
+        qs = q[shell[s, 0]:shell[s, 1]]
+        assert 0 < len(qs) <= 7
+        assert np.all(np.abs(np.linalg.norm(qs, axis=1) - kk) <= kk * 0.01 + 1e-12)
+        assert np.all(qs >= 0)
+        # integer multiples of the reciprocal box vector
+        m = qs / u
+        assert np.allclose(m, np.rint(m), atol=1e-9)
+
+
+def test_accumulator(oracle):
+    x = np.random.default_rng(1).normal(3.0, 2.0, 10001)
+    c, m, v = oracle.accumulate(x)
+    assert c == len(x)
+    assert np.isclose(m, x.mean(), rtol=1e-14)
+    assert np.isclose(v, x.var(), rtol=1e-12)
+
+
+def test_blocking_scheme_intervals(oracle):
+    # resolution 1 step, max lag 1000, block size 10, shift 3: 1, 3, 10, 30, 100, 300, 1000
+    c = oracle.Correlator(1000, 1, 10, 3, 10, 0, 4)
+    assert list(c.intervals()) == [1, 3, 10, 30, 100, 300, 1000]
+    c = oracle.Correlator(1000, 2, 4, 1, 4, 0, 4)
+    assert list(c.intervals()) == [2, 8, 32, 128, 512]
+
+
+def test_msd_ballistic_and_isf_lag0(oracle):
+    # ballistic particles: R(t) = R0 + v t, so MSD(lag) = <v^2> lag^2 exactly at
+    # every level; ISF at lag 0 is <|rho_k|^2>/N over the shell
+    n = 64
+    rng = np.random.default_rng(2)
+    r0 = rng.uniform(-5, 5, (n, 3))
+    v = rng.normal(0, 1, (n, 3))
+    dt = 0.01
+    q = np.array([[1.0, 0, 0], [0, 1.0, 0]])
+    shell = np.array([[0, 2]])
+    c = oracle.Correlator(200, 1, 5, 2, 5, 0, n, nq=2, shell=shell)
+    rhos = {}
+    for step in range(0, 401):
+        pos = r0 + v * step * dt
+        rho = oracle.density_mode(pos, q)
+        rhos[step] = rho
+        c.sample(step, pos, rho)
+    c.finalise()
+    cnt, mean, err = c.result("msd")
+    iv = c.intervals()
+    v2 = np.mean(np.sum(v * v, 1))
+    for lvl in range(len(iv)):
+        for lag in range(5):
+            if cnt[lvl, lag]:
+                assert np.isclose(mean[lvl, lag], v2 * (lag * iv[lvl] * dt) ** 2, rtol=1e-9, atol=1e-15)
+    icnt, imean, _ = c.result("isf")
+    assert icnt[0, 0, 0] > 0
+    # level 0, lag 0 averages |rho_k|^2/N over all processed origins; check bounds
+    vals = [np.mean(np.abs(rhos[s]) ** 2) / n for s in rhos]
+    assert min(vals) - 1e-12 <= imean[0, 0, 0] <= max(vals) + 1e-12
