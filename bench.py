@@ -1,0 +1,181 @@
+"""Benchmark: Lennard-Jones fluid, N = 1,048,576, on one MI355X.
+
+Workload (one "step" = one velocity-Verlet MD step of the whole system):
+fcc 64^3 lattice at rho = 0.8442, Maxwell-Boltzmann velocities at T = 0.722
+(seeded), LJ truncated and shifted at r_c = 2.5 sigma, neighbour skin 0.3 sigma
+with rebuilds on the half-skin criterion, NVE dt = 0.005 tau, Hilbert-curve
+sort every 100 steps, density modes every 1000 steps for 201 wavevectors with
+|k| in [2 pi / L, 10].  All state is resident on the device; the timed region
+is bracketed by device synchronisations.
+
+Prints one JSON line with the throughput (particle-steps/s), the HBM roofline
+of the dominant kernel (the pair-force kernel) measured with HIP events on its
+stream, and the FP64 CPU oracle timed on a bounded sample on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N = 64 ** 3 * 4
+DENSITY = 0.8442
+TEMPERATURE = 0.722
+R_CUT = 2.5
+SKIN = 0.3
+TIMESTEP = 0.005
+SORT_INTERVAL = 100
+MODES_INTERVAL = 1000
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_workload(seed):
+    from halmd_amd import observables, workloads
+    from halmd_amd.observables.utility import wavevector
+    w = workloads.lennard_jones(N, density=DENSITY, temperature=TEMPERATURE, r_cut=R_CUT, skin=SKIN,
+                                timestep=TIMESTEP, seed=seed, sort_interval=SORT_INTERVAL)
+    k = np.linspace(2 * np.pi / w.length[0], 10.0, 34)
+    wv = wavevector(w.box, k, tolerance=0.01, max_count=6)
+    dm = observables.density_mode(w.particle, wv)
+    w.sampler.on_sample_density_mode(dm, MODES_INTERVAL)
+    w.wavevector, w.density_mode = wv, dm
+    return w
+
+
+def kernel_times(w, steps):
+    """Average duration per launch (ms) of each kernel class over `steps` steps."""
+    from halmd_amd import _lib
+    mods = {"force": w.force, "neighbour": w.neighbour, "binning": w.neighbour.binning,
+            "integrate": w.integrator, "finalize": w.integrator, "sort": w.sort}
+    for name, m in mods.items():
+        _lib.call("halmd_profile_enable", m.handle, name.encode(), 4 * steps + 16)
+    w.sampler.run(steps)
+    out = {}
+    for name, m in mods.items():
+        import ctypes
+        ms, n = ctypes.c_double(), ctypes.c_uint64()
+        _lib.call("halmd_profile_collect", m.handle, name.encode(), ctypes.byref(ms), ctypes.byref(n))
+        _lib.call("halmd_profile_enable", m.handle, name.encode(), 0)
+        out[name] = (ms.value, n.value)
+    return out
+
+
+def cpu_baseline(w, budget_s):
+    """FP64 oracle (host backend restatement), one thread, same initial state."""
+    from oracle import oracle as orc
+    pot = orc.Potential(orc.POT_LJ, "shifted", 1.0, 1.0, R_CUT)
+    t0 = time.perf_counter()
+    sim = orc.Simulation(w.position, w.velocity, w.species, np.ones(N), w.length, pot, SKIN, TIMESTEP)
+    sim.time(0)                     # initial binning, lists and forces
+    setup = time.perf_counter() - t0
+    dt1 = sim.time(2)
+    steps = max(2, int((budget_s - setup) / max(dt1 / 2, 1e-6)))
+    steps = min(steps, 200)
+    dt = sim.time(steps)
+    total_steps = steps + 2
+    secs = dt + dt1
+    return {"value": N * total_steps / secs, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 oracle (host-backend restatement, single thread), same LJ N={N} state, "
+                      f"{total_steps} NVE steps after setup ({sim.rebuilds()} neighbour-list builds incl. setup), "
+                      f"{secs:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--warmup", type=int, default=400)
+    ap.add_argument("--profile-steps", type=int, default=500)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--seed", type=int, default=2024)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs one GPU per process; --gpus {args.gpus} is not supported "
+              "(independent systems shard across GPUs without exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import halmd_amd
+    arch, cus, mem = halmd_amd.device_info()
+    log(f"device {arch}, {cus} CUs, {mem / 2**30:.0f} GiB; building LJ N={N}")
+    t0 = time.perf_counter()
+    w = build_workload(args.seed)
+    log(f"setup {time.perf_counter() - t0:.1f} s; warmup {args.warmup} steps")
+    w.sampler.run(args.warmup)
+    log(f"timed {args.steps} steps")
+    ms = w.sampler.run_timed(args.steps)
+    value = N * args.steps / (ms / 1e3)
+    log(f"{ms / args.steps:.4f} ms/step, {value:.4e} particle-steps/s; per-kernel timing pass")
+
+    kt = kernel_times(w, args.profile_steps)
+    counts = w.neighbour.counts().astype(np.float64)
+    n_nb = counts.mean()
+    f_ms, f_n = kt["force"]
+    f_avg = f_ms / max(f_n, 1)
+    # algorithmic bytes per particle for the force kernel: r_i (hi+lo, 24 B),
+    # (n_nb + 1) list words (4 B each: entries + count), force (12 B)
+    force_bytes = N * (24 + 4 * (n_nb + 1) + 12)
+    achieved = force_bytes / (f_avg * 1e-3) / 1e9
+    kernels = {k: {"launches": int(n), "avg_ms": (ms_ / n if n else None)} for k, (ms_, n) in kt.items()}
+    log(f"force kernel {f_avg:.4f} ms/launch, {achieved:.0f} GB/s algorithmic; n_nb = {n_nb:.2f}")
+    thermo = w.thermodynamics
+    en_tot = thermo.en_tot()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("CPU baseline (FP64 oracle, 1 thread)")
+        cpu = cpu_baseline(w, args.cpu_seconds)
+        cpu["host_cpus"] = os.cpu_count()
+
+    out = {
+        "metric": "particle-steps/s, LJ N=1M",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "Lennard-Jones fluid N=1048576 (fcc 64^3, rho=0.8442, T=0.722), r_c=2.5 shifted, "
+                        "skin=0.3, NVE dt=0.005, Hilbert sort every 100 steps, density modes every 1000 steps",
+            "nparticle": N,
+            "precision": "FP32 pair forces, float-float positions/velocities, FP64 reductions",
+            "wavevectors": int(len(w.wavevector.value)),
+            "neighbours_mean": n_nb,
+            "neighbour_rebuilds": int(w.neighbour.rebuilds),
+            "en_tot_per_particle_end": en_tot,
+            "device": arch,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "pair_force_kernel<LJ, shifted>",
+            "bytes_per_launch": force_bytes,
+            "avg_ms": f_avg,
+        },
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

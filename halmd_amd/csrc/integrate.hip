@@ -1,0 +1,235 @@
+// Velocity-Verlet integrators on double-word (float-float) positions and
+// velocities for gfx950.
+//
+// integrate: v += f dt/2 / m; r += v dt; wrap r into the box, image += shift
+//            (halmd/mdsim/gpu/integrators/verlet_kernel.cu:37-72)
+// finalize:  v += f dt/2 / m (verlet_kernel.cu:77-98)
+// Nose-Hoover: the velocity scale factor lives on the device (written by the
+// chain kernel in reduce.hip), so neither half-step needs a host round trip
+// (the reference propagates the chain on the host, verlet_nvt_hoover.cpp:188-222).
+//
+// The time step is carried as a double word (dt_hi + dt_lo) in the drift, and
+// the box length as L_hi + L_lo in the wrap, so that neither the FP32 rounding
+// of dt nor of L enters the trajectory.  The integrate kernel also tracks the
+// largest squared displacement since the last neighbour-list build (high
+// words, minimum image), reduced per wave and folded into one device word with
+// atomicMax: the rebuild criterion of halmd/mdsim/gpu/max_displacement.cpp
+// without a separate pass over the positions.
+#include "kernels.hpp"
+
+namespace halmd {
+namespace {
+
+// double-word product (DWTimesDW1, Joldes, Muller & Popescu 2017)
+__device__ __forceinline__ ff ff_mul(ff x, ff y) {
+#pragma clang fp contract(off)
+    float const ch = x.hi * y.hi;
+    float const cl1 = __builtin_fmaf(x.hi, y.hi, -ch);
+    float const tl = x.hi * y.lo;
+    float const cl2 = __builtin_fmaf(x.lo, y.hi, tl);
+    float const cl3 = cl1 + cl2;
+    ff z;
+    fast_two_sum(ch, cl3, z.hi, z.lo);
+    return z;
+}
+
+__device__ __forceinline__ ff ff_from_double(double x) {
+    float const hi = float(x);
+    return ff{hi, float(x - double(hi))};
+}
+
+__global__ __launch_bounds__(256) void verlet_integrate_kernel(
+    float4* __restrict__ pos, float4* __restrict__ vel, float4 const* __restrict__ force, int4* __restrict__ image,
+    unsigned n, integrate_params p, double const* __restrict__ scale, float4 const* __restrict__ r0,
+    uint32_t* __restrict__ max_disp2, uint32_t* __restrict__ clear_slot)
+{
+    unsigned const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (clear_slot && i == 0) *clear_slot = 0u;
+    float d2 = 0.0f;
+    if (i < n) {
+        float4 rh = pos[2 * size_t(i)], rl = pos[2 * size_t(i) + 1];
+        float4 vh = vel[2 * size_t(i)], vl = vel[2 * size_t(i) + 1];
+        float4 const f = force[i];
+        float const m = vh.w;
+        ff vx{vh.x, vl.x}, vy{vh.y, vl.y}, vz{vh.z, vl.z};
+        if (scale) {
+            ff const s = ff_from_double(*scale);
+            vx = ff_mul(vx, s); vy = ff_mul(vy, s); vz = ff_mul(vz, s);
+        }
+        vx = ff_add_f(vx, f.x * p.half_dt / m);
+        vy = ff_add_f(vy, f.y * p.half_dt / m);
+        vz = ff_add_f(vz, f.z * p.half_dt / m);
+        ff const dt{p.dt_hi, p.dt_lo};
+        ff rx = ff_add(ff{rh.x, rl.x}, ff_mul(vx, dt));
+        ff ry = ff_add(ff{rh.y, rl.y}, ff_mul(vy, dt));
+        ff rz = ff_add(ff{rh.z, rl.z}, ff_mul(vz, dt));
+        float const ix = rintf(rx.hi * p.invL[0]);
+        float const iy = rintf(ry.hi * p.invL[1]);
+        float const iz = rintf(rz.hi * p.invL[2]);
+        if (ix != 0.0f) { rx = ff_add_f(rx, -ix * p.L_hi[0]); rx = ff_add_f(rx, -ix * p.L_lo[0]); }
+        if (iy != 0.0f) { ry = ff_add_f(ry, -iy * p.L_hi[1]); ry = ff_add_f(ry, -iy * p.L_lo[1]); }
+        if (iz != 0.0f) { rz = ff_add_f(rz, -iz * p.L_hi[2]); rz = ff_add_f(rz, -iz * p.L_lo[2]); }
+        pos[2 * size_t(i)] = make_float4(rx.hi, ry.hi, rz.hi, rh.w);
+        pos[2 * size_t(i) + 1] = make_float4(rx.lo, ry.lo, rz.lo, 0.0f);
+        vel[2 * size_t(i)] = make_float4(vx.hi, vy.hi, vz.hi, m);
+        vel[2 * size_t(i) + 1] = make_float4(vx.lo, vy.lo, vz.lo, 0.0f);
+        if (ix != 0.0f || iy != 0.0f || iz != 0.0f) {
+            int4 im = image[i];
+            im.x += int(ix); im.y += int(iy); im.z += int(iz);
+            image[i] = im;
+        }
+        if (r0) {
+            float4 const q = r0[i];
+            float dx = rx.hi - q.x, dy = ry.hi - q.y, dz = rz.hi - q.z;
+            dx -= rintf(dx * p.invL[0]) * p.L_hi[0];
+            dy -= rintf(dy * p.invL[1]) * p.L_hi[1];
+            dz -= rintf(dz * p.invL[2]) * p.L_hi[2];
+            d2 = dx * dx + dy * dy + dz * dz;
+        }
+    }
+    if (max_disp2) {
+        d2 = wave_max(d2);
+        if ((threadIdx.x % WAVE) == 0 && d2 > 0.0f) atomicMax(max_disp2, __float_as_uint(d2));
+    }
+}
+
+__global__ __launch_bounds__(256) void verlet_finalize_kernel(float4* __restrict__ vel, float4 const* __restrict__ force,
+                                                              unsigned n, float half_dt) {
+    unsigned const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 const vh = vel[2 * size_t(i)], vl = vel[2 * size_t(i) + 1];
+    float4 const f = force[i];
+    float const m = vh.w;
+    ff const vx = ff_add_f(ff{vh.x, vl.x}, f.x * half_dt / m);
+    ff const vy = ff_add_f(ff{vh.y, vl.y}, f.y * half_dt / m);
+    ff const vz = ff_add_f(ff{vh.z, vl.z}, f.z * half_dt / m);
+    vel[2 * size_t(i)] = make_float4(vx.hi, vy.hi, vz.hi, m);
+    vel[2 * size_t(i) + 1] = make_float4(vx.lo, vy.lo, vz.lo, 0.0f);
+}
+
+__global__ __launch_bounds__(256) void rescale_velocity_kernel(float4* __restrict__ vel, unsigned n,
+                                                               double const* __restrict__ scale) {
+    unsigned const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ff const s = ff_from_double(*scale);
+    float4 const vh = vel[2 * size_t(i)], vl = vel[2 * size_t(i) + 1];
+    ff const vx = ff_mul(ff{vh.x, vl.x}, s);
+    ff const vy = ff_mul(ff{vh.y, vl.y}, s);
+    ff const vz = ff_mul(ff{vh.z, vl.z}, s);
+    vel[2 * size_t(i)] = make_float4(vx.hi, vy.hi, vz.hi, vh.w);
+    vel[2 * size_t(i) + 1] = make_float4(vx.lo, vy.lo, vz.lo, 0.0f);
+}
+
+__global__ void copy_hi_kernel(float4 const* __restrict__ pos, float4* __restrict__ r0, unsigned n) {
+    unsigned const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) r0[i] = pos[2 * size_t(i)];
+}
+
+__global__ __launch_bounds__(256) void max_displacement_kernel(float4 const* __restrict__ pos,
+                                                               float4 const* __restrict__ r0, unsigned n, float Lx,
+                                                               float Ly, float Lz, float ix_, float iy_, float iz_,
+                                                               uint32_t* __restrict__ max_disp2) {
+    unsigned const i = blockIdx.x * blockDim.x + threadIdx.x;
+    float d2 = 0.0f;
+    if (i < n) {
+        float4 const r = pos[2 * size_t(i)], q = r0[i];
+        float dx = r.x - q.x, dy = r.y - q.y, dz = r.z - q.z;
+        dx -= rintf(dx * ix_) * Lx;
+        dy -= rintf(dy * iy_) * Ly;
+        dz -= rintf(dz * iz_) * Lz;
+        d2 = dx * dx + dy * dy + dz * dz;
+    }
+    d2 = wave_max(d2);
+    if ((threadIdx.x % WAVE) == 0 && d2 > 0.0f) atomicMax(max_disp2, __float_as_uint(d2));
+}
+
+// SplitMix64 finaliser (Steele, Lea & Flood, "Fast splittable pseudorandom
+// number generators", OOPSLA 2014) as a counter-based generator
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// uniform in (0, 1]
+__device__ __forceinline__ float u01(uint64_t x) { return float((x >> 40) + 1) * (1.0f / 16777216.0f); }
+
+__global__ __launch_bounds__(256) void andersen_finalize_kernel(float4* __restrict__ vel,
+                                                                float4 const* __restrict__ force,
+                                                                uint32_t const* __restrict__ id, unsigned n,
+                                                                float half_dt, float coll_prob, float sqrt_t,
+                                                                uint64_t seed, uint64_t step) {
+    unsigned const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 const vh = vel[2 * size_t(i)], vl = vel[2 * size_t(i) + 1];
+    float const m = vh.w;
+    // counter: (seed, step, particle id) -> independent of the memory order
+    uint64_t const key = mix64(seed ^ mix64(step * 0x632be59bd9b4e019ull + id[i]));
+    if (u01(mix64(key)) > coll_prob) {
+        float4 const f = force[i];
+        ff const vx = ff_add_f(ff{vh.x, vl.x}, f.x * half_dt / m);
+        ff const vy = ff_add_f(ff{vh.y, vl.y}, f.y * half_dt / m);
+        ff const vz = ff_add_f(ff{vh.z, vl.z}, f.z * half_dt / m);
+        vel[2 * size_t(i)] = make_float4(vx.hi, vy.hi, vz.hi, m);
+        vel[2 * size_t(i) + 1] = make_float4(vx.lo, vy.lo, vz.lo, 0.0f);
+    } else {
+        // Box-Muller from four uniforms; the heat bath draws N(0, sqrt(T)) per
+        // component (verlet_nvt_andersen.cpp finalize)
+        float const u1 = u01(mix64(key + 1)), u2 = u01(mix64(key + 2));
+        float const u3 = u01(mix64(key + 3)), u4 = u01(mix64(key + 4));
+        float const r1 = sqrt_t * sqrtf(-2.0f * logf(u1)), r2 = sqrt_t * sqrtf(-2.0f * logf(u3));
+        float s1, c1, s2, c2;
+        sincospif(2.0f * u2, &s1, &c1);
+        sincospif(2.0f * u4, &s2, &c2);
+        vel[2 * size_t(i)] = make_float4(r1 * c1, r1 * s1, r2 * c2, m);
+        vel[2 * size_t(i) + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+} // namespace
+
+void launch_verlet_integrate(float4* pos, float4* vel, float4 const* force, int4* image, unsigned n,
+                             integrate_params const& p, double const* scale, float4 const* r0, uint32_t* max_disp2,
+                             uint32_t* clear_slot, hipStream_t s) {
+    if (!n) return;
+    verlet_integrate_kernel<<<div_up(n, 256), 256, 0, s>>>(pos, vel, force, image, n, p, scale, r0, max_disp2,
+                                                           clear_slot);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_verlet_finalize(float4* vel, float4 const* force, unsigned n, float half_dt, hipStream_t s) {
+    if (!n) return;
+    verlet_finalize_kernel<<<div_up(n, 256), 256, 0, s>>>(vel, force, n, half_dt);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_rescale_velocity(float4* vel, unsigned n, double const* scale, hipStream_t s) {
+    if (!n) return;
+    rescale_velocity_kernel<<<div_up(n, 256), 256, 0, s>>>(vel, n, scale);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_copy_hi(float4 const* pos, float4* r0, unsigned n, hipStream_t s) {
+    if (!n) return;
+    copy_hi_kernel<<<div_up(n, 256), 256, 0, s>>>(pos, r0, n);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_max_displacement(float4 const* pos, float4 const* r0, unsigned n, float const* L, float const* invL,
+                             uint32_t* max_disp2, hipStream_t s) {
+    if (!n) return;
+    max_displacement_kernel<<<div_up(n, 256), 256, 0, s>>>(pos, r0, n, L[0], L[1], L[2], invL[0], invL[1], invL[2],
+                                                           max_disp2);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_andersen_finalize(float4* vel, float4 const* force, uint32_t const* id, unsigned n, float half_dt,
+                              float coll_prob, float sqrt_temperature, uint64_t seed, uint64_t step, hipStream_t s) {
+    if (!n) return;
+    andersen_finalize_kernel<<<div_up(n, 256), 256, 0, s>>>(vel, force, id, n, half_dt, coll_prob, sqrt_temperature,
+                                                            seed, step);
+    HIP_CHECK(hipGetLastError());
+}
+
+} // namespace halmd

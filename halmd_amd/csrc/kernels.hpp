@@ -1,0 +1,157 @@
+// Host-side launchers of the gfx950 kernels.  Every launcher is stream-ordered
+// and synchronises nothing; device pointers and POD parameters only.
+#pragma once
+
+#include "common.hpp"
+
+namespace halmd {
+
+// maximum number of particle species (pair tables are MAX_TYPES^2)
+constexpr unsigned MAX_TYPES = 4;
+
+// ---------------------------------------------------------------------------
+// scan / radix sort (sort.hip)
+// ---------------------------------------------------------------------------
+struct scan_workspace {
+    dbuf<uint32_t> partial;       // per-level block sums
+    std::vector<size_t> level_offset;
+    void reserve(size_t n);
+};
+// exclusive prefix sum of m uint32 values, in place
+void scan_exclusive(uint32_t* data, size_t m, scan_workspace& ws, hipStream_t s);
+
+struct radix_sort_workspace {
+    dbuf<uint32_t> keys_alt, vals_alt, counts;
+    scan_workspace scan;
+    size_t n_max = 0;
+    void reserve(size_t n);
+};
+// stable LSD radix sort of (key, value) pairs by the low key_bits bits of key,
+// in place; 8-bit digits, ceil(key_bits / 8) passes
+void radix_sort(uint32_t* keys, uint32_t* vals, size_t n, unsigned key_bits, radix_sort_workspace& ws, hipStream_t s);
+
+void launch_iota(uint32_t* out, size_t n, hipStream_t s);
+void launch_fill_u32(uint32_t* out, uint32_t value, size_t n, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// binning and neighbour lists (binning.hip)
+// ---------------------------------------------------------------------------
+struct cell_grid {
+    unsigned ncell[3];
+    float cell_length[3];
+};
+void launch_compute_cell(float4 const* pos, unsigned n, cell_grid g, uint32_t* keys, uint32_t* vals, hipStream_t s);
+void launch_cell_bounds(uint32_t const* sorted_keys, unsigned n, unsigned ncells, uint32_t* begin, uint32_t* end,
+                        hipStream_t s);
+void launch_gather_cell_pos(float4 const* pos, uint32_t const* sorted_vals, unsigned n, float4* cell_pos, hipStream_t s);
+
+struct neighbour_params {
+    cell_grid grid;
+    float L[3];
+    float invL[3];
+    unsigned ntype;
+    float rr_cut_skin[MAX_TYPES * MAX_TYPES];
+};
+// full neighbour lists; list layout [(i / 64) * 64 * cap + k * 64 + i % 64];
+// *max_count receives the largest per-particle count (may exceed cap)
+void launch_build_neighbours(float4 const* pos, unsigned n, uint32_t const* cell_begin, uint32_t const* cell_end,
+                             float4 const* cell_pos, uint32_t const* cell_val, neighbour_params const& p,
+                             unsigned cap, uint32_t* list, uint32_t* count, uint32_t* max_count, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// pair forces (force.hip)
+// ---------------------------------------------------------------------------
+enum potential_kind : int { POTENTIAL_LENNARD_JONES = 0, POTENTIAL_MORSE = 1 };
+enum truncation_kind : int { TRUNC_SHARP = 0, TRUNC_SHIFTED = 1, TRUNC_FORCE_SHIFTED = 2, TRUNC_SMOOTH_R4 = 3 };
+
+// per species pair, FP32
+struct pair_param {
+    float p[4];        // LJ: {48 eps / sigma^2, 4 eps, sigma^2, -}; Morse: {eps/A, 1/(sigma B), r_min_sigma/B, A}
+    float rr_cut, r_cut, en_cut, force_cut;
+};
+struct force_params {
+    int kind, trunc;
+    unsigned ntype;
+    float rri_smooth;
+    float L_hi[3], L_lo[3], invL[3];
+    pair_param pair[MAX_TYPES * MAX_TYPES];
+};
+// force_zero: overwrite force/en_pot/stress, else add to the stored values
+void launch_pair_force(float4 const* pos, unsigned n, uint32_t const* list, uint32_t const* count, unsigned cap,
+                       force_params const& p, bool aux, bool force_zero, float aux_weight, float4* force,
+                       float* stress, unsigned stride, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// integrators (integrate.hip)
+// ---------------------------------------------------------------------------
+struct integrate_params {
+    float dt_hi, dt_lo, half_dt;
+    float L_hi[3], L_lo[3], invL[3];
+};
+// kick-drift-wrap; optional velocity pre-scale (*scale, device) and optional
+// max-displacement tracking against r0 (max |r - r0|^2 as float bits folded
+// into *max_disp2; *clear_slot, the slot of the next step, is set to zero)
+void launch_verlet_integrate(float4* pos, float4* vel, float4 const* force, int4* image, unsigned n,
+                             integrate_params const& p, double const* scale, float4 const* r0,
+                             uint32_t* max_disp2, uint32_t* clear_slot, hipStream_t s);
+void launch_verlet_finalize(float4* vel, float4 const* force, unsigned n, float half_dt, hipStream_t s);
+void launch_rescale_velocity(float4* vel, unsigned n, double const* scale, hipStream_t s);
+void launch_copy_hi(float4 const* pos, float4* r0, unsigned n, hipStream_t s);
+void launch_max_displacement(float4 const* pos, float4 const* r0, unsigned n, float const* L, float const* invL,
+                             uint32_t* max_disp2, hipStream_t s);
+// Andersen collisions: with probability p per particle resample v ~ N(0, sqrt(T))
+// (counter-based RNG keyed by seed, step and particle id), else kick
+void launch_andersen_finalize(float4* vel, float4 const* force, uint32_t const* id, unsigned n, float half_dt,
+                              float coll_prob, float sqrt_temperature, uint64_t seed, uint64_t step, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// reductions (reduce.hip): deterministic two-pass FP64 block sums
+// ---------------------------------------------------------------------------
+constexpr unsigned REDUCE_BLOCKS = 240;
+constexpr unsigned REDUCE_THREADS = 256;
+// per-block partial sums of: [0] sum m v^2, [1..3] sum m v, [4] sum m,
+// [5] sum en_pot, [6] sum tr(stress), [7..12] sum stress components
+constexpr unsigned THERMO_NQ = 13;
+void launch_thermo_partials(float4 const* vel, float4 const* force, float const* stress, unsigned stride,
+                            unsigned n, bool aux, double* partials, hipStream_t s);
+// sum partials[REDUCE_BLOCKS][nq] in a fixed order into out[nq]
+void launch_reduce_final(double const* partials, unsigned nq, double* out, hipStream_t s);
+
+// NHC chain state on the device
+struct nhc_state {
+    double xi[2], v_xi[2], mass_xi[2];
+    double temperature, en_kin_target_2, timestep;
+    double scale;           // last velocity scale factor
+    double en_nhc_raw;      // T (dN xi0 + xi1) + sum m_xi v_xi^2 / 2 (not divided by N)
+};
+// sum m v^2 partials -> chain half-step; writes state->scale
+void launch_nhc_propagate(double const* partials, nhc_state* state, hipStream_t s);
+void launch_mv2_partials(float4 const* vel, unsigned n, double* partials, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Hilbert sort and particle rearrangement (hilbert.hip)
+// ---------------------------------------------------------------------------
+void launch_hilbert_map(float4 const* pos, unsigned n, float const* L, unsigned depth, uint32_t* code,
+                        uint32_t* index, hipStream_t s);
+void launch_rearrange(float4 const* pos_in, float4 const* vel_in, int4 const* image_in, uint32_t const* id_in,
+                      uint32_t const* perm, unsigned n, float4* pos_out, float4* vel_out, int4* image_out,
+                      uint32_t* id_out, hipStream_t s);
+void launch_reverse_id(uint32_t const* id, unsigned n, uint32_t* rid, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// observables (observables.hip)
+// ---------------------------------------------------------------------------
+// rho_k = sum_j exp(-i k.r_j); q as double3 [nq]; out [nq] (re, im) pairs
+constexpr unsigned DM_KCHUNK = 8;
+constexpr unsigned DM_BLOCKS = 240;
+void launch_density_mode(float4 const* pos, unsigned n, double const* q, unsigned nq, double* partials, double* out,
+                         hipStream_t s);
+// unwrapped positions r + image L in ID order, FP64
+void launch_sample_position(float4 const* pos, int4 const* image, uint32_t const* id, unsigned n, double const* L,
+                            double* out, hipStream_t s);
+// per-block partials of sum |r2 - r1|^2 and sum |r2 - r1|^4 over n particles,
+// written to partials[block * nq + qoff + {0, 1}]
+void launch_msd_partials(double const* r1, double const* r2, unsigned n, double* partials, unsigned nq, unsigned qoff,
+                         hipStream_t s);
+
+} // namespace halmd

@@ -1,0 +1,1083 @@
+// C++ module layer (see modules.hpp for the reference module each class mirrors).
+#include "modules.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <map>
+#include <numeric>
+
+namespace halmd {
+
+// ---------------------------------------------------------------------------
+// kernel_timer
+// ---------------------------------------------------------------------------
+kernel_timer::~kernel_timer() { disable(); }
+
+void kernel_timer::enable(size_t max_launches) {
+    disable();
+    events_.resize(2 * max_launches);
+    for (auto& e : events_) HIP_CHECK(hipEventCreate(&e));
+    next_ = 0;
+}
+
+void kernel_timer::disable() {
+    for (auto& e : events_) (void)hipEventDestroy(e);
+    events_.clear();
+    next_ = 0;
+}
+
+void kernel_timer::start(hipStream_t s) {
+    if (next_ + 2 <= events_.size()) HIP_CHECK(hipEventRecord(events_[next_], s));
+}
+
+void kernel_timer::stop(hipStream_t s) {
+    if (next_ + 2 <= events_.size()) {
+        HIP_CHECK(hipEventRecord(events_[next_ + 1], s));
+        next_ += 2;
+    }
+}
+
+std::pair<double, size_t> kernel_timer::collect() {
+    double total = 0;
+    size_t launches = next_ / 2;
+    for (size_t k = 0; k < next_; k += 2) {
+        HIP_CHECK(hipEventSynchronize(events_[k + 1]));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, events_[k], events_[k + 1]));
+        total += ms;
+    }
+    next_ = 0;
+    return {total, launches};
+}
+
+// ---------------------------------------------------------------------------
+// box: cuboid, origin-centred (halmd/mdsim/box.cpp:37-62)
+// ---------------------------------------------------------------------------
+box::box(double const* length) {
+    for (int k = 0; k < 3; ++k) {
+        require(length[k] > 0, "box: edge lengths must be positive");
+        L[k] = length[k];
+        split_ff(L[k], L_hi[k], L_lo[k]);
+        Lf[k] = float(L[k]);
+        invL[k] = float(1.0 / L[k]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// particle (halmd/mdsim/gpu/particle.cpp:55-253)
+// ---------------------------------------------------------------------------
+particle::particle(unsigned n_, unsigned nspecies_) : n(n_), nspecies(nspecies_) {
+    require(n > 0, "particle: number of particles must be positive");
+    require(nspecies >= 1 && nspecies <= MAX_TYPES, "particle: 1 <= nspecies <= 4");
+    n_pad = unsigned(round_up(n, PAD));
+    HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    pos.resize(2 * size_t(n_pad));
+    vel.resize(2 * size_t(n_pad));
+    force.resize(n_pad);
+    stress.resize(6 * size_t(n_pad));
+    image.resize(n_pad);
+    id.resize(n_pad);
+    reverse_id.resize(n_pad);
+    pos_alt.resize(2 * size_t(n_pad));
+    vel_alt.resize(2 * size_t(n_pad));
+    image_alt.resize(n_pad);
+    id_alt.resize(n_pad);
+    // zero state, species 0 and unit mass for real particles; ghost species in the padding
+    std::vector<float4> hp(2 * size_t(n_pad), make_float4(0, 0, 0, 0));
+    std::vector<float4> hv(2 * size_t(n_pad), make_float4(0, 0, 0, 0));
+    for (size_t i = 0; i < n_pad; ++i) {
+        uint32_t const t = i < n ? 0u : 0xFFFFFFFFu;
+        float w;
+        std::memcpy(&w, &t, 4);
+        hp[2 * i].w = w;
+        hv[2 * i].w = 1.0f;
+    }
+    pos.upload(hp.data(), hp.size(), stream);
+    vel.upload(hv.data(), hv.size(), stream);
+    HIP_CHECK(hipMemsetAsync(force, 0, force.bytes(), stream));
+    HIP_CHECK(hipMemsetAsync(stress, 0, stress.bytes(), stream));
+    HIP_CHECK(hipMemsetAsync(image, 0, image.bytes(), stream));
+    launch_iota(id, n_pad, stream);
+    launch_iota(reverse_id, n_pad, stream);
+    sync();
+}
+
+particle::~particle() {
+    (void)hipStreamSynchronize(stream);
+    (void)hipStreamDestroy(stream);
+}
+
+void particle::sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
+
+void particle::download_hilo(dbuf<float4> const& b, double* out) {
+    host_tmp_.resize(2 * size_t(n));
+    b.download(host_tmp_.data(), host_tmp_.size(), stream);
+    sync();
+    for (size_t i = 0; i < n; ++i) {
+        float4 const h = host_tmp_[2 * i], l = host_tmp_[2 * i + 1];
+        out[3 * i] = double(h.x) + double(l.x);
+        out[3 * i + 1] = double(h.y) + double(l.y);
+        out[3 * i + 2] = double(h.z) + double(l.z);
+    }
+}
+
+void particle::upload_hilo(dbuf<float4>& b, double const* in, bool) {
+    host_tmp_.resize(2 * size_t(n));
+    b.download(host_tmp_.data(), host_tmp_.size(), stream);
+    sync();
+    for (size_t i = 0; i < n; ++i) {
+        float4& h = host_tmp_[2 * i];
+        float4& l = host_tmp_[2 * i + 1];
+        split_ff(in[3 * i], h.x, l.x);
+        split_ff(in[3 * i + 1], h.y, l.y);
+        split_ff(in[3 * i + 2], h.z, l.z);
+        l.w = 0;
+    }
+    b.upload(host_tmp_.data(), host_tmp_.size(), stream);
+    sync();
+}
+
+void particle::set_position(double const* r) { upload_hilo(pos, r, true); positions_changed(); }
+void particle::get_position(double* r) { download_hilo(pos, r); }
+void particle::set_velocity(double const* v) { upload_hilo(vel, v, false); velocities_changed(); }
+void particle::get_velocity(double* v) { download_hilo(vel, v); }
+
+void particle::set_species(unsigned const* s) {
+    host_tmp_.resize(2 * size_t(n));
+    pos.download(host_tmp_.data(), host_tmp_.size(), stream);
+    sync();
+    for (size_t i = 0; i < n; ++i) {
+        require(s[i] < nspecies, "particle: species out of range");
+        std::memcpy(&host_tmp_[2 * i].w, &s[i], 4);
+    }
+    pos.upload(host_tmp_.data(), host_tmp_.size(), stream);
+    sync();
+    ++species_version;
+    force_valid = false;
+    aux_valid = false;
+}
+
+void particle::get_species(unsigned* s) {
+    host_tmp_.resize(2 * size_t(n));
+    pos.download(host_tmp_.data(), host_tmp_.size(), stream);
+    sync();
+    for (size_t i = 0; i < n; ++i) std::memcpy(&s[i], &host_tmp_[2 * i].w, 4);
+}
+
+void particle::set_mass(double const* m) {
+    host_tmp_.resize(2 * size_t(n));
+    vel.download(host_tmp_.data(), host_tmp_.size(), stream);
+    sync();
+    for (size_t i = 0; i < n; ++i) {
+        require(m[i] > 0, "particle: mass must be positive");
+        host_tmp_[2 * i].w = float(m[i]);
+    }
+    vel.upload(host_tmp_.data(), host_tmp_.size(), stream);
+    sync();
+    velocities_changed();
+}
+
+void particle::get_mass(double* m) {
+    host_tmp_.resize(2 * size_t(n));
+    vel.download(host_tmp_.data(), host_tmp_.size(), stream);
+    sync();
+    for (size_t i = 0; i < n; ++i) m[i] = host_tmp_[2 * i].w;
+}
+
+void particle::set_image(int const* im) {
+    std::vector<int4> h(n);
+    for (size_t i = 0; i < n; ++i) h[i] = make_int4(im[3 * i], im[3 * i + 1], im[3 * i + 2], 0);
+    image.upload(h.data(), n, stream);
+    sync();
+}
+
+void particle::get_image(int* im) {
+    std::vector<int4> h(n);
+    image.download(h.data(), n, stream);
+    sync();
+    for (size_t i = 0; i < n; ++i) { im[3 * i] = h[i].x; im[3 * i + 1] = h[i].y; im[3 * i + 2] = h[i].z; }
+}
+
+void particle::get_force(double* f) {
+    update_force();
+    std::vector<float4> h(n);
+    force.download(h.data(), n, stream);
+    sync();
+    for (size_t i = 0; i < n; ++i) { f[3 * i] = h[i].x; f[3 * i + 1] = h[i].y; f[3 * i + 2] = h[i].z; }
+}
+
+void particle::get_potential_energy(double* u) {
+    update_force(true);
+    std::vector<float4> h(n);
+    force.download(h.data(), n, stream);
+    sync();
+    for (size_t i = 0; i < n; ++i) u[i] = h[i].w;
+}
+
+void particle::get_stress_pot(double* s) {
+    update_force(true);
+    std::vector<float> h(6 * size_t(n_pad));
+    stress.download(h.data(), h.size(), stream);
+    sync();
+    for (size_t i = 0; i < n; ++i)
+        for (int c = 0; c < 6; ++c) s[6 * i + c] = h[c * size_t(n_pad) + i];
+}
+
+void particle::get_id(unsigned* v) { id.download(v, n, stream); sync(); }
+void particle::get_reverse_id(unsigned* v) { reverse_id.download(v, n, stream); sync(); }
+
+// force protocol of halmd/mdsim/gpu/particle.cpp:221-253: the first force
+// module overwrites, later ones accumulate; aux variables are computed only
+// when requested
+void particle::update_force(bool need_aux) {
+    if (force_valid && (!need_aux || aux_valid)) return;
+    aux_enabled = aux_enabled || need_aux;
+    if (forces.empty()) {
+        HIP_CHECK(hipMemsetAsync(force, 0, force.bytes(), stream));
+        if (aux_enabled) HIP_CHECK(hipMemsetAsync(stress, 0, stress.bytes(), stream));
+    } else {
+        bool first = true;
+        for (force_module* f : forces) {
+            f->apply(first);
+            first = false;
+        }
+    }
+    force_valid = true;
+    aux_valid = aux_enabled;
+    aux_enabled = false;
+}
+
+void particle::rearrange(uint32_t const* perm) {
+    launch_rearrange(pos, vel, image, id, perm, n, pos_alt, vel_alt, image_alt, id_alt, stream);
+    std::swap(pos, pos_alt);
+    std::swap(vel, vel_alt);
+    std::swap(image, image_alt);
+    std::swap(id, id_alt);
+    launch_reverse_id(id, n, reverse_id, stream);
+    ++order_version;
+    positions_changed();
+    velocities_changed();
+}
+
+// ---------------------------------------------------------------------------
+// potential
+// ---------------------------------------------------------------------------
+potential::potential(int kind_, unsigned nt1, unsigned nt2, double const* eps, double const* sig, double const* rmin,
+                     double const* dist)
+    : kind(kind_), ntype1(nt1), ntype2(nt2)
+{
+    require(kind == POTENTIAL_LENNARD_JONES || kind == POTENTIAL_MORSE, "potential: unknown kind");
+    require(nt1 >= 1 && nt1 <= MAX_TYPES && nt2 >= 1 && nt2 <= MAX_TYPES, "potential: 1 <= ntype <= 4");
+    size_t const n2 = size_t(nt1) * nt2;
+    epsilon.assign(eps, eps + n2);
+    sigma.assign(sig, sig + n2);
+    r_min.assign(n2, 0.0);
+    distortion.assign(n2, 1.0);
+    for (size_t i = 0; i < n2; ++i) {
+        require(sigma[i] > 0, "potential: sigma must be positive");
+        if (kind == POTENTIAL_MORSE) {
+            r_min[i] = rmin[i];
+            distortion[i] = dist[i];
+            require(distortion[i] > 0 && 2 * distortion[i] * distortion[i] - 1 != 0, "morse: invalid distortion");
+        }
+    }
+}
+
+void potential::base(double rr, unsigned ab, double& fval, double& en) const {
+    if (kind == POTENTIAL_LENNARD_JONES) {
+        double const s2 = sigma[ab] * sigma[ab];
+        double const rri = s2 / rr;
+        double const r6i = rri * rri * rri;
+        double const e = epsilon[ab] * r6i;
+        fval = 48 * rri * e * (r6i - 0.5) / s2;
+        en = 4 * e * (r6i - 1);
+    } else {
+        double const B = distortion[ab];
+        double const rsb = std::sqrt(rr) / sigma[ab] / B;
+        double const d = r_min[ab] / sigma[ab] / B - rsb;
+        double const e1 = std::exp(d);
+        double const A = 2 * B * B - 1;
+        double const eA = (A == 1) ? e1 : std::exp(A * d);
+        double const ee = epsilon[ab] * e1 / A;
+        fval = (A + 1) * ee * (eA - 1) * rsb / rr;
+        en = ee * (eA - A - 1);
+    }
+}
+
+void potential::truncate(int trunc_, double const* rcs, double smoothing) {
+    require(trunc_ >= TRUNC_SHARP && trunc_ <= TRUNC_SMOOTH_R4, "potential: unknown truncation");
+    trunc = trunc_;
+    size_t const n2 = size_t(ntype1) * ntype2;
+    r_cut_sigma.assign(rcs, rcs + n2);
+    r_cut.resize(n2); rr_cut.resize(n2); en_cut.assign(n2, 0); force_cut.assign(n2, 0);
+    h = smoothing;
+    if (trunc == TRUNC_SMOOTH_R4) require(h > 0, "smooth_r4: smoothing length must be positive");
+    for (size_t i = 0; i < n2; ++i) {
+        require(r_cut_sigma[i] > 0, "potential: cutoff must be positive");
+        r_cut[i] = sigma[i] * r_cut_sigma[i];
+        rr_cut[i] = r_cut[i] * r_cut[i];
+        double f, u;
+        base(rr_cut[i], unsigned(i), f, u);
+        en_cut[i] = u;
+        force_cut[i] = f * r_cut[i];
+    }
+}
+
+double potential::r_cut_max() const {
+    require(trunc >= 0, "potential: no truncation applied");
+    return *std::max_element(r_cut.begin(), r_cut.end());
+}
+
+void potential::eval(double rr, unsigned a, unsigned b, double& fval, double& en) const {
+    unsigned const ab = a * ntype2 + b;
+    base(rr, ab, fval, en);
+    switch (trunc) {
+    case TRUNC_SHIFTED: en -= en_cut[ab]; break;
+    case TRUNC_FORCE_SHIFTED: {
+        double const r = std::sqrt(rr);
+        fval -= force_cut[ab] / r;
+        en = en - en_cut[ab] + (r - r_cut[ab]) * force_cut[ab];
+        break;
+    }
+    case TRUNC_SMOOTH_R4: {
+        en -= en_cut[ab];
+        double const r = std::sqrt(rr);
+        double const dr = r - r_cut[ab];
+        double const rri = 1 / (h * h);
+        double const x2 = dr * dr * rri;
+        double const x4 = x2 * x2;
+        double const x4i = 1 / (1 + x4);
+        double const h0 = x4 * x4i;
+        double const h1 = 4 * dr * rri * x2 * x4i * x4i;
+        fval = h0 * fval - h1 * (en / r);
+        en = h0 * en;
+        break;
+    }
+    default: break;
+    }
+}
+
+force_params potential::device_params(box const& b) const {
+    require(trunc >= 0, "potential: no truncation applied");
+    require(ntype1 == ntype2, "potential: ntype1 != ntype2 is not supported by the device kernels");
+    force_params P;
+    std::memset(&P, 0, sizeof(P));
+    P.kind = kind;
+    P.trunc = trunc;
+    P.ntype = ntype2;
+    P.rri_smooth = trunc == TRUNC_SMOOTH_R4 ? float(1 / (h * h)) : 0.0f;
+    for (int k = 0; k < 3; ++k) { P.L_hi[k] = b.L_hi[k]; P.L_lo[k] = b.L_lo[k]; P.invL[k] = b.invL[k]; }
+    for (unsigned a = 0; a < ntype1; ++a) {
+        for (unsigned c = 0; c < ntype2; ++c) {
+            unsigned const ab = a * ntype2 + c;
+            pair_param& pp = P.pair[ab];
+            double const eps = epsilon[ab], sig = sigma[ab];
+            if (kind == POTENTIAL_LENNARD_JONES) {
+                pp.p[0] = float(48 * eps / (sig * sig));
+                pp.p[1] = float(4 * eps);
+                pp.p[2] = float(sig * sig);
+                pp.p[3] = 0;
+            } else {
+                double const B = distortion[ab];
+                double const A = 2 * B * B - 1;
+                pp.p[0] = float(eps / A);
+                pp.p[1] = float(1 / (sig * B));
+                pp.p[2] = float(r_min[ab] / sig / B);
+                pp.p[3] = float(A);
+            }
+            pp.rr_cut = float(rr_cut[ab]);
+            pp.r_cut = float(r_cut[ab]);
+            pp.en_cut = float(en_cut[ab]);
+            pp.force_cut = float(force_cut[ab]);
+        }
+    }
+    return P;
+}
+
+// ---------------------------------------------------------------------------
+// binning: cell grid floor(L / (rc_max + skin)) per dimension, compressed cells
+// ---------------------------------------------------------------------------
+binning::binning(particle& p_, box const& b_, double const* r_cut, unsigned nt1, unsigned nt2, double skin,
+                 double occupancy_)
+    : p(p_), b(b_), r_skin(skin), occupancy(occupancy_)
+{
+    require(skin >= 0, "binning: skin must be non-negative");
+    r_cut_max = *std::max_element(r_cut, r_cut + size_t(nt1) * nt2);
+    double const rcs = r_cut_max + r_skin;
+    ncells = 1;
+    for (int k = 0; k < 3; ++k) {
+        grid.ncell[k] = std::max(1u, unsigned(b.L[k] / rcs));
+        grid.cell_length[k] = float(b.L[k] / grid.ncell[k]);
+        ncells *= grid.ncell[k];
+    }
+    keys.resize(p.n);
+    vals.resize(p.n);
+    begin.resize(ncells);
+    end.resize(ncells);
+    cell_pos.resize(p.n);
+    ws.reserve(p.n);
+}
+
+void binning::update() {
+    if (version == p.position_version) return;
+    hipStream_t s = p.stream;
+    timer.start(s);
+    launch_compute_cell(p.pos, p.n, grid, keys, vals, s);
+    unsigned key_bits = 0;
+    while ((1ull << key_bits) < ncells) ++key_bits;
+    radix_sort(keys, vals, p.n, key_bits, ws, s);
+    launch_cell_bounds(keys, p.n, ncells, begin, end, s);
+    launch_gather_cell_pos(p.pos, vals, p.n, cell_pos, s);
+    timer.stop(s);
+    version = p.position_version;
+}
+
+void binning::cell_lists(unsigned* begin_out, unsigned* end_out, unsigned* index_out) {
+    update();
+    begin.download(begin_out, ncells, p.stream);
+    end.download(end_out, ncells, p.stream);
+    vals.download(index_out, p.n, p.stream);
+    p.sync();
+}
+
+// ---------------------------------------------------------------------------
+// max_displacement (halmd/mdsim/gpu/max_displacement.cpp:60-106)
+// ---------------------------------------------------------------------------
+max_displacement::max_displacement(particle& p_, box const& b_) : p(p_), b(b_) {
+    r0.resize(p.n);
+    d_max2.resize(2);
+    h_max2.resize(1);
+    zero();
+}
+
+void max_displacement::zero() {
+    launch_copy_hi(p.pos, r0, p.n, p.stream);
+    launch_fill_u32(d_max2, 0u, 2, p.stream);
+    slot = 0;
+    tracked_version = p.position_version;
+    computed_version = p.position_version;
+    value = 0;
+}
+
+uint32_t* max_displacement::fuse_slot() {
+    // the integrate kernel folds the new positions into the current slot and
+    // clears the other one, which the next step will use
+    tracked_version = p.position_version + 1;
+    return d_max2.get() + slot;
+}
+
+double max_displacement::compute() {
+    if (computed_version == p.position_version) return value;
+    hipStream_t s = p.stream;
+    if (tracked_version != p.position_version) {
+        launch_fill_u32(d_max2.get() + slot, 0u, 1, s);
+        launch_max_displacement(p.pos, r0, p.n, b.Lf, b.invL, d_max2.get() + slot, s);
+    }
+    HIP_CHECK(hipMemcpyAsync(h_max2.get(), d_max2.get() + slot, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    p.sync();
+    float m2;
+    std::memcpy(&m2, h_max2.get(), 4);
+    value = std::sqrt(double(m2));
+    computed_version = p.position_version;
+    slot ^= 1u;   // the next fused step writes the slot cleared by this one
+    tracked_version = 0;
+    return value;
+}
+
+// ---------------------------------------------------------------------------
+// neighbour lists from binning (halmd/mdsim/gpu/neighbours/from_binning.cpp)
+// ---------------------------------------------------------------------------
+neighbour::neighbour(particle& p_, binning& bin_, max_displacement& disp_, box const& b_, double const* rc,
+                     unsigned nt1, unsigned nt2, double skin, double occupancy_)
+    : p(p_), bin(bin_), disp(disp_), b(b_), r_skin(skin), occupancy(occupancy_)
+{
+    require(nt1 == nt2 && nt1 == p.nspecies, "neighbour: cutoff matrix must be nspecies x nspecies");
+    require(occupancy > 0 && occupancy <= 1, "neighbour: occupancy must be in (0, 1]");
+    for (int k = 0; k < 3; ++k)
+        require(bin.grid.ncell[k] >= 3,
+                "neighbour: the binning needs at least 3 cells per dimension (box too small for rc + skin)");
+    r_cut.assign(rc, rc + size_t(nt1) * nt2);
+    std::memset(&params, 0, sizeof(params));
+    params.grid = bin.grid;
+    for (int k = 0; k < 3; ++k) { params.L[k] = b.Lf[k]; params.invL[k] = b.invL[k]; }
+    params.ntype = nt2;
+    double rc_max = 0;
+    for (size_t i = 0; i < r_cut.size(); ++i) {
+        params.rr_cut_skin[i] = float(std::pow(r_cut[i] + r_skin, 2));
+        rc_max = std::max(rc_max, r_cut[i]);
+    }
+    // initial capacity from the mean number of neighbours within rc + skin at
+    // the given occupancy (from_binning.cpp:96-119), rounded to 8; grown on demand
+    double const rho = double(p.n) / b.volume();
+    double const nmean = 4.0 / 3.0 * M_PI * std::pow(rc_max + r_skin, 3) * rho;
+    cap = unsigned(round_up(size_t(std::ceil(nmean / occupancy)) + 8, 8));
+    list.resize(size_t(round_up(p.n, WAVE)) * cap);
+    count.resize(p.n);
+    d_maxcount.resize(1);
+    h_maxcount.resize(1);
+}
+
+void neighbour::require_current() {
+    if (built && built_order == p.order_version && disp.compute() <= r_skin / 2) return;
+    for (auto& f : on_prepend_update) f();
+    update();
+    disp.zero();
+    built_order = p.order_version;
+    built = true;
+    for (auto& f : on_append_update) f();
+}
+
+void neighbour::update() {
+    hipStream_t s = p.stream;
+    bin.update();
+    while (true) {
+        launch_fill_u32(d_maxcount, 0u, 1, s);
+        timer.start(s);
+        launch_build_neighbours(p.pos, p.n, bin.begin, bin.end, bin.cell_pos, bin.vals, params, cap, list, count,
+                                d_maxcount, s);
+        timer.stop(s);
+        HIP_CHECK(hipMemcpyAsync(h_maxcount.get(), d_maxcount.get(), 4, hipMemcpyDeviceToHost, s));
+        p.sync();
+        unsigned const m = h_maxcount[0];
+        if (m <= cap) break;
+        // overflow: grow to the observed maximum plus headroom and rebuild
+        cap = unsigned(round_up(size_t(m) + m / 8 + 8, 8));
+        list.resize(size_t(round_up(p.n, WAVE)) * cap);
+    }
+    ++rebuilds;
+}
+
+void neighbour::get_lists(unsigned* counts, unsigned* entries, unsigned stride) {
+    require_current();
+    std::vector<uint32_t> h(size_t(round_up(p.n, WAVE)) * cap);
+    list.download(h.data(), h.size(), p.stream);
+    count.download(counts, p.n, p.stream);
+    p.sync();
+    for (size_t i = 0; i < p.n; ++i) {
+        require(counts[i] <= stride, "neighbour: output stride smaller than a list");
+        for (size_t k = 0; k < counts[i]; ++k)
+            entries[i * stride + k] = h[(i & ~size_t(WAVE - 1)) * cap + k * WAVE + (i & (WAVE - 1))];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pair_trunc (halmd/mdsim/gpu/forces/pair_trunc.hpp:152-337)
+// ---------------------------------------------------------------------------
+pair_trunc::pair_trunc(potential const& pot_, particle& p_, box const& b_, neighbour& nb_, double w)
+    : pot(pot_), p(p_), b(b_), nb(nb_), aux_weight(w)
+{
+    require(pot.ntype1 == p.nspecies && pot.ntype2 == p.nspecies, "pair_trunc: potential shape != nspecies");
+    params = pot.device_params(b);
+    p.forces.push_back(this);
+}
+
+pair_trunc::~pair_trunc() {
+    auto& v = p.forces;
+    v.erase(std::remove(v.begin(), v.end(), static_cast<force_module*>(this)), v.end());
+}
+
+void pair_trunc::apply(bool force_zero) {
+    nb.require_current();
+    bool const aux = p.aux_enabled;
+    // a particle interacting with itself counts each pair twice (pair_trunc.hpp:291-294)
+    float const w = float(aux_weight / 2);
+    timer.start(p.stream);
+    launch_pair_force(p.pos, p.n, nb.list, nb.count, nb.cap, params, aux, force_zero, w, p.force, p.stress, p.n_pad,
+                      p.stream);
+    timer.stop(p.stream);
+}
+
+// ---------------------------------------------------------------------------
+// integrators
+// ---------------------------------------------------------------------------
+verlet::verlet(particle& p_, box const& b_, double dt_) : p(p_), b(b_) {
+    for (int k = 0; k < 3; ++k) { ip.L_hi[k] = b.L_hi[k]; ip.L_lo[k] = b.L_lo[k]; ip.invL[k] = b.invL[k]; }
+    set_timestep(dt_);
+}
+
+void verlet::set_timestep(double dt_) {
+    require(dt_ > 0, "integrator: timestep must be positive");
+    dt = dt_;
+    split_ff(dt, ip.dt_hi, ip.dt_lo);
+    ip.half_dt = float(dt / 2);
+}
+
+void verlet::integrate_kernel(double const* scale) {
+    p.update_force();
+    float4 const* r0 = nullptr;
+    uint32_t* slot = nullptr;
+    uint32_t* clear = nullptr;
+    if (fused_disp && fused_disp->computed_version == p.position_version) {
+        // positions are the ones the displacement was last evaluated for:
+        // track the next step inside the integrate kernel
+        r0 = fused_disp->r0;
+        slot = fused_disp->fuse_slot();
+        clear = fused_disp->d_max2.get() + (fused_disp->slot ^ 1u);
+    }
+    timer_integrate.start(p.stream);
+    launch_verlet_integrate(p.pos, p.vel, p.force, p.image, p.n, ip, scale, r0, slot, clear, p.stream);
+    timer_integrate.stop(p.stream);
+    p.positions_changed();
+    p.velocities_changed();
+}
+
+void verlet::integrate() { integrate_kernel(nullptr); }
+
+void verlet::finalize() {
+    p.update_force();
+    timer_finalize.start(p.stream);
+    launch_verlet_finalize(p.vel, p.force, p.n, ip.half_dt, p.stream);
+    timer_finalize.stop(p.stream);
+    p.velocities_changed();
+}
+
+verlet_nvt_hoover::verlet_nvt_hoover(particle& p_, box const& b_, double dt_, double T, double nu)
+    : verlet(p_, b_, dt_), temperature(T), resonance_frequency(nu)
+{
+    require(T > 0 && nu > 0, "verlet_nvt_hoover: temperature and resonance frequency must be positive");
+    st.resize(1);
+    partials.resize(REDUCE_BLOCKS);
+    nhc_state s0;
+    std::memset(&s0, 0, sizeof(s0));
+    s0.timestep = dt_;
+    s0.scale = 1;
+    upload_state(s0);
+    set_temperature(T);
+}
+
+void verlet_nvt_hoover::upload_state(nhc_state const& s) {
+    st.upload(&s, 1, p.stream);
+    p.sync();
+}
+
+void verlet_nvt_hoover::get_state(nhc_state& out) {
+    st.download(&out, 1, p.stream);
+    p.sync();
+}
+
+// chain masses of Martyna et al. (verlet_nvt_hoover.cpp:80-102)
+void verlet_nvt_hoover::set_temperature(double T) {
+    temperature = T;
+    nhc_state s;
+    get_state(s);
+    s.temperature = T;
+    s.en_kin_target_2 = 3.0 * p.n * T;
+    double const omega_sq = std::pow(2 * M_PI * resonance_frequency, 2);
+    s.mass_xi[0] = 3.0 * p.n * T / omega_sq;
+    s.mass_xi[1] = T / omega_sq;
+    upload_state(s);
+}
+
+void verlet_nvt_hoover::set_timestep(double dt_) {
+    verlet::set_timestep(dt_);
+    if (st.size()) {
+        nhc_state s;
+        get_state(s);
+        s.timestep = dt_;
+        upload_state(s);
+    }
+}
+
+void verlet_nvt_hoover::set_mass(double const* m) {
+    nhc_state s;
+    get_state(s);
+    s.mass_xi[0] = m[0];
+    s.mass_xi[1] = m[1];
+    upload_state(s);
+}
+
+void verlet_nvt_hoover::set_chain(double const* xi, double const* v_xi) {
+    nhc_state s;
+    get_state(s);
+    for (int k = 0; k < 2; ++k) { s.xi[k] = xi[k]; s.v_xi[k] = v_xi[k]; }
+    upload_state(s);
+}
+
+void verlet_nvt_hoover::integrate() {
+    p.update_force();
+    launch_mv2_partials(p.vel, p.n, partials, p.stream);
+    launch_nhc_propagate(partials, st, p.stream);
+    integrate_kernel(&st.get()->scale);
+}
+
+void verlet_nvt_hoover::finalize() {
+    verlet::finalize();
+    launch_mv2_partials(p.vel, p.n, partials, p.stream);
+    launch_nhc_propagate(partials, st, p.stream);
+    launch_rescale_velocity(p.vel, p.n, &st.get()->scale, p.stream);
+    p.velocities_changed();
+}
+
+double verlet_nvt_hoover::internal_energy() {
+    nhc_state s;
+    get_state(s);
+    return s.en_nhc_raw / p.n;
+}
+
+verlet_nvt_andersen::verlet_nvt_andersen(particle& p_, box const& b_, uint64_t seed_, double dt_, double T,
+                                         double rate_)
+    : verlet(p_, b_, dt_), seed(seed_), temperature(T), rate(rate_)
+{
+    require(T > 0 && rate_ >= 0, "verlet_nvt_andersen: invalid temperature or collision rate");
+}
+
+void verlet_nvt_andersen::finalize() {
+    p.update_force();
+    timer_finalize.start(p.stream);
+    launch_andersen_finalize(p.vel, p.force, p.id, p.n, ip.half_dt, float(rate * dt), float(std::sqrt(temperature)),
+                             seed, step++, p.stream);
+    timer_finalize.stop(p.stream);
+    p.velocities_changed();
+}
+
+// ---------------------------------------------------------------------------
+// hilbert (halmd/mdsim/gpu/sorts/hilbert.cpp:40-117)
+// ---------------------------------------------------------------------------
+hilbert::hilbert(particle& p_, box const& b_) : p(p_), b(b_) {
+    float const max_length = std::max(std::max(b.Lf[0], b.Lf[1]), b.Lf[2]);
+    depth = unsigned(std::ceil(std::log(max_length) / M_LN2));
+    depth = std::min(10u, depth);
+    code.resize(p.n);
+    index.resize(p.n);
+    ws.reserve(p.n);
+}
+
+void hilbert::order() {
+    hipStream_t s = p.stream;
+    timer.start(s);
+    launch_hilbert_map(p.pos, p.n, b.Lf, depth, code, index, s);
+    radix_sort(code, index, p.n, 3 * depth, ws, s);
+    p.rearrange(index);
+    timer.stop(s);
+    for (auto& f : on_order) f();
+}
+
+// ---------------------------------------------------------------------------
+// core (halmd/mdsim/core.cpp:30-63)
+// ---------------------------------------------------------------------------
+core::core(particle& p_) : p(p_) {}
+
+void core::mdstep() {
+    require(integ != nullptr, "core: no integrator");
+    if (!wired) {
+        // let the integrate kernel track the displacement of the neighbour
+        // list that serves this particle's pair force
+        for (force_module* f : p.forces)
+            if (auto* pt = dynamic_cast<pair_trunc*>(f))
+                if (&pt->nb.disp.p == &p) integ->fused_disp = &pt->nb.disp;
+        wired = true;
+    }
+    integ->integrate();
+    ++step;
+    if (sort && sort_interval && step % sort_interval == 0) sort->order();
+    integ->finalize();
+}
+
+// ---------------------------------------------------------------------------
+// thermodynamics (halmd/observables/gpu/thermodynamics.cpp,
+// halmd/observables/thermodynamics.hpp:80-90)
+// ---------------------------------------------------------------------------
+thermodynamics::thermodynamics(particle& p_, box const& b_) : p(p_), b(b_) {
+    partials.resize(size_t(REDUCE_BLOCKS) * THERMO_NQ);
+    out.resize(THERMO_NQ);
+    h_out.resize(THERMO_NQ);
+    std::fill(v, v + THERMO_NQ, 0.0);
+}
+
+void thermodynamics::compute(bool need_aux) {
+    bool const fresh = pos_version == p.position_version && vel_version == p.velocity_version;
+    if (fresh && (!need_aux || have_aux)) return;
+    p.update_force(need_aux);
+    bool const aux = p.aux_valid;
+    launch_thermo_partials(p.vel, p.force, p.stress, p.n_pad, p.n, aux, partials, p.stream);
+    launch_reduce_final(partials, THERMO_NQ, out, p.stream);
+    HIP_CHECK(hipMemcpyAsync(h_out.get(), out.get(), THERMO_NQ * sizeof(double), hipMemcpyDeviceToHost, p.stream));
+    p.sync();
+    std::copy(h_out.get(), h_out.get() + THERMO_NQ, v);
+    pos_version = p.position_version;
+    vel_version = p.velocity_version;
+    have_aux = aux;
+}
+
+double thermodynamics::en_kin() { compute(false); return 0.5 * v[0] / p.n; }
+double thermodynamics::en_pot() { compute(true); return v[5] / p.n; }
+double thermodynamics::virial() { compute(true); return v[6] / p.n; }
+double thermodynamics::temp() { return 2 * en_kin() / 3; }
+double thermodynamics::pressure() {
+    double const rho = p.n / b.volume();
+    return rho * (temp() + virial() / 3);
+}
+double thermodynamics::en_tot() { return en_pot() + en_kin(); }
+void thermodynamics::v_cm(double* o) {
+    compute(false);
+    for (int k = 0; k < 3; ++k) o[k] = v[1 + k] / v[4];
+}
+// potential part plus kinetic part m v (x) v, per volume (thermodynamics.cpp stress_tensor)
+void thermodynamics::stress_tensor(double* o) {
+    compute(true);
+    for (int k = 0; k < 6; ++k) o[k] = v[7 + k];
+}
+
+// ---------------------------------------------------------------------------
+// wavevector (halmd/observables/utility/wavevector.cpp:39-219,
+// halmd/algorithm/host/pick_lattice_points.hpp)
+// ---------------------------------------------------------------------------
+wavevector::wavevector(std::vector<double> const& k, double const* L, double tolerance, unsigned max_count)
+    : wavenumber(k)
+{
+    require(!k.empty(), "wavevector: empty wavenumber list");
+    double unit[3];
+    for (int j = 0; j < 3; ++j) unit[j] = 2 * M_PI / L[j];
+    size_t const nk = k.size();
+    std::vector<unsigned> cnt(nk, 0);
+    double const r_max = *std::max_element(k.begin(), k.end()) * (1 + tolerance);
+    unsigned msm[3];
+    for (int j = 0; j < 3; ++j) msm[j] = unsigned(r_max / unit[j]);
+    msm[1] += msm[0];
+    msm[2] += msm[1];
+    std::multimap<size_t, std::array<double, 3>> mm;
+    unsigned idx[3] = {0, 0, 0};
+    while (idx[2] <= msm[2]) {
+        unsigned hkl[3] = {idx[0], idx[1] - idx[0], idx[2] - idx[1]};
+        if (std::gcd(std::gcd(hkl[0], hkl[1]), hkl[2]) == 1) {
+            double const r0[3] = {unit[0] * hkl[0], unit[1] * hkl[1], unit[2] * hkl[2]};
+            double const r0n = std::sqrt(r0[0] * r0[0] + r0[1] * r0[1] + r0[2] * r0[2]);
+            for (size_t i = 0; i < nk; ++i) {
+                if (cnt[i] >= max_count) continue;
+                unsigned const nn = unsigned(std::floor(k[i] / r0n + 0.5));
+                if (nn > 0 && std::abs(nn * r0n - k[i]) <= k[i] * tolerance) {
+                    mm.insert({i, {nn * r0[0], nn * r0[1], nn * r0[2]}});
+                    ++cnt[i];
+                }
+            }
+        }
+        ++idx[0];
+        for (int j = 0; j < 2; ++j) {
+            if (idx[j] <= msm[j]) break;
+            idx[j] = 0;
+            ++idx[j + 1];
+        }
+    }
+    std::vector<double> kept;
+    for (size_t i = 0; i < nk; ++i) {
+        auto range = mm.equal_range(i);
+        if (range.first == range.second) continue;   // incompatible wavenumber is discarded
+        size_t const first = q.size() / 3;
+        for (auto it = range.first; it != range.second; ++it)
+            for (int j = 0; j < 3; ++j) q.push_back(it->second[j]);
+        shell.emplace_back(first, q.size() / 3);
+        kept.push_back(k[i]);
+    }
+    require(!q.empty(), "wavevector: constraints on wavevectors are incompatible with the box");
+    wavenumber = kept;
+}
+
+wavevector::wavevector(std::vector<double> const& k, double const* L) : wavenumber(k) {
+    require(!k.empty(), "wavevector: empty wavenumber list");
+    std::sort(wavenumber.begin(), wavenumber.end());
+    double const q_max = wavenumber.back();
+    double unit[3];
+    int max_n[3];
+    for (int j = 0; j < 3; ++j) {
+        unit[j] = 2 * M_PI / L[j];
+        max_n[j] = int(std::floor(q_max / unit[j]));
+    }
+    std::vector<std::array<double, 3>> all;
+    for (int z = -max_n[2]; z <= max_n[2]; ++z)
+        for (int y = -max_n[1]; y <= max_n[1]; ++y)
+            for (int x = -max_n[0]; x <= max_n[0]; ++x) {
+                std::array<double, 3> v{unit[0] * x, unit[1] * y, unit[2] * z};
+                if (v[0] * v[0] + v[1] * v[1] + v[2] * v[2] < q_max * q_max) all.push_back(v);
+            }
+    size_t start = 0;
+    std::vector<double> kept;
+    for (double kk : wavenumber) {
+        double const bound = kk * kk;
+        auto mid = std::partition(all.begin() + start, all.end(), [&](std::array<double, 3> const& v) {
+            return v[0] * v[0] + v[1] * v[1] + v[2] * v[2] < bound;
+        });
+        size_t const next = size_t(mid - all.begin());
+        if (next == start) continue;
+        shell.emplace_back(start, next);
+        kept.push_back(kk);
+        start = next;
+    }
+    for (auto const& v : all) for (int j = 0; j < 3; ++j) q.push_back(v[j]);
+    wavenumber = kept;
+}
+
+// ---------------------------------------------------------------------------
+// density_mode (halmd/observables/gpu/density_mode.cpp:77-137)
+// ---------------------------------------------------------------------------
+density_mode::density_mode(particle& p_, wavevector const& wv_) : p(p_), wv(wv_) {
+    unsigned const nq = unsigned(wv.q.size() / 3);
+    q.resize(std::max(3u * nq, 3u));
+    q.upload(wv.q.data(), wv.q.size(), p.stream);
+    partials.resize(size_t(div_up(nq, DM_KCHUNK)) * DM_BLOCKS * 2 * DM_KCHUNK);
+    out.resize(2 * size_t(nq));
+    h_out.resize(2 * size_t(nq));
+    p.sync();
+}
+
+std::shared_ptr<std::vector<std::complex<double>>> density_mode::acquire() {
+    if (last && version == p.position_version) return last;
+    unsigned const nq = unsigned(wv.q.size() / 3);
+    timer.start(p.stream);
+    launch_density_mode(p.pos, p.n, q, nq, partials, out, p.stream);
+    timer.stop(p.stream);
+    out.download(h_out.data(), h_out.size(), p.stream);
+    p.sync();
+    auto r = std::make_shared<std::vector<std::complex<double>>>(nq);
+    for (unsigned k = 0; k < nq; ++k) (*r)[k] = {h_out[2 * k], h_out[2 * k + 1]};
+    last = r;
+    version = p.position_version;
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// blocking scheme (halmd/observables/dynamics/blocking_scheme.cpp:35-200)
+// ---------------------------------------------------------------------------
+blocking_scheme::blocking_scheme(particle& p_, box const& b_, double timestep, double max_lag, double resolution,
+                                 unsigned bsz, unsigned shift, unsigned sep, uint64_t step0)
+    : p(p_), b(b_), block_size(bsz), separation(sep)
+{
+    require(block_size >= 2, "blocking_scheme: minimum block size is 2");
+    require(shift != 0, "blocking_scheme: coarse-graining shift must be non-zero");
+    require(shift < block_size, "blocking_scheme: coarse-graining shift must be less than block size");
+    require(resolution > 0, "blocking_scheme: resolution must be positive");
+    uint64_t const max_interval = uint64_t(std::llround(max_lag / timestep));
+    uint64_t s = uint64_t(std::llround(resolution / timestep));
+    require(s > 0, "blocking_scheme: resolution below one time step");
+    while (s <= max_interval) {
+        interval.push_back(s);
+        if (shift > 1) {
+            uint64_t const s_odd = s * shift;
+            if (s_odd <= max_interval) interval.push_back(s_odd);
+        }
+        s *= block_size;
+    }
+    require(!interval.empty(), "blocking_scheme: sampling interval of dynamic correlations is too large");
+    origin.assign(interval.size(), step0);
+    pos_blocks.resize(interval.size());
+    rho_blocks.resize(interval.size());
+    msd.resize(interval.size() * block_size);
+    msd_partials.resize(size_t(REDUCE_BLOCKS) * 2 * block_size);
+    msd_out.resize(2 * block_size);
+    h_msd.resize(2 * block_size);
+}
+
+blocking_scheme::pos_sample blocking_scheme::take_position_sample() {
+    auto s = std::make_shared<dbuf<double>>(3 * size_t(p.n));
+    launch_sample_position(p.pos, p.image, p.id, p.n, b.L, *s, p.stream);
+    return s;
+}
+
+void blocking_scheme::sample(uint64_t step) {
+    pos_sample ps;
+    rho_sample rs;
+    for (size_t i = 0; i < interval.size(); ++i) {
+        if (step >= origin[i] && (step - origin[i]) % interval[i] == 0) {
+            if (msd_enabled) {
+                if (!ps) ps = take_position_sample();
+                pos_blocks[i].push_back(ps);
+                if (pos_blocks[i].size() > block_size) pos_blocks[i].erase(pos_blocks[i].begin());
+            }
+            if (dm) {
+                if (!rs) rs = dm->acquire();
+                rho_blocks[i].push_back(rs);
+                if (rho_blocks[i].size() > block_size) rho_blocks[i].erase(rho_blocks[i].begin());
+            }
+            size_t const sz = msd_enabled ? pos_blocks[i].size() : rho_blocks[i].size();
+            if (sz == block_size) process(i);
+        }
+    }
+}
+
+void blocking_scheme::finalise() {
+    for (size_t i = 0; i < interval.size(); ++i) {
+        while (!(msd_enabled ? pos_blocks[i].empty() : rho_blocks[i].empty())) process(i);
+    }
+}
+
+void blocking_scheme::process(size_t level) {
+    if (msd_enabled && !pos_blocks[level].empty()) {
+        auto const& blk = pos_blocks[level];
+        unsigned const nl = unsigned(blk.size());
+        for (unsigned l = 0; l < nl; ++l)
+            launch_msd_partials(*blk.front(), *blk[l], p.n, msd_partials, 2 * nl, 2 * l, p.stream);
+        launch_reduce_final(msd_partials, 2 * nl, msd_out, p.stream);
+        HIP_CHECK(hipMemcpyAsync(h_msd.get(), msd_out.get(), 2 * nl * sizeof(double), hipMemcpyDeviceToHost, p.stream));
+        p.sync();
+        for (unsigned l = 0; l < nl; ++l) {
+            double const s1 = h_msd[2 * l], s2 = h_msd[2 * l + 1];
+            accumulator a;
+            a.n = p.n;
+            a.m = s1 / p.n;
+            a.v = std::max(0.0, s2 - s1 * s1 / p.n);
+            msd[level * block_size + l].merge(a);
+        }
+    }
+    if (dm && !rho_blocks[level].empty()) {
+        auto const& blk = rho_blocks[level];
+        size_t const nshell = dm->wv.shell.size();
+        if (isf.empty()) isf.resize(interval.size() * block_size * nshell);
+        auto const& r1 = *blk.front();
+        for (size_t l = 0; l < blk.size(); ++l) {
+            auto const& r2 = *blk[l];
+            for (size_t sh = 0; sh < nshell; ++sh) {
+                accumulator a;
+                for (size_t k = dm->wv.shell[sh].first; k < dm->wv.shell[sh].second; ++k)
+                    a((r1[k].real() * r2[k].real() + r1[k].imag() * r2[k].imag()) / p.n);
+                isf[(level * block_size + l) * nshell + sh].merge(a);
+            }
+        }
+    }
+    uint64_t const iv = interval[level];
+    uint64_t const skip = std::max<uint64_t>((separation + iv - 1) / iv, 1);
+    origin[level] += skip * iv;
+    for (uint64_t k = 0; k < skip; ++k) {
+        if (!pos_blocks[level].empty()) pos_blocks[level].erase(pos_blocks[level].begin());
+        if (!rho_blocks[level].empty()) rho_blocks[level].erase(rho_blocks[level].begin());
+    }
+}
+
+// ---------------------------------------------------------------------------
+// sampler (halmd/observables/sampler.cpp:42-130)
+// ---------------------------------------------------------------------------
+sampler::sampler(core& c_) : c(c_) {}
+
+void sampler::sample_now() {
+    for (auto& t : thermo) {
+        if (t.every && c.step % t.every == 0) {
+            double const ep = t.t->en_pot(), ek = t.t->en_kin(), w = t.t->virial();
+            double const en_nhc = t.t->integ ? t.t->integ->internal_energy() : 0.0;
+            t.rec.insert(t.rec.end(), {double(c.step), ep, ek, w, en_nhc});
+        }
+    }
+    for (auto& b : blocks) b.bs->sample(c.step);
+    for (auto& m : modes) {
+        if (m.every && c.step % m.every == 0) {
+            auto r = m.dm->acquire();
+            m.rec.push_back(double(c.step));
+            for (auto const& z : *r) { m.rec.push_back(z.real()); m.rec.push_back(z.imag()); }
+        }
+    }
+}
+
+void sampler::run(uint64_t steps) {
+    if (!started) {
+        sample_now();
+        started = true;
+    }
+    for (uint64_t k = 0; k < steps; ++k) {
+        // on_prepare: request aux variables for the force of a sample step
+        uint64_t const next = c.step + 1;
+        for (auto& t : thermo)
+            if (t.every && next % t.every == 0) c.p.aux_enable();
+        c.mdstep();
+        sample_now();
+    }
+}
+
+} // namespace halmd

@@ -1,0 +1,413 @@
+// C++ module layer of the MI355X MD backend.
+//
+// One class per module of the reference's GPU backend, with the same
+// responsibilities and lazy-update protocol:
+//   particle          halmd/mdsim/gpu/particle.{hpp,cpp}
+//   box               halmd/mdsim/box.{hpp,cpp}
+//   binning           halmd/mdsim/gpu/binning.{hpp,cpp}
+//   max_displacement  halmd/mdsim/gpu/max_displacement.{hpp,cpp}
+//   neighbour         halmd/mdsim/gpu/neighbours/from_binning.{hpp,cpp}
+//   potential         halmd/mdsim/gpu/potentials/pair/{lennard_jones,morse}.cpp + truncations/*
+//   pair_trunc        halmd/mdsim/gpu/forces/pair_trunc.hpp
+//   verlet, verlet_nvt_hoover, verlet_nvt_andersen   halmd/mdsim/gpu/integrators/*
+//   hilbert           halmd/mdsim/gpu/sorts/hilbert.{hpp,cpp}
+//   core              halmd/mdsim/core.cpp
+//   sampler           halmd/observables/sampler.cpp
+//   thermodynamics    halmd/observables/gpu/thermodynamics.cpp
+//   wavevector        halmd/observables/utility/wavevector.cpp
+//   density_mode      halmd/observables/gpu/density_mode.cpp
+//   blocking_scheme, msd / isf correlations   halmd/observables/dynamics/*
+// The reference's cache<> observers become version counters on the particle;
+// its signals become explicit hooks.  All device work of one simulation runs on
+// the particle's stream; host synchronisation happens only where a value must
+// reach the host (rebuild decision, overflow check, observable getters).
+#pragma once
+
+#include <cmath>
+#include <complex>
+#include <functional>
+#include <memory>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace halmd {
+
+// ---------------------------------------------------------------------------
+// event-pair timer for one kernel class (HIP events on the module's stream)
+// ---------------------------------------------------------------------------
+class kernel_timer {
+public:
+    ~kernel_timer();
+    void enable(size_t max_launches);
+    void disable();
+    void start(hipStream_t s);
+    void stop(hipStream_t s);
+    // synchronises, returns (total ms, launches) and resets
+    std::pair<double, size_t> collect();
+    bool enabled() const { return !events_.empty(); }
+private:
+    std::vector<hipEvent_t> events_;
+    size_t next_ = 0;
+};
+
+class box {
+public:
+    explicit box(double const* length);
+    double L[3];
+    float L_hi[3], L_lo[3], Lf[3], invL[3];
+    double volume() const { return L[0] * L[1] * L[2]; }
+};
+
+class force_module;
+
+class particle {
+public:
+    particle(unsigned n, unsigned nspecies);
+    ~particle();
+
+    unsigned n, n_pad, nspecies;
+    hipStream_t stream;
+
+    dbuf<float4> pos, vel, force;     // pos/vel: 2 float4 per particle (hi, lo)
+    dbuf<float> stress;               // 6 x n_pad, column-major
+    dbuf<int4> image;
+    dbuf<uint32_t> id, reverse_id;
+    dbuf<float4> pos_alt, vel_alt;
+    dbuf<int4> image_alt;
+    dbuf<uint32_t> id_alt;
+
+    // cache versions (halmd/utility/cache.hpp semantics)
+    uint64_t position_version = 1, velocity_version = 1, species_version = 1, order_version = 1;
+    bool force_valid = false, aux_valid = false, aux_enabled = false;
+    std::vector<force_module*> forces;
+
+    void set_position(double const* r);
+    void get_position(double* r);
+    void set_velocity(double const* v);
+    void get_velocity(double* v);
+    void set_species(unsigned const* s);
+    void get_species(unsigned* s);
+    void set_mass(double const* m);
+    void get_mass(double* m);
+    void set_image(int const* im);
+    void get_image(int* im);
+    void get_force(double* f);
+    void get_potential_energy(double* u);
+    void get_stress_pot(double* s);
+    void get_id(unsigned* v);
+    void get_reverse_id(unsigned* v);
+
+    void positions_changed() { ++position_version; force_valid = false; aux_valid = false; }
+    void velocities_changed() { ++velocity_version; }
+    // request en_pot and stress with the next force evaluation
+    void aux_enable() { aux_enabled = true; }
+    // make the force (and en_pot / stress if need_aux or aux_enable()d) valid
+    // for the current positions (particle.cpp:221-253)
+    void update_force(bool need_aux = false);
+    // reorder by a device permutation perm[new] = old
+    void rearrange(uint32_t const* perm);
+    void sync();
+
+private:
+    void download_hilo(dbuf<float4> const& b, double* out);
+    void upload_hilo(dbuf<float4>& b, double const* in, bool species);
+    std::vector<float4> host_tmp_;
+};
+
+class potential {
+public:
+    // kind: POTENTIAL_*; parameter matrices are ntype x ntype row-major
+    potential(int kind, unsigned ntype1, unsigned ntype2, double const* epsilon, double const* sigma,
+              double const* r_min, double const* distortion);
+    // apply a truncation (TRUNC_*); r_cut_sigma in units of sigma
+    void truncate(int trunc, double const* r_cut_sigma, double smoothing);
+    int kind, trunc = -1;
+    unsigned ntype1, ntype2;
+    std::vector<double> epsilon, sigma, r_min, distortion, r_cut_sigma, r_cut, rr_cut, en_cut, force_cut;
+    double h = 0;
+    double r_cut_max() const;
+    force_params device_params(box const& b) const;
+    // host evaluation in FP64 (diagnostics and tables)
+    void eval(double rr, unsigned a, unsigned b, double& fval, double& en) const;
+private:
+    void base(double rr, unsigned ab, double& fval, double& en) const;
+};
+
+class binning {
+public:
+    binning(particle& p, box const& b, double const* r_cut, unsigned ntype1, unsigned ntype2, double skin,
+            double occupancy);
+    particle& p;
+    box const& b;
+    double r_skin, occupancy, r_cut_max;
+    cell_grid grid;
+    unsigned ncells;
+    dbuf<uint32_t> keys, vals, begin, end;
+    dbuf<float4> cell_pos;
+    radix_sort_workspace ws;
+    uint64_t version = 0;             // position version of the last update
+    void update();
+    void cell_lists(unsigned* begin_out, unsigned* end_out, unsigned* index_out);
+    kernel_timer timer;
+};
+
+class max_displacement {
+public:
+    max_displacement(particle& p, box const& b);
+    particle& p;
+    box const& b;
+    dbuf<float4> r0;
+    dbuf<uint32_t> d_max2;            // two slots: current step and next step
+    hbuf<uint32_t> h_max2;
+    unsigned slot = 0;
+    uint64_t tracked_version = 0;     // position version whose max is in slot (fused in integrate)
+    uint64_t computed_version = 0;
+    double value = 0;
+    void zero();
+    double compute();
+    // pointers for the integrate kernel; marks the resulting version as tracked
+    uint32_t* fuse_slot();
+};
+
+class hilbert;
+
+class neighbour {
+public:
+    neighbour(particle& p, binning& bin, max_displacement& disp, box const& b, double const* r_cut, unsigned ntype1,
+              unsigned ntype2, double skin, double occupancy);
+    particle& p;
+    binning& bin;
+    max_displacement& disp;
+    box const& b;
+    double r_skin, occupancy;
+    std::vector<double> r_cut;
+    neighbour_params params;
+    unsigned cap;
+    dbuf<uint32_t> list, count, d_maxcount;
+    hbuf<uint32_t> h_maxcount;
+    uint64_t built_order = 0;
+    bool built = false;
+    uint64_t rebuilds = 0;
+    std::vector<std::function<void()>> on_prepend_update, on_append_update;
+    // lazy access: rebuild when reordered or max displacement > skin / 2
+    void require_current();
+    void update();
+    void get_lists(unsigned* counts, unsigned* entries, unsigned stride);
+    kernel_timer timer;
+};
+
+class force_module {
+public:
+    virtual ~force_module() = default;
+    virtual void apply(bool force_zero) = 0;
+};
+
+class pair_trunc : public force_module {
+public:
+    pair_trunc(potential const& pot, particle& p, box const& b, neighbour& nb, double aux_weight);
+    ~pair_trunc() override;
+    potential const& pot;
+    particle& p;
+    box const& b;
+    neighbour& nb;
+    double aux_weight;
+    force_params params;
+    void apply(bool force_zero) override;
+    kernel_timer timer;
+};
+
+class integrator {
+public:
+    virtual ~integrator() = default;
+    virtual void integrate() = 0;
+    virtual void finalize() = 0;
+    virtual void set_timestep(double dt) = 0;
+    virtual double timestep() const = 0;
+    virtual double internal_energy() { return 0; }
+    max_displacement* fused_disp = nullptr;   // track displacement in the integrate kernel
+};
+
+class verlet : public integrator {
+public:
+    verlet(particle& p, box const& b, double dt);
+    particle& p;
+    box const& b;
+    double dt;
+    integrate_params ip;
+    void set_timestep(double dt_) override;
+    double timestep() const override { return dt; }
+    void integrate() override;
+    void finalize() override;
+    kernel_timer timer_integrate, timer_finalize;
+protected:
+    void integrate_kernel(double const* scale);
+};
+
+class verlet_nvt_hoover : public verlet {
+public:
+    verlet_nvt_hoover(particle& p, box const& b, double dt, double temperature, double resonance_frequency);
+    double temperature, resonance_frequency;
+    dbuf<nhc_state> st;
+    dbuf<double> partials;
+    void set_temperature(double T);
+    void set_timestep(double dt_) override;
+    void set_mass(double const* m);
+    void get_state(nhc_state& out);
+    void set_chain(double const* xi, double const* v_xi);
+    void integrate() override;
+    void finalize() override;
+    double internal_energy() override;
+private:
+    void upload_state(nhc_state const& s);
+};
+
+class verlet_nvt_andersen : public verlet {
+public:
+    verlet_nvt_andersen(particle& p, box const& b, uint64_t seed, double dt, double temperature, double rate);
+    uint64_t seed, step = 0;
+    double temperature, rate;
+    void finalize() override;
+};
+
+class hilbert {
+public:
+    hilbert(particle& p, box const& b);
+    particle& p;
+    box const& b;
+    unsigned depth;
+    dbuf<uint32_t> code, index;
+    radix_sort_workspace ws;
+    void order();
+    std::vector<std::function<void()>> on_order;
+    kernel_timer timer;
+};
+
+class core {
+public:
+    explicit core(particle& p);
+    particle& p;
+    integrator* integ = nullptr;
+    hilbert* sort = nullptr;
+    unsigned sort_interval = 0;       // 0: sort only when the neighbour list asks (rebuild hook)
+    uint64_t step = 0;
+    bool wired = false;
+    void mdstep();
+};
+
+// ---------------------------------------------------------------------------
+// observables
+// ---------------------------------------------------------------------------
+class thermodynamics {
+public:
+    thermodynamics(particle& p, box const& b);
+    particle& p;
+    box const& b;
+    dbuf<double> partials, out;
+    hbuf<double> h_out;
+    integrator* integ = nullptr;      // for the NHC internal energy
+    uint64_t pos_version = 0, vel_version = 0;
+    bool have_aux = false;
+    double v[THERMO_NQ];
+    void compute(bool need_aux);
+    double en_kin();
+    double en_pot();
+    double virial();
+    double temp();
+    double pressure();
+    double en_tot();
+    void v_cm(double* out);
+    void stress_tensor(double* out);
+};
+
+class wavevector {
+public:
+    // shells of lattice wavevectors with |q| within tolerance of each wavenumber
+    wavevector(std::vector<double> const& wavenumber, double const* box_length, double tolerance, unsigned max_count);
+    // dense grid |q| < max(wavenumber), partitioned into shells
+    wavevector(std::vector<double> const& wavenumber, double const* box_length);
+    std::vector<double> wavenumber;
+    std::vector<double> q;                               // 3 per wavevector
+    std::vector<std::pair<size_t, size_t>> shell;
+};
+
+class density_mode {
+public:
+    density_mode(particle& p, wavevector const& wv);
+    particle& p;
+    wavevector const& wv;
+    dbuf<double> q, partials, out;
+    std::vector<double> h_out;
+    // rho_k for all wavevectors, (re, im) pairs; synchronises
+    std::shared_ptr<std::vector<std::complex<double>>> acquire();
+    uint64_t version = 0;
+    std::shared_ptr<std::vector<std::complex<double>>> last;
+    kernel_timer timer;
+};
+
+// Welford accumulator (halmd/numeric/accumulator.hpp:55-135)
+struct accumulator {
+    uint64_t n = 0;
+    double m = 0, v = 0;
+    void operator()(double x) {
+        double const t = x - m;
+        n++;
+        m += t / double(n);
+        v += t * (x - m);
+    }
+    void merge(accumulator const& a) {
+        if (a.n > 0) {
+            uint64_t const c = n + a.n;
+            v += a.v;
+            double const d = m - a.m;
+            v += d * d * double(n) * double(a.n) / double(c);
+            m = (double(n) * m + double(a.n) * a.m) / double(c);
+            n = c;
+        }
+    }
+    double error_of_mean() const { return n > 1 ? std::sqrt((v / double(n)) / double(n - 1)) : 0.0; }
+};
+
+// Multi-tau blocking scheme with MSD (device position samples) and ISF
+// (density-mode samples) correlations.
+class blocking_scheme {
+public:
+    blocking_scheme(particle& p, box const& b, double timestep, double max_lag, double resolution,
+                    unsigned block_size, unsigned shift, unsigned separation, uint64_t step0);
+    particle& p;
+    box const& b;
+    unsigned block_size, separation;
+    std::vector<uint64_t> interval, origin;
+    density_mode* dm = nullptr;       // ISF source (optional)
+    bool msd_enabled = false;
+    std::vector<std::pair<size_t, size_t>> shell;
+    using pos_sample = std::shared_ptr<dbuf<double>>;
+    using rho_sample = std::shared_ptr<std::vector<std::complex<double>>>;
+    std::vector<std::vector<pos_sample>> pos_blocks;
+    std::vector<std::vector<rho_sample>> rho_blocks;
+    std::vector<accumulator> msd, isf;
+    dbuf<double> msd_partials, msd_out;
+    hbuf<double> h_msd;
+    void sample(uint64_t step);
+    void finalise();
+private:
+    void process(size_t level);
+    pos_sample take_position_sample();
+};
+
+class sampler {
+public:
+    sampler(core& c);
+    core& c;
+    struct thermo_slot { thermodynamics* t; uint64_t every; std::vector<double> rec; };
+    std::vector<thermo_slot> thermo;
+    struct block_slot { blocking_scheme* bs; };
+    std::vector<block_slot> blocks;
+    struct mode_slot { density_mode* dm; uint64_t every; std::vector<double> rec; };
+    std::vector<mode_slot> modes;
+    bool started = false;
+    void run(uint64_t steps);
+    void sample_now();
+};
+
+} // namespace halmd

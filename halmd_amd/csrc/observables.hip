@@ -1,0 +1,153 @@
+// Observables on the device for gfx950: density modes, position samples and
+// mean-square displacement sums.
+//
+// density modes rho_k = sum_j exp(-i k.r_j)
+// (halmd/observables/gpu/density_mode_kernel.cu:45-108; sign convention of
+// halmd/observables/host/density_mode.cpp:85-87).  The reference re-reads all
+// positions once per wavevector and evaluates the phase in FP32 on the high
+// words.  Here each thread keeps DM_KCHUNK wavevectors in registers per pass
+// over its particles, the phase k.r is formed in FP64 from the double-word
+// position and reduced to [-1/2, 1/2] revolutions, and sin/cos of the reduced
+// phase are evaluated in FP32 (sincospi), so the phase error no longer grows
+// with |k| |r|.  Sums are FP64; partials per block are combined in a fixed
+// order.
+#include "kernels.hpp"
+
+namespace halmd {
+namespace {
+
+constexpr unsigned DM_THREADS = 256;
+
+__global__ __launch_bounds__(DM_THREADS) void density_mode_kernel(float4 const* __restrict__ pos, unsigned n,
+                                                                  double const* __restrict__ q, unsigned nq,
+                                                                  double* __restrict__ partials) {
+    __shared__ double s_q[DM_KCHUNK][3];
+    __shared__ double s_red[DM_THREADS / WAVE][2 * DM_KCHUNK];
+    unsigned const chunk = blockIdx.y;
+    if (threadIdx.x < 3 * DM_KCHUNK) {
+        unsigned const kk = threadIdx.x / 3, c = threadIdx.x % 3;
+        unsigned const k = chunk * DM_KCHUNK + kk;
+        // wavevectors in revolutions per unit length: k / 2 pi
+        s_q[kk][c] = k < nq ? q[3 * size_t(k) + c] * 0.15915494309189535 : 0.0;
+    }
+    __syncthreads();
+
+    double re[DM_KCHUNK], im[DM_KCHUNK];
+#pragma unroll
+    for (unsigned kk = 0; kk < DM_KCHUNK; ++kk) { re[kk] = 0; im[kk] = 0; }
+
+    for (unsigned i = blockIdx.x * DM_THREADS + threadIdx.x; i < n; i += gridDim.x * DM_THREADS) {
+        float4 const hi = pos[2 * size_t(i)], lo = pos[2 * size_t(i) + 1];
+        double const x = double(hi.x) + double(lo.x);
+        double const y = double(hi.y) + double(lo.y);
+        double const z = double(hi.z) + double(lo.z);
+#pragma unroll
+        for (unsigned kk = 0; kk < DM_KCHUNK; ++kk) {
+            double t = s_q[kk][0] * x + s_q[kk][1] * y + s_q[kk][2] * z;
+            t -= rint(t);
+            float s, c;
+            sincospif(2.0f * float(t), &s, &c);
+            re[kk] += c;
+            im[kk] -= s;
+        }
+    }
+
+    unsigned const lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+#pragma unroll
+    for (unsigned kk = 0; kk < DM_KCHUNK; ++kk) {
+        double a = re[kk], b = im[kk];
+        for (int off = WAVE / 2; off > 0; off >>= 1) {
+            a += __shfl_xor(a, off, WAVE);
+            b += __shfl_xor(b, off, WAVE);
+        }
+        if (lane == 0) { s_red[wave][2 * kk] = a; s_red[wave][2 * kk + 1] = b; }
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * DM_KCHUNK) {
+        double t = 0;
+        for (unsigned w = 0; w < DM_THREADS / WAVE; ++w) t += s_red[w][threadIdx.x];
+        partials[(size_t(chunk) * gridDim.x + blockIdx.x) * 2 * DM_KCHUNK + threadIdx.x] = t;
+    }
+}
+
+// out[2k + c] = sum over blocks of partials for wavevector k
+__global__ void density_mode_final_kernel(double const* __restrict__ partials, unsigned nq, unsigned nblocks,
+                                          double* __restrict__ out) {
+    unsigned const k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nq) return;
+    unsigned const chunk = k / DM_KCHUNK, kk = k % DM_KCHUNK;
+    double a = 0, b = 0;
+    for (unsigned blk = 0; blk < nblocks; ++blk) {
+        double const* p = partials + (size_t(chunk) * nblocks + blk) * 2 * DM_KCHUNK;
+        a += p[2 * kk];
+        b += p[2 * kk + 1];
+    }
+    out[2 * size_t(k)] = a;
+    out[2 * size_t(k) + 1] = b;
+}
+
+__global__ void sample_position_kernel(float4 const* __restrict__ pos, int4 const* __restrict__ image,
+                                       uint32_t const* __restrict__ id, unsigned n, double Lx, double Ly, double Lz,
+                                       double* __restrict__ out) {
+    unsigned const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 const hi = pos[2 * size_t(i)], lo = pos[2 * size_t(i) + 1];
+    int4 const im = image[i];
+    size_t const o = 3 * size_t(id[i]);
+    out[o] = double(hi.x) + double(lo.x) + double(im.x) * Lx;
+    out[o + 1] = double(hi.y) + double(lo.y) + double(im.y) * Ly;
+    out[o + 2] = double(hi.z) + double(lo.z) + double(im.z) * Lz;
+}
+
+__global__ __launch_bounds__(REDUCE_THREADS) void msd_partials_kernel(double const* __restrict__ r1,
+                                                                      double const* __restrict__ r2, unsigned n,
+                                                                      double* __restrict__ partials, unsigned nq,
+                                                                      unsigned qoff) {
+    __shared__ double s_red[REDUCE_THREADS / WAVE][2];
+    double a = 0, b = 0;
+    for (unsigned i = blockIdx.x * REDUCE_THREADS + threadIdx.x; i < n; i += REDUCE_BLOCKS * REDUCE_THREADS) {
+        double const dx = r2[3 * size_t(i)] - r1[3 * size_t(i)];
+        double const dy = r2[3 * size_t(i) + 1] - r1[3 * size_t(i) + 1];
+        double const dz = r2[3 * size_t(i) + 2] - r1[3 * size_t(i) + 2];
+        double const d2 = dx * dx + dy * dy + dz * dz;
+        a += d2;
+        b += d2 * d2;
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, WAVE);
+        b += __shfl_xor(b, off, WAVE);
+    }
+    if (threadIdx.x % WAVE == 0) { s_red[threadIdx.x / WAVE][0] = a; s_red[threadIdx.x / WAVE][1] = b; }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        double t = 0;
+        for (unsigned w = 0; w < REDUCE_THREADS / WAVE; ++w) t += s_red[w][threadIdx.x];
+        partials[size_t(blockIdx.x) * nq + qoff + threadIdx.x] = t;
+    }
+}
+
+} // namespace
+
+void launch_density_mode(float4 const* pos, unsigned n, double const* q, unsigned nq, double* partials, double* out,
+                         hipStream_t s) {
+    if (!nq) return;
+    dim3 grid(DM_BLOCKS, div_up(nq, DM_KCHUNK));
+    density_mode_kernel<<<grid, DM_THREADS, 0, s>>>(pos, n, q, nq, partials);
+    density_mode_final_kernel<<<div_up(nq, 64), 64, 0, s>>>(partials, nq, DM_BLOCKS, out);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_sample_position(float4 const* pos, int4 const* image, uint32_t const* id, unsigned n, double const* L,
+                            double* out, hipStream_t s) {
+    if (!n) return;
+    sample_position_kernel<<<div_up(n, 256), 256, 0, s>>>(pos, image, id, n, L[0], L[1], L[2], out);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_msd_partials(double const* r1, double const* r2, unsigned n, double* partials, unsigned nq, unsigned qoff,
+                         hipStream_t s) {
+    msd_partials_kernel<<<REDUCE_BLOCKS, REDUCE_THREADS, 0, s>>>(r1, r2, n, partials, nq, qoff);
+    HIP_CHECK(hipGetLastError());
+}
+
+} // namespace halmd

@@ -1,0 +1,256 @@
+// Wave64 exclusive scan and stable LSD radix sort for gfx950.
+//
+// Replaces the reference's radix sort and Blelloch scan
+// (halmd/algorithm/gpu/radix_sort.hpp:39-236, radix_sort_kernel.cu:76-230,
+// halmd/algorithm/gpu/scan.hpp:37-107).  Same contract: stable sort of
+// (uint32 key, uint32 value) pairs and an exclusive prefix sum, but designed
+// for 64-wide wavefronts: digits are ranked within a wave by a 64-bit ballot
+// "match" (eight ballots give the set of lanes holding the same 8-bit digit),
+// waves are combined through LDS, and only ceil(key_bits / 8) passes run
+// instead of always four.  The workspace is allocated once per module, never
+// per call.
+#include "kernels.hpp"
+
+namespace halmd {
+namespace {
+
+constexpr unsigned SCAN_THREADS = 256;
+constexpr unsigned SCAN_ITEMS = 8;
+constexpr unsigned SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+constexpr unsigned RS_THREADS = 256;
+constexpr unsigned RS_WAVES = RS_THREADS / WAVE;
+constexpr unsigned RS_ITEMS = 16;
+constexpr unsigned RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr unsigned RS_BUCKETS = 256;
+
+// exclusive block scan of one value per thread (blockDim.x == SCAN_THREADS)
+__device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t& total) {
+    __shared__ uint32_t s_wave[SCAN_THREADS / WAVE];
+    unsigned const lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    uint32_t x = v;
+#pragma unroll
+    for (unsigned off = 1; off < WAVE; off <<= 1) {
+        uint32_t y = __shfl_up(x, off, WAVE);
+        if (lane >= off) x += y;
+    }
+    if (lane == WAVE - 1) s_wave[wave] = x;
+    __syncthreads();
+    uint32_t prefix = 0;
+    total = 0;
+#pragma unroll
+    for (unsigned w = 0; w < SCAN_THREADS / WAVE; ++w) {
+        uint32_t t = s_wave[w];
+        if (w < wave) prefix += t;
+        total += t;
+    }
+    __syncthreads();
+    return prefix + x - v;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void scan_reduce_kernel(uint32_t const* __restrict__ in, size_t m,
+                                                                   uint32_t* __restrict__ partial) {
+    __shared__ uint32_t s_wave[SCAN_THREADS / WAVE];
+    size_t const base = size_t(blockIdx.x) * SCAN_TILE;
+    uint32_t sum = 0;
+#pragma unroll
+    for (unsigned k = 0; k < SCAN_ITEMS; ++k) {
+        size_t idx = base + k * SCAN_THREADS + threadIdx.x;
+        if (idx < m) sum += in[idx];
+    }
+    sum = wave_sum(sum);
+    if (threadIdx.x % WAVE == 0) s_wave[threadIdx.x / WAVE] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (unsigned w = 0; w < SCAN_THREADS / WAVE; ++w) t += s_wave[w];
+        partial[blockIdx.x] = t;
+    }
+}
+
+// each thread scans SCAN_ITEMS consecutive elements; offsets (if given) hold
+// the exclusive prefix of each tile
+__global__ __launch_bounds__(SCAN_THREADS) void scan_apply_kernel(uint32_t* __restrict__ data, size_t m,
+                                                                  uint32_t const* __restrict__ offsets) {
+    size_t const base = size_t(blockIdx.x) * SCAN_TILE + size_t(threadIdx.x) * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t sum = 0;
+#pragma unroll
+    for (unsigned k = 0; k < SCAN_ITEMS; ++k) {
+        size_t idx = base + k;
+        v[k] = idx < m ? data[idx] : 0u;
+        sum += v[k];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(sum, total);
+    if (offsets) run += offsets[blockIdx.x];
+#pragma unroll
+    for (unsigned k = 0; k < SCAN_ITEMS; ++k) {
+        size_t idx = base + k;
+        if (idx < m) data[idx] = run;
+        run += v[k];
+    }
+}
+
+// per-tile 256-bucket histogram, written digit-major: counts[d * nb + block]
+__global__ __launch_bounds__(RS_THREADS) void rs_histogram_kernel(uint32_t const* __restrict__ keys, size_t n,
+                                                                  unsigned shift, uint32_t* __restrict__ counts,
+                                                                  unsigned nb) {
+    __shared__ uint32_t hist[RS_BUCKETS];
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    size_t const base = size_t(blockIdx.x) * RS_TILE;
+#pragma unroll 4
+    for (unsigned k = 0; k < RS_ITEMS; ++k) {
+        size_t idx = base + k * RS_THREADS + threadIdx.x;
+        if (idx < n) atomicAdd(&hist[(keys[idx] >> shift) & (RS_BUCKETS - 1)], 1u);
+    }
+    __syncthreads();
+    counts[size_t(threadIdx.x) * nb + blockIdx.x] = hist[threadIdx.x];
+}
+
+// stable scatter: element order within a tile is (chunk, wave, lane)
+__global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(uint32_t const* __restrict__ kin,
+                                                                uint32_t const* __restrict__ vin,
+                                                                uint32_t* __restrict__ kout,
+                                                                uint32_t* __restrict__ vout, size_t n,
+                                                                unsigned shift, uint32_t const* __restrict__ counts,
+                                                                unsigned nb) {
+    __shared__ uint32_t s_base[RS_BUCKETS];
+    __shared__ uint32_t s_wcnt[RS_WAVES][RS_BUCKETS];
+    unsigned const tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
+    s_base[tid] = counts[size_t(tid) * nb + blockIdx.x];
+    size_t const base = size_t(blockIdx.x) * RS_TILE;
+    uint64_t const lt = (uint64_t(1) << lane) - 1;
+
+    for (unsigned c = 0; c < RS_ITEMS; ++c) {
+#pragma unroll
+        for (unsigned w = 0; w < RS_WAVES; ++w) s_wcnt[w][tid] = 0;
+        __syncthreads();
+
+        size_t const idx = base + c * RS_THREADS + tid;
+        bool const valid = idx < n;
+        uint32_t key = 0, val = 0;
+        if (valid) { key = kin[idx]; val = vin[idx]; }
+        unsigned const d = (key >> shift) & (RS_BUCKETS - 1);
+        // lanes of this wave holding the same digit
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (unsigned b = 0; b < 8; ++b) {
+            bool const bit = (d >> b) & 1u;
+            uint64_t const bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        unsigned const rank = __popcll(peers & lt);
+        if (valid && rank == 0) s_wcnt[wave][d] = __popcll(peers);
+        __syncthreads();
+
+        if (valid) {
+            uint32_t off = s_base[d] + rank;
+            for (unsigned w = 0; w < wave; ++w) off += s_wcnt[w][d];
+            kout[off] = key;
+            vout[off] = val;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+#pragma unroll
+        for (unsigned w = 0; w < RS_WAVES; ++w) add += s_wcnt[w][tid];
+        s_base[tid] += add;
+        __syncthreads();
+    }
+}
+
+__global__ void iota_kernel(uint32_t* out, size_t n) {
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = uint32_t(i);
+}
+
+__global__ void fill_u32_kernel(uint32_t* out, uint32_t value, size_t n) {
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = value;
+}
+
+void scan_level(uint32_t* data, size_t m, scan_workspace& ws, unsigned level, hipStream_t s) {
+    unsigned const nb = div_up(m, SCAN_TILE);
+    if (nb == 1) {
+        scan_apply_kernel<<<1, SCAN_THREADS, 0, s>>>(data, m, nullptr);
+        return;
+    }
+    if (level + 1 >= ws.level_offset.size() || ws.level_offset[level + 1] - ws.level_offset[level] < nb)
+        throw error(HALMD_ERR_LOGIC, "scan workspace too small");
+    uint32_t* partial = ws.partial.get() + ws.level_offset[level];
+    scan_reduce_kernel<<<nb, SCAN_THREADS, 0, s>>>(data, m, partial);
+    scan_level(partial, nb, ws, level + 1, s);
+    scan_apply_kernel<<<nb, SCAN_THREADS, 0, s>>>(data, m, partial);
+}
+
+} // namespace
+
+void scan_workspace::reserve(size_t n) {
+    std::vector<size_t> off{0};
+    size_t m = n;
+    while (true) {
+        size_t nb = div_up(m, SCAN_TILE);
+        if (nb <= 1) break;
+        off.push_back(off.back() + nb);
+        m = nb;
+    }
+    // per-level block counts grow monotonically with n, so the layout for the
+    // largest n serves every smaller one
+    if (level_offset.empty() || off.back() > level_offset.back()) {
+        partial.reserve(std::max<size_t>(off.back(), 1));
+        level_offset = off;
+    }
+}
+
+void scan_exclusive(uint32_t* data, size_t m, scan_workspace& ws, hipStream_t s) {
+    if (m == 0) return;
+    ws.reserve(m);
+    scan_level(data, m, ws, 0, s);
+    HIP_CHECK(hipGetLastError());
+}
+
+void radix_sort_workspace::reserve(size_t n) {
+    if (n <= n_max) return;
+    keys_alt.resize(n);
+    vals_alt.resize(n);
+    size_t nb = div_up(n, RS_TILE);
+    counts.resize(nb * RS_BUCKETS);
+    scan.reserve(nb * RS_BUCKETS);
+    n_max = n;
+}
+
+void radix_sort(uint32_t* keys, uint32_t* vals, size_t n, unsigned key_bits, radix_sort_workspace& ws,
+                hipStream_t s) {
+    if (n <= 1 || key_bits == 0) return;
+    require(key_bits <= 32, "radix_sort: key_bits > 32");
+    ws.reserve(n);
+    unsigned const passes = div_up(key_bits, 8);
+    unsigned const nb = div_up(n, RS_TILE);
+    uint32_t *kin = keys, *vin = vals, *kout = ws.keys_alt, *vout = ws.vals_alt;
+    for (unsigned p = 0; p < passes; ++p) {
+        unsigned const shift = 8 * p;
+        rs_histogram_kernel<<<nb, RS_THREADS, 0, s>>>(kin, n, shift, ws.counts, nb);
+        scan_exclusive(ws.counts, size_t(nb) * RS_BUCKETS, ws.scan, s);
+        rs_scatter_kernel<<<nb, RS_THREADS, 0, s>>>(kin, vin, kout, vout, n, shift, ws.counts, nb);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    if (passes % 2) {
+        HIP_CHECK(hipMemcpyAsync(keys, kin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(vals, vin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_iota(uint32_t* out, size_t n, hipStream_t s) {
+    if (!n) return;
+    iota_kernel<<<div_up(n, 256), 256, 0, s>>>(out, n);
+}
+
+void launch_fill_u32(uint32_t* out, uint32_t value, size_t n, hipStream_t s) {
+    if (!n) return;
+    fill_u32_kernel<<<div_up(n, 256), 256, 0, s>>>(out, value, n);
+}
+
+} // namespace halmd

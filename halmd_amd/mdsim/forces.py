@@ -1,0 +1,45 @@
+"""halmd.mdsim.forces: pair_trunc and the pair convenience constructor.
+
+Mirrors lua/halmd/mdsim/forces/{pair,pair_trunc}.lua.in over include/halmd_hip.h.
+"""
+import ctypes
+
+from .._lib import Handle, call, out_handle
+from . import neighbour as _neighbour
+
+__all__ = ["pair_trunc", "pair"]
+
+
+class pair_trunc(Handle):
+    """Truncated pair force (lua/halmd/mdsim/forces/pair_trunc.lua.in:109-167;
+    gpu/forces/pair_trunc.hpp:340-374).  Registers itself with the particle
+    instance, whose force is then evaluated lazily when read."""
+
+    _destroy = "halmd_mdsim_forces_pair_trunc_destroy"
+
+    def __init__(self, box, particle, potential, neighbour=None, weight=1.0):
+        super().__init__()
+        if isinstance(particle, (tuple, list)):
+            if particle[0] is not particle[1]:
+                raise ValueError("pair forces between two particle instances are not supported")
+            particle = particle[0]
+        if getattr(potential, "r_cut", None) is None:
+            raise ValueError("pair_trunc: the potential must be truncated first")
+        if neighbour is None or isinstance(neighbour, dict):
+            kw = dict(neighbour or {})
+            neighbour = _neighbour(particle=particle, box=box, r_cut=potential.r_cut, **kw)
+        self.neighbour = neighbour
+        self.potential = potential
+        self.particle = particle
+        self._deps = (box,)
+        self._h = out_handle()
+        call("halmd_mdsim_forces_pair_trunc_create", potential.handle, particle.handle, box.handle,
+             neighbour.handle, float(weight), ctypes.byref(self._h))
+
+    def apply(self):
+        call("halmd_mdsim_forces_pair_trunc_apply", self.handle)
+
+
+def pair(box, particle, potential, neighbour=None, weight=1.0):
+    """forces.pair{box, particle, potential, neighbour}: truncated potentials use pair_trunc."""
+    return pair_trunc(box=box, particle=particle, potential=potential, neighbour=neighbour, weight=weight)

@@ -1,0 +1,81 @@
+"""Named workloads built with the module API (synthetic, seeded inputs).
+
+Each builder returns a namespace holding the modules of one simulation and the
+FP64 initial state (positions, velocities, species) so that any checker can be
+started from exactly the same input.
+"""
+from types import SimpleNamespace
+
+import numpy as np
+
+from . import mdsim, observables
+from .mdsim import positions, velocities
+
+__all__ = ["fluid", "lennard_jones", "kob_andersen", "KA_EPSILON", "KA_SIGMA"]
+
+KA_EPSILON = np.array([[1.0, 1.5], [1.5, 0.5]])
+KA_SIGMA = np.array([[1.0, 0.8], [0.8, 0.88]])
+
+
+def fluid(nparticle, density, temperature, potential, skin=0.3, timestep=0.005, seed=1, species=None,
+          integrator="verlet", resonance_frequency=5.0, collision_rate=1.0, sort_interval=0, hilbert_on_rebuild=False,
+          jitter=0.0):
+    """fcc lattice at `density`, Maxwell-Boltzmann velocities at `temperature`."""
+    n = int(nparticle)
+    L = np.full(3, (n / density) ** (1.0 / 3))
+    r0, a = positions.fcc(n, L)
+    rng = np.random.default_rng(seed)
+    if jitter:
+        r0 = r0 + rng.uniform(-jitter, jitter, r0.shape) * a
+        r0 = (r0 + L / 2) % L - L / 2
+    nspecies = 1 if species is None else int(np.max(species)) + 1
+    nspecies = max(nspecies, potential.species)
+    v0 = velocities.maxwell_boltzmann(n, temperature, 1.0, seed=seed + 1)
+    box = mdsim.box(length=L)
+    particle = mdsim.particle(dimension=3, particles=n, species=nspecies)
+    particle.data["position"] = r0
+    particle.data["velocity"] = v0
+    sp = np.zeros(n, dtype=np.uint32) if species is None else np.asarray(species, dtype=np.uint32)
+    if species is not None:
+        particle.data["species"] = sp
+    neighbour = mdsim.neighbour(particle=particle, box=box, r_cut=potential.r_cut, skin=skin,
+                                disable_sorting=not hilbert_on_rebuild)
+    force = mdsim.forces.pair_trunc(box=box, particle=particle, potential=potential, neighbour=neighbour)
+    if integrator == "verlet":
+        integ = mdsim.integrators.verlet(box=box, particle=particle, timestep=timestep)
+    elif integrator == "verlet_nvt_hoover":
+        integ = mdsim.integrators.verlet_nvt_hoover(box=box, particle=particle, timestep=timestep,
+                                                    temperature=temperature,
+                                                    resonance_frequency=resonance_frequency)
+    elif integrator == "verlet_nvt_andersen":
+        integ = mdsim.integrators.verlet_nvt_andersen(box=box, particle=particle, timestep=timestep,
+                                                      temperature=temperature, rate=collision_rate, seed=seed)
+    else:
+        raise ValueError(integrator)
+    core = mdsim.core(particle)
+    core.set_integrator(integ)
+    sort = None
+    if sort_interval:
+        sort = mdsim.sorts.hilbert(particle=particle, box=box)
+        core.set_sort(sort, sort_interval)
+    sampler = observables.sampler(core)
+    thermo = observables.thermodynamics(box=box, particle=particle, integrator=integ)
+    return SimpleNamespace(box=box, particle=particle, potential=potential, neighbour=neighbour, force=force,
+                           integrator=integ, core=core, sort=sort, sampler=sampler, thermodynamics=thermo,
+                           length=L, position=r0, velocity=v0, species=sp, temperature=temperature,
+                           timestep=timestep, skin=skin, lattice_constant=a)
+
+
+def lennard_jones(nparticle, density=0.8442, temperature=1.2, r_cut=2.5, truncation="shifted", **kw):
+    pot = mdsim.potentials.pair.lennard_jones(epsilon=1.0, sigma=1.0).truncate(truncation, cutoff=r_cut)
+    return fluid(nparticle, density, temperature, pot, **kw)
+
+
+def kob_andersen(nparticle, density=1.2, temperature=0.7, fraction_b=0.2, seed=1, **kw):
+    """80:20 Kob-Andersen mixture, species assigned by a seeded shuffle."""
+    n = int(nparticle)
+    rng = np.random.default_rng(seed + 7)
+    species = np.zeros(n, dtype=np.uint32)
+    species[rng.permutation(n)[:int(round(fraction_b * n))]] = 1
+    pot = mdsim.potentials.pair.lennard_jones(epsilon=KA_EPSILON, sigma=KA_SIGMA).truncate("shifted", cutoff=2.5)
+    return fluid(n, density, temperature, pot, species=species, seed=seed, **kw)

@@ -1,0 +1,294 @@
+/*
+ * C-ABI of the MI355X (gfx950) molecular-dynamics backend.
+ *
+ * Drop-in boundary for HAL's MD GPU backend: each function below replaces a
+ * luaponte-bound constructor or method of the reference's C++ GPU modules
+ * (the objects that lua/halmd/mdsim/ and lua/halmd/observables/ construct).
+ * Names follow the Lua module path (halmd.mdsim.binning ->
+ * halmd_mdsim_binning_*), parameters keep the reference's meaning and order,
+ * and handles replace std::shared_ptr.  The reference interface each entry
+ * replaces is cited as file:line beside it.
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a HALMD_ERR_* code; the message
+ *    of the last error on the calling thread is returned by halmd_last_error().
+ *    No C++ exception crosses this boundary.
+ *  - Vectors are row-major double arrays (x, y, z per particle); matrices of
+ *    species-pair parameters are row-major double[nspecies * nspecies].
+ *  - Per-particle getters and setters use the current memory order of the
+ *    particle arrays (as the reference's particle.data); get_id /
+ *    get_reverse_id give the map to particle IDs.
+ *  - Work is stream-ordered on one HIP stream per particle instance; calls that
+ *    return values to the host synchronise that stream.
+ *  - Only three-dimensional systems and at most 4 species are supported.
+ */
+#ifndef HALMD_HIP_H
+#define HALMD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+#define HALMD_API extern "C" __attribute__((visibility("default")))
+#else
+#define HALMD_API extern __attribute__((visibility("default")))
+#endif
+
+#define HALMD_OK 0
+#define HALMD_ERR_INVALID_ARGUMENT 1
+#define HALMD_ERR_HIP 2
+#define HALMD_ERR_CAPACITY 3
+#define HALMD_ERR_LOGIC 4
+
+typedef struct halmd_box_s halmd_box;
+typedef struct halmd_particle_s halmd_particle;
+typedef struct halmd_potential_s halmd_potential;
+typedef struct halmd_binning_s halmd_binning;
+typedef struct halmd_max_displacement_s halmd_max_displacement;
+typedef struct halmd_neighbour_s halmd_neighbour;
+typedef struct halmd_force_s halmd_force;
+typedef struct halmd_integrator_s halmd_integrator;
+typedef struct halmd_sort_s halmd_sort;
+typedef struct halmd_core_s halmd_core;
+typedef struct halmd_thermodynamics_s halmd_thermodynamics;
+typedef struct halmd_wavevector_s halmd_wavevector;
+typedef struct halmd_density_mode_s halmd_density_mode;
+typedef struct halmd_blocking_scheme_s halmd_blocking_scheme;
+typedef struct halmd_sampler_s halmd_sampler;
+
+/* ---- library ---------------------------------------------------------- */
+HALMD_API const char* halmd_last_error(void);
+HALMD_API int halmd_device_info(char* name, size_t len, int* compute_units, size_t* total_memory);
+HALMD_API int halmd_device_synchronize(void);
+/* device memory for tests and tools (the modules own their buffers) */
+HALMD_API int halmd_device_alloc(size_t bytes, void** ptr);
+HALMD_API int halmd_device_free(void* ptr);
+HALMD_API int halmd_memcpy_htod(void* dst, const void* src, size_t bytes);
+HALMD_API int halmd_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+
+/* ---- algorithm (kernel seam; device pointers, synchronous) -------------- */
+/* stable sort of (key, value) by the low key_bits bits of key;
+ * halmd/algorithm/gpu/radix_sort.hpp:39-236 (radix_sort(keys, values)) */
+HALMD_API int halmd_algorithm_radix_sort(uint32_t* keys, uint32_t* values, size_t n, unsigned key_bits);
+/* exclusive prefix sum in place; halmd/algorithm/gpu/scan.hpp:37-107 */
+HALMD_API int halmd_algorithm_scan_exclusive(uint32_t* data, size_t n);
+
+/* ---- mdsim.box: halmd/mdsim/box.cpp:37-62, lua/halmd/mdsim/box.lua.in ---- */
+HALMD_API int halmd_mdsim_box_create(const double* length, halmd_box** out);
+HALMD_API int halmd_mdsim_box_destroy(halmd_box* box);
+HALMD_API int halmd_mdsim_box_length(const halmd_box* box, double* length);
+HALMD_API int halmd_mdsim_box_volume(const halmd_box* box, double* volume);
+
+/* ---- mdsim.particle: halmd/mdsim/gpu/particle.cpp:55-160 (particle(n, nspecies)),
+ *      lua/halmd/mdsim/particle.lua.in:229-290 ---------------------------- */
+HALMD_API int halmd_mdsim_particle_create(unsigned dimension, unsigned nparticle, unsigned nspecies,
+                                          halmd_particle** out);
+HALMD_API int halmd_mdsim_particle_destroy(halmd_particle* p);
+HALMD_API int halmd_mdsim_particle_nparticle(const halmd_particle* p, unsigned* n);
+HALMD_API int halmd_mdsim_particle_nspecies(const halmd_particle* p, unsigned* nspecies);
+/* particle.data["position"] etc. (particle_array_gpu.cpp:174-201); positions
+ * and velocities are split exactly into float-float (hi, lo) on upload */
+HALMD_API int halmd_mdsim_particle_set_position(halmd_particle* p, const double* r);
+HALMD_API int halmd_mdsim_particle_get_position(halmd_particle* p, double* r);
+HALMD_API int halmd_mdsim_particle_set_velocity(halmd_particle* p, const double* v);
+HALMD_API int halmd_mdsim_particle_get_velocity(halmd_particle* p, double* v);
+HALMD_API int halmd_mdsim_particle_set_species(halmd_particle* p, const unsigned* species);
+HALMD_API int halmd_mdsim_particle_get_species(halmd_particle* p, unsigned* species);
+HALMD_API int halmd_mdsim_particle_set_mass(halmd_particle* p, const double* mass);
+HALMD_API int halmd_mdsim_particle_get_mass(halmd_particle* p, double* mass);
+HALMD_API int halmd_mdsim_particle_set_image(halmd_particle* p, const int* image);
+HALMD_API int halmd_mdsim_particle_get_image(halmd_particle* p, int* image);
+HALMD_API int halmd_mdsim_particle_get_force(halmd_particle* p, double* f);
+HALMD_API int halmd_mdsim_particle_get_potential_energy(halmd_particle* p, double* en_pot);
+/* 6 components per particle: xx, yy, zz, xy, xz, yz (force_kernel.hpp:36-46) */
+HALMD_API int halmd_mdsim_particle_get_stress_pot(halmd_particle* p, double* stress);
+HALMD_API int halmd_mdsim_particle_get_id(halmd_particle* p, unsigned* id);
+HALMD_API int halmd_mdsim_particle_get_reverse_id(halmd_particle* p, unsigned* reverse_id);
+/* particle:aux_enable() (particle.cpp:221-253) */
+HALMD_API int halmd_mdsim_particle_aux_enable(halmd_particle* p);
+/* particle:rearrange(index) (particle.cpp:173-218); index[new] = old */
+HALMD_API int halmd_mdsim_particle_rearrange(halmd_particle* p, const unsigned* index);
+
+/* ---- mdsim.potentials.pair.*: lennard_jones.cpp:45-71, morse.cpp:41-61;
+ *      truncations: lua/halmd/mdsim/potentials/pair/adapters/truncate.lua.in -- */
+HALMD_API int halmd_mdsim_potentials_pair_lennard_jones_create(unsigned ntype1, unsigned ntype2,
+                                                               const double* epsilon, const double* sigma,
+                                                               halmd_potential** out);
+HALMD_API int halmd_mdsim_potentials_pair_morse_create(unsigned ntype1, unsigned ntype2, const double* epsilon,
+                                                       const double* sigma, const double* r_min,
+                                                       const double* distortion, halmd_potential** out);
+/* potential:truncate{truncation, cutoff = r_c / sigma [, smoothing]};
+ * truncation: "sharp", "shifted", "force_shifted", "smooth_r4" */
+HALMD_API int halmd_mdsim_potentials_pair_truncate(halmd_potential* pot, const char* truncation,
+                                                   const double* cutoff, double smoothing);
+HALMD_API int halmd_mdsim_potentials_pair_r_cut(const halmd_potential* pot, double* r_cut);
+/* FP64 host evaluation of (-U'(r)/r, U(r)) at squared distances rr */
+HALMD_API int halmd_mdsim_potentials_pair_evaluate(const halmd_potential* pot, unsigned a, unsigned b, size_t n,
+                                                   const double* rr, double* fval, double* en_pot);
+HALMD_API int halmd_mdsim_potentials_pair_destroy(halmd_potential* pot);
+
+/* ---- mdsim.binning: halmd/mdsim/gpu/binning.cpp:240-247
+ *      binning(particle, box, r_cut, skin, occupancy) ------------------------- */
+HALMD_API int halmd_mdsim_binning_create(halmd_particle* p, const halmd_box* box, const double* r_cut, double skin,
+                                         double occupancy, halmd_binning** out);
+HALMD_API int halmd_mdsim_binning_destroy(halmd_binning* b);
+HALMD_API int halmd_mdsim_binning_ncell(const halmd_binning* b, unsigned* ncell);
+HALMD_API int halmd_mdsim_binning_update(halmd_binning* b);
+/* cell c holds index[begin[c] .. end[c]) (ascending particle index) */
+HALMD_API int halmd_mdsim_binning_cell_lists(halmd_binning* b, unsigned* begin, unsigned* end, unsigned* index);
+
+/* ---- mdsim.max_displacement: halmd/mdsim/gpu/max_displacement.cpp:60-106 - */
+HALMD_API int halmd_mdsim_max_displacement_create(halmd_particle* p, const halmd_box* box,
+                                                  halmd_max_displacement** out);
+HALMD_API int halmd_mdsim_max_displacement_destroy(halmd_max_displacement* d);
+HALMD_API int halmd_mdsim_max_displacement_compute(halmd_max_displacement* d, double* value);
+HALMD_API int halmd_mdsim_max_displacement_zero(halmd_max_displacement* d);
+
+/* ---- mdsim.neighbours.from_binning: halmd/mdsim/gpu/neighbours/from_binning.cpp:328-339
+ *      from_binning(particle, particle, binning, displacement, box, r_cut, skin, occupancy) */
+HALMD_API int halmd_mdsim_neighbours_from_binning_create(halmd_particle* p, halmd_binning* binning,
+                                                         halmd_max_displacement* displacement, const halmd_box* box,
+                                                         const double* r_cut, double skin, double occupancy,
+                                                         halmd_neighbour** out);
+HALMD_API int halmd_mdsim_neighbours_from_binning_destroy(halmd_neighbour* nb);
+/* force a rebuild now (from_binning.cpp:162-282) */
+HALMD_API int halmd_mdsim_neighbours_from_binning_update(halmd_neighbour* nb);
+/* lists as rows of `stride` entries (rebuilding first if required) */
+HALMD_API int halmd_mdsim_neighbours_from_binning_lists(halmd_neighbour* nb, unsigned* counts, unsigned* entries,
+                                                        unsigned stride);
+HALMD_API int halmd_mdsim_neighbours_from_binning_size(const halmd_neighbour* nb, unsigned* size);
+/* number of neighbours of each particle (memory order) */
+HALMD_API int halmd_mdsim_neighbours_from_binning_counts(halmd_neighbour* nb, unsigned* counts);
+HALMD_API int halmd_mdsim_neighbours_from_binning_rebuilds(const halmd_neighbour* nb, uint64_t* count);
+/* neighbour:on_prepend_update(sort.order) (lua/halmd/mdsim/neighbour.lua.in:289-295) */
+HALMD_API int halmd_mdsim_neighbours_from_binning_on_prepend_update_sort(halmd_neighbour* nb, halmd_sort* sort);
+
+/* ---- mdsim.forces.pair_trunc: halmd/mdsim/gpu/forces/pair_trunc.hpp:340-374
+ *      pair_trunc(potential, particle, particle, box, neighbour, weight) ------ */
+HALMD_API int halmd_mdsim_forces_pair_trunc_create(const halmd_potential* pot, halmd_particle* p,
+                                                   const halmd_box* box, halmd_neighbour* nb, double aux_weight,
+                                                   halmd_force** out);
+HALMD_API int halmd_mdsim_forces_pair_trunc_destroy(halmd_force* f);
+/* force:apply() (pair_trunc.hpp:193-215): evaluate now (overwrites the force) */
+HALMD_API int halmd_mdsim_forces_pair_trunc_apply(halmd_force* f);
+
+/* ---- mdsim.integrators.*: verlet.cpp:124-155, verlet_nvt_hoover.cpp:270-313,
+ *      verlet_nvt_andersen.cpp:146-183 -------------------------------------- */
+HALMD_API int halmd_mdsim_integrators_verlet_create(halmd_particle* p, const halmd_box* box, double timestep,
+                                                    halmd_integrator** out);
+HALMD_API int halmd_mdsim_integrators_verlet_nvt_hoover_create(halmd_particle* p, const halmd_box* box,
+                                                               double timestep, double temperature,
+                                                               double resonance_frequency, halmd_integrator** out);
+HALMD_API int halmd_mdsim_integrators_verlet_nvt_andersen_create(halmd_particle* p, const halmd_box* box,
+                                                                 uint64_t seed, double timestep, double temperature,
+                                                                 double rate, halmd_integrator** out);
+HALMD_API int halmd_mdsim_integrators_destroy(halmd_integrator* i);
+HALMD_API int halmd_mdsim_integrators_integrate(halmd_integrator* i);
+HALMD_API int halmd_mdsim_integrators_finalize(halmd_integrator* i);
+HALMD_API int halmd_mdsim_integrators_set_timestep(halmd_integrator* i, double timestep);
+HALMD_API int halmd_mdsim_integrators_timestep(const halmd_integrator* i, double* timestep);
+/* verlet_nvt_hoover: set_temperature, position/velocity of the chain, set_mass,
+ * internal_energy (energy of the heat bath per particle) */
+HALMD_API int halmd_mdsim_integrators_verlet_nvt_hoover_set_temperature(halmd_integrator* i, double temperature);
+HALMD_API int halmd_mdsim_integrators_verlet_nvt_hoover_chain(halmd_integrator* i, double* xi, double* v_xi);
+HALMD_API int halmd_mdsim_integrators_verlet_nvt_hoover_set_chain(halmd_integrator* i, const double* xi,
+                                                                   const double* v_xi);
+HALMD_API int halmd_mdsim_integrators_verlet_nvt_hoover_set_mass(halmd_integrator* i, const double* mass);
+HALMD_API int halmd_mdsim_integrators_verlet_nvt_hoover_mass(halmd_integrator* i, double* mass);
+HALMD_API int halmd_mdsim_integrators_internal_energy(halmd_integrator* i, double* en);
+
+/* ---- mdsim.sorts.hilbert: halmd/mdsim/gpu/sorts/hilbert.cpp:129-156 ------- */
+HALMD_API int halmd_mdsim_sorts_hilbert_create(halmd_particle* p, const halmd_box* box, halmd_sort** out);
+HALMD_API int halmd_mdsim_sorts_hilbert_destroy(halmd_sort* s);
+HALMD_API int halmd_mdsim_sorts_hilbert_order(halmd_sort* s);
+HALMD_API int halmd_mdsim_sorts_hilbert_depth(const halmd_sort* s, unsigned* depth);
+
+/* ---- mdsim.core: halmd/mdsim/core.cpp:30-63 ------------------------------- */
+HALMD_API int halmd_mdsim_core_create(halmd_particle* p, halmd_core** out);
+HALMD_API int halmd_mdsim_core_destroy(halmd_core* c);
+HALMD_API int halmd_mdsim_core_set_integrator(halmd_core* c, halmd_integrator* i);
+/* sort every `interval` steps (0: only when connected to a neighbour rebuild) */
+HALMD_API int halmd_mdsim_core_set_sort(halmd_core* c, halmd_sort* s, unsigned interval);
+HALMD_API int halmd_mdsim_core_mdstep(halmd_core* c);
+HALMD_API int halmd_mdsim_core_step(const halmd_core* c, uint64_t* step);
+
+/* ---- observables.thermodynamics: halmd/observables/gpu/thermodynamics.cpp:67-191,
+ *      halmd/observables/thermodynamics.hpp:80-90 (mean values per particle) -- */
+HALMD_API int halmd_observables_thermodynamics_create(halmd_particle* p, const halmd_box* box,
+                                                      halmd_integrator* integrator, halmd_thermodynamics** out);
+HALMD_API int halmd_observables_thermodynamics_destroy(halmd_thermodynamics* t);
+HALMD_API int halmd_observables_thermodynamics_en_kin(halmd_thermodynamics* t, double* v);
+HALMD_API int halmd_observables_thermodynamics_en_pot(halmd_thermodynamics* t, double* v);
+HALMD_API int halmd_observables_thermodynamics_virial(halmd_thermodynamics* t, double* v);
+HALMD_API int halmd_observables_thermodynamics_pressure(halmd_thermodynamics* t, double* v);
+HALMD_API int halmd_observables_thermodynamics_temperature(halmd_thermodynamics* t, double* v);
+HALMD_API int halmd_observables_thermodynamics_en_tot(halmd_thermodynamics* t, double* v);
+HALMD_API int halmd_observables_thermodynamics_center_of_mass_velocity(halmd_thermodynamics* t, double* v);
+/* sum over particles of the potential part of the stress tensor (6 components) */
+HALMD_API int halmd_observables_thermodynamics_stress_tensor_pot(halmd_thermodynamics* t, double* v);
+
+/* ---- observables.utility.wavevector: halmd/observables/utility/wavevector.cpp:39-219 */
+HALMD_API int halmd_observables_utility_wavevector_create(const double* wavenumber, size_t nk,
+                                                          const double* box_length, double tolerance,
+                                                          unsigned max_count, halmd_wavevector** out);
+HALMD_API int halmd_observables_utility_wavevector_create_dense(const double* wavenumber, size_t nk,
+                                                                const double* box_length, halmd_wavevector** out);
+HALMD_API int halmd_observables_utility_wavevector_destroy(halmd_wavevector* w);
+HALMD_API int halmd_observables_utility_wavevector_count(const halmd_wavevector* w, size_t* nq, size_t* nshell);
+/* q: 3 * nq; shell: 2 * nshell [first, last); wavenumber: nshell */
+HALMD_API int halmd_observables_utility_wavevector_get(const halmd_wavevector* w, double* q, uint64_t* shell,
+                                                       double* wavenumber);
+
+/* ---- observables.density_mode: halmd/observables/gpu/density_mode.cpp:140-167 */
+HALMD_API int halmd_observables_density_mode_create(halmd_particle* p, const halmd_wavevector* w,
+                                                    halmd_density_mode** out);
+HALMD_API int halmd_observables_density_mode_destroy(halmd_density_mode* d);
+/* rho_k = sum_j exp(-i k.r_j): 2 doubles (re, im) per wavevector */
+HALMD_API int halmd_observables_density_mode_acquire(halmd_density_mode* d, double* rho);
+
+/* ---- observables.dynamics.blocking_scheme + correlation{msd, isf}:
+ *      halmd/observables/dynamics/blocking_scheme.cpp:35-200,
+ *      lua/halmd/observables/dynamics/blocking_scheme.lua.in ------------------- */
+HALMD_API int halmd_observables_dynamics_blocking_scheme_create(halmd_particle* p, const halmd_box* box,
+                                                                double timestep, double maximum_lag_time,
+                                                                double resolution, unsigned block_size,
+                                                                unsigned shift, unsigned separation,
+                                                                halmd_blocking_scheme** out);
+HALMD_API int halmd_observables_dynamics_blocking_scheme_destroy(halmd_blocking_scheme* b);
+HALMD_API int halmd_observables_dynamics_blocking_scheme_add_msd(halmd_blocking_scheme* b);
+HALMD_API int halmd_observables_dynamics_blocking_scheme_add_isf(halmd_blocking_scheme* b, halmd_density_mode* d);
+HALMD_API int halmd_observables_dynamics_blocking_scheme_sample(halmd_blocking_scheme* b, uint64_t step);
+HALMD_API int halmd_observables_dynamics_blocking_scheme_finalise(halmd_blocking_scheme* b);
+HALMD_API int halmd_observables_dynamics_blocking_scheme_levels(const halmd_blocking_scheme* b, unsigned* levels,
+                                                                unsigned* block_size);
+HALMD_API int halmd_observables_dynamics_blocking_scheme_intervals(const halmd_blocking_scheme* b,
+                                                                   uint64_t* interval);
+/* which: "msd" ([levels][block_size]) or "isf" ([levels][block_size][nshell]) */
+HALMD_API int halmd_observables_dynamics_blocking_scheme_result(const halmd_blocking_scheme* b, const char* which,
+                                                                uint64_t* count, double* mean, double* error);
+
+/* ---- observables.sampler: halmd/observables/sampler.cpp:42-130 ----------- */
+HALMD_API int halmd_observables_sampler_create(halmd_core* c, halmd_sampler** out);
+HALMD_API int halmd_observables_sampler_destroy(halmd_sampler* s);
+HALMD_API int halmd_observables_sampler_add_thermodynamics(halmd_sampler* s, halmd_thermodynamics* t,
+                                                           uint64_t every);
+HALMD_API int halmd_observables_sampler_add_blocking_scheme(halmd_sampler* s, halmd_blocking_scheme* b);
+/* acquire density modes every `every` steps; records are (step, re/im x nq) */
+HALMD_API int halmd_observables_sampler_add_density_mode(halmd_sampler* s, halmd_density_mode* d, uint64_t every);
+HALMD_API int halmd_observables_sampler_density_mode_records(halmd_sampler* s, unsigned k, double* out,
+                                                             size_t capacity, size_t* nrecords);
+HALMD_API int halmd_observables_sampler_run(halmd_sampler* s, uint64_t steps);
+/* run with HIP events on the simulation stream around the steps; elapsed ms */
+HALMD_API int halmd_observables_sampler_run_timed(halmd_sampler* s, uint64_t steps, double* ms);
+/* records (step, en_pot, en_kin, virial, en_nhc) of thermodynamics slot k */
+HALMD_API int halmd_observables_sampler_thermodynamics_records(halmd_sampler* s, unsigned k, double* out,
+                                                               size_t capacity, size_t* nrecords);
+
+/* ---- profiling: per-module kernel timers (HIP events on the stream) ------ */
+/* module: "force", "neighbour", "binning", "integrate", "finalize", "sort",
+ * "density_mode"; handle: the module's handle (force, neighbour, binning,
+ * integrator, integrator, sort, density_mode); max_launches = 0 disables */
+HALMD_API int halmd_profile_enable(void* handle, const char* module, size_t max_launches);
+HALMD_API int halmd_profile_collect(void* handle, const char* module, double* total_ms, uint64_t* launches);
+
+#endif /* HALMD_HIP_H */

@@ -706,7 +706,7 @@ void oracle_fp32_cell_index(int n, float const* pos4, unsigned const* ncell, flo
 // Writes counts[n] and the lists row-major in list[i*cap + k] (up to cap
 // entries per particle).  Returns the maximum count.
 int oracle_fp32_neighbour_list(int n, float const* pos4, unsigned const* ncell, float const* cell_length,
-                               float const* L, int ntype, float const* rr_cut_skin, int cap,
+                               float const* L, float const* invL, int ntype, float const* rr_cut_skin, int cap,
                                unsigned* counts, unsigned* list)
 {
     size_t ncells = size_t(ncell[0]) * ncell[1] * ncell[2];
@@ -735,7 +735,7 @@ int oracle_fp32_neighbour_list(int n, float const* pos4, unsigned const* ncell, 
                 float d[3];
                 for (int k = 0; k < 3; ++k) {
                     float dk = ri[k] - rj[k];
-                    float img = std::rint(dk / L[k]);
+                    float img = std::rint(dk * invL[k]);
                     float sh = img * L[k];
                     d[k] = dk - sh;
                 }

@@ -51,7 +51,7 @@ def lib():
             "oracle_pair_force": (None, [ctypes.c_int, _d, _i, _d, _vp, ctypes.c_double, ctypes.c_double, _d, _d, _d]),
             "oracle_pair_force_brute": (None, [ctypes.c_int, _d, _i, _d, _vp, _d, _d, _d]),
             "oracle_fp32_cell_index": (None, [ctypes.c_int, _f, _u, _f, _u]),
-            "oracle_fp32_neighbour_list": (ctypes.c_int, [ctypes.c_int, _f, _u, _f, _f, ctypes.c_int, _f, ctypes.c_int, _u, _u]),
+            "oracle_fp32_neighbour_list": (ctypes.c_int, [ctypes.c_int, _f, _u, _f, _f, _f, ctypes.c_int, _f, ctypes.c_int, _u, _u]),
             "oracle_fp32_hilbert_code": (None, [ctypes.c_int, _f, _f, ctypes.c_uint, _u]),
             "oracle_hilbert_depth": (ctypes.c_uint, [_d]),
             "oracle_hilbert_map_f64": (ctypes.c_uint, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint]),
@@ -196,16 +196,20 @@ def fp32_cell_index(pos4, ncell, cell_length):
 
 
 def fp32_neighbour_list(pos4, ncell, cell_length, L, rr_cut_skin, cap):
+    """L is the FP64 box length; the rule uses float(L) and float(1/L)."""
     pos4 = np.ascontiguousarray(pos4, dtype=np.float32)
     nc = np.ascontiguousarray(ncell, dtype=np.uint32)
     cl = np.ascontiguousarray(cell_length, dtype=np.float32)
+    L = np.asarray(L, dtype=np.float64)
     Lf = np.ascontiguousarray(L, dtype=np.float32)
+    invLf = np.ascontiguousarray(1.0 / L, dtype=np.float32)
     rr = np.ascontiguousarray(rr_cut_skin, dtype=np.float32)
     ntype = rr.shape[0]
     n = len(pos4)
     counts = np.zeros(n, dtype=np.uint32)
     lst = np.zeros(n * cap, dtype=np.uint32)
-    m = lib().oracle_fp32_neighbour_list(n, _p(pos4, _f), _p(nc, _u), _p(cl, _f), _p(Lf, _f), ntype, _p(rr, _f), cap,
+    m = lib().oracle_fp32_neighbour_list(n, _p(pos4, _f), _p(nc, _u), _p(cl, _f), _p(Lf, _f), _p(invLf, _f), ntype,
+                                         _p(rr, _f), cap,
                                          _p(counts, _u), _p(lst, _u))
     return counts, lst.reshape(n, cap), m
 

@@ -1,0 +1,265 @@
+"""Parity of the HIP MD path with the oracle, through the C-ABI.
+
+Integer results (cell lists, neighbour lists with their order, Hilbert
+permutations) are compared bit-exactly with the FP32 restatements of the
+device rules; floating-point results (forces, energies, virial, trajectories)
+with the FP64 host-backend restatement, within the tolerances stated below.
+"""
+import numpy as np
+import pytest
+
+from halmd_amd import mdsim, workloads
+
+pytestmark = pytest.mark.gpu
+
+# per-result tolerances against the FP64 oracle
+FORCE_TOL = 1e-5          # max_i |F_i - F_i^ref| / rms(|F^ref|)
+EN_POT_TOL = 1e-6         # relative, mean potential energy per particle
+VIRIAL_TOL = 1e-6         # relative, mean virial per particle
+
+
+def _pos4(pos, species):
+    p4 = np.zeros((len(pos), 4), dtype=np.float32)
+    p4[:, :3] = np.asarray(pos, dtype=np.float64).astype(np.float32)
+    p4[:, 3] = np.asarray(species, dtype=np.uint32).view(np.float32)
+    return p4
+
+
+def _force_error(f, fref):
+    rms = np.sqrt(np.mean(np.sum(fref ** 2, axis=1)))
+    return np.max(np.abs(f - fref)) / rms
+
+
+# ----------------------------------------------------------------------------
+# binning and neighbour lists: bit-exact against the FP32 rules
+# ----------------------------------------------------------------------------
+@pytest.mark.parametrize("n,rho", [(4000, 0.8442), (13500, 1.2)])
+def test_cell_lists_bit_exact(oracle, n, rho):
+    w = workloads.lennard_jones(n, density=rho, jitter=0.1, seed=n)
+    b = w.neighbour.binning
+    ncell = b.ncell
+    assert list(ncell) == list(oracle.binning_ncell(w.length, 2.5, 0.3))
+    begin, end, index = b.cell_lists()
+    cl = (w.length / ncell).astype(np.float32)
+    cell = oracle.fp32_cell_index(_pos4(w.position, w.species), ncell, cl)
+    expect = np.argsort(cell, kind="stable")
+    assert np.array_equal(index, expect)
+    counts = np.bincount(cell, minlength=int(np.prod(ncell)))
+    assert np.array_equal(end - begin, counts)
+    nz = counts > 0
+    assert np.array_equal(begin[nz], np.concatenate([[0], np.cumsum(counts)[:-1]])[nz])
+
+
+@pytest.mark.parametrize("kind", ["lj", "ka"])
+def test_neighbour_lists_bit_exact(oracle, kind):
+    if kind == "lj":
+        w = workloads.lennard_jones(6912, density=0.8442, jitter=0.15, seed=11)
+    else:
+        w = workloads.kob_andersen(8788, density=1.2, jitter=0.15, seed=12)
+    counts, entries = w.neighbour.lists()
+    ncell = w.neighbour.binning.ncell
+    cl = (w.length / ncell).astype(np.float32)
+    rcs = ((w.potential.r_cut + w.skin) ** 2).astype(np.float32)
+    ocounts, olists, m = oracle.fp32_neighbour_list(_pos4(w.position, w.species), ncell, cl, w.length, rcs,
+                                                    entries.shape[1])
+    assert m <= entries.shape[1]
+    assert np.array_equal(counts, ocounts)
+    for i in range(len(counts)):
+        assert np.array_equal(entries[i, :counts[i]], olists[i, :counts[i]]), i
+
+
+def test_neighbour_list_overflow_grows():
+    # start with a list capacity far below the neighbour count: the module
+    # must detect the overflow on the device and rebuild with a larger size
+    w = workloads.lennard_jones(4000, density=0.8442, seed=5)
+    nb2 = mdsim.neighbour(particle=w.particle, box=w.box, r_cut=w.potential.r_cut, skin=0.3, occupancy=1.0,
+                          disable_sorting=True)
+    small = nb2.size
+    counts, _ = nb2.lists()
+    assert nb2.size >= counts.max()
+    assert nb2.size >= small
+
+
+# ----------------------------------------------------------------------------
+# forces against the FP64 host backend
+# ----------------------------------------------------------------------------
+@pytest.mark.parametrize("truncation", ["shifted", "sharp", "force_shifted", "smooth_r4"])
+def test_lj_forces_match_fp64(oracle, truncation):
+    w = workloads.lennard_jones(6912, density=0.8442, jitter=0.15, seed=21, truncation=truncation)
+    pot = oracle.Potential(oracle.POT_LJ, truncation, 1.0, 1.0, 2.5, h=0.005)
+    fref, uref, sref = oracle.pair_force(w.position, w.species, w.length, pot, skin=0.3)
+    f = w.particle.data["force"]
+    assert _force_error(f, fref) < FORCE_TOL
+    u = w.particle.data["potential_energy"]
+    s = w.particle.data["stress_pot"]
+    assert abs(u.sum() - uref.sum()) / abs(uref.sum()) < EN_POT_TOL
+    assert np.max(np.abs(u - uref)) < 1e-5 * np.max(np.abs(uref))
+    tr, trref = s[:, :3].sum(), sref[:, :3].sum()
+    assert abs(tr - trref) / abs(trref) < VIRIAL_TOL
+    assert np.max(np.abs(s - sref)) < 1e-5 * np.max(np.abs(sref))
+
+
+def test_ka_forces_match_fp64(oracle):
+    w = workloads.kob_andersen(8788, density=1.2, jitter=0.15, seed=22)
+    pot = oracle.Potential(oracle.POT_LJ, "shifted", workloads.KA_EPSILON, workloads.KA_SIGMA, 2.5)
+    fref, uref, _ = oracle.pair_force(w.position, w.species, w.length, pot, skin=0.3)
+    assert _force_error(w.particle.data["force"], fref) < FORCE_TOL
+    u = w.particle.data["potential_energy"]
+    assert abs(u.sum() - uref.sum()) / abs(uref.sum()) < EN_POT_TOL
+
+
+def test_morse_forces_match_fp64(oracle):
+    # Morse D=1, a=6/sigma, r0=sigma, r_c=2.5 sigma: sigma_M = 1/6, r_cut_sigma = 15
+    pot_d = mdsim.potentials.pair.morse(epsilon=1.0, sigma=1.0 / 6, r_min=1.0, distortion=1.0).truncate(
+        "shifted", cutoff=15.0)
+    w = workloads.fluid(4000, 0.9, 1.0, pot_d, jitter=0.1, seed=23)
+    pot = oracle.Potential(oracle.POT_MORSE, "shifted", 1.0, 1.0 / 6, 15.0, r_min=1.0, distortion=1.0)
+    fref, uref, _ = oracle.pair_force(w.position, w.species, w.length, pot, skin=0.3)
+    assert _force_error(w.particle.data["force"], fref) < FORCE_TOL
+    u = w.particle.data["potential_energy"]
+    assert abs(u.sum() - uref.sum()) / abs(uref.sum()) < 1e-5
+
+
+def test_forces_large_box_newton3():
+    # size-independent property at the benchmark size: total force vanishes
+    w = workloads.lennard_jones(64 ** 3 * 4, density=0.8442, jitter=0.1, seed=24)
+    f = w.particle.data["force"]
+    rms = np.sqrt(np.mean(np.sum(f ** 2, axis=1)))
+    assert np.all(np.abs(f.sum(axis=0)) < 1e-3 * rms * np.sqrt(len(f)) * 1e-2)
+    counts, _ = w.neighbour.lists()
+    assert 60 < counts.mean() < 95
+
+
+# ----------------------------------------------------------------------------
+# integrators
+# ----------------------------------------------------------------------------
+def test_nve_trajectory_matches_fp64(oracle):
+    # short horizon, before chaos amplifies rounding: positions agree closely.
+    # smooth_r4 keeps the force continuous at r_c, so a pair that FP32 and FP64
+    # place on different sides of the cutoff cannot kick the trajectories apart
+    w = workloads.lennard_jones(4000, density=0.8442, temperature=1.2, jitter=0.05, seed=31, timestep=0.005,
+                                truncation="smooth_r4")
+    pot = oracle.Potential(oracle.POT_LJ, "smooth_r4", 1.0, 1.0, 2.5, h=0.005)
+    sim = oracle.Simulation(w.position, w.velocity, w.species, np.ones(4000), w.length, pot, 0.3, 0.005)
+    steps = 50
+    w.sampler.run(steps)
+    sim.run(steps)
+    ref = sim.state()
+    rid = w.particle.data["reverse_id"]
+    pos = w.particle.data["position"][rid]
+    img = w.particle.data["image"][rid]
+    vel = w.particle.data["velocity"][rid]
+    dr = (pos + img * w.length) - (ref["position"] + ref["image"] * w.length)
+    print(f"max |dr| = {np.max(np.abs(dr)):.3e}, max |dv| = {np.max(np.abs(vel - ref['velocity'])):.3e}")
+    assert np.max(np.abs(dr)) < 1e-6
+    assert np.max(np.abs(vel - ref["velocity"])) < 1e-5
+
+
+def test_nve_energy_conservation_and_drift(oracle):
+    w = workloads.lennard_jones(4000, density=0.8442, temperature=1.2, seed=32, timestep=0.005)
+    slot = w.sampler.on_sample_thermodynamics(w.thermodynamics, 100)
+    w.sampler.run(4000)
+    rec = w.sampler.thermodynamics_records(slot)
+    etot = rec[:, 1] + rec[:, 2]
+    t = rec[:, 0] * 0.005
+    late = t >= 5.0
+    slope = np.polyfit(t[late], etot[late], 1)[0]
+    # shifted (force-discontinuous) cutoff: compare with the FP64 oracle's own drift
+    pot = oracle.Potential(oracle.POT_LJ, "shifted", 1.0, 1.0, 2.5)
+    sim = oracle.Simulation(w.position, w.velocity, w.species, np.ones(4000), w.length, pot, 0.3, 0.005)
+    orec = sim.run(4000, every=100, record0=True)
+    oslope = np.polyfit(orec[late, 0] * 0.005, (orec[:, 1] + orec[:, 2])[late], 1)[0]
+    print(f"NVE drift per particle per tau: device {slope:.3e}, FP64 oracle {oslope:.3e}")
+    assert abs(slope) < 1e-4
+    assert abs(slope - oslope) < 2e-5
+    assert np.max(np.abs(etot[late] - etot[late].mean())) < 2e-3
+
+
+def test_nhc_matches_fp64(oracle):
+    w = workloads.lennard_jones(4000, density=0.8442, temperature=1.0, seed=33, timestep=0.002,
+                                integrator="verlet_nvt_hoover", resonance_frequency=5.0, jitter=0.05,
+                                truncation="smooth_r4")
+    pot = oracle.Potential(oracle.POT_LJ, "smooth_r4", 1.0, 1.0, 2.5, h=0.005)
+    sim = oracle.Simulation(w.position, w.velocity, w.species, np.ones(4000), w.length, pot, 0.3, 0.002,
+                            integrator=oracle.INT_NHC, temperature=1.0, nu=5.0)
+    steps = 40
+    w.sampler.run(steps)
+    sim.run(steps)
+    xi, vxi, _ = sim.nhc_state()
+    print("chain device", w.integrator.position, w.integrator.velocity, "oracle", xi, vxi)
+    assert np.allclose(w.integrator.position, xi, rtol=1e-4, atol=1e-10)
+    assert np.allclose(w.integrator.velocity, vxi, rtol=1e-4, atol=1e-9)
+    rid = w.particle.data["reverse_id"]
+    vel = w.particle.data["velocity"][rid]
+    assert np.max(np.abs(vel - sim.state()["velocity"])) < 1e-5
+    # conserved extended energy over a longer run
+    slot = w.sampler.on_sample_thermodynamics(w.thermodynamics, 50)
+    w.sampler.run(3000)
+    rec = w.sampler.thermodynamics_records(slot)
+    econs = rec[:, 1] + rec[:, 2] + rec[:, 4]
+    assert np.max(np.abs(econs - econs[0])) < 5e-3
+
+
+def test_andersen_thermalises():
+    from halmd_amd.mdsim import velocities
+    # heat bath at T = 1 acting on velocities drawn at T = 2
+    w = workloads.lennard_jones(4000, density=0.8442, temperature=1.0, seed=34, timestep=0.005,
+                                integrator="verlet_nvt_andersen", collision_rate=5.0)
+    w.particle.data["velocity"] = velocities.maxwell_boltzmann(4000, 2.0, seed=99)
+    slot = w.sampler.on_sample_thermodynamics(w.thermodynamics, 20)
+    w.sampler.run(3000)
+    rec = w.sampler.thermodynamics_records(slot)
+    temp = 2 * rec[-50:, 2] / 3
+    # single-sample spread of the kinetic temperature is T sqrt(2 / 3N); the mean
+    # of 50 correlated samples stays well inside 4.5 of those
+    sigma = np.sqrt(2.0 / 3 / 4000)
+    print(f"Andersen: mean T = {temp.mean():.4f}")
+    assert abs(temp.mean() - 1.0) < 4.5 * sigma
+
+
+# ----------------------------------------------------------------------------
+# Hilbert sort
+# ----------------------------------------------------------------------------
+def test_hilbert_order_bit_exact(oracle):
+    w = workloads.lennard_jones(13500, density=0.8442, jitter=0.2, seed=41)
+    s = mdsim.sorts.hilbert(particle=w.particle, box=w.box)
+    assert s.depth == oracle.hilbert_depth(w.length)
+    code = oracle.fp32_hilbert_code(_pos4(w.position, w.species), w.length, s.depth)
+    f_before = w.particle.ordered("force")
+    p_before = w.particle.data["position"]
+    s.order()
+    perm = np.argsort(code, kind="stable")
+    assert np.array_equal(w.particle.data["id"], perm)
+    rid = w.particle.data["reverse_id"]
+    assert np.array_equal(rid[perm], np.arange(len(perm)))
+    assert np.array_equal(w.particle.data["position"], p_before[perm])
+    # forces in ID order are unchanged by the reordering (up to summation order)
+    f_after = w.particle.ordered("force")
+    assert _force_error(f_after, f_before) < 1e-6
+
+
+def test_sorted_run_matches_unsorted():
+    a = workloads.lennard_jones(4000, density=0.8442, seed=42, jitter=0.05, sort_interval=10)
+    b = workloads.lennard_jones(4000, density=0.8442, seed=42, jitter=0.05)
+    a.sampler.run(30)
+    b.sampler.run(30)
+    pa = a.particle.ordered("position") + a.particle.ordered("image") * a.length
+    pb = b.particle.ordered("position") + b.particle.ordered("image") * b.length
+    assert np.max(np.abs(pa - pb)) < 1e-8
+
+
+# ----------------------------------------------------------------------------
+# thermodynamics
+# ----------------------------------------------------------------------------
+def test_thermodynamics_match_fp64(oracle):
+    w = workloads.kob_andersen(8788, density=1.2, temperature=0.7, jitter=0.1, seed=51)
+    pot = oracle.Potential(oracle.POT_LJ, "shifted", workloads.KA_EPSILON, workloads.KA_SIGMA, 2.5)
+    sim = oracle.Simulation(w.position, w.velocity, w.species, np.ones(8788), w.length, pot, 0.3, 0.005)
+    rec = sim.run(0, every=1, record0=True)[0]
+    t = w.thermodynamics
+    assert abs(t.en_pot() - rec[1]) < EN_POT_TOL * abs(rec[1])
+    assert abs(t.en_kin() - rec[2]) < 1e-12 * abs(rec[2])
+    assert abs(t.virial() - rec[3]) < VIRIAL_TOL * abs(rec[3])
+    assert np.allclose(t.center_of_mass_velocity(), 0, atol=1e-14)
+    rho = 8788 / np.prod(w.length)
+    assert abs(t.pressure() - rho * (2 * rec[2] / 3 + rec[3] / 3)) < 1e-5 * abs(t.pressure())

@@ -202,12 +202,13 @@ def test_fp32_neighbour_list_rules(oracle):
     pos4[:, :3] = pos.astype(np.float32)
     pos4[:, 3] = species.astype(np.uint32).view(np.float32)
     rcs = ((pot.r_cut + skin) ** 2).astype(np.float32)
-    counts, lst, m = oracle.fp32_neighbour_list(pos4, ncell, cl, L.astype(np.float32), rcs, 256)
+    counts, lst, m = oracle.fp32_neighbour_list(pos4, ncell, cl, L, rcs, 256)
     assert m <= 256
     p = pos4[:, :3]
     Lf = L.astype(np.float32)
+    invLf = (1.0 / L).astype(np.float32)
     d = p[:, None, :] - p[None, :, :]
-    d = d - np.rint(d / Lf) * Lf
+    d = d - np.rint(d * invLf) * Lf
     rr = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
     expect = rr <= rcs[species[:, None], species[None, :]]
     np.fill_diagonal(expect, False)
