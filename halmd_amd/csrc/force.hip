@@ -32,12 +32,12 @@ __device__ __forceinline__ void pair_eval(float rr, pair_param const& pp, float 
     } else {
         // Morse with distortion B (morse_kernel.hpp:46-66):
         // p = {eps / A, 1 / (sigma B), r_min_sigma / B, A}, A = 2 B^2 - 1
-        float const r = __builtin_amdgcn_sqrtf(rr);
+        float const r = sqrtf(rr);
         float const rsb = r * pp.p[1];
         float const d = pp.p[2] - rsb;
         float const A = pp.p[3];
-        float const e1 = __expf(d);
-        float const eA = (A == 1.0f) ? e1 : __expf(A * d);
+        float const e1 = expf(d);
+        float const eA = (A == 1.0f) ? e1 : expf(A * d);
         float const ee = pp.p[0] * e1;
         fval = (A + 1.0f) * ee * (eA - 1.0f) * rsb * __builtin_amdgcn_rcpf(rr);
         en = ee * (eA - A - 1.0f);
@@ -92,10 +92,7 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
     float fx = 0, fy = 0, fz = 0, cx = 0, cy = 0, cz = 0;
     float en = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
 
-    for (unsigned k = 0; k < cnt; ++k) {
-        unsigned const j = lp[size_t(k) * WAVE];
-        float4 const hj = pos[2 * size_t(j)];
-        float4 const lj = pos[2 * size_t(j) + 1];
+    auto pair = [&](float4 const hj, float4 const lj) {
         float const ix = rintf((hi.x - hj.x) * P.invL[0]);
         float const iy = rintf((hi.y - hj.y) * P.invL[1]);
         float const iz = rintf((hi.z - hj.z) * P.invL[2]);
@@ -104,7 +101,7 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
         float const dz = ((hi.z - iz * P.L_hi[2]) - hj.z) + ((lo.z - lj.z) - iz * P.L_lo[2]);
         float const rr = dx * dx + dy * dy + dz * dz;
         pair_param const& pp = s_pair[ti + __float_as_uint(hj.w)];
-        if (!(rr < pp.rr_cut)) continue;
+        if (!(rr < pp.rr_cut)) return;
         float fval, u;
         pair_eval<KIND, TRUNC>(rr, pp, P.rri_smooth, fval, u);
         kahan_add(fx, cx, fval * dx);
@@ -119,6 +116,26 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
             s4 += fval * dx * dz;
             s5 += fval * dy * dz;
         }
+    };
+
+    // four neighbours in flight per iteration: index loads first, then the
+    // 32-byte gathers, then the arithmetic (summation order stays k = 0, 1, ...)
+    unsigned k = 0;
+    for (; k + 4 <= cnt; k += 4) {
+        unsigned const j0 = lp[size_t(k) * WAVE], j1 = lp[size_t(k + 1) * WAVE];
+        unsigned const j2 = lp[size_t(k + 2) * WAVE], j3 = lp[size_t(k + 3) * WAVE];
+        float4 const h0 = pos[2 * size_t(j0)], l0 = pos[2 * size_t(j0) + 1];
+        float4 const h1 = pos[2 * size_t(j1)], l1 = pos[2 * size_t(j1) + 1];
+        float4 const h2 = pos[2 * size_t(j2)], l2 = pos[2 * size_t(j2) + 1];
+        float4 const h3 = pos[2 * size_t(j3)], l3 = pos[2 * size_t(j3) + 1];
+        pair(h0, l0);
+        pair(h1, l1);
+        pair(h2, l2);
+        pair(h3, l3);
+    }
+    for (; k < cnt; ++k) {
+        unsigned const j = lp[size_t(k) * WAVE];
+        pair(pos[2 * size_t(j)], pos[2 * size_t(j) + 1]);
     }
     float4 f = make_float4(fx - cx, fy - cy, fz - cz, AUX ? aux_weight * en : 0.0f);
     float st[6] = {aux_weight * s0, aux_weight * s1, aux_weight * s2, aux_weight * s3, aux_weight * s4,

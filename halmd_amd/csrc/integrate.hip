@@ -33,6 +33,23 @@ __device__ __forceinline__ ff ff_mul(ff x, ff y) {
     return z;
 }
 
+// fold the block maximum of d2 (>= 0) into *target (float bits); the atomic is
+// skipped when the stored value is already larger, so only blocks that raise
+// the running maximum contend for the word
+__device__ __forceinline__ void block_max_to(float d2, uint32_t* target) {
+    __shared__ float s_max[256 / WAVE];
+    d2 = wave_max(d2);
+    if (threadIdx.x % WAVE == 0) s_max[threadIdx.x / WAVE] = d2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = s_max[0];
+        for (unsigned w = 1; w < blockDim.x / WAVE; ++w) m = fmaxf(m, s_max[w]);
+        uint32_t const bits = __float_as_uint(m);
+        uint32_t const cur = __hip_atomic_load(target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (m > 0.0f && bits > cur) atomicMax(target, bits);
+    }
+}
+
 __device__ __forceinline__ ff ff_from_double(double x) {
     float const hi = float(x);
     return ff{hi, float(x - double(hi))};
@@ -87,10 +104,7 @@ __global__ __launch_bounds__(256) void verlet_integrate_kernel(
             d2 = dx * dx + dy * dy + dz * dz;
         }
     }
-    if (max_disp2) {
-        d2 = wave_max(d2);
-        if ((threadIdx.x % WAVE) == 0 && d2 > 0.0f) atomicMax(max_disp2, __float_as_uint(d2));
-    }
+    if (max_disp2) block_max_to(d2, max_disp2);
 }
 
 __global__ __launch_bounds__(256) void verlet_finalize_kernel(float4* __restrict__ vel, float4 const* __restrict__ force,
@@ -139,8 +153,7 @@ __global__ __launch_bounds__(256) void max_displacement_kernel(float4 const* __r
         dz -= rintf(dz * iz_) * Lz;
         d2 = dx * dx + dy * dy + dz * dz;
     }
-    d2 = wave_max(d2);
-    if ((threadIdx.x % WAVE) == 0 && d2 > 0.0f) atomicMax(max_disp2, __float_as_uint(d2));
+    block_max_to(d2, max_disp2);
 }
 
 // SplitMix64 finaliser (Steele, Lea & Flood, "Fast splittable pseudorandom

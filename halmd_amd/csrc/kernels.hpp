@@ -41,8 +41,9 @@ struct cell_grid {
     float cell_length[3];
 };
 void launch_compute_cell(float4 const* pos, unsigned n, cell_grid g, uint32_t* keys, uint32_t* vals, hipStream_t s);
-void launch_cell_bounds(uint32_t const* sorted_keys, unsigned n, unsigned ncells, uint32_t* begin, uint32_t* end,
-                        hipStream_t s);
+// offset[c] = first sorted index with key >= c, c = 0 .. ncells (ncells + 1 entries)
+void launch_cell_offsets(uint32_t const* sorted_keys, unsigned n, unsigned ncells, uint32_t* offset, hipStream_t s);
+// cell_pos[k] = high-word position of particle vals[k], w = (vals[k] << 2) | species
 void launch_gather_cell_pos(float4 const* pos, uint32_t const* sorted_vals, unsigned n, float4* cell_pos, hipStream_t s);
 
 struct neighbour_params {
@@ -54,9 +55,9 @@ struct neighbour_params {
 };
 // full neighbour lists; list layout [(i / 64) * 64 * cap + k * 64 + i % 64];
 // *max_count receives the largest per-particle count (may exceed cap)
-void launch_build_neighbours(float4 const* pos, unsigned n, uint32_t const* cell_begin, uint32_t const* cell_end,
-                             float4 const* cell_pos, uint32_t const* cell_val, neighbour_params const& p,
-                             unsigned cap, uint32_t* list, uint32_t* count, uint32_t* max_count, hipStream_t s);
+void launch_build_neighbours(float4 const* pos, unsigned n, uint32_t const* offset, float4 const* cell_pos,
+                             neighbour_params const& p, unsigned cap, uint32_t* list, uint32_t* count,
+                             uint32_t* max_count, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // pair forces (force.hip)

@@ -413,8 +413,7 @@ binning::binning(particle& p_, box const& b_, double const* r_cut, unsigned nt1,
     }
     keys.resize(p.n);
     vals.resize(p.n);
-    begin.resize(ncells);
-    end.resize(ncells);
+    offset.resize(size_t(ncells) + 1);
     cell_pos.resize(p.n);
     ws.reserve(p.n);
 }
@@ -427,7 +426,7 @@ void binning::update() {
     unsigned key_bits = 0;
     while ((1ull << key_bits) < ncells) ++key_bits;
     radix_sort(keys, vals, p.n, key_bits, ws, s);
-    launch_cell_bounds(keys, p.n, ncells, begin, end, s);
+    launch_cell_offsets(keys, p.n, ncells, offset, s);
     launch_gather_cell_pos(p.pos, vals, p.n, cell_pos, s);
     timer.stop(s);
     version = p.position_version;
@@ -435,10 +434,11 @@ void binning::update() {
 
 void binning::cell_lists(unsigned* begin_out, unsigned* end_out, unsigned* index_out) {
     update();
-    begin.download(begin_out, ncells, p.stream);
-    end.download(end_out, ncells, p.stream);
+    std::vector<uint32_t> off(size_t(ncells) + 1);
+    offset.download(off.data(), off.size(), p.stream);
     vals.download(index_out, p.n, p.stream);
     p.sync();
+    for (size_t c = 0; c < ncells; ++c) { begin_out[c] = off[c]; end_out[c] = off[c + 1]; }
 }
 
 // ---------------------------------------------------------------------------
@@ -534,8 +534,7 @@ void neighbour::update() {
     while (true) {
         launch_fill_u32(d_maxcount, 0u, 1, s);
         timer.start(s);
-        launch_build_neighbours(p.pos, p.n, bin.begin, bin.end, bin.cell_pos, bin.vals, params, cap, list, count,
-                                d_maxcount, s);
+        launch_build_neighbours(p.pos, p.n, bin.offset, bin.cell_pos, params, cap, list, count, d_maxcount, s);
         timer.stop(s);
         HIP_CHECK(hipMemcpyAsync(h_maxcount.get(), d_maxcount.get(), 4, hipMemcpyDeviceToHost, s));
         p.sync();

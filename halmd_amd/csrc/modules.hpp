@@ -143,7 +143,7 @@ public:
     double r_skin, occupancy, r_cut_max;
     cell_grid grid;
     unsigned ncells;
-    dbuf<uint32_t> keys, vals, begin, end;
+    dbuf<uint32_t> keys, vals, offset;   // offset: ncells + 1 entries
     dbuf<float4> cell_pos;
     radix_sort_workspace ws;
     uint64_t version = 0;             // position version of the last update

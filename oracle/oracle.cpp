@@ -697,12 +697,13 @@ void oracle_fp32_cell_index(int n, float const* pos4, unsigned const* ncell, flo
 }
 
 // Neighbour list rule of the device neighbour build (full lists).
-// Cell visiting order follows halmd/mdsim/gpu/neighbours/from_binning_kernel.cu:
-// 13 pairs of mutually opposite cells (offset j, then -j) for j preceding the
-// zero offset in lexicographic (x, y, z) order, then the own cell.  Within a
-// cell: ascending particle index.  Predicate: dr = r_i - r_j (FP32, hi words),
-// img = rint(dr / L), dr -= img * L, rr = (dx*dx + dy*dy) + dz*dz,
-// rr <= rr_cut_skin[a][b] (from_binning_kernel.cu:144-163).
+// Candidate order: the 9 rows (dz, dy) of neighbour cells in lexicographic
+// order (dz outer), in each row the cells x-1, x, x+1 (periodic), in each cell
+// ascending particle index (the order of the stable cell sort).
+// Predicate (halmd/mdsim/gpu/neighbours/from_binning_kernel.cu:144-163):
+// dr = r_i - r_j (FP32, hi words), img = rint(dr * invL), dr -= img * L,
+// rr = (dx*dx + dy*dy) + dz*dz, rr <= rr_cut_skin[a][b]; the device rule
+// multiplies by the FP32-rounded 1/L instead of dividing by L.
 // Writes counts[n] and the lists row-major in list[i*cap + k] (up to cap
 // entries per particle).  Returns the maximum count.
 int oracle_fp32_neighbour_list(int n, float const* pos4, unsigned const* ncell, float const* cell_length,
@@ -715,6 +716,7 @@ int oracle_fp32_neighbour_list(int n, float const* pos4, unsigned const* ncell, 
     std::vector<std::vector<unsigned>> cells(ncells);
     for (int i = 0; i < n; ++i) cells[cell[i]].push_back(unsigned(i));
     int maxc = 0;
+    int const nx = int(ncell[0]), ny = int(ncell[1]), nz = int(ncell[2]);
     for (int i = 0; i < n; ++i) {
         float const* ri = pos4 + 4 * size_t(i);
         unsigned ti;
@@ -722,13 +724,15 @@ int oracle_fp32_neighbour_list(int n, float const* pos4, unsigned const* ncell, 
         unsigned c = cell[i];
         int cx = int(c % ncell[0]), cy = int((c / ncell[0]) % ncell[1]), cz = int(c / ncell[0] / ncell[1]);
         unsigned count = 0;
-        auto visit = [&](int dx, int dy, int dz, bool self) {
-            int kx = (cx + int(ncell[0]) + dx) % int(ncell[0]);
-            int ky = (cy + int(ncell[1]) + dy) % int(ncell[1]);
-            int kz = (cz + int(ncell[2]) + dz) % int(ncell[2]);
-            auto const& cl = cells[(size_t(kz) * ncell[1] + ky) * ncell[0] + kx];
+        for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+            int kx = (cx + nx + dx) % nx;
+            int ky = (cy + ny + dy) % ny;
+            int kz = (cz + nz + dz) % nz;
+            auto const& cl = cells[(size_t(kz) * ny + ky) * nx + kx];
             for (unsigned j : cl) {
-                if (self && j == unsigned(i)) continue;
+                if (j == unsigned(i)) continue;
                 float const* rj = pos4 + 4 * size_t(j);
                 unsigned tj;
                 std::memcpy(&tj, rj + 3, 4);
@@ -747,16 +751,7 @@ int oracle_fp32_neighbour_list(int n, float const* pos4, unsigned const* ncell, 
                     count++;
                 }
             }
-        };
-        for (int dx = -1; dx <= 1; ++dx)
-        for (int dy = -1; dy <= 1; ++dy)
-        for (int dz = -1; dz <= 1; ++dz) {
-            if (dx == 0 && dy == 0 && dz == 0) goto self;
-            visit(dx, dy, dz, false);
-            visit(-dx, -dy, -dz, false);
         }
-    self:
-        visit(0, 0, 0, true);
         counts[i] = count;
         maxc = std::max(maxc, int(count));
     }

@@ -20,13 +20,16 @@ def test_density_modes_perfect_lattice():
     r, a = positions.fcc(n, L)
     p.data["position"] = r
     b = 2 * np.pi / a
-    w = wavevector(box, [2 * b, np.sqrt(3) * b], tolerance=1e-9, max_count=10)
-    dm = observables.density_mode(p, w)
-    rho = dm.acquire()
-    assert np.all(np.abs(np.abs(rho) - n) < 50 * n * np.finfo(np.float32).eps)
-    w0 = wavevector(box, [2 * np.pi / 15.0, b], tolerance=1e-9, max_count=10)
-    rho0 = observables.density_mode(p, w0).acquire()
-    assert np.all(np.abs(rho0) < 1e-4 * n)
+    w = wavevector(box, [2 * b, np.sqrt(3) * b, 2 * np.pi / 15.0, b], tolerance=1e-9, max_count=20)
+    rho = observables.density_mode(p, w).acquire()
+    # Bragg vectors of the fcc crystal: q a / 2 pi integer and all of one parity
+    hkl = w.value * a / (2 * np.pi)
+    integer = np.all(np.abs(hkl - np.rint(hkl)) < 1e-9, axis=1)
+    par = np.rint(hkl).astype(int) % 2
+    bragg = integer & (np.all(par == par[:, :1], axis=1))
+    assert bragg.sum() >= 4 and (~bragg).sum() >= 4
+    assert np.all(np.abs(np.abs(rho[bragg]) - n) < 50 * n * np.finfo(np.float32).eps)
+    assert np.all(np.abs(rho[~bragg]) < 50 * n * np.finfo(np.float32).eps)
 
 
 @pytest.mark.parametrize("n", [4000, 64 ** 3 * 4])
