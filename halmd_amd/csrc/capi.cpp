@@ -146,6 +146,17 @@ HALMD_API int halmd_mdsim_particle_create(unsigned dimension, unsigned n, unsign
         *out = handle<halmd_particle>(new particle(n, nspecies));
     });
 }
+HALMD_API int halmd_mdsim_particle_get_position_fixed(halmd_particle* hp, uint32_t* u) {
+    return guard([&] {
+        nonnull(u, "u");
+        auto* p = as<particle>(hp);
+        require(p->has_box, "particle: no box attached yet");
+        std::vector<uint4> h(p->n);
+        p->posq.download(h.data(), p->n, p->stream);
+        p->sync();
+        for (size_t i = 0; i < p->n; ++i) { u[3 * i] = h[i].x; u[3 * i + 1] = h[i].y; u[3 * i + 2] = h[i].z; }
+    });
+}
 HALMD_API int halmd_mdsim_particle_destroy(halmd_particle* p) {
     return guard([&] { delete reinterpret_cast<particle*>(p); });
 }

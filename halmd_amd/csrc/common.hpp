@@ -178,6 +178,36 @@ __device__ __forceinline__ ff ff_mul_f(ff x, float y) {
 
 __device__ __forceinline__ ff ff_neg(ff x) { return ff{-x.hi, -x.lo}; }
 
+// (xh, xl) * (yh, yl) (DWTimesDW1 of the same paper)
+__device__ __forceinline__ ff ff_mul(ff x, ff y) {
+#pragma clang fp contract(off)
+    float const ch = x.hi * y.hi;
+    float const cl1 = __builtin_fmaf(x.hi, y.hi, -ch);
+    float const tl = x.hi * y.lo;
+    float const cl2 = __builtin_fmaf(x.lo, y.hi, tl);
+    float const cl3 = cl1 + cl2;
+    ff z;
+    fast_two_sum(ch, cl3, z.hi, z.lo);
+    return z;
+}
+
+// square root of a double word: one Newton correction of the FP32 root
+__device__ __forceinline__ ff ff_sqrt(ff x) {
+#pragma clang fp contract(off)
+    float const s = __builtin_sqrtf(x.hi);
+    float const e = __builtin_fmaf(-s, s, x.hi);
+    float const c = (e + x.lo) / (2.0f * s);
+    ff z;
+    fast_two_sum(s, c, z.hi, z.lo);
+    return z;
+}
+
+// exact double word of a 32-bit integer
+__device__ __forceinline__ ff ff_from_int(int32_t d) {
+    float const hi = float(d);
+    return ff{hi, float(d - int32_t(hi))};
+}
+
 // ---------------------------------------------------------------------------
 // Wave-level helpers (wave64)
 // ---------------------------------------------------------------------------

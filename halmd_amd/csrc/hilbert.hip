@@ -68,9 +68,10 @@ __global__ void hilbert_map_kernel(float4 const* __restrict__ pos, unsigned n, f
 
 __global__ void rearrange_kernel(float4 const* __restrict__ pos_in, float4 const* __restrict__ vel_in,
                                  int4 const* __restrict__ image_in, uint32_t const* __restrict__ id_in,
-                                 uint32_t const* __restrict__ perm, unsigned n, float4* __restrict__ pos_out,
-                                 float4* __restrict__ vel_out, int4* __restrict__ image_out,
-                                 uint32_t* __restrict__ id_out) {
+                                 uint4 const* __restrict__ posq_in, uint32_t const* __restrict__ perm, unsigned n,
+                                 float4* __restrict__ pos_out, float4* __restrict__ vel_out,
+                                 int4* __restrict__ image_out, uint32_t* __restrict__ id_out,
+                                 uint4* __restrict__ posq_out) {
     unsigned const i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     size_t const j = perm[i];
@@ -80,6 +81,7 @@ __global__ void rearrange_kernel(float4 const* __restrict__ pos_in, float4 const
     vel_out[2 * size_t(i) + 1] = vel_in[2 * j + 1];
     image_out[i] = image_in[j];
     id_out[i] = id_in[j];
+    posq_out[i] = posq_in[j];
 }
 
 __global__ void reverse_id_kernel(uint32_t const* __restrict__ id, unsigned n, uint32_t* __restrict__ rid) {
@@ -97,11 +99,11 @@ void launch_hilbert_map(float4 const* pos, unsigned n, float const* L, unsigned 
 }
 
 void launch_rearrange(float4 const* pos_in, float4 const* vel_in, int4 const* image_in, uint32_t const* id_in,
-                      uint32_t const* perm, unsigned n, float4* pos_out, float4* vel_out, int4* image_out,
-                      uint32_t* id_out, hipStream_t s) {
+                      uint4 const* posq_in, uint32_t const* perm, unsigned n, float4* pos_out, float4* vel_out,
+                      int4* image_out, uint32_t* id_out, uint4* posq_out, hipStream_t s) {
     if (!n) return;
-    rearrange_kernel<<<div_up(n, 256), 256, 0, s>>>(pos_in, vel_in, image_in, id_in, perm, n, pos_out, vel_out,
-                                                    image_out, id_out);
+    rearrange_kernel<<<div_up(n, 256), 256, 0, s>>>(pos_in, vel_in, image_in, id_in, posq_in, perm, n, pos_out,
+                                                    vel_out, image_out, id_out, posq_out);
     HIP_CHECK(hipGetLastError());
 }
 

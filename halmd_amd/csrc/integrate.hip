@@ -20,17 +20,26 @@
 namespace halmd {
 namespace {
 
-// double-word product (DWTimesDW1, Joldes, Muller & Popescu 2017)
-__device__ __forceinline__ ff ff_mul(ff x, ff y) {
+// fixed-point coordinate u = floor(frac(x / L + 1/2) * 2^32) of the double-word
+// x = hi + lo, in FP64 without contraction (so that the CPU restatement of the
+// rule gives the same integers)
+__device__ __forceinline__ uint32_t quantize(float hi, float lo, double invL) {
 #pragma clang fp contract(off)
-    float const ch = x.hi * y.hi;
-    float const cl1 = __builtin_fmaf(x.hi, y.hi, -ch);
-    float const tl = x.hi * y.lo;
-    float const cl2 = __builtin_fmaf(x.lo, y.hi, tl);
-    float const cl3 = cl1 + cl2;
-    ff z;
-    fast_two_sum(ch, cl3, z.hi, z.lo);
-    return z;
+    double const x = double(hi) + double(lo);
+    double t = x * invL;
+    t = t + 0.5;
+    t = t - floor(t);
+    return uint32_t(uint64_t(t * 4294967296.0));
+}
+
+__device__ __forceinline__ uint4 quantize4(float4 const h, float4 const l, quant_params const& q) {
+    return make_uint4(quantize(h.x, l.x, q.invL[0]), quantize(h.y, l.y, q.invL[1]), quantize(h.z, l.z, q.invL[2]),
+                      __float_as_uint(h.w));
+}
+
+__global__ void quantize_kernel(float4 const* __restrict__ pos, unsigned n, quant_params q, uint4* __restrict__ posq) {
+    unsigned const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) posq[i] = quantize4(pos[2 * size_t(i)], pos[2 * size_t(i) + 1], q);
 }
 
 // fold the block maximum of d2 (>= 0) into *target (float bits); the atomic is
@@ -57,7 +66,7 @@ __device__ __forceinline__ ff ff_from_double(double x) {
 
 __global__ __launch_bounds__(256) void verlet_integrate_kernel(
     float4* __restrict__ pos, float4* __restrict__ vel, float4 const* __restrict__ force, int4* __restrict__ image,
-    unsigned n, integrate_params p, double const* __restrict__ scale, float4 const* __restrict__ r0,
+    uint4* __restrict__ posq, unsigned n, integrate_params p, double const* __restrict__ scale, float4 const* __restrict__ r0,
     uint32_t* __restrict__ max_disp2, uint32_t* __restrict__ clear_slot)
 {
     unsigned const i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -86,8 +95,10 @@ __global__ __launch_bounds__(256) void verlet_integrate_kernel(
         if (ix != 0.0f) { rx = ff_add_f(rx, -ix * p.L_hi[0]); rx = ff_add_f(rx, -ix * p.L_lo[0]); }
         if (iy != 0.0f) { ry = ff_add_f(ry, -iy * p.L_hi[1]); ry = ff_add_f(ry, -iy * p.L_lo[1]); }
         if (iz != 0.0f) { rz = ff_add_f(rz, -iz * p.L_hi[2]); rz = ff_add_f(rz, -iz * p.L_lo[2]); }
-        pos[2 * size_t(i)] = make_float4(rx.hi, ry.hi, rz.hi, rh.w);
-        pos[2 * size_t(i) + 1] = make_float4(rx.lo, ry.lo, rz.lo, 0.0f);
+        float4 const nh = make_float4(rx.hi, ry.hi, rz.hi, rh.w), nl = make_float4(rx.lo, ry.lo, rz.lo, 0.0f);
+        pos[2 * size_t(i)] = nh;
+        pos[2 * size_t(i) + 1] = nl;
+        posq[i] = quantize4(nh, nl, p.qp);
         vel[2 * size_t(i)] = make_float4(vx.hi, vy.hi, vz.hi, m);
         vel[2 * size_t(i) + 1] = make_float4(vx.lo, vy.lo, vz.lo, 0.0f);
         if (ix != 0.0f || iy != 0.0f || iz != 0.0f) {
@@ -202,12 +213,18 @@ __global__ __launch_bounds__(256) void andersen_finalize_kernel(float4* __restri
 
 } // namespace
 
-void launch_verlet_integrate(float4* pos, float4* vel, float4 const* force, int4* image, unsigned n,
+void launch_verlet_integrate(float4* pos, float4* vel, float4 const* force, int4* image, uint4* posq, unsigned n,
                              integrate_params const& p, double const* scale, float4 const* r0, uint32_t* max_disp2,
                              uint32_t* clear_slot, hipStream_t s) {
     if (!n) return;
-    verlet_integrate_kernel<<<div_up(n, 256), 256, 0, s>>>(pos, vel, force, image, n, p, scale, r0, max_disp2,
+    verlet_integrate_kernel<<<div_up(n, 256), 256, 0, s>>>(pos, vel, force, image, posq, n, p, scale, r0, max_disp2,
                                                            clear_slot);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_quantize(float4 const* pos, unsigned n, quant_params const& q, uint4* posq, hipStream_t s) {
+    if (!n) return;
+    quantize_kernel<<<div_up(n, 256), 256, 0, s>>>(pos, n, q, posq);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -34,28 +34,38 @@ void launch_iota(uint32_t* out, size_t n, hipStream_t s);
 void launch_fill_u32(uint32_t* out, uint32_t value, size_t n, hipStream_t s);
 
 // ---------------------------------------------------------------------------
+// fixed-point position snapshot (integrate.hip)
+// ---------------------------------------------------------------------------
+// posq[i] = {ux, uy, uz, species} with u = floor(frac(x / L + 1/2) * 2^32),
+// evaluated in FP64 from the double-word position x = hi + lo.  Differences
+// int32(ui - uj) are minimum-image separations in units of L / 2^32.
+struct quant_params {
+    double invL[3];     // 1 / L in FP64
+};
+void launch_quantize(float4 const* pos, unsigned n, quant_params const& q, uint4* posq, hipStream_t s);
+
+// ---------------------------------------------------------------------------
 // binning and neighbour lists (binning.hip)
 // ---------------------------------------------------------------------------
 struct cell_grid {
     unsigned ncell[3];
-    float cell_length[3];
 };
-void launch_compute_cell(float4 const* pos, unsigned n, cell_grid g, uint32_t* keys, uint32_t* vals, hipStream_t s);
+// cell index per component: (ux * ncell) >> 32, linearised x fastest
+void launch_compute_cell(uint4 const* posq, unsigned n, cell_grid g, uint32_t* keys, uint32_t* vals, hipStream_t s);
 // offset[c] = first sorted index with key >= c, c = 0 .. ncells (ncells + 1 entries)
 void launch_cell_offsets(uint32_t const* sorted_keys, unsigned n, unsigned ncells, uint32_t* offset, hipStream_t s);
-// cell_pos[k] = high-word position of particle vals[k], w = (vals[k] << 2) | species
-void launch_gather_cell_pos(float4 const* pos, uint32_t const* sorted_vals, unsigned n, float4* cell_pos, hipStream_t s);
+// cell_pos[k] = fixed-point position of particle vals[k], w = (vals[k] << 2) | species
+void launch_gather_cell_pos(uint4 const* posq, uint32_t const* sorted_vals, unsigned n, uint4* cell_pos, hipStream_t s);
 
 struct neighbour_params {
     cell_grid grid;
-    float L[3];
-    float invL[3];
+    float q[3];         // float(L) * 2^-32: length of one fixed-point unit
     unsigned ntype;
     float rr_cut_skin[MAX_TYPES * MAX_TYPES];
 };
 // full neighbour lists; list layout [(i / 64) * 64 * cap + k * 64 + i % 64];
 // *max_count receives the largest per-particle count (may exceed cap)
-void launch_build_neighbours(float4 const* pos, unsigned n, uint32_t const* offset, float4 const* cell_pos,
+void launch_build_neighbours(uint4 const* posq, unsigned n, uint32_t const* offset, uint4 const* cell_pos,
                              neighbour_params const& p, unsigned cap, uint32_t* list, uint32_t* count,
                              uint32_t* max_count, hipStream_t s);
 
@@ -68,17 +78,18 @@ enum truncation_kind : int { TRUNC_SHARP = 0, TRUNC_SHIFTED = 1, TRUNC_FORCE_SHI
 // per species pair, FP32
 struct pair_param {
     float p[4];        // LJ: {48 eps / sigma^2, 4 eps, sigma^2, -}; Morse: {eps/A, 1/(sigma B), r_min_sigma/B, A}
+    float p_lo[4];     // Morse: low words of p[1], p[2] (double-word constants of the exponent)
     float rr_cut, r_cut, en_cut, force_cut;
 };
 struct force_params {
     int kind, trunc;
     unsigned ntype;
     float rri_smooth;
-    float L_hi[3], L_lo[3], invL[3];
+    float q[3], q_lo[3];   // L * 2^-32 as a double word
     pair_param pair[MAX_TYPES * MAX_TYPES];
 };
 // force_zero: overwrite force/en_pot/stress, else add to the stored values
-void launch_pair_force(float4 const* pos, unsigned n, uint32_t const* list, uint32_t const* count, unsigned cap,
+void launch_pair_force(uint4 const* posq, unsigned n, uint32_t const* list, uint32_t const* count, unsigned cap,
                        force_params const& p, bool aux, bool force_zero, float aux_weight, float4* force,
                        float* stress, unsigned stride, hipStream_t s);
 
@@ -88,11 +99,13 @@ void launch_pair_force(float4 const* pos, unsigned n, uint32_t const* list, uint
 struct integrate_params {
     float dt_hi, dt_lo, half_dt;
     float L_hi[3], L_lo[3], invL[3];
+    quant_params qp;
 };
-// kick-drift-wrap; optional velocity pre-scale (*scale, device) and optional
-// max-displacement tracking against r0 (max |r - r0|^2 as float bits folded
-// into *max_disp2; *clear_slot, the slot of the next step, is set to zero)
-void launch_verlet_integrate(float4* pos, float4* vel, float4 const* force, int4* image, unsigned n,
+// kick-drift-wrap and fixed-point snapshot; optional velocity pre-scale
+// (*scale, device) and optional max-displacement tracking against r0 (max
+// |r - r0|^2 as float bits folded into *max_disp2; *clear_slot, the slot of
+// the next step, is set to zero)
+void launch_verlet_integrate(float4* pos, float4* vel, float4 const* force, int4* image, uint4* posq, unsigned n,
                              integrate_params const& p, double const* scale, float4 const* r0,
                              uint32_t* max_disp2, uint32_t* clear_slot, hipStream_t s);
 void launch_verlet_finalize(float4* vel, float4 const* force, unsigned n, float half_dt, hipStream_t s);
@@ -135,8 +148,8 @@ void launch_mv2_partials(float4 const* vel, unsigned n, double* partials, hipStr
 void launch_hilbert_map(float4 const* pos, unsigned n, float const* L, unsigned depth, uint32_t* code,
                         uint32_t* index, hipStream_t s);
 void launch_rearrange(float4 const* pos_in, float4 const* vel_in, int4 const* image_in, uint32_t const* id_in,
-                      uint32_t const* perm, unsigned n, float4* pos_out, float4* vel_out, int4* image_out,
-                      uint32_t* id_out, hipStream_t s);
+                      uint4 const* posq_in, uint32_t const* perm, unsigned n, float4* pos_out, float4* vel_out,
+                      int4* image_out, uint32_t* id_out, uint4* posq_out, hipStream_t s);
 void launch_reverse_id(uint32_t const* id, unsigned n, uint32_t* rid, hipStream_t s);
 
 // ---------------------------------------------------------------------------

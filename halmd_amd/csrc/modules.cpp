@@ -61,6 +61,9 @@ box::box(double const* length) {
         split_ff(L[k], L_hi[k], L_lo[k]);
         Lf[k] = float(L[k]);
         invL[k] = float(1.0 / L[k]);
+        q_hi[k] = std::ldexp(L_hi[k], -32);
+        q_lo[k] = std::ldexp(L_lo[k], -32);
+        qp.invL[k] = 1.0 / L[k];
     }
 }
 
@@ -74,6 +77,8 @@ particle::particle(unsigned n_, unsigned nspecies_) : n(n_), nspecies(nspecies_)
     HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     pos.resize(2 * size_t(n_pad));
     vel.resize(2 * size_t(n_pad));
+    posq.resize(n_pad);
+    posq_alt.resize(n_pad);
     force.resize(n_pad);
     stress.resize(6 * size_t(n_pad));
     image.resize(n_pad);
@@ -100,6 +105,24 @@ particle::particle(unsigned n_, unsigned nspecies_) : n(n_), nspecies(nspecies_)
     HIP_CHECK(hipMemsetAsync(image, 0, image.bytes(), stream));
     launch_iota(id, n_pad, stream);
     launch_iota(reverse_id, n_pad, stream);
+    requantize();
+    sync();
+}
+
+void particle::requantize() {
+    if (has_box) launch_quantize(pos, n, qp, posq, stream);
+}
+
+void particle::attach_box(box const& b) {
+    if (has_box) {
+        for (int k = 0; k < 3; ++k)
+            require(box_length[k] == b.L[k], "particle: already bound to a box of different size");
+        return;
+    }
+    qp = b.qp;
+    for (int k = 0; k < 3; ++k) box_length[k] = b.L[k];
+    has_box = true;
+    requantize();
     sync();
 }
 
@@ -138,7 +161,11 @@ void particle::upload_hilo(dbuf<float4>& b, double const* in, bool) {
     sync();
 }
 
-void particle::set_position(double const* r) { upload_hilo(pos, r, true); positions_changed(); }
+void particle::set_position(double const* r) {
+    upload_hilo(pos, r, true);
+    requantize();
+    positions_changed();
+}
 void particle::get_position(double* r) { download_hilo(pos, r); }
 void particle::set_velocity(double const* v) { upload_hilo(vel, v, false); velocities_changed(); }
 void particle::get_velocity(double* v) { download_hilo(vel, v); }
@@ -152,6 +179,7 @@ void particle::set_species(unsigned const* s) {
         std::memcpy(&host_tmp_[2 * i].w, &s[i], 4);
     }
     pos.upload(host_tmp_.data(), host_tmp_.size(), stream);
+    requantize();
     sync();
     ++species_version;
     force_valid = false;
@@ -249,7 +277,8 @@ void particle::update_force(bool need_aux) {
 }
 
 void particle::rearrange(uint32_t const* perm) {
-    launch_rearrange(pos, vel, image, id, perm, n, pos_alt, vel_alt, image_alt, id_alt, stream);
+    launch_rearrange(pos, vel, image, id, posq, perm, n, pos_alt, vel_alt, image_alt, id_alt, posq_alt, stream);
+    std::swap(posq, posq_alt);
     std::swap(pos, pos_alt);
     std::swap(vel, vel_alt);
     std::swap(image, image_alt);
@@ -367,7 +396,7 @@ force_params potential::device_params(box const& b) const {
     P.trunc = trunc;
     P.ntype = ntype2;
     P.rri_smooth = trunc == TRUNC_SMOOTH_R4 ? float(1 / (h * h)) : 0.0f;
-    for (int k = 0; k < 3; ++k) { P.L_hi[k] = b.L_hi[k]; P.L_lo[k] = b.L_lo[k]; P.invL[k] = b.invL[k]; }
+    for (int k = 0; k < 3; ++k) { P.q[k] = b.q_hi[k]; P.q_lo[k] = b.q_lo[k]; }
     for (unsigned a = 0; a < ntype1; ++a) {
         for (unsigned c = 0; c < ntype2; ++c) {
             unsigned const ab = a * ntype2 + c;
@@ -382,8 +411,8 @@ force_params potential::device_params(box const& b) const {
                 double const B = distortion[ab];
                 double const A = 2 * B * B - 1;
                 pp.p[0] = float(eps / A);
-                pp.p[1] = float(1 / (sig * B));
-                pp.p[2] = float(r_min[ab] / sig / B);
+                split_ff(1 / (sig * B), pp.p[1], pp.p_lo[1]);
+                split_ff(r_min[ab] / sig / B, pp.p[2], pp.p_lo[2]);
                 pp.p[3] = float(A);
             }
             pp.rr_cut = float(rr_cut[ab]);
@@ -402,13 +431,13 @@ binning::binning(particle& p_, box const& b_, double const* r_cut, unsigned nt1,
                  double occupancy_)
     : p(p_), b(b_), r_skin(skin), occupancy(occupancy_)
 {
+    p.attach_box(b);
     require(skin >= 0, "binning: skin must be non-negative");
     r_cut_max = *std::max_element(r_cut, r_cut + size_t(nt1) * nt2);
     double const rcs = r_cut_max + r_skin;
     ncells = 1;
     for (int k = 0; k < 3; ++k) {
         grid.ncell[k] = std::max(1u, unsigned(b.L[k] / rcs));
-        grid.cell_length[k] = float(b.L[k] / grid.ncell[k]);
         ncells *= grid.ncell[k];
     }
     keys.resize(p.n);
@@ -422,12 +451,12 @@ void binning::update() {
     if (version == p.position_version) return;
     hipStream_t s = p.stream;
     timer.start(s);
-    launch_compute_cell(p.pos, p.n, grid, keys, vals, s);
+    launch_compute_cell(p.posq, p.n, grid, keys, vals, s);
     unsigned key_bits = 0;
     while ((1ull << key_bits) < ncells) ++key_bits;
     radix_sort(keys, vals, p.n, key_bits, ws, s);
     launch_cell_offsets(keys, p.n, ncells, offset, s);
-    launch_gather_cell_pos(p.pos, vals, p.n, cell_pos, s);
+    launch_gather_cell_pos(p.posq, vals, p.n, cell_pos, s);
     timer.stop(s);
     version = p.position_version;
 }
@@ -445,6 +474,7 @@ void binning::cell_lists(unsigned* begin_out, unsigned* end_out, unsigned* index
 // max_displacement (halmd/mdsim/gpu/max_displacement.cpp:60-106)
 // ---------------------------------------------------------------------------
 max_displacement::max_displacement(particle& p_, box const& b_) : p(p_), b(b_) {
+    p.attach_box(b);
     r0.resize(p.n);
     d_max2.resize(2);
     h_max2.resize(1);
@@ -492,6 +522,7 @@ neighbour::neighbour(particle& p_, binning& bin_, max_displacement& disp_, box c
                      unsigned nt1, unsigned nt2, double skin, double occupancy_)
     : p(p_), bin(bin_), disp(disp_), b(b_), r_skin(skin), occupancy(occupancy_)
 {
+    p.attach_box(b);
     require(nt1 == nt2 && nt1 == p.nspecies, "neighbour: cutoff matrix must be nspecies x nspecies");
     require(occupancy > 0 && occupancy <= 1, "neighbour: occupancy must be in (0, 1]");
     for (int k = 0; k < 3; ++k)
@@ -500,7 +531,7 @@ neighbour::neighbour(particle& p_, binning& bin_, max_displacement& disp_, box c
     r_cut.assign(rc, rc + size_t(nt1) * nt2);
     std::memset(&params, 0, sizeof(params));
     params.grid = bin.grid;
-    for (int k = 0; k < 3; ++k) { params.L[k] = b.Lf[k]; params.invL[k] = b.invL[k]; }
+    for (int k = 0; k < 3; ++k) params.q[k] = b.q_hi[k];
     params.ntype = nt2;
     double rc_max = 0;
     for (size_t i = 0; i < r_cut.size(); ++i) {
@@ -534,7 +565,7 @@ void neighbour::update() {
     while (true) {
         launch_fill_u32(d_maxcount, 0u, 1, s);
         timer.start(s);
-        launch_build_neighbours(p.pos, p.n, bin.offset, bin.cell_pos, params, cap, list, count, d_maxcount, s);
+        launch_build_neighbours(p.posq, p.n, bin.offset, bin.cell_pos, params, cap, list, count, d_maxcount, s);
         timer.stop(s);
         HIP_CHECK(hipMemcpyAsync(h_maxcount.get(), d_maxcount.get(), 4, hipMemcpyDeviceToHost, s));
         p.sync();
@@ -566,6 +597,7 @@ void neighbour::get_lists(unsigned* counts, unsigned* entries, unsigned stride) 
 pair_trunc::pair_trunc(potential const& pot_, particle& p_, box const& b_, neighbour& nb_, double w)
     : pot(pot_), p(p_), b(b_), nb(nb_), aux_weight(w)
 {
+    p.attach_box(b);
     require(pot.ntype1 == p.nspecies && pot.ntype2 == p.nspecies, "pair_trunc: potential shape != nspecies");
     params = pot.device_params(b);
     p.forces.push_back(this);
@@ -582,7 +614,7 @@ void pair_trunc::apply(bool force_zero) {
     // a particle interacting with itself counts each pair twice (pair_trunc.hpp:291-294)
     float const w = float(aux_weight / 2);
     timer.start(p.stream);
-    launch_pair_force(p.pos, p.n, nb.list, nb.count, nb.cap, params, aux, force_zero, w, p.force, p.stress, p.n_pad,
+    launch_pair_force(p.posq, p.n, nb.list, nb.count, nb.cap, params, aux, force_zero, w, p.force, p.stress, p.n_pad,
                       p.stream);
     timer.stop(p.stream);
 }
@@ -591,7 +623,9 @@ void pair_trunc::apply(bool force_zero) {
 // integrators
 // ---------------------------------------------------------------------------
 verlet::verlet(particle& p_, box const& b_, double dt_) : p(p_), b(b_) {
+    p.attach_box(b);
     for (int k = 0; k < 3; ++k) { ip.L_hi[k] = b.L_hi[k]; ip.L_lo[k] = b.L_lo[k]; ip.invL[k] = b.invL[k]; }
+    ip.qp = b.qp;
     set_timestep(dt_);
 }
 
@@ -615,7 +649,7 @@ void verlet::integrate_kernel(double const* scale) {
         clear = fused_disp->d_max2.get() + (fused_disp->slot ^ 1u);
     }
     timer_integrate.start(p.stream);
-    launch_verlet_integrate(p.pos, p.vel, p.force, p.image, p.n, ip, scale, r0, slot, clear, p.stream);
+    launch_verlet_integrate(p.pos, p.vel, p.force, p.image, p.posq, p.n, ip, scale, r0, slot, clear, p.stream);
     timer_integrate.stop(p.stream);
     p.positions_changed();
     p.velocities_changed();
@@ -734,6 +768,7 @@ void verlet_nvt_andersen::finalize() {
 // hilbert (halmd/mdsim/gpu/sorts/hilbert.cpp:40-117)
 // ---------------------------------------------------------------------------
 hilbert::hilbert(particle& p_, box const& b_) : p(p_), b(b_) {
+    p.attach_box(b);
     float const max_length = std::max(std::max(b.Lf[0], b.Lf[1]), b.Lf[2]);
     depth = unsigned(std::ceil(std::log(max_length) / M_LN2));
     depth = std::min(10u, depth);
@@ -778,6 +813,7 @@ void core::mdstep() {
 // halmd/observables/thermodynamics.hpp:80-90)
 // ---------------------------------------------------------------------------
 thermodynamics::thermodynamics(particle& p_, box const& b_) : p(p_), b(b_) {
+    p.attach_box(b);
     partials.resize(size_t(REDUCE_BLOCKS) * THERMO_NQ);
     out.resize(THERMO_NQ);
     h_out.resize(THERMO_NQ);
@@ -941,6 +977,7 @@ blocking_scheme::blocking_scheme(particle& p_, box const& b_, double timestep, d
                                  unsigned bsz, unsigned shift, unsigned sep, uint64_t step0)
     : p(p_), b(b_), block_size(bsz), separation(sep)
 {
+    p.attach_box(b);
     require(block_size >= 2, "blocking_scheme: minimum block size is 2");
     require(shift != 0, "blocking_scheme: coarse-graining shift must be non-zero");
     require(shift < block_size, "blocking_scheme: coarse-graining shift must be less than block size");

@@ -56,6 +56,8 @@ public:
     explicit box(double const* length);
     double L[3];
     float L_hi[3], L_lo[3], Lf[3], invL[3];
+    float q_hi[3], q_lo[3];           // L * 2^-32 (fixed-point unit) as a double word
+    quant_params qp;                  // 1 / L in FP64
     double volume() const { return L[0] * L[1] * L[2]; }
 };
 
@@ -67,9 +69,13 @@ public:
     ~particle();
 
     unsigned n, n_pad, nspecies;
+    quant_params qp;
+    bool has_box = false;
+    double box_length[3] = {0, 0, 0};
     hipStream_t stream;
 
     dbuf<float4> pos, vel, force;     // pos/vel: 2 float4 per particle (hi, lo)
+    dbuf<uint4> posq, posq_alt;       // fixed-point snapshot of pos (written with every position update)
     dbuf<float> stress;               // 6 x n_pad, column-major
     dbuf<int4> image;
     dbuf<uint32_t> id, reverse_id;
@@ -99,6 +105,11 @@ public:
     void get_reverse_id(unsigned* v);
 
     void positions_changed() { ++position_version; force_valid = false; aux_valid = false; }
+    // recompute the fixed-point snapshot after a host-side change of pos
+    void requantize();
+    // bind the periodic box (done by every module constructed from particle
+    // and box); a particle belongs to one box
+    void attach_box(box const& b);
     void velocities_changed() { ++velocity_version; }
     // request en_pot and stress with the next force evaluation
     void aux_enable() { aux_enabled = true; }
@@ -144,7 +155,7 @@ public:
     cell_grid grid;
     unsigned ncells;
     dbuf<uint32_t> keys, vals, offset;   // offset: ncells + 1 entries
-    dbuf<float4> cell_pos;
+    dbuf<uint4> cell_pos;
     radix_sort_workspace ws;
     uint64_t version = 0;             // position version of the last update
     void update();

@@ -74,6 +74,10 @@ class _data:
             out = np.zeros(n)
             call("halmd_mdsim_particle_get_potential_energy", p.handle, dptr(out))
             return out
+        if name == "position_fixed":
+            out = np.zeros((n, 3), dtype=np.uint32)
+            call("halmd_mdsim_particle_get_position_fixed", p.handle, out.ctypes.data_as(ctypes.c_void_p))
+            return out
         if name == "stress_pot":
             out = np.zeros((n, 6))
             call("halmd_mdsim_particle_get_stress_pot", p.handle, dptr(out))

@@ -80,7 +80,9 @@ HALMD_API int halmd_mdsim_box_length(const halmd_box* box, double* length);
 HALMD_API int halmd_mdsim_box_volume(const halmd_box* box, double* volume);
 
 /* ---- mdsim.particle: halmd/mdsim/gpu/particle.cpp:55-160 (particle(n, nspecies)),
- *      lua/halmd/mdsim/particle.lua.in:229-290 ---------------------------- */
+ *      lua/halmd/mdsim/particle.lua.in:229-290.  The first module constructed
+ *      from (particle, box) binds the box, which fixes the periodic fixed-point
+ *      snapshot kept beside the float-float positions. ----------------------- */
 HALMD_API int halmd_mdsim_particle_create(unsigned dimension, unsigned nparticle, unsigned nspecies,
                                           halmd_particle** out);
 HALMD_API int halmd_mdsim_particle_destroy(halmd_particle* p);
@@ -104,6 +106,9 @@ HALMD_API int halmd_mdsim_particle_get_potential_energy(halmd_particle* p, doubl
 HALMD_API int halmd_mdsim_particle_get_stress_pot(halmd_particle* p, double* stress);
 HALMD_API int halmd_mdsim_particle_get_id(halmd_particle* p, unsigned* id);
 HALMD_API int halmd_mdsim_particle_get_reverse_id(halmd_particle* p, unsigned* reverse_id);
+/* fixed-point snapshot u = floor(frac(x / L + 1/2) 2^32) of each coordinate
+ * (3 per particle; requires a bound box) */
+HALMD_API int halmd_mdsim_particle_get_position_fixed(halmd_particle* p, uint32_t* u);
 /* particle:aux_enable() (particle.cpp:221-253) */
 HALMD_API int halmd_mdsim_particle_aux_enable(halmd_particle* p);
 /* particle:rearrange(index) (particle.cpp:173-218); index[new] = old */

@@ -12,16 +12,18 @@
 //      follows.  These give the expected values for forces, energies, the
 //      virial, trajectories, density modes and time correlations.
 //
-//  (2) FP32 restatements of the device-side integer rules whose results must be
-//      bit-exact: the cell index, the neighbour predicate and list order, and
-//      the Hilbert code.  They use IEEE single precision with no contraction
-//      (built with -ffp-contract=off on x86-64 SSE), the same operation order
-//      as the HIP kernels, and correctly rounded division and sqrt.
+//  (2) Restatements of the device-side rules whose results must be
+//      bit-exact: the fixed-point position snapshot, the cell index, the
+//      neighbour predicate and list order, and the Hilbert code.  They use IEEE
+//      single/double precision with no contraction (built with
+//      -ffp-contract=off on x86-64 SSE), the same operation order as the HIP
+//      kernels, and correctly rounded division and sqrt.
 //
 // Pinning (what each function is checked against, in tests/test_oracle.py):
 //  - potentials: the reference's golden (r, fval, U) tables;
 //  - forces: brute-force O(N^2) sums, Newton's third law, finite differences;
 //  - binning: the reference binning test invariant cell == floor(r*ncell/L);
+//  - fixed-point rules: integer closed forms, brute-force pair enumeration;
 //  - neighbour lists: brute-force pair enumeration;
 //  - integrators: ideal-gas conservation and NVE energy conservation;
 //  - NHC: the conserved extended energy;
@@ -678,20 +680,34 @@ void oracle_pair_force_brute(int n, double const* pos, int const* species, doubl
 // FP32 device rules (bit-exact references)
 // ===========================================================================
 
-// Cell index rule of the device binning (follows the reference device rule,
-// halmd/mdsim/gpu/binning_kernel.cu:34-54): per component
-// idx = uint(r / cell_length + ncell) mod ncell in IEEE single precision,
-// linearised with x fastest.  Ghost entries (type == ~0u) map to ~0u.
-void oracle_fp32_cell_index(int n, float const* pos4, unsigned const* ncell, float const* cell_length, unsigned* cell)
+// Fixed-point snapshot of the device (the binning, neighbour and force kernels
+// work on it): the FP64 input x is split into the float-float pair the device
+// stores (hi = float(x), lo = float(x - hi)), and u = floor(frac((hi + lo) / L
+// + 1/2) 2^32) is evaluated in FP64 without contraction, with 1/L in FP64.
+void oracle_fixed_point(int n, double const* pos, double const* L, uint32_t* u)
+{
+    double invL[3];
+    for (int k = 0; k < 3; ++k) invL[k] = 1.0 / L[k];
+    for (size_t i = 0; i < 3 * size_t(n); ++i) {
+        int const k = int(i % 3);
+        float const hi = float(pos[i]);
+        float const lo = float(pos[i] - double(hi));
+        double const x = double(hi) + double(lo);
+        double t = x * invL[k];
+        t = t + 0.5;
+        t = t - std::floor(t);
+        u[i] = uint32_t(uint64_t(t * 4294967296.0));
+    }
+}
+
+// Cell index rule of the device binning: per component (u * ncell) >> 32
+// (the reference device rule halmd/mdsim/gpu/binning_kernel.cu:34-54 with the
+// grid anchored at the lower box corner), linearised x fastest.
+void oracle_fixed_cell_index(int n, uint32_t const* u, unsigned const* ncell, unsigned* cell)
 {
     for (int i = 0; i < n; ++i) {
-        float const* r = pos4 + 4 * size_t(i);
         unsigned idx[3];
-        for (int k = 0; k < 3; ++k) {
-            float q = r[k] / cell_length[k];
-            float s = q + float(ncell[k]);
-            idx[k] = unsigned(s) % ncell[k];
-        }
+        for (int k = 0; k < 3; ++k) idx[k] = unsigned((uint64_t(u[3 * size_t(i) + k]) * ncell[k]) >> 32);
         cell[i] = (idx[2] * ncell[1] + idx[1]) * ncell[0] + idx[0];
     }
 }
@@ -700,27 +716,25 @@ void oracle_fp32_cell_index(int n, float const* pos4, unsigned const* ncell, flo
 // Candidate order: the 9 rows (dz, dy) of neighbour cells in lexicographic
 // order (dz outer), in each row the cells x-1, x, x+1 (periodic), in each cell
 // ascending particle index (the order of the stable cell sort).
-// Predicate (halmd/mdsim/gpu/neighbours/from_binning_kernel.cu:144-163):
-// dr = r_i - r_j (FP32, hi words), img = rint(dr * invL), dr -= img * L,
-// rr = (dx*dx + dy*dy) + dz*dz, rr <= rr_cut_skin[a][b]; the device rule
-// multiplies by the FP32-rounded 1/L instead of dividing by L.
+// Predicate (halmd/mdsim/gpu/neighbours/from_binning_kernel.cu:144-163 on the
+// fixed-point coordinates): d_k = float(int32(u_i,k - u_j,k)) * q_k (FP32),
+// rr = (dx*dx + dy*dy) + dz*dz, rr <= rr_cut_skin[a][b].
 // Writes counts[n] and the lists row-major in list[i*cap + k] (up to cap
 // entries per particle).  Returns the maximum count.
-int oracle_fp32_neighbour_list(int n, float const* pos4, unsigned const* ncell, float const* cell_length,
-                               float const* L, float const* invL, int ntype, float const* rr_cut_skin, int cap,
-                               unsigned* counts, unsigned* list)
+int oracle_fixed_neighbour_list(int n, uint32_t const* u, unsigned const* species, unsigned const* ncell,
+                                float const* q, int ntype, float const* rr_cut_skin, int cap, unsigned* counts,
+                                unsigned* list)
 {
     size_t ncells = size_t(ncell[0]) * ncell[1] * ncell[2];
     std::vector<unsigned> cell(n);
-    oracle_fp32_cell_index(n, pos4, ncell, cell_length, cell.data());
+    oracle_fixed_cell_index(n, u, ncell, cell.data());
     std::vector<std::vector<unsigned>> cells(ncells);
     for (int i = 0; i < n; ++i) cells[cell[i]].push_back(unsigned(i));
     int maxc = 0;
     int const nx = int(ncell[0]), ny = int(ncell[1]), nz = int(ncell[2]);
     for (int i = 0; i < n; ++i) {
-        float const* ri = pos4 + 4 * size_t(i);
-        unsigned ti;
-        std::memcpy(&ti, ri + 3, 4);
+        uint32_t const* ui = u + 3 * size_t(i);
+        unsigned const ti = species[i];
         unsigned c = cell[i];
         int cx = int(c % ncell[0]), cy = int((c / ncell[0]) % ncell[1]), cz = int(c / ncell[0] / ncell[1]);
         unsigned count = 0;
@@ -733,20 +747,16 @@ int oracle_fp32_neighbour_list(int n, float const* pos4, unsigned const* ncell, 
             auto const& cl = cells[(size_t(kz) * ny + ky) * nx + kx];
             for (unsigned j : cl) {
                 if (j == unsigned(i)) continue;
-                float const* rj = pos4 + 4 * size_t(j);
-                unsigned tj;
-                std::memcpy(&tj, rj + 3, 4);
+                uint32_t const* uj = u + 3 * size_t(j);
                 float d[3];
                 for (int k = 0; k < 3; ++k) {
-                    float dk = ri[k] - rj[k];
-                    float img = std::rint(dk * invL[k]);
-                    float sh = img * L[k];
-                    d[k] = dk - sh;
+                    int32_t const D = int32_t(ui[k] - uj[k]);
+                    d[k] = float(D) * q[k];
                 }
                 float rr = d[0] * d[0];
                 rr = rr + d[1] * d[1];
                 rr = rr + d[2] * d[2];
-                if (rr <= rr_cut_skin[ti * ntype + tj]) {
+                if (rr <= rr_cut_skin[ti * ntype + species[j]]) {
                     if (int(count) < cap) list[size_t(i) * cap + count] = j;
                     count++;
                 }

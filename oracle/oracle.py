@@ -50,8 +50,9 @@ def lib():
             "oracle_neighbour_half": (ctypes.c_long, [ctypes.c_int, _d, _i, _d, _vp, ctypes.c_double, ctypes.c_long, _l, _u]),
             "oracle_pair_force": (None, [ctypes.c_int, _d, _i, _d, _vp, ctypes.c_double, ctypes.c_double, _d, _d, _d]),
             "oracle_pair_force_brute": (None, [ctypes.c_int, _d, _i, _d, _vp, _d, _d, _d]),
-            "oracle_fp32_cell_index": (None, [ctypes.c_int, _f, _u, _f, _u]),
-            "oracle_fp32_neighbour_list": (ctypes.c_int, [ctypes.c_int, _f, _u, _f, _f, _f, ctypes.c_int, _f, ctypes.c_int, _u, _u]),
+            "oracle_fixed_point": (None, [ctypes.c_int, _d, _d, _u]),
+            "oracle_fixed_cell_index": (None, [ctypes.c_int, _u, _u, _u]),
+            "oracle_fixed_neighbour_list": (ctypes.c_int, [ctypes.c_int, _u, _u, _u, _f, ctypes.c_int, _f, ctypes.c_int, _u, _u]),
             "oracle_fp32_hilbert_code": (None, [ctypes.c_int, _f, _f, ctypes.c_uint, _u]),
             "oracle_hilbert_depth": (ctypes.c_uint, [_d]),
             "oracle_hilbert_map_f64": (ctypes.c_uint, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint]),
@@ -186,31 +187,40 @@ def pair_force_brute(pos, species, L, pot):
     return f, u, s
 
 
-def fp32_cell_index(pos4, ncell, cell_length):
-    pos4 = np.ascontiguousarray(pos4, dtype=np.float32)
-    nc = np.ascontiguousarray(ncell, dtype=np.uint32)
-    cl = np.ascontiguousarray(cell_length, dtype=np.float32)
-    out = np.zeros(len(pos4), dtype=np.uint32)
-    lib().oracle_fp32_cell_index(len(pos4), _p(pos4, _f), _p(nc, _u), _p(cl, _f), _p(out, _u))
+def fixed_point(pos, L):
+    """uint32 fixed-point coordinates [n, 3] of FP64 positions (device rule)."""
+    pos = _dd(pos)
+    L = _dd(L)
+    out = np.zeros(pos.shape, dtype=np.uint32)
+    lib().oracle_fixed_point(len(pos), _p(pos, _d), _p(L, _d), _p(out, _u))
     return out
 
 
-def fp32_neighbour_list(pos4, ncell, cell_length, L, rr_cut_skin, cap):
-    """L is the FP64 box length; the rule uses float(L) and float(1/L)."""
-    pos4 = np.ascontiguousarray(pos4, dtype=np.float32)
+def fixed_cell_index(u, ncell):
+    u = np.ascontiguousarray(u, dtype=np.uint32)
     nc = np.ascontiguousarray(ncell, dtype=np.uint32)
-    cl = np.ascontiguousarray(cell_length, dtype=np.float32)
-    L = np.asarray(L, dtype=np.float64)
-    Lf = np.ascontiguousarray(L, dtype=np.float32)
-    invLf = np.ascontiguousarray(1.0 / L, dtype=np.float32)
+    out = np.zeros(len(u), dtype=np.uint32)
+    lib().oracle_fixed_cell_index(len(u), _p(u, _u), _p(nc, _u), _p(out, _u))
+    return out
+
+
+def fixed_unit(L):
+    """float(L) * 2^-32 per dimension (FP32 length of one fixed-point unit)."""
+    return np.ldexp(np.asarray(L, dtype=np.float64).astype(np.float32), -32).astype(np.float32)
+
+
+def fixed_neighbour_list(u, species, ncell, L, rr_cut_skin, cap):
+    u = np.ascontiguousarray(u, dtype=np.uint32)
+    sp = np.ascontiguousarray(species, dtype=np.uint32)
+    nc = np.ascontiguousarray(ncell, dtype=np.uint32)
+    q = np.ascontiguousarray(fixed_unit(L), dtype=np.float32)
     rr = np.ascontiguousarray(rr_cut_skin, dtype=np.float32)
     ntype = rr.shape[0]
-    n = len(pos4)
+    n = len(u)
     counts = np.zeros(n, dtype=np.uint32)
     lst = np.zeros(n * cap, dtype=np.uint32)
-    m = lib().oracle_fp32_neighbour_list(n, _p(pos4, _f), _p(nc, _u), _p(cl, _f), _p(Lf, _f), _p(invLf, _f), ntype,
-                                         _p(rr, _f), cap,
-                                         _p(counts, _u), _p(lst, _u))
+    m = lib().oracle_fixed_neighbour_list(n, _p(u, _u), _p(sp, _u), _p(nc, _u), _p(q, _f), ntype, _p(rr, _f), cap,
+                                          _p(counts, _u), _p(lst, _u))
     return counts, lst.reshape(n, cap), m
 
 

@@ -189,60 +189,86 @@ def test_neighbour_half_list_is_brute_force_set(oracle):
     assert np.array_equal(got | got.T, m)     # and every pair
 
 
-def test_fp32_neighbour_list_rules(oracle):
-    # device rule: full lists; the set equals the FP32 predicate on all pairs,
-    # the lists are symmetric, and entries follow the documented cell order
+def test_fixed_point_rule(oracle):
+    # u = floor(frac(x / L + 1/2) 2^32): closed form on exactly representable
+    # inputs, periodicity, and one unit per L / 2^32
+    L = np.array([16.0, 32.0, 8.0])
+    x = np.array([[-8.0, -16.0, -4.0], [0.0, 0.0, 0.0], [4.0, -8.0, 1.0], [8.0, 16.0, 4.0 - 2.0 ** -30]])
+    u = oracle.fixed_point(x, L)
+    assert list(u[0]) == [0, 0, 0]
+    assert list(u[1]) == [2 ** 31] * 3
+    assert list(u[2]) == [3 * 2 ** 30, 2 ** 30, 2 ** 31 + 2 ** 29]
+    assert u[3, 0] == 0 and u[3, 1] == 0          # +L/2 wraps to -L/2
+    assert u[3, 2] == 2 ** 32 - 1                 # 4 - 2^-30 in a box of 8: half a unit below +L/2
+    rng = np.random.default_rng(3)
+    L = np.array([107.49, 37.35, 17.0])
+    p = rng.uniform(-L / 2, L / 2, (10000, 3))
+    u = oracle.fixed_point(p, L).astype(np.float64)
+    assert np.all(np.abs(u / 2 ** 32 - (p / L + 0.5)) < 2e-9)
+    # differences are minimum-image separations
+    i, j = rng.integers(0, 10000, (2, 1000))
+    d = (oracle.fixed_point(p, L)[i] - oracle.fixed_point(p, L)[j]).astype(np.int32) * (L / 2 ** 32)
+    dr = p[i] - p[j]
+    dr -= L * np.rint(dr / L)
+    assert np.max(np.abs(d - dr)) < 1e-7
+
+
+def test_fixed_cell_index_is_floor_rule(oracle):
+    # the reference binning invariant cell == floor(r ncell / L), for positions
+    # measured from the lower box corner (test/unit/mdsim/binning.cpp:146)
+    L = np.array([17.0, 19.0, 23.0])
+    pos, _ = oracle.lattice_fcc(4000, L)
+    k = 2 * np.pi / L
+    pos = pos + (0.9 / k) * np.sin(k * pos)
+    ncell = np.floor(L / 2.8).astype(np.uint32)
+    u = oracle.fixed_point(pos, L)
+    c = oracle.fixed_cell_index(u, ncell)
+    idx = np.floor((pos + L / 2) * ncell / L).astype(np.int64) % ncell
+    expect = (idx[:, 2] * ncell[1] + idx[:, 1]) * ncell[0] + idx[:, 0]
+    assert np.array_equal(c, expect)
+
+
+def test_fixed_neighbour_list_rules(oracle):
+    # device rule: full lists; the set equals the predicate evaluated on all
+    # pairs, the lists are symmetric, and entries follow the documented order
     pos, L, species, _ = _liquid(oracle, 2048, 0.8442, 5, species_frac=0.3)
     pot = _ka_potential(oracle)
     skin = 0.3
-    rcmax = float(np.max(pot.r_cut))
-    ncell = np.floor(L / (rcmax + skin)).astype(np.uint32)
-    cl = (L / ncell).astype(np.float32)
-    pos4 = np.zeros((len(pos), 4), dtype=np.float32)
-    pos4[:, :3] = pos.astype(np.float32)
-    pos4[:, 3] = species.astype(np.uint32).view(np.float32)
+    ncell = np.floor(L / (np.max(pot.r_cut) + skin)).astype(np.uint32)
     rcs = ((pot.r_cut + skin) ** 2).astype(np.float32)
-    counts, lst, m = oracle.fp32_neighbour_list(pos4, ncell, cl, L, rcs, 256)
+    u = oracle.fixed_point(pos, L)
+    counts, lst, m = oracle.fixed_neighbour_list(u, species, ncell, L, rcs, 256)
     assert m <= 256
-    p = pos4[:, :3]
-    Lf = L.astype(np.float32)
-    invLf = (1.0 / L).astype(np.float32)
-    d = p[:, None, :] - p[None, :, :]
-    d = d - np.rint(d * invLf) * Lf
+    q = oracle.fixed_unit(L)
+    D = (u[:, None, :] - u[None, :, :]).astype(np.int32)
+    d = D.astype(np.float32) * q
     rr = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
     expect = rr <= rcs[species[:, None], species[None, :]]
     np.fill_diagonal(expect, False)
     got = np.zeros_like(expect)
-    for i in range(len(p)):
+    for i in range(len(u)):
         got[i, lst[i, :counts[i]]] = True
         assert len(np.unique(lst[i, :counts[i]])) == counts[i]
     assert np.array_equal(got, expect)
     assert np.array_equal(got, got.T)
+    # and equals the FP64 set except for pairs within rounding of rc + skin
+    dr = pos[:, None, :] - pos[None, :, :]
+    dr -= L * np.rint(dr / L)
+    r64 = np.sqrt(np.sum(dr * dr, -1))
+    cut = np.sqrt(rcs[species[:, None], species[None, :]].astype(np.float64))
+    clear = np.abs(r64 - cut) > 1e-5
+    np.fill_diagonal(clear, False)
+    assert np.array_equal(got[clear], (r64 < cut)[clear])
     # order: rows (dz, dy) lexicographic, cells x-1, x, x+1 in each row,
     # ascending particle index in each cell
-    cell = oracle.fp32_cell_index(pos4, ncell, cl)
+    cell = oracle.fixed_cell_index(u, ncell)
     cxyz = np.stack([cell % ncell[0], (cell // ncell[0]) % ncell[1], cell // ncell[0] // ncell[1]], 1).astype(int)
-    for i in range(0, len(p), 97):
+    for i in range(0, len(u), 97):
         js = lst[i, :counts[i]]
         off = (cxyz[js] - cxyz[i] + 1) % ncell.astype(int) - 1
         keys = [((o[2], o[1], o[0]), j) for o, j in zip(off, js)]
         assert keys == sorted(keys)
         assert np.all(np.abs(off) <= 1)
-
-
-def test_fp32_cell_index_matches_floor_rule(oracle):
-    n = 4000
-    L = np.array([16.79, 16.79, 16.79])
-    pos, _ = oracle.lattice_fcc(n, L)
-    pos4 = np.zeros((n, 4), np.float32)
-    pos4[:, :3] = pos
-    ncell = np.array([5, 5, 5], np.uint32)
-    cl = (L / ncell).astype(np.float32)
-    c = oracle.fp32_cell_index(pos4, ncell, cl)
-    # numpy float32 restatement of the same operations
-    q = (pos4[:, :3] / cl).astype(np.float32) + ncell.astype(np.float32)
-    idx = q.astype(np.uint32) % ncell
-    assert np.array_equal(c, (idx[:, 2] * ncell[1] + idx[:, 1]) * ncell[0] + idx[:, 0])
 
 
 # ----------------------------------------------------------------------------
