@@ -69,6 +69,18 @@ def kernel_times(w, steps):
     return out
 
 
+def calibration_copy(nbytes):
+    import ctypes
+    from halmd_amd import _lib
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    _lib.call("halmd_device_alloc", nbytes, ctypes.byref(a))
+    _lib.call("halmd_device_alloc", nbytes, ctypes.byref(b))
+    _lib.call("halmd_device_copy", b, a, nbytes)
+    _lib.call("halmd_device_free", a)
+    _lib.call("halmd_device_free", b)
+    log(f"calibration copy: {nbytes} bytes read + {nbytes} bytes written")
+
+
 def cpu_baseline(w, budget_s):
     """FP64 oracle (host backend restatement), one thread, same initial state."""
     from oracle import oracle as orc
@@ -98,6 +110,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="run one copy kernel of this size first (hardware-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs one GPU per process; --gpus {args.gpus} is not supported "
@@ -106,6 +120,8 @@ def main():
 
     import halmd_amd
     arch, cus, mem = halmd_amd.device_info()
+    if args.calibration_copy_mib:
+        calibration_copy(args.calibration_copy_mib << 20)
     log(f"device {arch}, {cus} CUs, {mem / 2**30:.0f} GiB; building LJ N={N}")
     t0 = time.perf_counter()
     w = build_workload(args.seed)

@@ -104,6 +104,14 @@ HALMD_API int halmd_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
     return guard([&] { HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
 }
 
+HALMD_API int halmd_device_copy(void* dst, const void* src, size_t bytes) {
+    return guard([&] {
+        require(bytes % 16 == 0, "device_copy: size must be a multiple of 16 bytes");
+        launch_copy(src, dst, bytes, nullptr);
+        HIP_CHECK(hipStreamSynchronize(nullptr));
+    });
+}
+
 // ---- algorithm ---------------------------------------------------------------
 HALMD_API int halmd_algorithm_radix_sort(uint32_t* keys, uint32_t* values, size_t n, unsigned key_bits) {
     return guard([&] {

@@ -32,6 +32,8 @@ void radix_sort(uint32_t* keys, uint32_t* vals, size_t n, unsigned key_bits, rad
 
 void launch_iota(uint32_t* out, size_t n, hipStream_t s);
 void launch_fill_u32(uint32_t* out, uint32_t value, size_t n, hipStream_t s);
+// copy of bytes / 16 16-byte words (a multiple of 16 bytes)
+void launch_copy(void const* src, void* dst, size_t bytes, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // fixed-point position snapshot (integrate.hip)

@@ -165,6 +165,13 @@ __global__ void iota_kernel(uint32_t* out, size_t n) {
     if (i < n) out[i] = uint32_t(i);
 }
 
+// plain streaming copy (16 bytes per lane); the reference traffic for
+// hardware-counter calibration
+__global__ __launch_bounds__(256) void copy_kernel(uint4 const* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 __global__ void fill_u32_kernel(uint32_t* out, uint32_t value, size_t n) {
     size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) out[i] = value;
@@ -246,6 +253,13 @@ void radix_sort(uint32_t* keys, uint32_t* vals, size_t n, unsigned key_bits, rad
 void launch_iota(uint32_t* out, size_t n, hipStream_t s) {
     if (!n) return;
     iota_kernel<<<div_up(n, 256), 256, 0, s>>>(out, n);
+}
+
+void launch_copy(void const* src, void* dst, size_t bytes, hipStream_t s) {
+    size_t const n = bytes / sizeof(uint4);
+    if (!n) return;
+    copy_kernel<<<div_up(n, 256), 256, 0, s>>>(static_cast<uint4 const*>(src), static_cast<uint4*>(dst), n);
+    HIP_CHECK(hipGetLastError());
 }
 
 void launch_fill_u32(uint32_t* out, uint32_t value, size_t n, hipStream_t s) {

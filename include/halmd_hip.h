@@ -65,6 +65,9 @@ HALMD_API int halmd_device_alloc(size_t bytes, void** ptr);
 HALMD_API int halmd_device_free(void* ptr);
 HALMD_API int halmd_memcpy_htod(void* dst, const void* src, size_t bytes);
 HALMD_API int halmd_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+/* streaming device-to-device copy kernel (bytes a multiple of 16): the
+ * reference traffic for calibrating hardware memory counters */
+HALMD_API int halmd_device_copy(void* dst, const void* src, size_t bytes);
 
 /* ---- algorithm (kernel seam; device pointers, synchronous) -------------- */
 /* stable sort of (key, value) by the low key_bits bits of key;
