@@ -160,11 +160,15 @@ class Handle:
             raise HalmdError(1, f"{type(self).__name__} is not initialised")
         return self._h
 
-    def __del__(self):
+    def close(self):
+        """Destroy the C-ABI object now (a force module then leaves its particle)."""
         h = getattr(self, "_h", None)
         if h is not None and h.value and self._destroy and _lib is not None:
             getattr(_lib, self._destroy)(h)
             self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 def device_info():

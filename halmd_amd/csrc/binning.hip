@@ -20,12 +20,16 @@
 // rr <= (rc_ab + skin)^2 in FP32 (from_binning_kernel.cu:144-163), rr formed
 // as (dx*dx + dy*dy) + dz*dz without contraction.  Candidate order: the 9 rows
 // (dz, dy) in lexicographic order, in each row the cells x-1, x, x+1, in each
-// cell ascending particle index.  The list is stored per wavefront of 64
-// particles, column-major inside the wavefront: entry k of particle i sits at
-// (i / 64) * 64 * cap + k * 64 + i % 64, so one wave reading entry k touches
-// 256 contiguous bytes.  A per-particle count replaces the placeholder
-// terminator, and no memset of the list is needed.
-#include "kernels.hpp"
+// cell ascending particle index.
+//
+// List layout (blocked by wavefront, 16 entries per lane and block): entry k
+// of particle i sits at (i / 64) * 64 * cap + (k / 16) * 1024 + (i % 64) * 16 +
+// k % 16.  Each lane stages its next 16 entries in LDS and stores them as one
+// 64-byte block (one whole L2 line), so the HBM write is whole lines rather
+// than 4-byte stores scattered over many partially written lines; a wave
+// reading one block reads 4 KiB contiguously.  A per-particle count replaces
+// the placeholder terminator, and no memset of the list is needed.
+#include "segment.hpp"
 
 namespace halmd {
 namespace {
@@ -70,8 +74,22 @@ __global__ void gather_cell_pos_kernel(uint4 const* __restrict__ posq, uint32_t 
 struct nb_state {
     uint4 ri;
     unsigned i, ti, c, cap;
-    uint32_t* out;
+    uint32_t* out;      // first block of this particle: wave base + lane * NB_BLOCK
+    uint32_t* buf;      // this thread's NB_BLOCK staging words in LDS
 };
+
+// LDS staging stride per thread: odd, so that the 64 lanes of a wave hit
+// distinct banks whatever their entry counts
+constexpr unsigned NB_STAGE = NB_BLOCK + 1;
+
+__device__ __forceinline__ void flush_block(nb_state& s, unsigned block) {
+    if ((block + 1) * NB_BLOCK <= s.cap) {
+        uint4* const dst = reinterpret_cast<uint4*>(s.out + size_t(block) * NB_BLOCK * WAVE);
+#pragma unroll
+        for (unsigned v = 0; v < NB_BLOCK / 4; ++v)
+            dst[v] = make_uint4(s.buf[4 * v], s.buf[4 * v + 1], s.buf[4 * v + 2], s.buf[4 * v + 3]);
+    }
+}
 
 __device__ __forceinline__ void test_candidate(nb_state& s, uint4 const rj, neighbour_params const& p,
                                                float const* s_rr) {
@@ -84,7 +102,8 @@ __device__ __forceinline__ void test_candidate(nb_state& s, uint4 const rj, neig
     rr = rr + dy * dy;
     rr = rr + dz * dz;
     if (j != s.i && rr <= s_rr[s.ti + (rj.w & 3u)]) {
-        if (s.c < s.cap) s.out[size_t(s.c) * WAVE] = j;
+        s.buf[s.c % NB_BLOCK] = j;
+        if (s.c % NB_BLOCK == NB_BLOCK - 1) flush_block(s, s.c / NB_BLOCK);
         ++s.c;
     }
 }
@@ -108,6 +127,7 @@ __global__ __launch_bounds__(256) void build_neighbours_kernel(
     uint32_t* __restrict__ count, uint32_t* __restrict__ max_count)
 {
     __shared__ float s_rr[MAX_TYPES * MAX_TYPES];
+    __shared__ uint32_t s_buf[256 * NB_STAGE];
     if (threadIdx.x < MAX_TYPES * MAX_TYPES) s_rr[threadIdx.x] = p.rr_cut_skin[threadIdx.x];
     __syncthreads();
 
@@ -119,7 +139,8 @@ __global__ __launch_bounds__(256) void build_neighbours_kernel(
         s.i = i;
         s.ti = (s.ri.w & 3u) * p.ntype;
         s.cap = cap;
-        s.out = list + size_t(i & ~(WAVE - 1)) * cap + (i & (WAVE - 1));
+        s.out = list + size_t(i & ~(WAVE - 1)) * cap + (i & (WAVE - 1)) * NB_BLOCK;
+        s.buf = s_buf + threadIdx.x * NB_STAGE;
         unsigned const nx = p.grid.ncell[0], ny = p.grid.ncell[1], nz = p.grid.ncell[2];
         unsigned const cx = cell_component(s.ri.x, nx);
         unsigned const cy = cell_component(s.ri.y, ny);
@@ -142,12 +163,135 @@ __global__ __launch_bounds__(256) void build_neighbours_kernel(
                 }
             }
         }
+        if (s.c % NB_BLOCK) flush_block(s, s.c / NB_BLOCK);   // last, partial block
         count[i] = s.c;
     }
     // largest count of the wave -> one atomic per wave
     unsigned m = s.c;
     for (int off = WAVE / 2; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off, WAVE));
     if ((threadIdx.x % WAVE) == 0 && m > 0) atomicMax(max_count, m);
+}
+
+// --- cell segments: 16-bit slots into the LDS-staged neighbourhood union ----
+
+constexpr unsigned NB16_STAGE = NB16_BLOCK + 2;   // staging halfwords per thread (odd word count)
+
+struct nb16_state {
+    uint4 ri;
+    unsigned own, ti, c, cap;
+    uint4* out;          // first block of this particle
+    uint16_t* buf;       // this thread's staging slots in LDS (4-byte aligned)
+};
+
+// one lane block (NB16_BLOCK slots = 32 bytes) from the LDS staging slots
+__device__ __forceinline__ void flush16(nb16_state& s, unsigned block) {
+    if ((block + 1) * NB16_BLOCK <= s.cap) {
+        uint4* const dst = s.out + size_t(block) * (WAVE * NB16_BLOCK * 2 / 16);
+        uint32_t const* const w = reinterpret_cast<uint32_t const*>(s.buf);
+#pragma unroll
+        for (unsigned v = 0; v < NB16_BLOCK / 8; ++v) dst[v] = make_uint4(w[4 * v], w[4 * v + 1], w[4 * v + 2], w[4 * v + 3]);
+    }
+}
+
+__device__ __forceinline__ void append16(nb16_state& s, unsigned slot) {
+    unsigned const e = s.c % NB16_BLOCK;
+    s.buf[e] = uint16_t(slot);
+    if (e == NB16_BLOCK - 1) flush16(s, s.c / NB16_BLOCK);
+    ++s.c;
+}
+
+// the list entry is the record's byte offset in the staged union
+template <bool MULTI, bool ISO, bool SELF>
+__device__ __forceinline__ void test16(nb16_state& s, uint4 const rj, unsigned slot, neighbour_params const& p,
+                                       float const* s_rr, float rr_cut0) {
+#pragma clang fp contract(off)
+    bool inside;
+    if (ISO) {
+        float const dx = float(int32_t(s.ri.x - rj.x));
+        float const dy = float(int32_t(s.ri.y - rj.y));
+        float const dz = float(int32_t(s.ri.z - rj.z));
+        float rq = dx * dx;
+        rq = __builtin_fmaf(dy, dy, rq);
+        rq = __builtin_fmaf(dz, dz, rq);
+        inside = rq <= (MULTI ? s_rr[s.ti + (rj.w & 3u)] : rr_cut0);
+    } else {
+        float const dx = float(int32_t(s.ri.x - rj.x)) * p.q[0];
+        float const dy = float(int32_t(s.ri.y - rj.y)) * p.q[1];
+        float const dz = float(int32_t(s.ri.z - rj.z)) * p.q[2];
+        float rr = dx * dx;
+        rr = rr + dy * dy;
+        rr = rr + dz * dz;
+        inside = rr <= (MULTI ? s_rr[s.ti + (rj.w & 3u)] : rr_cut0);
+    }
+    if ((!SELF || slot != s.own) && inside) append16(s, slot * 16u);
+}
+
+// candidates [t, e) of one row, four records in flight
+template <bool MULTI, bool ISO, bool SELF>
+__device__ __forceinline__ void scan16(nb16_state& s, unsigned t, unsigned const e, uint4 const* s_rec,
+                                       neighbour_params const& p, float const* s_rr, float rr_cut0) {
+    for (; t + 4 <= e; t += 4) {
+        uint4 const r0 = s_rec[t], r1 = s_rec[t + 1], r2 = s_rec[t + 2], r3 = s_rec[t + 3];
+        test16<MULTI, ISO, SELF>(s, r0, t, p, s_rr, rr_cut0);
+        test16<MULTI, ISO, SELF>(s, r1, t + 1, p, s_rr, rr_cut0);
+        test16<MULTI, ISO, SELF>(s, r2, t + 2, p, s_rr, rr_cut0);
+        test16<MULTI, ISO, SELF>(s, r3, t + 3, p, s_rr, rr_cut0);
+    }
+    for (; t < e; ++t) test16<MULTI, ISO, SELF>(s, s_rec[t], t, p, s_rr, rr_cut0);
+}
+
+template <bool MULTI, bool ISO>
+__global__ __launch_bounds__(SEG_THREADS) void build_staged_kernel(
+    uint4 const* __restrict__ posq, uint32_t const* __restrict__ off, seg_params sp, neighbour_params p,
+    unsigned cap, uint16_t* __restrict__ list, uint32_t* __restrict__ count, uint32_t* __restrict__ stats)
+{
+    extern __shared__ uint4 s_rec[];
+    __shared__ seg_stage st;
+    __shared__ unsigned s_bound[9 * (SEG_MAX + 3)];
+    __shared__ float s_rr[MAX_TYPES * MAX_TYPES];
+    __shared__ uint16_t s_buf[SEG_THREADS * NB16_STAGE];
+
+    unsigned const seg = seg_of_block(sp, blockIdx.x);
+    if (seg >= sp.nseg) return;   // the whole workgroup
+    if (threadIdx.x < MAX_TYPES * MAX_TYPES)
+        s_rr[threadIdx.x] = ISO ? p.rr_cut_skin_q[threadIdx.x] : p.rr_cut_skin[threadIdx.x];
+    unsigned const U = stage_union(sp, seg, off, posq, s_rec, st, s_bound);
+    if (threadIdx.x == 0) atomicMax(&stats[1], U + 1);
+    if (U + 1 > sp.lds_cap) return;   // uniform: the caller enlarges the staging and rebuilds
+
+    unsigned const nx = p.grid.ncell[0];
+    float const rr_cut0 = ISO ? p.rr_cut_skin_q[0] : p.rr_cut_skin[0];
+    unsigned cmax = 0;
+    for (unsigned i = st.pb + threadIdx.x; i < st.pe; i += SEG_THREADS) {
+        nb16_state s;
+        s.own = s_bound[4 * (SEG_MAX + 3) + 1] + (i - st.pb);
+        s.ri = s_rec[s.own];
+        s.ti = (s.ri.w & 3u) * p.ntype;
+        s.c = 0;
+        s.cap = cap;
+        s.out = reinterpret_cast<uint4*>(list + size_t(i & ~(WAVE - 1)) * cap + (i & (WAVE - 1)) * NB16_BLOCK);
+        s.buf = s_buf + threadIdx.x * NB16_STAGE;
+        unsigned const k = cell_component(s.ri.x, nx) - st.x0;   // 0 .. w - 1
+        // rows 0 .. 8; only row 4 (dz = dy = 0) holds the particle itself
+        for (unsigned r = 0; r < 9; ++r) {
+            unsigned const t = s_bound[r * (SEG_MAX + 3) + k];
+            unsigned const e = s_bound[r * (SEG_MAX + 3) + k + 3];
+            if (r == 4)
+                scan16<MULTI, ISO, true>(s, t, e, s_rec, p, s_rr, rr_cut0);
+            else
+                scan16<MULTI, ISO, false>(s, t, e, s_rec, p, s_rr, rr_cut0);
+        }
+        unsigned const c = s.c;
+        if (c <= cap) {
+            // pad with the sentinel to a multiple of 8 (cap is a multiple of 16)
+            while (s.c % 8) append16(s, U * 16u);
+            if (s.c % NB16_BLOCK) flush16(s, s.c / NB16_BLOCK);   // last, partial block
+        }
+        count[i] = c;
+        cmax = max(cmax, c);
+    }
+    for (int o = WAVE / 2; o > 0; o >>= 1) cmax = max(cmax, (unsigned)__shfl_xor((int)cmax, o, WAVE));
+    if ((threadIdx.x % WAVE) == 0 && cmax > 0) atomicMax(&stats[0], cmax);
 }
 
 } // namespace
@@ -175,6 +319,31 @@ void launch_build_neighbours(uint4 const* posq, unsigned n, uint32_t const* offs
     if (!n) return;
     build_neighbours_kernel<<<div_up(n, 256), 256, 0, s>>>(posq, n, offset, cell_pos, p, cap, list, count,
                                                            max_count);
+    HIP_CHECK(hipGetLastError());
+}
+
+} // namespace halmd
+
+namespace halmd {
+
+void launch_build_neighbours_staged(uint4 const* posq, uint32_t const* offset, seg_params const& sp,
+                                    neighbour_params const& p, unsigned cap, uint16_t* list, uint32_t* count,
+                                    uint32_t* stats, hipStream_t s) {
+    if (!sp.nseg) return;
+    size_t const lds = size_t(sp.lds_cap) * sizeof(uint4);
+    unsigned const grid = SEG_XCD * sp.chunk;
+    if (p.ntype > 1) {
+        if (p.iso)
+            build_staged_kernel<true, true><<<grid, SEG_THREADS, lds, s>>>(posq, offset, sp, p, cap, list, count, stats);
+        else
+            build_staged_kernel<true, false><<<grid, SEG_THREADS, lds, s>>>(posq, offset, sp, p, cap, list, count, stats);
+    } else {
+        if (p.iso)
+            build_staged_kernel<false, true><<<grid, SEG_THREADS, lds, s>>>(posq, offset, sp, p, cap, list, count, stats);
+        else
+            build_staged_kernel<false, false><<<grid, SEG_THREADS, lds, s>>>(posq, offset, sp, p, cap, list, count,
+                                                                          stats);
+    }
     HIP_CHECK(hipGetLastError());
 }
 

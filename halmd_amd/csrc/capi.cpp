@@ -338,6 +338,12 @@ HALMD_API int halmd_mdsim_neighbours_from_binning_counts(halmd_neighbour* hnb, u
 HALMD_API int halmd_mdsim_neighbours_from_binning_size(const halmd_neighbour* nb, unsigned* size) {
     return guard([&] { *size = as<neighbour>(nb)->cap; });
 }
+HALMD_API int halmd_mdsim_neighbours_from_binning_entry_bytes(const halmd_neighbour* nb, unsigned* bytes) {
+    return guard([&] {
+        nonnull(bytes, "bytes");
+        *bytes = as<neighbour>(nb)->staged ? 2u : 4u;
+    });
+}
 HALMD_API int halmd_mdsim_neighbours_from_binning_rebuilds(const halmd_neighbour* nb, uint64_t* count) {
     return guard([&] { *count = as<neighbour>(nb)->rebuilds; });
 }

@@ -4,8 +4,8 @@
 // with the potentials lennard_jones / morse (potentials/pair/*_kernel.hpp) and
 // the truncations sharp / shifted / force_shifted / smooth_r4
 // (potentials/pair/truncations/*_kernel.cuh).  One thread per particle walks
-// its neighbour list (wave-blocked column-major, 256 contiguous bytes per wave
-// and entry) and gathers the 16-byte fixed-point record of each neighbour.
+// its neighbour list (blocks of 16 entries per lane, contiguous over the wave)
+// and gathers the 16-byte fixed-point record of each neighbour.
 //
 // Precision: the separation is the exact integer difference of the fixed-point
 // coordinates (minimum image by 32-bit wrap-around), so r_ij carries ~1e-7 of
@@ -13,10 +13,11 @@
 // reference's high-word difference has.  For the Morse potential, whose
 // exponent argument amplifies the error of r by r_min / sigma, r is formed as a
 // double word from the exact integers.  Pair potentials are FP32; the
-// per-particle force sum uses compensated (Kahan) summation, the counterpart of
-// the reference's dsfloat accumulator.  Energy and stress (aux) are FP32 per
+// per-particle force sum is compensated: plain FP32 over each group of 8 pair
+// forces, Kahan summation over the groups (the counterpart of the reference's
+// dsfloat accumulator).  Energy and stress (aux) are FP32 per
 // particle; the reductions over particles are FP64.
-#include "kernels.hpp"
+#include "segment.hpp"
 
 namespace halmd {
 namespace {
@@ -35,12 +36,34 @@ __device__ __forceinline__ float morse_argument(int32_t Dx, int32_t Dy, int32_t 
     return a.hi + a.lo;
 }
 
+// 1/x: hardware reciprocal (~1 ulp, not symmetric about the exact value) plus
+// one Newton step, which leaves it within ~0.5 ulp and unbiased: summed virials
+// over millions of pairs would otherwise carry the approximation's bias
+__device__ __forceinline__ float rcp_nr(float x) {
+    float const r = __builtin_amdgcn_rcpf(x);
+    float const e = __builtin_fmaf(-x, r, 1.0f);
+    return __builtin_fmaf(r, e, r);
+}
+
+// separation component from the exact integer difference: D * q with q =
+// L 2^-32 as a double word (q_hi alone is float(L) 2^-32, whose rounding would
+// scale every separation by the same factor and bias summed virials)
+__device__ __forceinline__ float separation(int32_t D, float q_hi, float q_lo) {
+    float const d = float(D);
+    return __builtin_fmaf(d, q_hi, d * q_lo);
+}
+
+// x * q for q = q_hi + q_lo
+__device__ __forceinline__ float separation_scale(float x, float q_hi, float q_lo) {
+    return __builtin_fmaf(x, q_hi, x * q_lo);
+}
+
 template <int KIND, int TRUNC>
 __device__ __forceinline__ void pair_eval(float rr, float morse_d, pair_param const& pp, float rri_smooth,
                                           float& fval, float& en) {
     if (KIND == POTENTIAL_LENNARD_JONES) {
         // rri = sigma^2 / rr; fval = 48 eps rri ri6 (ri6 - 1/2) / sigma^2; U = 4 eps ri6 (ri6 - 1)
-        float const rri = pp.p[2] * __builtin_amdgcn_rcpf(rr);
+        float const rri = pp.p[2] * rcp_nr(rr);
         float const ri6 = rri * rri * rri;
         fval = pp.p[0] * rri * ri6 * (ri6 - 0.5f);
         en = pp.p[1] * ri6 * (ri6 - 1.0f);
@@ -53,25 +76,25 @@ __device__ __forceinline__ void pair_eval(float rr, float morse_d, pair_param co
         float const e1 = expf(d);
         float const eA = (A == 1.0f) ? e1 : expf(A * d);
         float const ee = pp.p[0] * e1;
-        fval = (A + 1.0f) * ee * (eA - 1.0f) * rsb * __builtin_amdgcn_rcpf(rr);
+        fval = (A + 1.0f) * ee * (eA - 1.0f) * rsb * rcp_nr(rr);
         en = ee * (eA - A - 1.0f);
     }
     if (TRUNC == TRUNC_SHIFTED) {
         en -= pp.en_cut;
     } else if (TRUNC == TRUNC_FORCE_SHIFTED) {
-        float const r = __builtin_amdgcn_sqrtf(rr);
-        fval -= pp.force_cut * __builtin_amdgcn_rcpf(r);
+        float const r = sqrtf(rr);
+        fval -= pp.force_cut * rcp_nr(r);
         en = en - pp.en_cut + (r - pp.r_cut) * pp.force_cut;
     } else if (TRUNC == TRUNC_SMOOTH_R4) {
         en -= pp.en_cut;
-        float const r = __builtin_amdgcn_sqrtf(rr);
+        float const r = sqrtf(rr);
         float const dr = r - pp.r_cut;
         float const x2 = dr * dr * rri_smooth;
         float const x4 = x2 * x2;
-        float const x4i = __builtin_amdgcn_rcpf(1.0f + x4);
+        float const x4i = rcp_nr(1.0f + x4);
         float const h0 = x4 * x4i;
         float const h1 = 4.0f * dr * rri_smooth * x2 * x4i * x4i;
-        fval = h0 * fval - h1 * (en * __builtin_amdgcn_rcpf(r));
+        fval = h0 * fval - h1 * (en * rcp_nr(r));
         en = h0 * en;
     }
 }
@@ -100,25 +123,32 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
     uint4 const qi = posq[i];
     unsigned const ti = (qi.w & 3u) * P.ntype;
     unsigned const cnt = count[i];
-    uint32_t const* lp = list + size_t(i & ~(WAVE - 1)) * cap + (i & (WAVE - 1));
+    // group h of 8 entries (half of block h / 2) of this particle: two uint4 at
+    // lp + (h / 2) * 64 * NB_BLOCK / 4 + (h % 2) * 2
+    uint4 const* lp = reinterpret_cast<uint4 const*>(list + size_t(i & ~(WAVE - 1)) * cap +
+                                                     (i & (WAVE - 1)) * NB_BLOCK);
+    auto group = [&](unsigned h) { return lp + size_t(h / 2) * (WAVE * NB_BLOCK / 4) + (h % 2) * 2; };
 
+    // two-level compensated sum: plain FP32 over each group of 8 pair forces,
+    // then a Kahan step per group
     float fx = 0, fy = 0, fz = 0, cx = 0, cy = 0, cz = 0;
+    float bx = 0, by = 0, bz = 0;
     float en = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
 
     auto pair = [&](uint4 const qj) {
         int32_t const Dx = int32_t(qi.x - qj.x), Dy = int32_t(qi.y - qj.y), Dz = int32_t(qi.z - qj.z);
-        float const dx = float(Dx) * P.q[0];
-        float const dy = float(Dy) * P.q[1];
-        float const dz = float(Dz) * P.q[2];
+        float const dx = separation(Dx, P.q[0], P.q_lo[0]);
+        float const dy = separation(Dy, P.q[1], P.q_lo[1]);
+        float const dz = separation(Dz, P.q[2], P.q_lo[2]);
         float const rr = dx * dx + dy * dy + dz * dz;
         pair_param const& pp = s_pair[ti + (qj.w & 3u)];
         if (!(rr < pp.rr_cut)) return;
         float const md = KIND == POTENTIAL_MORSE ? morse_argument(Dx, Dy, Dz, P, pp) : 0.0f;
         float fval, u;
         pair_eval<KIND, TRUNC>(rr, md, pp, P.rri_smooth, fval, u);
-        kahan_add(fx, cx, fval * dx);
-        kahan_add(fy, cy, fval * dy);
-        kahan_add(fz, cz, fval * dz);
+        bx += fval * dx;
+        by += fval * dy;
+        bz += fval * dz;
         if (AUX) {
             en += u;
             s0 += fval * dx * dx;
@@ -129,20 +159,33 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
             s5 += fval * dy * dz;
         }
     };
+    auto fold = [&]() {
+        kahan_add(fx, cx, bx);
+        kahan_add(fy, cy, by);
+        kahan_add(fz, cz, bz);
+        bx = by = bz = 0;
+    };
 
-    // four neighbours in flight per iteration: index loads first, then the
+    // eight neighbours in flight per group: two 16-byte index loads, eight
     // 16-byte gathers, then the arithmetic (summation order stays k = 0, 1, ...)
-    unsigned k = 0;
-    for (; k + 4 <= cnt; k += 4) {
-        unsigned const j0 = lp[size_t(k) * WAVE], j1 = lp[size_t(k + 1) * WAVE];
-        unsigned const j2 = lp[size_t(k + 2) * WAVE], j3 = lp[size_t(k + 3) * WAVE];
-        uint4 const q0 = posq[j0], q1 = posq[j1], q2 = posq[j2], q3 = posq[j3];
-        pair(q0);
-        pair(q1);
-        pair(q2);
-        pair(q3);
+    unsigned const full = cnt / 8;
+    for (unsigned h = 0; h < full; ++h) {
+        uint4 const* const g = group(h);
+        uint4 const a = g[0], c = g[1];
+        uint4 const q0 = posq[a.x], q1 = posq[a.y], q2 = posq[a.z], q3 = posq[a.w];
+        uint4 const q4 = posq[c.x], q5 = posq[c.y], q6 = posq[c.z], q7 = posq[c.w];
+        pair(q0); pair(q1); pair(q2); pair(q3);
+        pair(q4); pair(q5); pair(q6); pair(q7);
+        fold();
     }
-    for (; k < cnt; ++k) pair(posq[lp[size_t(k) * WAVE]]);
+    unsigned const rest = cnt % 8;
+    if (rest) {
+        uint4 const* const g = group(full);
+        uint4 const a = g[0], c = g[1];
+        unsigned const js[8] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w};
+        for (unsigned k = 0; k < rest; ++k) pair(posq[js[k]]);
+        fold();
+    }
     float4 f = make_float4(fx - cx, fy - cy, fz - cz, AUX ? aux_weight * en : 0.0f);
     float st[6] = {aux_weight * s0, aux_weight * s1, aux_weight * s2, aux_weight * s3, aux_weight * s4,
                    aux_weight * s5};
@@ -159,6 +202,144 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
     if (AUX) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) stress[c * size_t(stride) + i] = st[c];
+    }
+}
+
+// Cell-segment variant: the workgroup stages its segment's neighbourhood union
+// (current fixed-point positions) in LDS, then every lane walks the 16-bit slot
+// list of one particle of the segment, padded to a multiple of 8 with a
+// sentinel record beyond every cutoff (no per-entry bounds test).
+//  - ISO (cubic box): separations stay in fixed-point units, d = float(D);
+//    rr = (d.d) q^2 with q^2 a double word, and the force sum of fval * d is
+//    scaled by q (double word) once per particle; the same quantities as the
+//    general path, with unbiased roundings and fewer operations.
+//  - MULTI: pair parameters by species from LDS; otherwise the one parameter
+//    set lives in scalar registers.
+// Summation: FP32 over blocks of 16 pair forces, Kahan over the blocks.
+template <int KIND, int TRUNC, bool AUX, bool ISO, bool MULTI>
+__global__ __launch_bounds__(SEG_THREADS) void pair_force_staged_kernel(
+    uint4 const* __restrict__ posq, uint32_t const* __restrict__ off, seg_params sp,
+    uint16_t const* __restrict__ list, uint32_t const* __restrict__ count, unsigned cap, force_params P,
+    bool force_zero, float aux_weight, float4* __restrict__ force, float* __restrict__ stress, unsigned stride)
+{
+    extern __shared__ uint4 s_rec[];
+    __shared__ seg_stage st;
+    __shared__ pair_param s_pair[MAX_TYPES * MAX_TYPES];
+
+    unsigned const seg = seg_of_block(sp, blockIdx.x);
+    if (seg >= sp.nseg) return;   // the whole workgroup
+    if (MULTI && threadIdx.x < MAX_TYPES * MAX_TYPES) s_pair[threadIdx.x] = P.pair[threadIdx.x];
+    stage_union(sp, seg, off, posq, s_rec, st, nullptr);   // fits: the list build checked it
+
+    for (unsigned i = st.pb + threadIdx.x; i < st.pe; i += SEG_THREADS) {
+        uint4 const qi = posq[i];
+        unsigned const ti = (qi.w & 3u) * P.ntype;
+        unsigned const n8 = (count[i] + 7u) & ~7u;
+        uint4 const* const lp =
+            reinterpret_cast<uint4 const*>(list + size_t(i & ~(WAVE - 1)) * cap + (i & (WAVE - 1)) * NB16_BLOCK);
+
+        float fx = 0, fy = 0, fz = 0, cx = 0, cy = 0, cz = 0;
+        float bx = 0, by = 0, bz = 0;
+        float en = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
+
+        // branch-free: most candidates lie inside the cutoff, so the wave
+        // evaluates every pair and a select drops those beyond it
+        auto pair = [&](uint4 const qj) {
+            int32_t const Dx = int32_t(qi.x - qj.x), Dy = int32_t(qi.y - qj.y), Dz = int32_t(qi.z - qj.z);
+            pair_param const& pp = MULTI ? s_pair[ti + (qj.w & 3u)] : P.pair[0];
+            float dx, dy, dz, rr;
+            bool inside;
+            if (ISO) {
+                dx = float(Dx);
+                dy = float(Dy);
+                dz = float(Dz);
+                float const rq = dx * dx + dy * dy + dz * dz;
+                inside = rq < pp.rr_cut_q;
+                rr = __builtin_fmaf(rq, P.q2[0], rq * P.q2[1]);
+            } else {
+                dx = separation(Dx, P.q[0], P.q_lo[0]);
+                dy = separation(Dy, P.q[1], P.q_lo[1]);
+                dz = separation(Dz, P.q[2], P.q_lo[2]);
+                rr = dx * dx + dy * dy + dz * dz;
+                inside = rr < pp.rr_cut;
+            }
+            float const md = KIND == POTENTIAL_MORSE ? morse_argument(Dx, Dy, Dz, P, pp) : 0.0f;
+            float fval, u;
+            pair_eval<KIND, TRUNC>(rr, md, pp, P.rri_smooth, fval, u);
+            fval = inside ? fval : 0.0f;
+            bx += fval * dx;
+            by += fval * dy;
+            bz += fval * dz;
+            if (AUX) {
+                en += inside ? u : 0.0f;
+                s0 += fval * dx * dx;
+                s1 += fval * dy * dy;
+                s2 += fval * dz * dz;
+                s3 += fval * dx * dy;
+                s4 += fval * dx * dz;
+                s5 += fval * dy * dz;
+            }
+        };
+        // list entries are byte offsets of records in the staged union
+        char const* const lds = reinterpret_cast<char const*>(s_rec);
+        auto rec = [&](unsigned byte_off) { return *reinterpret_cast<uint4 const*>(lds + byte_off); };
+        auto group = [&](uint4 const w) {
+            uint4 const q0 = rec(w.x & 0xffffu), q1 = rec(w.x >> 16), q2 = rec(w.y & 0xffffu), q3 = rec(w.y >> 16),
+                        q4 = rec(w.z & 0xffffu), q5 = rec(w.z >> 16), q6 = rec(w.w & 0xffffu), q7 = rec(w.w >> 16);
+            pair(q0); pair(q1); pair(q2); pair(q3);
+            pair(q4); pair(q5); pair(q6); pair(q7);
+        };
+        auto fold = [&]() {
+            kahan_add(fx, cx, bx);
+            kahan_add(fy, cy, by);
+            kahan_add(fz, cz, bz);
+            bx = by = bz = 0;
+        };
+        // blocks of 16 slots (two uint4 = two groups); the next block is
+        // loaded while the current one is evaluated
+        constexpr size_t BLOCK_STRIDE = WAVE * NB16_BLOCK * 2 / 16;   // uint4 between blocks of a lane
+        // (unconditional loads, the block index clamped into this particle's
+        // cap / 16 blocks, so that the wait for block b does not also wait
+        // for the prefetch of block b + 1)
+        unsigned const last = cap / NB16_BLOCK - 1;
+        uint4 w0 = lp[0], w1 = lp[1];
+        for (unsigned b = 0; b * NB16_BLOCK < n8; ++b) {
+            unsigned const bn = min(b + 1, last);
+            uint4 const n0 = lp[bn * BLOCK_STRIDE], n1 = lp[bn * BLOCK_STRIDE + 1];
+            group(w0);
+            if (n8 - b * NB16_BLOCK > 8) group(w1);
+            fold();
+            w0 = n0;
+            w1 = n1;
+        }
+        float4 f;
+        float sc[6];
+        if (ISO) {
+            // back from fixed-point units: force * q, stress * q^2
+            f = make_float4(separation_scale(fx - cx, P.q[0], P.q_lo[0]), separation_scale(fy - cy, P.q[0], P.q_lo[0]),
+                            separation_scale(fz - cz, P.q[0], P.q_lo[0]), AUX ? aux_weight * en : 0.0f);
+            float const s2q = aux_weight * P.q2[0], s2l = aux_weight * P.q2[1];
+            float const sv[6] = {s0, s1, s2, s3, s4, s5};
+#pragma unroll
+            for (int c = 0; c < 6; ++c) sc[c] = __builtin_fmaf(sv[c], s2q, sv[c] * s2l);
+        } else {
+            f = make_float4(fx - cx, fy - cy, fz - cz, AUX ? aux_weight * en : 0.0f);
+            sc[0] = aux_weight * s0; sc[1] = aux_weight * s1; sc[2] = aux_weight * s2;
+            sc[3] = aux_weight * s3; sc[4] = aux_weight * s4; sc[5] = aux_weight * s5;
+        }
+        if (!force_zero) {
+            float4 const old = force[i];
+            f.x += old.x; f.y += old.y; f.z += old.z; f.w += old.w;
+            if (AUX) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) sc[c] += stress[c * size_t(stride) + i];
+            }
+        }
+        force[i] = f;
+        if (AUX) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) stress[c * size_t(stride) + i] = sc[c];
+        }
     }
 }
 
@@ -199,6 +380,53 @@ void dispatch_trunc(force_launch const& a) {
     }
 }
 
+struct staged_launch {
+    uint4 const* posq;
+    uint32_t const* off;
+    seg_params const& sp;
+    uint16_t const* list;
+    uint32_t const* count;
+    unsigned cap;
+    force_params const& p;
+    bool aux, force_zero;
+    float w;
+    float4* force;
+    float* stress;
+    unsigned stride;
+    hipStream_t s;
+};
+
+template <int KIND, int TRUNC, bool AUX, bool ISO, bool MULTI>
+void staged_go(staged_launch const& a) {
+    size_t const lds = size_t(a.sp.lds_cap) * sizeof(uint4);
+    unsigned const grid = SEG_XCD * a.sp.chunk;
+    pair_force_staged_kernel<KIND, TRUNC, AUX, ISO, MULTI><<<grid, SEG_THREADS, lds, a.s>>>(
+        a.posq, a.off, a.sp, a.list, a.count, a.cap, a.p, a.force_zero, a.w, a.force, a.stress, a.stride);
+}
+
+template <int KIND, int TRUNC>
+void staged_aux(staged_launch const& a) {
+    // energies and stresses are sampled rarely: one general variant for them
+    bool const multi = a.p.ntype > 1;
+    if (a.aux)
+        staged_go<KIND, TRUNC, true, false, true>(a);
+    else if (a.p.iso)
+        multi ? staged_go<KIND, TRUNC, false, true, true>(a) : staged_go<KIND, TRUNC, false, true, false>(a);
+    else
+        multi ? staged_go<KIND, TRUNC, false, false, true>(a) : staged_go<KIND, TRUNC, false, false, false>(a);
+}
+
+template <int KIND>
+void staged_trunc(staged_launch const& a) {
+    switch (a.p.trunc) {
+    case TRUNC_SHARP: staged_aux<KIND, TRUNC_SHARP>(a); break;
+    case TRUNC_SHIFTED: staged_aux<KIND, TRUNC_SHIFTED>(a); break;
+    case TRUNC_FORCE_SHIFTED: staged_aux<KIND, TRUNC_FORCE_SHIFTED>(a); break;
+    case TRUNC_SMOOTH_R4: staged_aux<KIND, TRUNC_SMOOTH_R4>(a); break;
+    default: throw error(HALMD_ERR_INVALID_ARGUMENT, "unknown truncation");
+    }
+}
+
 } // namespace
 
 void launch_pair_force(uint4 const* posq, unsigned n, uint32_t const* list, uint32_t const* count, unsigned cap,
@@ -210,6 +438,20 @@ void launch_pair_force(uint4 const* posq, unsigned n, uint32_t const* list, uint
         dispatch_trunc<POTENTIAL_LENNARD_JONES>(a);
     else if (p.kind == POTENTIAL_MORSE)
         dispatch_trunc<POTENTIAL_MORSE>(a);
+    else
+        throw error(HALMD_ERR_INVALID_ARGUMENT, "unknown potential");
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_pair_force_staged(uint4 const* posq, uint32_t const* offset, seg_params const& sp, uint16_t const* list,
+                              uint32_t const* count, unsigned cap, force_params const& p, bool aux, bool force_zero,
+                              float aux_weight, float4* force, float* stress, unsigned stride, hipStream_t s) {
+    if (!sp.nseg) return;
+    staged_launch const a{posq, offset, sp, list, count, cap, p, aux, force_zero, aux_weight, force, stress, stride, s};
+    if (p.kind == POTENTIAL_LENNARD_JONES)
+        staged_trunc<POTENTIAL_LENNARD_JONES>(a);
+    else if (p.kind == POTENTIAL_MORSE)
+        staged_trunc<POTENTIAL_MORSE>(a);
     else
         throw error(HALMD_ERR_INVALID_ARGUMENT, "unknown potential");
     HIP_CHECK(hipGetLastError());

@@ -64,12 +64,107 @@ struct neighbour_params {
     float q[3];         // float(L) * 2^-32: length of one fixed-point unit
     unsigned ntype;
     float rr_cut_skin[MAX_TYPES * MAX_TYPES];
+    // cubic box (q equal in all dimensions): predicate in fixed-point units,
+    // rq = fma(dz, dz, fma(dy, dy, dx * dx)) <= float(rr_cut_skin / q^2)
+    unsigned iso;
+    float rr_cut_skin_q[MAX_TYPES * MAX_TYPES];
 };
-// full neighbour lists; list layout [(i / 64) * 64 * cap + k * 64 + i % 64];
-// *max_count receives the largest per-particle count (may exceed cap)
+// full neighbour lists, blocked layout: entry k of particle i at
+// (i / 64) * 64 * cap + (k / NB_BLOCK) * 64 * NB_BLOCK + (i % 64) * NB_BLOCK + k % NB_BLOCK
+// (cap a multiple of NB_BLOCK); *max_count receives the largest per-particle
+// count (may exceed cap)
+constexpr unsigned NB_BLOCK = 16;
+__host__ __device__ inline size_t list_index(size_t i, size_t k, size_t cap) {
+    return (i & ~size_t(WAVE - 1)) * cap + (k / NB_BLOCK) * WAVE * NB_BLOCK + (i & (WAVE - 1)) * NB_BLOCK +
+           k % NB_BLOCK;
+}
 void launch_build_neighbours(uint4 const* posq, unsigned n, uint32_t const* offset, uint4 const* cell_pos,
                              neighbour_params const& p, unsigned cap, uint32_t* list, uint32_t* count,
                              uint32_t* max_count, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// cell segments (binning.hip, force.hip): particles stored in cell order, one
+// workgroup per segment of up to S consecutive cells along x in one (y, z)
+// row.  The union of the 27-cell neighbourhoods of a segment's cells is 9 rows
+// (dz, dy) of S' + 2 cells (x0 - 1 .. x0 + S', periodic), each one or two
+// contiguous ranges of particle memory; it is staged in LDS, rows in (dz, dy)
+// lexicographic order, cells in that x order.  Neighbour lists then hold
+// 16-bit LDS slots of that union.
+// ---------------------------------------------------------------------------
+constexpr unsigned SEG_MAX = 32;          // largest S
+constexpr unsigned SEG_THREADS = 256;     // workgroup size of segment kernels
+constexpr unsigned SEG_XCD = 8;           // workgroups are dealt round-robin to 8 XCDs
+struct seg_params {
+    cell_grid grid;
+    unsigned S;           // cells per segment, 1 <= S <= min(SEG_MAX, nx - 2)
+    unsigned nsx;         // segments per row, ceil(nx / S)
+    unsigned nseg;        // nsx * ny * nz
+    unsigned chunk;       // ceil(nseg / SEG_XCD): consecutive segments per XCD
+    unsigned lds_cap;     // LDS records per workgroup (>= every union size)
+};
+// workgroup b serves segment (b % 8) * chunk + b / 8, so that each XCD walks a
+// contiguous run of segments and re-reads shared halo rows from its own L2
+__host__ __device__ inline unsigned seg_of_block(seg_params const& sp, unsigned b) {
+    return (b % SEG_XCD) * sp.chunk + b / SEG_XCD;
+}
+struct seg_rows {
+    unsigned x0, sp;              // first cell (x) and number of cells of the segment
+    unsigned pb, pe;              // particle range of the segment
+    unsigned g[9][2], n[9][2];    // per row: up to two memory ranges (begin, length)
+};
+// geometry of segment `seg` from the cell offsets (device or host memory)
+__host__ __device__ inline void seg_row(seg_params const& sp, unsigned seg, unsigned r, uint32_t const* off,
+                                        unsigned* g, unsigned* len) {
+    unsigned const nx = sp.grid.ncell[0], ny = sp.grid.ncell[1], nz = sp.grid.ncell[2];
+    unsigned const sx = seg % sp.nsx, t = seg / sp.nsx;
+    unsigned const y = t % ny, z = t / ny;
+    unsigned const x0 = sx * sp.S;
+    unsigned const w = x0 + sp.S <= nx ? sp.S : nx - x0;
+    unsigned const kz = (z + nz + r / 3 - 1) % nz, ky = (y + ny + r % 3 - 1) % ny;
+    unsigned const row = (kz * ny + ky) * nx;
+    if (x0 > 0 && x0 + w < nx) {
+        g[0] = off[row + x0 - 1]; len[0] = off[row + x0 + w + 1] - g[0];
+        g[1] = 0; len[1] = 0;
+    } else if (x0 == 0) {
+        g[0] = off[row + nx - 1]; len[0] = off[row + nx] - g[0];
+        g[1] = off[row]; len[1] = off[row + w + 1] - g[1];
+    } else {
+        g[0] = off[row + x0 - 1]; len[0] = off[row + nx] - g[0];
+        g[1] = off[row]; len[1] = off[row + 1] - g[1];
+    }
+}
+__host__ __device__ inline void seg_particles(seg_params const& sp, unsigned seg, uint32_t const* off,
+                                              unsigned& x0, unsigned& w, unsigned& pb, unsigned& pe) {
+    unsigned const nx = sp.grid.ncell[0], ny = sp.grid.ncell[1];
+    unsigned const sx = seg % sp.nsx, t = seg / sp.nsx;
+    unsigned const y = t % ny, z = t / ny;
+    x0 = sx * sp.S;
+    w = x0 + sp.S <= nx ? sp.S : nx - x0;
+    unsigned const row = (z * ny + y) * nx;
+    pb = off[row + x0];
+    pe = off[row + x0 + w];
+}
+// 16-bit slot lists (an entry is the byte offset 16 * slot of the record in
+// the staged union, so a union holds at most 4096 records): entry k of
+// particle i (memory order) at
+// (i / 64) * 64 * cap + (k / 16) * 1024 + (i % 64) * 16 + k % 16 (uint16 units;
+// one lane block = 16 entries = 32 bytes, the write granule of the memory
+// system; cap a multiple of 16)
+constexpr unsigned NB16_BLOCK = 16;
+constexpr unsigned SEG_MAX_RECORDS = 4096;   // staged union plus sentinel
+__host__ __device__ inline size_t list16_index(size_t i, size_t k, size_t cap) {
+    return (i & ~size_t(WAVE - 1)) * cap + (k / NB16_BLOCK) * WAVE * NB16_BLOCK + (i & (WAVE - 1)) * NB16_BLOCK +
+           k % NB16_BLOCK;
+}
+// Lists are padded with the slot U of a sentinel record (staged after the
+// union, at least a third of the box away from every particle of the segment
+// in y and z) to a multiple of 8 entries; count[i] excludes the padding.
+// stats[0]: largest count (may exceed cap); stats[1]: largest staging size
+// U + 1 (a segment whose staging exceeds sp.lds_cap builds nothing; the caller
+// retries)
+void launch_build_neighbours_staged(uint4 const* posq, uint32_t const* offset, seg_params const& sp,
+                                    neighbour_params const& p, unsigned cap, uint16_t* list, uint32_t* count,
+                                    uint32_t* stats, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // pair forces (force.hip)
@@ -82,18 +177,26 @@ struct pair_param {
     float p[4];        // LJ: {48 eps / sigma^2, 4 eps, sigma^2, -}; Morse: {eps/A, 1/(sigma B), r_min_sigma/B, A}
     float p_lo[4];     // Morse: low words of p[1], p[2] (double-word constants of the exponent)
     float rr_cut, r_cut, en_cut, force_cut;
+    float rr_cut_q;    // rr_cut in fixed-point units squared, rr_cut / q^2 (cubic boxes)
 };
 struct force_params {
     int kind, trunc;
     unsigned ntype;
     float rri_smooth;
     float q[3], q_lo[3];   // L * 2^-32 as a double word
+    unsigned iso;          // cubic box: q the same in all dimensions
+    float q2[2];           // q^2 as a double word (cubic boxes)
     pair_param pair[MAX_TYPES * MAX_TYPES];
 };
 // force_zero: overwrite force/en_pot/stress, else add to the stored values
 void launch_pair_force(uint4 const* posq, unsigned n, uint32_t const* list, uint32_t const* count, unsigned cap,
                        force_params const& p, bool aux, bool force_zero, float aux_weight, float4* force,
                        float* stress, unsigned stride, hipStream_t s);
+
+// the same over 16-bit slot lists of cell segments (particles in cell order)
+void launch_pair_force_staged(uint4 const* posq, uint32_t const* offset, seg_params const& sp, uint16_t const* list,
+                              uint32_t const* count, unsigned cap, force_params const& p, bool aux, bool force_zero,
+                              float aux_weight, float4* force, float* stress, unsigned stride, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // integrators (integrate.hip)

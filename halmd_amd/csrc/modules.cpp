@@ -265,6 +265,9 @@ void particle::update_force(bool need_aux) {
         HIP_CHECK(hipMemsetAsync(force, 0, force.bytes(), stream));
         if (aux_enabled) HIP_CHECK(hipMemsetAsync(stress, 0, stress.bytes(), stream));
     } else {
+        // every neighbour list first (a rebuild may reorder the particles),
+        // then the forces
+        for (force_module* f : forces) f->prepare();
         bool first = true;
         for (force_module* f : forces) {
             f->apply(first);
@@ -397,6 +400,9 @@ force_params potential::device_params(box const& b) const {
     P.ntype = ntype2;
     P.rri_smooth = trunc == TRUNC_SMOOTH_R4 ? float(1 / (h * h)) : 0.0f;
     for (int k = 0; k < 3; ++k) { P.q[k] = b.q_hi[k]; P.q_lo[k] = b.q_lo[k]; }
+    P.iso = b.L[0] == b.L[1] && b.L[1] == b.L[2];
+    double const q = std::ldexp(b.L[0], -32);
+    split_ff(q * q, P.q2[0], P.q2[1]);
     for (unsigned a = 0; a < ntype1; ++a) {
         for (unsigned c = 0; c < ntype2; ++c) {
             unsigned const ab = a * ntype2 + c;
@@ -419,6 +425,7 @@ force_params potential::device_params(box const& b) const {
             pp.r_cut = float(r_cut[ab]);
             pp.en_cut = float(en_cut[ab]);
             pp.force_cut = float(force_cut[ab]);
+            pp.rr_cut_q = float(rr_cut[ab] / (q * q));
         }
     }
     return P;
@@ -447,17 +454,32 @@ binning::binning(particle& p_, box const& b_, double const* r_cut, unsigned nt1,
     ws.reserve(p.n);
 }
 
-void binning::update() {
-    if (version == p.position_version) return;
+void binning::update(bool need_cell_pos) {
+    hipStream_t s = p.stream;
+    if (version != p.position_version) {
+        timer.start(s);
+        launch_compute_cell(p.posq, p.n, grid, keys, vals, s);
+        unsigned key_bits = 0;
+        while ((1ull << key_bits) < ncells) ++key_bits;
+        radix_sort(keys, vals, p.n, key_bits, ws, s);
+        launch_cell_offsets(keys, p.n, ncells, offset, s);
+        timer.stop(s);
+        version = p.position_version;
+    }
+    if (need_cell_pos && cell_pos_version != version) {
+        launch_gather_cell_pos(p.posq, vals, p.n, cell_pos, s);
+        cell_pos_version = version;
+    }
+}
+
+void binning::reorder_particles() {
+    update(false);
     hipStream_t s = p.stream;
     timer.start(s);
-    launch_compute_cell(p.posq, p.n, grid, keys, vals, s);
-    unsigned key_bits = 0;
-    while ((1ull << key_bits) < ncells) ++key_bits;
-    radix_sort(keys, vals, p.n, key_bits, ws, s);
-    launch_cell_offsets(keys, p.n, ncells, offset, s);
-    launch_gather_cell_pos(p.posq, vals, p.n, cell_pos, s);
+    p.rearrange(vals);
+    launch_iota(vals, p.n, s);        // sorted order is now memory order
     timer.stop(s);
+    // keys and offsets describe the new order as well
     version = p.position_version;
 }
 
@@ -538,15 +560,55 @@ neighbour::neighbour(particle& p_, binning& bin_, max_displacement& disp_, box c
         params.rr_cut_skin[i] = float(std::pow(r_cut[i] + r_skin, 2));
         rc_max = std::max(rc_max, r_cut[i]);
     }
+    params.iso = params.q[0] == params.q[1] && params.q[1] == params.q[2];
+    for (size_t i = 0; i < r_cut.size(); ++i)
+        params.rr_cut_skin_q[i] =
+            float(double(params.rr_cut_skin[i]) / (double(params.q[0]) * double(params.q[0])));
     // initial capacity from the mean number of neighbours within rc + skin at
     // the given occupancy (from_binning.cpp:96-119), rounded to 8; grown on demand
     double const rho = double(p.n) / b.volume();
     double const nmean = 4.0 / 3.0 * M_PI * std::pow(rc_max + r_skin, 3) * rho;
-    cap = unsigned(round_up(size_t(std::ceil(nmean / occupancy)) + 8, 8));
-    list.resize(size_t(round_up(p.n, WAVE)) * cap);
     count.resize(p.n);
-    d_maxcount.resize(1);
-    h_maxcount.resize(1);
+    d_maxcount.resize(2);
+    h_maxcount.resize(2);
+    // the first neighbour module of a particle instance keeps it in cell order
+    // and uses the cell-segment kernels; the rules need nx >= 3 (S <= nx - 2)
+    staged = p.order_owner == nullptr;
+    if (staged) {
+        p.order_owner = this;
+        cap = unsigned(round_up(size_t(std::ceil(nmean / occupancy)) + NB16_BLOCK, NB16_BLOCK));
+        list16.resize(size_t(round_up(p.n, WAVE)) * cap);
+        int dev = 0, lds = 0;
+        HIP_CHECK(hipGetDevice(&dev));
+        HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        // static LDS of the build kernel: staging words, cell bounds, segment table
+        size_t const static_lds = SEG_THREADS * (NB16_BLOCK / 2 + 1) * 4 + 9 * (SEG_MAX + 3) * 4 + 1024;
+        require(size_t(lds) > static_lds + 27 * 16, "neighbour: device LDS too small");
+        lds_limit = std::min<size_t>((size_t(lds) - static_lds) / sizeof(uint4), SEG_MAX_RECORDS);
+        // about SEG_TARGET particles per segment at the mean cell occupancy
+        double const occ = double(p.n) / bin.ncells;
+        constexpr double SEG_TARGET = 224;
+        unsigned S = unsigned(std::max(1.0, std::floor(SEG_TARGET / std::max(occ, 1e-9))));
+        plan_segments(S);
+        double const expect = 9.0 * (sp.S + 2) * occ;
+        sp.lds_cap = unsigned(std::min<double>(lds_limit, round_up(size_t(1.3 * expect) + 64, 64)));
+    } else {
+        cap = unsigned(round_up(size_t(std::ceil(nmean / occupancy)) + NB_BLOCK, NB_BLOCK));
+        list.resize(size_t(round_up(p.n, WAVE)) * cap);
+    }
+}
+
+neighbour::~neighbour() {
+    if (p.order_owner == this) p.order_owner = nullptr;
+}
+
+void neighbour::plan_segments(unsigned S) {
+    unsigned const nx = bin.grid.ncell[0];
+    sp.grid = bin.grid;
+    sp.S = std::max(1u, std::min({S, SEG_MAX, nx - 2}));
+    sp.nsx = unsigned(div_up(nx, sp.S));
+    sp.nseg = sp.nsx * bin.grid.ncell[1] * bin.grid.ncell[2];
+    sp.chunk = unsigned(div_up(sp.nseg, SEG_XCD));
 }
 
 void neighbour::require_current() {
@@ -561,33 +623,100 @@ void neighbour::require_current() {
 
 void neighbour::update() {
     hipStream_t s = p.stream;
-    bin.update();
+    if (!staged) {
+        bin.update(true);
+        while (true) {
+            launch_fill_u32(d_maxcount, 0u, 1, s);
+            timer.start(s);
+            launch_build_neighbours(p.posq, p.n, bin.offset, bin.cell_pos, params, cap, list, count, d_maxcount, s);
+            timer.stop(s);
+            HIP_CHECK(hipMemcpyAsync(h_maxcount.get(), d_maxcount.get(), 4, hipMemcpyDeviceToHost, s));
+            p.sync();
+            unsigned const m = h_maxcount[0];
+            if (m <= cap) break;
+            // overflow: grow to the observed maximum plus headroom and rebuild
+            cap = unsigned(round_up(size_t(m) + m / 8 + NB_BLOCK, NB_BLOCK));
+            list.resize(size_t(round_up(p.n, WAVE)) * cap);
+        }
+        ++rebuilds;
+        return;
+    }
+    // memory order := cell order, then the segment lists
+    bin.reorder_particles();
     while (true) {
-        launch_fill_u32(d_maxcount, 0u, 1, s);
+        launch_fill_u32(d_maxcount, 0u, 2, s);
         timer.start(s);
-        launch_build_neighbours(p.posq, p.n, bin.offset, bin.cell_pos, params, cap, list, count, d_maxcount, s);
+        launch_build_neighbours_staged(p.posq, bin.offset, sp, params, cap, list16, count, d_maxcount, s);
         timer.stop(s);
-        HIP_CHECK(hipMemcpyAsync(h_maxcount.get(), d_maxcount.get(), 4, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipMemcpyAsync(h_maxcount.get(), d_maxcount.get(), 8, hipMemcpyDeviceToHost, s));
         p.sync();
-        unsigned const m = h_maxcount[0];
-        if (m <= cap) break;
-        // overflow: grow to the observed maximum plus headroom and rebuild
-        cap = unsigned(round_up(size_t(m) + m / 8 + 8, 8));
-        list.resize(size_t(round_up(p.n, WAVE)) * cap);
+        unsigned const m = h_maxcount[0], u = h_maxcount[1];
+        if (u > sp.lds_cap) {
+            // a union outgrew the staging: enlarge it, else narrow the segments
+            if (u + u / 8 <= lds_limit) {
+                sp.lds_cap = unsigned(round_up(size_t(u) + u / 8, 64));
+            } else if (u <= lds_limit) {
+                sp.lds_cap = unsigned(lds_limit);
+            } else {
+                require(sp.S > 1, "neighbour: cell occupancy too high to stage 27 cells in LDS");
+                unsigned const S = std::max(1u, unsigned(sp.S * double(lds_limit) / u) - 1);
+                plan_segments(std::min(S, sp.S - 1));
+                sp.lds_cap = unsigned(lds_limit);
+            }
+            continue;
+        }
+        if (m > cap) {
+            cap = unsigned(round_up(size_t(m) + m / 8 + NB16_BLOCK, NB16_BLOCK));
+            list16.resize(size_t(round_up(p.n, WAVE)) * cap);
+            continue;
+        }
+        // the force kernel stages exactly the largest union
+        sp_force = sp;
+        sp_force.lds_cap = std::max(1u, u);
+        break;
     }
     ++rebuilds;
 }
 
 void neighbour::get_lists(unsigned* counts, unsigned* entries, unsigned stride) {
     require_current();
-    std::vector<uint32_t> h(size_t(round_up(p.n, WAVE)) * cap);
-    list.download(h.data(), h.size(), p.stream);
-    count.download(counts, p.n, p.stream);
+    hipStream_t s = p.stream;
+    count.download(counts, p.n, s);
+    if (!staged) {
+        std::vector<uint32_t> h(size_t(round_up(p.n, WAVE)) * cap);
+        list.download(h.data(), h.size(), s);
+        p.sync();
+        for (size_t i = 0; i < p.n; ++i) {
+            require(counts[i] <= stride, "neighbour: output stride smaller than a list");
+            for (size_t k = 0; k < counts[i]; ++k) entries[i * stride + k] = h[list_index(i, k, cap)];
+        }
+        return;
+    }
+    // slots -> particle indices through each segment's union
+    std::vector<uint16_t> h(size_t(round_up(p.n, WAVE)) * cap);
+    std::vector<uint32_t> off(size_t(bin.ncells) + 1);
+    list16.download(h.data(), h.size(), s);
+    bin.offset.download(off.data(), off.size(), s);
     p.sync();
-    for (size_t i = 0; i < p.n; ++i) {
-        require(counts[i] <= stride, "neighbour: output stride smaller than a list");
-        for (size_t k = 0; k < counts[i]; ++k)
-            entries[i * stride + k] = h[(i & ~size_t(WAVE - 1)) * cap + k * WAVE + (i & (WAVE - 1))];
+    std::vector<uint32_t> slot_to_index;
+    for (unsigned seg = 0; seg < sp.nseg; ++seg) {
+        slot_to_index.clear();
+        for (unsigned r = 0; r < 9; ++r) {
+            unsigned g[2], len[2];
+            seg_row(sp, seg, r, off.data(), g, len);
+            for (unsigned pc = 0; pc < 2; ++pc)
+                for (unsigned t = 0; t < len[pc]; ++t) slot_to_index.push_back(g[pc] + t);
+        }
+        unsigned x0, w, pb, pe;
+        seg_particles(sp, seg, off.data(), x0, w, pb, pe);
+        for (size_t i = pb; i < pe; ++i) {
+            require(counts[i] <= stride, "neighbour: output stride smaller than a list");
+            for (size_t k = 0; k < counts[i]; ++k) {
+                unsigned const slot = h[list16_index(i, k, cap)] / 16u;
+                require(slot < slot_to_index.size(), "neighbour: slot outside the segment union");
+                entries[i * stride + k] = slot_to_index[slot];
+            }
+        }
     }
 }
 
@@ -601,12 +730,16 @@ pair_trunc::pair_trunc(potential const& pot_, particle& p_, box const& b_, neigh
     require(pot.ntype1 == p.nspecies && pot.ntype2 == p.nspecies, "pair_trunc: potential shape != nspecies");
     params = pot.device_params(b);
     p.forces.push_back(this);
+    p.force_valid = p.aux_valid = false;
 }
 
 pair_trunc::~pair_trunc() {
     auto& v = p.forces;
     v.erase(std::remove(v.begin(), v.end(), static_cast<force_module*>(this)), v.end());
+    p.force_valid = p.aux_valid = false;
 }
+
+void pair_trunc::prepare() { nb.require_current(); }
 
 void pair_trunc::apply(bool force_zero) {
     nb.require_current();
@@ -614,8 +747,12 @@ void pair_trunc::apply(bool force_zero) {
     // a particle interacting with itself counts each pair twice (pair_trunc.hpp:291-294)
     float const w = float(aux_weight / 2);
     timer.start(p.stream);
-    launch_pair_force(p.posq, p.n, nb.list, nb.count, nb.cap, params, aux, force_zero, w, p.force, p.stress, p.n_pad,
-                      p.stream);
+    if (nb.staged)
+        launch_pair_force_staged(p.posq, nb.bin.offset, nb.sp_force, nb.list16, nb.count, nb.cap, params, aux,
+                                 force_zero, w, p.force, p.stress, p.n_pad, p.stream);
+    else
+        launch_pair_force(p.posq, p.n, nb.list, nb.count, nb.cap, params, aux, force_zero, w, p.force, p.stress,
+                          p.n_pad, p.stream);
     timer.stop(p.stream);
 }
 

@@ -87,6 +87,9 @@ public:
     uint64_t position_version = 1, velocity_version = 1, species_version = 1, order_version = 1;
     bool force_valid = false, aux_valid = false, aux_enabled = false;
     std::vector<force_module*> forces;
+    // the neighbour module that keeps the particles in its cell order (at
+    // most one per particle instance; others use global-index lists)
+    void const* order_owner = nullptr;
 
     void set_position(double const* r);
     void get_position(double* r);
@@ -158,7 +161,12 @@ public:
     dbuf<uint4> cell_pos;
     radix_sort_workspace ws;
     uint64_t version = 0;             // position version of the last update
-    void update();
+    uint64_t cell_pos_version = 0;    // position version cell_pos was gathered at
+    // cell keys, permutation and offsets for the current positions; the
+    // compact records cell_pos only if need_cell_pos
+    void update(bool need_cell_pos = true);
+    // rearrange the particles into cell order (stable); offsets stay valid
+    void reorder_particles();
     void cell_lists(unsigned* begin_out, unsigned* end_out, unsigned* index_out);
     kernel_timer timer;
 };
@@ -187,6 +195,7 @@ class neighbour {
 public:
     neighbour(particle& p, binning& bin, max_displacement& disp, box const& b, double const* r_cut, unsigned ntype1,
               unsigned ntype2, double skin, double occupancy);
+    ~neighbour();
     particle& p;
     binning& bin;
     max_displacement& disp;
@@ -197,6 +206,12 @@ public:
     unsigned cap;
     dbuf<uint32_t> list, count, d_maxcount;
     hbuf<uint32_t> h_maxcount;
+    // cell-segment lists (this module keeps the particles in cell order)
+    bool staged = false;
+    seg_params sp{}, sp_force{};      // build staging / force staging (lds_cap = largest union)
+    size_t lds_limit = 0;             // records of dynamic LDS a build workgroup may use
+    dbuf<uint16_t> list16;
+    void plan_segments(unsigned S);
     uint64_t built_order = 0;
     bool built = false;
     uint64_t rebuilds = 0;
@@ -211,6 +226,8 @@ public:
 class force_module {
 public:
     virtual ~force_module() = default;
+    // bring inputs up to date (neighbour lists; may reorder the particles)
+    virtual void prepare() {}
     virtual void apply(bool force_zero) = 0;
 };
 
@@ -224,6 +241,7 @@ public:
     neighbour& nb;
     double aux_weight;
     force_params params;
+    void prepare() override;
     void apply(bool force_zero) override;
     kernel_timer timer;
 };

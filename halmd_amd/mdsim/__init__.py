@@ -232,6 +232,13 @@ class neighbour(Handle):
         return v.value
 
     @property
+    def entry_bytes(self):
+        """Bytes per list entry: 2 (16-bit slots, cell-ordered particles) or 4 (indices)."""
+        v = ctypes.c_uint()
+        call("halmd_mdsim_neighbours_from_binning_entry_bytes", self.handle, ctypes.byref(v))
+        return v.value
+
+    @property
     def rebuilds(self):
         v = ctypes.c_uint64()
         call("halmd_mdsim_neighbours_from_binning_rebuilds", self.handle, ctypes.byref(v))

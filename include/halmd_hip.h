@@ -165,6 +165,11 @@ HALMD_API int halmd_mdsim_neighbours_from_binning_update(halmd_neighbour* nb);
 HALMD_API int halmd_mdsim_neighbours_from_binning_lists(halmd_neighbour* nb, unsigned* counts, unsigned* entries,
                                                         unsigned stride);
 HALMD_API int halmd_mdsim_neighbours_from_binning_size(const halmd_neighbour* nb, unsigned* size);
+/* storage of one list entry in bytes: 2 for the module that keeps its particles
+ * in cell order (16-bit slots into a staged cell neighbourhood), 4 otherwise
+ * (particle indices); no counterpart in the reference (its entries are 4-byte
+ * indices, from_binning.hpp:105) */
+HALMD_API int halmd_mdsim_neighbours_from_binning_entry_bytes(const halmd_neighbour* nb, unsigned* bytes);
 /* number of neighbours of each particle (memory order) */
 HALMD_API int halmd_mdsim_neighbours_from_binning_counts(halmd_neighbour* nb, unsigned* counts);
 HALMD_API int halmd_mdsim_neighbours_from_binning_rebuilds(const halmd_neighbour* nb, uint64_t* count);

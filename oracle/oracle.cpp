@@ -718,7 +718,9 @@ void oracle_fixed_cell_index(int n, uint32_t const* u, unsigned const* ncell, un
 // ascending particle index (the order of the stable cell sort).
 // Predicate (halmd/mdsim/gpu/neighbours/from_binning_kernel.cu:144-163 on the
 // fixed-point coordinates): d_k = float(int32(u_i,k - u_j,k)) * q_k (FP32),
-// rr = (dx*dx + dy*dy) + dz*dz, rr <= rr_cut_skin[a][b].
+// rr = (dx*dx + dy*dy) + dz*dz, rr <= rr_cut_skin[a][b]; for a cubic box
+// (q_x = q_y = q_z) in fixed-point units instead: d_k = float(int32(...)),
+// rq = fma(dz, dz, fma(dy, dy, dx*dx)), rq <= float(rr_cut_skin[a][b] / q^2).
 // Writes counts[n] and the lists row-major in list[i*cap + k] (up to cap
 // entries per particle).  Returns the maximum count.
 int oracle_fixed_neighbour_list(int n, uint32_t const* u, unsigned const* species, unsigned const* ncell,
@@ -732,6 +734,11 @@ int oracle_fixed_neighbour_list(int n, uint32_t const* u, unsigned const* specie
     for (int i = 0; i < n; ++i) cells[cell[i]].push_back(unsigned(i));
     int maxc = 0;
     int const nx = int(ncell[0]), ny = int(ncell[1]), nz = int(ncell[2]);
+    // cubic box (equal FP32 units): the predicate runs in fixed-point units
+    // against float(rr_cut_skin / q^2), evaluated in FP64 from the FP32 inputs
+    bool const iso = q[0] == q[1] && q[1] == q[2];
+    std::vector<float> cut_q(size_t(ntype) * ntype);
+    for (size_t k = 0; k < cut_q.size(); ++k) cut_q[k] = float(double(rr_cut_skin[k]) / (double(q[0]) * double(q[0])));
     for (int i = 0; i < n; ++i) {
         uint32_t const* ui = u + 3 * size_t(i);
         unsigned const ti = species[i];
@@ -748,15 +755,28 @@ int oracle_fixed_neighbour_list(int n, uint32_t const* u, unsigned const* specie
             for (unsigned j : cl) {
                 if (j == unsigned(i)) continue;
                 uint32_t const* uj = u + 3 * size_t(j);
-                float d[3];
-                for (int k = 0; k < 3; ++k) {
-                    int32_t const D = int32_t(ui[k] - uj[k]);
-                    d[k] = float(D) * q[k];
+                bool inside;
+                if (iso) {
+                    // cubic box: squared distance in fixed-point units, one
+                    // conversion per component, products fused as written
+                    float d[3];
+                    for (int k = 0; k < 3; ++k) d[k] = float(int32_t(ui[k] - uj[k]));
+                    float rq = d[0] * d[0];
+                    rq = std::fmaf(d[1], d[1], rq);
+                    rq = std::fmaf(d[2], d[2], rq);
+                    inside = rq <= cut_q[ti * ntype + species[j]];
+                } else {
+                    float d[3];
+                    for (int k = 0; k < 3; ++k) {
+                        int32_t const D = int32_t(ui[k] - uj[k]);
+                        d[k] = float(D) * q[k];
+                    }
+                    float rr = d[0] * d[0];
+                    rr = rr + d[1] * d[1];
+                    rr = rr + d[2] * d[2];
+                    inside = rr <= rr_cut_skin[ti * ntype + species[j]];
                 }
-                float rr = d[0] * d[0];
-                rr = rr + d[1] * d[1];
-                rr = rr + d[2] * d[2];
-                if (rr <= rr_cut_skin[ti * ntype + species[j]]) {
+                if (inside) {
                     if (int(count) < cap) list[size_t(i) * cap + count] = j;
                     count++;
                 }
