@@ -32,6 +32,11 @@ TIMESTEP = 0.005
 SORT_INTERVAL = 100
 MODES_INTERVAL = 1000
 HBM_PEAK_GBS = 8000.0
+# hardware-counter summary of the pair-force kernel on this workload (rocprofv3
+# --pmc FETCH_SIZE / WRITE_SIZE, calibrated against a copy kernel); committed
+# beside the kernel-time summary it was measured with
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "pair_force_lj_n1m_counters.md")
 
 
 def log(msg):
@@ -81,6 +86,26 @@ def calibration_copy(nbytes):
     log(f"calibration copy: {nbytes} bytes read + {nbytes} bytes written")
 
 
+def counter_summary(kernel):
+    """Figures recorded for `kernel` in the committed counter summary:
+    lines 'key: value' inside the section headed by the kernel's name."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return {}
+    out, inside = {}, False
+    for line in text.splitlines():
+        if line.startswith("## "):
+            inside = line[3:].strip().strip("`") == kernel
+        elif inside and ":" in line and line.startswith("- "):
+            k, v = line[2:].split(":", 1)
+            try:
+                out[k.strip()] = float(v.split()[0])
+            except (ValueError, IndexError):
+                pass
+    return out
+
+
 def cpu_baseline(w, budget_s):
     """FP64 oracle (host backend restatement), one thread, same initial state."""
     from oracle import oracle as orc
@@ -109,6 +134,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=500)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default cell-segment shape")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="run one copy kernel of this size first (hardware-counter calibration)")
@@ -125,6 +151,9 @@ def main():
     log(f"device {arch}, {cus} CUs, {mem / 2**30:.0f} GiB; building LJ N={N}")
     t0 = time.perf_counter()
     w = build_workload(args.seed)
+    if not args.no_tune:
+        cells = w.force.tune()
+        log(f"autotuned cell segments: {w.neighbour.segments} (cells, workgroup size)")
     log(f"setup {time.perf_counter() - t0:.1f} s; warmup {args.warmup} steps")
     w.sampler.run(args.warmup)
     log(f"timed {args.steps} steps")
@@ -137,10 +166,16 @@ def main():
     n_nb = counts.mean()
     f_ms, f_n = kt["force"]
     f_avg = f_ms / max(f_n, 1)
-    # algorithmic bytes per particle for the force kernel: r_i (hi+lo, 24 B),
-    # (n_nb + 1) list words (4 B each: entries + count), force (12 B)
+    # algorithmic bytes per particle for the force kernel (fixed definition of
+    # unique compulsory traffic with float-float state): r_i (hi+lo, 24 B),
+    # (n_nb + 1) list words of 4 B (entries + count), force (12 B)
     force_bytes = N * (24 + 4 * (n_nb + 1) + 12)
     achieved = force_bytes / (f_avg * 1e-3) / 1e9
+    staged = w.neighbour.entry_bytes == 2
+    kname = ("pair_force_staged_kernel<0, 1, false, true, false>" if staged
+             else "pair_force_kernel<0, 1, false>")
+    prof = counter_summary(kname)
+    traffic = prof.get("traffic_bytes_per_launch")
     kernels = {k: {"launches": int(n), "avg_ms": (ms_ / n if n else None)} for k, (ms_, n) in kt.items()}
     log(f"force kernel {f_avg:.4f} ms/launch, {achieved:.0f} GB/s algorithmic; n_nb = {n_nb:.2f}")
     thermo = w.thermodynamics
@@ -173,6 +208,7 @@ def main():
             "wavevectors": int(len(w.wavevector.value)),
             "neighbours_mean": n_nb,
             "neighbour_rebuilds": int(w.neighbour.rebuilds),
+            "cell_segments": list(w.neighbour.segments),
             "en_tot_per_particle_end": en_tot,
             "device": arch,
         },
@@ -182,10 +218,14 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "pair_force_kernel<LJ, shifted>",
+            "traffic": traffic,
+            "kernel": kname,
             "bytes_per_launch": force_bytes,
             "avg_ms": f_avg,
+            # from the counter summary: fraction of the VALU issue rate
+            # (wave64 instruction = 4 cycles on 1024 SIMDs) the kernel used
+            "valu_issue_frac": prof.get("valu_issue_frac"),
+            "counters": os.path.relpath(TRAFFIC_PROFILE, ROOT) if prof else None,
         },
         "kernels": kernels,
         "cpu_baseline": cpu,

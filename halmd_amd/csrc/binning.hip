@@ -38,6 +38,29 @@ __device__ __forceinline__ unsigned cell_component(uint32_t u, unsigned nc) {
     return unsigned((uint64_t(u) * nc) >> 32);
 }
 
+// neighbour predicate on fixed-point records (cubic box: fixed-point units)
+template <bool ISO>
+__device__ __forceinline__ bool pair_inside(uint4 const ri, uint4 const rj, neighbour_params const& p, float cut) {
+#pragma clang fp contract(off)
+    if (ISO) {
+        float const dx = float(int32_t(ri.x - rj.x));
+        float const dy = float(int32_t(ri.y - rj.y));
+        float const dz = float(int32_t(ri.z - rj.z));
+        float rq = dx * dx;
+        rq = __builtin_fmaf(dy, dy, rq);
+        rq = __builtin_fmaf(dz, dz, rq);
+        return rq <= cut;
+    } else {
+        float const dx = float(int32_t(ri.x - rj.x)) * p.q[0];
+        float const dy = float(int32_t(ri.y - rj.y)) * p.q[1];
+        float const dz = float(int32_t(ri.z - rj.z)) * p.q[2];
+        float rr = dx * dx;
+        rr = rr + dy * dy;
+        rr = rr + dz * dz;
+        return rr <= cut;
+    }
+}
+
 __global__ void compute_cell_kernel(uint4 const* __restrict__ posq, unsigned n, cell_grid g,
                                     uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -91,36 +114,32 @@ __device__ __forceinline__ void flush_block(nb_state& s, unsigned block) {
     }
 }
 
+template <bool ISO>
 __device__ __forceinline__ void test_candidate(nb_state& s, uint4 const rj, neighbour_params const& p,
                                                float const* s_rr) {
-#pragma clang fp contract(off)
     uint32_t const j = rj.w >> 2;
-    float const dx = float(int32_t(s.ri.x - rj.x)) * p.q[0];
-    float const dy = float(int32_t(s.ri.y - rj.y)) * p.q[1];
-    float const dz = float(int32_t(s.ri.z - rj.z)) * p.q[2];
-    float rr = dx * dx;
-    rr = rr + dy * dy;
-    rr = rr + dz * dz;
-    if (j != s.i && rr <= s_rr[s.ti + (rj.w & 3u)]) {
+    if (j != s.i && pair_inside<ISO>(s.ri, rj, p, s_rr[s.ti + (rj.w & 3u)])) {
         s.buf[s.c % NB_BLOCK] = j;
         if (s.c % NB_BLOCK == NB_BLOCK - 1) flush_block(s, s.c / NB_BLOCK);
         ++s.c;
     }
 }
 
+template <bool ISO>
 __device__ __forceinline__ void scan_range(nb_state& s, unsigned b, unsigned e, uint4 const* __restrict__ cell_pos,
                                            neighbour_params const& p, float const* s_rr) {
     unsigned k = b;
     for (; k + 4 <= e; k += 4) {
         uint4 const r0 = cell_pos[k], r1 = cell_pos[k + 1], r2 = cell_pos[k + 2], r3 = cell_pos[k + 3];
-        test_candidate(s, r0, p, s_rr);
-        test_candidate(s, r1, p, s_rr);
-        test_candidate(s, r2, p, s_rr);
-        test_candidate(s, r3, p, s_rr);
+        test_candidate<ISO>(s, r0, p, s_rr);
+        test_candidate<ISO>(s, r1, p, s_rr);
+        test_candidate<ISO>(s, r2, p, s_rr);
+        test_candidate<ISO>(s, r3, p, s_rr);
     }
-    for (; k < e; ++k) test_candidate(s, cell_pos[k], p, s_rr);
+    for (; k < e; ++k) test_candidate<ISO>(s, cell_pos[k], p, s_rr);
 }
 
+template <bool ISO>
 __global__ __launch_bounds__(256) void build_neighbours_kernel(
     uint4 const* __restrict__ posq, unsigned n, uint32_t const* __restrict__ offset,
     uint4 const* __restrict__ cell_pos, neighbour_params p, unsigned cap, uint32_t* __restrict__ list,
@@ -128,7 +147,8 @@ __global__ __launch_bounds__(256) void build_neighbours_kernel(
 {
     __shared__ float s_rr[MAX_TYPES * MAX_TYPES];
     __shared__ uint32_t s_buf[256 * NB_STAGE];
-    if (threadIdx.x < MAX_TYPES * MAX_TYPES) s_rr[threadIdx.x] = p.rr_cut_skin[threadIdx.x];
+    if (threadIdx.x < MAX_TYPES * MAX_TYPES)
+        s_rr[threadIdx.x] = ISO ? p.rr_cut_skin_q[threadIdx.x] : p.rr_cut_skin[threadIdx.x];
     __syncthreads();
 
     unsigned const i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -153,13 +173,13 @@ __global__ __launch_bounds__(256) void build_neighbours_kernel(
                 unsigned const row = (kz * ny + ky) * nx;
                 if (cx > 0 && cx + 1 < nx) {
                     // cells x-1, x, x+1 are one contiguous range
-                    scan_range(s, offset[row + xm], offset[row + xp + 1], cell_pos, p, s_rr);
+                    scan_range<ISO>(s, offset[row + xm], offset[row + xp + 1], cell_pos, p, s_rr);
                 } else if (cx == 0) {
-                    scan_range(s, offset[row + xm], offset[row + xm + 1], cell_pos, p, s_rr);
-                    scan_range(s, offset[row], offset[row + 2], cell_pos, p, s_rr);
+                    scan_range<ISO>(s, offset[row + xm], offset[row + xm + 1], cell_pos, p, s_rr);
+                    scan_range<ISO>(s, offset[row], offset[row + 2], cell_pos, p, s_rr);
                 } else {
-                    scan_range(s, offset[row + xm], offset[row + nx], cell_pos, p, s_rr);
-                    scan_range(s, offset[row], offset[row + 1], cell_pos, p, s_rr);
+                    scan_range<ISO>(s, offset[row + xm], offset[row + nx], cell_pos, p, s_rr);
+                    scan_range<ISO>(s, offset[row], offset[row + 1], cell_pos, p, s_rr);
                 }
             }
         }
@@ -172,31 +192,66 @@ __global__ __launch_bounds__(256) void build_neighbours_kernel(
     if ((threadIdx.x % WAVE) == 0 && m > 0) atomicMax(max_count, m);
 }
 
-// --- cell segments: 16-bit slots into the LDS-staged neighbourhood union ----
+// --- all pairs (from_particle): tiles of 256 records in LDS, ascending j ----
 
-constexpr unsigned NB16_STAGE = NB16_BLOCK + 2;   // staging halfwords per thread (odd word count)
+template <bool ISO>
+__global__ __launch_bounds__(256) void build_all_pairs_kernel(uint4 const* __restrict__ posq, unsigned n,
+                                                              neighbour_params p, unsigned cap,
+                                                              uint32_t* __restrict__ list,
+                                                              uint32_t* __restrict__ count,
+                                                              uint32_t* __restrict__ max_count) {
+    __shared__ uint4 s_tile[256];
+    __shared__ float s_rr[MAX_TYPES * MAX_TYPES];
+    __shared__ uint32_t s_buf[256 * NB_STAGE];
+    if (threadIdx.x < MAX_TYPES * MAX_TYPES)
+        s_rr[threadIdx.x] = ISO ? p.rr_cut_skin_q[threadIdx.x] : p.rr_cut_skin[threadIdx.x];
+    unsigned const i = blockIdx.x * blockDim.x + threadIdx.x;
+    nb_state s;
+    s.c = 0;
+    s.i = i;
+    s.cap = cap;
+    s.ri = i < n ? posq[i] : make_uint4(0, 0, 0, 0);
+    s.ti = (s.ri.w & 3u) * p.ntype;
+    s.out = list + size_t(i & ~(WAVE - 1)) * cap + (i & (WAVE - 1)) * NB_BLOCK;
+    s.buf = s_buf + threadIdx.x * NB_STAGE;
+    for (unsigned base = 0; base < n; base += 256) {
+        __syncthreads();
+        if (base + threadIdx.x < n) s_tile[threadIdx.x] = posq[base + threadIdx.x];
+        __syncthreads();
+        if (i >= n) continue;
+        unsigned const m = min(256u, n - base);
+        for (unsigned t = 0; t < m; ++t) {
+            uint4 const rj = s_tile[t];
+            unsigned const j = base + t;
+            if (j != i && pair_inside<ISO>(s.ri, rj, p, s_rr[s.ti + (rj.w & 3u)])) {
+                s.buf[s.c % NB_BLOCK] = j;
+                if (s.c % NB_BLOCK == NB_BLOCK - 1) flush_block(s, s.c / NB_BLOCK);
+                ++s.c;
+            }
+        }
+    }
+    if (i < n) {
+        if (s.c % NB_BLOCK) flush_block(s, s.c / NB_BLOCK);
+        count[i] = s.c;
+    }
+    unsigned m = i < n ? s.c : 0u;
+    for (int off = WAVE / 2; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off, WAVE));
+    if ((threadIdx.x % WAVE) == 0 && m > 0) atomicMax(max_count, m);
+}
+
+// --- cell segments: 16-bit slots into the LDS-staged neighbourhood union ----
 
 struct nb16_state {
     uint4 ri;
     unsigned own, ti, c, cap;
-    uint4* out;          // first block of this particle
-    uint16_t* buf;       // this thread's staging slots in LDS (4-byte aligned)
+    uint16_t* out;       // entry 0 of this particle (lane block base)
 };
 
-// one lane block (NB16_BLOCK slots = 32 bytes) from the LDS staging slots
-__device__ __forceinline__ void flush16(nb16_state& s, unsigned block) {
-    if ((block + 1) * NB16_BLOCK <= s.cap) {
-        uint4* const dst = s.out + size_t(block) * (WAVE * NB16_BLOCK * 2 / 16);
-        uint32_t const* const w = reinterpret_cast<uint32_t const*>(s.buf);
-#pragma unroll
-        for (unsigned v = 0; v < NB16_BLOCK / 8; ++v) dst[v] = make_uint4(w[4 * v], w[4 * v + 1], w[4 * v + 2], w[4 * v + 3]);
-    }
-}
-
-__device__ __forceinline__ void append16(nb16_state& s, unsigned slot) {
-    unsigned const e = s.c % NB16_BLOCK;
-    s.buf[e] = uint16_t(slot);
-    if (e == NB16_BLOCK - 1) flush16(s, s.c / NB16_BLOCK);
+// 2-byte stores straight to the blocked list: a lane fills one 32-byte block
+// at a time and a wave's blocks share 2 KiB, so the lines are complete in L2
+// before they are written back (no staging buffer, hence more workgroups per CU)
+__device__ __forceinline__ void append16(nb16_state& s, unsigned entry) {
+    if (s.c < s.cap) s.out[size_t(s.c / NB16_BLOCK) * (WAVE * NB16_BLOCK) + s.c % NB16_BLOCK] = uint16_t(entry);
     ++s.c;
 }
 
@@ -241,7 +296,7 @@ __device__ __forceinline__ void scan16(nb16_state& s, unsigned t, unsigned const
 }
 
 template <bool MULTI, bool ISO>
-__global__ __launch_bounds__(SEG_THREADS) void build_staged_kernel(
+__global__ __launch_bounds__(SEG_THREADS_MAX) void build_staged_kernel(
     uint4 const* __restrict__ posq, uint32_t const* __restrict__ off, seg_params sp, neighbour_params p,
     unsigned cap, uint16_t* __restrict__ list, uint32_t* __restrict__ count, uint32_t* __restrict__ stats)
 {
@@ -249,7 +304,6 @@ __global__ __launch_bounds__(SEG_THREADS) void build_staged_kernel(
     __shared__ seg_stage st;
     __shared__ unsigned s_bound[9 * (SEG_MAX + 3)];
     __shared__ float s_rr[MAX_TYPES * MAX_TYPES];
-    __shared__ uint16_t s_buf[SEG_THREADS * NB16_STAGE];
 
     unsigned const seg = seg_of_block(sp, blockIdx.x);
     if (seg >= sp.nseg) return;   // the whole workgroup
@@ -262,15 +316,14 @@ __global__ __launch_bounds__(SEG_THREADS) void build_staged_kernel(
     unsigned const nx = p.grid.ncell[0];
     float const rr_cut0 = ISO ? p.rr_cut_skin_q[0] : p.rr_cut_skin[0];
     unsigned cmax = 0;
-    for (unsigned i = st.pb + threadIdx.x; i < st.pe; i += SEG_THREADS) {
+    for (unsigned i = st.pb + threadIdx.x; i < st.pe; i += blockDim.x) {
         nb16_state s;
         s.own = s_bound[4 * (SEG_MAX + 3) + 1] + (i - st.pb);
         s.ri = s_rec[s.own];
         s.ti = (s.ri.w & 3u) * p.ntype;
         s.c = 0;
         s.cap = cap;
-        s.out = reinterpret_cast<uint4*>(list + size_t(i & ~(WAVE - 1)) * cap + (i & (WAVE - 1)) * NB16_BLOCK);
-        s.buf = s_buf + threadIdx.x * NB16_STAGE;
+        s.out = list + size_t(i & ~(WAVE - 1)) * cap + (i & (WAVE - 1)) * NB16_BLOCK;
         unsigned const k = cell_component(s.ri.x, nx) - st.x0;   // 0 .. w - 1
         // rows 0 .. 8; only row 4 (dz = dy = 0) holds the particle itself
         for (unsigned r = 0; r < 9; ++r) {
@@ -282,11 +335,9 @@ __global__ __launch_bounds__(SEG_THREADS) void build_staged_kernel(
                 scan16<MULTI, ISO, false>(s, t, e, s_rec, p, s_rr, rr_cut0);
         }
         unsigned const c = s.c;
-        if (c <= cap) {
-            // pad with the sentinel to a multiple of 8 (cap is a multiple of 16)
+        // pad with the sentinel to a multiple of 8 (cap is a multiple of 16)
+        if (c <= cap)
             while (s.c % 8) append16(s, U * 16u);
-            if (s.c % NB16_BLOCK) flush16(s, s.c / NB16_BLOCK);   // last, partial block
-        }
         count[i] = c;
         cmax = max(cmax, c);
     }
@@ -317,8 +368,12 @@ void launch_build_neighbours(uint4 const* posq, unsigned n, uint32_t const* offs
                              neighbour_params const& p, unsigned cap, uint32_t* list, uint32_t* count,
                              uint32_t* max_count, hipStream_t s) {
     if (!n) return;
-    build_neighbours_kernel<<<div_up(n, 256), 256, 0, s>>>(posq, n, offset, cell_pos, p, cap, list, count,
-                                                           max_count);
+    if (p.iso)
+        build_neighbours_kernel<true><<<div_up(n, 256), 256, 0, s>>>(posq, n, offset, cell_pos, p, cap, list, count,
+                                                                     max_count);
+    else
+        build_neighbours_kernel<false><<<div_up(n, 256), 256, 0, s>>>(posq, n, offset, cell_pos, p, cap, list, count,
+                                                                      max_count);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -334,16 +389,30 @@ void launch_build_neighbours_staged(uint4 const* posq, uint32_t const* offset, s
     unsigned const grid = SEG_XCD * sp.chunk;
     if (p.ntype > 1) {
         if (p.iso)
-            build_staged_kernel<true, true><<<grid, SEG_THREADS, lds, s>>>(posq, offset, sp, p, cap, list, count, stats);
+            build_staged_kernel<true, true><<<grid, sp.threads, lds, s>>>(posq, offset, sp, p, cap, list, count, stats);
         else
-            build_staged_kernel<true, false><<<grid, SEG_THREADS, lds, s>>>(posq, offset, sp, p, cap, list, count, stats);
+            build_staged_kernel<true, false><<<grid, sp.threads, lds, s>>>(posq, offset, sp, p, cap, list, count, stats);
     } else {
         if (p.iso)
-            build_staged_kernel<false, true><<<grid, SEG_THREADS, lds, s>>>(posq, offset, sp, p, cap, list, count, stats);
+            build_staged_kernel<false, true><<<grid, sp.threads, lds, s>>>(posq, offset, sp, p, cap, list, count, stats);
         else
-            build_staged_kernel<false, false><<<grid, SEG_THREADS, lds, s>>>(posq, offset, sp, p, cap, list, count,
+            build_staged_kernel<false, false><<<grid, sp.threads, lds, s>>>(posq, offset, sp, p, cap, list, count,
                                                                           stats);
     }
+    HIP_CHECK(hipGetLastError());
+}
+
+} // namespace halmd
+
+namespace halmd {
+
+void launch_build_neighbours_all_pairs(uint4 const* posq, unsigned n, neighbour_params const& p, unsigned cap,
+                                       uint32_t* list, uint32_t* count, uint32_t* max_count, hipStream_t s) {
+    if (!n) return;
+    if (p.iso)
+        build_all_pairs_kernel<true><<<div_up(n, 256), 256, 0, s>>>(posq, n, p, cap, list, count, max_count);
+    else
+        build_all_pairs_kernel<false><<<div_up(n, 256), 256, 0, s>>>(posq, n, p, cap, list, count, max_count);
     HIP_CHECK(hipGetLastError());
 }
 

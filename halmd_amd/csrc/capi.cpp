@@ -5,6 +5,7 @@
 #include <string>
 
 #include "modules.hpp"
+#include <typeinfo>
 
 using namespace halmd;
 
@@ -307,8 +308,20 @@ HALMD_API int halmd_mdsim_neighbours_from_binning_create(halmd_particle* p, halm
         auto* bb = as<binning>(bin);
         auto* dd = as<max_displacement>(disp);
         require(&bb->p == pp && &dd->p == pp, "from_binning: binning and displacement must belong to the particle");
-        *out = handle<halmd_neighbour>(new neighbour(*pp, *bb, *dd, *as<box>(b), r_cut, pp->nspecies, pp->nspecies,
+        *out = handle<halmd_neighbour>(new neighbour(*pp, bb, *dd, *as<box>(b), r_cut, pp->nspecies, pp->nspecies,
                                                      skin, occupancy));
+    });
+}
+HALMD_API int halmd_mdsim_neighbours_from_particle_create(halmd_particle* p, halmd_max_displacement* disp,
+                                                          const halmd_box* b, const double* r_cut, double skin,
+                                                          double occupancy, halmd_neighbour** out) {
+    return guard([&] {
+        nonnull(r_cut, "r_cut"); nonnull(out, "out");
+        auto* pp = as<particle>(p);
+        auto* dd = as<max_displacement>(disp);
+        require(&dd->p == pp, "from_particle: displacement must belong to the particle");
+        *out = handle<halmd_neighbour>(new neighbour(*pp, nullptr, *dd, *as<box>(b), r_cut, pp->nspecies,
+                                                     pp->nspecies, skin, occupancy));
     });
 }
 HALMD_API int halmd_mdsim_neighbours_from_binning_destroy(halmd_neighbour* nb) {
@@ -344,6 +357,15 @@ HALMD_API int halmd_mdsim_neighbours_from_binning_entry_bytes(const halmd_neighb
         *bytes = as<neighbour>(nb)->staged ? 2u : 4u;
     });
 }
+HALMD_API int halmd_mdsim_neighbours_from_binning_segments(const halmd_neighbour* nb, unsigned* cells,
+                                                           unsigned* threads) {
+    return guard([&] {
+        nonnull(cells, "cells"); nonnull(threads, "threads");
+        auto const* n = as<neighbour>(nb);
+        *cells = n->staged ? n->sp.S : 0u;
+        *threads = n->staged ? n->sp.threads : 0u;
+    });
+}
 HALMD_API int halmd_mdsim_neighbours_from_binning_rebuilds(const halmd_neighbour* nb, uint64_t* count) {
     return guard([&] { *count = as<neighbour>(nb)->rebuilds; });
 }
@@ -365,6 +387,12 @@ HALMD_API int halmd_mdsim_forces_pair_trunc_create(const halmd_potential* pot, h
         auto* n = as<neighbour>(nb);
         require(&n->p == pp, "pair_trunc: neighbour list of another particle instance");
         *out = handle<halmd_force>(new pair_trunc(*as<potential>(pot), *pp, *as<box>(b), *n, w));
+    });
+}
+HALMD_API int halmd_mdsim_forces_pair_trunc_tune(halmd_force* f, unsigned* cells) {
+    return guard([&] {
+        nonnull(cells, "cells");
+        *cells = as<pair_trunc>(f)->tune();
     });
 }
 HALMD_API int halmd_mdsim_forces_pair_trunc_destroy(halmd_force* f) {
@@ -420,6 +448,14 @@ HALMD_API int halmd_mdsim_integrators_set_timestep(halmd_integrator* i, double d
 }
 HALMD_API int halmd_mdsim_integrators_timestep(const halmd_integrator* i, double* dt) {
     return guard([&] { *dt = as<integrator>(i)->timestep(); });
+}
+HALMD_API int halmd_mdsim_integrators_verlet_defer_finalize(halmd_integrator* i, int enable) {
+    return guard([&] {
+        auto* v = dynamic_cast<verlet*>(as<integrator>(i));
+        require(v != nullptr && typeid(*v) == typeid(verlet), "defer_finalize: not a plain verlet integrator");
+        if (!enable) v->p.settle_velocity();
+        v->defer_finalize = enable != 0;
+    });
 }
 HALMD_API int halmd_mdsim_integrators_verlet_nvt_hoover_set_temperature(halmd_integrator* i, double T) {
     return guard([&] { nhc(i)->set_temperature(T); });

@@ -217,7 +217,7 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
 //    set lives in scalar registers.
 // Summation: FP32 over blocks of 16 pair forces, Kahan over the blocks.
 template <int KIND, int TRUNC, bool AUX, bool ISO, bool MULTI>
-__global__ __launch_bounds__(SEG_THREADS) void pair_force_staged_kernel(
+__global__ __launch_bounds__(SEG_THREADS_MAX) void pair_force_staged_kernel(
     uint4 const* __restrict__ posq, uint32_t const* __restrict__ off, seg_params sp,
     uint16_t const* __restrict__ list, uint32_t const* __restrict__ count, unsigned cap, force_params P,
     bool force_zero, float aux_weight, float4* __restrict__ force, float* __restrict__ stress, unsigned stride)
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(SEG_THREADS) void pair_force_staged_kernel(
     if (MULTI && threadIdx.x < MAX_TYPES * MAX_TYPES) s_pair[threadIdx.x] = P.pair[threadIdx.x];
     stage_union(sp, seg, off, posq, s_rec, st, nullptr);   // fits: the list build checked it
 
-    for (unsigned i = st.pb + threadIdx.x; i < st.pe; i += SEG_THREADS) {
+    for (unsigned i = st.pb + threadIdx.x; i < st.pe; i += blockDim.x) {
         uint4 const qi = posq[i];
         unsigned const ti = (qi.w & 3u) * P.ntype;
         unsigned const n8 = (count[i] + 7u) & ~7u;
@@ -400,7 +400,7 @@ template <int KIND, int TRUNC, bool AUX, bool ISO, bool MULTI>
 void staged_go(staged_launch const& a) {
     size_t const lds = size_t(a.sp.lds_cap) * sizeof(uint4);
     unsigned const grid = SEG_XCD * a.sp.chunk;
-    pair_force_staged_kernel<KIND, TRUNC, AUX, ISO, MULTI><<<grid, SEG_THREADS, lds, a.s>>>(
+    pair_force_staged_kernel<KIND, TRUNC, AUX, ISO, MULTI><<<grid, a.sp.threads, lds, a.s>>>(
         a.posq, a.off, a.sp, a.list, a.count, a.cap, a.p, a.force_zero, a.w, a.force, a.stress, a.stride);
 }
 

@@ -66,8 +66,8 @@ __device__ __forceinline__ ff ff_from_double(double x) {
 
 __global__ __launch_bounds__(256) void verlet_integrate_kernel(
     float4* __restrict__ pos, float4* __restrict__ vel, float4 const* __restrict__ force, int4* __restrict__ image,
-    uint4* __restrict__ posq, unsigned n, integrate_params p, double const* __restrict__ scale, float4 const* __restrict__ r0,
-    uint32_t* __restrict__ max_disp2, uint32_t* __restrict__ clear_slot)
+    uint4* __restrict__ posq, unsigned n, integrate_params p, float pre_half_dt, double const* __restrict__ scale,
+    float4 const* __restrict__ r0, uint32_t* __restrict__ max_disp2, uint32_t* __restrict__ clear_slot)
 {
     unsigned const i = blockIdx.x * blockDim.x + threadIdx.x;
     if (clear_slot && i == 0) *clear_slot = 0u;
@@ -78,6 +78,12 @@ __global__ __launch_bounds__(256) void verlet_integrate_kernel(
         float4 const f = force[i];
         float const m = vh.w;
         ff vx{vh.x, vl.x}, vy{vh.y, vl.y}, vz{vh.z, vl.z};
+        if (pre_half_dt != 0.0f) {
+            // finalize of the previous step, as verlet_finalize_kernel computes it
+            vx = ff_add_f(vx, f.x * pre_half_dt / m);
+            vy = ff_add_f(vy, f.y * pre_half_dt / m);
+            vz = ff_add_f(vz, f.z * pre_half_dt / m);
+        }
         if (scale) {
             ff const s = ff_from_double(*scale);
             vx = ff_mul(vx, s); vy = ff_mul(vy, s); vz = ff_mul(vz, s);
@@ -214,11 +220,11 @@ __global__ __launch_bounds__(256) void andersen_finalize_kernel(float4* __restri
 } // namespace
 
 void launch_verlet_integrate(float4* pos, float4* vel, float4 const* force, int4* image, uint4* posq, unsigned n,
-                             integrate_params const& p, double const* scale, float4 const* r0, uint32_t* max_disp2,
-                             uint32_t* clear_slot, hipStream_t s) {
+                             integrate_params const& p, float pre_half_dt, double const* scale, float4 const* r0,
+                             uint32_t* max_disp2, uint32_t* clear_slot, hipStream_t s) {
     if (!n) return;
-    verlet_integrate_kernel<<<div_up(n, 256), 256, 0, s>>>(pos, vel, force, image, posq, n, p, scale, r0, max_disp2,
-                                                           clear_slot);
+    verlet_integrate_kernel<<<div_up(n, 256), 256, 0, s>>>(pos, vel, force, image, posq, n, p, pre_half_dt, scale, r0,
+                                                           max_disp2, clear_slot);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -82,6 +82,11 @@ void launch_build_neighbours(uint4 const* posq, unsigned n, uint32_t const* offs
                              neighbour_params const& p, unsigned cap, uint32_t* list, uint32_t* count,
                              uint32_t* max_count, hipStream_t s);
 
+// lists from all particle pairs (small boxes): ascending j, the same predicate
+// and list layout as launch_build_neighbours
+void launch_build_neighbours_all_pairs(uint4 const* posq, unsigned n, neighbour_params const& p, unsigned cap,
+                                       uint32_t* list, uint32_t* count, uint32_t* max_count, hipStream_t s);
+
 // ---------------------------------------------------------------------------
 // cell segments (binning.hip, force.hip): particles stored in cell order, one
 // workgroup per segment of up to S consecutive cells along x in one (y, z)
@@ -92,7 +97,7 @@ void launch_build_neighbours(uint4 const* posq, unsigned n, uint32_t const* offs
 // 16-bit LDS slots of that union.
 // ---------------------------------------------------------------------------
 constexpr unsigned SEG_MAX = 32;          // largest S
-constexpr unsigned SEG_THREADS = 256;     // workgroup size of segment kernels
+constexpr unsigned SEG_THREADS_MAX = 256; // largest workgroup of the segment kernels
 constexpr unsigned SEG_XCD = 8;           // workgroups are dealt round-robin to 8 XCDs
 struct seg_params {
     cell_grid grid;
@@ -101,6 +106,7 @@ struct seg_params {
     unsigned nseg;        // nsx * ny * nz
     unsigned chunk;       // ceil(nseg / SEG_XCD): consecutive segments per XCD
     unsigned lds_cap;     // LDS records per workgroup (>= every union size)
+    unsigned threads;     // workgroup size (a multiple of 64, <= SEG_THREADS_MAX)
 };
 // workgroup b serves segment (b % 8) * chunk + b / 8, so that each XCD walks a
 // contiguous run of segments and re-reads shared halo rows from its own L2
@@ -206,12 +212,14 @@ struct integrate_params {
     float L_hi[3], L_lo[3], invL[3];
     quant_params qp;
 };
-// kick-drift-wrap and fixed-point snapshot; optional velocity pre-scale
-// (*scale, device) and optional max-displacement tracking against r0 (max
-// |r - r0|^2 as float bits folded into *max_disp2; *clear_slot, the slot of
-// the next step, is set to zero)
+// kick-drift-wrap and fixed-point snapshot; optional deferred finalize kick
+// of the previous step (pre_half_dt != 0: v += F dt/2m first, the same
+// arithmetic as the finalize kernel), optional velocity pre-scale (*scale,
+// device) and optional max-displacement tracking against r0 (max |r - r0|^2 as
+// float bits folded into *max_disp2; *clear_slot, the slot of the next step,
+// is set to zero)
 void launch_verlet_integrate(float4* pos, float4* vel, float4 const* force, int4* image, uint4* posq, unsigned n,
-                             integrate_params const& p, double const* scale, float4 const* r0,
+                             integrate_params const& p, float pre_half_dt, double const* scale, float4 const* r0,
                              uint32_t* max_disp2, uint32_t* clear_slot, hipStream_t s);
 void launch_verlet_finalize(float4* vel, float4 const* force, unsigned n, float half_dt, hipStream_t s);
 void launch_rescale_velocity(float4* vel, unsigned n, double const* scale, hipStream_t s);

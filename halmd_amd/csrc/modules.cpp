@@ -2,6 +2,7 @@
 #include "modules.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <array>
 #include <cmath>
 #include <map>
@@ -161,16 +162,31 @@ void particle::upload_hilo(dbuf<float4>& b, double const* in, bool) {
     sync();
 }
 
+void particle::settle_velocity() {
+    if (pending_half_dt == 0) return;
+    launch_verlet_finalize(vel, force, n, pending_half_dt, stream);
+    pending_half_dt = 0;
+}
+
 void particle::set_position(double const* r) {
+    settle_velocity();
     upload_hilo(pos, r, true);
     requantize();
     positions_changed();
 }
 void particle::get_position(double* r) { download_hilo(pos, r); }
-void particle::set_velocity(double const* v) { upload_hilo(vel, v, false); velocities_changed(); }
-void particle::get_velocity(double* v) { download_hilo(vel, v); }
+void particle::set_velocity(double const* v) {
+    pending_half_dt = 0;   // overwritten
+    upload_hilo(vel, v, false);
+    velocities_changed();
+}
+void particle::get_velocity(double* v) {
+    settle_velocity();
+    download_hilo(vel, v);
+}
 
 void particle::set_species(unsigned const* s) {
+    settle_velocity();
     host_tmp_.resize(2 * size_t(n));
     pos.download(host_tmp_.data(), host_tmp_.size(), stream);
     sync();
@@ -194,6 +210,7 @@ void particle::get_species(unsigned* s) {
 }
 
 void particle::set_mass(double const* m) {
+    settle_velocity();
     host_tmp_.resize(2 * size_t(n));
     vel.download(host_tmp_.data(), host_tmp_.size(), stream);
     sync();
@@ -260,6 +277,7 @@ void particle::get_reverse_id(unsigned* v) { reverse_id.download(v, n, stream); 
 // when requested
 void particle::update_force(bool need_aux) {
     if (force_valid && (!need_aux || aux_valid)) return;
+    settle_velocity();   // a pending kick refers to the force about to be replaced
     aux_enabled = aux_enabled || need_aux;
     if (forces.empty()) {
         HIP_CHECK(hipMemsetAsync(force, 0, force.bytes(), stream));
@@ -280,6 +298,7 @@ void particle::update_force(bool need_aux) {
 }
 
 void particle::rearrange(uint32_t const* perm) {
+    settle_velocity();
     launch_rearrange(pos, vel, image, id, posq, perm, n, pos_alt, vel_alt, image_alt, id_alt, posq_alt, stream);
     std::swap(posq, posq_alt);
     std::swap(pos, pos_alt);
@@ -539,20 +558,22 @@ double max_displacement::compute() {
 
 // ---------------------------------------------------------------------------
 // neighbour lists from binning (halmd/mdsim/gpu/neighbours/from_binning.cpp)
+// or from all particle pairs (halmd/mdsim/gpu/neighbours/from_particle.cpp)
 // ---------------------------------------------------------------------------
-neighbour::neighbour(particle& p_, binning& bin_, max_displacement& disp_, box const& b_, double const* rc,
+neighbour::neighbour(particle& p_, binning* bin_, max_displacement& disp_, box const& b_, double const* rc,
                      unsigned nt1, unsigned nt2, double skin, double occupancy_)
     : p(p_), bin(bin_), disp(disp_), b(b_), r_skin(skin), occupancy(occupancy_)
 {
     p.attach_box(b);
     require(nt1 == nt2 && nt1 == p.nspecies, "neighbour: cutoff matrix must be nspecies x nspecies");
     require(occupancy > 0 && occupancy <= 1, "neighbour: occupancy must be in (0, 1]");
-    for (int k = 0; k < 3; ++k)
-        require(bin.grid.ncell[k] >= 3,
-                "neighbour: the binning needs at least 3 cells per dimension (box too small for rc + skin)");
+    if (bin)
+        for (int k = 0; k < 3; ++k)
+            require(bin->grid.ncell[k] >= 3,
+                    "neighbour: the binning needs at least 3 cells per dimension (box too small for rc + skin)");
     r_cut.assign(rc, rc + size_t(nt1) * nt2);
     std::memset(&params, 0, sizeof(params));
-    params.grid = bin.grid;
+    if (bin) params.grid = bin->grid;
     for (int k = 0; k < 3; ++k) params.q[k] = b.q_hi[k];
     params.ntype = nt2;
     double rc_max = 0;
@@ -573,7 +594,7 @@ neighbour::neighbour(particle& p_, binning& bin_, max_displacement& disp_, box c
     h_maxcount.resize(2);
     // the first neighbour module of a particle instance keeps it in cell order
     // and uses the cell-segment kernels; the rules need nx >= 3 (S <= nx - 2)
-    staged = p.order_owner == nullptr;
+    staged = bin && p.order_owner == nullptr;
     if (staged) {
         p.order_owner = this;
         cap = unsigned(round_up(size_t(std::ceil(nmean / occupancy)) + NB16_BLOCK, NB16_BLOCK));
@@ -581,17 +602,16 @@ neighbour::neighbour(particle& p_, binning& bin_, max_displacement& disp_, box c
         int dev = 0, lds = 0;
         HIP_CHECK(hipGetDevice(&dev));
         HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-        // static LDS of the build kernel: staging words, cell bounds, segment table
-        size_t const static_lds = SEG_THREADS * (NB16_BLOCK / 2 + 1) * 4 + 9 * (SEG_MAX + 3) * 4 + 1024;
+        // static LDS of the build kernel: cell bounds, segment table, cutoffs
+        size_t const static_lds = 9 * (SEG_MAX + 3) * 4 + 1024;
         require(size_t(lds) > static_lds + 27 * 16, "neighbour: device LDS too small");
         lds_limit = std::min<size_t>((size_t(lds) - static_lds) / sizeof(uint4), SEG_MAX_RECORDS);
-        // about SEG_TARGET particles per segment at the mean cell occupancy
-        double const occ = double(p.n) / bin.ncells;
-        constexpr double SEG_TARGET = 224;
-        unsigned S = unsigned(std::max(1.0, std::floor(SEG_TARGET / std::max(occ, 1e-9))));
-        plan_segments(S);
-        double const expect = 9.0 * (sp.S + 2) * occ;
-        sp.lds_cap = unsigned(std::min<double>(lds_limit, round_up(size_t(1.3 * expect) + 64, 64)));
+        // about `target` particles per segment (per 256-thread workgroup) at
+        // the mean cell occupancy (pair_trunc::tune refines the width by
+        // measurement); HALMD_SEG_TARGET overrides the target
+        double target = 200;
+        if (char const* e = std::getenv("HALMD_SEG_TARGET")) target = std::max(1.0, std::atof(e));
+        set_segments(unsigned(std::max(1.0, std::round(target / std::max(mean_occupancy(), 1e-9)))), SEG_THREADS_MAX);
     } else {
         cap = unsigned(round_up(size_t(std::ceil(nmean / occupancy)) + NB_BLOCK, NB_BLOCK));
         list.resize(size_t(round_up(p.n, WAVE)) * cap);
@@ -602,12 +622,22 @@ neighbour::~neighbour() {
     if (p.order_owner == this) p.order_owner = nullptr;
 }
 
+double neighbour::mean_occupancy() const { return bin ? double(p.n) / bin->ncells : 0.0; }
+
+void neighbour::set_segments(unsigned S, unsigned threads) {
+    require(threads % WAVE == 0 && threads >= WAVE && threads <= SEG_THREADS_MAX, "neighbour: invalid workgroup size");
+    plan_segments(S);
+    sp.threads = threads;
+    double const expect = 9.0 * (sp.S + 2) * mean_occupancy();
+    sp.lds_cap = unsigned(std::min<double>(lds_limit, round_up(size_t(1.3 * expect) + 64, 64)));
+}
+
 void neighbour::plan_segments(unsigned S) {
-    unsigned const nx = bin.grid.ncell[0];
-    sp.grid = bin.grid;
+    unsigned const nx = bin->grid.ncell[0];
+    sp.grid = bin->grid;
     sp.S = std::max(1u, std::min({S, SEG_MAX, nx - 2}));
     sp.nsx = unsigned(div_up(nx, sp.S));
-    sp.nseg = sp.nsx * bin.grid.ncell[1] * bin.grid.ncell[2];
+    sp.nseg = sp.nsx * bin->grid.ncell[1] * bin->grid.ncell[2];
     sp.chunk = unsigned(div_up(sp.nseg, SEG_XCD));
 }
 
@@ -624,11 +654,15 @@ void neighbour::require_current() {
 void neighbour::update() {
     hipStream_t s = p.stream;
     if (!staged) {
-        bin.update(true);
+        if (bin) bin->update(true);
         while (true) {
             launch_fill_u32(d_maxcount, 0u, 1, s);
             timer.start(s);
-            launch_build_neighbours(p.posq, p.n, bin.offset, bin.cell_pos, params, cap, list, count, d_maxcount, s);
+            if (bin)
+                launch_build_neighbours(p.posq, p.n, bin->offset, bin->cell_pos, params, cap, list, count, d_maxcount,
+                                        s);
+            else
+                launch_build_neighbours_all_pairs(p.posq, p.n, params, cap, list, count, d_maxcount, s);
             timer.stop(s);
             HIP_CHECK(hipMemcpyAsync(h_maxcount.get(), d_maxcount.get(), 4, hipMemcpyDeviceToHost, s));
             p.sync();
@@ -642,11 +676,11 @@ void neighbour::update() {
         return;
     }
     // memory order := cell order, then the segment lists
-    bin.reorder_particles();
+    bin->reorder_particles();
     while (true) {
         launch_fill_u32(d_maxcount, 0u, 2, s);
         timer.start(s);
-        launch_build_neighbours_staged(p.posq, bin.offset, sp, params, cap, list16, count, d_maxcount, s);
+        launch_build_neighbours_staged(p.posq, bin->offset, sp, params, cap, list16, count, d_maxcount, s);
         timer.stop(s);
         HIP_CHECK(hipMemcpyAsync(h_maxcount.get(), d_maxcount.get(), 8, hipMemcpyDeviceToHost, s));
         p.sync();
@@ -694,9 +728,9 @@ void neighbour::get_lists(unsigned* counts, unsigned* entries, unsigned stride) 
     }
     // slots -> particle indices through each segment's union
     std::vector<uint16_t> h(size_t(round_up(p.n, WAVE)) * cap);
-    std::vector<uint32_t> off(size_t(bin.ncells) + 1);
+    std::vector<uint32_t> off(size_t(bin->ncells) + 1);
     list16.download(h.data(), h.size(), s);
-    bin.offset.download(off.data(), off.size(), s);
+    bin->offset.download(off.data(), off.size(), s);
     p.sync();
     std::vector<uint32_t> slot_to_index;
     for (unsigned seg = 0; seg < sp.nseg; ++seg) {
@@ -741,6 +775,49 @@ pair_trunc::~pair_trunc() {
 
 void pair_trunc::prepare() { nb.require_current(); }
 
+unsigned pair_trunc::tune() {
+    if (!nb.staged) return 0;
+    hipStream_t s = p.stream;
+    unsigned const S0 = nb.sp.S, nx = nb.bin->grid.ncell[0];
+    unsigned const lo = S0 > 4 ? S0 - 4 : 1, hi = std::min({S0 + 4, SEG_MAX, nx - 2});
+    constexpr int REPEAT = 5;
+    constexpr double REBUILD_INTERVAL = 10;   // steps between rebuilds assumed in the score
+    hipEvent_t ev[3];
+    for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+    unsigned best_S = S0, best_T = nb.sp.threads;
+    double best_score = 1e300;
+    auto trial = [&](unsigned S, unsigned T) {
+        nb.set_segments(S, T);
+        HIP_CHECK(hipEventRecord(ev[0], s));
+        nb.built = false;
+        nb.require_current();
+        HIP_CHECK(hipEventRecord(ev[1], s));
+        for (int k = 0; k < REPEAT; ++k) apply(true);
+        HIP_CHECK(hipEventRecord(ev[2], s));
+        HIP_CHECK(hipEventSynchronize(ev[2]));
+        float t_build = 0, t_force = 0;
+        HIP_CHECK(hipEventElapsedTime(&t_build, ev[0], ev[1]));
+        HIP_CHECK(hipEventElapsedTime(&t_force, ev[1], ev[2]));
+        double const score = t_force / REPEAT + t_build / REBUILD_INTERVAL;
+        if (score < best_score) { best_score = score; best_S = S; best_T = T; }
+    };
+    try {
+        trial(S0, nb.sp.threads);   // warm start (first build after a reorder is slower)
+        best_score = 1e300;
+        for (unsigned T : {128u, 192u, 256u})
+            for (unsigned S = lo; S <= hi; ++S) trial(S, T);
+    } catch (...) {
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        throw;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    nb.set_segments(best_S, best_T);
+    nb.built = false;
+    nb.require_current();
+    p.force_valid = p.aux_valid = false;
+    return nb.sp.S;
+}
+
 void pair_trunc::apply(bool force_zero) {
     nb.require_current();
     bool const aux = p.aux_enabled;
@@ -748,7 +825,7 @@ void pair_trunc::apply(bool force_zero) {
     float const w = float(aux_weight / 2);
     timer.start(p.stream);
     if (nb.staged)
-        launch_pair_force_staged(p.posq, nb.bin.offset, nb.sp_force, nb.list16, nb.count, nb.cap, params, aux,
+        launch_pair_force_staged(p.posq, nb.bin->offset, nb.sp_force, nb.list16, nb.count, nb.cap, params, aux,
                                  force_zero, w, p.force, p.stress, p.n_pad, p.stream);
     else
         launch_pair_force(p.posq, p.n, nb.list, nb.count, nb.cap, params, aux, force_zero, w, p.force, p.stress,
@@ -785,8 +862,9 @@ void verlet::integrate_kernel(double const* scale) {
         slot = fused_disp->fuse_slot();
         clear = fused_disp->d_max2.get() + (fused_disp->slot ^ 1u);
     }
+    float const pre = p.take_pending_kick();
     timer_integrate.start(p.stream);
-    launch_verlet_integrate(p.pos, p.vel, p.force, p.image, p.posq, p.n, ip, scale, r0, slot, clear, p.stream);
+    launch_verlet_integrate(p.pos, p.vel, p.force, p.image, p.posq, p.n, ip, pre, scale, r0, slot, clear, p.stream);
     timer_integrate.stop(p.stream);
     p.positions_changed();
     p.velocities_changed();
@@ -796,9 +874,13 @@ void verlet::integrate() { integrate_kernel(nullptr); }
 
 void verlet::finalize() {
     p.update_force();
-    timer_finalize.start(p.stream);
-    launch_verlet_finalize(p.vel, p.force, p.n, ip.half_dt, p.stream);
-    timer_finalize.stop(p.stream);
+    if (defer_finalize) {
+        p.pending_half_dt = ip.half_dt;
+    } else {
+        timer_finalize.start(p.stream);
+        launch_verlet_finalize(p.vel, p.force, p.n, ip.half_dt, p.stream);
+        timer_finalize.stop(p.stream);
+    }
     p.velocities_changed();
 }
 
@@ -806,6 +888,7 @@ verlet_nvt_hoover::verlet_nvt_hoover(particle& p_, box const& b_, double dt_, do
     : verlet(p_, b_, dt_), temperature(T), resonance_frequency(nu)
 {
     require(T > 0 && nu > 0, "verlet_nvt_hoover: temperature and resonance frequency must be positive");
+    defer_finalize = false;
     st.resize(1);
     partials.resize(REDUCE_BLOCKS);
     nhc_state s0;
@@ -866,6 +949,7 @@ void verlet_nvt_hoover::set_chain(double const* xi, double const* v_xi) {
 
 void verlet_nvt_hoover::integrate() {
     p.update_force();
+    p.settle_velocity();
     launch_mv2_partials(p.vel, p.n, partials, p.stream);
     launch_nhc_propagate(partials, st, p.stream);
     integrate_kernel(&st.get()->scale);
@@ -894,6 +978,7 @@ verlet_nvt_andersen::verlet_nvt_andersen(particle& p_, box const& b_, uint64_t s
 
 void verlet_nvt_andersen::finalize() {
     p.update_force();
+    p.settle_velocity();
     timer_finalize.start(p.stream);
     launch_andersen_finalize(p.vel, p.force, p.id, p.n, ip.half_dt, float(rate * dt), float(std::sqrt(temperature)),
                              seed, step++, p.stream);
@@ -962,6 +1047,7 @@ void thermodynamics::compute(bool need_aux) {
     if (fresh && (!need_aux || have_aux)) return;
     p.update_force(need_aux);
     bool const aux = p.aux_valid;
+    p.settle_velocity();
     launch_thermo_partials(p.vel, p.force, p.stress, p.n_pad, p.n, aux, partials, p.stream);
     launch_reduce_final(partials, THERMO_NQ, out, p.stream);
     HIP_CHECK(hipMemcpyAsync(h_out.get(), out.get(), THERMO_NQ * sizeof(double), hipMemcpyDeviceToHost, p.stream));

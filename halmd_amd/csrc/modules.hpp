@@ -108,6 +108,14 @@ public:
     void get_reverse_id(unsigned* v);
 
     void positions_changed() { ++position_version; force_valid = false; aux_valid = false; }
+    // Deferred half kick of the plain Verlet finalize, v += F dt / 2m with the
+    // force of the current positions.  The next integrate kernel applies it
+    // together with its own half kick (the same arithmetic, one pass fewer);
+    // any reader of the velocities, or of the force it refers to, settles it
+    // first.  0 = none pending.
+    float pending_half_dt = 0;
+    void settle_velocity();
+    float take_pending_kick() { float const h = pending_half_dt; pending_half_dt = 0; return h; }
     // recompute the fixed-point snapshot after a host-side change of pos
     void requantize();
     // bind the periodic box (done by every module constructed from particle
@@ -193,11 +201,13 @@ class hilbert;
 
 class neighbour {
 public:
-    neighbour(particle& p, binning& bin, max_displacement& disp, box const& b, double const* r_cut, unsigned ntype1,
+    // bin == nullptr: lists from all particle pairs (from_particle), for
+    // boxes with fewer than 3 cells of rc + skin in some dimension
+    neighbour(particle& p, binning* bin, max_displacement& disp, box const& b, double const* r_cut, unsigned ntype1,
               unsigned ntype2, double skin, double occupancy);
     ~neighbour();
     particle& p;
-    binning& bin;
+    binning* bin;
     max_displacement& disp;
     box const& b;
     double r_skin, occupancy;
@@ -212,6 +222,10 @@ public:
     size_t lds_limit = 0;             // records of dynamic LDS a build workgroup may use
     dbuf<uint16_t> list16;
     void plan_segments(unsigned S);
+    // segments of S cells, workgroups of `threads`, with the staging capacity
+    // expected for them
+    void set_segments(unsigned S, unsigned threads);
+    double mean_occupancy() const;
     uint64_t built_order = 0;
     bool built = false;
     uint64_t rebuilds = 0;
@@ -243,6 +257,10 @@ public:
     force_params params;
     void prepare() override;
     void apply(bool force_zero) override;
+    // pick the cell-segment width and workgroup size of the neighbour lists
+    // by timing list builds and force evaluations around the current choice;
+    // returns the width kept (0 if the lists are not cell-segment lists)
+    unsigned tune();
     kernel_timer timer;
 };
 
@@ -269,6 +287,9 @@ public:
     void integrate() override;
     void finalize() override;
     kernel_timer timer_integrate, timer_finalize;
+    // leave the finalize kick pending for the next integrate kernel (plain
+    // NVE Verlet; the thermostats need the kicked velocities at once)
+    bool defer_finalize = true;
 protected:
     void integrate_kernel(double const* scale);
 };

@@ -202,15 +202,25 @@ class neighbour(Handle):
         rc = _matrix(r_cut, particle.nspecies)
         if binning is None:
             binning = globals()["binning"](particle, box, rc, skin=skin)
+        # the cell algorithm needs 3 cells of rc + skin per dimension; fall
+        # back to all pairs otherwise (neighbour.lua.in:216-226), or when
+        # binning=False is passed
+        if binning is not False and min(binning.ncell) < 3:
+            binning = False
         if displacement is None:
             displacement = max_displacement(particle, box)
-        self.binning = binning
+        self.binning = binning or None
+        self.algorithm = "from_binning" if binning else "from_particle"
         self.displacement = displacement
         self.particle = particle
         self._deps = (particle, box)
         self._h = out_handle()
-        call("halmd_mdsim_neighbours_from_binning_create", particle.handle, binning.handle, displacement.handle,
-             box.handle, dptr(rc), float(skin), float(occupancy), ctypes.byref(self._h))
+        if binning:
+            call("halmd_mdsim_neighbours_from_binning_create", particle.handle, binning.handle, displacement.handle,
+                 box.handle, dptr(rc), float(skin), float(occupancy), ctypes.byref(self._h))
+        else:
+            call("halmd_mdsim_neighbours_from_particle_create", particle.handle, displacement.handle, box.handle,
+                 dptr(rc), float(skin), float(occupancy), ctypes.byref(self._h))
         self.r_skin = float(skin)
         self.sort = None
         if not disable_sorting:
@@ -230,6 +240,13 @@ class neighbour(Handle):
         v = ctypes.c_uint()
         call("halmd_mdsim_neighbours_from_binning_size", self.handle, ctypes.byref(v))
         return v.value
+
+    @property
+    def segments(self):
+        """(cells per segment, workgroup size) of cell-segment lists, (0, 0) otherwise."""
+        c, t = ctypes.c_uint(), ctypes.c_uint()
+        call("halmd_mdsim_neighbours_from_binning_segments", self.handle, ctypes.byref(c), ctypes.byref(t))
+        return c.value, t.value
 
     @property
     def entry_bytes(self):

@@ -39,6 +39,13 @@ class pair_trunc(Handle):
     def apply(self):
         call("halmd_mdsim_forces_pair_trunc_apply", self.handle)
 
+    def tune(self):
+        """Autotune the cell-segment width of the neighbour lists (timed on the
+        device); returns the cells per segment kept, 0 if not applicable."""
+        v = ctypes.c_uint()
+        call("halmd_mdsim_forces_pair_trunc_tune", self.handle, ctypes.byref(v))
+        return v.value
+
 
 def pair(box, particle, potential, neighbour=None, weight=1.0):
     """forces.pair{box, particle, potential, neighbour}: truncated potentials use pair_trunc."""

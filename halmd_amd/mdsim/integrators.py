@@ -40,6 +40,10 @@ class verlet(_integrator):
         call("halmd_mdsim_integrators_verlet_create", particle.handle, box.handle, float(timestep),
              ctypes.byref(self._h))
 
+    def defer_finalize(self, enable=True):
+        """Fold the finalize half kick into the next integrate pass (default on)."""
+        call("halmd_mdsim_integrators_verlet_defer_finalize", self.handle, int(bool(enable)))
+
 
 class verlet_nvt_hoover(_integrator):
     """Nose-Hoover chain (M=2) thermostat (integrators/verlet_nvt_hoover.lua.in:165-255;

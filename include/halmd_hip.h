@@ -159,6 +159,14 @@ HALMD_API int halmd_mdsim_neighbours_from_binning_create(halmd_particle* p, halm
                                                          const double* r_cut, double skin, double occupancy,
                                                          halmd_neighbour** out);
 HALMD_API int halmd_mdsim_neighbours_from_binning_destroy(halmd_neighbour* nb);
+/* ---- mdsim.neighbours.from_particle: halmd/mdsim/gpu/neighbours/from_particle.cpp:45-200
+ *      from_particle(particle, particle, displacement, box, r_cut, skin, occupancy): all
+ *      pairs, for boxes with fewer than 3 cells of rc + skin (neighbour.lua.in:216-226).
+ *      The handle is a neighbour handle: update, lists, counts, size, rebuilds and
+ *      destroy are the from_binning functions. */
+HALMD_API int halmd_mdsim_neighbours_from_particle_create(halmd_particle* p, halmd_max_displacement* displacement,
+                                                          const halmd_box* box, const double* r_cut, double skin,
+                                                          double occupancy, halmd_neighbour** out);
 /* force a rebuild now (from_binning.cpp:162-282) */
 HALMD_API int halmd_mdsim_neighbours_from_binning_update(halmd_neighbour* nb);
 /* lists as rows of `stride` entries (rebuilding first if required) */
@@ -170,6 +178,10 @@ HALMD_API int halmd_mdsim_neighbours_from_binning_size(const halmd_neighbour* nb
  * (particle indices); no counterpart in the reference (its entries are 4-byte
  * indices, from_binning.hpp:105) */
 HALMD_API int halmd_mdsim_neighbours_from_binning_entry_bytes(const halmd_neighbour* nb, unsigned* bytes);
+/* cell-segment shape of the lists (cells per segment, workgroup size); zeros
+ * for global-index lists.  No counterpart in the reference */
+HALMD_API int halmd_mdsim_neighbours_from_binning_segments(const halmd_neighbour* nb, unsigned* cells,
+                                                           unsigned* threads);
 /* number of neighbours of each particle (memory order) */
 HALMD_API int halmd_mdsim_neighbours_from_binning_counts(halmd_neighbour* nb, unsigned* counts);
 HALMD_API int halmd_mdsim_neighbours_from_binning_rebuilds(const halmd_neighbour* nb, uint64_t* count);
@@ -182,6 +194,11 @@ HALMD_API int halmd_mdsim_forces_pair_trunc_create(const halmd_potential* pot, h
                                                    const halmd_box* box, halmd_neighbour* nb, double aux_weight,
                                                    halmd_force** out);
 HALMD_API int halmd_mdsim_forces_pair_trunc_destroy(halmd_force* f);
+/* autotuner: time list builds and force evaluations for cell-segment widths
+ * around the current one and keep the fastest (cells per segment in *cells;
+ * 0 when the neighbour lists are not cell-segment lists).  No counterpart in
+ * the reference (its block size is a fixed option, from_binning.cpp:328-339) */
+HALMD_API int halmd_mdsim_forces_pair_trunc_tune(halmd_force* f, unsigned* cells);
 /* force:apply() (pair_trunc.hpp:193-215): evaluate now (overwrites the force) */
 HALMD_API int halmd_mdsim_forces_pair_trunc_apply(halmd_force* f);
 
@@ -200,6 +217,11 @@ HALMD_API int halmd_mdsim_integrators_integrate(halmd_integrator* i);
 HALMD_API int halmd_mdsim_integrators_finalize(halmd_integrator* i);
 HALMD_API int halmd_mdsim_integrators_set_timestep(halmd_integrator* i, double timestep);
 HALMD_API int halmd_mdsim_integrators_timestep(const halmd_integrator* i, double* timestep);
+/* plain velocity-Verlet only: leave the finalize half kick pending and fold it
+ * into the next integrate kernel (enable = 1, the default) or apply it in its
+ * own pass (enable = 0); trajectories are bitwise the same either way.  No
+ * counterpart in the reference (verlet.cpp:64-121 always runs two passes) */
+HALMD_API int halmd_mdsim_integrators_verlet_defer_finalize(halmd_integrator* i, int enable);
 /* verlet_nvt_hoover: set_temperature, position/velocity of the chain, set_mass,
  * internal_energy (energy of the heat bath per particle) */
 HALMD_API int halmd_mdsim_integrators_verlet_nvt_hoover_set_temperature(halmd_integrator* i, double temperature);

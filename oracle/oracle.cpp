@@ -723,6 +723,51 @@ void oracle_fixed_cell_index(int n, uint32_t const* u, unsigned const* ncell, un
 // rq = fma(dz, dz, fma(dy, dy, dx*dx)), rq <= float(rr_cut_skin[a][b] / q^2).
 // Writes counts[n] and the lists row-major in list[i*cap + k] (up to cap
 // entries per particle).  Returns the maximum count.
+// All-pairs rule of the device from_particle build (halmd/mdsim/gpu/neighbours/
+// from_particle_kernel.cu:33-118 on the fixed-point coordinates): candidates
+// j = 0 .. n-1 ascending, j != i, with the predicate of the rule below (cubic
+// box: fixed-point units).  Returns the maximum count.
+int oracle_all_pairs_neighbour_list(int n, uint32_t const* u, unsigned const* species, float const* q, int ntype,
+                                    float const* rr_cut_skin, int cap, unsigned* counts, unsigned* list)
+{
+    bool const iso = q[0] == q[1] && q[1] == q[2];
+    std::vector<float> cut_q(size_t(ntype) * ntype);
+    for (size_t k = 0; k < cut_q.size(); ++k) cut_q[k] = float(double(rr_cut_skin[k]) / (double(q[0]) * double(q[0])));
+    int maxc = 0;
+    for (int i = 0; i < n; ++i) {
+        uint32_t const* ui = u + 3 * size_t(i);
+        unsigned count = 0;
+        for (int j = 0; j < n; ++j) {
+            if (j == i) continue;
+            uint32_t const* uj = u + 3 * size_t(j);
+            bool inside;
+            unsigned const ab = species[i] * ntype + species[j];
+            if (iso) {
+                float d[3];
+                for (int k = 0; k < 3; ++k) d[k] = float(int32_t(ui[k] - uj[k]));
+                float rq = d[0] * d[0];
+                rq = std::fmaf(d[1], d[1], rq);
+                rq = std::fmaf(d[2], d[2], rq);
+                inside = rq <= cut_q[ab];
+            } else {
+                float d[3];
+                for (int k = 0; k < 3; ++k) d[k] = float(int32_t(ui[k] - uj[k])) * q[k];
+                float rr = d[0] * d[0];
+                rr = rr + d[1] * d[1];
+                rr = rr + d[2] * d[2];
+                inside = rr <= rr_cut_skin[ab];
+            }
+            if (inside) {
+                if (int(count) < cap) list[size_t(i) * cap + count] = unsigned(j);
+                count++;
+            }
+        }
+        counts[i] = count;
+        maxc = std::max(maxc, int(count));
+    }
+    return maxc;
+}
+
 int oracle_fixed_neighbour_list(int n, uint32_t const* u, unsigned const* species, unsigned const* ncell,
                                 float const* q, int ntype, float const* rr_cut_skin, int cap, unsigned* counts,
                                 unsigned* list)

@@ -53,6 +53,7 @@ def lib():
             "oracle_fixed_point": (None, [ctypes.c_int, _d, _d, _u]),
             "oracle_fixed_cell_index": (None, [ctypes.c_int, _u, _u, _u]),
             "oracle_fixed_neighbour_list": (ctypes.c_int, [ctypes.c_int, _u, _u, _u, _f, ctypes.c_int, _f, ctypes.c_int, _u, _u]),
+            "oracle_all_pairs_neighbour_list": (ctypes.c_int, [ctypes.c_int, _u, _u, _f, ctypes.c_int, _f, ctypes.c_int, _u, _u]),
             "oracle_fp32_hilbert_code": (None, [ctypes.c_int, _f, _f, ctypes.c_uint, _u]),
             "oracle_hilbert_depth": (ctypes.c_uint, [_d]),
             "oracle_hilbert_map_f64": (ctypes.c_uint, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint]),
@@ -207,6 +208,21 @@ def fixed_cell_index(u, ncell):
 def fixed_unit(L):
     """float(L) * 2^-32 per dimension (FP32 length of one fixed-point unit)."""
     return np.ldexp(np.asarray(L, dtype=np.float64).astype(np.float32), -32).astype(np.float32)
+
+
+def all_pairs_neighbour_list(u, species, L, rr_cut_skin, cap):
+    """Device from_particle rule: (counts, lists[n, cap], max count), ascending j."""
+    u = np.ascontiguousarray(u, dtype=np.uint32)
+    sp = np.ascontiguousarray(species, dtype=np.uint32)
+    q = np.ascontiguousarray(fixed_unit(L), dtype=np.float32)
+    rr = np.ascontiguousarray(rr_cut_skin, dtype=np.float32)
+    ntype = rr.shape[0]
+    n = len(u)
+    counts = np.zeros(n, dtype=np.uint32)
+    lst = np.zeros(n * cap, dtype=np.uint32)
+    m = lib().oracle_all_pairs_neighbour_list(n, _p(u, _u), _p(sp, _u), _p(q, _f), ntype, _p(rr, _f), cap,
+                                              _p(counts, _u), _p(lst, _u))
+    return counts, lst.reshape(n, cap), m
 
 
 def fixed_neighbour_list(u, species, ncell, L, rr_cut_skin, cap):

@@ -228,10 +228,43 @@ def test_fixed_cell_index_is_floor_rule(oracle):
     assert np.array_equal(c, expect)
 
 
+def test_cubic_rule_cell_and_all_pairs_agree(oracle):
+    # cubic box: the fixed-point-units predicate.  The cell algorithm and the
+    # all-pairs algorithm are separate code paths over the same predicate:
+    # their sets agree, all-pairs lists ascend, both are symmetric, and both
+    # equal the FP64 sets outside a band of rounding width around rc + skin
+    pos, L, species, _ = _liquid(oracle, 1500, 0.8442, 7, species_frac=0.3)
+    pot = _ka_potential(oracle)
+    skin = 0.3
+    ncell = np.floor(L / (np.max(pot.r_cut) + skin)).astype(np.uint32)
+    rcs = ((pot.r_cut + skin) ** 2).astype(np.float32)
+    u = oracle.fixed_point(pos, L)
+    c1, l1, _ = oracle.fixed_neighbour_list(u, species, ncell, L, rcs, 256)
+    c2, l2, _ = oracle.all_pairs_neighbour_list(u, species, L, rcs, 256)
+    assert np.array_equal(c1, c2)
+    got = np.zeros((len(u), len(u)), bool)
+    for i in range(len(u)):
+        assert np.array_equal(np.sort(l1[i, :c1[i]]), l2[i, :c2[i]])
+        assert np.all(np.diff(l2[i, :c2[i]].astype(np.int64)) > 0)
+        got[i, l2[i, :c2[i]]] = True
+    assert np.array_equal(got, got.T)
+    dr = pos[:, None, :] - pos[None, :, :]
+    dr -= L * np.rint(dr / L)
+    r64 = np.sqrt(np.sum(dr * dr, -1))
+    cut = np.sqrt(rcs[species[:, None], species[None, :]].astype(np.float64))
+    clear = np.abs(r64 - cut) > 1e-5
+    np.fill_diagonal(clear, False)
+    assert np.array_equal(got[clear], (r64 < cut)[clear])
+
+
 def test_fixed_neighbour_list_rules(oracle):
     # device rule: full lists; the set equals the predicate evaluated on all
-    # pairs, the lists are symmetric, and entries follow the documented order
+    # pairs, the lists are symmetric, and entries follow the documented order.
+    # A slightly non-cubic box exercises the FP32 physical-units predicate,
+    # which numpy evaluates exactly as written (no fused products)
     pos, L, species, _ = _liquid(oracle, 2048, 0.8442, 5, species_frac=0.3)
+    L = L * np.array([1.0, 1.0 + 1e-3, 1.0 - 1e-3])
+    pos = pos * (L / L[0])
     pot = _ka_potential(oracle)
     skin = 0.3
     ncell = np.floor(L / (np.max(pot.r_cut) + skin)).astype(np.uint32)
@@ -240,6 +273,7 @@ def test_fixed_neighbour_list_rules(oracle):
     counts, lst, m = oracle.fixed_neighbour_list(u, species, ncell, L, rcs, 256)
     assert m <= 256
     q = oracle.fixed_unit(L)
+    assert not (q[0] == q[1] == q[2])
     D = (u[:, None, :] - u[None, :, :]).astype(np.int32)
     d = D.astype(np.float32) * q
     rr = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
