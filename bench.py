@@ -126,6 +126,60 @@ def cpu_baseline(w, budget_s):
                       f"{secs:.1f} s"}
 
 
+def batched(args):
+    """Batched ensembles: 256 seeded systems (N log-uniform in [500, 32000], rho
+    in [0.7, 1.2], T in [0.5, 2.0], LJ / WCA / Morse), NVE dt = 0.003, stepped
+    together; against the plain per-job path (each system alone through the
+    per-system modules, one after another) on every 16th system."""
+    from halmd_amd import workloads
+    specs = workloads.batched_specs(args.systems, seed=args.seed)
+    ntot = sum(s["n"] for s in specs)
+    t0 = time.perf_counter()
+    e = workloads.ensemble_from_specs(specs, timestep=0.003)
+    log(f"ensemble of {len(specs)} systems, {ntot} particles, setup {time.perf_counter() - t0:.1f} s")
+    e.step(args.warmup)
+    ms = e.step_timed(args.steps)
+    value = ntot * args.steps / (ms / 1e3)
+    log(f"batched: {ms / args.steps:.3f} ms/step, {value:.4e} particle-steps/s")
+    pot0, kin0 = e.energies()
+    sample = specs[::16]
+    plain_steps = max(10, args.steps // 5)
+    psum, tsum = 0.0, 0.0
+    for sp in sample:
+        w = workloads.single_from_spec(sp, timestep=0.003)
+        w.sampler.run(5)
+        t = w.sampler.run_timed(plain_steps)
+        psum += sp["n"] * plain_steps
+        tsum += t / 1e3
+    plain = psum / tsum
+    log(f"plain per-job path on {len(sample)} systems: {plain:.4e} particle-steps/s")
+    out = {
+        "metric": "aggregate particle-steps/s, batched ensemble of 256 systems vs plain per-job GPU path",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{len(specs)} independent NVE systems, N log-uniform in [500, 32000] (total {ntot}), "
+                        "rho U[0.7, 1.2], T U[0.5, 2.0], LJ rc 2.5 / WCA / Morse rc 2.5, dt 0.003, skin 0.3",
+            "systems": len(specs),
+            "particles": ntot,
+            "plain_per_job_particle_steps_per_s": plain,
+            "plain_sample": f"every 16th system ({len(sample)}), {plain_steps} timed steps each, sequential",
+            "speedup_vs_plain": value / plain,
+            "en_tot_per_particle_range": [float(np.min(pot0 + kin0)), float(np.max(pot0 + kin0))],
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +189,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default cell-segment shape")
+    ap.add_argument("--workload", default="lj", choices=["lj", "batched"],
+                    help="lj: LJ fluid N = 2^20 (default); batched: ensemble of independent systems")
+    ap.add_argument("--systems", type=int, default=256, help="systems of the batched workload")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="run one copy kernel of this size first (hardware-counter calibration)")
@@ -145,6 +202,8 @@ def main():
         sys.exit(2)
 
     import halmd_amd
+    if args.workload == "batched":
+        return batched(args)
     arch, cus, mem = halmd_amd.device_info()
     if args.calibration_copy_mib:
         calibration_copy(args.calibration_copy_mib << 20)

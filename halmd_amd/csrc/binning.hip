@@ -295,18 +295,15 @@ __device__ __forceinline__ void scan16(nb16_state& s, unsigned t, unsigned const
     for (; t < e; ++t) test16<MULTI, ISO, SELF>(s, s_rec[t], t, p, s_rr, rr_cut0);
 }
 
+// One cell segment of the staged list build (the whole workgroup calls it);
+// shared by the single-system kernel and the ensemble kernel.  stats[0]
+// receives the largest count, stats[1] the staging size U + 1.
 template <bool MULTI, bool ISO>
-__global__ __launch_bounds__(SEG_THREADS_MAX) void build_staged_kernel(
-    uint4 const* __restrict__ posq, uint32_t const* __restrict__ off, seg_params sp, neighbour_params p,
-    unsigned cap, uint16_t* __restrict__ list, uint32_t* __restrict__ count, uint32_t* __restrict__ stats)
+__device__ __forceinline__ void staged_build_segment(
+    uint4 const* __restrict__ posq, uint32_t const* __restrict__ off, seg_params const& sp, unsigned seg,
+    neighbour_params const& p, unsigned cap, uint16_t* __restrict__ list, uint32_t* __restrict__ count,
+    uint32_t* __restrict__ stats, uint4* s_rec, seg_stage& st, unsigned* s_bound, float* s_rr)
 {
-    extern __shared__ uint4 s_rec[];
-    __shared__ seg_stage st;
-    __shared__ unsigned s_bound[9 * (SEG_MAX + 3)];
-    __shared__ float s_rr[MAX_TYPES * MAX_TYPES];
-
-    unsigned const seg = seg_of_block(sp, blockIdx.x);
-    if (seg >= sp.nseg) return;   // the whole workgroup
     if (threadIdx.x < MAX_TYPES * MAX_TYPES)
         s_rr[threadIdx.x] = ISO ? p.rr_cut_skin_q[threadIdx.x] : p.rr_cut_skin[threadIdx.x];
     unsigned const U = stage_union(sp, seg, off, posq, s_rec, st, s_bound);
@@ -343,6 +340,170 @@ __global__ __launch_bounds__(SEG_THREADS_MAX) void build_staged_kernel(
     }
     for (int o = WAVE / 2; o > 0; o >>= 1) cmax = max(cmax, (unsigned)__shfl_xor((int)cmax, o, WAVE));
     if ((threadIdx.x % WAVE) == 0 && cmax > 0) atomicMax(&stats[0], cmax);
+}
+
+template <bool MULTI, bool ISO>
+__global__ __launch_bounds__(SEG_THREADS_MAX) void build_staged_kernel(
+    uint4 const* __restrict__ posq, uint32_t const* __restrict__ off, seg_params sp, neighbour_params p,
+    unsigned cap, uint16_t* __restrict__ list, uint32_t* __restrict__ count, uint32_t* __restrict__ stats)
+{
+    extern __shared__ uint4 s_rec[];
+    __shared__ seg_stage st;
+    __shared__ unsigned s_bound[9 * (SEG_MAX + 3)];
+    __shared__ float s_rr[MAX_TYPES * MAX_TYPES];
+
+    unsigned const seg = seg_of_block(sp, blockIdx.x);
+    if (seg >= sp.nseg) return;   // the whole workgroup
+    staged_build_segment<MULTI, ISO>(posq, off, sp, seg, p, cap, list, count, stats, s_rec, st, s_bound, s_rr);
+}
+
+// --- ensembles: rebuild of a listed subset of systems --------------------
+
+__device__ __forceinline__ unsigned cell_of(uint4 const r, cell_grid const& g) {
+    unsigned const x = cell_component(r.x, g.ncell[0]);
+    unsigned const y = cell_component(r.y, g.ncell[1]);
+    unsigned const z = cell_component(r.z, g.ncell[2]);
+    return (z * g.ncell[1] + y) * g.ncell[0] + x;
+}
+
+__global__ __launch_bounds__(256) void ens_cells_kernel(uint4 const* __restrict__ posq,
+                                                        ens_desc const* __restrict__ descs,
+                                                        uint32_t const* __restrict__ rsys,
+                                                        uint32_t const* __restrict__ rpos,
+                                                        uint32_t const* __restrict__ rblk, unsigned cell_bits,
+                                                        uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    uint32_t const e = rblk[blockIdx.x];
+    unsigned const rank = e >> 16, lb = e & 0xffffu;
+    ens_desc const& d = descs[rsys[rank]];
+    unsigned const li = lb * 256 + threadIdx.x;
+    if (li >= d.n) return;
+    unsigned const c = rpos[rank] + li;
+    keys[c] = (rank << cell_bits) | cell_of(posq[d.base + li], d.np.grid);
+    vals[c] = li;
+}
+
+// one workgroup per listed system: offset[c] = first slot with key >= c
+__global__ __launch_bounds__(256) void ens_offsets_kernel(ens_desc const* __restrict__ descs,
+                                                          uint32_t const* __restrict__ rsys,
+                                                          uint32_t const* __restrict__ rpos, unsigned cell_bits,
+                                                          uint32_t const* __restrict__ keys,
+                                                          uint32_t* __restrict__ offsets) {
+    unsigned const rank = blockIdx.x;
+    ens_desc const& d = descs[rsys[rank]];
+    unsigned const lo = rpos[rank];
+    for (unsigned c = threadIdx.x; c <= d.ncells; c += blockDim.x) {
+        uint32_t const target = (rank << cell_bits) | c;
+        unsigned a = 0, b = d.n;   // lower bound in keys[lo, lo + n)
+        while (a < b) {
+            unsigned const m = (a + b) / 2;
+            if (keys[lo + m] < target) a = m + 1; else b = m;
+        }
+        offsets[d.cell_base + c] = a;
+    }
+}
+
+__global__ __launch_bounds__(256) void ens_gather_kernel(
+    float4 const* __restrict__ pos, float4 const* __restrict__ vel, int4 const* __restrict__ image,
+    uint32_t const* __restrict__ id, uint4 const* __restrict__ posq, float4* __restrict__ pos_t,
+    float4* __restrict__ vel_t, int4* __restrict__ image_t, uint32_t* __restrict__ id_t, uint4* __restrict__ posq_t,
+    ens_desc const* __restrict__ descs, uint32_t const* __restrict__ rsys, uint32_t const* __restrict__ rpos,
+    uint32_t const* __restrict__ rblk, uint32_t const* __restrict__ vals) {
+    uint32_t const e = rblk[blockIdx.x];
+    unsigned const rank = e >> 16, lb = e & 0xffffu;
+    ens_desc const& d = descs[rsys[rank]];
+    unsigned const li = lb * 256 + threadIdx.x;
+    if (li >= d.n) return;
+    size_t const dst = d.base + li, src = d.base + vals[rpos[rank] + li];
+    pos_t[2 * dst] = pos[2 * src];
+    pos_t[2 * dst + 1] = pos[2 * src + 1];
+    vel_t[2 * dst] = vel[2 * src];
+    vel_t[2 * dst + 1] = vel[2 * src + 1];
+    image_t[dst] = image[src];
+    id_t[dst] = id[src];
+    posq_t[dst] = posq[src];
+}
+
+__global__ __launch_bounds__(256) void ens_copyback_kernel(
+    float4* __restrict__ pos, float4* __restrict__ vel, int4* __restrict__ image, uint32_t* __restrict__ id,
+    uint4* __restrict__ posq, float4* __restrict__ r0, float4 const* __restrict__ pos_t,
+    float4 const* __restrict__ vel_t, int4 const* __restrict__ image_t, uint32_t const* __restrict__ id_t,
+    uint4 const* __restrict__ posq_t, ens_desc const* __restrict__ descs, uint32_t const* __restrict__ rsys,
+    uint32_t const* __restrict__ rblk) {
+    uint32_t const e = rblk[blockIdx.x];
+    unsigned const rank = e >> 16, lb = e & 0xffffu;
+    ens_desc const& d = descs[rsys[rank]];
+    unsigned const li = lb * 256 + threadIdx.x;
+    if (li >= d.n) return;
+    size_t const i = d.base + li;
+    float4 const h = pos_t[2 * i];
+    pos[2 * i] = h;
+    pos[2 * i + 1] = pos_t[2 * i + 1];
+    vel[2 * i] = vel_t[2 * i];
+    vel[2 * i + 1] = vel_t[2 * i + 1];
+    image[i] = image_t[i];
+    id[i] = id_t[i];
+    posq[i] = posq_t[i];
+    r0[i] = h;
+}
+
+// entries (s << 16) | block
+__global__ __launch_bounds__(256) void ens_r0_kernel(float4 const* __restrict__ pos, float4* __restrict__ r0,
+                                                     ens_desc const* __restrict__ descs,
+                                                     uint32_t const* __restrict__ rblk) {
+    uint32_t const e = rblk[blockIdx.x];
+    ens_desc const& d = descs[e >> 16];
+    unsigned const li = (e & 0xffffu) * 256 + threadIdx.x;
+    if (li < d.n) r0[d.base + li] = pos[2 * size_t(d.base + li)];
+}
+
+__global__ __launch_bounds__(SEG_THREADS_MAX) void ens_build_kernel(
+    uint4 const* __restrict__ posq, uint32_t const* __restrict__ offsets, ens_desc const* __restrict__ descs,
+    uint32_t const* __restrict__ rseg, uint16_t* __restrict__ list, uint32_t* __restrict__ count,
+    uint32_t* __restrict__ stats)
+{
+    extern __shared__ uint4 s_rec[];
+    __shared__ seg_stage st;
+    __shared__ unsigned s_bound[9 * (SEG_MAX + 3)];
+    __shared__ float s_rr[MAX_TYPES * MAX_TYPES];
+    uint32_t const e = rseg[blockIdx.x];
+    unsigned const sys = e >> 16, seg = e & 0xffffu;
+    ens_desc const& d = descs[sys];
+    staged_build_segment<false, true>(posq + d.base, offsets + d.cell_base, d.sp, seg, d.np, d.cap,
+                                      list + d.list_off, count + d.base, stats + 2 * sys, s_rec, st, s_bound, s_rr);
+}
+
+// all-pairs lists of one small system per workgroup (the system in LDS);
+// entries are particle indices of the system, ascending
+__global__ __launch_bounds__(256) void ens_build_small_kernel(uint4 const* __restrict__ posq,
+                                                              ens_desc const* __restrict__ descs,
+                                                              uint32_t const* __restrict__ rsmall,
+                                                              uint16_t* __restrict__ list,
+                                                              uint32_t* __restrict__ count,
+                                                              uint32_t* __restrict__ stats) {
+    extern __shared__ uint4 s_rec[];
+    unsigned const sys = rsmall[blockIdx.x];
+    ens_desc const& d = descs[sys];
+    for (unsigned t = threadIdx.x; t < d.n; t += blockDim.x) s_rec[t] = posq[d.base + t];
+    __syncthreads();
+    float const cut = d.np.rr_cut_skin_q[0];
+    uint16_t* const L = list + d.list_off;
+    unsigned cmax = 0;
+    for (unsigned i = threadIdx.x; i < d.n; i += blockDim.x) {
+        uint4 const ri = s_rec[i];
+        uint16_t* const out = L + size_t(i & ~(WAVE - 1)) * d.cap + (i & (WAVE - 1)) * NB16_BLOCK;
+        unsigned c = 0;
+        for (unsigned j = 0; j < d.n; ++j) {
+            if (j != i && pair_inside<true>(ri, s_rec[j], d.np, cut)) {
+                if (c < d.cap) out[size_t(c / NB16_BLOCK) * (WAVE * NB16_BLOCK) + c % NB16_BLOCK] = uint16_t(j);
+                ++c;
+            }
+        }
+        count[d.base + i] = c;
+        cmax = max(cmax, c);
+    }
+    for (int o = WAVE / 2; o > 0; o >>= 1) cmax = max(cmax, (unsigned)__shfl_xor((int)cmax, o, WAVE));
+    if ((threadIdx.x % WAVE) == 0 && cmax > 0) atomicMax(&stats[2 * sys], cmax);
+    if (threadIdx.x == 0) atomicMax(&stats[2 * sys + 1], d.n + 1);
 }
 
 } // namespace
@@ -413,6 +574,62 @@ void launch_build_neighbours_all_pairs(uint4 const* posq, unsigned n, neighbour_
         build_all_pairs_kernel<true><<<div_up(n, 256), 256, 0, s>>>(posq, n, p, cap, list, count, max_count);
     else
         build_all_pairs_kernel<false><<<div_up(n, 256), 256, 0, s>>>(posq, n, p, cap, list, count, max_count);
+    HIP_CHECK(hipGetLastError());
+}
+
+} // namespace halmd
+
+namespace halmd {
+
+void launch_ens_cells(uint4 const* posq, ens_desc const* descs, uint32_t const* rsys, uint32_t const* rpos,
+                      uint32_t const* rblk, unsigned nrblk, unsigned cell_bits, uint32_t* keys, uint32_t* vals,
+                      hipStream_t s) {
+    if (!nrblk) return;
+    ens_cells_kernel<<<nrblk, 256, 0, s>>>(posq, descs, rsys, rpos, rblk, cell_bits, keys, vals);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_ens_offsets(ens_desc const* descs, uint32_t const* rsys, uint32_t const* rpos, unsigned nr,
+                        unsigned cell_bits, uint32_t const* sorted_keys, uint32_t* offsets, hipStream_t s) {
+    if (!nr) return;
+    ens_offsets_kernel<<<nr, 256, 0, s>>>(descs, rsys, rpos, cell_bits, sorted_keys, offsets);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_ens_reorder(float4* pos, float4* vel, int4* image, uint32_t* id, uint4* posq, float4* r0,
+                        float4* pos_t, float4* vel_t, int4* image_t, uint32_t* id_t, uint4* posq_t,
+                        ens_desc const* descs, uint32_t const* rsys, uint32_t const* rpos, uint32_t const* rblk,
+                        unsigned nrblk, uint32_t const* sorted_vals, hipStream_t s) {
+    if (!nrblk) return;
+    ens_gather_kernel<<<nrblk, 256, 0, s>>>(pos, vel, image, id, posq, pos_t, vel_t, image_t, id_t, posq_t, descs,
+                                            rsys, rpos, rblk, sorted_vals);
+    HIP_CHECK(hipGetLastError());
+    ens_copyback_kernel<<<nrblk, 256, 0, s>>>(pos, vel, image, id, posq, r0, pos_t, vel_t, image_t, id_t, posq_t,
+                                              descs, rsys, rblk);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_ens_r0(float4 const* pos, float4* r0, ens_desc const* descs, uint32_t const* rblk_sys, unsigned nrblk,
+                   hipStream_t s) {
+    if (!nrblk) return;
+    ens_r0_kernel<<<nrblk, 256, 0, s>>>(pos, r0, descs, rblk_sys);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_ens_build(uint4 const* posq, uint32_t const* offsets, ens_desc const* descs, uint32_t const* rseg,
+                      unsigned nrseg, unsigned threads, unsigned lds_records, uint16_t* list, uint32_t* count,
+                      uint32_t* stats, hipStream_t s) {
+    if (!nrseg) return;
+    ens_build_kernel<<<nrseg, threads, size_t(lds_records) * sizeof(uint4), s>>>(posq, offsets, descs, rseg, list,
+                                                                                  count, stats);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_ens_build_small(uint4 const* posq, ens_desc const* descs, uint32_t const* rsmall, unsigned nsmall,
+                            unsigned max_n, uint16_t* list, uint32_t* count, uint32_t* stats, hipStream_t s) {
+    if (!nsmall) return;
+    ens_build_small_kernel<<<nsmall, 256, size_t(max_n) * sizeof(uint4), s>>>(posq, descs, rsmall, list, count,
+                                                                              stats);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -5,6 +5,7 @@
 #include <string>
 
 #include "modules.hpp"
+#include "ensemble.hpp"
 #include <typeinfo>
 
 using namespace halmd;
@@ -726,6 +727,67 @@ HALMD_API int halmd_observables_sampler_thermodynamics_records(halmd_sampler* s,
         *nrec = r.size() / 5;
         if (out) std::copy(r.begin(), r.begin() + std::min(r.size(), 5 * capacity), out);
     });
+}
+
+// ---- ensembles ------------------------------------------------------------------------------------
+HALMD_API int halmd_ensemble_create(halmd_ensemble** out) {
+    return guard([&] { nonnull(out, "out"); *out = handle<halmd_ensemble>(new ensemble()); });
+}
+HALMD_API int halmd_ensemble_destroy(halmd_ensemble* e) {
+    return guard([&] { delete reinterpret_cast<ensemble*>(e); });
+}
+HALMD_API int halmd_ensemble_add(halmd_ensemble* e, unsigned n, double length, int kind, const char* truncation,
+                                 double epsilon, double sigma, double r_cut_sigma, double r_min, double distortion,
+                                 double smoothing, double timestep, double skin, const double* position,
+                                 const double* velocity, unsigned* index) {
+    return guard([&] {
+        nonnull(position, "position"); nonnull(velocity, "velocity"); nonnull(index, "index");
+        require(kind == POTENTIAL_LENNARD_JONES || kind == POTENTIAL_MORSE, "ensemble: unknown potential kind");
+        *index = as<ensemble>(e)->add(n, length, kind, truncation_code(truncation), epsilon, sigma, r_cut_sigma, r_min,
+                                     distortion, smoothing, timestep, skin, position, velocity);
+    });
+}
+HALMD_API int halmd_ensemble_commit(halmd_ensemble* e) {
+    return guard([&] { as<ensemble>(e)->commit(); });
+}
+HALMD_API int halmd_ensemble_step(halmd_ensemble* e, uint64_t steps) {
+    return guard([&] {
+        auto* x = as<ensemble>(e);
+        x->step(unsigned(steps));
+        HIP_CHECK(hipStreamSynchronize(x->stream));
+    });
+}
+HALMD_API int halmd_ensemble_step_timed(halmd_ensemble* e, uint64_t steps, double* ms) {
+    return guard([&] {
+        nonnull(ms, "ms");
+        auto* x = as<ensemble>(e);
+        hipEvent_t e0, e1;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipEventRecord(e0, x->stream));
+        x->step(unsigned(steps));
+        HIP_CHECK(hipEventRecord(e1, x->stream));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float t = 0;
+        HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms = t;
+    });
+}
+HALMD_API int halmd_ensemble_size(const halmd_ensemble* e, unsigned* systems) {
+    return guard([&] { nonnull(systems, "systems"); *systems = as<ensemble>(e)->size(); });
+}
+HALMD_API int halmd_ensemble_thermo(halmd_ensemble* e, double* out) {
+    return guard([&] { nonnull(out, "out"); as<ensemble>(e)->thermo(out); });
+}
+HALMD_API int halmd_ensemble_state(halmd_ensemble* e, unsigned system, double* position, double* velocity,
+                                   int* image) {
+    return guard([&] { as<ensemble>(e)->get_state(system, position, velocity, image); });
+}
+HALMD_API int halmd_ensemble_rebuilds(const halmd_ensemble* e, unsigned system, uint64_t* count) {
+    return guard([&] { nonnull(count, "count"); *count = as<ensemble>(e)->rebuilds(system); });
 }
 
 // ---- profiling ----------------------------------------------------------------------------------

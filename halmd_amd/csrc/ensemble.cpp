@@ -1,0 +1,437 @@
+// Batched ensembles (see ensemble.hpp).
+#include "ensemble.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace halmd {
+
+struct ensemble::sys_host {
+    unsigned n = 0;
+    double L = 0, dt = 0, skin = 0, r_cut = 0;
+    int kind = 0, trunc = 0;
+    std::unique_ptr<box> bx;
+    std::unique_ptr<potential> pot;
+    std::vector<double> r, v;            // initial state until commit
+    // layout
+    unsigned base = 0, blk_base = 0, nblk = 0, cell_base = 0, ncells = 0, ncell = 0;
+    bool small = false;
+    unsigned S = 0, nsx = 0, nseg = 0, cap = 0, lds_cap = 0;
+    size_t list_off = 0;
+    uint64_t rebuilds = 0;
+    ens_desc d{};
+};
+
+namespace {
+unsigned bits_for(size_t v) {   // smallest b with v < 2^b
+    unsigned b = 0;
+    while (b < 32 && (size_t(1) << b) <= v) ++b;
+    return b;
+}
+} // namespace
+
+ensemble::ensemble() { HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)); }
+
+ensemble::~ensemble() { (void)hipStreamDestroy(stream); }
+
+uint64_t ensemble::rebuilds(unsigned s) const {
+    require(s < sys_.size(), "ensemble: system index out of range");
+    return sys_[s]->rebuilds;
+}
+
+unsigned ensemble::add(unsigned n, double length, int kind, int trunc, double epsilon, double sigma,
+                       double r_cut_sigma, double r_min, double distortion, double smoothing, double timestep,
+                       double skin, double const* position, double const* velocity) {
+    require(!committed_, "ensemble: systems are added before commit");
+    require(n >= 2 && n < 65536 * 256, "ensemble: particle count out of range");
+    require(length > 0 && timestep > 0 && skin >= 0, "ensemble: invalid box, timestep or skin");
+    auto h = std::make_unique<sys_host>();
+    h->n = n;
+    h->L = length;
+    h->dt = timestep;
+    h->skin = skin;
+    h->kind = kind;
+    h->trunc = trunc;
+    double const Lv[3] = {length, length, length};
+    h->bx = std::make_unique<box>(Lv);
+    h->pot = std::make_unique<potential>(kind, 1, 1, &epsilon, &sigma, &r_min, &distortion);
+    h->pot->truncate(trunc, &r_cut_sigma, smoothing);
+    h->r_cut = h->pot->r_cut[0];
+    require(h->r_cut + skin < length / 2, "ensemble: the box must exceed twice rc + skin (minimum image)");
+    h->r.assign(position, position + 3 * size_t(n));
+    h->v.assign(velocity, velocity + 3 * size_t(n));
+    sys_.push_back(std::move(h));
+    return unsigned(sys_.size() - 1);
+}
+
+// list capacity and offsets of every system (on commit and after an overflow)
+void ensemble::layout_lists() {
+    size_t off = 0;
+    for (auto& h : sys_) {
+        h->list_off = off;
+        off += size_t(round_up(h->n, WAVE)) * h->cap;
+    }
+    list_words_ = off;
+    list_.resize(list_words_);
+}
+
+void ensemble::upload_descs() {
+    std::vector<ens_desc> d(sys_.size());
+    for (size_t s = 0; s < sys_.size(); ++s) {
+        auto& h = *sys_[s];
+        h.d.cap = h.cap;
+        h.d.list_off = h.list_off;
+        h.d.sp.lds_cap = h.lds_cap;
+        d[s] = h.d;
+    }
+    descs_.resize(d.size());
+    descs_.upload(d.data(), d.size(), stream);
+    HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+void ensemble::commit() {
+    require(!committed_, "ensemble: already committed");
+    require(!sys_.empty(), "ensemble: no systems");
+    require(sys_.size() < 65536, "ensemble: at most 65535 systems");
+    // layout: particles on 256-particle blocks, cells, segments, lists
+    unsigned base = 0, cells = 0;
+    for (auto& hp : sys_) {
+        auto& h = *hp;
+        h.base = base;
+        h.blk_base = base / ENS_BLOCK;
+        h.nblk = unsigned(div_up(h.n, ENS_BLOCK));
+        base += h.nblk * ENS_BLOCK;
+        double const rcs = h.r_cut + h.skin;
+        h.ncell = std::max(1u, unsigned(h.L / rcs));
+        h.small = h.ncell < 3;
+        double const rho = h.n / (h.L * h.L * h.L);
+        double const nmean = 4.0 / 3.0 * M_PI * rcs * rcs * rcs * rho;
+        h.cap = unsigned(round_up(size_t(std::ceil(nmean / 0.5)) + NB16_BLOCK, NB16_BLOCK));
+        if (h.small) {
+            require(h.n <= 4096, "ensemble: a box under 3 cells of rc + skin holds at most 4096 particles");
+            h.cap = std::min(h.cap, unsigned(round_up(h.n, NB16_BLOCK)));
+        } else {
+            h.cell_base = cells;
+            h.ncells = h.ncell * h.ncell * h.ncell;
+            cells += h.ncells + 1;
+            double const occ = double(h.n) / h.ncells;
+            h.S = std::max(1u, std::min({unsigned(std::round(200.0 / std::max(occ, 1e-9))), SEG_MAX, h.ncell - 2}));
+            h.nsx = unsigned(div_up(h.ncell, h.S));
+            h.nseg = h.nsx * h.ncell * h.ncell;
+            require(h.nseg < 65536, "ensemble: too many cell segments in one system");
+            double const expect = 9.0 * (h.S + 2) * occ;
+            h.lds_cap = unsigned(std::min<double>(SEG_MAX_RECORDS, round_up(size_t(1.3 * expect) + 64, 64)));
+        }
+        // descriptor (the list fields are filled by upload_descs)
+        ens_desc& d = h.d;
+        std::memset(&d, 0, sizeof(d));
+        d.base = h.base;
+        d.n = h.n;
+        d.blk_base = h.blk_base;
+        d.cell_base = h.cell_base;
+        d.ncells = h.ncells;
+        d.small = h.small;
+        d.skin_half_sq = float(h.skin * h.skin / 4);
+        d.sp.grid.ncell[0] = d.sp.grid.ncell[1] = d.sp.grid.ncell[2] = h.ncell;
+        d.sp.S = h.S;
+        d.sp.nsx = h.nsx;
+        d.sp.nseg = h.nseg;
+        d.sp.chunk = 0;
+        d.sp.threads = threads_;
+        box const& b = *h.bx;
+        d.np.grid = d.sp.grid;
+        for (int k = 0; k < 3; ++k) d.np.q[k] = b.q_hi[k];
+        d.np.ntype = 1;
+        d.np.rr_cut_skin[0] = float(rcs * rcs);
+        d.np.iso = 1;
+        d.np.rr_cut_skin_q[0] = float(double(d.np.rr_cut_skin[0]) / (double(d.np.q[0]) * double(d.np.q[0])));
+        split_ff(h.dt, d.ip.dt_hi, d.ip.dt_lo);
+        d.ip.half_dt = float(h.dt / 2);
+        for (int k = 0; k < 3; ++k) {
+            d.ip.L_hi[k] = b.L_hi[k];
+            d.ip.L_lo[k] = b.L_lo[k];
+            d.ip.invL[k] = b.invL[k];
+        }
+        d.ip.qp = b.qp;
+        d.P = h.pot->device_params(b);
+    }
+    total_ = base;
+    nblocks_ = base / ENS_BLOCK;
+    cells_total_ = cells;
+    layout_lists();
+
+    // device state
+    size_t const N = total_;
+    pos_.resize(2 * N); vel_.resize(2 * N); pos_t_.resize(2 * N); vel_t_.resize(2 * N);
+    force_.resize(N); aux_force_.resize(N); r0_.resize(N);
+    posq_.resize(N); posq_t_.resize(N); image_.resize(N); image_t_.resize(N);
+    id_.resize(N); id_t_.resize(N); count_.resize(N);
+    stress_.resize(6 * N);
+    offsets_.resize(std::max<size_t>(cells_total_, 1));
+    disp2_.resize(sys_.size()); stats_.resize(2 * sys_.size());
+    thermo_.resize(4 * sys_.size());
+    h_disp2_.resize(sys_.size()); h_stats_.resize(2 * sys_.size()); h_thermo_.resize(4 * sys_.size());
+    keys_.resize(N); vals_.resize(N);
+    ws_.reserve(N);
+    {
+        std::vector<float4> hp(2 * N, make_float4(0, 0, 0, 0)), hv(2 * N, make_float4(0, 0, 0, 0));
+        std::vector<int4> him(N, make_int4(0, 0, 0, 0));
+        std::vector<uint32_t> hid(N, 0);
+        std::vector<uint16_t> bs(nblocks_);
+        for (size_t s = 0; s < sys_.size(); ++s) {
+            auto& h = *sys_[s];
+            for (unsigned b = 0; b < h.nblk; ++b) bs[h.blk_base + b] = uint16_t(s);
+            for (unsigned i = 0; i < h.n; ++i) {
+                size_t const g = h.base + i;
+                float rh[3], rl[3], vh[3], vl[3];
+                for (int k = 0; k < 3; ++k) {
+                    // stored as given, as particle::set_position does (the
+                    // first integrate pass wraps and counts images)
+                    split_ff(h.r[3 * size_t(i) + k], rh[k], rl[k]);
+                    split_ff(h.v[3 * size_t(i) + k], vh[k], vl[k]);
+                }
+                hp[2 * g] = make_float4(rh[0], rh[1], rh[2], 0.0f);   // species 0
+                hp[2 * g + 1] = make_float4(rl[0], rl[1], rl[2], 0.0f);
+                hv[2 * g] = make_float4(vh[0], vh[1], vh[2], 1.0f);   // unit mass
+                hv[2 * g + 1] = make_float4(vl[0], vl[1], vl[2], 0.0f);
+                hid[g] = i;
+            }
+            h.r.clear(); h.r.shrink_to_fit();
+            h.v.clear(); h.v.shrink_to_fit();
+        }
+        pos_.upload(hp.data(), hp.size(), stream);
+        vel_.upload(hv.data(), hv.size(), stream);
+        image_.upload(him.data(), him.size(), stream);
+        id_.upload(hid.data(), hid.size(), stream);
+        block_sys_.resize(nblocks_);
+        block_sys_.upload(bs.data(), bs.size(), stream);
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+    for (auto& h : sys_) launch_quantize(pos_.get() + 2 * size_t(h->base), h->n, h->d.ip.qp, posq_.get() + h->base, stream);
+    upload_descs();
+
+    // force classes: (kind, truncation) with their workgroup tables
+    classes_.clear();
+    for (size_t s = 0; s < sys_.size(); ++s) {
+        auto& h = *sys_[s];
+        auto it = std::find_if(classes_.begin(), classes_.end(),
+                               [&](force_class const& c) { return c.kind == h.kind && c.trunc == h.trunc; });
+        if (it == classes_.end()) {
+            classes_.push_back(force_class{h.kind, h.trunc, {}, {}, 0, 0, 0, 0});
+            it = classes_.end() - 1;
+        }
+        if (h.small) {
+            it->small.push_back(uint32_t(s));
+            it->small_max_n = std::max(it->small_max_n, h.n);
+        } else {
+            for (unsigned g = 0; g < h.nseg; ++g) it->seg.push_back(uint32_t(s) << 16 | g);
+        }
+    }
+    std::vector<uint32_t> ft;
+    for (auto& c : classes_) {
+        c.seg_off = ft.size();
+        ft.insert(ft.end(), c.seg.begin(), c.seg.end());
+        c.small_off = ft.size();
+        ft.insert(ft.end(), c.small.begin(), c.small.end());
+    }
+    fclass_tables_.resize(std::max<size_t>(ft.size(), 1));
+    fclass_tables_.upload(ft.data(), ft.size(), stream);
+    HIP_CHECK(hipStreamSynchronize(stream));
+
+    committed_ = true;
+    std::vector<unsigned> all(sys_.size());
+    for (size_t s = 0; s < all.size(); ++s) all[s] = unsigned(s);
+    rebuild(all);
+    forces(false, force_, nullptr);
+    pending_ = false;
+}
+
+void ensemble::rebuild(std::vector<unsigned> const& R0) {
+    std::vector<unsigned> R = R0;
+    while (true) {
+        std::vector<unsigned> large, small;
+        for (unsigned s : R) (sys_[s]->small ? small : large).push_back(s);
+        // tables: rsys, rpos, rblk (cell systems), rseg, rsmall, rblk_small
+        std::vector<uint32_t> rsys, rpos, rblk, rseg, rblk_small;
+        size_t npos = 0, max_ncells = 0;
+        unsigned lds = 64, small_max_n = 0;
+        for (size_t k = 0; k < large.size(); ++k) {
+            auto& h = *sys_[large[k]];
+            rsys.push_back(large[k]);
+            rpos.push_back(uint32_t(npos));
+            npos += h.n;
+            max_ncells = std::max<size_t>(max_ncells, h.ncells);
+            for (unsigned b = 0; b < h.nblk; ++b) rblk.push_back(uint32_t(k) << 16 | b);
+            for (unsigned g = 0; g < h.nseg; ++g) rseg.push_back(uint32_t(large[k]) << 16 | g);
+            lds = std::max(lds, h.lds_cap);
+        }
+        for (unsigned s : small) {
+            auto& h = *sys_[s];
+            small_max_n = std::max(small_max_n, h.n);
+            for (unsigned b = 0; b < h.nblk; ++b) rblk_small.push_back(uint32_t(s) << 16 | b);
+        }
+        unsigned const cell_bits = bits_for(max_ncells);
+        unsigned const key_bits = cell_bits + bits_for(large.size() ? large.size() - 1 : 0);
+        require(key_bits <= 32, "ensemble: too many cells to sort in one pass");
+        size_t const o_rsys = 0, o_rpos = o_rsys + rsys.size(), o_rblk = o_rpos + rpos.size(),
+                     o_rseg = o_rblk + rblk.size(), o_small = o_rseg + rseg.size(),
+                     o_rbs = o_small + small.size(), total = o_rbs + rblk_small.size();
+        h_tables_.resize(std::max(h_tables_.size(), total + 1));
+        tables_.reserve(total + 1);
+        uint32_t* t = h_tables_.get();
+        std::copy(rsys.begin(), rsys.end(), t + o_rsys);
+        std::copy(rpos.begin(), rpos.end(), t + o_rpos);
+        std::copy(rblk.begin(), rblk.end(), t + o_rblk);
+        std::copy(rseg.begin(), rseg.end(), t + o_rseg);
+        std::copy(small.begin(), small.end(), t + o_small);
+        std::copy(rblk_small.begin(), rblk_small.end(), t + o_rbs);
+        if (total) tables_.upload(t, total, stream);
+        uint32_t* const T = tables_.get();
+
+        launch_fill_u32(stats_, 0u, 2 * sys_.size(), stream);
+        if (!large.empty()) {
+            launch_ens_cells(posq_, descs_, T + o_rsys, T + o_rpos, T + o_rblk, unsigned(rblk.size()), cell_bits,
+                             keys_, vals_, stream);
+            radix_sort(keys_, vals_, npos, std::max(1u, key_bits), ws_, stream);
+            launch_ens_offsets(descs_, T + o_rsys, T + o_rpos, unsigned(rsys.size()), cell_bits, keys_, offsets_,
+                               stream);
+            launch_ens_reorder(pos_, vel_, image_, id_, posq_, r0_, pos_t_, vel_t_, image_t_, id_t_, posq_t_, descs_,
+                               T + o_rsys, T + o_rpos, T + o_rblk, unsigned(rblk.size()), vals_, stream);
+            launch_ens_build(posq_, offsets_, descs_, T + o_rseg, unsigned(rseg.size()), threads_, lds, list_,
+                             count_, stats_, stream);
+        }
+        if (!small.empty()) {
+            launch_ens_r0(pos_, r0_, descs_, T + o_rbs, unsigned(rblk_small.size()), stream);
+            launch_ens_build_small(posq_, descs_, T + o_small, unsigned(small.size()), small_max_n, list_, count_,
+                                   stats_, stream);
+        }
+        stats_.download(h_stats_.get(), 2 * sys_.size(), stream);
+        HIP_CHECK(hipStreamSynchronize(stream));
+        // capacity checks: LDS staging first (nothing built), then list size
+        bool relayout = false, redesc = false;
+        std::vector<unsigned> again;
+        for (unsigned s : R) {
+            auto& h = *sys_[s];
+            unsigned const m = h_stats_[2 * s], u = h_stats_[2 * s + 1];
+            if (!h.small && u > h.lds_cap) {
+                require(u <= SEG_MAX_RECORDS, "ensemble: a cell neighbourhood exceeds the LDS staging limit");
+                h.lds_cap = unsigned(std::min<size_t>(SEG_MAX_RECORDS, round_up(size_t(u) + u / 8, 64)));
+                redesc = true;
+                again.push_back(s);
+            } else if (m > h.cap) {
+                h.cap = unsigned(round_up(size_t(m) + m / 8 + NB16_BLOCK, NB16_BLOCK));
+                relayout = true;
+            }
+        }
+        if (relayout) {
+            // new list offsets for every system: all lists are rebuilt
+            layout_lists();
+            upload_descs();
+            R.resize(sys_.size());
+            for (size_t s = 0; s < R.size(); ++s) R[s] = unsigned(s);
+            continue;
+        }
+        if (redesc) {
+            upload_descs();
+            R = again;
+            continue;
+        }
+        break;
+    }
+    for (unsigned s : R0) ++sys_[s]->rebuilds;
+    // the force kernels stage at most the largest staging of their class
+    for (auto& c : classes_) {
+        c.lds_records = 64;
+        for (uint32_t e : c.seg) c.lds_records = std::max(c.lds_records, sys_[e >> 16]->lds_cap);
+    }
+}
+
+void ensemble::forces(bool aux, float4* force, float* stress) {
+    for (auto& c : classes_)
+        launch_ens_force(c.kind, c.trunc, aux, posq_, offsets_, descs_, fclass_tables_.get() + c.seg_off,
+                         unsigned(c.seg.size()), threads_, c.lds_records, fclass_tables_.get() + c.small_off,
+                         unsigned(c.small.size()), c.small_max_n, list_, count_, force, stress, total_, stream);
+}
+
+void ensemble::settle() {
+    if (!pending_) return;
+    launch_ens_settle(vel_, force_, descs_, block_sys_, nblocks_, stream);
+    pending_ = false;
+}
+
+void ensemble::step(unsigned steps) {
+    require(committed_, "ensemble: commit() first");
+    std::vector<unsigned> R;
+    for (unsigned k = 0; k < steps; ++k) {
+        launch_fill_u32(disp2_, 0u, sys_.size(), stream);
+        launch_ens_integrate(pos_, vel_, force_, image_, posq_, r0_, descs_, block_sys_, nblocks_, pending_, disp2_,
+                             stream);
+        pending_ = false;
+        disp2_.download(h_disp2_.get(), sys_.size(), stream);
+        HIP_CHECK(hipStreamSynchronize(stream));
+        R.clear();
+        for (size_t s = 0; s < sys_.size(); ++s) {
+            // the single-system rule: rebuild when max |r - r0| > skin / 2
+            float d2;
+            uint32_t const bits = h_disp2_[s];
+            std::memcpy(&d2, &bits, 4);
+            if (std::sqrt(double(d2)) > sys_[s]->skin / 2) R.push_back(unsigned(s));
+        }
+        if (!R.empty()) rebuild(R);
+        forces(false, force_, nullptr);
+        pending_ = true;   // the finalize kick, folded into the next integrate
+        ++steps_done;
+    }
+}
+
+void ensemble::thermo(double* out) {
+    require(committed_, "ensemble: commit() first");
+    forces(true, aux_force_, stress_);
+    std::vector<uint32_t> all(sys_.size());
+    for (size_t s = 0; s < all.size(); ++s) all[s] = uint32_t(s);
+    tables_.reserve(all.size());
+    tables_.upload(all.data(), all.size(), stream);
+    launch_ens_thermo(vel_, force_, aux_force_, stress_, total_, descs_, tables_, unsigned(all.size()), pending_,
+                      thermo_, stream);
+    thermo_.download(out, 4 * sys_.size(), stream);
+    HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+void ensemble::get_state(unsigned s, double* position, double* velocity, int* image) {
+    require(committed_ && s < sys_.size(), "ensemble: system index out of range");
+    settle();
+    auto& h = *sys_[s];
+    size_t const n = h.n;
+    std::vector<float4> hp(2 * n), hv(2 * n);
+    std::vector<int4> him(n);
+    std::vector<uint32_t> hid(n);
+    HIP_CHECK(hipMemcpyAsync(hp.data(), pos_.get() + 2 * size_t(h.base), 2 * n * sizeof(float4),
+                             hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipMemcpyAsync(hv.data(), vel_.get() + 2 * size_t(h.base), 2 * n * sizeof(float4),
+                             hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipMemcpyAsync(him.data(), image_.get() + h.base, n * sizeof(int4), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipMemcpyAsync(hid.data(), id_.get() + h.base, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    for (size_t i = 0; i < n; ++i) {
+        size_t const t = hid[i];   // particle id = index at add()
+        require(t < n, "ensemble: corrupt particle ids");
+        if (position) {
+            position[3 * t] = double(hp[2 * i].x) + hp[2 * i + 1].x;
+            position[3 * t + 1] = double(hp[2 * i].y) + hp[2 * i + 1].y;
+            position[3 * t + 2] = double(hp[2 * i].z) + hp[2 * i + 1].z;
+        }
+        if (velocity) {
+            velocity[3 * t] = double(hv[2 * i].x) + hv[2 * i + 1].x;
+            velocity[3 * t + 1] = double(hv[2 * i].y) + hv[2 * i + 1].y;
+            velocity[3 * t + 2] = double(hv[2 * i].z) + hv[2 * i + 1].z;
+        }
+        if (image) {
+            image[3 * t] = him[i].x;
+            image[3 * t + 1] = him[i].y;
+            image[3 * t + 2] = him[i].z;
+        }
+    }
+}
+
+} // namespace halmd

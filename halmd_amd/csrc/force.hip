@@ -216,18 +216,16 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
 //  - MULTI: pair parameters by species from LDS; otherwise the one parameter
 //    set lives in scalar registers.
 // Summation: FP32 over blocks of 16 pair forces, Kahan over the blocks.
+// One cell segment of the staged force: the whole workgroup calls it for the
+// segment `seg` of the system described by (posq, off, sp, list, count, cap,
+// P); shared by the single-system kernel and the ensemble kernel.
 template <int KIND, int TRUNC, bool AUX, bool ISO, bool MULTI>
-__global__ __launch_bounds__(SEG_THREADS_MAX) void pair_force_staged_kernel(
-    uint4 const* __restrict__ posq, uint32_t const* __restrict__ off, seg_params sp,
-    uint16_t const* __restrict__ list, uint32_t const* __restrict__ count, unsigned cap, force_params P,
-    bool force_zero, float aux_weight, float4* __restrict__ force, float* __restrict__ stress, unsigned stride)
+__device__ __forceinline__ void staged_force_segment(
+    uint4 const* __restrict__ posq, uint32_t const* __restrict__ off, seg_params const& sp, unsigned seg,
+    uint16_t const* __restrict__ list, uint32_t const* __restrict__ count, unsigned cap, force_params const& P,
+    bool force_zero, float aux_weight, float4* __restrict__ force, float* __restrict__ stress, unsigned stride,
+    uint4* s_rec, seg_stage& st, pair_param* s_pair)
 {
-    extern __shared__ uint4 s_rec[];
-    __shared__ seg_stage st;
-    __shared__ pair_param s_pair[MAX_TYPES * MAX_TYPES];
-
-    unsigned const seg = seg_of_block(sp, blockIdx.x);
-    if (seg >= sp.nseg) return;   // the whole workgroup
     if (MULTI && threadIdx.x < MAX_TYPES * MAX_TYPES) s_pair[threadIdx.x] = P.pair[threadIdx.x];
     stage_union(sp, seg, off, posq, s_rec, st, nullptr);   // fits: the list build checked it
 
@@ -342,6 +340,101 @@ __global__ __launch_bounds__(SEG_THREADS_MAX) void pair_force_staged_kernel(
         }
     }
 }
+template <int KIND, int TRUNC, bool AUX, bool ISO, bool MULTI>
+__global__ __launch_bounds__(SEG_THREADS_MAX) void pair_force_staged_kernel(
+    uint4 const* __restrict__ posq, uint32_t const* __restrict__ off, seg_params sp,
+    uint16_t const* __restrict__ list, uint32_t const* __restrict__ count, unsigned cap, force_params P,
+    bool force_zero, float aux_weight, float4* __restrict__ force, float* __restrict__ stress, unsigned stride)
+{
+    extern __shared__ uint4 s_rec[];
+    __shared__ seg_stage st;
+    __shared__ pair_param s_pair[MAX_TYPES * MAX_TYPES];
+
+    unsigned const seg = seg_of_block(sp, blockIdx.x);
+    if (seg >= sp.nseg) return;   // the whole workgroup
+    staged_force_segment<KIND, TRUNC, AUX, ISO, MULTI>(posq, off, sp, seg, list, count, cap, P, force_zero,
+                                                       aux_weight, force, stress, stride, s_rec, st, s_pair);
+}
+
+
+// ensembles: one workgroup per (system, segment) of a (kind, truncation)
+// class; the per-system parameters come from the descriptor table
+template <int KIND, int TRUNC, bool AUX>
+__global__ __launch_bounds__(SEG_THREADS_MAX) void ens_force_kernel(
+    uint4 const* __restrict__ posq, uint32_t const* __restrict__ offsets, ens_desc const* __restrict__ descs,
+    uint32_t const* __restrict__ fseg, uint16_t const* __restrict__ list, uint32_t const* __restrict__ count,
+    float4* __restrict__ force, float* __restrict__ stress, unsigned stride)
+{
+    extern __shared__ uint4 s_rec[];
+    __shared__ seg_stage st;
+    __shared__ pair_param s_pair[MAX_TYPES * MAX_TYPES];
+    uint32_t const e = fseg[blockIdx.x];
+    unsigned const sys = e >> 16, seg = e & 0xffffu;
+    ens_desc const& d = descs[sys];
+    staged_force_segment<KIND, TRUNC, AUX, true, false>(
+        posq + d.base, offsets + d.cell_base, d.sp, seg, list + d.list_off, count + d.base, d.cap, d.P, true, 0.5f,
+        force + d.base, AUX ? stress + d.base : nullptr, stride, s_rec, st, s_pair);
+}
+
+// all-pairs systems: one workgroup per system, the system in LDS; list
+// entries are particle indices (the arithmetic of the cubic-box path)
+template <int KIND, int TRUNC, bool AUX>
+__global__ __launch_bounds__(256) void ens_force_small_kernel(
+    uint4 const* __restrict__ posq, ens_desc const* __restrict__ descs, uint32_t const* __restrict__ fsmall,
+    uint16_t const* __restrict__ list, uint32_t const* __restrict__ count, float4* __restrict__ force,
+    float* __restrict__ stress, unsigned stride)
+{
+    extern __shared__ uint4 s_rec[];
+    ens_desc const& d = descs[fsmall[blockIdx.x]];
+    for (unsigned t = threadIdx.x; t < d.n; t += blockDim.x) s_rec[t] = posq[d.base + t];
+    __syncthreads();
+    force_params const& P = d.P;
+    pair_param const& pp = P.pair[0];
+    uint16_t const* const L = list + d.list_off;
+    for (unsigned i = threadIdx.x; i < d.n; i += blockDim.x) {
+        uint4 const qi = s_rec[i];
+        unsigned const cnt = count[d.base + i];
+        uint16_t const* const lp = L + size_t(i & ~(WAVE - 1)) * d.cap + (i & (WAVE - 1)) * NB16_BLOCK;
+        float fx = 0, fy = 0, fz = 0, cx = 0, cy = 0, cz = 0, bx = 0, by = 0, bz = 0;
+        float en = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
+        for (unsigned k = 0; k < cnt; ++k) {
+            uint4 const qj = s_rec[lp[size_t(k / NB16_BLOCK) * (WAVE * NB16_BLOCK) + k % NB16_BLOCK]];
+            float const dx = float(int32_t(qi.x - qj.x)), dy = float(int32_t(qi.y - qj.y)),
+                        dz = float(int32_t(qi.z - qj.z));
+            float const rq = dx * dx + dy * dy + dz * dz;
+            bool const inside = rq < pp.rr_cut_q;
+            float const rr = __builtin_fmaf(rq, P.q2[0], rq * P.q2[1]);
+            float const md = KIND == POTENTIAL_MORSE
+                ? morse_argument(int32_t(qi.x - qj.x), int32_t(qi.y - qj.y), int32_t(qi.z - qj.z), P, pp)
+                : 0.0f;
+            float fval, u;
+            pair_eval<KIND, TRUNC>(rr, md, pp, P.rri_smooth, fval, u);
+            fval = inside ? fval : 0.0f;
+            bx += fval * dx;
+            by += fval * dy;
+            bz += fval * dz;
+            if (AUX) {
+                en += inside ? u : 0.0f;
+                s0 += fval * dx * dx; s1 += fval * dy * dy; s2 += fval * dz * dz;
+                s3 += fval * dx * dy; s4 += fval * dx * dz; s5 += fval * dy * dz;
+            }
+            if (k % NB16_BLOCK == NB16_BLOCK - 1 || k + 1 == cnt) {
+                kahan_add(fx, cx, bx);
+                kahan_add(fy, cy, by);
+                kahan_add(fz, cz, bz);
+                bx = by = bz = 0;
+            }
+        }
+        float const qh = P.q[0], ql = P.q_lo[0];
+        force[d.base + i] = make_float4(separation_scale(fx - cx, qh, ql), separation_scale(fy - cy, qh, ql),
+                                        separation_scale(fz - cz, qh, ql), AUX ? 0.5f * en : 0.0f);
+        if (AUX) {
+            float const sv[6] = {s0, s1, s2, s3, s4, s5};
+            for (int c = 0; c < 6; ++c)
+                stress[c * size_t(stride) + d.base + i] = __builtin_fmaf(sv[c], 0.5f * P.q2[0], sv[c] * (0.5f * P.q2[1]));
+        }
+    }
+}
 
 struct force_launch {
     uint4 const* posq;
@@ -452,6 +545,77 @@ void launch_pair_force_staged(uint4 const* posq, uint32_t const* offset, seg_par
         staged_trunc<POTENTIAL_LENNARD_JONES>(a);
     else if (p.kind == POTENTIAL_MORSE)
         staged_trunc<POTENTIAL_MORSE>(a);
+    else
+        throw error(HALMD_ERR_INVALID_ARGUMENT, "unknown potential");
+    HIP_CHECK(hipGetLastError());
+}
+
+namespace {
+template <int KIND, int TRUNC, bool AUX>
+void ens_force_go(uint4 const* posq, uint32_t const* offsets, ens_desc const* descs, uint32_t const* fseg,
+                  unsigned nfseg, unsigned threads, unsigned lds_records, uint32_t const* fsmall, unsigned nsmall,
+                  unsigned small_max_n, uint16_t const* list, uint32_t const* count, float4* force, float* stress,
+                  unsigned stride, hipStream_t s) {
+    if (nfseg)
+        ens_force_kernel<KIND, TRUNC, AUX><<<nfseg, threads, size_t(lds_records) * sizeof(uint4), s>>>(
+            posq, offsets, descs, fseg, list, count, force, stress, stride);
+    if (nsmall)
+        ens_force_small_kernel<KIND, TRUNC, AUX><<<nsmall, 256, size_t(small_max_n) * sizeof(uint4), s>>>(
+            posq, descs, fsmall, list, count, force, stress, stride);
+}
+
+template <int KIND, int TRUNC>
+void ens_force_aux(bool aux, uint4 const* posq, uint32_t const* offsets, ens_desc const* descs, uint32_t const* fseg,
+                   unsigned nfseg, unsigned threads, unsigned lds_records, uint32_t const* fsmall, unsigned nsmall,
+                   unsigned small_max_n, uint16_t const* list, uint32_t const* count, float4* force, float* stress,
+                   unsigned stride, hipStream_t s) {
+    if (aux)
+        ens_force_go<KIND, TRUNC, true>(posq, offsets, descs, fseg, nfseg, threads, lds_records, fsmall, nsmall,
+                                        small_max_n, list, count, force, stress, stride, s);
+    else
+        ens_force_go<KIND, TRUNC, false>(posq, offsets, descs, fseg, nfseg, threads, lds_records, fsmall, nsmall,
+                                         small_max_n, list, count, force, stress, stride, s);
+}
+
+template <int KIND>
+void ens_force_trunc(int trunc, bool aux, uint4 const* posq, uint32_t const* offsets, ens_desc const* descs,
+                     uint32_t const* fseg, unsigned nfseg, unsigned threads, unsigned lds_records,
+                     uint32_t const* fsmall, unsigned nsmall, unsigned small_max_n, uint16_t const* list,
+                     uint32_t const* count, float4* force, float* stress, unsigned stride, hipStream_t s) {
+    switch (trunc) {
+    case TRUNC_SHARP:
+        ens_force_aux<KIND, TRUNC_SHARP>(aux, posq, offsets, descs, fseg, nfseg, threads, lds_records, fsmall, nsmall,
+                                         small_max_n, list, count, force, stress, stride, s);
+        break;
+    case TRUNC_SHIFTED:
+        ens_force_aux<KIND, TRUNC_SHIFTED>(aux, posq, offsets, descs, fseg, nfseg, threads, lds_records, fsmall,
+                                           nsmall, small_max_n, list, count, force, stress, stride, s);
+        break;
+    case TRUNC_FORCE_SHIFTED:
+        ens_force_aux<KIND, TRUNC_FORCE_SHIFTED>(aux, posq, offsets, descs, fseg, nfseg, threads, lds_records, fsmall,
+                                                 nsmall, small_max_n, list, count, force, stress, stride, s);
+        break;
+    case TRUNC_SMOOTH_R4:
+        ens_force_aux<KIND, TRUNC_SMOOTH_R4>(aux, posq, offsets, descs, fseg, nfseg, threads, lds_records, fsmall,
+                                             nsmall, small_max_n, list, count, force, stress, stride, s);
+        break;
+    default: throw error(HALMD_ERR_INVALID_ARGUMENT, "unknown truncation");
+    }
+}
+} // namespace
+
+void launch_ens_force(int kind, int trunc, bool aux, uint4 const* posq, uint32_t const* offsets,
+                      ens_desc const* descs, uint32_t const* fseg, unsigned nfseg, unsigned threads,
+                      unsigned lds_records, uint32_t const* fsmall, unsigned nsmall, unsigned small_max_n,
+                      uint16_t const* list, uint32_t const* count, float4* force, float* stress, unsigned stride,
+                      hipStream_t s) {
+    if (!nfseg && !nsmall) return;
+    if (kind == POTENTIAL_LENNARD_JONES)
+        ens_force_trunc<POTENTIAL_LENNARD_JONES>(trunc, aux, posq, offsets, descs, fseg, nfseg, threads, lds_records,
+                                                 fsmall, nsmall, small_max_n, list, count, force, stress, stride, s);
+    else if (kind == POTENTIAL_MORSE)
+        ens_force_trunc<POTENTIAL_MORSE>(trunc, aux, posq, offsets, descs, fseg, nfseg, threads, lds_records, fsmall,
+                                         nsmall, small_max_n, list, count, force, stress, stride, s);
     else
         throw error(HALMD_ERR_INVALID_ARGUMENT, "unknown potential");
     HIP_CHECK(hipGetLastError());

@@ -64,6 +64,66 @@ __device__ __forceinline__ ff ff_from_double(double x) {
     return ff{hi, float(x - double(hi))};
 }
 
+// Kick (optionally after the previous step's deferred finalize kick and a
+// velocity scale), drift, wrap, image update and fixed-point snapshot of
+// particle i; returns |r - r0|^2 (0 without r0).  Shared by the single-system
+// and the ensemble integrate kernels.
+__device__ __forceinline__ float integrate_particle(float4* __restrict__ pos, float4* __restrict__ vel,
+                                                   float4 const* __restrict__ force, int4* __restrict__ image,
+                                                   uint4* __restrict__ posq, unsigned i, integrate_params const& p,
+                                                   float pre_half_dt, double const* __restrict__ scale,
+                                                   float4 const* __restrict__ r0) {
+    float d2 = 0.0f;
+    float4 rh = pos[2 * size_t(i)], rl = pos[2 * size_t(i) + 1];
+    float4 vh = vel[2 * size_t(i)], vl = vel[2 * size_t(i) + 1];
+    float4 const f = force[i];
+    float const m = vh.w;
+    ff vx{vh.x, vl.x}, vy{vh.y, vl.y}, vz{vh.z, vl.z};
+    if (pre_half_dt != 0.0f) {
+        // finalize of the previous step, as verlet_finalize_kernel computes it
+        vx = ff_add_f(vx, f.x * pre_half_dt / m);
+        vy = ff_add_f(vy, f.y * pre_half_dt / m);
+        vz = ff_add_f(vz, f.z * pre_half_dt / m);
+    }
+    if (scale) {
+        ff const s = ff_from_double(*scale);
+        vx = ff_mul(vx, s); vy = ff_mul(vy, s); vz = ff_mul(vz, s);
+    }
+    vx = ff_add_f(vx, f.x * p.half_dt / m);
+    vy = ff_add_f(vy, f.y * p.half_dt / m);
+    vz = ff_add_f(vz, f.z * p.half_dt / m);
+    ff const dt{p.dt_hi, p.dt_lo};
+    ff rx = ff_add(ff{rh.x, rl.x}, ff_mul(vx, dt));
+    ff ry = ff_add(ff{rh.y, rl.y}, ff_mul(vy, dt));
+    ff rz = ff_add(ff{rh.z, rl.z}, ff_mul(vz, dt));
+    float const ix = rintf(rx.hi * p.invL[0]);
+    float const iy = rintf(ry.hi * p.invL[1]);
+    float const iz = rintf(rz.hi * p.invL[2]);
+    if (ix != 0.0f) { rx = ff_add_f(rx, -ix * p.L_hi[0]); rx = ff_add_f(rx, -ix * p.L_lo[0]); }
+    if (iy != 0.0f) { ry = ff_add_f(ry, -iy * p.L_hi[1]); ry = ff_add_f(ry, -iy * p.L_lo[1]); }
+    if (iz != 0.0f) { rz = ff_add_f(rz, -iz * p.L_hi[2]); rz = ff_add_f(rz, -iz * p.L_lo[2]); }
+    float4 const nh = make_float4(rx.hi, ry.hi, rz.hi, rh.w), nl = make_float4(rx.lo, ry.lo, rz.lo, 0.0f);
+    pos[2 * size_t(i)] = nh;
+    pos[2 * size_t(i) + 1] = nl;
+    posq[i] = quantize4(nh, nl, p.qp);
+    vel[2 * size_t(i)] = make_float4(vx.hi, vy.hi, vz.hi, m);
+    vel[2 * size_t(i) + 1] = make_float4(vx.lo, vy.lo, vz.lo, 0.0f);
+    if (ix != 0.0f || iy != 0.0f || iz != 0.0f) {
+        int4 im = image[i];
+        im.x += int(ix); im.y += int(iy); im.z += int(iz);
+        image[i] = im;
+    }
+    if (r0) {
+        float4 const q = r0[i];
+        float dx = rx.hi - q.x, dy = ry.hi - q.y, dz = rz.hi - q.z;
+        dx -= rintf(dx * p.invL[0]) * p.L_hi[0];
+        dy -= rintf(dy * p.invL[1]) * p.L_hi[1];
+        dz -= rintf(dz * p.invL[2]) * p.L_hi[2];
+        d2 = dx * dx + dy * dy + dz * dz;
+    }
+    return d2;
+}
+
 __global__ __launch_bounds__(256) void verlet_integrate_kernel(
     float4* __restrict__ pos, float4* __restrict__ vel, float4 const* __restrict__ force, int4* __restrict__ image,
     uint4* __restrict__ posq, unsigned n, integrate_params p, float pre_half_dt, double const* __restrict__ scale,
@@ -72,55 +132,7 @@ __global__ __launch_bounds__(256) void verlet_integrate_kernel(
     unsigned const i = blockIdx.x * blockDim.x + threadIdx.x;
     if (clear_slot && i == 0) *clear_slot = 0u;
     float d2 = 0.0f;
-    if (i < n) {
-        float4 rh = pos[2 * size_t(i)], rl = pos[2 * size_t(i) + 1];
-        float4 vh = vel[2 * size_t(i)], vl = vel[2 * size_t(i) + 1];
-        float4 const f = force[i];
-        float const m = vh.w;
-        ff vx{vh.x, vl.x}, vy{vh.y, vl.y}, vz{vh.z, vl.z};
-        if (pre_half_dt != 0.0f) {
-            // finalize of the previous step, as verlet_finalize_kernel computes it
-            vx = ff_add_f(vx, f.x * pre_half_dt / m);
-            vy = ff_add_f(vy, f.y * pre_half_dt / m);
-            vz = ff_add_f(vz, f.z * pre_half_dt / m);
-        }
-        if (scale) {
-            ff const s = ff_from_double(*scale);
-            vx = ff_mul(vx, s); vy = ff_mul(vy, s); vz = ff_mul(vz, s);
-        }
-        vx = ff_add_f(vx, f.x * p.half_dt / m);
-        vy = ff_add_f(vy, f.y * p.half_dt / m);
-        vz = ff_add_f(vz, f.z * p.half_dt / m);
-        ff const dt{p.dt_hi, p.dt_lo};
-        ff rx = ff_add(ff{rh.x, rl.x}, ff_mul(vx, dt));
-        ff ry = ff_add(ff{rh.y, rl.y}, ff_mul(vy, dt));
-        ff rz = ff_add(ff{rh.z, rl.z}, ff_mul(vz, dt));
-        float const ix = rintf(rx.hi * p.invL[0]);
-        float const iy = rintf(ry.hi * p.invL[1]);
-        float const iz = rintf(rz.hi * p.invL[2]);
-        if (ix != 0.0f) { rx = ff_add_f(rx, -ix * p.L_hi[0]); rx = ff_add_f(rx, -ix * p.L_lo[0]); }
-        if (iy != 0.0f) { ry = ff_add_f(ry, -iy * p.L_hi[1]); ry = ff_add_f(ry, -iy * p.L_lo[1]); }
-        if (iz != 0.0f) { rz = ff_add_f(rz, -iz * p.L_hi[2]); rz = ff_add_f(rz, -iz * p.L_lo[2]); }
-        float4 const nh = make_float4(rx.hi, ry.hi, rz.hi, rh.w), nl = make_float4(rx.lo, ry.lo, rz.lo, 0.0f);
-        pos[2 * size_t(i)] = nh;
-        pos[2 * size_t(i) + 1] = nl;
-        posq[i] = quantize4(nh, nl, p.qp);
-        vel[2 * size_t(i)] = make_float4(vx.hi, vy.hi, vz.hi, m);
-        vel[2 * size_t(i) + 1] = make_float4(vx.lo, vy.lo, vz.lo, 0.0f);
-        if (ix != 0.0f || iy != 0.0f || iz != 0.0f) {
-            int4 im = image[i];
-            im.x += int(ix); im.y += int(iy); im.z += int(iz);
-            image[i] = im;
-        }
-        if (r0) {
-            float4 const q = r0[i];
-            float dx = rx.hi - q.x, dy = ry.hi - q.y, dz = rz.hi - q.z;
-            dx -= rintf(dx * p.invL[0]) * p.L_hi[0];
-            dy -= rintf(dy * p.invL[1]) * p.L_hi[1];
-            dz -= rintf(dz * p.invL[2]) * p.L_hi[2];
-            d2 = dx * dx + dy * dy + dz * dz;
-        }
-    }
+    if (i < n) d2 = integrate_particle(pos, vel, force, image, posq, i, p, pre_half_dt, scale, r0);
     if (max_disp2) block_max_to(d2, max_disp2);
 }
 
@@ -217,6 +229,41 @@ __global__ __launch_bounds__(256) void andersen_finalize_kernel(float4* __restri
     }
 }
 
+// ensembles: block b serves system block_sys[b] (systems start on block
+// boundaries); one displacement maximum per system
+__global__ __launch_bounds__(ENS_BLOCK) void ens_integrate_kernel(
+    float4* __restrict__ pos, float4* __restrict__ vel, float4 const* __restrict__ force, int4* __restrict__ image,
+    uint4* __restrict__ posq, float4 const* __restrict__ r0, ens_desc const* __restrict__ descs,
+    uint16_t const* __restrict__ block_sys, bool pending, uint32_t* __restrict__ disp2)
+{
+    unsigned const s = block_sys[blockIdx.x];
+    ens_desc const& d = descs[s];
+    unsigned const li = (blockIdx.x - d.blk_base) * ENS_BLOCK + threadIdx.x;
+    float d2 = 0.0f;
+    if (li < d.n)
+        d2 = integrate_particle(pos, vel, force, image, posq, d.base + li, d.ip, pending ? d.ip.half_dt : 0.0f,
+                                nullptr, r0);
+    block_max_to(d2, disp2 + s);
+}
+
+// apply the deferred finalize kick of every system (before velocities are read)
+__global__ __launch_bounds__(ENS_BLOCK) void ens_settle_kernel(float4* __restrict__ vel, float4 const* __restrict__ force,
+                                                               ens_desc const* __restrict__ descs,
+                                                               uint16_t const* __restrict__ block_sys) {
+    ens_desc const& d = descs[block_sys[blockIdx.x]];
+    unsigned const li = (blockIdx.x - d.blk_base) * ENS_BLOCK + threadIdx.x;
+    if (li >= d.n) return;
+    size_t const i = d.base + li;
+    float4 const vh = vel[2 * i], vl = vel[2 * i + 1];
+    float4 const f = force[i];
+    float const m = vh.w;
+    ff const vx = ff_add_f(ff{vh.x, vl.x}, f.x * d.ip.half_dt / m);
+    ff const vy = ff_add_f(ff{vh.y, vl.y}, f.y * d.ip.half_dt / m);
+    ff const vz = ff_add_f(ff{vh.z, vl.z}, f.z * d.ip.half_dt / m);
+    vel[2 * i] = make_float4(vx.hi, vy.hi, vz.hi, m);
+    vel[2 * i + 1] = make_float4(vx.lo, vy.lo, vz.lo, 0.0f);
+}
+
 } // namespace
 
 void launch_verlet_integrate(float4* pos, float4* vel, float4 const* force, int4* image, uint4* posq, unsigned n,
@@ -265,6 +312,22 @@ void launch_andersen_finalize(float4* vel, float4 const* force, uint32_t const* 
     if (!n) return;
     andersen_finalize_kernel<<<div_up(n, 256), 256, 0, s>>>(vel, force, id, n, half_dt, coll_prob, sqrt_temperature,
                                                             seed, step);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_ens_integrate(float4* pos, float4* vel, float4 const* force, int4* image, uint4* posq,
+                          float4 const* r0, ens_desc const* descs, uint16_t const* block_sys, unsigned nblocks,
+                          bool pending, uint32_t* disp2, hipStream_t s) {
+    if (!nblocks) return;
+    ens_integrate_kernel<<<nblocks, ENS_BLOCK, 0, s>>>(pos, vel, force, image, posq, r0, descs, block_sys, pending,
+                                                       disp2);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_ens_settle(float4* vel, float4 const* force, ens_desc const* descs, uint16_t const* block_sys,
+                       unsigned nblocks, hipStream_t s) {
+    if (!nblocks) return;
+    ens_settle_kernel<<<nblocks, ENS_BLOCK, 0, s>>>(vel, force, descs, block_sys);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -232,6 +232,72 @@ void launch_andersen_finalize(float4* vel, float4 const* force, uint32_t const* 
                               float coll_prob, float sqrt_temperature, uint64_t seed, uint64_t step, hipStream_t s);
 
 // ---------------------------------------------------------------------------
+// ensembles: many independent systems in shared arrays (ensemble.cpp drives
+// these).  System s owns particles [base, base + n) (base a multiple of 256),
+// cell offsets [cell_base, cell_base + ncells] (particle indices relative to
+// base), list entries from list_off (uint16 units, per-system capacity cap).
+// Each kernel runs all systems (or a listed subset) in one launch.
+// ---------------------------------------------------------------------------
+constexpr unsigned ENS_BLOCK = 256;          // particles per integrate block
+struct ens_desc {
+    unsigned base, n, blk_base;              // first particle, particles, first integrate block
+    unsigned cell_base, ncells;
+    unsigned small;                          // all-pairs lists (fewer than 3 cells per dimension)
+    unsigned cap;                            // list entries per particle
+    unsigned pad_;
+    unsigned long long list_off;
+    float skin_half_sq;                      // rebuild when |r - r0|^2 exceeds it
+    seg_params sp;
+    neighbour_params np;
+    integrate_params ip;
+    force_params P;
+};
+// step part 1: deferred kick (if pending), kick, drift, snapshot; per-system
+// maximum displacement (float bits) into disp2[s]
+void launch_ens_integrate(float4* pos, float4* vel, float4 const* force, int4* image, uint4* posq,
+                          float4 const* r0, ens_desc const* descs, uint16_t const* block_sys, unsigned nblocks,
+                          bool pending, uint32_t* disp2, hipStream_t s);
+// the deferred finalize kick of every system, applied now
+void launch_ens_settle(float4* vel, float4 const* force, ens_desc const* descs, uint16_t const* block_sys,
+                       unsigned nblocks, hipStream_t s);
+// rebuild of the listed systems.  rblk: per 256-particle block of a listed
+// cell system, (rank << 16) | block; rpos[rank]: its first compacted slot
+void launch_ens_cells(uint4 const* posq, ens_desc const* descs, uint32_t const* rsys, uint32_t const* rpos,
+                      uint32_t const* rblk, unsigned nrblk, unsigned cell_bits, uint32_t* keys, uint32_t* vals,
+                      hipStream_t s);
+void launch_ens_offsets(ens_desc const* descs, uint32_t const* rsys, uint32_t const* rpos, unsigned nr,
+                        unsigned cell_bits, uint32_t const* sorted_keys, uint32_t* offsets, hipStream_t s);
+// reorder each listed system into its cell order (sorted vals are local
+// indices in the old order); r0 := new positions (hi words)
+void launch_ens_reorder(float4* pos, float4* vel, int4* image, uint32_t* id, uint4* posq, float4* r0,
+                        float4* pos_t, float4* vel_t, int4* image_t, uint32_t* id_t, uint4* posq_t,
+                        ens_desc const* descs, uint32_t const* rsys, uint32_t const* rpos, uint32_t const* rblk,
+                        unsigned nrblk, uint32_t const* sorted_vals, hipStream_t s);
+// r0 := positions of the listed systems (all-pairs systems, no reorder)
+void launch_ens_r0(float4 const* pos, float4* r0, ens_desc const* descs, uint32_t const* rblk_sys, unsigned nrblk,
+                   hipStream_t s);
+// list builds: rseg (s << 16) | segment for cell systems; rsmall the
+// all-pairs systems (one workgroup each); stats[2 s .. 2 s + 1]
+void launch_ens_build(uint4 const* posq, uint32_t const* offsets, ens_desc const* descs, uint32_t const* rseg,
+                      unsigned nrseg, unsigned threads, unsigned lds_records, uint16_t* list, uint32_t* count,
+                      uint32_t* stats, hipStream_t s);
+void launch_ens_build_small(uint4 const* posq, ens_desc const* descs, uint32_t const* rsmall, unsigned nsmall,
+                            unsigned max_n, uint16_t* list, uint32_t* count, uint32_t* stats, hipStream_t s);
+// forces of one (kind, truncation) class: fseg (s << 16) | segment, fsmall
+// the all-pairs systems; aux: energy in force.w and stress (stride = total)
+void launch_ens_force(int kind, int trunc, bool aux, uint4 const* posq, uint32_t const* offsets,
+                      ens_desc const* descs, uint32_t const* fseg, unsigned nfseg, unsigned threads,
+                      unsigned lds_records, uint32_t const* fsmall, unsigned nsmall, unsigned small_max_n,
+                      uint16_t const* list, uint32_t const* count, float4* force, float* stress, unsigned stride,
+                      hipStream_t s);
+// per-system FP64 sums, one workgroup per listed system: out[4 s] = sum m v^2
+// (velocities after the pending kick, when pending), [1] = sum en_pot, [2] =
+// sum of the stress traces (both from aux_force / stress when given), [3] = n
+void launch_ens_thermo(float4 const* vel, float4 const* force, float4 const* aux_force, float const* stress,
+                       unsigned stride, ens_desc const* descs, uint32_t const* sys, unsigned nsys, bool pending,
+                       double* out, hipStream_t s);
+
+// ---------------------------------------------------------------------------
 // reductions (reduce.hip): deterministic two-pass FP64 block sums
 // ---------------------------------------------------------------------------
 constexpr unsigned REDUCE_BLOCKS = 240;

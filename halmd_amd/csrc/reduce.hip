@@ -153,4 +153,69 @@ void launch_nhc_propagate(double const* partials, nhc_state* state, hipStream_t 
     HIP_CHECK(hipGetLastError());
 }
 
+namespace {
+
+__device__ __forceinline__ double block_sum_f64(double v, double* scratch) {
+    for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    unsigned const lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    __syncthreads();
+    if (lane == 0) scratch[wave] = v;
+    __syncthreads();
+    double t = 0;
+    for (unsigned k = 0; k < blockDim.x / WAVE; ++k) t += scratch[k];   // fixed order
+    return t;
+}
+
+// one workgroup per listed system; FP64 sums in a fixed order
+__global__ __launch_bounds__(256) void ens_thermo_kernel(float4 const* __restrict__ vel,
+                                                         float4 const* __restrict__ force,
+                                                         float4 const* __restrict__ aux_force,
+                                                         float const* __restrict__ stress, unsigned stride,
+                                                         ens_desc const* __restrict__ descs,
+                                                         uint32_t const* __restrict__ sys, bool pending,
+                                                         double* __restrict__ out) {
+    __shared__ double scratch[256 / WAVE];
+    unsigned const s = sys[blockIdx.x];
+    ens_desc const& d = descs[s];
+    double mv2 = 0, en = 0, vir = 0;
+    for (unsigned li = threadIdx.x; li < d.n; li += blockDim.x) {
+        size_t const i = d.base + li;
+        float4 const vh = vel[2 * i], vl = vel[2 * i + 1];
+        float const m = vh.w;
+        ff vx{vh.x, vl.x}, vy{vh.y, vl.y}, vz{vh.z, vl.z};
+        if (pending) {
+            // the deferred finalize kick, as the integrate kernel will apply it
+            float4 const f = force[i];
+            vx = ff_add_f(vx, f.x * d.ip.half_dt / m);
+            vy = ff_add_f(vy, f.y * d.ip.half_dt / m);
+            vz = ff_add_f(vz, f.z * d.ip.half_dt / m);
+        }
+        double const x = double(vx.hi) + vx.lo, y = double(vy.hi) + vy.lo, z = double(vz.hi) + vz.lo;
+        mv2 += double(m) * (x * x + y * y + z * z);
+        if (aux_force) {
+            en += aux_force[i].w;
+            vir += double(stress[i]) + stress[stride + i] + stress[2 * size_t(stride) + i];
+        }
+    }
+    mv2 = block_sum_f64(mv2, scratch);
+    en = block_sum_f64(en, scratch);
+    vir = block_sum_f64(vir, scratch);
+    if (threadIdx.x == 0) {
+        out[4 * size_t(s)] = mv2;
+        out[4 * size_t(s) + 1] = en;
+        out[4 * size_t(s) + 2] = vir;
+        out[4 * size_t(s) + 3] = d.n;
+    }
+}
+
+} // namespace
+
+void launch_ens_thermo(float4 const* vel, float4 const* force, float4 const* aux_force, float const* stress,
+                       unsigned stride, ens_desc const* descs, uint32_t const* sys, unsigned nsys, bool pending,
+                       double* out, hipStream_t s) {
+    if (!nsys) return;
+    ens_thermo_kernel<<<nsys, 256, 0, s>>>(vel, force, aux_force, stress, stride, descs, sys, pending, out);
+    HIP_CHECK(hipGetLastError());
+}
+
 } // namespace halmd

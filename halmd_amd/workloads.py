@@ -17,20 +17,28 @@ KA_EPSILON = np.array([[1.0, 1.5], [1.5, 0.5]])
 KA_SIGMA = np.array([[1.0, 0.8], [0.8, 0.88]])
 
 
-def fluid(nparticle, density, temperature, potential, skin=0.3, timestep=0.005, seed=1, species=None,
-          integrator="verlet", resonance_frequency=5.0, collision_rate=1.0, sort_interval=0, hilbert_on_rebuild=False,
-          jitter=0.0):
-    """fcc lattice at `density`, Maxwell-Boltzmann velocities at `temperature`."""
-    n = int(nparticle)
+def system_inputs(n, density, temperature, seed=1, jitter=0.0):
+    """(box edges, positions, velocities, lattice constant) of an fcc fluid
+    start: lattice at `density`, optional uniform jitter (in lattice
+    constants), Maxwell-Boltzmann velocities at `temperature` (seed + 1)."""
     L = np.full(3, (n / density) ** (1.0 / 3))
     r0, a = positions.fcc(n, L)
     rng = np.random.default_rng(seed)
     if jitter:
         r0 = r0 + rng.uniform(-jitter, jitter, r0.shape) * a
         r0 = (r0 + L / 2) % L - L / 2
+    v0 = velocities.maxwell_boltzmann(n, temperature, 1.0, seed=seed + 1)
+    return L, r0, v0, a
+
+
+def fluid(nparticle, density, temperature, potential, skin=0.3, timestep=0.005, seed=1, species=None,
+          integrator="verlet", resonance_frequency=5.0, collision_rate=1.0, sort_interval=0, hilbert_on_rebuild=False,
+          jitter=0.0):
+    """fcc lattice at `density`, Maxwell-Boltzmann velocities at `temperature`."""
+    n = int(nparticle)
+    L, r0, v0, a = system_inputs(n, density, temperature, seed, jitter)
     nspecies = 1 if species is None else int(np.max(species)) + 1
     nspecies = max(nspecies, potential.species)
-    v0 = velocities.maxwell_boltzmann(n, temperature, 1.0, seed=seed + 1)
     box = mdsim.box(length=L)
     particle = mdsim.particle(dimension=3, particles=n, species=nspecies)
     particle.data["position"] = r0
@@ -79,3 +87,47 @@ def kob_andersen(nparticle, density=1.2, temperature=0.7, fraction_b=0.2, seed=1
     species[rng.permutation(n)[:int(round(fraction_b * n))]] = 1
     pot = mdsim.potentials.pair.lennard_jones(epsilon=KA_EPSILON, sigma=KA_SIGMA).truncate("shifted", cutoff=2.5)
     return fluid(n, density, temperature, pot, species=species, seed=seed, **kw)
+
+
+# ---------------------------------------------------------------------------
+# batched ensembles of independent systems
+# ---------------------------------------------------------------------------
+def batched_specs(count=256, seed=2024, n_range=(500, 32000), density_range=(0.7, 1.2),
+                  temperature_range=(0.5, 2.0), potentials=("lj", "wca", "morse")):
+    """Seeded system mix: N log-uniform in n_range, density and temperature
+    uniform, potential drawn from `potentials` (names of ensemble.POTENTIALS)."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.log(n_range[0]), np.log(n_range[1])
+    out = []
+    for k in range(count):
+        out.append(dict(n=int(np.exp(rng.uniform(lo, hi))), density=float(rng.uniform(*density_range)),
+                        temperature=float(rng.uniform(*temperature_range)),
+                        potential=str(potentials[rng.integers(len(potentials))]), seed=int(seed * 1000 + k)))
+    return out
+
+
+def spec_potential(name):
+    """The single-system potential object of an ensemble potential name."""
+    from .ensemble import POTENTIALS
+    kind, eps, sigma, rc, r_min, dist = POTENTIALS[name]
+    if kind == 0:
+        return mdsim.potentials.pair.lennard_jones(epsilon=eps, sigma=sigma).truncate("shifted", cutoff=rc)
+    return mdsim.potentials.pair.morse(epsilon=eps, sigma=sigma, r_min=r_min, distortion=dist).truncate(
+        "shifted", cutoff=rc)
+
+
+def single_from_spec(spec, timestep=0.003, skin=0.3):
+    """The system of `spec` alone, through the per-system modules."""
+    return fluid(spec["n"], spec["density"], spec["temperature"], spec_potential(spec["potential"]), skin=skin,
+                 timestep=timestep, seed=spec["seed"])
+
+
+def ensemble_from_specs(specs, timestep=0.003, skin=0.3):
+    """All systems of `specs` in one committed ensemble (inputs as fluid() makes them)."""
+    from .ensemble import ensemble
+    e = ensemble()
+    for sp in specs:
+        L, r0, v0, _ = system_inputs(sp["n"], sp["density"], sp["temperature"], sp["seed"])
+        e.add(r0, v0, L[0], potential=sp["potential"], timestep=timestep, skin=skin)
+    e.commit()
+    return e

@@ -55,6 +55,7 @@ typedef struct halmd_wavevector_s halmd_wavevector;
 typedef struct halmd_density_mode_s halmd_density_mode;
 typedef struct halmd_blocking_scheme_s halmd_blocking_scheme;
 typedef struct halmd_sampler_s halmd_sampler;
+typedef struct halmd_ensemble_s halmd_ensemble;
 
 /* ---- library ---------------------------------------------------------- */
 HALMD_API const char* halmd_last_error(void);
@@ -318,6 +319,34 @@ HALMD_API int halmd_observables_sampler_run_timed(halmd_sampler* s, uint64_t ste
 /* records (step, en_pot, en_kin, virial, en_nhc) of thermodynamics slot k */
 HALMD_API int halmd_observables_sampler_thermodynamics_records(halmd_sampler* s, unsigned k, double* out,
                                                                size_t capacity, size_t* nrecords);
+
+/* ---- ensembles: many independent NVE systems stepped in shared launches --
+ * No counterpart in the reference (it runs one system per process,
+ * doc/usage/multi-gpu.rst.in:4-47).  Each system: cubic box, one species of
+ * unit mass, a truncated pair potential (kind as POTENTIAL_* of the potential
+ * create functions: 0 Lennard-Jones, 1 Morse; truncation by name), velocity
+ * Verlet; neighbour lists as the first neighbour module of a particle
+ * instance builds them (all-pairs lists under 3 cells per dimension). */
+HALMD_API int halmd_ensemble_create(halmd_ensemble** out);
+HALMD_API int halmd_ensemble_destroy(halmd_ensemble* e);
+/* positions/velocities row-major n x 3; *index receives the system index */
+HALMD_API int halmd_ensemble_add(halmd_ensemble* e, unsigned n, double length, int kind, const char* truncation,
+                                 double epsilon, double sigma, double r_cut_sigma, double r_min, double distortion,
+                                 double smoothing, double timestep, double skin, const double* position,
+                                 const double* velocity, unsigned* index);
+/* lay out all systems, build lists and forces; no systems are added after it */
+HALMD_API int halmd_ensemble_commit(halmd_ensemble* e);
+HALMD_API int halmd_ensemble_step(halmd_ensemble* e, uint64_t steps);
+/* steps with HIP events on the ensemble stream around them; elapsed ms */
+HALMD_API int halmd_ensemble_step_timed(halmd_ensemble* e, uint64_t steps, double* ms);
+HALMD_API int halmd_ensemble_size(const halmd_ensemble* e, unsigned* systems);
+/* per system 4 doubles: sum m v^2, sum en_pot, sum of stress traces, n */
+HALMD_API int halmd_ensemble_thermo(halmd_ensemble* e, double* out);
+/* one system in particle-ID order (n x 3 each; null skips): positions in the
+ * box, velocities, periodic images */
+HALMD_API int halmd_ensemble_state(halmd_ensemble* e, unsigned system, double* position, double* velocity,
+                                   int* image);
+HALMD_API int halmd_ensemble_rebuilds(const halmd_ensemble* e, unsigned system, uint64_t* count);
 
 /* ---- profiling: per-module kernel timers (HIP events on the stream) ------ */
 /* module: "force", "neighbour", "binning", "integrate", "finalize", "sort",

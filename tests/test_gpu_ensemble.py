@@ -1,0 +1,102 @@
+"""Batched ensembles on the GPU: each system's result is the same alone and in the mix."""
+import numpy as np
+import pytest
+
+from halmd_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+# a small mix: the three potentials, a box under 3 cells (all-pairs lists),
+# uneven sizes and densities
+SPECS = [
+    dict(n=4000, density=0.8442, temperature=1.2, potential="lj", seed=11),
+    dict(n=3000, density=1.0, temperature=0.8, potential="wca", seed=12),
+    dict(n=2500, density=0.9, temperature=1.0, potential="morse", seed=13),
+    dict(n=500, density=1.2, temperature=1.5, potential="lj", seed=14),
+    dict(n=13500, density=0.75, temperature=0.6, potential="lj", seed=15),
+    dict(n=6000, density=1.15, temperature=1.8, potential="wca", seed=16),
+]
+DT = 0.003
+
+
+def _alone(spec, steps):
+    w = workloads.single_from_spec(spec, timestep=DT)
+    w.sampler.run(steps)
+    rid = w.particle.data["reverse_id"]
+    pos = w.particle.data["position"][rid]
+    vel = w.particle.data["velocity"][rid]
+    img = w.particle.data["image"][rid]
+    return pos, vel, img, w
+
+
+def test_each_system_same_alone_and_in_the_mix():
+    steps = 150
+    e = workloads.ensemble_from_specs(SPECS, timestep=DT)
+    e.step(steps)
+    for k, spec in enumerate(SPECS):
+        r_e, v_e, im_e = e.state(k)
+        r_a, v_a, im_a, w = _alone(spec, steps)
+        L = w.length
+        small = int(L[0] / (w.potential.r_cut[0, 0] + 0.3)) < 3
+        print(f"system {k} ({spec['potential']}, N={spec['n']}, small={small}): rebuilds {e.rebuilds(k)} vs "
+              f"{w.neighbour.rebuilds}; max |dr| {np.max(np.abs((r_e + im_e * L) - (r_a + im_a * L))):.2e}")
+        if small:
+            # all-pairs lists: the single path's force kernel differs in rounding
+            assert np.max(np.abs((r_e + im_e * L) - (r_a + im_a * L))) < 1e-4
+            assert np.max(np.abs(v_e - v_a)) < 1e-3
+        else:
+            # the same device code and the same rebuild rule: bitwise
+            assert np.array_equal(r_e, r_a)
+            assert np.array_equal(v_e, v_a)
+            assert np.array_equal(im_e, im_a)
+            assert e.rebuilds(k) == w.neighbour.rebuilds
+
+
+def test_ensemble_energies_match_single_path():
+    e = workloads.ensemble_from_specs(SPECS, timestep=DT)
+    e.step(40)
+    en_pot, en_kin = e.energies()
+    for k, spec in enumerate(SPECS):
+        _, _, _, w = _alone(spec, 40)
+        t = w.thermodynamics
+        assert abs(en_pot[k] - t.en_pot()) < 1e-5 * max(1.0, abs(t.en_pot()))
+        assert abs(en_kin[k] - t.en_kin()) < 1e-6 * t.en_kin()
+
+
+def test_ensemble_conserves_energy():
+    e = workloads.ensemble_from_specs(SPECS, timestep=DT)
+    p0, k0 = e.energies()
+    e.step(1500)
+    p1, k1 = e.energies()
+    drift = np.abs((p1 + k1) - (p0 + k0))
+    print("energy change per particle:", drift)
+    assert np.all(drift < 5e-3)
+    # and the state reads back in the box, with finite values
+    for k in range(len(SPECS)):
+        r, v, _ = e.state(k)
+        assert np.all(np.isfinite(r)) and np.all(np.isfinite(v))
+
+
+def test_ensemble_overflow_regrows_lists():
+    # a cluster at three times the mean density overflows the list capacity
+    # and the LDS staging sized from the mean; the ensemble regrows both (new
+    # list layout for every system) and rebuilds
+    from halmd_amd.ensemble import ensemble
+    rng = np.random.default_rng(3)
+    e = ensemble()
+    for sp in SPECS[:2]:
+        L, r0, v0, _ = workloads.system_inputs(sp["n"], sp["density"], sp["temperature"], sp["seed"])
+        e.add(r0, v0, L[0], potential=sp["potential"], timestep=DT)
+    n, L = 6000, 24.0
+    m = int(np.ceil(n ** (1 / 3)))
+    g = np.stack(np.meshgrid(*[np.arange(m)] * 3, indexing="ij"), -1).reshape(-1, 3)[:n]
+    r = -L / 2 + (g + 0.5) * 0.69 + rng.normal(0, 0.01, (n, 3))   # rho ~ 3 in a corner
+    e.add(r, rng.normal(0, 0.5, (n, 3)), L, potential="wca", timestep=DT)
+    e.commit()
+    e.step(5)
+    pot, kin = e.energies()
+    assert np.all(np.isfinite(pot)) and np.all(np.isfinite(kin))
+    # the other systems are unaffected by the relayout: same as alone
+    r_e, v_e, _ = e.state(0)
+    r_a, v_a, _, _ = _alone(SPECS[0], 5)
+    assert np.array_equal(r_e, r_a) and np.array_equal(v_e, v_a)
