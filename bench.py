@@ -212,7 +212,7 @@ def main():
     w = build_workload(args.seed)
     if not args.no_tune:
         cells = w.force.tune()
-        log(f"autotuned cell segments: {w.neighbour.segments} (cells, workgroup size)")
+        log(f"autotuned cell segments: {w.neighbour.segments} (cells, rows, workgroup size)")
     log(f"setup {time.perf_counter() - t0:.1f} s; warmup {args.warmup} steps")
     w.sampler.run(args.warmup)
     log(f"timed {args.steps} steps")

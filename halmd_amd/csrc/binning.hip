@@ -312,25 +312,32 @@ __device__ __forceinline__ void staged_build_segment(
 
     unsigned const nx = p.grid.ncell[0];
     float const rr_cut0 = ISO ? p.rr_cut_skin_q[0] : p.rr_cut_skin[0];
+    unsigned const hh = st.q.h + 2, T = st.own_pre[st.q.h];
     unsigned cmax = 0;
-    for (unsigned i = st.pb + threadIdx.x; i < st.pe; i += blockDim.x) {
+    for (unsigned t = threadIdx.x; t < T; t += blockDim.x) {
+        unsigned j, i;
+        seg_own(st, t, j, i);
+        unsigned const self = hh + j + 1;   // union row of the particle's own (y, z) row
         nb16_state s;
-        s.own = s_bound[4 * (SEG_MAX + 3) + 1] + (i - st.pb);
+        s.own = s_bound[self * SEG_BOUND + 1] + (t - st.own_pre[j]);
         s.ri = s_rec[s.own];
         s.ti = (s.ri.w & 3u) * p.ntype;
         s.c = 0;
         s.cap = cap;
         s.out = list + size_t(i & ~(WAVE - 1)) * cap + (i & (WAVE - 1)) * NB16_BLOCK;
-        unsigned const k = cell_component(s.ri.x, nx) - st.x0;   // 0 .. w - 1
-        // rows 0 .. 8; only row 4 (dz = dy = 0) holds the particle itself
-        for (unsigned r = 0; r < 9; ++r) {
-            unsigned const t = s_bound[r * (SEG_MAX + 3) + k];
-            unsigned const e = s_bound[r * (SEG_MAX + 3) + k + 3];
-            if (r == 4)
-                scan16<MULTI, ISO, true>(s, t, e, s_rec, p, s_rr, rr_cut0);
-            else
-                scan16<MULTI, ISO, false>(s, t, e, s_rec, p, s_rr, rr_cut0);
-        }
+        unsigned const k = cell_component(s.ri.x, nx) - st.q.x0;   // 0 .. w - 1
+        // the 9 rows (dz, dy) in lexicographic order; only the own row holds
+        // the particle itself
+        for (unsigned dz = 0; dz < 3; ++dz)
+            for (unsigned dy = 0; dy < 3; ++dy) {
+                unsigned const ur = dz * hh + j + dy;
+                unsigned const b = s_bound[ur * SEG_BOUND + k];
+                unsigned const e = s_bound[ur * SEG_BOUND + k + 3];
+                if (ur == self)
+                    scan16<MULTI, ISO, true>(s, b, e, s_rec, p, s_rr, rr_cut0);
+                else
+                    scan16<MULTI, ISO, false>(s, b, e, s_rec, p, s_rr, rr_cut0);
+            }
         unsigned const c = s.c;
         // pad with the sentinel to a multiple of 8 (cap is a multiple of 16)
         if (c <= cap)
@@ -349,7 +356,7 @@ __global__ __launch_bounds__(SEG_THREADS_MAX) void build_staged_kernel(
 {
     extern __shared__ uint4 s_rec[];
     __shared__ seg_stage st;
-    __shared__ unsigned s_bound[9 * (SEG_MAX + 3)];
+    __shared__ unsigned s_bound[SEG_NR * SEG_BOUND];
     __shared__ float s_rr[MAX_TYPES * MAX_TYPES];
 
     unsigned const seg = seg_of_block(sp, blockIdx.x);
@@ -463,7 +470,7 @@ __global__ __launch_bounds__(SEG_THREADS_MAX) void ens_build_kernel(
 {
     extern __shared__ uint4 s_rec[];
     __shared__ seg_stage st;
-    __shared__ unsigned s_bound[9 * (SEG_MAX + 3)];
+    __shared__ unsigned s_bound[SEG_NR * SEG_BOUND];
     __shared__ float s_rr[MAX_TYPES * MAX_TYPES];
     uint32_t const e = rseg[blockIdx.x];
     unsigned const sys = e >> 16, seg = e & 0xffffu;

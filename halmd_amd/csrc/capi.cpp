@@ -359,11 +359,12 @@ HALMD_API int halmd_mdsim_neighbours_from_binning_entry_bytes(const halmd_neighb
     });
 }
 HALMD_API int halmd_mdsim_neighbours_from_binning_segments(const halmd_neighbour* nb, unsigned* cells,
-                                                           unsigned* threads) {
+                                                           unsigned* rows, unsigned* threads) {
     return guard([&] {
-        nonnull(cells, "cells"); nonnull(threads, "threads");
+        nonnull(cells, "cells"); nonnull(rows, "rows"); nonnull(threads, "threads");
         auto const* n = as<neighbour>(nb);
         *cells = n->staged ? n->sp.S : 0u;
+        *rows = n->staged ? n->sp.R : 0u;
         *threads = n->staged ? n->sp.threads : 0u;
     });
 }

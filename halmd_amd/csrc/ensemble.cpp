@@ -17,7 +17,7 @@ struct ensemble::sys_host {
     // layout
     unsigned base = 0, blk_base = 0, nblk = 0, cell_base = 0, ncells = 0, ncell = 0;
     bool small = false;
-    unsigned S = 0, nsx = 0, nseg = 0, cap = 0, lds_cap = 0;
+    unsigned S = 0, R = 0, nsx = 0, nsy = 0, nseg = 0, cap = 0, lds_cap = 0;
     size_t list_off = 0;
     uint64_t rebuilds = 0;
     ens_desc d{};
@@ -116,11 +116,14 @@ void ensemble::commit() {
             h.ncells = h.ncell * h.ncell * h.ncell;
             cells += h.ncells + 1;
             double const occ = double(h.n) / h.ncells;
-            h.S = std::max(1u, std::min({unsigned(std::round(200.0 / std::max(occ, 1e-9))), SEG_MAX, h.ncell - 2}));
+            cell_grid g;
+            g.ncell[0] = g.ncell[1] = g.ncell[2] = h.ncell;
+            segment_shape(occ, g, 200.0, h.S, h.R);
             h.nsx = unsigned(div_up(h.ncell, h.S));
-            h.nseg = h.nsx * h.ncell * h.ncell;
+            h.nsy = unsigned(div_up(h.ncell, h.R));
+            h.nseg = h.nsx * h.nsy * h.ncell;
             require(h.nseg < 65536, "ensemble: too many cell segments in one system");
-            double const expect = 9.0 * (h.S + 2) * occ;
+            double const expect = 3.0 * (h.R + 2) * (h.S + 2) * occ;
             h.lds_cap = unsigned(std::min<double>(SEG_MAX_RECORDS, round_up(size_t(1.3 * expect) + 64, 64)));
         }
         // descriptor (the list fields are filled by upload_descs)
@@ -135,7 +138,9 @@ void ensemble::commit() {
         d.skin_half_sq = float(h.skin * h.skin / 4);
         d.sp.grid.ncell[0] = d.sp.grid.ncell[1] = d.sp.grid.ncell[2] = h.ncell;
         d.sp.S = h.S;
+        d.sp.R = h.R;
         d.sp.nsx = h.nsx;
+        d.sp.nsy = h.nsy;
         d.sp.nseg = h.nseg;
         d.sp.chunk = 0;
         d.sp.threads = threads_;

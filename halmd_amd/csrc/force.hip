@@ -229,7 +229,10 @@ __device__ __forceinline__ void staged_force_segment(
     if (MULTI && threadIdx.x < MAX_TYPES * MAX_TYPES) s_pair[threadIdx.x] = P.pair[threadIdx.x];
     stage_union(sp, seg, off, posq, s_rec, st, nullptr);   // fits: the list build checked it
 
-    for (unsigned i = st.pb + threadIdx.x; i < st.pe; i += blockDim.x) {
+    unsigned const T = st.own_pre[st.q.h];
+    for (unsigned t = threadIdx.x; t < T; t += blockDim.x) {
+        unsigned j, i;
+        seg_own(st, t, j, i);
         uint4 const qi = posq[i];
         unsigned const ti = (qi.w & 3u) * P.ntype;
         unsigned const n8 = (count[i] + 7u) & ~7u;

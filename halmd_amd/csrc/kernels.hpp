@@ -99,11 +99,15 @@ void launch_build_neighbours_all_pairs(uint4 const* posq, unsigned n, neighbour_
 constexpr unsigned SEG_MAX = 32;          // largest S
 constexpr unsigned SEG_THREADS_MAX = 256; // largest workgroup of the segment kernels
 constexpr unsigned SEG_XCD = 8;           // workgroups are dealt round-robin to 8 XCDs
+constexpr unsigned SEG_RMAX = 8;           // largest R
+constexpr unsigned SEG_NR = 3 * (SEG_RMAX + 2);   // most union rows
 struct seg_params {
     cell_grid grid;
-    unsigned S;           // cells per segment, 1 <= S <= min(SEG_MAX, nx - 2)
-    unsigned nsx;         // segments per row, ceil(nx / S)
-    unsigned nseg;        // nsx * ny * nz
+    unsigned S;           // cells per segment along x, 1 <= S <= min(SEG_MAX, nx - 2)
+    unsigned R;           // rows per segment along y, 1 <= R <= min(SEG_RMAX, ny - 2)
+    unsigned nsx;         // segments along x, ceil(nx / S)
+    unsigned nsy;         // segments along y, ceil(ny / R)
+    unsigned nseg;        // nsx * nsy * nz
     unsigned chunk;       // ceil(nseg / SEG_XCD): consecutive segments per XCD
     unsigned lds_cap;     // LDS records per workgroup (>= every union size)
     unsigned threads;     // workgroup size (a multiple of 64, <= SEG_THREADS_MAX)
@@ -113,21 +117,33 @@ struct seg_params {
 __host__ __device__ inline unsigned seg_of_block(seg_params const& sp, unsigned b) {
     return (b % SEG_XCD) * sp.chunk + b / SEG_XCD;
 }
-struct seg_rows {
-    unsigned x0, sp;              // first cell (x) and number of cells of the segment
-    unsigned pb, pe;              // particle range of the segment
-    unsigned g[9][2], n[9][2];    // per row: up to two memory ranges (begin, length)
+// position of segment `seg`: cells x0 .. x0 + w - 1 of rows y0 .. y0 + h - 1 in plane z
+struct seg_place {
+    unsigned x0, w, y0, h, z;
 };
-// geometry of segment `seg` from the cell offsets (device or host memory)
-__host__ __device__ inline void seg_row(seg_params const& sp, unsigned seg, unsigned r, uint32_t const* off,
+__host__ __device__ inline seg_place seg_locate(seg_params const& sp, unsigned seg) {
+    unsigned const nx = sp.grid.ncell[0], ny = sp.grid.ncell[1];
+    seg_place q;
+    unsigned const sx = seg % sp.nsx, t = seg / sp.nsx;
+    unsigned const sy = t % sp.nsy;
+    q.z = t / sp.nsy;
+    q.x0 = sx * sp.S;
+    q.w = q.x0 + sp.S <= nx ? sp.S : nx - q.x0;
+    q.y0 = sy * sp.R;
+    q.h = q.y0 + sp.R <= ny ? sp.R : ny - q.y0;
+    return q;
+}
+// Union row ur (0 .. 3 (h + 2) - 1; plane dz = ur / (h + 2) - 1, row y0 - 1 +
+// ur % (h + 2), periodic) of the segment's 27-cell neighbourhoods: cells
+// x0 - 1 .. x0 + w as up to two memory ranges (begin, length), from the cell
+// offsets (device or host memory)
+__host__ __device__ inline void seg_row(seg_params const& sp, seg_place const& q, unsigned ur, uint32_t const* off,
                                         unsigned* g, unsigned* len) {
     unsigned const nx = sp.grid.ncell[0], ny = sp.grid.ncell[1], nz = sp.grid.ncell[2];
-    unsigned const sx = seg % sp.nsx, t = seg / sp.nsx;
-    unsigned const y = t % ny, z = t / ny;
-    unsigned const x0 = sx * sp.S;
-    unsigned const w = x0 + sp.S <= nx ? sp.S : nx - x0;
-    unsigned const kz = (z + nz + r / 3 - 1) % nz, ky = (y + ny + r % 3 - 1) % ny;
+    unsigned const hh = q.h + 2;
+    unsigned const kz = (q.z + nz + ur / hh - 1) % nz, ky = (q.y0 + ny + ur % hh - 1) % ny;
     unsigned const row = (kz * ny + ky) * nx;
+    unsigned const x0 = q.x0, w = q.w;
     if (x0 > 0 && x0 + w < nx) {
         g[0] = off[row + x0 - 1]; len[0] = off[row + x0 + w + 1] - g[0];
         g[1] = 0; len[1] = 0;
@@ -139,16 +155,13 @@ __host__ __device__ inline void seg_row(seg_params const& sp, unsigned seg, unsi
         g[1] = off[row]; len[1] = off[row + 1] - g[1];
     }
 }
-__host__ __device__ inline void seg_particles(seg_params const& sp, unsigned seg, uint32_t const* off,
-                                              unsigned& x0, unsigned& w, unsigned& pb, unsigned& pe) {
+// the segment's own particles of row j (0 .. h - 1): memory range [begin, begin + count)
+__host__ __device__ inline void seg_own_row(seg_params const& sp, seg_place const& q, unsigned j, uint32_t const* off,
+                                            unsigned& begin, unsigned& count) {
     unsigned const nx = sp.grid.ncell[0], ny = sp.grid.ncell[1];
-    unsigned const sx = seg % sp.nsx, t = seg / sp.nsx;
-    unsigned const y = t % ny, z = t / ny;
-    x0 = sx * sp.S;
-    w = x0 + sp.S <= nx ? sp.S : nx - x0;
-    unsigned const row = (z * ny + y) * nx;
-    pb = off[row + x0];
-    pe = off[row + x0 + w];
+    unsigned const row = (q.z * ny + q.y0 + j) * nx;
+    begin = off[row + q.x0];
+    count = off[row + q.x0 + q.w] - begin;
 }
 // 16-bit slot lists (an entry is the byte offset 16 * slot of the record in
 // the staged union, so a union holds at most 4096 records): entry k of

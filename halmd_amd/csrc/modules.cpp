@@ -603,7 +603,7 @@ neighbour::neighbour(particle& p_, binning* bin_, max_displacement& disp_, box c
         HIP_CHECK(hipGetDevice(&dev));
         HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
         // static LDS of the build kernel: cell bounds, segment table, cutoffs
-        size_t const static_lds = 9 * (SEG_MAX + 3) * 4 + 1024;
+        size_t const static_lds = SEG_NR * (SEG_MAX + 3) * 4 + 2048;
         require(size_t(lds) > static_lds + 27 * 16, "neighbour: device LDS too small");
         lds_limit = std::min<size_t>((size_t(lds) - static_lds) / sizeof(uint4), SEG_MAX_RECORDS);
         // about `target` particles per segment (per 256-thread workgroup) at
@@ -611,7 +611,9 @@ neighbour::neighbour(particle& p_, binning* bin_, max_displacement& disp_, box c
         // measurement); HALMD_SEG_TARGET overrides the target
         double target = 200;
         if (char const* e = std::getenv("HALMD_SEG_TARGET")) target = std::max(1.0, std::atof(e));
-        set_segments(unsigned(std::max(1.0, std::round(target / std::max(mean_occupancy(), 1e-9)))), SEG_THREADS_MAX);
+        unsigned S, R;
+        segment_shape(mean_occupancy(), bin->grid, target, S, R);
+        set_segments(S, R, SEG_THREADS_MAX);
     } else {
         cap = unsigned(round_up(size_t(std::ceil(nmean / occupancy)) + NB_BLOCK, NB_BLOCK));
         list.resize(size_t(round_up(p.n, WAVE)) * cap);
@@ -622,22 +624,33 @@ neighbour::~neighbour() {
     if (p.order_owner == this) p.order_owner = nullptr;
 }
 
+void segment_shape(double occupancy, cell_grid const& g, double target, unsigned& S, unsigned& R) {
+    // about `target` particles per segment, the owned block near square in (x, y)
+    double const cells = std::max(1.0, target / std::max(occupancy, 1e-9));
+    R = unsigned(std::max(1.0, std::round(std::sqrt(cells))));
+    R = std::max(1u, std::min({R, SEG_RMAX, g.ncell[1] - 2}));
+    S = unsigned(std::max(1.0, std::round(cells / R)));
+    S = std::max(1u, std::min({S, SEG_MAX, g.ncell[0] - 2}));
+}
+
 double neighbour::mean_occupancy() const { return bin ? double(p.n) / bin->ncells : 0.0; }
 
-void neighbour::set_segments(unsigned S, unsigned threads) {
+void neighbour::set_segments(unsigned S, unsigned R, unsigned threads) {
     require(threads % WAVE == 0 && threads >= WAVE && threads <= SEG_THREADS_MAX, "neighbour: invalid workgroup size");
-    plan_segments(S);
+    plan_segments(S, R);
     sp.threads = threads;
-    double const expect = 9.0 * (sp.S + 2) * mean_occupancy();
+    double const expect = 3.0 * (sp.R + 2) * (sp.S + 2) * mean_occupancy();
     sp.lds_cap = unsigned(std::min<double>(lds_limit, round_up(size_t(1.3 * expect) + 64, 64)));
 }
 
-void neighbour::plan_segments(unsigned S) {
-    unsigned const nx = bin->grid.ncell[0];
+void neighbour::plan_segments(unsigned S, unsigned R) {
+    unsigned const nx = bin->grid.ncell[0], ny = bin->grid.ncell[1];
     sp.grid = bin->grid;
     sp.S = std::max(1u, std::min({S, SEG_MAX, nx - 2}));
+    sp.R = std::max(1u, std::min({R, SEG_RMAX, ny - 2}));
     sp.nsx = unsigned(div_up(nx, sp.S));
-    sp.nseg = sp.nsx * bin->grid.ncell[1] * bin->grid.ncell[2];
+    sp.nsy = unsigned(div_up(ny, sp.R));
+    sp.nseg = sp.nsx * sp.nsy * bin->grid.ncell[2];
     sp.chunk = unsigned(div_up(sp.nseg, SEG_XCD));
 }
 
@@ -692,9 +705,12 @@ void neighbour::update() {
             } else if (u <= lds_limit) {
                 sp.lds_cap = unsigned(lds_limit);
             } else {
-                require(sp.S > 1, "neighbour: cell occupancy too high to stage 27 cells in LDS");
-                unsigned const S = std::max(1u, unsigned(sp.S * double(lds_limit) / u) - 1);
-                plan_segments(std::min(S, sp.S - 1));
+                // narrower segments: fewer rows first, then fewer cells per row
+                require(sp.S > 1 || sp.R > 1, "neighbour: cell occupancy too high to stage 27 cells in LDS");
+                if (sp.R > 1)
+                    plan_segments(sp.S, sp.R - 1);
+                else
+                    plan_segments(std::min(std::max(1u, unsigned(sp.S * double(lds_limit) / u) - 1), sp.S - 1), 1);
                 sp.lds_cap = unsigned(lds_limit);
             }
             continue;
@@ -735,20 +751,23 @@ void neighbour::get_lists(unsigned* counts, unsigned* entries, unsigned stride) 
     std::vector<uint32_t> slot_to_index;
     for (unsigned seg = 0; seg < sp.nseg; ++seg) {
         slot_to_index.clear();
-        for (unsigned r = 0; r < 9; ++r) {
+        seg_place const q = seg_locate(sp, seg);
+        for (unsigned ur = 0; ur < 3 * (q.h + 2); ++ur) {
             unsigned g[2], len[2];
-            seg_row(sp, seg, r, off.data(), g, len);
+            seg_row(sp, q, ur, off.data(), g, len);
             for (unsigned pc = 0; pc < 2; ++pc)
                 for (unsigned t = 0; t < len[pc]; ++t) slot_to_index.push_back(g[pc] + t);
         }
-        unsigned x0, w, pb, pe;
-        seg_particles(sp, seg, off.data(), x0, w, pb, pe);
-        for (size_t i = pb; i < pe; ++i) {
-            require(counts[i] <= stride, "neighbour: output stride smaller than a list");
-            for (size_t k = 0; k < counts[i]; ++k) {
-                unsigned const slot = h[list16_index(i, k, cap)] / 16u;
-                require(slot < slot_to_index.size(), "neighbour: slot outside the segment union");
-                entries[i * stride + k] = slot_to_index[slot];
+        for (unsigned j = 0; j < q.h; ++j) {
+            unsigned pb, pn;
+            seg_own_row(sp, q, j, off.data(), pb, pn);
+            for (size_t i = pb; i < pb + pn; ++i) {
+                require(counts[i] <= stride, "neighbour: output stride smaller than a list");
+                for (size_t k = 0; k < counts[i]; ++k) {
+                    unsigned const slot = h[list16_index(i, k, cap)] / 16u;
+                    require(slot < slot_to_index.size(), "neighbour: slot outside the segment union");
+                    entries[i * stride + k] = slot_to_index[slot];
+                }
             }
         }
     }
@@ -778,16 +797,17 @@ void pair_trunc::prepare() { nb.require_current(); }
 unsigned pair_trunc::tune() {
     if (!nb.staged) return 0;
     hipStream_t s = p.stream;
-    unsigned const S0 = nb.sp.S, nx = nb.bin->grid.ncell[0];
-    unsigned const lo = S0 > 4 ? S0 - 4 : 1, hi = std::min({S0 + 4, SEG_MAX, nx - 2});
+    unsigned const S0 = nb.sp.S, R0 = nb.sp.R, nx = nb.bin->grid.ncell[0], ny = nb.bin->grid.ncell[1];
+    unsigned const slo = S0 > 2 ? S0 - 2 : 1, shi = std::min({S0 + 2, SEG_MAX, nx - 2});
+    unsigned const rlo = R0 > 1 ? R0 - 1 : 1, rhi = std::min({R0 + 1, SEG_RMAX, ny - 2});
     constexpr int REPEAT = 5;
     constexpr double REBUILD_INTERVAL = 10;   // steps between rebuilds assumed in the score
     hipEvent_t ev[3];
     for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
-    unsigned best_S = S0, best_T = nb.sp.threads;
+    unsigned best_S = S0, best_R = R0, best_T = nb.sp.threads;
     double best_score = 1e300;
-    auto trial = [&](unsigned S, unsigned T) {
-        nb.set_segments(S, T);
+    auto trial = [&](unsigned S, unsigned R, unsigned T) {
+        nb.set_segments(S, R, T);
         HIP_CHECK(hipEventRecord(ev[0], s));
         nb.built = false;
         nb.require_current();
@@ -799,19 +819,20 @@ unsigned pair_trunc::tune() {
         HIP_CHECK(hipEventElapsedTime(&t_build, ev[0], ev[1]));
         HIP_CHECK(hipEventElapsedTime(&t_force, ev[1], ev[2]));
         double const score = t_force / REPEAT + t_build / REBUILD_INTERVAL;
-        if (score < best_score) { best_score = score; best_S = S; best_T = T; }
+        if (score < best_score) { best_score = score; best_S = S; best_R = R; best_T = T; }
     };
     try {
-        trial(S0, nb.sp.threads);   // warm start (first build after a reorder is slower)
+        trial(S0, R0, nb.sp.threads);   // warm start (first build after a reorder is slower)
         best_score = 1e300;
         for (unsigned T : {128u, 192u, 256u})
-            for (unsigned S = lo; S <= hi; ++S) trial(S, T);
+            for (unsigned R = rlo; R <= rhi; ++R)
+                for (unsigned S = slo; S <= shi; ++S) trial(S, R, T);
     } catch (...) {
         for (auto& e : ev) (void)hipEventDestroy(e);
         throw;
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
-    nb.set_segments(best_S, best_T);
+    nb.set_segments(best_S, best_R, best_T);
     nb.built = false;
     nb.require_current();
     p.force_valid = p.aux_valid = false;

@@ -199,6 +199,10 @@ public:
 
 class hilbert;
 
+// cells S along x and rows R along y of a segment holding about `target`
+// particles at the given mean cell occupancy (within the grid's limits)
+void segment_shape(double occupancy, cell_grid const& g, double target, unsigned& S, unsigned& R);
+
 class neighbour {
 public:
     // bin == nullptr: lists from all particle pairs (from_particle), for
@@ -221,10 +225,10 @@ public:
     seg_params sp{}, sp_force{};      // build staging / force staging (lds_cap = largest union)
     size_t lds_limit = 0;             // records of dynamic LDS a build workgroup may use
     dbuf<uint16_t> list16;
-    void plan_segments(unsigned S);
-    // segments of S cells, workgroups of `threads`, with the staging capacity
-    // expected for them
-    void set_segments(unsigned S, unsigned threads);
+    void plan_segments(unsigned S, unsigned R);
+    // segments of S cells by R rows, workgroups of `threads`, with the staging
+    // capacity expected for them
+    void set_segments(unsigned S, unsigned R, unsigned threads);
     double mean_occupancy() const;
     uint64_t built_order = 0;
     bool built = false;

@@ -1,6 +1,11 @@
 // Device side of the cell-segment decomposition: staging of a segment's
 // neighbourhood union in LDS (shared by the neighbour-list build and the pair
 // force kernels, which must agree slot for slot).
+//
+// A segment is S cells along x by R rows along y in one z-plane.  Its union
+// holds 3 (R + 2) rows (dz outer, then y), each the cells x0 - 1 .. x0 + S of
+// one (y, z) row; a particle's 9 candidate rows (dz, dy) are 9 of them, in
+// the same (dz, dy) order.
 #pragma once
 
 #include "kernels.hpp"
@@ -8,52 +13,82 @@
 namespace halmd {
 
 struct seg_stage {
-    unsigned base[10];             // LDS start of each row; base[9] = union size
-    unsigned g[9][2], n[9][2];     // memory ranges of each row
-    unsigned x0, w, pb, pe;        // first cell, cells, particle range of the segment
+    seg_place q;
+    unsigned nr;                            // union rows, 3 (h + 2)
+    unsigned base[SEG_NR + 1];              // LDS start of each union row; base[nr] = U
+    unsigned g[SEG_NR][2], n[SEG_NR][2];    // memory ranges of each union row
+    unsigned own_begin[SEG_RMAX];           // memory start of the segment's particles in row j
+    unsigned own_pre[SEG_RMAX + 1];         // particles of rows before j; own_pre[h] = all
 };
 
-__device__ __forceinline__ unsigned seg_row_start(seg_params const& sp, unsigned seg, unsigned r) {
+constexpr unsigned SEG_BOUND = SEG_MAX + 3;   // bound words per union row
+
+__device__ __forceinline__ unsigned seg_row_start(seg_params const& sp, seg_place const& q, unsigned ur) {
     unsigned const nx = sp.grid.ncell[0], ny = sp.grid.ncell[1], nz = sp.grid.ncell[2];
-    unsigned const t = seg / sp.nsx;
-    unsigned const y = t % ny, z = t / ny;
-    unsigned const kz = (z + nz + r / 3 - 1) % nz, ky = (y + ny + r % 3 - 1) % ny;
+    unsigned const hh = q.h + 2;
+    unsigned const kz = (q.z + nz + ur / hh - 1) % nz, ky = (q.y0 + ny + ur % hh - 1) % ny;
     return (kz * ny + ky) * nx;
 }
 
-// Called by the whole workgroup.  Fills st (and, if bound is given, the LDS
-// start of each of the w + 2 cells of every row: bound[r * (SEG_MAX + 3) + k],
-// k = 0 .. w + 2), then, if U + 1 <= sp.lds_cap, copies the union into
-// s_rec[0 .. U) and a sentinel record into s_rec[U]: fixed-point y and z half
-// a box from the centre of the segment's row, hence at least L / 3 >= rc +
-// skin from each of its particles in both (ny, nz >= 3).  Returns the union
-// size U; every thread sees the same value.
+// own particle t (0 .. own_pre[h] - 1) of the segment: its row j and memory index
+__device__ __forceinline__ void seg_own(seg_stage const& st, unsigned t, unsigned& j, unsigned& i) {
+    unsigned r = 0;
+    for (unsigned k = 1; k < st.q.h; ++k) r += t >= st.own_pre[k] ? 1u : 0u;
+    j = r;
+    i = st.own_begin[r] + (t - st.own_pre[r]);
+}
+
+// Called by the whole workgroup (at least 64 threads).  Fills st (and, if
+// bound is given, the LDS start of each of the w + 2 cells of every union row:
+// bound[ur * SEG_BOUND + k], k = 0 .. w + 2), then, if U + 1 <= sp.lds_cap,
+// copies the union into s_rec[0 .. U) and a sentinel record into s_rec[U]:
+// fixed-point y and z half a box from the centre of the segment's rows and
+// plane, hence at least one cell width (>= rc + skin) from each of its
+// particles in both (h <= ny - 2, nz >= 3).  Returns the union size U; every
+// thread sees the same value.
 __device__ __forceinline__ unsigned stage_union(seg_params const& sp, unsigned seg, uint32_t const* __restrict__ off,
                                                 uint4 const* __restrict__ posq, uint4* s_rec, seg_stage& st,
                                                 unsigned* bound) {
     unsigned const tid = threadIdx.x;
-    if (tid < 9) seg_row(sp, seg, tid, off, st.g[tid], st.n[tid]);
-    if (tid == 9) seg_particles(sp, seg, off, st.x0, st.w, st.pb, st.pe);
+    seg_place const q = seg_locate(sp, seg);
+    unsigned const nr = 3 * (q.h + 2);
+    if (tid < nr) seg_row(sp, q, tid, off, st.g[tid], st.n[tid]);
+    if (tid >= 32 && tid < 32 + q.h) {
+        unsigned b, c;
+        seg_own_row(sp, q, tid - 32, off, b, c);
+        st.own_begin[tid - 32] = b;
+        st.own_pre[tid - 32 + 1] = c;
+    }
+    if (tid == 0) {
+        st.q = q;
+        st.nr = nr;
+    }
     __syncthreads();
     if (tid == 0) {
         unsigned b = 0;
-        for (unsigned r = 0; r < 9; ++r) {
+        for (unsigned r = 0; r < nr; ++r) {
             st.base[r] = b;
             b += st.n[r][0] + st.n[r][1];
         }
-        st.base[9] = b;
+        st.base[nr] = b;
+        unsigned p = 0;
+        st.own_pre[0] = 0;
+        for (unsigned j = 0; j < q.h; ++j) {
+            p += st.own_pre[j + 1];
+            st.own_pre[j + 1] = p;
+        }
     }
     __syncthreads();
-    unsigned const U = st.base[9];
-    if (bound && tid < 9) {
-        // cells x0 - 1 .. x0 + w (periodic) of row tid, in LDS order
+    unsigned const U = st.base[nr];
+    if (bound && tid < nr) {
+        // cells x0 - 1 .. x0 + w (periodic) of union row tid, in LDS order
         unsigned const nx = sp.grid.ncell[0];
-        unsigned const row = seg_row_start(sp, seg, tid);
-        unsigned* const bd = bound + tid * (SEG_MAX + 3);
+        unsigned const row = seg_row_start(sp, q, tid);
+        unsigned* const bd = bound + tid * SEG_BOUND;
         unsigned b = st.base[tid];
         bd[0] = b;
-        for (unsigned k = 0; k < st.w + 2; ++k) {
-            unsigned const x = (st.x0 + nx - 1 + k) % nx;
+        for (unsigned k = 0; k < q.w + 2; ++k) {
+            unsigned const x = (q.x0 + nx - 1 + k) % nx;
             b += off[row + x + 1] - off[row + x];
             bd[k + 1] = b;
         }
@@ -61,10 +96,8 @@ __device__ __forceinline__ unsigned stage_union(seg_params const& sp, unsigned s
     if (U + 1 <= sp.lds_cap) {
         if (tid == 0) {
             unsigned const ny = sp.grid.ncell[1], nz = sp.grid.ncell[2];
-            unsigned const t = seg / sp.nsx;
-            unsigned const y = t % ny, z = t / ny;
-            uint32_t const cy = uint32_t(((2ull * y + 1) << 31) / ny);   // (y + 1/2) 2^32 / ny
-            uint32_t const cz = uint32_t(((2ull * z + 1) << 31) / nz);
+            uint32_t const cy = uint32_t(((2ull * q.y0 + q.h) << 31) / ny);   // (y0 + h/2) 2^32 / ny
+            uint32_t const cz = uint32_t(((2ull * q.z + 1) << 31) / nz);
             s_rec[U] = make_uint4(0u, cy + 0x80000000u, cz + 0x80000000u, 0u);
         }
         // flattened copy: every thread issues eight independent loads before
@@ -73,11 +106,13 @@ __device__ __forceinline__ unsigned stage_union(seg_params const& sp, unsigned s
         // past U load record 0 (U >= 1 inside the loop) and store nothing.
         auto fetch = [&](unsigned u) -> uint4 {
             u = u < U ? u : 0u;
-            unsigned r = 0;
-#pragma unroll
-            for (unsigned q = 1; q < 9; ++q) r += u >= st.base[q] ? 1u : 0u;
-            unsigned const o = u - st.base[r];
-            unsigned const src = o < st.n[r][0] ? st.g[r][0] + o : st.g[r][1] + (o - st.n[r][0]);
+            unsigned lo = 0, hi = nr;   // union row: last r with base[r] <= u
+            while (hi - lo > 1) {
+                unsigned const m = (lo + hi) / 2;
+                if (st.base[m] <= u) lo = m; else hi = m;
+            }
+            unsigned const o = u - st.base[lo];
+            unsigned const src = o < st.n[lo][0] ? st.g[lo][0] + o : st.g[lo][1] + (o - st.n[lo][0]);
             return posq[src];
         };
         unsigned const B = blockDim.x;

@@ -243,10 +243,11 @@ class neighbour(Handle):
 
     @property
     def segments(self):
-        """(cells per segment, workgroup size) of cell-segment lists, (0, 0) otherwise."""
-        c, t = ctypes.c_uint(), ctypes.c_uint()
-        call("halmd_mdsim_neighbours_from_binning_segments", self.handle, ctypes.byref(c), ctypes.byref(t))
-        return c.value, t.value
+        """(cells along x, rows along y, workgroup size) of cell-segment lists, zeros otherwise."""
+        c, r, t = ctypes.c_uint(), ctypes.c_uint(), ctypes.c_uint()
+        call("halmd_mdsim_neighbours_from_binning_segments", self.handle, ctypes.byref(c), ctypes.byref(r),
+             ctypes.byref(t))
+        return c.value, r.value, t.value
 
     @property
     def entry_bytes(self):

@@ -179,10 +179,11 @@ HALMD_API int halmd_mdsim_neighbours_from_binning_size(const halmd_neighbour* nb
  * (particle indices); no counterpart in the reference (its entries are 4-byte
  * indices, from_binning.hpp:105) */
 HALMD_API int halmd_mdsim_neighbours_from_binning_entry_bytes(const halmd_neighbour* nb, unsigned* bytes);
-/* cell-segment shape of the lists (cells per segment, workgroup size); zeros
- * for global-index lists.  No counterpart in the reference */
+/* cell-segment shape of the lists (cells along x and rows along y per
+ * segment, workgroup size); zeros for global-index lists.  No counterpart in
+ * the reference */
 HALMD_API int halmd_mdsim_neighbours_from_binning_segments(const halmd_neighbour* nb, unsigned* cells,
-                                                           unsigned* threads);
+                                                           unsigned* rows, unsigned* threads);
 /* number of neighbours of each particle (memory order) */
 HALMD_API int halmd_mdsim_neighbours_from_binning_counts(halmd_neighbour* nb, unsigned* counts);
 HALMD_API int halmd_mdsim_neighbours_from_binning_rebuilds(const halmd_neighbour* nb, uint64_t* count);

@@ -144,8 +144,8 @@ def test_autotune_keeps_parity(oracle):
     # keeps must give lists and forces that still follow the rules
     w = workloads.lennard_jones(13500, density=0.8442, temperature=1.0, jitter=0.1, seed=36)
     cells = w.force.tune()
-    S, T = w.neighbour.segments
-    assert cells == S >= 1 and T in (128, 192, 256)
+    S, R, T = w.neighbour.segments
+    assert cells == S >= 1 and R >= 1 and T in (128, 192, 256)
     f = w.particle.ordered("force")
     pot = oracle.Potential(oracle.POT_LJ, "shifted", 1.0, 1.0, 2.5)
     fref, _, _ = oracle.pair_force(w.position, w.species, w.length, pot, skin=0.3)
