@@ -15,8 +15,9 @@
 // double word from the exact integers.  Pair potentials are FP32; the
 // per-particle force sum is compensated: plain FP32 over each group of 8 pair
 // forces, Kahan summation over the groups (the counterpart of the reference's
-// dsfloat accumulator).  Energy and stress (aux) are FP32 per
-// particle; the reductions over particles are FP64.
+// dsfloat accumulator).  Energy and stress (aux) are FP32 per particle
+// (compensated the same way in the cell-segment kernel); the reductions over
+// particles are FP64.
 #include "segment.hpp"
 
 namespace halmd {
@@ -209,16 +210,30 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
 // (current fixed-point positions) in LDS, then every lane walks the 16-bit slot
 // list of one particle of the segment, padded to a multiple of 8 with a
 // sentinel record beyond every cutoff (no per-entry bounds test).
-//  - ISO (cubic box): separations stay in fixed-point units, d = float(D);
-//    rr = (d.d) q^2 with q^2 a double word, and the force sum of fval * d is
-//    scaled by q (double word) once per particle; the same quantities as the
-//    general path, with unbiased roundings and fewer operations.
+//  - ISO (cubic box): separations stay in fixed-point units, d = float(D), and
+//    the force sum of fval * d is scaled by q (double word) once per particle.
+//    Lennard-Jones with a sharp or shifted cutoff works on rq = d.d directly:
+//    (sigma / r)^2 = (sigma^2 / q^2) / rq with sigma^2 / q^2 a double word, and
+//    with one species the factor 48 eps / sigma^2 joins q in the per-particle
+//    scale.  Other potentials form rr = rq q^2 (double word).  The same
+//    quantities as the general path, with unbiased roundings and fewer
+//    operations.
 //  - MULTI: pair parameters by species from LDS; otherwise the one parameter
 //    set lives in scalar registers.
-// Summation: FP32 over blocks of 16 pair forces, Kahan over the blocks.
+// Summation: FP32 over blocks of 16 pairs, Kahan over the blocks (force,
+// energy and stress alike).
+// Lanes: lane t % 64 of the wave that covers particles t - t % 64 .. + 63.  The
+// segment's last wave is mostly empty (its particle count mod 64), so there the
+// list of each of its `active` particles is split into k = 64 / active (at most
+// SPLIT_MAX) runs of whole 16-slot blocks, one lane each, and lane p adds the
+// runs of lanes p + active, p + 2 active, ... in that order (Kahan).  Both the
+// split and the order depend only on the segment's particle count, never on
+// the workgroup size.
 // One cell segment of the staged force: the whole workgroup calls it for the
 // segment `seg` of the system described by (posq, off, sp, list, count, cap,
 // P); shared by the single-system kernel and the ensemble kernel.
+constexpr unsigned SPLIT_MAX = 8;
+
 template <int KIND, int TRUNC, bool AUX, bool ISO, bool MULTI>
 __device__ __forceinline__ void staged_force_segment(
     uint4 const* __restrict__ posq, uint32_t const* __restrict__ off, seg_params const& sp, unsigned seg,
@@ -226,47 +241,82 @@ __device__ __forceinline__ void staged_force_segment(
     bool force_zero, float aux_weight, float4* __restrict__ force, float* __restrict__ stress, unsigned stride,
     uint4* s_rec, seg_stage& st, pair_param* s_pair)
 {
+    constexpr bool LJQ =
+        ISO && KIND == POTENTIAL_LENNARD_JONES && (TRUNC == TRUNC_SHARP || TRUNC == TRUNC_SHIFTED);
+    constexpr bool P0_OUT = LJQ && !MULTI;   // 48 eps / sigma^2 applied per particle
     if (MULTI && threadIdx.x < MAX_TYPES * MAX_TYPES) s_pair[threadIdx.x] = P.pair[threadIdx.x];
     stage_union(sp, seg, off, posq, s_rec, st, nullptr);   // fits: the list build checked it
 
     unsigned const T = st.own_pre[st.q.h];
-    for (unsigned t = threadIdx.x; t < T; t += blockDim.x) {
+    unsigned const lane = threadIdx.x % WAVE;
+    // wave-uniform loop: every lane of a wave runs the same iterations (the
+    // cross-lane sums below need the whole wave)
+    for (unsigned t0 = threadIdx.x - lane; t0 < T; t0 += blockDim.x) {
+        unsigned const active = min(T - t0, WAVE);
+        unsigned const k = min(WAVE / active, SPLIT_MAX);   // lanes per particle: 1 in full waves
+        unsigned const part = lane / active;                // run of the list this lane takes
+        unsigned const t = t0 + lane % active;
+        bool const mine = part < k;
+
         unsigned j, i;
         seg_own(st, t, j, i);
         uint4 const qi = posq[i];
         unsigned const ti = (qi.w & 3u) * P.ntype;
         unsigned const n8 = (count[i] + 7u) & ~7u;
+        unsigned const nblk = (n8 + NB16_BLOCK - 1) / NB16_BLOCK;
+        unsigned const b_begin = mine ? nblk * part / k : 0u;
+        unsigned const b_end = mine ? nblk * (part + 1) / k : 0u;
         uint4 const* const lp =
             reinterpret_cast<uint4 const*>(list + size_t(i & ~(WAVE - 1)) * cap + (i & (WAVE - 1)) * NB16_BLOCK);
 
         float fx = 0, fy = 0, fz = 0, cx = 0, cy = 0, cz = 0;
         float bx = 0, by = 0, bz = 0;
-        float en = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
+        float en = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;   // per block of 16 pairs
+        float ka[7] = {}, kc[7] = {};   // aux sums over the blocks (Kahan)
 
         // branch-free: most candidates lie inside the cutoff, so the wave
         // evaluates every pair and a select drops those beyond it
         auto pair = [&](uint4 const qj) {
             int32_t const Dx = int32_t(qi.x - qj.x), Dy = int32_t(qi.y - qj.y), Dz = int32_t(qi.z - qj.z);
             pair_param const& pp = MULTI ? s_pair[ti + (qj.w & 3u)] : P.pair[0];
-            float dx, dy, dz, rr;
+            float dx, dy, dz, fval, u = 0.0f;
             bool inside;
-            if (ISO) {
+            if (LJQ) {
                 dx = float(Dx);
                 dy = float(Dy);
                 dz = float(Dz);
                 float const rq = dx * dx + dy * dy + dz * dz;
                 inside = rq < pp.rr_cut_q;
-                rr = __builtin_fmaf(rq, P.q2[0], rq * P.q2[1]);
+                float const r = rcp_nr(rq);
+                float const sr2 = __builtin_fmaf(r, pp.p_lo[0], r * pp.p_lo[1]);   // (sigma / r)^2
+                float const x2 = sr2 * sr2;
+                float const x4 = x2 * x2;
+                float const h = __builtin_fmaf(x2, sr2, -0.5f);                    // (sigma / r)^6 - 1/2
+                fval = P0_OUT ? x4 * h : pp.p[0] * x4 * h;
+                if (AUX) {
+                    float const ri6 = x2 * sr2;
+                    u = pp.p[1] * __builtin_fmaf(ri6, ri6, -ri6);
+                    if (TRUNC == TRUNC_SHIFTED) u -= pp.en_cut;
+                }
             } else {
-                dx = separation(Dx, P.q[0], P.q_lo[0]);
-                dy = separation(Dy, P.q[1], P.q_lo[1]);
-                dz = separation(Dz, P.q[2], P.q_lo[2]);
-                rr = dx * dx + dy * dy + dz * dz;
-                inside = rr < pp.rr_cut;
+                float rr;
+                if (ISO) {
+                    dx = float(Dx);
+                    dy = float(Dy);
+                    dz = float(Dz);
+                    float const rq = dx * dx + dy * dy + dz * dz;
+                    inside = rq < pp.rr_cut_q;
+                    rr = __builtin_fmaf(rq, P.q2[0], rq * P.q2[1]);
+                } else {
+                    dx = separation(Dx, P.q[0], P.q_lo[0]);
+                    dy = separation(Dy, P.q[1], P.q_lo[1]);
+                    dz = separation(Dz, P.q[2], P.q_lo[2]);
+                    rr = dx * dx + dy * dy + dz * dz;
+                    inside = rr < pp.rr_cut;
+                }
+                float const md = KIND == POTENTIAL_MORSE ? morse_argument(Dx, Dy, Dz, P, pp) : 0.0f;
+                pair_eval<KIND, TRUNC>(rr, md, pp, P.rri_smooth, fval, u);
             }
-            float const md = KIND == POTENTIAL_MORSE ? morse_argument(Dx, Dy, Dz, P, pp) : 0.0f;
-            float fval, u;
-            pair_eval<KIND, TRUNC>(rr, md, pp, P.rri_smooth, fval, u);
             fval = inside ? fval : 0.0f;
             bx += fval * dx;
             by += fval * dy;
@@ -295,6 +345,12 @@ __device__ __forceinline__ void staged_force_segment(
             kahan_add(fy, cy, by);
             kahan_add(fz, cz, bz);
             bx = by = bz = 0;
+            if (AUX) {
+                float const blk[7] = {en, s0, s1, s2, s3, s4, s5};
+#pragma unroll
+                for (int c = 0; c < 7; ++c) kahan_add(ka[c], kc[c], blk[c]);
+                en = s0 = s1 = s2 = s3 = s4 = s5 = 0;
+            }
         };
         // blocks of 16 slots (two uint4 = two groups); the next block is
         // loaded while the current one is evaluated
@@ -303,8 +359,9 @@ __device__ __forceinline__ void staged_force_segment(
         // cap / 16 blocks, so that the wait for block b does not also wait
         // for the prefetch of block b + 1)
         unsigned const last = cap / NB16_BLOCK - 1;
-        uint4 w0 = lp[0], w1 = lp[1];
-        for (unsigned b = 0; b * NB16_BLOCK < n8; ++b) {
+        unsigned const bf = min(b_begin, last);
+        uint4 w0 = lp[bf * BLOCK_STRIDE], w1 = lp[bf * BLOCK_STRIDE + 1];
+        for (unsigned b = b_begin; b < b_end; ++b) {
             unsigned const bn = min(b + 1, last);
             uint4 const n0 = lp[bn * BLOCK_STRIDE], n1 = lp[bn * BLOCK_STRIDE + 1];
             group(w0);
@@ -313,18 +370,51 @@ __device__ __forceinline__ void staged_force_segment(
             w0 = n0;
             w1 = n1;
         }
+        float rx = fx - cx, ry = fy - cy, rz = fz - cz;
+        if (AUX) {
+            en = ka[0] - kc[0]; s0 = ka[1] - kc[1]; s1 = ka[2] - kc[2]; s2 = ka[3] - kc[3];
+            s3 = ka[4] - kc[4]; s4 = ka[5] - kc[5]; s5 = ka[6] - kc[6];
+        }
+        if (k > 1) {   // wave-uniform
+            // lane p (part 0) collects the runs of lanes p + active, ... in order
+            // (the shuffles read every lane's own run, never a running sum)
+            float sx = rx, sy = ry, sz = rz, ex = 0, ey = 0, ez = 0;
+            float const aux[7] = {en, s0, s1, s2, s3, s4, s5};
+            float acc[7] = {en, s0, s1, s2, s3, s4, s5};
+            for (unsigned m = 1; m < k; ++m) {
+                unsigned const src = lane + m * active;
+                int const sl = int(src < WAVE ? src : lane);
+                kahan_add(sx, ex, __shfl(rx, sl, WAVE));
+                kahan_add(sy, ey, __shfl(ry, sl, WAVE));
+                kahan_add(sz, ez, __shfl(rz, sl, WAVE));
+                if (AUX) {
+#pragma unroll
+                    for (int c = 0; c < 7; ++c) acc[c] += __shfl(aux[c], sl, WAVE);
+                }
+            }
+            rx = sx - ex;
+            ry = sy - ey;
+            rz = sz - ez;
+            if (AUX) {
+                en = acc[0]; s0 = acc[1]; s1 = acc[2]; s2 = acc[3]; s3 = acc[4]; s4 = acc[5]; s5 = acc[6];
+            }
+        }
+        if (part != 0) continue;
         float4 f;
         float sc[6];
         if (ISO) {
-            // back from fixed-point units: force * q, stress * q^2
-            f = make_float4(separation_scale(fx - cx, P.q[0], P.q_lo[0]), separation_scale(fy - cy, P.q[0], P.q_lo[0]),
-                            separation_scale(fz - cz, P.q[0], P.q_lo[0]), AUX ? aux_weight * en : 0.0f);
-            float const s2q = aux_weight * P.q2[0], s2l = aux_weight * P.q2[1];
+            // back from fixed-point units: force * q, stress * q^2 (times
+            // 48 eps / sigma^2 where that was taken out of the pair sum)
+            float const f_hi = P0_OUT ? P.q_p0[0] : P.q[0], f_lo = P0_OUT ? P.q_p0[1] : P.q_lo[0];
+            float const s_hi = P0_OUT ? P.q2_p0[0] : P.q2[0], s_lo = P0_OUT ? P.q2_p0[1] : P.q2[1];
+            f = make_float4(separation_scale(rx, f_hi, f_lo), separation_scale(ry, f_hi, f_lo),
+                            separation_scale(rz, f_hi, f_lo), AUX ? aux_weight * en : 0.0f);
+            float const s2q = aux_weight * s_hi, s2l = aux_weight * s_lo;
             float const sv[6] = {s0, s1, s2, s3, s4, s5};
 #pragma unroll
             for (int c = 0; c < 6; ++c) sc[c] = __builtin_fmaf(sv[c], s2q, sv[c] * s2l);
         } else {
-            f = make_float4(fx - cx, fy - cy, fz - cz, AUX ? aux_weight * en : 0.0f);
+            f = make_float4(rx, ry, rz, AUX ? aux_weight * en : 0.0f);
             sc[0] = aux_weight * s0; sc[1] = aux_weight * s1; sc[2] = aux_weight * s2;
             sc[3] = aux_weight * s3; sc[4] = aux_weight * s4; sc[5] = aux_weight * s5;
         }

@@ -194,7 +194,8 @@ enum truncation_kind : int { TRUNC_SHARP = 0, TRUNC_SHIFTED = 1, TRUNC_FORCE_SHI
 // per species pair, FP32
 struct pair_param {
     float p[4];        // LJ: {48 eps / sigma^2, 4 eps, sigma^2, -}; Morse: {eps/A, 1/(sigma B), r_min_sigma/B, A}
-    float p_lo[4];     // Morse: low words of p[1], p[2] (double-word constants of the exponent)
+    float p_lo[4];     // Morse: low words of p[1], p[2] (double-word constants of the exponent);
+                       // LJ, cubic boxes: sigma^2 / q^2 as a double word in p_lo[0], p_lo[1]
     float rr_cut, r_cut, en_cut, force_cut;
     float rr_cut_q;    // rr_cut in fixed-point units squared, rr_cut / q^2 (cubic boxes)
 };
@@ -205,6 +206,7 @@ struct force_params {
     float q[3], q_lo[3];   // L * 2^-32 as a double word
     unsigned iso;          // cubic box: q the same in all dimensions
     float q2[2];           // q^2 as a double word (cubic boxes)
+    float q_p0[2], q2_p0[2];   // q p[0] and q^2 p[0] of pair 0 as double words (cubic boxes, one species)
     pair_param pair[MAX_TYPES * MAX_TYPES];
 };
 // force_zero: overwrite force/en_pot/stress, else add to the stored values

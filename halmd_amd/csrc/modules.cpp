@@ -432,6 +432,11 @@ force_params potential::device_params(box const& b) const {
                 pp.p[1] = float(4 * eps);
                 pp.p[2] = float(sig * sig);
                 pp.p[3] = 0;
+                split_ff(sig * sig / (q * q), pp.p_lo[0], pp.p_lo[1]);
+                if (ab == 0) {
+                    split_ff(q * (48 * eps / (sig * sig)), P.q_p0[0], P.q_p0[1]);
+                    split_ff(q * q * (48 * eps / (sig * sig)), P.q2_p0[0], P.q2_p0[1]);
+                }
             } else {
                 double const B = distortion[ab];
                 double const A = 2 * B * B - 1;
