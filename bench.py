@@ -189,6 +189,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default cell-segment shape")
+    ap.add_argument("--segments", default=None,
+                    help="cell-segment shape 'cells,rows,threads' instead of autotuning (profiling runs)")
     ap.add_argument("--workload", default="lj", choices=["lj", "batched"],
                     help="lj: LJ fluid N = 2^20 (default); batched: ensemble of independent systems")
     ap.add_argument("--systems", type=int, default=256, help="systems of the batched workload")
@@ -210,7 +212,10 @@ def main():
     log(f"device {arch}, {cus} CUs, {mem / 2**30:.0f} GiB; building LJ N={N}")
     t0 = time.perf_counter()
     w = build_workload(args.seed)
-    if not args.no_tune:
+    if args.segments:
+        w.neighbour.set_segments(*[int(x) for x in args.segments.split(",")])
+        log(f"cell segments set to {w.neighbour.segments} (cells, rows, workgroup size)")
+    elif not args.no_tune:
         cells = w.force.tune()
         log(f"autotuned cell segments: {w.neighbour.segments} (cells, rows, workgroup size)")
     log(f"setup {time.perf_counter() - t0:.1f} s; warmup {args.warmup} steps")

@@ -368,6 +368,15 @@ HALMD_API int halmd_mdsim_neighbours_from_binning_segments(const halmd_neighbour
         *threads = n->staged ? n->sp.threads : 0u;
     });
 }
+HALMD_API int halmd_mdsim_neighbours_from_binning_set_segments(halmd_neighbour* nb, unsigned cells, unsigned rows,
+                                                               unsigned threads) {
+    return guard([&] {
+        auto* n = as<neighbour>(nb);
+        if (!n->staged) return;
+        n->set_segments(cells, rows, threads);
+        n->built = false;
+    });
+}
 HALMD_API int halmd_mdsim_neighbours_from_binning_rebuilds(const halmd_neighbour* nb, uint64_t* count) {
     return guard([&] { *count = as<neighbour>(nb)->rebuilds; });
 }

@@ -38,6 +38,17 @@ __device__ __forceinline__ void seg_own(seg_stage const& st, unsigned t, unsigne
     i = st.own_begin[r] + (t - st.own_pre[r]);
 }
 
+// inclusive sum over the lanes of a wave
+__device__ __forceinline__ unsigned wave_inclusive_sum(unsigned x) {
+    unsigned const lane = threadIdx.x % WAVE;
+#pragma unroll
+    for (unsigned d = 1; d < WAVE; d <<= 1) {
+        unsigned const t = __shfl_up(x, d, WAVE);
+        if (lane >= d) x += t;
+    }
+    return x;
+}
+
 // Called by the whole workgroup (at least 64 threads).  Fills st (and, if
 // bound is given, the LDS start of each of the w + 2 cells of every union row:
 // bound[ur * SEG_BOUND + k], k = 0 .. w + 2), then, if U + 1 <= sp.lds_cap,
@@ -46,40 +57,42 @@ __device__ __forceinline__ void seg_own(seg_stage const& st, unsigned t, unsigne
 // plane, hence at least one cell width (>= rc + skin) from each of its
 // particles in both (h <= ny - 2, nz >= 3).  Returns the union size U; every
 // thread sees the same value.
+//
+// Wave 0 locates the segment and its rows (lanes 0 .. nr - 1 the union rows,
+// lanes 32 .. 32 + h - 1 the own rows) and forms both prefix sums by a wave
+// scan; then wave v copies union rows v, v + W, ... (W waves), each in chunks
+// of four 64-record loads per lane issued before their stores.
 __device__ __forceinline__ unsigned stage_union(seg_params const& sp, unsigned seg, uint32_t const* __restrict__ off,
                                                 uint4 const* __restrict__ posq, uint4* s_rec, seg_stage& st,
                                                 unsigned* bound) {
-    unsigned const tid = threadIdx.x;
-    seg_place const q = seg_locate(sp, seg);
-    unsigned const nr = 3 * (q.h + 2);
-    if (tid < nr) seg_row(sp, q, tid, off, st.g[tid], st.n[tid]);
-    if (tid >= 32 && tid < 32 + q.h) {
-        unsigned b, c;
-        seg_own_row(sp, q, tid - 32, off, b, c);
-        st.own_begin[tid - 32] = b;
-        st.own_pre[tid - 32 + 1] = c;
-    }
-    if (tid == 0) {
-        st.q = q;
-        st.nr = nr;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        unsigned b = 0;
-        for (unsigned r = 0; r < nr; ++r) {
-            st.base[r] = b;
-            b += st.n[r][0] + st.n[r][1];
+    unsigned const tid = threadIdx.x, lane = tid % WAVE;
+    if (tid < WAVE) {
+        seg_place const q = seg_locate(sp, seg);
+        unsigned const nr = 3 * (q.h + 2);   // <= SEG_NR = 30 < 32
+        unsigned rows = 0, own = 0;
+        if (lane < nr) {
+            seg_row(sp, q, lane, off, st.g[lane], st.n[lane]);
+            rows = st.n[lane][0] + st.n[lane][1];
         }
-        st.base[nr] = b;
-        unsigned p = 0;
-        st.own_pre[0] = 0;
-        for (unsigned j = 0; j < q.h; ++j) {
-            p += st.own_pre[j + 1];
-            st.own_pre[j + 1] = p;
+        if (lane >= 32 && lane < 32 + q.h) {
+            unsigned b;
+            seg_own_row(sp, q, lane - 32, off, b, own);
+            st.own_begin[lane - 32] = b;
+        }
+        unsigned const rsum = wave_inclusive_sum(rows);   // lanes >= nr carry the total U
+        unsigned const osum = wave_inclusive_sum(own);    // (own is zero below lane 32)
+        if (lane < nr) st.base[lane] = rsum - rows;
+        if (lane == nr) st.base[nr] = rsum;
+        if (lane >= 32 && lane < 32 + q.h) st.own_pre[lane - 32 + 1] = osum;
+        if (lane == 0) {
+            st.q = q;
+            st.nr = nr;
+            st.own_pre[0] = 0;
         }
     }
     __syncthreads();
-    unsigned const U = st.base[nr];
+    seg_place const q = st.q;
+    unsigned const nr = st.nr, U = st.base[nr];
     if (bound && tid < nr) {
         // cells x0 - 1 .. x0 + w (periodic) of union row tid, in LDS order
         unsigned const nx = sp.grid.ncell[0];
@@ -100,34 +113,23 @@ __device__ __forceinline__ unsigned stage_union(seg_params const& sp, unsigned s
             uint32_t const cz = uint32_t(((2ull * q.z + 1) << 31) / nz);
             s_rec[U] = make_uint4(0u, cy + 0x80000000u, cz + 0x80000000u, 0u);
         }
-        // flattened copy: every thread issues eight independent loads before
-        // it stores any of them, so a workgroup waits for about
-        // U / (8 * blockDim) memory latencies instead of one per row.  Slots
-        // past U load record 0 (U >= 1 inside the loop) and store nothing.
-        auto fetch = [&](unsigned u) -> uint4 {
-            u = u < U ? u : 0u;
-            unsigned lo = 0, hi = nr;   // union row: last r with base[r] <= u
-            while (hi - lo > 1) {
-                unsigned const m = (lo + hi) / 2;
-                if (st.base[m] <= u) lo = m; else hi = m;
+        unsigned const W = blockDim.x / WAVE;
+        for (unsigned r = tid / WAVE; r < nr; r += W) {   // wave-uniform
+            unsigned const b = st.base[r], g0 = st.g[r][0], n0 = st.n[r][0], g1 = st.g[r][1];
+            unsigned const n = n0 + st.n[r][1];
+            for (unsigned u0 = 0; u0 < n; u0 += 4 * WAVE) {
+                // slots at or past n read record 0 and store nothing
+                auto src = [&](unsigned u) { return u < n ? (u < n0 ? g0 + u : g1 + (u - n0)) : 0u; };
+                unsigned const u = u0 + lane;
+                uint4 const a0 = posq[src(u)];
+                uint4 const a1 = u0 + WAVE < n ? posq[src(u + WAVE)] : a0;
+                uint4 const a2 = u0 + 2 * WAVE < n ? posq[src(u + 2 * WAVE)] : a0;
+                uint4 const a3 = u0 + 3 * WAVE < n ? posq[src(u + 3 * WAVE)] : a0;
+                if (u < n) s_rec[b + u] = a0;
+                if (u + WAVE < n) s_rec[b + u + WAVE] = a1;
+                if (u + 2 * WAVE < n) s_rec[b + u + 2 * WAVE] = a2;
+                if (u + 3 * WAVE < n) s_rec[b + u + 3 * WAVE] = a3;
             }
-            unsigned const o = u - st.base[lo];
-            unsigned const src = o < st.n[lo][0] ? st.g[lo][0] + o : st.g[lo][1] + (o - st.n[lo][0]);
-            return posq[src];
-        };
-        unsigned const B = blockDim.x;
-        for (unsigned u0 = tid; u0 < U; u0 += 8 * B) {
-            uint4 const a0 = fetch(u0), a1 = fetch(u0 + B), a2 = fetch(u0 + 2 * B), a3 = fetch(u0 + 3 * B);
-            uint4 const a4 = fetch(u0 + 4 * B), a5 = fetch(u0 + 5 * B), a6 = fetch(u0 + 6 * B),
-                        a7 = fetch(u0 + 7 * B);
-            s_rec[u0] = a0;
-            if (u0 + B < U) s_rec[u0 + B] = a1;
-            if (u0 + 2 * B < U) s_rec[u0 + 2 * B] = a2;
-            if (u0 + 3 * B < U) s_rec[u0 + 3 * B] = a3;
-            if (u0 + 4 * B < U) s_rec[u0 + 4 * B] = a4;
-            if (u0 + 5 * B < U) s_rec[u0 + 5 * B] = a5;
-            if (u0 + 6 * B < U) s_rec[u0 + 6 * B] = a6;
-            if (u0 + 7 * B < U) s_rec[u0 + 7 * B] = a7;
         }
     }
     __syncthreads();

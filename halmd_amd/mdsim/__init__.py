@@ -249,6 +249,10 @@ class neighbour(Handle):
              ctypes.byref(t))
         return c.value, r.value, t.value
 
+    def set_segments(self, cells, rows, threads):
+        """Sets the cell-segment shape; the lists are rebuilt at their next use."""
+        call("halmd_mdsim_neighbours_from_binning_set_segments", self.handle, int(cells), int(rows), int(threads))
+
     @property
     def entry_bytes(self):
         """Bytes per list entry: 2 (16-bit slots, cell-ordered particles) or 4 (indices)."""

@@ -184,6 +184,12 @@ HALMD_API int halmd_mdsim_neighbours_from_binning_entry_bytes(const halmd_neighb
  * the reference */
 HALMD_API int halmd_mdsim_neighbours_from_binning_segments(const halmd_neighbour* nb, unsigned* cells,
                                                            unsigned* rows, unsigned* threads);
+/* sets the cell-segment shape (clamped to the grid; threads a multiple of 64,
+ * at most 256) and rebuilds the lists at their next use; ignored for
+ * global-index lists.  No counterpart in the reference (its from_binning
+ * tuning knobs are the algorithm and unroll options, from_binning.cpp:328-339) */
+HALMD_API int halmd_mdsim_neighbours_from_binning_set_segments(halmd_neighbour* nb, unsigned cells, unsigned rows,
+                                                               unsigned threads);
 /* number of neighbours of each particle (memory order) */
 HALMD_API int halmd_mdsim_neighbours_from_binning_counts(halmd_neighbour* nb, unsigned* counts);
 HALMD_API int halmd_mdsim_neighbours_from_binning_rebuilds(const halmd_neighbour* nb, uint64_t* count);
