@@ -12,9 +12,11 @@
 // the box length as L_hi + L_lo in the wrap, so that neither the FP32 rounding
 // of dt nor of L enters the trajectory.  The integrate kernel also tracks the
 // largest squared displacement since the last neighbour-list build (high
-// words, minimum image), reduced per wave and folded into one device word with
-// atomicMax: the rebuild criterion of halmd/mdsim/gpu/max_displacement.cpp
-// without a separate pass over the positions.
+// words, minimum image), reduced per wave and folded with atomicMax into one
+// of DISP_SLOTS device words on separate cache lines (by wave index, so that
+// no line sees many atomics; the reader takes the largest): the rebuild criterion of
+// halmd/mdsim/gpu/max_displacement.cpp without a separate pass over the
+// positions.
 #include "kernels.hpp"
 
 namespace halmd {
@@ -42,21 +44,16 @@ __global__ void quantize_kernel(float4 const* __restrict__ pos, unsigned n, quan
     if (i < n) posq[i] = quantize4(pos[2 * size_t(i)], pos[2 * size_t(i) + 1], q);
 }
 
-// fold the block maximum of d2 (>= 0) into *target (float bits); the atomic is
-// skipped when the stored value is already larger, so only blocks that raise
-// the running maximum contend for the word
-__device__ __forceinline__ void block_max_to(float d2, uint32_t* target) {
-    __shared__ float s_max[256 / WAVE];
+// fold the wave's maximum of d2 (>= 0) into *word (float bits) with an atomic
+// whose result is not used: no barrier, and no wave waits for it
+__device__ __forceinline__ void wave_max_to(float d2, uint32_t* word) {
     d2 = wave_max(d2);
-    if (threadIdx.x % WAVE == 0) s_max[threadIdx.x / WAVE] = d2;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float m = s_max[0];
-        for (unsigned w = 1; w < blockDim.x / WAVE; ++w) m = fmaxf(m, s_max[w]);
-        uint32_t const bits = __float_as_uint(m);
-        uint32_t const cur = __hip_atomic_load(target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (m > 0.0f && bits > cur) atomicMax(target, bits);
-    }
+    if (threadIdx.x % WAVE == 0 && d2 > 0.0f) atomicMax(word, __float_as_uint(d2));
+}
+
+// the slot of this wave in a slot set
+__device__ __forceinline__ uint32_t* disp_slot(uint32_t* slots) {
+    return slots + ((blockIdx.x * blockDim.x + threadIdx.x) / WAVE) % DISP_SLOTS * DISP_STRIDE;
 }
 
 __device__ __forceinline__ ff ff_from_double(double x) {
@@ -74,24 +71,29 @@ __device__ __forceinline__ float integrate_particle(float4* __restrict__ pos, fl
                                                    float pre_half_dt, double const* __restrict__ scale,
                                                    float4 const* __restrict__ r0) {
     float d2 = 0.0f;
+    // every load up front (r0 too): a load issued after the stores would
+    // wait for them and add a second memory round trip to each wave
     float4 rh = pos[2 * size_t(i)], rl = pos[2 * size_t(i) + 1];
     float4 vh = vel[2 * size_t(i)], vl = vel[2 * size_t(i) + 1];
     float4 const f = force[i];
+    float4 const q0 = r0 ? r0[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float const m = vh.w;
     ff vx{vh.x, vl.x}, vy{vh.y, vl.y}, vz{vh.z, vl.z};
     if (pre_half_dt != 0.0f) {
         // finalize of the previous step, as verlet_finalize_kernel computes it
-        vx = ff_add_f(vx, f.x * pre_half_dt / m);
-        vy = ff_add_f(vy, f.y * pre_half_dt / m);
-        vz = ff_add_f(vz, f.z * pre_half_dt / m);
+        float const k = pre_half_dt / m;
+        vx = ff_add_f(vx, f.x * k);
+        vy = ff_add_f(vy, f.y * k);
+        vz = ff_add_f(vz, f.z * k);
     }
     if (scale) {
         ff const s = ff_from_double(*scale);
         vx = ff_mul(vx, s); vy = ff_mul(vy, s); vz = ff_mul(vz, s);
     }
-    vx = ff_add_f(vx, f.x * p.half_dt / m);
-    vy = ff_add_f(vy, f.y * p.half_dt / m);
-    vz = ff_add_f(vz, f.z * p.half_dt / m);
+    float const k = p.half_dt / m;
+    vx = ff_add_f(vx, f.x * k);
+    vy = ff_add_f(vy, f.y * k);
+    vz = ff_add_f(vz, f.z * k);
     ff const dt{p.dt_hi, p.dt_lo};
     ff rx = ff_add(ff{rh.x, rl.x}, ff_mul(vx, dt));
     ff ry = ff_add(ff{rh.y, rl.y}, ff_mul(vy, dt));
@@ -114,8 +116,7 @@ __device__ __forceinline__ float integrate_particle(float4* __restrict__ pos, fl
         image[i] = im;
     }
     if (r0) {
-        float4 const q = r0[i];
-        float dx = rx.hi - q.x, dy = ry.hi - q.y, dz = rz.hi - q.z;
+        float dx = rx.hi - q0.x, dy = ry.hi - q0.y, dz = rz.hi - q0.z;
         dx -= rintf(dx * p.invL[0]) * p.L_hi[0];
         dy -= rintf(dy * p.invL[1]) * p.L_hi[1];
         dz -= rintf(dz * p.invL[2]) * p.L_hi[2];
@@ -130,10 +131,10 @@ __global__ __launch_bounds__(256) void verlet_integrate_kernel(
     float4 const* __restrict__ r0, uint32_t* __restrict__ max_disp2, uint32_t* __restrict__ clear_slot)
 {
     unsigned const i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (clear_slot && i == 0) *clear_slot = 0u;
+    if (clear_slot && i < DISP_SLOTS) clear_slot[i * DISP_STRIDE] = 0u;
     float d2 = 0.0f;
     if (i < n) d2 = integrate_particle(pos, vel, force, image, posq, i, p, pre_half_dt, scale, r0);
-    if (max_disp2) block_max_to(d2, max_disp2);
+    if (max_disp2) wave_max_to(d2, disp_slot(max_disp2));
 }
 
 __global__ __launch_bounds__(256) void verlet_finalize_kernel(float4* __restrict__ vel, float4 const* __restrict__ force,
@@ -143,9 +144,10 @@ __global__ __launch_bounds__(256) void verlet_finalize_kernel(float4* __restrict
     float4 const vh = vel[2 * size_t(i)], vl = vel[2 * size_t(i) + 1];
     float4 const f = force[i];
     float const m = vh.w;
-    ff const vx = ff_add_f(ff{vh.x, vl.x}, f.x * half_dt / m);
-    ff const vy = ff_add_f(ff{vh.y, vl.y}, f.y * half_dt / m);
-    ff const vz = ff_add_f(ff{vh.z, vl.z}, f.z * half_dt / m);
+    float const k = half_dt / m;   // the same kick as the deferred one in integrate_particle
+    ff const vx = ff_add_f(ff{vh.x, vl.x}, f.x * k);
+    ff const vy = ff_add_f(ff{vh.y, vl.y}, f.y * k);
+    ff const vz = ff_add_f(ff{vh.z, vl.z}, f.z * k);
     vel[2 * size_t(i)] = make_float4(vx.hi, vy.hi, vz.hi, m);
     vel[2 * size_t(i) + 1] = make_float4(vx.lo, vy.lo, vz.lo, 0.0f);
 }
@@ -182,7 +184,7 @@ __global__ __launch_bounds__(256) void max_displacement_kernel(float4 const* __r
         dz -= rintf(dz * iz_) * Lz;
         d2 = dx * dx + dy * dy + dz * dz;
     }
-    block_max_to(d2, max_disp2);
+    wave_max_to(d2, disp_slot(max_disp2));
 }
 
 // SplitMix64 finaliser (Steele, Lea & Flood, "Fast splittable pseudorandom
@@ -210,9 +212,10 @@ __global__ __launch_bounds__(256) void andersen_finalize_kernel(float4* __restri
     uint64_t const key = mix64(seed ^ mix64(step * 0x632be59bd9b4e019ull + id[i]));
     if (u01(mix64(key)) > coll_prob) {
         float4 const f = force[i];
-        ff const vx = ff_add_f(ff{vh.x, vl.x}, f.x * half_dt / m);
-        ff const vy = ff_add_f(ff{vh.y, vl.y}, f.y * half_dt / m);
-        ff const vz = ff_add_f(ff{vh.z, vl.z}, f.z * half_dt / m);
+        float const k = half_dt / m;
+        ff const vx = ff_add_f(ff{vh.x, vl.x}, f.x * k);
+        ff const vy = ff_add_f(ff{vh.y, vl.y}, f.y * k);
+        ff const vz = ff_add_f(ff{vh.z, vl.z}, f.z * k);
         vel[2 * size_t(i)] = make_float4(vx.hi, vy.hi, vz.hi, m);
         vel[2 * size_t(i) + 1] = make_float4(vx.lo, vy.lo, vz.lo, 0.0f);
     } else {
@@ -243,7 +246,7 @@ __global__ __launch_bounds__(ENS_BLOCK) void ens_integrate_kernel(
     if (li < d.n)
         d2 = integrate_particle(pos, vel, force, image, posq, d.base + li, d.ip, pending ? d.ip.half_dt : 0.0f,
                                 nullptr, r0);
-    block_max_to(d2, disp2 + s);
+    wave_max_to(d2, disp2 + s);
 }
 
 // apply the deferred finalize kick of every system (before velocities are read)
@@ -257,9 +260,10 @@ __global__ __launch_bounds__(ENS_BLOCK) void ens_settle_kernel(float4* __restric
     float4 const vh = vel[2 * i], vl = vel[2 * i + 1];
     float4 const f = force[i];
     float const m = vh.w;
-    ff const vx = ff_add_f(ff{vh.x, vl.x}, f.x * d.ip.half_dt / m);
-    ff const vy = ff_add_f(ff{vh.y, vl.y}, f.y * d.ip.half_dt / m);
-    ff const vz = ff_add_f(ff{vh.z, vl.z}, f.z * d.ip.half_dt / m);
+    float const k = d.ip.half_dt / m;
+    ff const vx = ff_add_f(ff{vh.x, vl.x}, f.x * k);
+    ff const vy = ff_add_f(ff{vh.y, vl.y}, f.y * k);
+    ff const vz = ff_add_f(ff{vh.z, vl.z}, f.z * k);
     vel[2 * i] = make_float4(vx.hi, vy.hi, vz.hi, m);
     vel[2 * i + 1] = make_float4(vx.lo, vy.lo, vz.lo, 0.0f);
 }

@@ -222,6 +222,13 @@ void launch_pair_force_staged(uint4 const* posq, uint32_t const* offset, seg_par
 // ---------------------------------------------------------------------------
 // integrators (integrate.hip)
 // ---------------------------------------------------------------------------
+// displacement maxima: each wave folds its maximum into slot (global wave
+// index) % DISP_SLOTS of a slot set, the slots DISP_STRIDE words (256 bytes)
+// apart so that their atomics proceed on different cache lines; the reader
+// takes the largest slot
+constexpr unsigned DISP_SLOTS = 64;
+constexpr unsigned DISP_STRIDE = 64;
+constexpr unsigned DISP_SET = DISP_SLOTS * DISP_STRIDE;   // words per slot set
 struct integrate_params {
     float dt_hi, dt_lo, half_dt;
     float L_hi[3], L_lo[3], invL[3];
@@ -231,8 +238,8 @@ struct integrate_params {
 // of the previous step (pre_half_dt != 0: v += F dt/2m first, the same
 // arithmetic as the finalize kernel), optional velocity pre-scale (*scale,
 // device) and optional max-displacement tracking against r0 (max |r - r0|^2 as
-// float bits folded into *max_disp2; *clear_slot, the slot of the next step,
-// is set to zero)
+// float bits folded into the slot set at max_disp2; the slots of the set at
+// clear_slot, that of the next step, are set to zero)
 void launch_verlet_integrate(float4* pos, float4* vel, float4 const* force, int4* image, uint4* posq, unsigned n,
                              integrate_params const& p, float pre_half_dt, double const* scale, float4 const* r0,
                              uint32_t* max_disp2, uint32_t* clear_slot, hipStream_t s);

@@ -522,14 +522,14 @@ void binning::cell_lists(unsigned* begin_out, unsigned* end_out, unsigned* index
 max_displacement::max_displacement(particle& p_, box const& b_) : p(p_), b(b_) {
     p.attach_box(b);
     r0.resize(p.n);
-    d_max2.resize(2);
-    h_max2.resize(1);
+    d_max2.resize(2 * DISP_SET);
+    h_max2.resize(DISP_SLOTS);
     zero();
 }
 
 void max_displacement::zero() {
     launch_copy_hi(p.pos, r0, p.n, p.stream);
-    launch_fill_u32(d_max2, 0u, 2, p.stream);
+    launch_fill_u32(d_max2, 0u, 2 * DISP_SET, p.stream);
     slot = 0;
     tracked_version = p.position_version;
     computed_version = p.position_version;
@@ -540,20 +540,26 @@ uint32_t* max_displacement::fuse_slot() {
     // the integrate kernel folds the new positions into the current slot and
     // clears the other one, which the next step will use
     tracked_version = p.position_version + 1;
-    return d_max2.get() + slot;
+    return d_max2.get() + slot * DISP_SET;
 }
+
+uint32_t* max_displacement::next_slot() { return d_max2.get() + (slot ^ 1u) * DISP_SET; }
 
 double max_displacement::compute() {
     if (computed_version == p.position_version) return value;
     hipStream_t s = p.stream;
+    uint32_t* const cur = d_max2.get() + slot * DISP_SET;
     if (tracked_version != p.position_version) {
-        launch_fill_u32(d_max2.get() + slot, 0u, 1, s);
-        launch_max_displacement(p.pos, r0, p.n, b.Lf, b.invL, d_max2.get() + slot, s);
+        launch_fill_u32(cur, 0u, DISP_SET, s);
+        launch_max_displacement(p.pos, r0, p.n, b.Lf, b.invL, cur, s);
     }
-    HIP_CHECK(hipMemcpyAsync(h_max2.get(), d_max2.get() + slot, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpy2DAsync(h_max2.get(), sizeof(uint32_t), cur, DISP_STRIDE * sizeof(uint32_t), sizeof(uint32_t),
+                               DISP_SLOTS, hipMemcpyDeviceToHost, s));
     p.sync();
+    // the words hold float bits of non-negative values: integer order is float order
+    uint32_t const bits = *std::max_element(h_max2.get(), h_max2.get() + DISP_SLOTS);
     float m2;
-    std::memcpy(&m2, h_max2.get(), 4);
+    std::memcpy(&m2, &bits, 4);
     value = std::sqrt(double(m2));
     computed_version = p.position_version;
     slot ^= 1u;   // the next fused step writes the slot cleared by this one
@@ -886,7 +892,7 @@ void verlet::integrate_kernel(double const* scale) {
         // track the next step inside the integrate kernel
         r0 = fused_disp->r0;
         slot = fused_disp->fuse_slot();
-        clear = fused_disp->d_max2.get() + (fused_disp->slot ^ 1u);
+        clear = fused_disp->next_slot();
     }
     float const pre = p.take_pending_kick();
     timer_integrate.start(p.stream);

@@ -185,7 +185,7 @@ public:
     particle& p;
     box const& b;
     dbuf<float4> r0;
-    dbuf<uint32_t> d_max2;            // two slots: current step and next step
+    dbuf<uint32_t> d_max2;            // two slot sets (DISP_SET words): current step and next step
     hbuf<uint32_t> h_max2;
     unsigned slot = 0;
     uint64_t tracked_version = 0;     // position version whose max is in slot (fused in integrate)
@@ -195,6 +195,7 @@ public:
     double compute();
     // pointers for the integrate kernel; marks the resulting version as tracked
     uint32_t* fuse_slot();
+    uint32_t* next_slot();
 };
 
 class hilbert;
