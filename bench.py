@@ -180,6 +180,131 @@ def batched(args):
     print(json.dumps(out), flush=True)
 
 
+def serving(args):
+    """Served job stream: Poisson arrivals of LJ / Kob-Andersen / WCA jobs (N
+    log-normal, median 2e4, clipped to [1e3, 2e5]; rho in [0.8, 1.2]; T in
+    [0.45, 1.5]; steps log-uniform in [1e3, 1e5] times --step-scale), each with
+    blocking-scheme MSD and ISF at 8 wavenumbers, through one server in real
+    time; against the plain per-job path replayed first-in first-out with
+    service times measured on a sample of the stream's jobs."""
+    from collections import deque
+    from halmd_amd import workloads
+    from halmd_amd.serve import server
+    stream = workloads.serving_stream(rate=args.rate, duration=args.duration, seed=args.seed,
+                                      step_scale=args.step_scale)
+    dt = 0.003
+    k = list(workloads.SERVING_WAVENUMBERS)
+    corr = dict(block_size=10, shift=2, separation=10, wavenumber=k, tolerance=0.01, max_count=7)
+    t0 = time.perf_counter()
+    inputs = [workloads.system_inputs(j["n"], j["density"], j["temperature"], j["seed"]) for j in stream]
+    log(f"{len(stream)} jobs over {args.duration:.0f} s, inputs ready in {time.perf_counter() - t0:.1f} s")
+    s = server(budget_bytes=args.budget_gib << 30, max_jobs=args.max_jobs, tick_steps=args.tick)
+    offered = sum(j["n"] * j["steps"] for j in stream) / args.duration
+    pending = deque(range(len(stream)))
+    ids = {}
+    start = s.now()
+    ticks = 0
+    while pending or s.stats()["active"] or s.stats()["queued"]:
+        now = s.now() - start
+        while pending and stream[pending[0]]["arrival"] <= now:
+            q = pending.popleft()
+            j = stream[q]
+            L, r0, v0, _ = inputs[q]
+            ids[q] = s.submit(r0, v0, L[0], j["steps"], potential=j["potential"], timestep=dt,
+                              species=workloads.spec_species(j), arrival=start + j["arrival"], **corr)
+            inputs[q] = None
+        st = s.stats()
+        if not st["active"] and not st["queued"]:
+            time.sleep(max(0.0, stream[pending[0]]["arrival"] - (s.now() - start)))
+            continue
+        s.tick()
+        ticks += 1
+        if ticks % 50 == 0:
+            log(f"t = {s.now() - start:6.1f} s: {st['active']} running, {st['queued']} queued, "
+                f"{len(ids)} submitted of {len(stream)}")
+    end = s.now()
+    lat = np.array([s.info(ids[q])["done"] - (start + stream[q]["arrival"]) for q in range(len(stream))])
+    wait = np.array([s.info(ids[q])["admitted"] - (start + stream[q]["arrival"]) for q in range(len(stream))])
+    psteps = s.stats()["particle_steps"]
+    value = psteps / (end - start)
+    log(f"served: {value:.4e} particle-steps/s over {end - start:.1f} s; latency p50 {np.percentile(lat, 50):.2f} s, "
+        f"p99 {np.percentile(lat, 99):.2f} s; offered {offered:.3e} particle-steps/s")
+
+    # plain per-job path: one job at a time through the per-system modules
+    # (with the same correlators); time per step fitted as a + b N on a sample
+    from halmd_amd import observables
+    from halmd_amd.observables.dynamics import blocking_scheme
+    from halmd_amd.observables.utility import wavevector
+    rng = np.random.default_rng(args.seed)
+    sample = sorted(rng.choice(len(stream), size=min(8, len(stream)), replace=False),
+                    key=lambda q: stream[q]["n"])
+    ns, ts, setups = [], [], []
+    for q in sample:
+        j = stream[q]
+        c0 = time.perf_counter()
+        w = workloads.single_from_spec(j, timestep=dt)
+        steps = -(-j["steps"] // args.tick) * args.tick   # whole ticks, as the server runs them
+        nsteps = min(steps, 300)
+        bs = blocking_scheme(w.box, w.particle, timestep=dt, max_lag=steps * dt, resolution=args.tick * dt,
+                             block_size=10, shift=2, separation=10)
+        bs.correlate_msd()
+        bs.correlate_isf(observables.density_mode(w.particle, wavevector(w.box, k, tolerance=0.01, max_count=7)))
+        w.sampler.on_sample_blocking_scheme(bs)
+        w.sampler.run(1)
+        setups.append(time.perf_counter() - c0)
+        ms = w.sampler.run_timed(nsteps)
+        ns.append(j["n"])
+        ts.append(ms / 1e3 / nsteps)
+    b, a = np.polyfit(ns, ts, 1)
+    setup = float(np.median(setups))
+    service = [setup + -(-j["steps"] // args.tick) * args.tick * max(a + b * j["n"], 0.0) for j in stream]
+    fin, plat = 0.0, []
+    for j, sv in zip(stream, service):
+        fin = max(fin, j["arrival"]) + sv
+        plat.append(fin - j["arrival"])
+    plat = np.array(plat)
+    plain = sum(j["n"] * j["steps"] for j in stream) / fin
+    log(f"plain per-job path (FIFO replay): {plain:.4e} particle-steps/s; latency p50 "
+        f"{np.percentile(plat, 50):.2f} s, p99 {np.percentile(plat, 99):.2f} s")
+    out = {
+        "metric": "served job stream: aggregate particle-steps/s and p50/p99 job latency vs plain per-job GPU path",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": ticks,
+        "warmup": 0,
+        "ms_per_step": (end - start) * 1e3 / max(ticks, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"Poisson {args.rate:g} jobs/s for {args.duration:g} s ({len(stream)} jobs): LJ / KA / WCA, "
+                        "N log-normal (median 2e4, log-sigma 0.75, clipped [1e3, 2e5]), rho U[0.8, 1.2], "
+                        f"T U[0.45, 1.5], steps log-uniform [1e3, 1e5] x {args.step_scale:g}, dt {dt}, "
+                        "MSD + ISF at 8 wavenumbers (blocking scheme, block 10)",
+            "tick_steps": args.tick,
+            "max_jobs": args.max_jobs,
+            "budget_gib": args.budget_gib,
+            "offered_particle_steps_per_s": offered,
+            "latency_p50_s": float(np.percentile(lat, 50)),
+            "latency_p99_s": float(np.percentile(lat, 99)),
+            "queue_wait_p50_s": float(np.percentile(wait, 50)),
+            "queue_wait_p99_s": float(np.percentile(wait, 99)),
+            "makespan_s": end - start,
+            "plain_per_job_particle_steps_per_s": plain,
+            "plain_latency_p50_s": float(np.percentile(plat, 50)),
+            "plain_latency_p99_s": float(np.percentile(plat, 99)),
+            "plain_method": f"FIFO replay of the same arrivals; time per step a + b N fitted on {len(sample)} "
+                            f"sampled jobs ({min(ns)}..{max(ns)} particles, up to 300 timed steps each, same "
+                            f"correlators), median setup {setup:.3f} s per job",
+            "speedup_vs_plain": value / plain,
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,8 +316,15 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default cell-segment shape")
     ap.add_argument("--segments", default=None,
                     help="cell-segment shape 'cells,rows,threads' instead of autotuning (profiling runs)")
-    ap.add_argument("--workload", default="lj", choices=["lj", "batched"],
-                    help="lj: LJ fluid N = 2^20 (default); batched: ensemble of independent systems")
+    ap.add_argument("--workload", default="lj", choices=["lj", "batched", "serving"],
+                    help="lj: LJ fluid N = 2^20 (default); batched: ensemble of independent systems; "
+                         "serving: Poisson job stream through the continuous-batching server")
+    ap.add_argument("--rate", type=float, default=50.0, help="serving: job arrivals per second")
+    ap.add_argument("--duration", type=float, default=10.0, help="serving: seconds of arrivals")
+    ap.add_argument("--step-scale", type=float, default=1.0 / 30, help="serving: factor on the job step counts")
+    ap.add_argument("--tick", type=int, default=50, help="serving: steps per tick")
+    ap.add_argument("--max-jobs", type=int, default=256, help="serving: running-job cap")
+    ap.add_argument("--budget-gib", type=int, default=128, help="serving: device memory budget for jobs")
     ap.add_argument("--systems", type=int, default=256, help="systems of the batched workload")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
@@ -200,12 +332,14 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs one GPU per process; --gpus {args.gpus} is not supported "
-              "(independent systems shard across GPUs without exchange; see DESIGN.md)", file=sys.stderr)
+              "(independent systems shard across GPUs without exchange)", file=sys.stderr)
         sys.exit(2)
 
     import halmd_amd
     if args.workload == "batched":
         return batched(args)
+    if args.workload == "serving":
+        return serving(args)
     arch, cus, mem = halmd_amd.device_info()
     if args.calibration_copy_mib:
         calibration_copy(args.calibration_copy_mib << 20)

@@ -463,6 +463,7 @@ __global__ __launch_bounds__(256) void ens_r0_kernel(float4 const* __restrict__ 
     if (li < d.n) r0[d.base + li] = pos[2 * size_t(d.base + li)];
 }
 
+template <bool MULTI>
 __global__ __launch_bounds__(SEG_THREADS_MAX) void ens_build_kernel(
     uint4 const* __restrict__ posq, uint32_t const* __restrict__ offsets, ens_desc const* __restrict__ descs,
     uint32_t const* __restrict__ rseg, uint16_t* __restrict__ list, uint32_t* __restrict__ count,
@@ -475,7 +476,7 @@ __global__ __launch_bounds__(SEG_THREADS_MAX) void ens_build_kernel(
     uint32_t const e = rseg[blockIdx.x];
     unsigned const sys = e >> 16, seg = e & 0xffffu;
     ens_desc const& d = descs[sys];
-    staged_build_segment<false, true>(posq + d.base, offsets + d.cell_base, d.sp, seg, d.np, d.cap,
+    staged_build_segment<MULTI, true>(posq + d.base, offsets + d.cell_base, d.sp, seg, d.np, d.cap,
                                       list + d.list_off, count + d.base, stats + 2 * sys, s_rec, st, s_bound, s_rr);
 }
 
@@ -624,11 +625,14 @@ void launch_ens_r0(float4 const* pos, float4* r0, ens_desc const* descs, uint32_
 }
 
 void launch_ens_build(uint4 const* posq, uint32_t const* offsets, ens_desc const* descs, uint32_t const* rseg,
-                      unsigned nrseg, unsigned threads, unsigned lds_records, uint16_t* list, uint32_t* count,
-                      uint32_t* stats, hipStream_t s) {
+                      unsigned nrseg, unsigned threads, unsigned lds_records, bool multi, uint16_t* list,
+                      uint32_t* count, uint32_t* stats, hipStream_t s) {
     if (!nrseg) return;
-    ens_build_kernel<<<nrseg, threads, size_t(lds_records) * sizeof(uint4), s>>>(posq, offsets, descs, rseg, list,
-                                                                                  count, stats);
+    size_t const lds = size_t(lds_records) * sizeof(uint4);
+    if (multi)
+        ens_build_kernel<true><<<nrseg, threads, lds, s>>>(posq, offsets, descs, rseg, list, count, stats);
+    else
+        ens_build_kernel<false><<<nrseg, threads, lds, s>>>(posq, offsets, descs, rseg, list, count, stats);
     HIP_CHECK(hipGetLastError());
 }
 

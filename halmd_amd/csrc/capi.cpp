@@ -6,6 +6,7 @@
 
 #include "modules.hpp"
 #include "ensemble.hpp"
+#include "serve.hpp"
 #include <typeinfo>
 
 using namespace halmd;
@@ -753,12 +754,35 @@ HALMD_API int halmd_ensemble_add(halmd_ensemble* e, unsigned n, double length, i
     return guard([&] {
         nonnull(position, "position"); nonnull(velocity, "velocity"); nonnull(index, "index");
         require(kind == POTENTIAL_LENNARD_JONES || kind == POTENTIAL_MORSE, "ensemble: unknown potential kind");
-        *index = as<ensemble>(e)->add(n, length, kind, truncation_code(truncation), epsilon, sigma, r_cut_sigma, r_min,
-                                     distortion, smoothing, timestep, skin, position, velocity);
+        *index = as<ensemble>(e)->add(n, length, kind, truncation_code(truncation), 1, &epsilon, &sigma, &r_cut_sigma,
+                                     &r_min, &distortion, smoothing, timestep, skin, position, velocity, nullptr);
+    });
+}
+HALMD_API int halmd_ensemble_add_mixture(halmd_ensemble* e, unsigned n, double length, int kind,
+                                         const char* truncation, unsigned ntype, const double* epsilon,
+                                         const double* sigma, const double* r_cut_sigma, const double* r_min,
+                                         const double* distortion, double smoothing, double timestep, double skin,
+                                         const double* position, const double* velocity, const unsigned* species,
+                                         unsigned* index) {
+    return guard([&] {
+        nonnull(position, "position"); nonnull(velocity, "velocity"); nonnull(index, "index");
+        nonnull(epsilon, "epsilon"); nonnull(sigma, "sigma"); nonnull(r_cut_sigma, "r_cut_sigma");
+        nonnull(species, "species");
+        require(kind == POTENTIAL_LENNARD_JONES || kind == POTENTIAL_MORSE, "ensemble: unknown potential kind");
+        std::vector<double> ones(size_t(ntype) * ntype, 1.0);
+        *index = as<ensemble>(e)->add(n, length, kind, truncation_code(truncation), ntype, epsilon, sigma,
+                                     r_cut_sigma, r_min ? r_min : ones.data(), distortion ? distortion : ones.data(),
+                                     smoothing, timestep, skin, position, velocity, species);
     });
 }
 HALMD_API int halmd_ensemble_commit(halmd_ensemble* e) {
     return guard([&] { as<ensemble>(e)->commit(); });
+}
+HALMD_API int halmd_ensemble_retire(halmd_ensemble* e, unsigned system) {
+    return guard([&] { as<ensemble>(e)->retire(system); });
+}
+HALMD_API int halmd_ensemble_apply(halmd_ensemble* e) {
+    return guard([&] { as<ensemble>(e)->apply(); });
 }
 HALMD_API int halmd_ensemble_step(halmd_ensemble* e, uint64_t steps) {
     return guard([&] {
@@ -801,6 +825,135 @@ HALMD_API int halmd_ensemble_rebuilds(const halmd_ensemble* e, unsigned system, 
 }
 
 // ---- profiling ----------------------------------------------------------------------------------
+// ---- serving -------------------------------------------------------------------
+HALMD_API int halmd_server_create(size_t budget_bytes, unsigned max_jobs, unsigned tick_steps, halmd_server** out) {
+    return guard([&] {
+        nonnull(out, "out");
+        *out = handle<halmd_server>(new server(budget_bytes, max_jobs, tick_steps));
+    });
+}
+HALMD_API int halmd_server_destroy(halmd_server* s) {
+    return guard([&] { delete reinterpret_cast<server*>(s); });
+}
+HALMD_API int halmd_server_now(const halmd_server* s, double* seconds) {
+    return guard([&] { nonnull(seconds, "seconds"); *seconds = as<server>(s)->now(); });
+}
+HALMD_API int halmd_server_submit(halmd_server* s, unsigned n, double length, int kind, const char* truncation,
+                                  unsigned ntype, const double* epsilon, const double* sigma,
+                                  const double* r_cut_sigma, double timestep, double skin, const double* position,
+                                  const double* velocity, const unsigned* species, uint64_t steps, double arrival,
+                                  unsigned block_size, unsigned shift, unsigned separation, const double* wavenumber,
+                                  unsigned nwavenumber, double tolerance, unsigned max_count, int keep_state,
+                                  uint64_t* id) {
+    return guard([&] {
+        nonnull(position, "position"); nonnull(velocity, "velocity"); nonnull(id, "id");
+        nonnull(epsilon, "epsilon"); nonnull(sigma, "sigma"); nonnull(r_cut_sigma, "r_cut_sigma");
+        require(kind == POTENTIAL_LENNARD_JONES || kind == POTENTIAL_MORSE, "server: unknown potential kind");
+        require(ntype >= 1 && ntype <= MAX_TYPES, "server: invalid number of species");
+        require(nwavenumber == 0 || wavenumber, "server: wavenumbers missing");
+        job_request r;
+        size_t const nn = size_t(ntype) * ntype;
+        r.n = n;
+        r.length = length;
+        r.kind = kind;
+        r.trunc = truncation_code(truncation);
+        r.ntype = ntype;
+        r.epsilon.assign(epsilon, epsilon + nn);
+        r.sigma.assign(sigma, sigma + nn);
+        r.r_cut_sigma.assign(r_cut_sigma, r_cut_sigma + nn);
+        r.timestep = timestep;
+        r.skin = skin;
+        r.position.assign(position, position + 3 * size_t(n));
+        r.velocity.assign(velocity, velocity + 3 * size_t(n));
+        if (species) r.species.assign(species, species + n);
+        r.steps = steps;
+        r.arrival = arrival;
+        r.block_size = block_size;
+        r.shift = shift;
+        r.separation = separation;
+        if (nwavenumber) r.wavenumber.assign(wavenumber, wavenumber + nwavenumber);
+        r.tolerance = tolerance;
+        r.max_count = max_count;
+        r.keep_state = keep_state != 0;
+        *id = as<server>(s)->submit(std::move(r));
+    });
+}
+HALMD_API int halmd_server_tick(halmd_server* s, unsigned* active) {
+    return guard([&] {
+        unsigned const a = as<server>(s)->tick();
+        if (active) *active = a;
+    });
+}
+HALMD_API int halmd_server_stats(const halmd_server* s, unsigned* active, unsigned* queued, size_t* used_bytes,
+                                 uint64_t* particle_steps) {
+    return guard([&] {
+        auto const* x = as<server>(s);
+        if (active) *active = x->active();
+        if (queued) *queued = x->queued();
+        if (used_bytes) *used_bytes = x->used();
+        if (particle_steps) *particle_steps = x->particle_steps();
+    });
+}
+HALMD_API int halmd_server_job_info(const halmd_server* s, uint64_t id, int* status, double* times,
+                                    double* energies) {
+    return guard([&] {
+        auto const& j = as<server>(s)->get(id);
+        if (status) *status = j.status;
+        if (times) { times[0] = j.req.arrival; times[1] = j.t_admit; times[2] = j.t_done; }
+        if (energies) { energies[0] = j.en_pot; energies[1] = j.en_kin; }
+    });
+}
+HALMD_API int halmd_server_job_levels(const halmd_server* s, uint64_t id, unsigned* levels, unsigned* block_size,
+                                      unsigned* nshell) {
+    return guard([&] {
+        auto const& j = as<server>(s)->get(id);
+        require(j.corr != nullptr, "server: job not admitted yet");
+        if (levels) *levels = j.corr->levels();
+        if (block_size) *block_size = j.corr->block_size();
+        if (nshell) *nshell = j.corr->nshell();
+    });
+}
+HALMD_API int halmd_server_job_msd(const halmd_server* s, uint64_t id, double* time, double* mean, uint64_t* count) {
+    return guard([&] {
+        auto const& j = as<server>(s)->get(id);
+        require(j.corr != nullptr, "server: job not admitted yet");
+        auto const& c = *j.corr;
+        for (unsigned lv = 0; lv < c.levels(); ++lv) {
+            for (unsigned l = 0; l < c.block_size(); ++l) {
+                size_t const k = size_t(lv) * c.block_size() + l;
+                if (time) time[k] = double(c.interval[lv]) * l * c.timestep;
+                if (mean) mean[k] = c.msd[k].m;
+                if (count) count[k] = c.msd[k].n;
+            }
+        }
+    });
+}
+HALMD_API int halmd_server_job_isf(const halmd_server* s, uint64_t id, double* mean, uint64_t* count) {
+    return guard([&] {
+        auto const& j = as<server>(s)->get(id);
+        require(j.corr != nullptr, "server: job not admitted yet");
+        auto const& c = *j.corr;
+        for (size_t k = 0; k < c.isf.size(); ++k) {
+            if (mean) mean[k] = c.isf[k].m;
+            if (count) count[k] = c.isf[k].n;
+        }
+    });
+}
+HALMD_API int halmd_server_job_state(const halmd_server* s, uint64_t id, double* position, double* velocity,
+                                     int* image) {
+    return guard([&] {
+        auto const& j = as<server>(s)->get(id);
+        require(j.status == 2 && !j.position.empty(), "server: no final state (finished job with keep_state)");
+        size_t const m = 3 * size_t(j.req.n);
+        if (position) std::copy(j.position.begin(), j.position.begin() + m, position);
+        if (velocity) std::copy(j.velocity.begin(), j.velocity.begin() + m, velocity);
+        if (image) std::copy(j.image.begin(), j.image.begin() + m, image);
+    });
+}
+HALMD_API int halmd_server_job_release(halmd_server* s, uint64_t id) {
+    return guard([&] { as<server>(s)->release(id); });
+}
+
 HALMD_API int halmd_profile_enable(void* h, const char* module, size_t max_launches) {
     return guard([&] {
         auto& t = timer_of(h, module);

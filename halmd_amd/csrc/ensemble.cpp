@@ -8,12 +8,15 @@
 namespace halmd {
 
 struct ensemble::sys_host {
-    unsigned n = 0;
-    double L = 0, dt = 0, skin = 0, r_cut = 0;
+    uint64_t uid = 0;
+    bool on_device = false, retired = false;
+    unsigned n = 0, ntype = 1;
+    double L = 0, dt = 0, skin = 0, r_cut_max = 0;
     int kind = 0, trunc = 0;
     std::unique_ptr<box> bx;
     std::unique_ptr<potential> pot;
-    std::vector<double> r, v;            // initial state until commit
+    std::vector<double> r, v;            // initial state until placed
+    std::vector<unsigned> species;
     // layout
     unsigned base = 0, blk_base = 0, nblk = 0, cell_base = 0, ncells = 0, ncell = 0;
     bool small = false;
@@ -29,6 +32,11 @@ unsigned bits_for(size_t v) {   // smallest b with v < 2^b
     while (b < 32 && (size_t(1) << b) <= v) ++b;
     return b;
 }
+
+template <typename T>
+void d2d(T* dst, T const* src, size_t count, hipStream_t s) {
+    if (count) HIP_CHECK(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToDevice, s));
+}
 } // namespace
 
 ensemble::ensemble() { HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)); }
@@ -40,14 +48,47 @@ uint64_t ensemble::rebuilds(unsigned s) const {
     return sys_[s]->rebuilds;
 }
 
-unsigned ensemble::add(unsigned n, double length, int kind, int trunc, double epsilon, double sigma,
-                       double r_cut_sigma, double r_min, double distortion, double smoothing, double timestep,
-                       double skin, double const* position, double const* velocity) {
-    require(!committed_, "ensemble: systems are added before commit");
+uint64_t ensemble::uid(unsigned s) const {
+    require(s < sys_.size(), "ensemble: system index out of range");
+    return sys_[s]->uid;
+}
+
+int ensemble::index_of(uint64_t u) const {
+    for (size_t s = 0; s < sys_.size(); ++s)
+        if (sys_[s]->uid == u) return int(s);
+    return -1;
+}
+
+ensemble::view ensemble::system_view(unsigned s) const {
+    require(s < sys_.size() && sys_[s]->on_device, "ensemble: system not placed");
+    auto const& h = *sys_[s];
+    return view{pos_.get() + 2 * size_t(h.base), image_.get() + h.base, id_.get() + h.base, h.n, h.L};
+}
+
+size_t ensemble::footprint(unsigned n, double length, double r_cut_max, double skin) {
+    // state and its reorder copies, forces, snapshots, sort keys (~330 B),
+    // the 16-bit lists at the capacity prepare() chooses, and the cell offsets
+    double const rcs = r_cut_max + skin;
+    double const rho = n / (length * length * length);
+    double const nmean = 4.0 / 3.0 * M_PI * rcs * rcs * rcs * rho;
+    size_t const cap = round_up(size_t(std::ceil(nmean / 0.5)) + NB16_BLOCK, NB16_BLOCK);
+    size_t const ncell = std::max<size_t>(1, size_t(length / rcs));
+    return size_t(round_up(n, ENS_BLOCK)) * (330 + 2 * cap) + 4 * (ncell * ncell * ncell + 1);
+}
+
+unsigned ensemble::add(unsigned n, double length, int kind, int trunc, unsigned ntype, double const* epsilon,
+                       double const* sigma, double const* r_cut_sigma, double const* r_min, double const* distortion,
+                       double smoothing, double timestep, double skin, double const* position, double const* velocity,
+                       unsigned const* species) {
     require(n >= 2 && n < 65536 * 256, "ensemble: particle count out of range");
     require(length > 0 && timestep > 0 && skin >= 0, "ensemble: invalid box, timestep or skin");
+    require(ntype >= 1 && ntype <= MAX_TYPES, "ensemble: invalid number of species");
+    require(ntype == 1 || kind == POTENTIAL_LENNARD_JONES, "ensemble: several species need Lennard-Jones");
+    require(sys_.size() < 65535, "ensemble: at most 65535 systems");
     auto h = std::make_unique<sys_host>();
+    h->uid = next_uid_++;
     h->n = n;
+    h->ntype = ntype;
     h->L = length;
     h->dt = timestep;
     h->skin = skin;
@@ -55,25 +96,123 @@ unsigned ensemble::add(unsigned n, double length, int kind, int trunc, double ep
     h->trunc = trunc;
     double const Lv[3] = {length, length, length};
     h->bx = std::make_unique<box>(Lv);
-    h->pot = std::make_unique<potential>(kind, 1, 1, &epsilon, &sigma, &r_min, &distortion);
-    h->pot->truncate(trunc, &r_cut_sigma, smoothing);
-    h->r_cut = h->pot->r_cut[0];
-    require(h->r_cut + skin < length / 2, "ensemble: the box must exceed twice rc + skin (minimum image)");
+    h->pot = std::make_unique<potential>(kind, ntype, ntype, epsilon, sigma, r_min, distortion);
+    h->pot->truncate(trunc, r_cut_sigma, smoothing);
+    h->r_cut_max = h->pot->r_cut_max();
+    require(h->r_cut_max + skin < length / 2, "ensemble: the box must exceed twice rc + skin (minimum image)");
     h->r.assign(position, position + 3 * size_t(n));
     h->v.assign(velocity, velocity + 3 * size_t(n));
+    h->species.assign(n, 0u);
+    if (species) {
+        for (unsigned i = 0; i < n; ++i) {
+            require(species[i] < ntype, "ensemble: species out of range");
+            h->species[i] = species[i];
+        }
+    }
     sys_.push_back(std::move(h));
     return unsigned(sys_.size() - 1);
 }
 
-// list capacity and offsets of every system (on commit and after an overflow)
-void ensemble::layout_lists() {
+void ensemble::retire(unsigned s) {
+    require(s < sys_.size(), "ensemble: system index out of range");
+    sys_[s]->retired = true;
+}
+
+// everything of a system but its place in the shared arrays
+void ensemble::prepare(sys_host& h) {
+    double const rcs = h.r_cut_max + h.skin;
+    h.nblk = unsigned(div_up(h.n, ENS_BLOCK));
+    h.ncell = std::max(1u, unsigned(h.L / rcs));
+    h.small = h.ncell < 3;
+    double const rho = h.n / (h.L * h.L * h.L);
+    double const nmean = 4.0 / 3.0 * M_PI * rcs * rcs * rcs * rho;
+    h.cap = unsigned(round_up(size_t(std::ceil(nmean / 0.5)) + NB16_BLOCK, NB16_BLOCK));
+    h.ncells = 0;
+    if (h.small) {
+        require(h.n <= 4096, "ensemble: a box under 3 cells of rc + skin holds at most 4096 particles");
+        require(h.ntype == 1, "ensemble: a box under 3 cells of rc + skin holds one species");
+        h.cap = std::min(h.cap, unsigned(round_up(h.n, NB16_BLOCK)));
+    } else {
+        h.ncells = h.ncell * h.ncell * h.ncell;
+        double const occ = double(h.n) / h.ncells;
+        cell_grid g;
+        g.ncell[0] = g.ncell[1] = g.ncell[2] = h.ncell;
+        segment_shape(occ, g, 200.0, h.S, h.R);
+        h.nsx = unsigned(div_up(h.ncell, h.S));
+        h.nsy = unsigned(div_up(h.ncell, h.R));
+        h.nseg = h.nsx * h.nsy * h.ncell;
+        require(h.nseg < 65536, "ensemble: too many cell segments in one system");
+        double const expect = 3.0 * (h.R + 2) * (h.S + 2) * occ;
+        h.lds_cap = unsigned(std::min<double>(SEG_MAX_RECORDS, round_up(size_t(1.3 * expect) + 64, 64)));
+    }
+    // descriptor (the layout fields are filled by place_all, the list fields
+    // by upload_descs)
+    ens_desc& d = h.d;
+    std::memset(&d, 0, sizeof(d));
+    d.n = h.n;
+    d.ncells = h.ncells;
+    d.small = h.small;
+    d.skin_half_sq = float(h.skin * h.skin / 4);
+    d.sp.grid.ncell[0] = d.sp.grid.ncell[1] = d.sp.grid.ncell[2] = h.ncell;
+    d.sp.S = h.S;
+    d.sp.R = h.R;
+    d.sp.nsx = h.nsx;
+    d.sp.nsy = h.nsy;
+    d.sp.nseg = h.nseg;
+    d.sp.chunk = 0;
+    d.sp.threads = threads_;
+    box const& b = *h.bx;
+    d.np.grid = d.sp.grid;
+    for (int k = 0; k < 3; ++k) d.np.q[k] = b.q_hi[k];
+    d.np.ntype = h.ntype;
+    d.np.iso = 1;
+    for (unsigned ab = 0; ab < h.ntype * h.ntype; ++ab) {
+        // the single-system rule: (rc_ab + skin)^2, and in fixed-point units
+        d.np.rr_cut_skin[ab] = float(std::pow(h.pot->r_cut[ab] + h.skin, 2));
+        d.np.rr_cut_skin_q[ab] = float(double(d.np.rr_cut_skin[ab]) / (double(d.np.q[0]) * double(d.np.q[0])));
+    }
+    split_ff(h.dt, d.ip.dt_hi, d.ip.dt_lo);
+    d.ip.half_dt = float(h.dt / 2);
+    for (int k = 0; k < 3; ++k) {
+        d.ip.L_hi[k] = b.L_hi[k];
+        d.ip.L_lo[k] = b.L_lo[k];
+        d.ip.invL[k] = b.invL[k];
+    }
+    d.ip.qp = b.qp;
+    d.P = h.pot->device_params(b);
+}
+
+// particles on 256-particle blocks, then cells, then lists, in system order
+void ensemble::place_all() {
+    unsigned base = 0, cells = 0;
+    for (auto& hp : sys_) {
+        auto& h = *hp;
+        h.base = base;
+        h.blk_base = base / ENS_BLOCK;
+        base += h.nblk * ENS_BLOCK;
+        h.cell_base = 0;
+        if (!h.small) {
+            h.cell_base = cells;
+            cells += h.ncells + 1;
+        }
+        h.d.base = h.base;
+        h.d.blk_base = h.blk_base;
+        h.d.cell_base = h.cell_base;
+    }
+    total_ = base;
+    nblocks_ = base / ENS_BLOCK;
+    cells_total_ = cells;
+    list_words_ = layout_lists();
+}
+
+// list offsets of every system at their present capacities; returns the total
+size_t ensemble::layout_lists() {
     size_t off = 0;
     for (auto& h : sys_) {
         h->list_off = off;
         off += size_t(round_up(h->n, WAVE)) * h->cap;
     }
-    list_words_ = off;
-    list_.resize(list_words_);
+    return off;
 }
 
 void ensemble::upload_descs() {
@@ -85,145 +224,22 @@ void ensemble::upload_descs() {
         h.d.sp.lds_cap = h.lds_cap;
         d[s] = h.d;
     }
-    descs_.resize(d.size());
-    descs_.upload(d.data(), d.size(), stream);
+    descs_.resize(std::max<size_t>(d.size(), 1));
+    if (!d.empty()) descs_.upload(d.data(), d.size(), stream);
     HIP_CHECK(hipStreamSynchronize(stream));
 }
 
-void ensemble::commit() {
-    require(!committed_, "ensemble: already committed");
-    require(!sys_.empty(), "ensemble: no systems");
-    require(sys_.size() < 65536, "ensemble: at most 65535 systems");
-    // layout: particles on 256-particle blocks, cells, segments, lists
-    unsigned base = 0, cells = 0;
-    for (auto& hp : sys_) {
-        auto& h = *hp;
-        h.base = base;
-        h.blk_base = base / ENS_BLOCK;
-        h.nblk = unsigned(div_up(h.n, ENS_BLOCK));
-        base += h.nblk * ENS_BLOCK;
-        double const rcs = h.r_cut + h.skin;
-        h.ncell = std::max(1u, unsigned(h.L / rcs));
-        h.small = h.ncell < 3;
-        double const rho = h.n / (h.L * h.L * h.L);
-        double const nmean = 4.0 / 3.0 * M_PI * rcs * rcs * rcs * rho;
-        h.cap = unsigned(round_up(size_t(std::ceil(nmean / 0.5)) + NB16_BLOCK, NB16_BLOCK));
-        if (h.small) {
-            require(h.n <= 4096, "ensemble: a box under 3 cells of rc + skin holds at most 4096 particles");
-            h.cap = std::min(h.cap, unsigned(round_up(h.n, NB16_BLOCK)));
-        } else {
-            h.cell_base = cells;
-            h.ncells = h.ncell * h.ncell * h.ncell;
-            cells += h.ncells + 1;
-            double const occ = double(h.n) / h.ncells;
-            cell_grid g;
-            g.ncell[0] = g.ncell[1] = g.ncell[2] = h.ncell;
-            segment_shape(occ, g, 200.0, h.S, h.R);
-            h.nsx = unsigned(div_up(h.ncell, h.S));
-            h.nsy = unsigned(div_up(h.ncell, h.R));
-            h.nseg = h.nsx * h.nsy * h.ncell;
-            require(h.nseg < 65536, "ensemble: too many cell segments in one system");
-            double const expect = 3.0 * (h.R + 2) * (h.S + 2) * occ;
-            h.lds_cap = unsigned(std::min<double>(SEG_MAX_RECORDS, round_up(size_t(1.3 * expect) + 64, 64)));
-        }
-        // descriptor (the list fields are filled by upload_descs)
-        ens_desc& d = h.d;
-        std::memset(&d, 0, sizeof(d));
-        d.base = h.base;
-        d.n = h.n;
-        d.blk_base = h.blk_base;
-        d.cell_base = h.cell_base;
-        d.ncells = h.ncells;
-        d.small = h.small;
-        d.skin_half_sq = float(h.skin * h.skin / 4);
-        d.sp.grid.ncell[0] = d.sp.grid.ncell[1] = d.sp.grid.ncell[2] = h.ncell;
-        d.sp.S = h.S;
-        d.sp.R = h.R;
-        d.sp.nsx = h.nsx;
-        d.sp.nsy = h.nsy;
-        d.sp.nseg = h.nseg;
-        d.sp.chunk = 0;
-        d.sp.threads = threads_;
-        box const& b = *h.bx;
-        d.np.grid = d.sp.grid;
-        for (int k = 0; k < 3; ++k) d.np.q[k] = b.q_hi[k];
-        d.np.ntype = 1;
-        d.np.rr_cut_skin[0] = float(rcs * rcs);
-        d.np.iso = 1;
-        d.np.rr_cut_skin_q[0] = float(double(d.np.rr_cut_skin[0]) / (double(d.np.q[0]) * double(d.np.q[0])));
-        split_ff(h.dt, d.ip.dt_hi, d.ip.dt_lo);
-        d.ip.half_dt = float(h.dt / 2);
-        for (int k = 0; k < 3; ++k) {
-            d.ip.L_hi[k] = b.L_hi[k];
-            d.ip.L_lo[k] = b.L_lo[k];
-            d.ip.invL[k] = b.invL[k];
-        }
-        d.ip.qp = b.qp;
-        d.P = h.pot->device_params(b);
-    }
-    total_ = base;
-    nblocks_ = base / ENS_BLOCK;
-    cells_total_ = cells;
-    layout_lists();
-
-    // device state
-    size_t const N = total_;
-    pos_.resize(2 * N); vel_.resize(2 * N); pos_t_.resize(2 * N); vel_t_.resize(2 * N);
-    force_.resize(N); aux_force_.resize(N); r0_.resize(N);
-    posq_.resize(N); posq_t_.resize(N); image_.resize(N); image_t_.resize(N);
-    id_.resize(N); id_t_.resize(N); count_.resize(N);
-    stress_.resize(6 * N);
-    offsets_.resize(std::max<size_t>(cells_total_, 1));
-    disp2_.resize(sys_.size()); stats_.resize(2 * sys_.size());
-    thermo_.resize(4 * sys_.size());
-    h_disp2_.resize(sys_.size()); h_stats_.resize(2 * sys_.size()); h_thermo_.resize(4 * sys_.size());
-    keys_.resize(N); vals_.resize(N);
-    ws_.reserve(N);
-    {
-        std::vector<float4> hp(2 * N, make_float4(0, 0, 0, 0)), hv(2 * N, make_float4(0, 0, 0, 0));
-        std::vector<int4> him(N, make_int4(0, 0, 0, 0));
-        std::vector<uint32_t> hid(N, 0);
-        std::vector<uint16_t> bs(nblocks_);
-        for (size_t s = 0; s < sys_.size(); ++s) {
-            auto& h = *sys_[s];
-            for (unsigned b = 0; b < h.nblk; ++b) bs[h.blk_base + b] = uint16_t(s);
-            for (unsigned i = 0; i < h.n; ++i) {
-                size_t const g = h.base + i;
-                float rh[3], rl[3], vh[3], vl[3];
-                for (int k = 0; k < 3; ++k) {
-                    // stored as given, as particle::set_position does (the
-                    // first integrate pass wraps and counts images)
-                    split_ff(h.r[3 * size_t(i) + k], rh[k], rl[k]);
-                    split_ff(h.v[3 * size_t(i) + k], vh[k], vl[k]);
-                }
-                hp[2 * g] = make_float4(rh[0], rh[1], rh[2], 0.0f);   // species 0
-                hp[2 * g + 1] = make_float4(rl[0], rl[1], rl[2], 0.0f);
-                hv[2 * g] = make_float4(vh[0], vh[1], vh[2], 1.0f);   // unit mass
-                hv[2 * g + 1] = make_float4(vl[0], vl[1], vl[2], 0.0f);
-                hid[g] = i;
-            }
-            h.r.clear(); h.r.shrink_to_fit();
-            h.v.clear(); h.v.shrink_to_fit();
-        }
-        pos_.upload(hp.data(), hp.size(), stream);
-        vel_.upload(hv.data(), hv.size(), stream);
-        image_.upload(him.data(), him.size(), stream);
-        id_.upload(hid.data(), hid.size(), stream);
-        block_sys_.resize(nblocks_);
-        block_sys_.upload(bs.data(), bs.size(), stream);
-        HIP_CHECK(hipStreamSynchronize(stream));
-    }
-    for (auto& h : sys_) launch_quantize(pos_.get() + 2 * size_t(h->base), h->n, h->d.ip.qp, posq_.get() + h->base, stream);
-    upload_descs();
-
-    // force classes: (kind, truncation) with their workgroup tables
+// force classes: (kind, truncation, several species) with their workgroup tables
+void ensemble::build_classes() {
     classes_.clear();
     for (size_t s = 0; s < sys_.size(); ++s) {
         auto& h = *sys_[s];
-        auto it = std::find_if(classes_.begin(), classes_.end(),
-                               [&](force_class const& c) { return c.kind == h.kind && c.trunc == h.trunc; });
+        bool const multi = h.ntype > 1;
+        auto it = std::find_if(classes_.begin(), classes_.end(), [&](force_class const& c) {
+            return c.kind == h.kind && c.trunc == h.trunc && c.multi == multi;
+        });
         if (it == classes_.end()) {
-            classes_.push_back(force_class{h.kind, h.trunc, {}, {}, 0, 0, 0, 0});
+            classes_.push_back(force_class{h.kind, h.trunc, multi, {}, {}, 0, 0, 0, 0});
             it = classes_.end() - 1;
         }
         if (h.small) {
@@ -239,17 +255,125 @@ void ensemble::commit() {
         ft.insert(ft.end(), c.seg.begin(), c.seg.end());
         c.small_off = ft.size();
         ft.insert(ft.end(), c.small.begin(), c.small.end());
+        c.lds_records = 64;
+        for (uint32_t e : c.seg) c.lds_records = std::max(c.lds_records, sys_[e >> 16]->lds_cap);
     }
     fclass_tables_.resize(std::max<size_t>(ft.size(), 1));
-    fclass_tables_.upload(ft.data(), ft.size(), stream);
+    if (!ft.empty()) fclass_tables_.upload(ft.data(), ft.size(), stream);
     HIP_CHECK(hipStreamSynchronize(stream));
+}
 
+void ensemble::apply() {
+    bool changed = !committed_;
+    for (auto& h : sys_) changed = changed || !h->on_device || h->retired;
+    if (!changed) return;
+    if (committed_) settle();
+    // where the surviving systems are now
+    struct old_place {
+        sys_host* h;
+        unsigned base, cell_base;
+        size_t list_off;
+    };
+    std::vector<old_place> old;
+    for (auto& h : sys_)
+        if (h->on_device && !h->retired) old.push_back({h.get(), h->base, h->cell_base, h->list_off});
+    sys_.erase(std::remove_if(sys_.begin(), sys_.end(), [](auto const& h) { return h->retired; }), sys_.end());
+    std::vector<unsigned> fresh;
+    for (size_t s = 0; s < sys_.size(); ++s) {
+        if (!sys_[s]->on_device) {
+            prepare(*sys_[s]);
+            fresh.push_back(unsigned(s));
+        }
+    }
+    place_all();
+
+    // new state arrays: survivors moved unchanged, new systems uploaded
+    size_t const N = total_;
+    dbuf<float4> pos(2 * N), vel(2 * N), force(N), r0(N);
+    dbuf<uint4> posq(N);
+    dbuf<int4> image(N);
+    dbuf<uint32_t> id(N), count(N), offsets(std::max<size_t>(cells_total_, 1));
+    dbuf<uint16_t> list(std::max<size_t>(list_words_, 1));
+    for (auto const& o : old) {
+        auto const& h = *o.h;
+        size_t const n = h.n;
+        d2d(pos.get() + 2 * size_t(h.base), pos_.get() + 2 * size_t(o.base), 2 * n, stream);
+        d2d(vel.get() + 2 * size_t(h.base), vel_.get() + 2 * size_t(o.base), 2 * n, stream);
+        d2d(force.get() + h.base, force_.get() + o.base, n, stream);
+        d2d(r0.get() + h.base, r0_.get() + o.base, n, stream);
+        d2d(posq.get() + h.base, posq_.get() + o.base, n, stream);
+        d2d(image.get() + h.base, image_.get() + o.base, n, stream);
+        d2d(id.get() + h.base, id_.get() + o.base, n, stream);
+        d2d(count.get() + h.base, count_.get() + o.base, n, stream);
+        d2d(list.get() + h.list_off, list_.get() + o.list_off, size_t(round_up(n, WAVE)) * h.cap, stream);
+        if (!h.small) d2d(offsets.get() + h.cell_base, offsets_.get() + o.cell_base, size_t(h.ncells) + 1, stream);
+    }
+    for (unsigned s : fresh) {
+        auto& h = *sys_[s];
+        size_t const n = h.n;
+        std::vector<float4> hp(2 * n), hv(2 * n);
+        std::vector<int4> him(n, make_int4(0, 0, 0, 0));
+        std::vector<uint32_t> hid(n);
+        for (size_t i = 0; i < n; ++i) {
+            float rh[3], rl[3], vh[3], vl[3];
+            for (int k = 0; k < 3; ++k) {
+                // stored as given, as particle::set_position does (the first
+                // integrate pass wraps and counts images)
+                split_ff(h.r[3 * i + k], rh[k], rl[k]);
+                split_ff(h.v[3 * i + k], vh[k], vl[k]);
+            }
+            float w;   // species as integer bits, as particle::set_species stores it
+            std::memcpy(&w, &h.species[i], 4);
+            hp[2 * i] = make_float4(rh[0], rh[1], rh[2], w);
+            hp[2 * i + 1] = make_float4(rl[0], rl[1], rl[2], 0.0f);
+            hv[2 * i] = make_float4(vh[0], vh[1], vh[2], 1.0f);   // unit mass
+            hv[2 * i + 1] = make_float4(vl[0], vl[1], vl[2], 0.0f);
+            hid[i] = unsigned(i);
+        }
+        HIP_CHECK(hipMemcpyAsync(pos.get() + 2 * size_t(h.base), hp.data(), hp.size() * sizeof(float4),
+                                 hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(vel.get() + 2 * size_t(h.base), hv.data(), hv.size() * sizeof(float4),
+                                 hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(image.get() + h.base, him.data(), n * sizeof(int4), hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(id.get() + h.base, hid.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+        launch_quantize(pos.get() + 2 * size_t(h.base), h.n, h.d.ip.qp, posq.get() + h.base, stream);
+        HIP_CHECK(hipStreamSynchronize(stream));   // the host staging goes out of scope
+        h.r.clear(); h.r.shrink_to_fit();
+        h.v.clear(); h.v.shrink_to_fit();
+        h.species.clear(); h.species.shrink_to_fit();
+        h.on_device = true;
+    }
+    HIP_CHECK(hipStreamSynchronize(stream));   // the old arrays are released below
+    pos_ = std::move(pos); vel_ = std::move(vel); force_ = std::move(force); r0_ = std::move(r0);
+    posq_ = std::move(posq); image_ = std::move(image); id_ = std::move(id); count_ = std::move(count);
+    offsets_ = std::move(offsets); list_ = std::move(list);
+
+    // scratch sized for the new layout
+    pos_t_.resize(2 * N); vel_t_.resize(2 * N); aux_force_.resize(N);
+    posq_t_.resize(N); image_t_.resize(N); id_t_.resize(N);
+    stress_.resize(6 * N);
+    keys_.resize(N); vals_.resize(N);
+    ws_.reserve(std::max<size_t>(N, 1));
+    size_t const ns = std::max<size_t>(sys_.size(), 1);
+    disp2_.resize(ns); stats_.resize(2 * ns); thermo_.resize(4 * ns);
+    h_disp2_.resize(ns); h_stats_.resize(2 * ns); h_thermo_.resize(4 * ns);
+    {
+        std::vector<uint16_t> bs(std::max<unsigned>(nblocks_, 1), 0);
+        for (size_t s = 0; s < sys_.size(); ++s)
+            for (unsigned b = 0; b < sys_[s]->nblk; ++b) bs[sys_[s]->blk_base + b] = uint16_t(s);
+        block_sys_.resize(bs.size());
+        block_sys_.upload(bs.data(), bs.size(), stream);
+    }
+    upload_descs();
+    build_classes();
     committed_ = true;
-    std::vector<unsigned> all(sys_.size());
-    for (size_t s = 0; s < all.size(); ++s) all[s] = unsigned(s);
-    rebuild(all);
-    forces(false, force_, nullptr);
     pending_ = false;
+    if (!fresh.empty()) {
+        rebuild(fresh);
+        // the survivors' forces are recomputed from the same positions and
+        // lists by the same kernels: unchanged
+        forces(false, force_, nullptr);
+    }
 }
 
 void ensemble::rebuild(std::vector<unsigned> const& R0) {
@@ -261,6 +385,7 @@ void ensemble::rebuild(std::vector<unsigned> const& R0) {
         std::vector<uint32_t> rsys, rpos, rblk, rseg, rblk_small;
         size_t npos = 0, max_ncells = 0;
         unsigned lds = 64, small_max_n = 0;
+        bool multi = false;
         for (size_t k = 0; k < large.size(); ++k) {
             auto& h = *sys_[large[k]];
             rsys.push_back(large[k]);
@@ -270,6 +395,7 @@ void ensemble::rebuild(std::vector<unsigned> const& R0) {
             for (unsigned b = 0; b < h.nblk; ++b) rblk.push_back(uint32_t(k) << 16 | b);
             for (unsigned g = 0; g < h.nseg; ++g) rseg.push_back(uint32_t(large[k]) << 16 | g);
             lds = std::max(lds, h.lds_cap);
+            multi = multi || h.ntype > 1;
         }
         for (unsigned s : small) {
             auto& h = *sys_[s];
@@ -303,7 +429,7 @@ void ensemble::rebuild(std::vector<unsigned> const& R0) {
                                stream);
             launch_ens_reorder(pos_, vel_, image_, id_, posq_, r0_, pos_t_, vel_t_, image_t_, id_t_, posq_t_, descs_,
                                T + o_rsys, T + o_rpos, T + o_rblk, unsigned(rblk.size()), vals_, stream);
-            launch_ens_build(posq_, offsets_, descs_, T + o_rseg, unsigned(rseg.size()), threads_, lds, list_,
+            launch_ens_build(posq_, offsets_, descs_, T + o_rseg, unsigned(rseg.size()), threads_, lds, multi, list_,
                              count_, stats_, stream);
         }
         if (!small.empty()) {
@@ -330,11 +456,23 @@ void ensemble::rebuild(std::vector<unsigned> const& R0) {
             }
         }
         if (relayout) {
-            // new list offsets for every system: all lists are rebuilt
-            layout_lists();
+            // new list offsets; the lists of the systems not being rebuilt
+            // move unchanged, those of R are built again
+            std::vector<size_t> old_off(sys_.size());
+            for (size_t s = 0; s < sys_.size(); ++s) old_off[s] = sys_[s]->list_off;
+            list_words_ = layout_lists();
+            dbuf<uint16_t> moved(std::max<size_t>(list_words_, 1));
+            std::vector<char> in_r(sys_.size(), 0);
+            for (unsigned s : R) in_r[s] = 1;
+            for (size_t s = 0; s < sys_.size(); ++s) {
+                auto const& h = *sys_[s];
+                if (!in_r[s])
+                    d2d(moved.get() + h.list_off, list_.get() + old_off[s], size_t(round_up(h.n, WAVE)) * h.cap,
+                        stream);
+            }
+            HIP_CHECK(hipStreamSynchronize(stream));
+            list_ = std::move(moved);
             upload_descs();
-            R.resize(sys_.size());
-            for (size_t s = 0; s < R.size(); ++s) R[s] = unsigned(s);
             continue;
         }
         if (redesc) {
@@ -354,7 +492,7 @@ void ensemble::rebuild(std::vector<unsigned> const& R0) {
 
 void ensemble::forces(bool aux, float4* force, float* stress) {
     for (auto& c : classes_)
-        launch_ens_force(c.kind, c.trunc, aux, posq_, offsets_, descs_, fclass_tables_.get() + c.seg_off,
+        launch_ens_force(c.kind, c.trunc, aux, c.multi, posq_, offsets_, descs_, fclass_tables_.get() + c.seg_off,
                          unsigned(c.seg.size()), threads_, c.lds_records, fclass_tables_.get() + c.small_off,
                          unsigned(c.small.size()), c.small_max_n, list_, count_, force, stress, total_, stream);
 }
@@ -367,6 +505,11 @@ void ensemble::settle() {
 
 void ensemble::step(unsigned steps) {
     require(committed_, "ensemble: commit() first");
+    apply();
+    if (sys_.empty()) {
+        steps_done += steps;
+        return;
+    }
     std::vector<unsigned> R;
     for (unsigned k = 0; k < steps; ++k) {
         launch_fill_u32(disp2_, 0u, sys_.size(), stream);
@@ -392,6 +535,8 @@ void ensemble::step(unsigned steps) {
 
 void ensemble::thermo(double* out) {
     require(committed_, "ensemble: commit() first");
+    apply();
+    if (sys_.empty()) return;
     forces(true, aux_force_, stress_);
     std::vector<uint32_t> all(sys_.size());
     for (size_t s = 0; s < all.size(); ++s) all[s] = uint32_t(s);
@@ -404,7 +549,7 @@ void ensemble::thermo(double* out) {
 }
 
 void ensemble::get_state(unsigned s, double* position, double* velocity, int* image) {
-    require(committed_ && s < sys_.size(), "ensemble: system index out of range");
+    require(committed_ && s < sys_.size() && sys_[s]->on_device, "ensemble: system index out of range");
     settle();
     auto& h = *sys_[s];
     size_t const n = h.n;

@@ -5,10 +5,16 @@
 // every system, one 4-byte-per-system readback of the displacement maxima, a
 // batched rebuild of the systems whose displacement passed half the skin
 // (one launch per rebuild stage for all of them), and one force launch per
-// (potential, truncation) class.  The per-lane arithmetic is the
+// (potential, truncation, species) class.  The per-lane arithmetic is the
 // single-system path's own device code (staged_force_segment,
 // staged_build_segment, integrate_particle), so a system gives the same
 // result alone and in the mix.
+//
+// Membership changes between steps (continuous batching): systems added after
+// the first apply() and systems retired are placed by the next apply(), which
+// moves every surviving system's device state (particles, lists, cell
+// offsets) unchanged into the new layout, so that a system's trajectory does
+// not depend on which others share the arrays, or when they come and go.
 #pragma once
 
 #include "modules.hpp"
@@ -23,13 +29,19 @@ public:
     ensemble();
     ~ensemble();
 
-    // one cubic NVE system, single species of unit mass; positions and
-    // velocities row-major n x 3 (FP64).  Returns its index.  Systems are
-    // added before commit().
-    unsigned add(unsigned n, double length, int kind, int trunc, double epsilon, double sigma, double r_cut_sigma,
-                 double r_min, double distortion, double smoothing, double timestep, double skin,
-                 double const* position, double const* velocity);
-    void commit();
+    // one cubic NVE system of unit-mass particles; positions and velocities
+    // row-major n x 3 (FP64), species (0 .. ntype - 1, null for all 0) and
+    // ntype x ntype pair parameters.  Returns its index, valid until the next
+    // apply() that retires a system before it.
+    unsigned add(unsigned n, double length, int kind, int trunc, unsigned ntype, double const* epsilon,
+                 double const* sigma, double const* r_cut_sigma, double const* r_min, double const* distortion,
+                 double smoothing, double timestep, double skin, double const* position, double const* velocity,
+                 unsigned const* species);
+    // marks system `sys` for removal at the next apply()
+    void retire(unsigned sys);
+    // places added systems and drops retired ones (the first call places all)
+    void apply();
+    void commit() { apply(); }
     void step(unsigned steps);
     // per system: {sum m v^2, sum en_pot, sum tr stress, n} (FP64)
     void thermo(double* out);
@@ -38,6 +50,21 @@ public:
     void get_state(unsigned sys, double* position, double* velocity, int* image);
     unsigned size() const { return unsigned(sys_.size()); }
     uint64_t rebuilds(unsigned sys) const;
+    uint64_t uid(unsigned sys) const;
+    // index of the system with this uid, or -1
+    int index_of(uint64_t uid) const;
+    // device arrays of system `sys` (placed systems): positions (hi, lo pairs),
+    // images and particle ids from its first particle on; n and box length
+    struct view {
+        float4 const* pos;
+        int4 const* image;
+        uint32_t const* id;
+        unsigned n;
+        double L;
+    };
+    view system_view(unsigned sys) const;
+    // device bytes per particle that a system of this shape occupies
+    static size_t footprint(unsigned n, double length, double r_cut_max, double skin);
     uint64_t steps_done = 0;
     hipStream_t stream;
 
@@ -45,6 +72,7 @@ public:
 private:
     std::vector<std::unique_ptr<sys_host>> sys_;
     bool committed_ = false, pending_ = false;
+    uint64_t next_uid_ = 1;
     unsigned total_ = 0, nblocks_ = 0, threads_ = SEG_THREADS_MAX;
     size_t list_words_ = 0, cells_total_ = 0;
 
@@ -62,6 +90,7 @@ private:
 
     struct force_class {
         int kind, trunc;
+        bool multi;
         std::vector<uint32_t> seg, small;   // (s << 16) | segment; small systems
         size_t seg_off = 0, small_off = 0;  // in fclass_tables_
         unsigned lds_records = 0, small_max_n = 0;
@@ -69,8 +98,11 @@ private:
     std::vector<force_class> classes_;
     dbuf<uint32_t> fclass_tables_;
 
-    void layout_lists();
+    void prepare(sys_host& h);
+    void place_all();
+    size_t layout_lists();
     void upload_descs();
+    void build_classes();
     void rebuild(std::vector<unsigned> const& R);
     void forces(bool aux, float4* force, float* stress);
     void settle();

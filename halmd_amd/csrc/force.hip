@@ -452,7 +452,7 @@ __global__ __launch_bounds__(SEG_THREADS_MAX) void pair_force_staged_kernel(
 
 // ensembles: one workgroup per (system, segment) of a (kind, truncation)
 // class; the per-system parameters come from the descriptor table
-template <int KIND, int TRUNC, bool AUX>
+template <int KIND, int TRUNC, bool AUX, bool MULTI>
 __global__ __launch_bounds__(SEG_THREADS_MAX) void ens_force_kernel(
     uint4 const* __restrict__ posq, uint32_t const* __restrict__ offsets, ens_desc const* __restrict__ descs,
     uint32_t const* __restrict__ fseg, uint16_t const* __restrict__ list, uint32_t const* __restrict__ count,
@@ -464,7 +464,7 @@ __global__ __launch_bounds__(SEG_THREADS_MAX) void ens_force_kernel(
     uint32_t const e = fseg[blockIdx.x];
     unsigned const sys = e >> 16, seg = e & 0xffffu;
     ens_desc const& d = descs[sys];
-    staged_force_segment<KIND, TRUNC, AUX, true, false>(
+    staged_force_segment<KIND, TRUNC, AUX, true, MULTI>(
         posq + d.base, offsets + d.cell_base, d.sp, seg, list + d.list_off, count + d.base, d.cap, d.P, true, 0.5f,
         force + d.base, AUX ? stress + d.base : nullptr, stride, s_rec, st, s_pair);
 }
@@ -645,70 +645,76 @@ void launch_pair_force_staged(uint4 const* posq, uint32_t const* offset, seg_par
 
 namespace {
 template <int KIND, int TRUNC, bool AUX>
-void ens_force_go(uint4 const* posq, uint32_t const* offsets, ens_desc const* descs, uint32_t const* fseg,
-                  unsigned nfseg, unsigned threads, unsigned lds_records, uint32_t const* fsmall, unsigned nsmall,
-                  unsigned small_max_n, uint16_t const* list, uint32_t const* count, float4* force, float* stress,
-                  unsigned stride, hipStream_t s) {
-    if (nfseg)
-        ens_force_kernel<KIND, TRUNC, AUX><<<nfseg, threads, size_t(lds_records) * sizeof(uint4), s>>>(
-            posq, offsets, descs, fseg, list, count, force, stress, stride);
+void ens_force_go(bool multi, uint4 const* posq, uint32_t const* offsets, ens_desc const* descs,
+                  uint32_t const* fseg, unsigned nfseg, unsigned threads, unsigned lds_records,
+                  uint32_t const* fsmall, unsigned nsmall, unsigned small_max_n, uint16_t const* list,
+                  uint32_t const* count, float4* force, float* stress, unsigned stride, hipStream_t s) {
+    size_t const lds = size_t(lds_records) * sizeof(uint4);
+    if (nfseg) {
+        if constexpr (KIND == POTENTIAL_LENNARD_JONES) {
+            if (multi) {
+                ens_force_kernel<KIND, TRUNC, AUX, true><<<nfseg, threads, lds, s>>>(posq, offsets, descs, fseg, list,
+                                                                                    count, force, stress, stride);
+                nfseg = 0;
+            }
+        } else {
+            if (multi) throw error(HALMD_ERR_INVALID_ARGUMENT, "ensemble: several species need Lennard-Jones");
+        }
+        if (nfseg)
+            ens_force_kernel<KIND, TRUNC, AUX, false><<<nfseg, threads, lds, s>>>(posq, offsets, descs, fseg, list,
+                                                                                 count, force, stress, stride);
+    }
     if (nsmall)
         ens_force_small_kernel<KIND, TRUNC, AUX><<<nsmall, 256, size_t(small_max_n) * sizeof(uint4), s>>>(
             posq, descs, fsmall, list, count, force, stress, stride);
 }
 
 template <int KIND, int TRUNC>
-void ens_force_aux(bool aux, uint4 const* posq, uint32_t const* offsets, ens_desc const* descs, uint32_t const* fseg,
-                   unsigned nfseg, unsigned threads, unsigned lds_records, uint32_t const* fsmall, unsigned nsmall,
-                   unsigned small_max_n, uint16_t const* list, uint32_t const* count, float4* force, float* stress,
-                   unsigned stride, hipStream_t s) {
+void ens_force_aux(bool aux, bool multi, uint4 const* posq, uint32_t const* offsets, ens_desc const* descs,
+                   uint32_t const* fseg, unsigned nfseg, unsigned threads, unsigned lds_records,
+                   uint32_t const* fsmall, unsigned nsmall, unsigned small_max_n, uint16_t const* list,
+                   uint32_t const* count, float4* force, float* stress, unsigned stride, hipStream_t s) {
     if (aux)
-        ens_force_go<KIND, TRUNC, true>(posq, offsets, descs, fseg, nfseg, threads, lds_records, fsmall, nsmall,
-                                        small_max_n, list, count, force, stress, stride, s);
+        ens_force_go<KIND, TRUNC, true>(multi, posq, offsets, descs, fseg, nfseg, threads, lds_records, fsmall,
+                                        nsmall, small_max_n, list, count, force, stress, stride, s);
     else
-        ens_force_go<KIND, TRUNC, false>(posq, offsets, descs, fseg, nfseg, threads, lds_records, fsmall, nsmall,
-                                         small_max_n, list, count, force, stress, stride, s);
+        ens_force_go<KIND, TRUNC, false>(multi, posq, offsets, descs, fseg, nfseg, threads, lds_records, fsmall,
+                                         nsmall, small_max_n, list, count, force, stress, stride, s);
 }
 
 template <int KIND>
-void ens_force_trunc(int trunc, bool aux, uint4 const* posq, uint32_t const* offsets, ens_desc const* descs,
-                     uint32_t const* fseg, unsigned nfseg, unsigned threads, unsigned lds_records,
-                     uint32_t const* fsmall, unsigned nsmall, unsigned small_max_n, uint16_t const* list,
-                     uint32_t const* count, float4* force, float* stress, unsigned stride, hipStream_t s) {
+void ens_force_trunc(int trunc, bool aux, bool multi, uint4 const* posq, uint32_t const* offsets,
+                     ens_desc const* descs, uint32_t const* fseg, unsigned nfseg, unsigned threads,
+                     unsigned lds_records, uint32_t const* fsmall, unsigned nsmall, unsigned small_max_n,
+                     uint16_t const* list, uint32_t const* count, float4* force, float* stress, unsigned stride,
+                     hipStream_t s) {
+#define HALMD_ENS_TRUNC(T)                                                                                         \
+    ens_force_aux<KIND, T>(aux, multi, posq, offsets, descs, fseg, nfseg, threads, lds_records, fsmall, nsmall,   \
+                           small_max_n, list, count, force, stress, stride, s)
     switch (trunc) {
-    case TRUNC_SHARP:
-        ens_force_aux<KIND, TRUNC_SHARP>(aux, posq, offsets, descs, fseg, nfseg, threads, lds_records, fsmall, nsmall,
-                                         small_max_n, list, count, force, stress, stride, s);
-        break;
-    case TRUNC_SHIFTED:
-        ens_force_aux<KIND, TRUNC_SHIFTED>(aux, posq, offsets, descs, fseg, nfseg, threads, lds_records, fsmall,
-                                           nsmall, small_max_n, list, count, force, stress, stride, s);
-        break;
-    case TRUNC_FORCE_SHIFTED:
-        ens_force_aux<KIND, TRUNC_FORCE_SHIFTED>(aux, posq, offsets, descs, fseg, nfseg, threads, lds_records, fsmall,
-                                                 nsmall, small_max_n, list, count, force, stress, stride, s);
-        break;
-    case TRUNC_SMOOTH_R4:
-        ens_force_aux<KIND, TRUNC_SMOOTH_R4>(aux, posq, offsets, descs, fseg, nfseg, threads, lds_records, fsmall,
-                                             nsmall, small_max_n, list, count, force, stress, stride, s);
-        break;
+    case TRUNC_SHARP: HALMD_ENS_TRUNC(TRUNC_SHARP); break;
+    case TRUNC_SHIFTED: HALMD_ENS_TRUNC(TRUNC_SHIFTED); break;
+    case TRUNC_FORCE_SHIFTED: HALMD_ENS_TRUNC(TRUNC_FORCE_SHIFTED); break;
+    case TRUNC_SMOOTH_R4: HALMD_ENS_TRUNC(TRUNC_SMOOTH_R4); break;
     default: throw error(HALMD_ERR_INVALID_ARGUMENT, "unknown truncation");
     }
+#undef HALMD_ENS_TRUNC
 }
 } // namespace
 
-void launch_ens_force(int kind, int trunc, bool aux, uint4 const* posq, uint32_t const* offsets,
+void launch_ens_force(int kind, int trunc, bool aux, bool multi, uint4 const* posq, uint32_t const* offsets,
                       ens_desc const* descs, uint32_t const* fseg, unsigned nfseg, unsigned threads,
                       unsigned lds_records, uint32_t const* fsmall, unsigned nsmall, unsigned small_max_n,
                       uint16_t const* list, uint32_t const* count, float4* force, float* stress, unsigned stride,
                       hipStream_t s) {
     if (!nfseg && !nsmall) return;
     if (kind == POTENTIAL_LENNARD_JONES)
-        ens_force_trunc<POTENTIAL_LENNARD_JONES>(trunc, aux, posq, offsets, descs, fseg, nfseg, threads, lds_records,
-                                                 fsmall, nsmall, small_max_n, list, count, force, stress, stride, s);
+        ens_force_trunc<POTENTIAL_LENNARD_JONES>(trunc, aux, multi, posq, offsets, descs, fseg, nfseg, threads,
+                                                 lds_records, fsmall, nsmall, small_max_n, list, count, force,
+                                                 stress, stride, s);
     else if (kind == POTENTIAL_MORSE)
-        ens_force_trunc<POTENTIAL_MORSE>(trunc, aux, posq, offsets, descs, fseg, nfseg, threads, lds_records, fsmall,
-                                         nsmall, small_max_n, list, count, force, stress, stride, s);
+        ens_force_trunc<POTENTIAL_MORSE>(trunc, aux, multi, posq, offsets, descs, fseg, nfseg, threads, lds_records,
+                                         fsmall, nsmall, small_max_n, list, count, force, stress, stride, s);
     else
         throw error(HALMD_ERR_INVALID_ARGUMENT, "unknown potential");
     HIP_CHECK(hipGetLastError());

@@ -299,15 +299,17 @@ void launch_ens_reorder(float4* pos, float4* vel, int4* image, uint32_t* id, uin
 void launch_ens_r0(float4 const* pos, float4* r0, ens_desc const* descs, uint32_t const* rblk_sys, unsigned nrblk,
                    hipStream_t s);
 // list builds: rseg (s << 16) | segment for cell systems; rsmall the
-// all-pairs systems (one workgroup each); stats[2 s .. 2 s + 1]
+// all-pairs systems (one workgroup each); stats[2 s .. 2 s + 1].  multi: cutoffs
+// by species pair (the same lists as the one-species rule when ntype is 1)
 void launch_ens_build(uint4 const* posq, uint32_t const* offsets, ens_desc const* descs, uint32_t const* rseg,
-                      unsigned nrseg, unsigned threads, unsigned lds_records, uint16_t* list, uint32_t* count,
-                      uint32_t* stats, hipStream_t s);
+                      unsigned nrseg, unsigned threads, unsigned lds_records, bool multi, uint16_t* list,
+                      uint32_t* count, uint32_t* stats, hipStream_t s);
 void launch_ens_build_small(uint4 const* posq, ens_desc const* descs, uint32_t const* rsmall, unsigned nsmall,
                             unsigned max_n, uint16_t* list, uint32_t* count, uint32_t* stats, hipStream_t s);
-// forces of one (kind, truncation) class: fseg (s << 16) | segment, fsmall
-// the all-pairs systems; aux: energy in force.w and stress (stride = total)
-void launch_ens_force(int kind, int trunc, bool aux, uint4 const* posq, uint32_t const* offsets,
+// forces of one (kind, truncation, species) class: fseg (s << 16) | segment,
+// fsmall the all-pairs systems; aux: energy in force.w and stress (stride =
+// total); multi: pair parameters by species (Lennard-Jones, cell systems)
+void launch_ens_force(int kind, int trunc, bool aux, bool multi, uint4 const* posq, uint32_t const* offsets,
                       ens_desc const* descs, uint32_t const* fseg, unsigned nfseg, unsigned threads,
                       unsigned lds_records, uint32_t const* fsmall, unsigned nsmall, unsigned small_max_n,
                       uint16_t const* list, uint32_t const* count, float4* force, float* stress, unsigned stride,

@@ -1,0 +1,295 @@
+// Continuous-batching MD service (see serve.hpp).
+#include "serve.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace halmd {
+
+// ---------------------------------------------------------------------------
+// job_correlator: the blocking scheme of blocking_scheme.cpp:35-200 (levels of
+// intervals resolution * block_size^l, with resolution * shift * block_size^l
+// interleaved; a full block correlates its first sample with every sample, then
+// drops ceil(separation / interval) samples), with the arithmetic of the
+// single-system blocking_scheme module
+// ---------------------------------------------------------------------------
+job_correlator::job_correlator(unsigned n, double length, double dt, uint64_t resolution_steps, uint64_t max_steps,
+                               unsigned bsz, unsigned shift, unsigned sep, std::vector<double> const& wavenumber,
+                               double tolerance, unsigned max_count, hipStream_t stream)
+    : timestep(dt), n_(n), block_size_(bsz), separation_(sep), s_(stream)
+{
+    require(block_size_ >= 2, "correlator: minimum block size is 2");
+    require(shift != 0 && shift < block_size_, "correlator: coarse-graining shift must be in 1 .. block size - 1");
+    require(resolution_steps > 0, "correlator: resolution below one step");
+    for (int k = 0; k < 3; ++k) L_[k] = length;
+    for (uint64_t s = resolution_steps; s <= max_steps; s *= block_size_) {
+        interval.push_back(s);
+        if (shift > 1 && s * shift <= max_steps) interval.push_back(s * shift);
+    }
+    if (interval.empty()) interval.push_back(resolution_steps);
+    origin.assign(interval.size(), 0);
+    pos_blocks_.resize(interval.size());
+    rho_blocks_.resize(interval.size());
+    msd.resize(interval.size() * block_size_);
+    msd_partials_.resize(size_t(REDUCE_BLOCKS) * 2 * block_size_);
+    msd_out_.resize(interval.size() * 2 * block_size_);
+    h_msd_.resize(interval.size() * 2 * block_size_);
+    if (!wavenumber.empty()) {
+        wv = std::make_unique<wavevector>(wavenumber, L_, tolerance, max_count);
+        nq_ = unsigned(wv->q.size() / 3);
+        q_.resize(std::max(3u * nq_, 3u));
+        q_.upload(wv->q.data(), wv->q.size(), s_);
+        dm_partials_.resize(size_t(div_up(nq_, DM_KCHUNK)) * DM_BLOCKS * 2 * DM_KCHUNK);
+        h_rho_.resize(interval.size() * block_size_ * 2 * size_t(std::max(nq_, 1u)));
+        isf.resize(interval.size() * block_size_ * wv->shell.size());
+        HIP_CHECK(hipStreamSynchronize(s_));
+    }
+}
+
+job_correlator::~job_correlator() {
+    // samples may still be read by enqueued work
+    (void)hipStreamSynchronize(s_);
+}
+
+job_correlator::sample_ptr job_correlator::take(std::vector<std::unique_ptr<dbuf<double>>>& pool, size_t size) {
+    dbuf<double>* b;
+    if (pool.empty()) {
+        b = new dbuf<double>(size);
+    } else {
+        b = pool.back().release();
+        pool.pop_back();
+    }
+    return sample_ptr(b, [&pool](dbuf<double>* p) { pool.emplace_back(p); });
+}
+
+void job_correlator::sample(uint64_t step, ensemble::view const& v) {
+    require(v.n == n_, "correlator: particle count changed");
+    sample_ptr ps, rs;
+    for (unsigned i = 0; i < interval.size(); ++i) {
+        if (step < origin[i] || (step - origin[i]) % interval[i] != 0) continue;
+        if (!ps) {
+            ps = take(pos_pool_, 3 * size_t(n_));
+            launch_sample_position(v.pos, v.image, v.id, n_, L_, *ps, s_);
+        }
+        pos_blocks_[i].push_back(ps);
+        if (pos_blocks_[i].size() > block_size_) pos_blocks_[i].erase(pos_blocks_[i].begin());
+        if (nq_) {
+            if (!rs) {
+                rs = take(rho_pool_, 2 * size_t(nq_));
+                launch_density_mode(v.pos, n_, q_, nq_, dm_partials_, *rs, s_);
+            }
+            rho_blocks_[i].push_back(rs);
+            if (rho_blocks_[i].size() > block_size_) rho_blocks_[i].erase(rho_blocks_[i].begin());
+        }
+        if (pos_blocks_[i].size() == block_size_) process(i);
+    }
+}
+
+void job_correlator::process(unsigned level) {
+    auto& blk = pos_blocks_[level];
+    unsigned const nl = unsigned(blk.size());
+    if (!nl) return;
+    size_t const mo = size_t(level) * 2 * block_size_;
+    for (unsigned l = 0; l < nl; ++l)
+        launch_msd_partials(*blk.front(), *blk[l], n_, msd_partials_, 2 * nl, 2 * l, s_);
+    launch_reduce_final(msd_partials_, 2 * nl, msd_out_.get() + mo, s_);
+    HIP_CHECK(hipMemcpyAsync(h_msd_.get() + mo, msd_out_.get() + mo, 2 * nl * sizeof(double), hipMemcpyDeviceToHost,
+                             s_));
+    if (nq_) {
+        auto const& rb = rho_blocks_[level];
+        for (unsigned l = 0; l < nl; ++l) {
+            size_t const ro = (size_t(level) * block_size_ + l) * 2 * nq_;
+            HIP_CHECK(hipMemcpyAsync(h_rho_.get() + ro, rb[l]->get(), 2 * nq_ * sizeof(double),
+                                     hipMemcpyDeviceToHost, s_));
+        }
+    }
+    pending_.emplace_back(level, nl);
+    uint64_t const iv = interval[level];
+    uint64_t const skip = std::max<uint64_t>((separation_ + iv - 1) / iv, 1);
+    origin[level] += skip * iv;
+    for (uint64_t k = 0; k < skip; ++k) {
+        if (!pos_blocks_[level].empty()) pos_blocks_[level].erase(pos_blocks_[level].begin());
+        if (!rho_blocks_[level].empty()) rho_blocks_[level].erase(rho_blocks_[level].begin());
+    }
+}
+
+bool job_correlator::flush() {
+    bool any = false;
+    for (unsigned i = 0; i < interval.size(); ++i) {
+        if (!pos_blocks_[i].empty()) {
+            process(i);
+            any = true;
+        }
+    }
+    return any;
+}
+
+void job_correlator::collect() {
+    for (auto const& [level, nl] : pending_) {
+        size_t const mo = size_t(level) * 2 * block_size_;
+        for (unsigned l = 0; l < nl; ++l) {
+            double const s1 = h_msd_[mo + 2 * l], s2 = h_msd_[mo + 2 * l + 1];
+            accumulator a;
+            a.n = n_;
+            a.m = s1 / n_;
+            a.v = std::max(0.0, s2 - s1 * s1 / n_);
+            msd[size_t(level) * block_size_ + l].merge(a);
+        }
+        if (nq_) {
+            size_t const nshell = wv->shell.size();
+            double const* r1 = h_rho_.get() + size_t(level) * block_size_ * 2 * nq_;
+            for (unsigned l = 0; l < nl; ++l) {
+                double const* r2 = h_rho_.get() + (size_t(level) * block_size_ + l) * 2 * nq_;
+                for (size_t sh = 0; sh < nshell; ++sh) {
+                    accumulator a;
+                    for (size_t k = wv->shell[sh].first; k < wv->shell[sh].second; ++k)
+                        a((r1[2 * k] * r2[2 * k] + r1[2 * k + 1] * r2[2 * k + 1]) / n_);
+                    isf[(size_t(level) * block_size_ + l) * nshell + sh].merge(a);
+                }
+            }
+        }
+    }
+    pending_.clear();
+}
+
+// ---------------------------------------------------------------------------
+// server
+// ---------------------------------------------------------------------------
+server::server(size_t budget_bytes, unsigned max_jobs, unsigned tick_steps)
+    : budget_(budget_bytes), max_jobs_(max_jobs), tick_(tick_steps), t0_(std::chrono::steady_clock::now())
+{
+    require(max_jobs_ >= 1, "server: at least one running job");
+    require(tick_ >= 1, "server: a tick is at least one step");
+}
+
+double server::now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
+}
+
+size_t server::job_footprint(job_request const& r) const {
+    // ensemble state, lists and cells, plus at most block_size position
+    // samples (3 FP64) per correlator level
+    double rc = 0;
+    for (size_t k = 0; k < r.r_cut_sigma.size(); ++k) rc = std::max(rc, r.r_cut_sigma[k] * r.sigma[k]);
+    size_t levels = 1;
+    for (uint64_t s = tick_; s * r.block_size <= std::max<uint64_t>(r.steps, tick_); s *= r.block_size) ++levels;
+    if (r.shift > 1) levels *= 2;
+    return ensemble::footprint(r.n, r.length, rc, r.skin) + levels * r.block_size * 24 * size_t(r.n);
+}
+
+uint64_t server::submit(job_request r) {
+    require(r.n >= 2, "server: a job needs particles");
+    require(r.position.size() == 3 * size_t(r.n) && r.velocity.size() == 3 * size_t(r.n),
+            "server: positions and velocities are n x 3");
+    size_t const nn = size_t(r.ntype) * r.ntype;
+    require(r.epsilon.size() == nn && r.sigma.size() == nn && r.r_cut_sigma.size() == nn,
+            "server: pair parameters are ntype x ntype");
+    require(r.species.empty() || r.species.size() == r.n, "server: one species per particle");
+    require(r.steps >= 1, "server: a job runs at least one step");
+    auto j = std::make_unique<job>();
+    j->id = next_id_++;
+    j->footprint = job_footprint(r);
+    j->req = std::move(r);
+    job* p = j.get();
+    jobs_.emplace(p->id, std::move(j));
+    queue_.push_back(p);
+    return p->id;
+}
+
+server::job const& server::get(uint64_t id) const {
+    auto it = jobs_.find(id);
+    require(it != jobs_.end(), "server: unknown job");
+    return *it->second;
+}
+
+void server::release(uint64_t id) {
+    auto it = jobs_.find(id);
+    require(it != jobs_.end(), "server: unknown job");
+    require(it->second->status == 2, "server: only finished jobs are released");
+    jobs_.erase(it);
+}
+
+ensemble::view server::view_of(job const& j) const {
+    int const s = ens.index_of(j.uid);
+    require(s >= 0, "server: job not in the ensemble");
+    return ens.system_view(unsigned(s));
+}
+
+unsigned server::tick() {
+    // admission in arrival order while the footprint fits (a job larger than
+    // the whole budget runs when nothing else does)
+    std::vector<job*> fresh;
+    while (!queue_.empty() && active_.size() < max_jobs_) {
+        job& j = *queue_.front();
+        if (used_ + j.footprint > budget_ && !active_.empty()) break;
+        auto& r = j.req;
+        std::vector<double> ones(size_t(r.ntype) * r.ntype, 1.0);
+        unsigned const idx =
+            ens.add(r.n, r.length, r.kind, r.trunc, r.ntype, r.epsilon.data(), r.sigma.data(), r.r_cut_sigma.data(),
+                    ones.data(), ones.data(), 0.005, r.timestep, r.skin, r.position.data(), r.velocity.data(),
+                    r.species.empty() ? nullptr : r.species.data());
+        j.uid = ens.uid(idx);
+        r.position.clear(); r.position.shrink_to_fit();
+        r.velocity.clear(); r.velocity.shrink_to_fit();
+        r.species.clear(); r.species.shrink_to_fit();
+        uint64_t const steps = (r.steps + tick_ - 1) / tick_ * tick_;
+        j.corr = std::make_unique<job_correlator>(r.n, r.length, r.timestep, tick_, steps, r.block_size, r.shift,
+                                                  r.separation, r.wavenumber, r.tolerance, r.max_count, ens.stream);
+        j.status = 1;
+        j.t_admit = now();
+        used_ += j.footprint;
+        queue_.pop_front();
+        active_.push_back(&j);
+        fresh.push_back(&j);
+    }
+    ens.apply();
+    for (job* j : fresh) j->corr->sample(0, view_of(*j));
+    if (!active_.empty()) {
+        ens.step(tick_);
+        for (job* j : active_) {
+            j->step += tick_;
+            psteps_ += uint64_t(tick_) * j->req.n;
+            j->corr->sample(j->step, view_of(*j));
+        }
+    }
+    HIP_CHECK(hipStreamSynchronize(ens.stream));
+    for (job* j : active_) j->corr->collect();
+
+    // completion
+    std::vector<job*> done;
+    for (job* j : active_)
+        if (j->step >= j->req.steps) done.push_back(j);
+    if (!done.empty()) {
+        bool more = true;
+        while (more) {
+            more = false;
+            for (job* j : done) more = j->corr->flush() || more;
+            HIP_CHECK(hipStreamSynchronize(ens.stream));
+            for (job* j : done) j->corr->collect();
+        }
+        std::vector<double> th(4 * size_t(ens.size()));
+        ens.thermo(th.data());
+        for (job* j : done) {
+            unsigned const s = unsigned(ens.index_of(j->uid));
+            double const n = th[4 * size_t(s) + 3];
+            j->en_pot = th[4 * size_t(s) + 1] / n;
+            j->en_kin = 0.5 * th[4 * size_t(s)] / n;
+            if (j->req.keep_state) {
+                j->position.resize(3 * size_t(j->req.n));
+                j->velocity.resize(3 * size_t(j->req.n));
+                j->image.resize(3 * size_t(j->req.n));
+                ens.get_state(s, j->position.data(), j->velocity.data(), j->image.data());
+            }
+            ens.retire(s);
+            j->status = 2;
+            j->t_done = now();
+            used_ -= j->footprint;
+        }
+        active_.erase(std::remove_if(active_.begin(), active_.end(), [](job* j) { return j->status == 2; }),
+                      active_.end());
+    }
+    return unsigned(active_.size());
+}
+
+} // namespace halmd

@@ -1,0 +1,133 @@
+// Continuous-batching MD service over one ensemble.
+//
+// Jobs (one cubic NVE system each, with its step count and its dynamic
+// correlators) arrive at any time.  At every tick boundary the server admits
+// queued jobs in arrival order while their device footprint fits the budget
+// and the job cap, places them into the running ensemble (every running job's
+// state stays where the ensemble moves it, unchanged), advances all running
+// jobs by one tick of steps in shared launches, samples each job's
+// correlators, and retires the jobs that reached their step count.  Particle
+// state and correlator samples stay on the device for the life of a job.
+//
+// No counterpart in the reference, which runs one system per process
+// (doc/usage/multi-gpu.rst.in:4-47); the correlators follow its blocking
+// scheme (observables/dynamics/blocking_scheme.cpp:35-200) with mean-square
+// displacement and intermediate scattering function
+// (observables/dynamics/{mean_square_displacement,intermediate_scattering_function}.hpp).
+#pragma once
+
+#include "ensemble.hpp"
+
+#include <chrono>
+#include <deque>
+#include <map>
+
+namespace halmd {
+
+// Blocking-scheme correlator of one job whose samples come from its range of
+// the ensemble arrays: position samples (FP64, particle-ID order) and density
+// modes stay on the device in a pool owned by the correlator; processing a
+// level enqueues the MSD reductions and the density-mode copies, and collect()
+// folds them into FP64 accumulators after a stream synchronisation.
+class job_correlator {
+public:
+    job_correlator(unsigned n, double length, double timestep, uint64_t resolution_steps, uint64_t max_steps,
+                   unsigned block_size, unsigned shift, unsigned separation, std::vector<double> const& wavenumber,
+                   double tolerance, unsigned max_count, hipStream_t stream);
+    ~job_correlator();
+    // enqueue the sample of `step` (levels due at it) from the system's arrays
+    void sample(uint64_t step, ensemble::view const& v);
+    // enqueue the processing of every level that still holds samples; false
+    // when none does
+    bool flush();
+    // after a synchronisation of the stream: fold the processed levels into
+    // the accumulators
+    void collect();
+
+    unsigned levels() const { return unsigned(interval.size()); }
+    unsigned block_size() const { return block_size_; }
+    unsigned nshell() const { return unsigned(wv ? wv->shell.size() : 0); }
+    std::vector<uint64_t> interval;
+    std::vector<accumulator> msd, isf;   // [level][lag], [level][lag][shell]
+    double timestep;
+
+private:
+    using sample_ptr = std::shared_ptr<dbuf<double>>;
+    unsigned n_, block_size_, separation_, nq_ = 0;
+    double L_[3];
+    hipStream_t s_;
+    std::vector<uint64_t> origin;
+    std::unique_ptr<wavevector> wv;
+    dbuf<double> q_, dm_partials_, msd_partials_, msd_out_;
+    hbuf<double> h_msd_, h_rho_;
+    // pools of sample buffers: a buffer returns to its pool when no block
+    // holds it, and a later sample may reuse it on the same stream
+    std::vector<std::unique_ptr<dbuf<double>>> pos_pool_, rho_pool_;
+    std::vector<std::vector<sample_ptr>> pos_blocks_, rho_blocks_;
+    std::vector<std::pair<unsigned, unsigned>> pending_;   // (level, samples) processed since collect()
+    sample_ptr take(std::vector<std::unique_ptr<dbuf<double>>>& pool, size_t size);
+    void process(unsigned level);
+};
+
+struct job_request {
+    unsigned n = 0;
+    double length = 0;
+    int kind = 0, trunc = 0;
+    unsigned ntype = 1;
+    std::vector<double> epsilon, sigma, r_cut_sigma;
+    double timestep = 0.005, skin = 0.3;
+    std::vector<double> position, velocity;
+    std::vector<unsigned> species;
+    uint64_t steps = 0;
+    double arrival = 0;   // server clock, seconds
+    // correlators: blocking scheme with the tick as resolution
+    unsigned block_size = 10, shift = 2, separation = 0;
+    std::vector<double> wavenumber;
+    double tolerance = 0.01;
+    unsigned max_count = 7;
+    bool keep_state = false;
+};
+
+class server {
+public:
+    server(size_t budget_bytes, unsigned max_jobs, unsigned tick_steps);
+    uint64_t submit(job_request r);
+    // one tick: admission, placement, tick_steps steps, correlator samples,
+    // completion; returns the number of running jobs
+    unsigned tick();
+    double now() const;
+
+    struct job {
+        uint64_t id = 0;
+        job_request req;
+        int status = 0;   // 0 queued, 1 running, 2 done
+        uint64_t step = 0, uid = 0;
+        size_t footprint = 0;
+        double t_admit = 0, t_done = 0;
+        double en_pot = 0, en_kin = 0;   // per particle at completion
+        std::unique_ptr<job_correlator> corr;
+        std::vector<double> position, velocity;
+        std::vector<int> image;
+    };
+    job const& get(uint64_t id) const;
+    void release(uint64_t id);
+    unsigned active() const { return unsigned(active_.size()); }
+    unsigned queued() const { return unsigned(queue_.size()); }
+    size_t used() const { return used_; }
+    uint64_t particle_steps() const { return psteps_; }
+
+    ensemble ens;
+
+private:
+    size_t budget_, used_ = 0;
+    unsigned max_jobs_, tick_;
+    std::chrono::steady_clock::time_point t0_;
+    std::map<uint64_t, std::unique_ptr<job>> jobs_;
+    std::deque<job*> queue_;
+    std::vector<job*> active_;
+    uint64_t next_id_ = 1, psteps_ = 0;
+    size_t job_footprint(job_request const& r) const;
+    ensemble::view view_of(job const& j) const;
+};
+
+} // namespace halmd

@@ -11,7 +11,7 @@ import numpy as np
 from . import mdsim, observables
 from .mdsim import positions, velocities
 
-__all__ = ["fluid", "lennard_jones", "kob_andersen", "KA_EPSILON", "KA_SIGMA"]
+__all__ = ["fluid", "lennard_jones", "kob_andersen", "KA_EPSILON", "KA_SIGMA", "serving_stream"]
 
 KA_EPSILON = np.array([[1.0, 1.5], [1.5, 0.5]])
 KA_SIGMA = np.array([[1.0, 0.8], [0.8, 0.88]])
@@ -82,9 +82,7 @@ def lennard_jones(nparticle, density=0.8442, temperature=1.2, r_cut=2.5, truncat
 def kob_andersen(nparticle, density=1.2, temperature=0.7, fraction_b=0.2, seed=1, **kw):
     """80:20 Kob-Andersen mixture, species assigned by a seeded shuffle."""
     n = int(nparticle)
-    rng = np.random.default_rng(seed + 7)
-    species = np.zeros(n, dtype=np.uint32)
-    species[rng.permutation(n)[:int(round(fraction_b * n))]] = 1
+    species = ka_species(n, seed, fraction_b)
     pot = mdsim.potentials.pair.lennard_jones(epsilon=KA_EPSILON, sigma=KA_SIGMA).truncate("shifted", cutoff=2.5)
     return fluid(n, density, temperature, pot, species=species, seed=seed, **kw)
 
@@ -116,10 +114,22 @@ def spec_potential(name):
         "shifted", cutoff=rc)
 
 
+def ka_species(n, seed, fraction_b=0.2):
+    """Species of the Kob-Andersen mixture: a seeded shuffle, as kob_andersen() assigns them."""
+    rng = np.random.default_rng(seed + 7)
+    species = np.zeros(n, dtype=np.uint32)
+    species[rng.permutation(n)[:int(round(fraction_b * n))]] = 1
+    return species
+
+
+def spec_species(spec):
+    return ka_species(spec["n"], spec["seed"]) if spec["potential"] == "ka" else None
+
+
 def single_from_spec(spec, timestep=0.003, skin=0.3):
     """The system of `spec` alone, through the per-system modules."""
     return fluid(spec["n"], spec["density"], spec["temperature"], spec_potential(spec["potential"]), skin=skin,
-                 timestep=timestep, seed=spec["seed"])
+                 timestep=timestep, seed=spec["seed"], species=spec_species(spec))
 
 
 def ensemble_from_specs(specs, timestep=0.003, skin=0.3):
@@ -128,6 +138,35 @@ def ensemble_from_specs(specs, timestep=0.003, skin=0.3):
     e = ensemble()
     for sp in specs:
         L, r0, v0, _ = system_inputs(sp["n"], sp["density"], sp["temperature"], sp["seed"])
-        e.add(r0, v0, L[0], potential=sp["potential"], timestep=timestep, skin=skin)
+        e.add(r0, v0, L[0], potential=sp["potential"], timestep=timestep, skin=skin, species=spec_species(sp))
     e.commit()
     return e
+
+
+# ---------------------------------------------------------------------------
+# served job streams
+# ---------------------------------------------------------------------------
+SERVING_WAVENUMBERS = (1.5, 2.5, 3.5, 4.5, 5.5, 6.5, 7.25, 8.5)
+
+
+def serving_stream(rate=50.0, duration=60.0, seed=2025, step_scale=1.0, n_median=2.0e4, n_sigma=0.75,
+                   n_clip=(1000, 200000), density_range=(0.8, 1.2), temperature_range=(0.45, 1.5),
+                   steps_range=(1.0e3, 1.0e5), potentials=("lj", "ka", "wca")):
+    """Poisson arrivals of MD jobs: arrival times (s) at `rate` over `duration`,
+    N log-normal (median n_median, log-sigma n_sigma, clipped), density and
+    temperature uniform, steps log-uniform in steps_range times step_scale,
+    potential drawn from `potentials`."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    lo, hi = np.log(steps_range[0]), np.log(steps_range[1])
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= duration:
+            break
+        n = int(np.clip(np.exp(rng.normal(np.log(n_median), n_sigma)), *n_clip))
+        steps = max(1, int(round(np.exp(rng.uniform(lo, hi)) * step_scale)))
+        out.append(dict(arrival=t, n=n, density=float(rng.uniform(*density_range)),
+                        temperature=float(rng.uniform(*temperature_range)),
+                        potential=str(potentials[rng.integers(len(potentials))]), steps=steps,
+                        seed=int(seed * 100000 + len(out))))
+    return out

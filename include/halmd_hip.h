@@ -56,6 +56,7 @@ typedef struct halmd_density_mode_s halmd_density_mode;
 typedef struct halmd_blocking_scheme_s halmd_blocking_scheme;
 typedef struct halmd_sampler_s halmd_sampler;
 typedef struct halmd_ensemble_s halmd_ensemble;
+typedef struct halmd_server_s halmd_server;
 
 /* ---- library ---------------------------------------------------------- */
 HALMD_API const char* halmd_last_error(void);
@@ -329,11 +330,14 @@ HALMD_API int halmd_observables_sampler_thermodynamics_records(halmd_sampler* s,
 
 /* ---- ensembles: many independent NVE systems stepped in shared launches --
  * No counterpart in the reference (it runs one system per process,
- * doc/usage/multi-gpu.rst.in:4-47).  Each system: cubic box, one species of
- * unit mass, a truncated pair potential (kind as POTENTIAL_* of the potential
- * create functions: 0 Lennard-Jones, 1 Morse; truncation by name), velocity
- * Verlet; neighbour lists as the first neighbour module of a particle
- * instance builds them (all-pairs lists under 3 cells per dimension). */
+ * doc/usage/multi-gpu.rst.in:4-47).  Each system: cubic box, unit-mass
+ * particles of one or several species, a truncated pair potential (kind as
+ * POTENTIAL_* of the potential create functions: 0 Lennard-Jones, 1 Morse;
+ * truncation by name), velocity Verlet; neighbour lists as the first
+ * neighbour module of a particle instance builds them (all-pairs lists under
+ * 3 cells per dimension).  Systems may be added and retired between steps:
+ * the next apply (or step) places them and moves every other system's device
+ * state unchanged, so a system's trajectory is the same alone and in any mix. */
 HALMD_API int halmd_ensemble_create(halmd_ensemble** out);
 HALMD_API int halmd_ensemble_destroy(halmd_ensemble* e);
 /* positions/velocities row-major n x 3; *index receives the system index */
@@ -341,8 +345,21 @@ HALMD_API int halmd_ensemble_add(halmd_ensemble* e, unsigned n, double length, i
                                  double epsilon, double sigma, double r_cut_sigma, double r_min, double distortion,
                                  double smoothing, double timestep, double skin, const double* position,
                                  const double* velocity, unsigned* index);
-/* lay out all systems, build lists and forces; no systems are added after it */
+/* several species: ntype x ntype parameter matrices (row-major), species
+ * per particle (0 .. ntype - 1); Lennard-Jones only */
+HALMD_API int halmd_ensemble_add_mixture(halmd_ensemble* e, unsigned n, double length, int kind,
+                                         const char* truncation, unsigned ntype, const double* epsilon,
+                                         const double* sigma, const double* r_cut_sigma, const double* r_min,
+                                         const double* distortion, double smoothing, double timestep, double skin,
+                                         const double* position, const double* velocity, const unsigned* species,
+                                         unsigned* index);
+/* lay out all systems, build lists and forces (the first placement) */
 HALMD_API int halmd_ensemble_commit(halmd_ensemble* e);
+/* marks a system for removal; the next apply drops it (indices of later
+ * systems decrease by one) */
+HALMD_API int halmd_ensemble_retire(halmd_ensemble* e, unsigned system);
+/* places added systems and drops retired ones */
+HALMD_API int halmd_ensemble_apply(halmd_ensemble* e);
 HALMD_API int halmd_ensemble_step(halmd_ensemble* e, uint64_t steps);
 /* steps with HIP events on the ensemble stream around them; elapsed ms */
 HALMD_API int halmd_ensemble_step_timed(halmd_ensemble* e, uint64_t steps, double* ms);
@@ -354,6 +371,49 @@ HALMD_API int halmd_ensemble_thermo(halmd_ensemble* e, double* out);
 HALMD_API int halmd_ensemble_state(halmd_ensemble* e, unsigned system, double* position, double* velocity,
                                    int* image);
 HALMD_API int halmd_ensemble_rebuilds(const halmd_ensemble* e, unsigned system, uint64_t* count);
+
+/* ---- serving: continuous batching of MD jobs over one ensemble -----------
+ * No counterpart in the reference (one system per process,
+ * doc/usage/multi-gpu.rst.in:4-47); for callers that stream many jobs through
+ * one GPU.  A job is one ensemble system (cubic box, unit masses, Lennard-Jones
+ * or Morse with r_min = distortion = 1) with a step count and blocking-scheme
+ * correlators (observables/dynamics/blocking_scheme.cpp:35-200): MSD, and the
+ * ISF on the wavevector shells of the given wavenumbers when nwavenumber > 0,
+ * at the tick as resolution.  Jobs are admitted at tick boundaries in
+ * submission order while their device footprint fits budget_bytes and fewer
+ * than max_jobs run; their state and correlator samples stay on the device. */
+HALMD_API int halmd_server_create(size_t budget_bytes, unsigned max_jobs, unsigned tick_steps, halmd_server** out);
+HALMD_API int halmd_server_destroy(halmd_server* s);
+/* seconds since the server was created (the clock of arrival times) */
+HALMD_API int halmd_server_now(const halmd_server* s, double* seconds);
+/* queues one job; arrival (server clock) is the reference of its latency */
+HALMD_API int halmd_server_submit(halmd_server* s, unsigned n, double length, int kind, const char* truncation,
+                                  unsigned ntype, const double* epsilon, const double* sigma,
+                                  const double* r_cut_sigma, double timestep, double skin, const double* position,
+                                  const double* velocity, const unsigned* species, uint64_t steps, double arrival,
+                                  unsigned block_size, unsigned shift, unsigned separation, const double* wavenumber,
+                                  unsigned nwavenumber, double tolerance, unsigned max_count, int keep_state,
+                                  uint64_t* id);
+/* one tick: admit, place, tick_steps steps of every running job, sample,
+ * complete; *active receives the number of running jobs */
+HALMD_API int halmd_server_tick(halmd_server* s, unsigned* active);
+HALMD_API int halmd_server_stats(const halmd_server* s, unsigned* active, unsigned* queued, size_t* used_bytes,
+                                 uint64_t* particle_steps);
+/* status 0 queued, 1 running, 2 done; times: arrival, admission, completion
+ * (server clock); energies: en_pot, en_kin per particle at completion */
+HALMD_API int halmd_server_job_info(const halmd_server* s, uint64_t id, int* status, double* times,
+                                    double* energies);
+HALMD_API int halmd_server_job_levels(const halmd_server* s, uint64_t id, unsigned* levels, unsigned* block_size,
+                                      unsigned* nshell);
+/* levels x block_size entries: lag time, mean, count (the lag-0 column is 0) */
+HALMD_API int halmd_server_job_msd(const halmd_server* s, uint64_t id, double* time, double* mean, uint64_t* count);
+/* levels x block_size x nshell entries: mean, count */
+HALMD_API int halmd_server_job_isf(const halmd_server* s, uint64_t id, double* mean, uint64_t* count);
+/* final state of a finished job submitted with keep_state (particle-ID order) */
+HALMD_API int halmd_server_job_state(const halmd_server* s, uint64_t id, double* position, double* velocity,
+                                     int* image);
+/* forgets a finished job */
+HALMD_API int halmd_server_job_release(halmd_server* s, uint64_t id);
 
 /* ---- profiling: per-module kernel timers (HIP events on the stream) ------ */
 /* module: "force", "neighbour", "binning", "integrate", "finalize", "sort",

@@ -125,3 +125,20 @@ def test_input_generators_match_oracle(oracle):
     v = velocities.maxwell_boltzmann(10000, 1.5, 1.0, seed=4)
     assert np.abs(v.mean(0)).max() < 1e-15
     assert abs(np.mean(v ** 2) - 1.5) < 0.05
+
+
+def test_serving_stream_statistics():
+    from halmd_amd.workloads import serving_stream
+    jobs = serving_stream(rate=50.0, duration=60.0, seed=7)
+    # Poisson arrivals: about rate * duration jobs, increasing times in [0, duration)
+    assert 2700 < len(jobs) < 3300
+    t = np.array([j["arrival"] for j in jobs])
+    assert np.all(np.diff(t) > 0) and t[0] >= 0 and t[-1] < 60.0
+    n = np.array([j["n"] for j in jobs])
+    assert n.min() >= 1000 and n.max() <= 200000
+    assert 15000 < np.median(n) < 25000
+    steps = np.array([j["steps"] for j in jobs])
+    assert steps.min() >= 1000 and steps.max() <= 100000
+    assert {j["potential"] for j in jobs} == {"lj", "ka", "wca"}
+    scaled = serving_stream(rate=50.0, duration=5.0, seed=7, step_scale=0.01)
+    assert all(1 <= j["steps"] <= 1000 for j in scaled)
