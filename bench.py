@@ -226,6 +226,8 @@ def serving(args):
     lat = np.array([s.info(ids[q])["done"] - (start + stream[q]["arrival"]) for q in range(len(stream))])
     wait = np.array([s.info(ids[q])["admitted"] - (start + stream[q]["arrival"]) for q in range(len(stream))])
     psteps = s.stats()["particle_steps"]
+    phases = s.phases()
+    log("tick phases (s): " + ", ".join(f"{k} {v:.2f}" for k, v in phases.items()))
     value = psteps / (end - start)
     log(f"served: {value:.4e} particle-steps/s over {end - start:.1f} s; latency p50 {np.percentile(lat, 50):.2f} s, "
         f"p99 {np.percentile(lat, 99):.2f} s; offered {offered:.3e} particle-steps/s")
@@ -293,6 +295,7 @@ def serving(args):
             "queue_wait_p50_s": float(np.percentile(wait, 50)),
             "queue_wait_p99_s": float(np.percentile(wait, 99)),
             "makespan_s": end - start,
+            "tick_phase_seconds": phases,
             "plain_per_job_particle_steps_per_s": plain,
             "plain_latency_p50_s": float(np.percentile(plat, 50)),
             "plain_latency_p99_s": float(np.percentile(plat, 99)),

@@ -894,6 +894,12 @@ HALMD_API int halmd_server_stats(const halmd_server* s, unsigned* active, unsign
         if (particle_steps) *particle_steps = x->particle_steps();
     });
 }
+HALMD_API int halmd_server_phases(const halmd_server* s, double* seconds) {
+    return guard([&] {
+        nonnull(seconds, "seconds");
+        for (int k = 0; k < 4; ++k) seconds[k] = as<server>(s)->phase_seconds[k];
+    });
+}
 HALMD_API int halmd_server_job_info(const halmd_server* s, uint64_t id, int* status, double* times,
                                     double* energies) {
     return guard([&] {

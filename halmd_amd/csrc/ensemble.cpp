@@ -33,10 +33,17 @@ unsigned bits_for(size_t v) {   // smallest b with v < 2^b
     return b;
 }
 
+// grow-only allocation with a quarter of headroom (contents are not kept)
 template <typename T>
-void d2d(T* dst, T const* src, size_t count, hipStream_t s) {
-    if (count) HIP_CHECK(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToDevice, s));
+void grow(dbuf<T>& b, size_t n) {
+    n = std::max<size_t>(n, 1);
+    if (n > b.size()) b.resize(std::max(n, b.size() + b.size() / 4));
 }
+
+template <typename T>
+uint4* u4(dbuf<T>& b) { return reinterpret_cast<uint4*>(b.get()); }
+template <typename T>
+uint4 const* cu4(dbuf<T> const& b) { return reinterpret_cast<uint4 const*>(b.get()); }
 } // namespace
 
 ensemble::ensemble() { HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)); }
@@ -192,8 +199,8 @@ void ensemble::place_all() {
         base += h.nblk * ENS_BLOCK;
         h.cell_base = 0;
         if (!h.small) {
-            h.cell_base = cells;
-            cells += h.ncells + 1;
+            h.cell_base = cells;   // multiples of 4 words: ranges move in 16-byte units
+            cells += unsigned(round_up(size_t(h.ncells) + 1, 4));
         }
         h.d.base = h.base;
         h.d.blk_base = h.blk_base;
@@ -224,7 +231,7 @@ void ensemble::upload_descs() {
         h.d.sp.lds_cap = h.lds_cap;
         d[s] = h.d;
     }
-    descs_.resize(std::max<size_t>(d.size(), 1));
+    grow(descs_, d.size());
     if (!d.empty()) descs_.upload(d.data(), d.size(), stream);
     HIP_CHECK(hipStreamSynchronize(stream));
 }
@@ -258,7 +265,7 @@ void ensemble::build_classes() {
         c.lds_records = 64;
         for (uint32_t e : c.seg) c.lds_records = std::max(c.lds_records, sys_[e >> 16]->lds_cap);
     }
-    fclass_tables_.resize(std::max<size_t>(ft.size(), 1));
+    grow(fclass_tables_, ft.size());
     if (!ft.empty()) fclass_tables_.upload(ft.data(), ft.size(), stream);
     HIP_CHECK(hipStreamSynchronize(stream));
 }
@@ -287,33 +294,51 @@ void ensemble::apply() {
     }
     place_all();
 
-    // new state arrays: survivors moved unchanged, new systems uploaded
+    // new state in the twin arrays: survivors moved unchanged (one range
+    // kernel per array), new systems uploaded; then the twins are swapped in
     size_t const N = total_;
-    dbuf<float4> pos(2 * N), vel(2 * N), force(N), r0(N);
-    dbuf<uint4> posq(N);
-    dbuf<int4> image(N);
-    dbuf<uint32_t> id(N), count(N), offsets(std::max<size_t>(cells_total_, 1));
-    dbuf<uint16_t> list(std::max<size_t>(list_words_, 1));
-    for (auto const& o : old) {
-        auto const& h = *o.h;
-        size_t const n = h.n;
-        d2d(pos.get() + 2 * size_t(h.base), pos_.get() + 2 * size_t(o.base), 2 * n, stream);
-        d2d(vel.get() + 2 * size_t(h.base), vel_.get() + 2 * size_t(o.base), 2 * n, stream);
-        d2d(force.get() + h.base, force_.get() + o.base, n, stream);
-        d2d(r0.get() + h.base, r0_.get() + o.base, n, stream);
-        d2d(posq.get() + h.base, posq_.get() + o.base, n, stream);
-        d2d(image.get() + h.base, image_.get() + o.base, n, stream);
-        d2d(id.get() + h.base, id_.get() + o.base, n, stream);
-        d2d(count.get() + h.base, count_.get() + o.base, n, stream);
-        d2d(list.get() + h.list_off, list_.get() + o.list_off, size_t(round_up(n, WAVE)) * h.cap, stream);
-        if (!h.small) d2d(offsets.get() + h.cell_base, offsets_.get() + o.cell_base, size_t(h.ncells) + 1, stream);
+    grow(pos_t_, 2 * N); grow(vel_t_, 2 * N); grow(force_t_, N); grow(r0_t_, N); grow(posq_t_, N);
+    grow(image_t_, N); grow(id_t_, N); grow(count_t_, N); grow(list_t_, list_words_); grow(offsets_t_, cells_total_);
+    {
+        std::vector<ens_move> mp, ml, mc;
+        for (auto const& o : old) {
+            auto const& h = *o.h;
+            mp.push_back({o.base, h.base, size_t(h.nblk) * ENS_BLOCK});
+            ml.push_back({o.list_off, h.list_off, size_t(round_up(h.n, WAVE)) * h.cap});
+            if (!h.small) mc.push_back({o.cell_base, h.cell_base, round_up(size_t(h.ncells) + 1, 4)});
+        }
+        std::vector<ens_move> all(mp);
+        all.insert(all.end(), ml.begin(), ml.end());
+        all.insert(all.end(), mc.begin(), mc.end());
+        grow(moves_, all.size());
+        if (!all.empty()) moves_.upload(all.data(), all.size(), stream);
+        ens_move const* M = moves_.get();
+        unsigned const np = unsigned(mp.size()), nl = unsigned(ml.size()), nc = unsigned(mc.size());
+        launch_ens_move(u4(pos_t_), cu4(pos_), M, np, 2, 1, stream);
+        launch_ens_move(u4(vel_t_), cu4(vel_), M, np, 2, 1, stream);
+        launch_ens_move(u4(force_t_), cu4(force_), M, np, 1, 1, stream);
+        launch_ens_move(u4(r0_t_), cu4(r0_), M, np, 1, 1, stream);
+        launch_ens_move(u4(posq_t_), cu4(posq_), M, np, 1, 1, stream);
+        launch_ens_move(u4(image_t_), cu4(image_), M, np, 1, 1, stream);
+        launch_ens_move(u4(id_t_), cu4(id_), M, np, 1, 4, stream);
+        launch_ens_move(u4(count_t_), cu4(count_), M, np, 1, 4, stream);
+        launch_ens_move(u4(list_t_), cu4(list_), M + np, nl, 1, 8, stream);
+        launch_ens_move(u4(offsets_t_), cu4(offsets_), M + np + nl, nc, 1, 4, stream);
+        // (moves_ is rewritten only after the synchronisation below)
     }
+    // host staging of the new systems, alive until the uploads are done
+    std::vector<std::vector<char>> staging;
+    auto stage = [&](size_t bytes) -> char* {
+        staging.emplace_back(bytes);
+        return staging.back().data();
+    };
     for (unsigned s : fresh) {
         auto& h = *sys_[s];
         size_t const n = h.n;
-        std::vector<float4> hp(2 * n), hv(2 * n);
-        std::vector<int4> him(n, make_int4(0, 0, 0, 0));
-        std::vector<uint32_t> hid(n);
+        auto* hp = reinterpret_cast<float4*>(stage(2 * n * sizeof(float4)));
+        auto* hv = reinterpret_cast<float4*>(stage(2 * n * sizeof(float4)));
+        auto* him = reinterpret_cast<int4*>(stage(n * sizeof(int4)));
+        auto* hid = reinterpret_cast<uint32_t*>(stage(n * sizeof(uint32_t)));
         for (size_t i = 0; i < n; ++i) {
             float rh[3], rl[3], vh[3], vl[3];
             for (int k = 0; k < 3; ++k) {
@@ -328,40 +353,41 @@ void ensemble::apply() {
             hp[2 * i + 1] = make_float4(rl[0], rl[1], rl[2], 0.0f);
             hv[2 * i] = make_float4(vh[0], vh[1], vh[2], 1.0f);   // unit mass
             hv[2 * i + 1] = make_float4(vl[0], vl[1], vl[2], 0.0f);
+            him[i] = make_int4(0, 0, 0, 0);
             hid[i] = unsigned(i);
         }
-        HIP_CHECK(hipMemcpyAsync(pos.get() + 2 * size_t(h.base), hp.data(), hp.size() * sizeof(float4),
+        HIP_CHECK(hipMemcpyAsync(pos_t_.get() + 2 * size_t(h.base), hp, 2 * n * sizeof(float4),
                                  hipMemcpyHostToDevice, stream));
-        HIP_CHECK(hipMemcpyAsync(vel.get() + 2 * size_t(h.base), hv.data(), hv.size() * sizeof(float4),
+        HIP_CHECK(hipMemcpyAsync(vel_t_.get() + 2 * size_t(h.base), hv, 2 * n * sizeof(float4),
                                  hipMemcpyHostToDevice, stream));
-        HIP_CHECK(hipMemcpyAsync(image.get() + h.base, him.data(), n * sizeof(int4), hipMemcpyHostToDevice, stream));
-        HIP_CHECK(hipMemcpyAsync(id.get() + h.base, hid.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
-        launch_quantize(pos.get() + 2 * size_t(h.base), h.n, h.d.ip.qp, posq.get() + h.base, stream);
-        HIP_CHECK(hipStreamSynchronize(stream));   // the host staging goes out of scope
+        HIP_CHECK(hipMemcpyAsync(image_t_.get() + h.base, him, n * sizeof(int4), hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(id_t_.get() + h.base, hid, n * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+        launch_quantize(pos_t_.get() + 2 * size_t(h.base), h.n, h.d.ip.qp, posq_t_.get() + h.base, stream);
         h.r.clear(); h.r.shrink_to_fit();
         h.v.clear(); h.v.shrink_to_fit();
         h.species.clear(); h.species.shrink_to_fit();
         h.on_device = true;
     }
-    HIP_CHECK(hipStreamSynchronize(stream));   // the old arrays are released below
-    pos_ = std::move(pos); vel_ = std::move(vel); force_ = std::move(force); r0_ = std::move(r0);
-    posq_ = std::move(posq); image_ = std::move(image); id_ = std::move(id); count_ = std::move(count);
-    offsets_ = std::move(offsets); list_ = std::move(list);
+    HIP_CHECK(hipStreamSynchronize(stream));
+    staging.clear();
+    std::swap(pos_, pos_t_); std::swap(vel_, vel_t_); std::swap(force_, force_t_); std::swap(r0_, r0_t_);
+    std::swap(posq_, posq_t_); std::swap(image_, image_t_); std::swap(id_, id_t_); std::swap(count_, count_t_);
+    std::swap(list_, list_t_); std::swap(offsets_, offsets_t_);
 
-    // scratch sized for the new layout
-    pos_t_.resize(2 * N); vel_t_.resize(2 * N); aux_force_.resize(N);
-    posq_t_.resize(N); image_t_.resize(N); id_t_.resize(N);
-    stress_.resize(6 * N);
-    keys_.resize(N); vals_.resize(N);
+    // scratch sized for the new layout (the twins serve the reorder)
+    grow(pos_t_, 2 * N); grow(vel_t_, 2 * N); grow(posq_t_, N); grow(image_t_, N); grow(id_t_, N);
+    grow(aux_force_, N); grow(stress_, 6 * N); grow(keys_, N); grow(vals_, N);
     ws_.reserve(std::max<size_t>(N, 1));
     size_t const ns = std::max<size_t>(sys_.size(), 1);
-    disp2_.resize(ns); stats_.resize(2 * ns); thermo_.resize(4 * ns);
-    h_disp2_.resize(ns); h_stats_.resize(2 * ns); h_thermo_.resize(4 * ns);
+    grow(disp2_, ns); grow(stats_, 2 * ns); grow(thermo_, 4 * ns);
+    if (h_disp2_.size() < ns) {
+        h_disp2_.resize(ns + ns / 4); h_stats_.resize(2 * (ns + ns / 4)); h_thermo_.resize(4 * (ns + ns / 4));
+    }
     {
         std::vector<uint16_t> bs(std::max<unsigned>(nblocks_, 1), 0);
         for (size_t s = 0; s < sys_.size(); ++s)
             for (unsigned b = 0; b < sys_[s]->nblk; ++b) bs[sys_[s]->blk_base + b] = uint16_t(s);
-        block_sys_.resize(bs.size());
+        grow(block_sys_, bs.size());
         block_sys_.upload(bs.data(), bs.size(), stream);
     }
     upload_descs();
@@ -370,10 +396,34 @@ void ensemble::apply() {
     pending_ = false;
     if (!fresh.empty()) {
         rebuild(fresh);
-        // the survivors' forces are recomputed from the same positions and
-        // lists by the same kernels: unchanged
-        forces(false, force_, nullptr);
+        forces_of(fresh);
     }
+}
+
+void ensemble::forces_of(std::vector<unsigned> const& R) {
+    std::vector<char> in(sys_.size(), 0);
+    for (unsigned s : R) in[s] = 1;
+    std::vector<uint32_t> t;
+    std::vector<size_t> off;
+    for (auto const& c : classes_) {
+        off.push_back(t.size());
+        for (uint32_t e : c.seg)
+            if (in[e >> 16]) t.push_back(e);
+        off.push_back(t.size());
+        for (uint32_t s : c.small)
+            if (in[s]) t.push_back(s);
+    }
+    off.push_back(t.size());
+    grow(fsub_tables_, t.size());
+    if (!t.empty()) fsub_tables_.upload(t.data(), t.size(), stream);
+    for (size_t k = 0; k < classes_.size(); ++k) {
+        auto const& c = classes_[k];
+        size_t const a = off[2 * k], b = off[2 * k + 1], e = off[2 * k + 2];
+        launch_ens_force(c.kind, c.trunc, false, c.multi, posq_, offsets_, descs_, fsub_tables_.get() + a,
+                         unsigned(b - a), threads_, c.lds_records, fsub_tables_.get() + b, unsigned(e - b),
+                         c.small_max_n, list_, count_, force_, nullptr, total_, stream);
+    }
+    HIP_CHECK(hipStreamSynchronize(stream));   // the table upload reads host memory
 }
 
 void ensemble::rebuild(std::vector<unsigned> const& R0) {
@@ -461,17 +511,19 @@ void ensemble::rebuild(std::vector<unsigned> const& R0) {
             std::vector<size_t> old_off(sys_.size());
             for (size_t s = 0; s < sys_.size(); ++s) old_off[s] = sys_[s]->list_off;
             list_words_ = layout_lists();
-            dbuf<uint16_t> moved(std::max<size_t>(list_words_, 1));
+            grow(list_t_, list_words_);
             std::vector<char> in_r(sys_.size(), 0);
             for (unsigned s : R) in_r[s] = 1;
+            std::vector<ens_move> ml;
             for (size_t s = 0; s < sys_.size(); ++s) {
                 auto const& h = *sys_[s];
-                if (!in_r[s])
-                    d2d(moved.get() + h.list_off, list_.get() + old_off[s], size_t(round_up(h.n, WAVE)) * h.cap,
-                        stream);
+                if (!in_r[s]) ml.push_back({old_off[s], h.list_off, size_t(round_up(h.n, WAVE)) * h.cap});
             }
+            grow(moves_, ml.size());
+            if (!ml.empty()) moves_.upload(ml.data(), ml.size(), stream);
+            launch_ens_move(u4(list_t_), cu4(list_), moves_.get(), unsigned(ml.size()), 1, 8, stream);
             HIP_CHECK(hipStreamSynchronize(stream));
-            list_ = std::move(moved);
+            std::swap(list_, list_t_);
             upload_descs();
             continue;
         }

@@ -76,12 +76,15 @@ private:
     unsigned total_ = 0, nblocks_ = 0, threads_ = SEG_THREADS_MAX;
     size_t list_words_ = 0, cells_total_ = 0;
 
-    dbuf<float4> pos_, vel_, force_, r0_, pos_t_, vel_t_, aux_force_;
+    // state arrays and their twins (the reorder and placement targets, then
+    // swapped in); all sized with headroom and reallocated only to grow
+    dbuf<float4> pos_, vel_, force_, r0_, pos_t_, vel_t_, force_t_, r0_t_, aux_force_;
     dbuf<uint4> posq_, posq_t_;
     dbuf<int4> image_, image_t_;
-    dbuf<uint32_t> id_, id_t_, count_, offsets_, disp2_, stats_, keys_, vals_, tables_;
+    dbuf<uint32_t> id_, id_t_, count_, count_t_, offsets_, offsets_t_, disp2_, stats_, keys_, vals_, tables_;
     dbuf<float> stress_;
-    dbuf<uint16_t> list_, block_sys_;
+    dbuf<uint16_t> list_, list_t_, block_sys_;
+    dbuf<ens_move> moves_;
     dbuf<ens_desc> descs_;
     dbuf<double> thermo_;
     hbuf<uint32_t> h_disp2_, h_stats_, h_tables_;
@@ -96,7 +99,7 @@ private:
         unsigned lds_records = 0, small_max_n = 0;
     };
     std::vector<force_class> classes_;
-    dbuf<uint32_t> fclass_tables_;
+    dbuf<uint32_t> fclass_tables_, fsub_tables_;
 
     void prepare(sys_host& h);
     void place_all();
@@ -105,6 +108,7 @@ private:
     void build_classes();
     void rebuild(std::vector<unsigned> const& R);
     void forces(bool aux, float4* force, float* stress);
+    void forces_of(std::vector<unsigned> const& R);   // plain forces of the listed systems only
     void settle();
 };
 

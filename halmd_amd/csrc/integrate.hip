@@ -328,6 +328,23 @@ void launch_ens_integrate(float4* pos, float4* vel, float4 const* force, int4* i
     HIP_CHECK(hipGetLastError());
 }
 
+// blockIdx.y selects the range, blockIdx.x strides through it
+__global__ __launch_bounds__(256) void ens_move_kernel(uint4* __restrict__ dst, uint4 const* __restrict__ src,
+                                                       ens_move const* __restrict__ moves, unsigned num,
+                                                       unsigned den) {
+    ens_move const m = moves[blockIdx.y];
+    size_t const s0 = size_t(m.src) * num / den, d0 = size_t(m.dst) * num / den, len = size_t(m.n) * num / den;
+    for (size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x; k < len; k += size_t(gridDim.x) * blockDim.x)
+        dst[d0 + k] = src[s0 + k];
+}
+
+void launch_ens_move(uint4* dst, uint4 const* src, ens_move const* moves, unsigned nmoves, unsigned num, unsigned den,
+                     hipStream_t s) {
+    if (!nmoves) return;
+    ens_move_kernel<<<dim3(32, nmoves), 256, 0, s>>>(dst, src, moves, num, den);
+    HIP_CHECK(hipGetLastError());
+}
+
 void launch_ens_settle(float4* vel, float4 const* force, ens_desc const* descs, uint16_t const* block_sys,
                        unsigned nblocks, hipStream_t s) {
     if (!nblocks) return;

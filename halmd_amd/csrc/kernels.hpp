@@ -296,6 +296,14 @@ void launch_ens_reorder(float4* pos, float4* vel, int4* image, uint32_t* id, uin
                         ens_desc const* descs, uint32_t const* rsys, uint32_t const* rpos, uint32_t const* rblk,
                         unsigned nrblk, uint32_t const* sorted_vals, hipStream_t s);
 // r0 := positions of the listed systems (all-pairs systems, no reorder)
+// moves of whole ranges between two arrays of 16-byte units: range k copies
+// units [src_k * num / den, + n_k * num / den) of src to dst from dst_k * num / den
+// (num / den: 16-byte units per element, every product an integer)
+struct ens_move {
+    unsigned long long src, dst, n;
+};
+void launch_ens_move(uint4* dst, uint4 const* src, ens_move const* moves, unsigned nmoves, unsigned num, unsigned den,
+                     hipStream_t s);
 void launch_ens_r0(float4 const* pos, float4* r0, ens_desc const* descs, uint32_t const* rblk_sys, unsigned nrblk,
                    hipStream_t s);
 // list builds: rseg (s << 16) | segment for cell systems; rsmall the

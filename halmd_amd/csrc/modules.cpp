@@ -1128,7 +1128,8 @@ wavevector::wavevector(std::vector<double> const& k, double const* L, double tol
     msm[2] += msm[1];
     std::multimap<size_t, std::array<double, 3>> mm;
     unsigned idx[3] = {0, 0, 0};
-    while (idx[2] <= msm[2]) {
+    size_t full = 0;   // shells holding max_count vectors: once all are, no triple adds any
+    while (idx[2] <= msm[2] && full < nk) {
         unsigned hkl[3] = {idx[0], idx[1] - idx[0], idx[2] - idx[1]};
         if (std::gcd(std::gcd(hkl[0], hkl[1]), hkl[2]) == 1) {
             double const r0[3] = {unit[0] * hkl[0], unit[1] * hkl[1], unit[2] * hkl[2]};
@@ -1138,7 +1139,7 @@ wavevector::wavevector(std::vector<double> const& k, double const* L, double tol
                 unsigned const nn = unsigned(std::floor(k[i] / r0n + 0.5));
                 if (nn > 0 && std::abs(nn * r0n - k[i]) <= k[i] * tolerance) {
                     mm.insert({i, {nn * r0[0], nn * r0[1], nn * r0[2]}});
-                    ++cnt[i];
+                    if (++cnt[i] == max_count) ++full;
                 }
             }
         }

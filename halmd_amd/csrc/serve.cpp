@@ -15,9 +15,9 @@ namespace halmd {
 // single-system blocking_scheme module
 // ---------------------------------------------------------------------------
 job_correlator::job_correlator(unsigned n, double length, double dt, uint64_t resolution_steps, uint64_t max_steps,
-                               unsigned bsz, unsigned shift, unsigned sep, std::vector<double> const& wavenumber,
-                               double tolerance, unsigned max_count, hipStream_t stream)
-    : timestep(dt), n_(n), block_size_(bsz), separation_(sep), s_(stream)
+                               unsigned bsz, unsigned shift, unsigned sep, std::unique_ptr<wavevector> shells,
+                               hipStream_t stream)
+    : timestep(dt), n_(n), block_size_(bsz), separation_(sep), s_(stream), wv(std::move(shells))
 {
     require(block_size_ >= 2, "correlator: minimum block size is 2");
     require(shift != 0 && shift < block_size_, "correlator: coarse-graining shift must be in 1 .. block size - 1");
@@ -35,8 +35,7 @@ job_correlator::job_correlator(unsigned n, double length, double dt, uint64_t re
     msd_partials_.resize(size_t(REDUCE_BLOCKS) * 2 * block_size_);
     msd_out_.resize(interval.size() * 2 * block_size_);
     h_msd_.resize(interval.size() * 2 * block_size_);
-    if (!wavenumber.empty()) {
-        wv = std::make_unique<wavevector>(wavenumber, L_, tolerance, max_count);
+    if (wv) {
         nq_ = unsigned(wv->q.size() / 3);
         q_.resize(std::max(3u * nq_, 3u));
         q_.upload(wv->q.data(), wv->q.size(), s_);
@@ -161,6 +160,63 @@ server::server(size_t budget_bytes, unsigned max_jobs, unsigned tick_steps)
 {
     require(max_jobs_ >= 1, "server: at least one running job");
     require(tick_ >= 1, "server: a tick is at least one step");
+    worker_ = std::thread([this] { prepare_loop(); });
+}
+
+server::~server() {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
+    }
+    cv_.notify_all();
+    worker_.join();
+}
+
+static std::unique_ptr<wavevector> make_shells(std::vector<double> const& k, double length, double tolerance,
+                                               unsigned max_count) {
+    if (k.empty()) return nullptr;
+    double const L[3] = {length, length, length};
+    return std::make_unique<wavevector>(k, L, tolerance, max_count);
+}
+
+// host-only work: the lattice search of each queued job's ISF wavevectors
+void server::prepare_loop() {
+    std::unique_lock<std::mutex> lk(mu_);
+    while (true) {
+        cv_.wait(lk, [this] { return stop_ || !prep_.empty(); });
+        if (stop_) return;
+        shell_task t = std::move(prep_.front());
+        prep_.pop_front();
+        taken_[t.id] = false;
+        lk.unlock();
+        std::unique_ptr<wavevector> w;
+        try {
+            w = make_shells(t.wavenumber, t.length, t.tolerance, t.max_count);
+        } catch (...) {
+            w = nullptr;   // admission builds them again and reports the error
+        }
+        lk.lock();
+        bool const taken = taken_[t.id];
+        taken_.erase(t.id);
+        if (!taken && w) ready_[t.id] = std::move(w);
+    }
+}
+
+std::unique_ptr<wavevector> server::take_shells(job& j) {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        auto r = ready_.find(j.id);
+        if (r != ready_.end()) {
+            auto w = std::move(r->second);
+            ready_.erase(r);
+            return w;
+        }
+        auto it = std::find_if(prep_.begin(), prep_.end(), [&](shell_task const& t) { return t.id == j.id; });
+        if (it != prep_.end()) prep_.erase(it);   // not started: built here
+        auto b = taken_.find(j.id);
+        if (b != taken_.end()) b->second = true;   // being built: the worker drops its result
+    }
+    return make_shells(j.req.wavenumber, j.req.length, j.req.tolerance, j.req.max_count);
 }
 
 double server::now() const {
@@ -193,6 +249,11 @@ uint64_t server::submit(job_request r) {
     j->req = std::move(r);
     job* p = j.get();
     jobs_.emplace(p->id, std::move(j));
+    if (!p->req.wavenumber.empty()) {
+        std::lock_guard<std::mutex> lk(mu_);
+        prep_.push_back({p->id, p->req.wavenumber, p->req.length, p->req.tolerance, p->req.max_count});
+    }
+    cv_.notify_one();
     queue_.push_back(p);
     return p->id;
 }
@@ -217,6 +278,13 @@ ensemble::view server::view_of(job const& j) const {
 }
 
 unsigned server::tick() {
+    double t_mark = now();
+    auto phase = [&](int k) {
+        HIP_CHECK(hipStreamSynchronize(ens.stream));
+        double const t = now();
+        phase_seconds[k] += t - t_mark;
+        t_mark = t;
+    };
     // admission in arrival order while the footprint fits (a job larger than
     // the whole budget runs when nothing else does)
     std::vector<job*> fresh;
@@ -224,6 +292,7 @@ unsigned server::tick() {
         job& j = *queue_.front();
         if (used_ + j.footprint > budget_ && !active_.empty()) break;
         auto& r = j.req;
+        auto shells = take_shells(j);   // first: an invalid wavenumber set fails before the job is placed
         std::vector<double> ones(size_t(r.ntype) * r.ntype, 1.0);
         unsigned const idx =
             ens.add(r.n, r.length, r.kind, r.trunc, r.ntype, r.epsilon.data(), r.sigma.data(), r.r_cut_sigma.data(),
@@ -235,7 +304,7 @@ unsigned server::tick() {
         r.species.clear(); r.species.shrink_to_fit();
         uint64_t const steps = (r.steps + tick_ - 1) / tick_ * tick_;
         j.corr = std::make_unique<job_correlator>(r.n, r.length, r.timestep, tick_, steps, r.block_size, r.shift,
-                                                  r.separation, r.wavenumber, r.tolerance, r.max_count, ens.stream);
+                                                  r.separation, std::move(shells), ens.stream);
         j.status = 1;
         j.t_admit = now();
         used_ += j.footprint;
@@ -245,8 +314,10 @@ unsigned server::tick() {
     }
     ens.apply();
     for (job* j : fresh) j->corr->sample(0, view_of(*j));
+    phase(0);
     if (!active_.empty()) {
         ens.step(tick_);
+        phase(1);
         for (job* j : active_) {
             j->step += tick_;
             psteps_ += uint64_t(tick_) * j->req.n;
@@ -255,6 +326,7 @@ unsigned server::tick() {
     }
     HIP_CHECK(hipStreamSynchronize(ens.stream));
     for (job* j : active_) j->corr->collect();
+    phase(2);
 
     // completion
     std::vector<job*> done;
@@ -289,6 +361,7 @@ unsigned server::tick() {
         active_.erase(std::remove_if(active_.begin(), active_.end(), [](job* j) { return j->status == 2; }),
                       active_.end());
     }
+    phase(3);
     return unsigned(active_.size());
 }
 

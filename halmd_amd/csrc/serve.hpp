@@ -19,8 +19,11 @@
 #include "ensemble.hpp"
 
 #include <chrono>
+#include <condition_variable>
 #include <deque>
 #include <map>
+#include <mutex>
+#include <thread>
 
 namespace halmd {
 
@@ -31,9 +34,10 @@ namespace halmd {
 // folds them into FP64 accumulators after a stream synchronisation.
 class job_correlator {
 public:
+    // shells: the ISF wavevectors (null for none)
     job_correlator(unsigned n, double length, double timestep, uint64_t resolution_steps, uint64_t max_steps,
-                   unsigned block_size, unsigned shift, unsigned separation, std::vector<double> const& wavenumber,
-                   double tolerance, unsigned max_count, hipStream_t stream);
+                   unsigned block_size, unsigned shift, unsigned separation, std::unique_ptr<wavevector> shells,
+                   hipStream_t stream);
     ~job_correlator();
     // enqueue the sample of `step` (levels due at it) from the system's arrays
     void sample(uint64_t step, ensemble::view const& v);
@@ -91,6 +95,7 @@ struct job_request {
 class server {
 public:
     server(size_t budget_bytes, unsigned max_jobs, unsigned tick_steps);
+    ~server();
     uint64_t submit(job_request r);
     // one tick: admission, placement, tick_steps steps, correlator samples,
     // completion; returns the number of running jobs
@@ -115,6 +120,9 @@ public:
     unsigned queued() const { return unsigned(queue_.size()); }
     size_t used() const { return used_; }
     uint64_t particle_steps() const { return psteps_; }
+    // host wall-clock seconds spent in the phases of all ticks: admission and
+    // placement, steps, correlator samples, completion
+    double phase_seconds[4] = {0, 0, 0, 0};
 
     ensemble ens;
 
@@ -126,6 +134,23 @@ private:
     std::deque<job*> queue_;
     std::vector<job*> active_;
     uint64_t next_id_ = 1, psteps_ = 0;
+    // host-side preparation of the ISF wavevectors of queued jobs, in
+    // submission order, beside the ticks
+    struct shell_task {
+        uint64_t id;
+        std::vector<double> wavenumber;
+        double length, tolerance;
+        unsigned max_count;
+    };
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<shell_task> prep_;                               // not started
+    std::map<uint64_t, std::unique_ptr<wavevector>> ready_;     // finished
+    std::map<uint64_t, bool> taken_;                            // admitted while being prepared
+    bool stop_ = false;
+    std::thread worker_;
+    void prepare_loop();
+    std::unique_ptr<wavevector> take_shells(job& j);
     size_t job_footprint(job_request const& r) const;
     ensemble::view view_of(job const& j) const;
 };

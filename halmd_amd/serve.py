@@ -64,6 +64,12 @@ class server(Handle):
         call("halmd_server_stats", self.handle, ctypes.byref(a), ctypes.byref(q), ctypes.byref(u), ctypes.byref(p))
         return dict(active=a.value, queued=q.value, used_bytes=u.value, particle_steps=p.value)
 
+    def phases(self):
+        """Host seconds over all ticks: admission/placement, steps, samples, completion."""
+        t = np.zeros(4)
+        call("halmd_server_phases", self.handle, dptr(t))
+        return dict(zip(("admit", "step", "sample", "complete"), t))
+
     def info(self, jid):
         st = ctypes.c_int()
         t = np.zeros(3)

@@ -399,6 +399,9 @@ HALMD_API int halmd_server_submit(halmd_server* s, unsigned n, double length, in
 HALMD_API int halmd_server_tick(halmd_server* s, unsigned* active);
 HALMD_API int halmd_server_stats(const halmd_server* s, unsigned* active, unsigned* queued, size_t* used_bytes,
                                  uint64_t* particle_steps);
+/* host seconds spent over all ticks in: admission and placement, steps,
+ * correlator samples, completion (4 doubles) */
+HALMD_API int halmd_server_phases(const halmd_server* s, double* seconds);
 /* status 0 queued, 1 running, 2 done; times: arrival, admission, completion
  * (server clock); energies: en_pot, en_kin per particle at completion */
 HALMD_API int halmd_server_job_info(const halmd_server* s, uint64_t id, int* status, double* times,
