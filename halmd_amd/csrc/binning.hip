@@ -389,24 +389,28 @@ __global__ __launch_bounds__(256) void ens_cells_kernel(uint4 const* __restrict_
     vals[c] = li;
 }
 
-// one workgroup per listed system: offset[c] = first slot with key >= c
+// offset[c] = first slot of cell c in each listed system's sorted keys: the
+// particle at slot k writes the cells after its predecessor's up to its own
+// (the last particle also those after it, with n), as cell_offsets_kernel does
 __global__ __launch_bounds__(256) void ens_offsets_kernel(ens_desc const* __restrict__ descs,
                                                           uint32_t const* __restrict__ rsys,
-                                                          uint32_t const* __restrict__ rpos, unsigned cell_bits,
+                                                          uint32_t const* __restrict__ rpos,
+                                                          uint32_t const* __restrict__ rblk, unsigned cell_bits,
                                                           uint32_t const* __restrict__ keys,
                                                           uint32_t* __restrict__ offsets) {
-    unsigned const rank = blockIdx.x;
+    uint32_t const e = rblk[blockIdx.x];
+    unsigned const rank = e >> 16, lb = e & 0xffffu;
     ens_desc const& d = descs[rsys[rank]];
+    unsigned const k = lb * 256 + threadIdx.x;
+    if (k >= d.n) return;
+    uint32_t const mask = (1u << cell_bits) - 1u;
     unsigned const lo = rpos[rank];
-    for (unsigned c = threadIdx.x; c <= d.ncells; c += blockDim.x) {
-        uint32_t const target = (rank << cell_bits) | c;
-        unsigned a = 0, b = d.n;   // lower bound in keys[lo, lo + n)
-        while (a < b) {
-            unsigned const m = (a + b) / 2;
-            if (keys[lo + m] < target) a = m + 1; else b = m;
-        }
-        offsets[d.cell_base + c] = a;
-    }
+    int const prev = k == 0 ? -1 : int(keys[lo + k - 1] & mask);
+    int const cur = int(keys[lo + k] & mask);
+    uint32_t* const off = offsets + d.cell_base;
+    for (int c = prev + 1; c <= cur; ++c) off[c] = k;
+    if (k + 1 == d.n)
+        for (int c = cur + 1; c <= int(d.ncells); ++c) off[c] = d.n;
 }
 
 __global__ __launch_bounds__(256) void ens_gather_kernel(
@@ -597,10 +601,11 @@ void launch_ens_cells(uint4 const* posq, ens_desc const* descs, uint32_t const* 
     HIP_CHECK(hipGetLastError());
 }
 
-void launch_ens_offsets(ens_desc const* descs, uint32_t const* rsys, uint32_t const* rpos, unsigned nr,
-                        unsigned cell_bits, uint32_t const* sorted_keys, uint32_t* offsets, hipStream_t s) {
-    if (!nr) return;
-    ens_offsets_kernel<<<nr, 256, 0, s>>>(descs, rsys, rpos, cell_bits, sorted_keys, offsets);
+void launch_ens_offsets(ens_desc const* descs, uint32_t const* rsys, uint32_t const* rpos, uint32_t const* rblk,
+                        unsigned nrblk, unsigned cell_bits, uint32_t const* sorted_keys, uint32_t* offsets,
+                        hipStream_t s) {
+    if (!nrblk) return;
+    ens_offsets_kernel<<<nrblk, 256, 0, s>>>(descs, rsys, rpos, rblk, cell_bits, sorted_keys, offsets);
     HIP_CHECK(hipGetLastError());
 }
 

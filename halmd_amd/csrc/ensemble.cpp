@@ -379,7 +379,7 @@ void ensemble::apply() {
     grow(aux_force_, N); grow(stress_, 6 * N); grow(keys_, N); grow(vals_, N);
     ws_.reserve(std::max<size_t>(N, 1));
     size_t const ns = std::max<size_t>(sys_.size(), 1);
-    grow(disp2_, ns); grow(stats_, 2 * ns); grow(thermo_, 4 * ns);
+    grow(disp2_, ns * DISP_STRIDE); grow(stats_, 2 * ns); grow(thermo_, 4 * ns);
     if (h_disp2_.size() < ns) {
         h_disp2_.resize(ns + ns / 4); h_stats_.resize(2 * (ns + ns / 4)); h_thermo_.resize(4 * (ns + ns / 4));
     }
@@ -475,8 +475,8 @@ void ensemble::rebuild(std::vector<unsigned> const& R0) {
             launch_ens_cells(posq_, descs_, T + o_rsys, T + o_rpos, T + o_rblk, unsigned(rblk.size()), cell_bits,
                              keys_, vals_, stream);
             radix_sort(keys_, vals_, npos, std::max(1u, key_bits), ws_, stream);
-            launch_ens_offsets(descs_, T + o_rsys, T + o_rpos, unsigned(rsys.size()), cell_bits, keys_, offsets_,
-                               stream);
+            launch_ens_offsets(descs_, T + o_rsys, T + o_rpos, T + o_rblk, unsigned(rblk.size()), cell_bits, keys_,
+                               offsets_, stream);
             launch_ens_reorder(pos_, vel_, image_, id_, posq_, r0_, pos_t_, vel_t_, image_t_, id_t_, posq_t_, descs_,
                                T + o_rsys, T + o_rpos, T + o_rblk, unsigned(rblk.size()), vals_, stream);
             launch_ens_build(posq_, offsets_, descs_, T + o_rseg, unsigned(rseg.size()), threads_, lds, multi, list_,
@@ -564,11 +564,13 @@ void ensemble::step(unsigned steps) {
     }
     std::vector<unsigned> R;
     for (unsigned k = 0; k < steps; ++k) {
-        launch_fill_u32(disp2_, 0u, sys_.size(), stream);
+        // one word per system, a cache line apart
+        launch_fill_u32(disp2_, 0u, sys_.size() * DISP_STRIDE, stream);
         launch_ens_integrate(pos_, vel_, force_, image_, posq_, r0_, descs_, block_sys_, nblocks_, pending_, disp2_,
                              stream);
         pending_ = false;
-        disp2_.download(h_disp2_.get(), sys_.size(), stream);
+        HIP_CHECK(hipMemcpy2DAsync(h_disp2_.get(), sizeof(uint32_t), disp2_.get(), DISP_STRIDE * sizeof(uint32_t),
+                                   sizeof(uint32_t), sys_.size(), hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
         R.clear();
         for (size_t s = 0; s < sys_.size(); ++s) {

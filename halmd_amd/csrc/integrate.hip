@@ -246,7 +246,7 @@ __global__ __launch_bounds__(ENS_BLOCK) void ens_integrate_kernel(
     if (li < d.n)
         d2 = integrate_particle(pos, vel, force, image, posq, d.base + li, d.ip, pending ? d.ip.half_dt : 0.0f,
                                 nullptr, r0);
-    wave_max_to(d2, disp2 + s);
+    wave_max_to(d2, disp2 + size_t(s) * DISP_STRIDE);
 }
 
 // apply the deferred finalize kick of every system (before velocities are read)

@@ -275,7 +275,8 @@ struct ens_desc {
     force_params P;
 };
 // step part 1: deferred kick (if pending), kick, drift, snapshot; per-system
-// maximum displacement (float bits) into disp2[s]
+// maximum displacement (float bits) into disp2[s * DISP_STRIDE] (one cache
+// line per system)
 void launch_ens_integrate(float4* pos, float4* vel, float4 const* force, int4* image, uint4* posq,
                           float4 const* r0, ens_desc const* descs, uint16_t const* block_sys, unsigned nblocks,
                           bool pending, uint32_t* disp2, hipStream_t s);
@@ -287,8 +288,9 @@ void launch_ens_settle(float4* vel, float4 const* force, ens_desc const* descs, 
 void launch_ens_cells(uint4 const* posq, ens_desc const* descs, uint32_t const* rsys, uint32_t const* rpos,
                       uint32_t const* rblk, unsigned nrblk, unsigned cell_bits, uint32_t* keys, uint32_t* vals,
                       hipStream_t s);
-void launch_ens_offsets(ens_desc const* descs, uint32_t const* rsys, uint32_t const* rpos, unsigned nr,
-                        unsigned cell_bits, uint32_t const* sorted_keys, uint32_t* offsets, hipStream_t s);
+void launch_ens_offsets(ens_desc const* descs, uint32_t const* rsys, uint32_t const* rpos, uint32_t const* rblk,
+                        unsigned nrblk, unsigned cell_bits, uint32_t const* sorted_keys, uint32_t* offsets,
+                        hipStream_t s);
 // reorder each listed system into its cell order (sorted vals are local
 // indices in the old order); r0 := new positions (hi words)
 void launch_ens_reorder(float4* pos, float4* vel, int4* image, uint32_t* id, uint4* posq, float4* r0,
@@ -378,5 +380,13 @@ void launch_sample_position(float4 const* pos, int4 const* image, uint32_t const
 // written to partials[block * nq + qoff + {0, 1}]
 void launch_msd_partials(double const* r1, double const* r2, unsigned n, double* partials, unsigned nq, unsigned qoff,
                          hipStream_t s);
+// the pairs (r1, r2[l]), l < nl, into partials[block][qoff + 2 l + {0, 1}] with
+// one launch per MSD_LAGS pairs; each pair sums exactly as launch_msd_partials
+constexpr unsigned MSD_LAGS = 32;
+struct msd_samples {
+    double const* p[MSD_LAGS];
+};
+void launch_msd_partials_lags(double const* r1, double const* const* r2, unsigned nl, unsigned n, double* partials,
+                              unsigned nq, unsigned qoff, hipStream_t s);
 
 } // namespace halmd

@@ -13,6 +13,8 @@
 // order.
 #include "kernels.hpp"
 
+#include <algorithm>
+
 namespace halmd {
 namespace {
 
@@ -99,11 +101,15 @@ __global__ void sample_position_kernel(float4 const* __restrict__ pos, int4 cons
     out[o + 2] = double(hi.z) + double(lo.z) + double(im.z) * Lz;
 }
 
+// block row y: the pair (r1, r2.p[y]) into partials[block][qoff + 2 y + {0, 1}];
+// every row runs the same grid-stride order as a one-pair launch
 __global__ __launch_bounds__(REDUCE_THREADS) void msd_partials_kernel(double const* __restrict__ r1,
-                                                                      double const* __restrict__ r2, unsigned n,
+                                                                      msd_samples r2s, unsigned n,
                                                                       double* __restrict__ partials, unsigned nq,
                                                                       unsigned qoff) {
     __shared__ double s_red[REDUCE_THREADS / WAVE][2];
+    double const* __restrict__ r2 = r2s.p[blockIdx.y];
+    qoff += 2 * blockIdx.y;
     double a = 0, b = 0;
     for (unsigned i = blockIdx.x * REDUCE_THREADS + threadIdx.x; i < n; i += REDUCE_BLOCKS * REDUCE_THREADS) {
         double const dx = r2[3 * size_t(i)] - r1[3 * size_t(i)];
@@ -146,8 +152,18 @@ void launch_sample_position(float4 const* pos, int4 const* image, uint32_t const
 
 void launch_msd_partials(double const* r1, double const* r2, unsigned n, double* partials, unsigned nq, unsigned qoff,
                          hipStream_t s) {
-    msd_partials_kernel<<<REDUCE_BLOCKS, REDUCE_THREADS, 0, s>>>(r1, r2, n, partials, nq, qoff);
-    HIP_CHECK(hipGetLastError());
+    launch_msd_partials_lags(r1, &r2, 1, n, partials, nq, qoff, s);
+}
+
+void launch_msd_partials_lags(double const* r1, double const* const* r2, unsigned nl, unsigned n, double* partials,
+                              unsigned nq, unsigned qoff, hipStream_t s) {
+    for (unsigned l0 = 0; l0 < nl; l0 += MSD_LAGS) {
+        unsigned const m = std::min(MSD_LAGS, nl - l0);
+        msd_samples t{};
+        for (unsigned l = 0; l < m; ++l) t.p[l] = r2[l0 + l];
+        msd_partials_kernel<<<dim3(REDUCE_BLOCKS, m), REDUCE_THREADS, 0, s>>>(r1, t, n, partials, nq, qoff + 2 * l0);
+        HIP_CHECK(hipGetLastError());
+    }
 }
 
 } // namespace halmd

@@ -40,7 +40,7 @@ job_correlator::job_correlator(unsigned n, double length, double dt, uint64_t re
         q_.resize(std::max(3u * nq_, 3u));
         q_.upload(wv->q.data(), wv->q.size(), s_);
         dm_partials_.resize(size_t(div_up(nq_, DM_KCHUNK)) * DM_BLOCKS * 2 * DM_KCHUNK);
-        h_rho_.resize(interval.size() * block_size_ * 2 * size_t(std::max(nq_, 1u)));
+        dm_out_.resize(2 * size_t(nq_));
         isf.resize(interval.size() * block_size_ * wv->shell.size());
         HIP_CHECK(hipStreamSynchronize(s_));
     }
@@ -51,32 +51,46 @@ job_correlator::~job_correlator() {
     (void)hipStreamSynchronize(s_);
 }
 
-job_correlator::sample_ptr job_correlator::take(std::vector<std::unique_ptr<dbuf<double>>>& pool, size_t size) {
+job_correlator::pos_ptr job_correlator::take_pos() {
     dbuf<double>* b;
-    if (pool.empty()) {
-        b = new dbuf<double>(size);
+    if (pos_pool_.empty()) {
+        b = new dbuf<double>(3 * size_t(n_));
     } else {
-        b = pool.back().release();
-        pool.pop_back();
+        b = pos_pool_.back().release();
+        pos_pool_.pop_back();
     }
-    return sample_ptr(b, [&pool](dbuf<double>* p) { pool.emplace_back(p); });
+    return pos_ptr(b, [this](dbuf<double>* p) { pos_pool_.emplace_back(p); });
+}
+
+job_correlator::rho_ptr job_correlator::take_rho() {
+    hbuf<double>* b;
+    if (rho_pool_.empty()) {
+        b = new hbuf<double>(2 * size_t(nq_));
+    } else {
+        b = rho_pool_.back().release();
+        rho_pool_.pop_back();
+    }
+    return rho_ptr(b, [this](hbuf<double>* p) { rho_pool_.emplace_back(p); });
 }
 
 void job_correlator::sample(uint64_t step, ensemble::view const& v) {
     require(v.n == n_, "correlator: particle count changed");
-    sample_ptr ps, rs;
+    pos_ptr ps;
+    rho_ptr rs;
     for (unsigned i = 0; i < interval.size(); ++i) {
         if (step < origin[i] || (step - origin[i]) % interval[i] != 0) continue;
         if (!ps) {
-            ps = take(pos_pool_, 3 * size_t(n_));
+            ps = take_pos();
             launch_sample_position(v.pos, v.image, v.id, n_, L_, *ps, s_);
         }
         pos_blocks_[i].push_back(ps);
         if (pos_blocks_[i].size() > block_size_) pos_blocks_[i].erase(pos_blocks_[i].begin());
         if (nq_) {
             if (!rs) {
-                rs = take(rho_pool_, 2 * size_t(nq_));
-                launch_density_mode(v.pos, n_, q_, nq_, dm_partials_, *rs, s_);
+                rs = take_rho();
+                launch_density_mode(v.pos, n_, q_, nq_, dm_partials_, dm_out_, s_);
+                HIP_CHECK(hipMemcpyAsync(rs->get(), dm_out_.get(), 2 * nq_ * sizeof(double), hipMemcpyDeviceToHost,
+                                         s_));
             }
             rho_blocks_[i].push_back(rs);
             if (rho_blocks_[i].size() > block_size_) rho_blocks_[i].erase(rho_blocks_[i].begin());
@@ -90,20 +104,15 @@ void job_correlator::process(unsigned level) {
     unsigned const nl = unsigned(blk.size());
     if (!nl) return;
     size_t const mo = size_t(level) * 2 * block_size_;
-    for (unsigned l = 0; l < nl; ++l)
-        launch_msd_partials(*blk.front(), *blk[l], n_, msd_partials_, 2 * nl, 2 * l, s_);
+    std::vector<double const*> r2(nl);
+    for (unsigned l = 0; l < nl; ++l) r2[l] = blk[l]->get();
+    launch_msd_partials_lags(*blk.front(), r2.data(), nl, n_, msd_partials_, 2 * nl, 0, s_);
     launch_reduce_final(msd_partials_, 2 * nl, msd_out_.get() + mo, s_);
     HIP_CHECK(hipMemcpyAsync(h_msd_.get() + mo, msd_out_.get() + mo, 2 * nl * sizeof(double), hipMemcpyDeviceToHost,
                              s_));
-    if (nq_) {
-        auto const& rb = rho_blocks_[level];
-        for (unsigned l = 0; l < nl; ++l) {
-            size_t const ro = (size_t(level) * block_size_ + l) * 2 * nq_;
-            HIP_CHECK(hipMemcpyAsync(h_rho_.get() + ro, rb[l]->get(), 2 * nq_ * sizeof(double),
-                                     hipMemcpyDeviceToHost, s_));
-        }
-    }
-    pending_.emplace_back(level, nl);
+    processed pr{level, nl, {}};
+    if (nq_) pr.rho.assign(rho_blocks_[level].begin(), rho_blocks_[level].end());
+    pending_.push_back(std::move(pr));
     uint64_t const iv = interval[level];
     uint64_t const skip = std::max<uint64_t>((separation_ + iv - 1) / iv, 1);
     origin[level] += skip * iv;
@@ -125,26 +134,26 @@ bool job_correlator::flush() {
 }
 
 void job_correlator::collect() {
-    for (auto const& [level, nl] : pending_) {
-        size_t const mo = size_t(level) * 2 * block_size_;
-        for (unsigned l = 0; l < nl; ++l) {
+    for (auto const& pr : pending_) {
+        size_t const mo = size_t(pr.level) * 2 * block_size_;
+        for (unsigned l = 0; l < pr.nl; ++l) {
             double const s1 = h_msd_[mo + 2 * l], s2 = h_msd_[mo + 2 * l + 1];
             accumulator a;
             a.n = n_;
             a.m = s1 / n_;
             a.v = std::max(0.0, s2 - s1 * s1 / n_);
-            msd[size_t(level) * block_size_ + l].merge(a);
+            msd[size_t(pr.level) * block_size_ + l].merge(a);
         }
         if (nq_) {
             size_t const nshell = wv->shell.size();
-            double const* r1 = h_rho_.get() + size_t(level) * block_size_ * 2 * nq_;
-            for (unsigned l = 0; l < nl; ++l) {
-                double const* r2 = h_rho_.get() + (size_t(level) * block_size_ + l) * 2 * nq_;
+            double const* r1 = pr.rho.front()->get();
+            for (unsigned l = 0; l < pr.nl; ++l) {
+                double const* r2 = pr.rho[l]->get();
                 for (size_t sh = 0; sh < nshell; ++sh) {
                     accumulator a;
                     for (size_t k = wv->shell[sh].first; k < wv->shell[sh].second; ++k)
                         a((r1[2 * k] * r2[2 * k] + r1[2 * k + 1] * r2[2 * k + 1]) / n_);
-                    isf[(size_t(level) * block_size_ + l) * nshell + sh].merge(a);
+                    isf[(size_t(pr.level) * block_size_ + l) * nshell + sh].merge(a);
                 }
             }
         }

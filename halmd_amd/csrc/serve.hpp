@@ -28,10 +28,11 @@
 namespace halmd {
 
 // Blocking-scheme correlator of one job whose samples come from its range of
-// the ensemble arrays: position samples (FP64, particle-ID order) and density
-// modes stay on the device in a pool owned by the correlator; processing a
-// level enqueues the MSD reductions and the density-mode copies, and collect()
-// folds them into FP64 accumulators after a stream synchronisation.
+// the ensemble arrays: position samples (FP64, particle-ID order) stay on the
+// device in a pool owned by the correlator; each density-mode sample (2 FP64
+// per wavevector) is copied once to a pinned host buffer.  Processing a level
+// enqueues one MSD reduction over all its lags; collect() folds the results
+// into FP64 accumulators after a stream synchronisation.
 class job_correlator {
 public:
     // shells: the ISF wavevectors (null for none)
@@ -56,20 +57,29 @@ public:
     double timestep;
 
 private:
-    using sample_ptr = std::shared_ptr<dbuf<double>>;
+    using pos_ptr = std::shared_ptr<dbuf<double>>;
+    using rho_ptr = std::shared_ptr<hbuf<double>>;
     unsigned n_, block_size_, separation_, nq_ = 0;
     double L_[3];
     hipStream_t s_;
     std::vector<uint64_t> origin;
     std::unique_ptr<wavevector> wv;
-    dbuf<double> q_, dm_partials_, msd_partials_, msd_out_;
-    hbuf<double> h_msd_, h_rho_;
-    // pools of sample buffers: a buffer returns to its pool when no block
-    // holds it, and a later sample may reuse it on the same stream
-    std::vector<std::unique_ptr<dbuf<double>>> pos_pool_, rho_pool_;
-    std::vector<std::vector<sample_ptr>> pos_blocks_, rho_blocks_;
-    std::vector<std::pair<unsigned, unsigned>> pending_;   // (level, samples) processed since collect()
-    sample_ptr take(std::vector<std::unique_ptr<dbuf<double>>>& pool, size_t size);
+    dbuf<double> q_, dm_partials_, dm_out_, msd_partials_, msd_out_;
+    hbuf<double> h_msd_;
+    // pools of sample buffers: a buffer returns to its pool when nothing holds
+    // it; a later sample may reuse a device buffer on the same stream, and a
+    // host buffer only after collect() released the work that reads it
+    std::vector<std::unique_ptr<dbuf<double>>> pos_pool_;
+    std::vector<std::unique_ptr<hbuf<double>>> rho_pool_;
+    std::vector<std::vector<pos_ptr>> pos_blocks_;
+    std::vector<std::vector<rho_ptr>> rho_blocks_;
+    struct processed {
+        unsigned level, nl;
+        std::vector<rho_ptr> rho;   // the level's density modes, held until collect()
+    };
+    std::vector<processed> pending_;
+    pos_ptr take_pos();
+    rho_ptr take_rho();
     void process(unsigned level);
 };
 
