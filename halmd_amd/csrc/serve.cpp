@@ -16,8 +16,9 @@ namespace halmd {
 // ---------------------------------------------------------------------------
 job_correlator::job_correlator(unsigned n, double length, double dt, uint64_t resolution_steps, uint64_t max_steps,
                                unsigned bsz, unsigned shift, unsigned sep, std::unique_ptr<wavevector> shells,
-                               hipStream_t stream)
-    : timestep(dt), n_(n), block_size_(bsz), separation_(sep), s_(stream), wv(std::move(shells))
+                               hipStream_t stream, device_cache& dcache, host_cache& hcache)
+    : timestep(dt), n_(n), block_size_(bsz), separation_(sep), s_(stream), dcache_(dcache), hcache_(hcache),
+      wv(std::move(shells))
 {
     require(block_size_ >= 2, "correlator: minimum block size is 2");
     require(shift != 0 && shift < block_size_, "correlator: coarse-graining shift must be in 1 .. block size - 1");
@@ -32,29 +33,38 @@ job_correlator::job_correlator(unsigned n, double length, double dt, uint64_t re
     pos_blocks_.resize(interval.size());
     rho_blocks_.resize(interval.size());
     msd.resize(interval.size() * block_size_);
-    msd_partials_.resize(size_t(REDUCE_BLOCKS) * 2 * block_size_);
-    msd_out_.resize(interval.size() * 2 * block_size_);
-    h_msd_.resize(interval.size() * 2 * block_size_);
+    msd_partials_ = dcache_.take(size_t(REDUCE_BLOCKS) * 2 * block_size_);
+    msd_out_ = dcache_.take(interval.size() * 2 * block_size_);
+    h_msd_ = hcache_.take(interval.size() * 2 * block_size_);
     if (wv) {
         nq_ = unsigned(wv->q.size() / 3);
-        q_.resize(std::max(3u * nq_, 3u));
-        q_.upload(wv->q.data(), wv->q.size(), s_);
-        dm_partials_.resize(size_t(div_up(nq_, DM_KCHUNK)) * DM_BLOCKS * 2 * DM_KCHUNK);
-        dm_out_.resize(2 * size_t(nq_));
+        q_ = dcache_.take(std::max(3u * nq_, 3u));
+        q_->upload(wv->q.data(), wv->q.size(), s_);
+        dm_partials_ = dcache_.take(size_t(div_up(nq_, DM_KCHUNK)) * DM_BLOCKS * 2 * DM_KCHUNK);
+        dm_out_ = dcache_.take(2 * size_t(nq_));
         isf.resize(interval.size() * block_size_ * wv->shell.size());
         HIP_CHECK(hipStreamSynchronize(s_));
     }
 }
 
 job_correlator::~job_correlator() {
-    // samples may still be read by enqueued work
+    // samples may still be read by enqueued work; then every buffer goes back
     (void)hipStreamSynchronize(s_);
+    pending_.clear();
+    pos_blocks_.clear();
+    rho_blocks_.clear();
+    for (auto& b : pos_pool_) dcache_.give(std::move(b));
+    for (auto& b : rho_pool_) hcache_.give(std::move(b));
+    pos_pool_.clear();
+    rho_pool_.clear();
+    for (auto* b : {&q_, &dm_partials_, &dm_out_, &msd_partials_, &msd_out_}) dcache_.give(std::move(*b));
+    hcache_.give(std::move(h_msd_));
 }
 
 job_correlator::pos_ptr job_correlator::take_pos() {
     dbuf<double>* b;
     if (pos_pool_.empty()) {
-        b = new dbuf<double>(3 * size_t(n_));
+        b = dcache_.take(3 * size_t(n_)).release();
     } else {
         b = pos_pool_.back().release();
         pos_pool_.pop_back();
@@ -65,7 +75,7 @@ job_correlator::pos_ptr job_correlator::take_pos() {
 job_correlator::rho_ptr job_correlator::take_rho() {
     hbuf<double>* b;
     if (rho_pool_.empty()) {
-        b = new hbuf<double>(2 * size_t(nq_));
+        b = hcache_.take(2 * size_t(nq_)).release();
     } else {
         b = rho_pool_.back().release();
         rho_pool_.pop_back();
@@ -88,8 +98,8 @@ void job_correlator::sample(uint64_t step, ensemble::view const& v) {
         if (nq_) {
             if (!rs) {
                 rs = take_rho();
-                launch_density_mode(v.pos, n_, q_, nq_, dm_partials_, dm_out_, s_);
-                HIP_CHECK(hipMemcpyAsync(rs->get(), dm_out_.get(), 2 * nq_ * sizeof(double), hipMemcpyDeviceToHost,
+                launch_density_mode(v.pos, n_, *q_, nq_, *dm_partials_, *dm_out_, s_);
+                HIP_CHECK(hipMemcpyAsync(rs->get(), dm_out_->get(), 2 * nq_ * sizeof(double), hipMemcpyDeviceToHost,
                                          s_));
             }
             rho_blocks_[i].push_back(rs);
@@ -106,9 +116,9 @@ void job_correlator::process(unsigned level) {
     size_t const mo = size_t(level) * 2 * block_size_;
     std::vector<double const*> r2(nl);
     for (unsigned l = 0; l < nl; ++l) r2[l] = blk[l]->get();
-    launch_msd_partials_lags(*blk.front(), r2.data(), nl, n_, msd_partials_, 2 * nl, 0, s_);
-    launch_reduce_final(msd_partials_, 2 * nl, msd_out_.get() + mo, s_);
-    HIP_CHECK(hipMemcpyAsync(h_msd_.get() + mo, msd_out_.get() + mo, 2 * nl * sizeof(double), hipMemcpyDeviceToHost,
+    launch_msd_partials_lags(*blk.front(), r2.data(), nl, n_, *msd_partials_, 2 * nl, 0, s_);
+    launch_reduce_final(*msd_partials_, 2 * nl, msd_out_->get() + mo, s_);
+    HIP_CHECK(hipMemcpyAsync(h_msd_->get() + mo, msd_out_->get() + mo, 2 * nl * sizeof(double), hipMemcpyDeviceToHost,
                              s_));
     processed pr{level, nl, {}};
     if (nq_) pr.rho.assign(rho_blocks_[level].begin(), rho_blocks_[level].end());
@@ -137,7 +147,7 @@ void job_correlator::collect() {
     for (auto const& pr : pending_) {
         size_t const mo = size_t(pr.level) * 2 * block_size_;
         for (unsigned l = 0; l < pr.nl; ++l) {
-            double const s1 = h_msd_[mo + 2 * l], s2 = h_msd_[mo + 2 * l + 1];
+            double const s1 = (*h_msd_)[mo + 2 * l], s2 = (*h_msd_)[mo + 2 * l + 1];
             accumulator a;
             a.n = n_;
             a.m = s1 / n_;
@@ -313,7 +323,7 @@ unsigned server::tick() {
         r.species.clear(); r.species.shrink_to_fit();
         uint64_t const steps = (r.steps + tick_ - 1) / tick_ * tick_;
         j.corr = std::make_unique<job_correlator>(r.n, r.length, r.timestep, tick_, steps, r.block_size, r.shift,
-                                                  r.separation, std::move(shells), ens.stream);
+                                                  r.separation, std::move(shells), ens.stream, dcache_, hcache_);
         j.status = 1;
         j.t_admit = now();
         used_ += j.footprint;

@@ -27,6 +27,30 @@
 
 namespace halmd {
 
+// Buffers released by finished jobs, reused by later ones (device and pinned
+// host memory, by element count: a request takes the smallest free buffer
+// that holds it, if that buffer is at most twice the request)
+template <class B>
+class buffer_cache {
+public:
+    std::unique_ptr<B> take(size_t n) {
+        auto it = free_.lower_bound(n);
+        if (it != free_.end() && it->first <= 2 * n) {
+            auto b = std::move(it->second);
+            free_.erase(it);
+            return b;
+        }
+        return std::make_unique<B>(n);
+    }
+    void give(std::unique_ptr<B> b) {
+        if (b && b->size()) free_.emplace(b->size(), std::move(b));
+    }
+private:
+    std::multimap<size_t, std::unique_ptr<B>> free_;
+};
+using device_cache = buffer_cache<dbuf<double>>;
+using host_cache = buffer_cache<hbuf<double>>;
+
 // Blocking-scheme correlator of one job whose samples come from its range of
 // the ensemble arrays: position samples (FP64, particle-ID order) stay on the
 // device in a pool owned by the correlator; each density-mode sample (2 FP64
@@ -35,10 +59,11 @@ namespace halmd {
 // into FP64 accumulators after a stream synchronisation.
 class job_correlator {
 public:
-    // shells: the ISF wavevectors (null for none)
+    // shells: the ISF wavevectors (null for none); buffers come from and
+    // return to the caches (which outlive the correlator)
     job_correlator(unsigned n, double length, double timestep, uint64_t resolution_steps, uint64_t max_steps,
                    unsigned block_size, unsigned shift, unsigned separation, std::unique_ptr<wavevector> shells,
-                   hipStream_t stream);
+                   hipStream_t stream, device_cache& dcache, host_cache& hcache);
     ~job_correlator();
     // enqueue the sample of `step` (levels due at it) from the system's arrays
     void sample(uint64_t step, ensemble::view const& v);
@@ -62,10 +87,12 @@ private:
     unsigned n_, block_size_, separation_, nq_ = 0;
     double L_[3];
     hipStream_t s_;
+    device_cache& dcache_;
+    host_cache& hcache_;
     std::vector<uint64_t> origin;
     std::unique_ptr<wavevector> wv;
-    dbuf<double> q_, dm_partials_, dm_out_, msd_partials_, msd_out_;
-    hbuf<double> h_msd_;
+    std::unique_ptr<dbuf<double>> q_, dm_partials_, dm_out_, msd_partials_, msd_out_;
+    std::unique_ptr<hbuf<double>> h_msd_;
     // pools of sample buffers: a buffer returns to its pool when nothing holds
     // it; a later sample may reuse a device buffer on the same stream, and a
     // host buffer only after collect() released the work that reads it
@@ -137,6 +164,8 @@ public:
     ensemble ens;
 
 private:
+    device_cache dcache_;   // (declared before the jobs: their correlators return buffers here)
+    host_cache hcache_;
     size_t budget_, used_ = 0;
     unsigned max_jobs_, tick_;
     std::chrono::steady_clock::time_point t0_;
