@@ -244,14 +244,19 @@ __global__ __launch_bounds__(256) void build_all_pairs_kernel(uint4 const* __res
 struct nb16_state {
     uint4 ri;
     unsigned own, ti, c, cap;
-    uint16_t* out;       // entry 0 of this particle (lane block base)
+    uint16_t* list;      // the system's lists (uniform over the workgroup)
+    unsigned base;       // element index of entry 0 of this particle (lane block)
 };
 
 // 2-byte stores straight to the blocked list: a lane fills one 32-byte block
 // at a time and a wave's blocks share 2 KiB, so the lines are complete in L2
-// before they are written back (no staging buffer, hence more workgroups per CU)
+// before they are written back (no staging buffer, hence more workgroups per
+// CU).  The address is a 32-bit element index from the uniform base.  Entries
+// past the capacity overwrite the particle's last slot instead of branching:
+// the count still exceeds the capacity, and the caller rebuilds larger lists.
 __device__ __forceinline__ void append16(nb16_state& s, unsigned entry) {
-    if (s.c < s.cap) s.out[size_t(s.c / NB16_BLOCK) * (WAVE * NB16_BLOCK) + s.c % NB16_BLOCK] = uint16_t(entry);
+    unsigned const c = min(s.c, s.cap - 1u);
+    s.list[s.base + (c / NB16_BLOCK) * (WAVE * NB16_BLOCK) + c % NB16_BLOCK] = uint16_t(entry);
     ++s.c;
 }
 
@@ -324,7 +329,8 @@ __device__ __forceinline__ void staged_build_segment(
         s.ti = (s.ri.w & 3u) * p.ntype;
         s.c = 0;
         s.cap = cap;
-        s.out = list + size_t(i & ~(WAVE - 1)) * cap + (i & (WAVE - 1)) * NB16_BLOCK;
+        s.list = list;
+        s.base = (i & ~(WAVE - 1)) * cap + (i & (WAVE - 1)) * NB16_BLOCK;
         unsigned const k = cell_component(s.ri.x, nx) - st.q.x0;   // 0 .. w - 1
         // the 9 rows (dz, dy) in lexicographic order; only the own row holds
         // the particle itself
