@@ -635,6 +635,12 @@ HALMD_API int halmd_observables_dynamics_blocking_scheme_add_isf(halmd_blocking_
         bs->dm = dm;
     });
 }
+HALMD_API int halmd_observables_dynamics_blocking_scheme_add_mqd(halmd_blocking_scheme* b) {
+    return guard([&] { as<blocking_scheme>(b)->mqd_enabled = true; });
+}
+HALMD_API int halmd_observables_dynamics_blocking_scheme_add_vacf(halmd_blocking_scheme* b) {
+    return guard([&] { as<blocking_scheme>(b)->vacf_enabled = true; });
+}
 HALMD_API int halmd_observables_dynamics_blocking_scheme_sample(halmd_blocking_scheme* b, uint64_t step) {
     return guard([&] { as<blocking_scheme>(b)->sample(step); });
 }
@@ -660,8 +666,8 @@ HALMD_API int halmd_observables_dynamics_blocking_scheme_result(const halmd_bloc
         std::string const w(which);
         std::vector<accumulator> const* v = nullptr;
         size_t size = 0;
-        if (w == "msd") {
-            v = &bs->msd;
+        if (w == "msd" || w == "mqd" || w == "vacf") {
+            v = w == "msd" ? &bs->msd : w == "mqd" ? &bs->mqd : &bs->vacf;
             size = bs->interval.size() * bs->block_size;
         } else if (w == "isf") {
             require(bs->dm != nullptr, "blocking_scheme: no ISF correlation");

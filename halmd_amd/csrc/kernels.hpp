@@ -388,5 +388,13 @@ struct msd_samples {
 };
 void launch_msd_partials_lags(double const* r1, double const* const* r2, unsigned nl, unsigned n, double* partials,
                               unsigned nq, unsigned qoff, hipStream_t s);
+// the same for the time correlations (halmd/observables/dynamics/*.hpp) of
+// position samples, mean square (TCF_MSD) and quartic (TCF_MQD) displacement,
+// and of velocity samples, the velocity autocorrelation (TCF_VACF)
+enum tcf_kind : int { TCF_MSD = 0, TCF_MQD = 1, TCF_VACF = 2 };
+void launch_tcf_partials_lags(int kind, double const* r1, double const* const* r2, unsigned nl, unsigned n,
+                              double* partials, unsigned nq, unsigned qoff, hipStream_t s);
+// velocity sample in particle-ID order: out[3 id + k] = v_hi + v_lo (FP64)
+void launch_sample_velocity(float4 const* vel, uint32_t const* id, unsigned n, double* out, hipStream_t s);
 
 } // namespace halmd

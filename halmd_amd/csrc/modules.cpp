@@ -1254,9 +1254,26 @@ blocking_scheme::blocking_scheme(particle& p_, box const& b_, double timestep, d
     pos_blocks.resize(interval.size());
     rho_blocks.resize(interval.size());
     msd.resize(interval.size() * block_size);
-    msd_partials.resize(size_t(REDUCE_BLOCKS) * 2 * block_size);
-    msd_out.resize(2 * block_size);
-    h_msd.resize(2 * block_size);
+    mqd.resize(interval.size() * block_size);
+    vacf.resize(interval.size() * block_size);
+    vel_blocks.resize(interval.size());
+    // up to three correlations (MSD, MQD, VACF) of block_size lags in one reduction
+    msd_partials.resize(size_t(REDUCE_BLOCKS) * 6 * block_size);
+    msd_out.resize(6 * block_size);
+    h_msd.resize(6 * block_size);
+}
+
+blocking_scheme::pos_sample blocking_scheme::take_velocity_sample() {
+    p.settle_velocity();
+    auto s = std::make_shared<dbuf<double>>(3 * size_t(p.n));
+    launch_sample_velocity(p.vel, p.id, p.n, *s, p.stream);
+    return s;
+}
+
+size_t blocking_scheme::block_length(size_t level) const {
+    if (msd_enabled || mqd_enabled) return pos_blocks[level].size();
+    if (vacf_enabled) return vel_blocks[level].size();
+    return rho_blocks[level].size();
 }
 
 blocking_scheme::pos_sample blocking_scheme::take_position_sample() {
@@ -1266,48 +1283,68 @@ blocking_scheme::pos_sample blocking_scheme::take_position_sample() {
 }
 
 void blocking_scheme::sample(uint64_t step) {
-    pos_sample ps;
+    pos_sample ps, vs;
     rho_sample rs;
     for (size_t i = 0; i < interval.size(); ++i) {
         if (step >= origin[i] && (step - origin[i]) % interval[i] == 0) {
-            if (msd_enabled) {
+            if (msd_enabled || mqd_enabled) {
                 if (!ps) ps = take_position_sample();
                 pos_blocks[i].push_back(ps);
                 if (pos_blocks[i].size() > block_size) pos_blocks[i].erase(pos_blocks[i].begin());
+            }
+            if (vacf_enabled) {
+                if (!vs) vs = take_velocity_sample();
+                vel_blocks[i].push_back(vs);
+                if (vel_blocks[i].size() > block_size) vel_blocks[i].erase(vel_blocks[i].begin());
             }
             if (dm) {
                 if (!rs) rs = dm->acquire();
                 rho_blocks[i].push_back(rs);
                 if (rho_blocks[i].size() > block_size) rho_blocks[i].erase(rho_blocks[i].begin());
             }
-            size_t const sz = msd_enabled ? pos_blocks[i].size() : rho_blocks[i].size();
-            if (sz == block_size) process(i);
+            if (block_length(i) == block_size) process(i);
         }
     }
 }
 
 void blocking_scheme::finalise() {
     for (size_t i = 0; i < interval.size(); ++i) {
-        while (!(msd_enabled ? pos_blocks[i].empty() : rho_blocks[i].empty())) process(i);
+        while (block_length(i) > 0) process(i);
     }
 }
 
 void blocking_scheme::process(size_t level) {
-    if (msd_enabled && !pos_blocks[level].empty()) {
-        auto const& blk = pos_blocks[level];
-        unsigned const nl = unsigned(blk.size());
-        for (unsigned l = 0; l < nl; ++l)
-            launch_msd_partials(*blk.front(), *blk[l], p.n, msd_partials, 2 * nl, 2 * l, p.stream);
-        launch_reduce_final(msd_partials, 2 * nl, msd_out, p.stream);
-        HIP_CHECK(hipMemcpyAsync(h_msd.get(), msd_out.get(), 2 * nl * sizeof(double), hipMemcpyDeviceToHost, p.stream));
+    // MSD, MQD (position samples) and VACF (velocity samples): every enabled
+    // correlation of the block in one reduction and one read
+    struct tcf { int kind; bool on; std::vector<pos_sample> const* blk; std::vector<accumulator>* acc; };
+    tcf const all[3] = {{TCF_MSD, msd_enabled, &pos_blocks[level], &msd},
+                        {TCF_MQD, mqd_enabled, &pos_blocks[level], &mqd},
+                        {TCF_VACF, vacf_enabled, &vel_blocks[level], &vacf}};
+    std::vector<tcf> on;
+    for (auto const& t : all)
+        if (t.on && !t.blk->empty()) on.push_back(t);
+    if (!on.empty()) {
+        unsigned const nl = unsigned(on.front().blk->size());
+        unsigned const nq = 2 * nl * unsigned(on.size());
+        for (size_t k = 0; k < on.size(); ++k) {
+            auto const& blk = *on[k].blk;
+            std::vector<double const*> r2(nl);
+            for (unsigned l = 0; l < nl; ++l) r2[l] = blk[l]->get();
+            launch_tcf_partials_lags(on[k].kind, *blk.front(), r2.data(), nl, p.n, msd_partials, nq,
+                                     unsigned(2 * nl * k), p.stream);
+        }
+        launch_reduce_final(msd_partials, nq, msd_out, p.stream);
+        HIP_CHECK(hipMemcpyAsync(h_msd.get(), msd_out.get(), nq * sizeof(double), hipMemcpyDeviceToHost, p.stream));
         p.sync();
-        for (unsigned l = 0; l < nl; ++l) {
-            double const s1 = h_msd[2 * l], s2 = h_msd[2 * l + 1];
-            accumulator a;
-            a.n = p.n;
-            a.m = s1 / p.n;
-            a.v = std::max(0.0, s2 - s1 * s1 / p.n);
-            msd[level * block_size + l].merge(a);
+        for (size_t k = 0; k < on.size(); ++k) {
+            for (unsigned l = 0; l < nl; ++l) {
+                double const s1 = h_msd[2 * nl * k + 2 * l], s2 = h_msd[2 * nl * k + 2 * l + 1];
+                accumulator a;
+                a.n = p.n;
+                a.m = s1 / p.n;
+                a.v = std::max(0.0, s2 - s1 * s1 / p.n);
+                (*on[k].acc)[level * block_size + l].merge(a);
+            }
         }
     }
     if (dm && !rho_blocks[level].empty()) {
@@ -1330,6 +1367,7 @@ void blocking_scheme::process(size_t level) {
     origin[level] += skip * iv;
     for (uint64_t k = 0; k < skip; ++k) {
         if (!pos_blocks[level].empty()) pos_blocks[level].erase(pos_blocks[level].begin());
+        if (!vel_blocks[level].empty()) vel_blocks[level].erase(vel_blocks[level].begin());
         if (!rho_blocks[level].empty()) rho_blocks[level].erase(rho_blocks[level].begin());
     }
 }

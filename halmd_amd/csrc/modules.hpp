@@ -434,13 +434,13 @@ public:
     unsigned block_size, separation;
     std::vector<uint64_t> interval, origin;
     density_mode* dm = nullptr;       // ISF source (optional)
-    bool msd_enabled = false;
+    bool msd_enabled = false, mqd_enabled = false, vacf_enabled = false;
     std::vector<std::pair<size_t, size_t>> shell;
     using pos_sample = std::shared_ptr<dbuf<double>>;
     using rho_sample = std::shared_ptr<std::vector<std::complex<double>>>;
-    std::vector<std::vector<pos_sample>> pos_blocks;
+    std::vector<std::vector<pos_sample>> pos_blocks, vel_blocks;
     std::vector<std::vector<rho_sample>> rho_blocks;
-    std::vector<accumulator> msd, isf;
+    std::vector<accumulator> msd, mqd, vacf, isf;
     dbuf<double> msd_partials, msd_out;
     hbuf<double> h_msd;
     void sample(uint64_t step);
@@ -448,6 +448,8 @@ public:
 private:
     void process(size_t level);
     pos_sample take_position_sample();
+    pos_sample take_velocity_sample();
+    size_t block_length(size_t level) const;
 };
 
 class sampler {

@@ -88,6 +88,18 @@ __global__ void density_mode_final_kernel(double const* __restrict__ partials, u
     out[2 * size_t(k) + 1] = b;
 }
 
+// velocity sample in particle-ID order (hi + lo in FP64)
+__global__ void sample_velocity_kernel(float4 const* __restrict__ vel, uint32_t const* __restrict__ id, unsigned n,
+                                       double* __restrict__ out) {
+    unsigned const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 const hi = vel[2 * size_t(i)], lo = vel[2 * size_t(i) + 1];
+    size_t const o = 3 * size_t(id[i]);
+    out[o] = double(hi.x) + double(lo.x);
+    out[o + 1] = double(hi.y) + double(lo.y);
+    out[o + 2] = double(hi.z) + double(lo.z);
+}
+
 __global__ void sample_position_kernel(float4 const* __restrict__ pos, int4 const* __restrict__ image,
                                        uint32_t const* __restrict__ id, unsigned n, double Lx, double Ly, double Lz,
                                        double* __restrict__ out) {
@@ -101,9 +113,12 @@ __global__ void sample_position_kernel(float4 const* __restrict__ pos, int4 cons
     out[o + 2] = double(hi.z) + double(lo.z) + double(im.z) * Lz;
 }
 
-// block row y: the pair (r1, r2.p[y]) into partials[block][qoff + 2 y + {0, 1}];
-// every row runs the same grid-stride order as a one-pair launch
-__global__ __launch_bounds__(REDUCE_THREADS) void msd_partials_kernel(double const* __restrict__ r1,
+// block row y: the per-particle value x of the pair (r1, r2.p[y]) summed as
+// (sum x, sum x^2) into partials[block][qoff + 2 y + {0, 1}]; every row runs the
+// same grid-stride order as a one-pair launch.  x by KIND
+// (observables/dynamics/*.hpp): TCF_MSD |dr|^2, TCF_MQD |dr|^4, TCF_VACF v1.v2
+template <int KIND>
+__global__ __launch_bounds__(REDUCE_THREADS) void tcf_partials_kernel(double const* __restrict__ r1,
                                                                       msd_samples r2s, unsigned n,
                                                                       double* __restrict__ partials, unsigned nq,
                                                                       unsigned qoff) {
@@ -112,10 +127,17 @@ __global__ __launch_bounds__(REDUCE_THREADS) void msd_partials_kernel(double con
     qoff += 2 * blockIdx.y;
     double a = 0, b = 0;
     for (unsigned i = blockIdx.x * REDUCE_THREADS + threadIdx.x; i < n; i += REDUCE_BLOCKS * REDUCE_THREADS) {
-        double const dx = r2[3 * size_t(i)] - r1[3 * size_t(i)];
-        double const dy = r2[3 * size_t(i) + 1] - r1[3 * size_t(i) + 1];
-        double const dz = r2[3 * size_t(i) + 2] - r1[3 * size_t(i) + 2];
-        double const d2 = dx * dx + dy * dy + dz * dz;
+        double d2;
+        if (KIND == TCF_VACF) {
+            d2 = r1[3 * size_t(i)] * r2[3 * size_t(i)] + r1[3 * size_t(i) + 1] * r2[3 * size_t(i) + 1] +
+                 r1[3 * size_t(i) + 2] * r2[3 * size_t(i) + 2];
+        } else {
+            double const dx = r2[3 * size_t(i)] - r1[3 * size_t(i)];
+            double const dy = r2[3 * size_t(i) + 1] - r1[3 * size_t(i) + 1];
+            double const dz = r2[3 * size_t(i) + 2] - r1[3 * size_t(i) + 2];
+            d2 = dx * dx + dy * dy + dz * dz;
+            if (KIND == TCF_MQD) d2 = d2 * d2;
+        }
         a += d2;
         b += d2 * d2;
     }
@@ -152,18 +174,38 @@ void launch_sample_position(float4 const* pos, int4 const* image, uint32_t const
 
 void launch_msd_partials(double const* r1, double const* r2, unsigned n, double* partials, unsigned nq, unsigned qoff,
                          hipStream_t s) {
-    launch_msd_partials_lags(r1, &r2, 1, n, partials, nq, qoff, s);
+    launch_tcf_partials_lags(TCF_MSD, r1, &r2, 1, n, partials, nq, qoff, s);
 }
 
 void launch_msd_partials_lags(double const* r1, double const* const* r2, unsigned nl, unsigned n, double* partials,
                               unsigned nq, unsigned qoff, hipStream_t s) {
+    launch_tcf_partials_lags(TCF_MSD, r1, r2, nl, n, partials, nq, qoff, s);
+}
+
+void launch_tcf_partials_lags(int kind, double const* r1, double const* const* r2, unsigned nl, unsigned n,
+                              double* partials, unsigned nq, unsigned qoff, hipStream_t s) {
     for (unsigned l0 = 0; l0 < nl; l0 += MSD_LAGS) {
         unsigned const m = std::min(MSD_LAGS, nl - l0);
         msd_samples t{};
         for (unsigned l = 0; l < m; ++l) t.p[l] = r2[l0 + l];
-        msd_partials_kernel<<<dim3(REDUCE_BLOCKS, m), REDUCE_THREADS, 0, s>>>(r1, t, n, partials, nq, qoff + 2 * l0);
+        dim3 const grid(REDUCE_BLOCKS, m);
+        unsigned const q = qoff + 2 * l0;
+        if (kind == TCF_MSD)
+            tcf_partials_kernel<TCF_MSD><<<grid, REDUCE_THREADS, 0, s>>>(r1, t, n, partials, nq, q);
+        else if (kind == TCF_MQD)
+            tcf_partials_kernel<TCF_MQD><<<grid, REDUCE_THREADS, 0, s>>>(r1, t, n, partials, nq, q);
+        else if (kind == TCF_VACF)
+            tcf_partials_kernel<TCF_VACF><<<grid, REDUCE_THREADS, 0, s>>>(r1, t, n, partials, nq, q);
+        else
+            throw error(HALMD_ERR_INVALID_ARGUMENT, "unknown time correlation");
         HIP_CHECK(hipGetLastError());
     }
+}
+
+void launch_sample_velocity(float4 const* vel, uint32_t const* id, unsigned n, double* out, hipStream_t s) {
+    if (!n) return;
+    sample_velocity_kernel<<<div_up(n, 256), 256, 0, s>>>(vel, id, n, out);
+    HIP_CHECK(hipGetLastError());
 }
 
 } // namespace halmd

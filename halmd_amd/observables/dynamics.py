@@ -46,6 +46,12 @@ class blocking_scheme(Handle):
     def correlate_msd(self):
         call("halmd_observables_dynamics_blocking_scheme_add_msd", self.handle)
 
+    def correlate_mqd(self):
+        call("halmd_observables_dynamics_blocking_scheme_add_mqd", self.handle)
+
+    def correlate_vacf(self):
+        call("halmd_observables_dynamics_blocking_scheme_add_vacf", self.handle)
+
     def correlate_isf(self, density_mode):
         call("halmd_observables_dynamics_blocking_scheme_add_isf", self.handle, density_mode.handle)
         self.density_mode = density_mode
@@ -57,8 +63,8 @@ class blocking_scheme(Handle):
         call("halmd_observables_dynamics_blocking_scheme_finalise", self.handle)
 
     def result(self, which):
-        """(count, mean, error) arrays; which = 'msd' or 'isf'."""
-        if which == "msd":
+        """(count, mean, error) arrays; which = 'msd', 'mqd', 'vacf' or 'isf'."""
+        if which in ("msd", "mqd", "vacf"):
             shape = (self.levels, self.block_size)
         else:
             shape = (self.levels, self.block_size, len(self.density_mode.wavevector.shell))

@@ -301,13 +301,18 @@ HALMD_API int halmd_observables_dynamics_blocking_scheme_create(halmd_particle* 
 HALMD_API int halmd_observables_dynamics_blocking_scheme_destroy(halmd_blocking_scheme* b);
 HALMD_API int halmd_observables_dynamics_blocking_scheme_add_msd(halmd_blocking_scheme* b);
 HALMD_API int halmd_observables_dynamics_blocking_scheme_add_isf(halmd_blocking_scheme* b, halmd_density_mode* d);
+/* correlation{mean_quartic_displacement} and correlation{velocity_autocorrelation}
+ * (observables/dynamics/{mean_quartic_displacement,velocity_autocorrelation}.hpp,
+ * gpu/dynamics/{mean_quartic_displacement,velocity_autocorrelation}.cpp) */
+HALMD_API int halmd_observables_dynamics_blocking_scheme_add_mqd(halmd_blocking_scheme* b);
+HALMD_API int halmd_observables_dynamics_blocking_scheme_add_vacf(halmd_blocking_scheme* b);
 HALMD_API int halmd_observables_dynamics_blocking_scheme_sample(halmd_blocking_scheme* b, uint64_t step);
 HALMD_API int halmd_observables_dynamics_blocking_scheme_finalise(halmd_blocking_scheme* b);
 HALMD_API int halmd_observables_dynamics_blocking_scheme_levels(const halmd_blocking_scheme* b, unsigned* levels,
                                                                 unsigned* block_size);
 HALMD_API int halmd_observables_dynamics_blocking_scheme_intervals(const halmd_blocking_scheme* b,
                                                                    uint64_t* interval);
-/* which: "msd" ([levels][block_size]) or "isf" ([levels][block_size][nshell]) */
+/* which: "msd", "mqd", "vacf" ([levels][block_size]) or "isf" ([levels][block_size][nshell]) */
 HALMD_API int halmd_observables_dynamics_blocking_scheme_result(const halmd_blocking_scheme* b, const char* which,
                                                                 uint64_t* count, double* mean, double* error);
 

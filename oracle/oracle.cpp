@@ -1061,11 +1061,12 @@ long oracle_wavevector_shell(int nk, double const* wavenumber, double const* L, 
 
 // ---------------------------------------------------------------------------
 // Blocking-scheme correlator driven by a caller-supplied sample stream.
-// A sample is (step, unwrapped positions [3n], density modes [2 nq]).
-// Correlates MSD (mean over particles of |R2 - R1|^2,
-// halmd/observables/dynamics/mean_square_displacement.hpp:31-43) and ISF
-// (per shell mean of Re[rho1 rho2*] / N,
-// halmd/observables/dynamics/intermediate_scattering_function.hpp:89-113).
+// A sample is (step, unwrapped positions [3n], density modes [2 nq],
+// velocities [3n]).  Correlates MSD (mean over particles of |R2 - R1|^2,
+// halmd/observables/dynamics/mean_square_displacement.hpp:31-43), MQD (of
+// |R2 - R1|^4, mean_quartic_displacement.hpp:33-42), VACF (of v1.v2,
+// velocity_autocorrelation.hpp:33-38) and ISF (per shell mean of
+// Re[rho1 rho2*] / N, intermediate_scattering_function.hpp:89-113).
 // Per level and lag the result is a Welford accumulator; the MSD accumulator
 // merges the per-sample-pair accumulator over particles (as the device reduction
 // of halmd/observables/gpu/dynamics/mean_square_displacement.cpp:55-68 does).
@@ -1074,9 +1075,9 @@ struct correlator_t {
     blocking_scheme_t bs;
     int n, nq, nshell;
     std::vector<long> shell;   // 2 per shell
-    std::vector<std::vector<double>> pos_store, rho_store;
+    std::vector<std::vector<double>> pos_store, rho_store, vel_store;
     std::vector<std::vector<int>> blocks;  // sample ids
-    std::vector<accumulator_t> msd, isf;   // [level][lag] and [level][lag][shell]
+    std::vector<accumulator_t> msd, mqd, vacf, isf;   // [level][lag] and [level][lag][shell]
     double norm;
 
     correlator_t(uint64_t max_interval, uint64_t res, unsigned bsz, unsigned shift, unsigned sep, uint64_t step0,
@@ -1086,6 +1087,8 @@ struct correlator_t {
     {
         blocks.resize(bs.interval.size());
         msd.resize(bs.interval.size() * bsz);
+        mqd.resize(bs.interval.size() * bsz);
+        vacf.resize(bs.interval.size() * bsz);
         isf.resize(bs.interval.size() * bsz * size_t(nshell));
     }
 
@@ -1097,12 +1100,23 @@ struct correlator_t {
             auto const& p1 = pos_store[first];
             auto const& p2 = pos_store[second];
             if (!p1.empty()) {
-                accumulator_t acc;
+                accumulator_t acc, acc4;
                 for (int i = 0; i < n; ++i) {
                     double d0 = p2[3 * i] - p1[3 * i], d1 = p2[3 * i + 1] - p1[3 * i + 1], d2 = p2[3 * i + 2] - p1[3 * i + 2];
-                    acc(d0 * d0 + d1 * d1 + d2 * d2);
+                    double const rr = d0 * d0 + d1 * d1 + d2 * d2;
+                    acc(rr);
+                    acc4(rr * rr);
                 }
                 msd[level * bs.block_size + l].merge(acc);
+                mqd[level * bs.block_size + l].merge(acc4);
+            }
+            auto const& v1 = vel_store[first];
+            auto const& v2 = vel_store[second];
+            if (!v1.empty()) {
+                accumulator_t acc;
+                for (int i = 0; i < n; ++i)
+                    acc(v1[3 * i] * v2[3 * i] + v1[3 * i + 1] * v2[3 * i + 1] + v1[3 * i + 2] * v2[3 * i + 2]);
+                vacf[level * bs.block_size + l].merge(acc);
             }
             auto const& r1 = rho_store[first];
             auto const& r2 = rho_store[second];
@@ -1127,10 +1141,11 @@ struct correlator_t {
         for (uint64_t k = 0; k < skip && !blocks[level].empty(); ++k) blocks[level].erase(blocks[level].begin());
     }
 
-    void sample(uint64_t step, double const* pos, double const* rho) {
+    void sample(uint64_t step, double const* pos, double const* rho, double const* vel) {
         int sid = int(pos_store.size());
         pos_store.emplace_back(pos ? std::vector<double>(pos, pos + 3 * size_t(n)) : std::vector<double>());
         rho_store.emplace_back(rho ? std::vector<double>(rho, rho + 2 * size_t(nq)) : std::vector<double>());
+        vel_store.emplace_back(vel ? std::vector<double>(vel, vel + 3 * size_t(n)) : std::vector<double>());
         for (size_t i = 0; i < bs.interval.size(); ++i) {
             if (step >= bs.origin[i] && (step - bs.origin[i]) % bs.interval[i] == 0) {
                 blocks[i].push_back(sid);
@@ -1159,16 +1174,17 @@ void oracle_correlator_intervals(void* h, long* out)
     auto* c = static_cast<correlator_t*>(h);
     for (size_t i = 0; i < c->bs.interval.size(); ++i) out[i] = long(c->bs.interval[i]);
 }
-void oracle_correlator_sample(void* h, long step, double const* pos, double const* rho)
+void oracle_correlator_sample(void* h, long step, double const* pos, double const* rho, double const* vel)
 {
-    static_cast<correlator_t*>(h)->sample(uint64_t(step), pos, rho);
+    static_cast<correlator_t*>(h)->sample(uint64_t(step), pos, rho, vel);
 }
 void oracle_correlator_finalise(void* h) { static_cast<correlator_t*>(h)->finalise(); }
-// results: count, mean, error_of_mean per [level][lag] (msd) and [level][lag][shell] (isf)
+// results: count, mean, error_of_mean per [level][lag] (0 msd, 2 mqd, 3 vacf)
+// and [level][lag][shell] (1 isf)
 void oracle_correlator_result(void* h, int which, long* count, double* mean, double* error)
 {
     auto* c = static_cast<correlator_t*>(h);
-    auto const& v = which == 0 ? c->msd : c->isf;
+    auto const& v = which == 0 ? c->msd : which == 1 ? c->isf : which == 2 ? c->mqd : c->vacf;
     for (size_t i = 0; i < v.size(); ++i) {
         count[i] = long(v[i].n);
         mean[i] = v[i].m;

@@ -72,7 +72,7 @@ def lib():
             "oracle_correlator_destroy": (None, [_vp]),
             "oracle_correlator_levels": (ctypes.c_int, [_vp]),
             "oracle_correlator_intervals": (None, [_vp, _l]),
-            "oracle_correlator_sample": (None, [_vp, ctypes.c_long, _d, _d]),
+            "oracle_correlator_sample": (None, [_vp, ctypes.c_long, _d, _d, _d]),
             "oracle_correlator_finalise": (None, [_vp]),
             "oracle_correlator_result": (None, [_vp, ctypes.c_int, _l, _d, _d]),
             "oracle_accumulate": (None, [ctypes.c_int, _d, _l, _d, _d]),
@@ -323,7 +323,7 @@ def wavevector_shell(wavenumber, L, tolerance, max_count):
 
 
 class Correlator:
-    """Blocking-scheme MSD / ISF correlator driven by explicit samples."""
+    """Blocking-scheme MSD / MQD / VACF / ISF correlator driven by explicit samples."""
 
     def __init__(self, max_interval, resolution, block_size, shift, separation, step0, n, nq=0, shell=None):
         shell = np.zeros((0, 2), dtype=np.int64) if shell is None else np.ascontiguousarray(shell, dtype=np.int64)
@@ -343,21 +343,24 @@ class Correlator:
         lib().oracle_correlator_intervals(self.h, _p(out, _l))
         return out
 
-    def sample(self, step, pos=None, rho=None):
+    def sample(self, step, pos=None, rho=None, vel=None):
         p = _dd(pos) if pos is not None else None
+        v = _dd(vel) if vel is not None else None
         r = None
         if rho is not None:
             r = np.ascontiguousarray(np.stack([np.real(rho), np.imag(rho)], axis=-1), dtype=np.float64)
-        lib().oracle_correlator_sample(self.h, step, _p(p, _d), _p(r, _d))
+        lib().oracle_correlator_sample(self.h, step, _p(p, _d), _p(r, _d), _p(v, _d))
 
     def finalise(self):
         lib().oracle_correlator_finalise(self.h)
 
     def result(self, which):
-        shape = (self.levels, self.block_size) if which == "msd" else (self.levels, self.block_size, self.nshell)
+        """(count, mean, error of mean); which = 'msd', 'mqd', 'vacf' or 'isf'."""
+        code = {"msd": 0, "isf": 1, "mqd": 2, "vacf": 3}[which]
+        shape = (self.levels, self.block_size) if code != 1 else (self.levels, self.block_size, self.nshell)
         size = int(np.prod(shape))
         c = np.zeros(size, dtype=np.int64); m = np.zeros(size); e = np.zeros(size)
-        lib().oracle_correlator_result(self.h, 0 if which == "msd" else 1, _p(c, _l), _p(m, _d), _p(e, _d))
+        lib().oracle_correlator_result(self.h, code, _p(c, _l), _p(m, _d), _p(e, _d))
         return c.reshape(shape), m.reshape(shape), e.reshape(shape)
 
 

@@ -85,6 +85,46 @@ def test_msd_isf_blocking_scheme_matches_oracle(oracle):
     assert np.all(np.diff(lvl0) > 0)
 
 
+def test_mqd_vacf_blocking_scheme_matches_oracle(oracle):
+    w = workloads.lennard_jones(2048, density=0.8442, temperature=1.5, jitter=0.1, seed=73, timestep=0.005,
+                                sort_interval=25)
+    bs = blocking_scheme(w.box, w.particle, timestep=0.005, max_lag=200 * 0.005, resolution=2 * 0.005,
+                         block_size=5, shift=2, separation=5)
+    bs.correlate_msd()
+    bs.correlate_mqd()
+    bs.correlate_vacf()
+    oc = oracle.Correlator(200, 2, 5, 2, 5, 0, 2048)
+    L = w.length
+    for step in range(0, 301):
+        if step:
+            w.core.mdstep()
+        due = any(step % iv == 0 for iv in bs.interval)
+        bs.sample(step)
+        if due:
+            pos = w.particle.ordered("position") + w.particle.ordered("image") * L
+            oc.sample(step, pos, vel=w.particle.ordered("velocity"))
+    bs.finalise()
+    oc.finalise()
+    for name in ("msd", "mqd", "vacf"):
+        c, m, e = bs.result(name)
+        oc_c, oc_m, oc_e = oc.result(name)
+        assert np.array_equal(c, oc_c), name
+        nz = c > 0
+        print(f"{name}: max rel err {np.max(np.abs(m[nz] - oc_m[nz]) / np.maximum(np.abs(oc_m[nz]), 1e-300)):.2e}")
+        assert np.allclose(m[nz], oc_m[nz], rtol=1e-9, atol=1e-14), name
+        assert np.allclose(e[nz], oc_e[nz], rtol=1e-5, atol=1e-12), name
+    # physics: VACF(0) = <v^2> = 2 e_kin (unit mass) and the VACF decays;
+    # MQD(0) = MSD(0) = 0; MQD >= MSD^2 (Jensen)
+    vc, vm, _ = bs.result("vacf")
+    assert np.isclose(vm[0, 0], 2 * w.thermodynamics.en_kin(), rtol=0.1)
+    assert np.all(np.abs(vm[vc > 0]) <= 1.05 * vm[0, 0])
+    qc, qm, _ = bs.result("mqd")
+    sc, sm, _ = bs.result("msd")
+    assert qm[0, 0] == 0 and sm[0, 0] == 0
+    nz = (qc > 0) & (sm > 0)
+    assert np.all(qm[nz] >= sm[nz] ** 2 * (1 - 1e-12))
+
+
 def test_sampler_density_mode_slot():
     w = workloads.lennard_jones(4000, density=0.8442, seed=72)
     wv = wavevector(w.box, np.linspace(0.5, 10, 20), tolerance=0.05, max_count=10)

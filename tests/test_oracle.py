@@ -498,3 +498,35 @@ def test_msd_ballistic_and_isf_lag0(oracle):
     # level 0, lag 0 averages |rho_k|^2/N over all processed origins; check bounds
     vals = [np.mean(np.abs(rhos[s]) ** 2) / n for s in rhos]
     assert min(vals) - 1e-12 <= imean[0, 0, 0] <= max(vals) + 1e-12
+
+
+def test_mqd_ballistic_and_vacf_alternating(oracle):
+    # ballistic R(t) = R0 + v t: MQD(lag) = <|v|^4> lag^4; velocities that flip
+    # sign every step: VACF(lag) = (-1)^(lag interval) <|v|^2> (pins the sample
+    # pairing and index)
+    n = 48
+    rng = np.random.default_rng(5)
+    r0 = rng.uniform(-5, 5, (n, 3))
+    v = rng.normal(0, 1, (n, 3))
+    dt = 0.01
+    c = oracle.Correlator(300, 1, 6, 2, 6, 0, n)
+    for step in range(0, 601):
+        c.sample(step, pos=r0 + v * step * dt, vel=v * (-1.0) ** step)
+    c.finalise()
+    iv = c.intervals()
+    v2 = np.mean(np.sum(v * v, 1))
+    v4 = np.mean(np.sum(v * v, 1) ** 2)
+    mc, mm, _ = c.result("mqd")
+    vc, vm, _ = c.result("vacf")
+    checked = 0
+    for lvl in range(len(iv)):
+        for lag in range(6):
+            if mc[lvl, lag]:
+                assert np.isclose(mm[lvl, lag], v4 * (lag * iv[lvl] * dt) ** 4, rtol=1e-9, atol=1e-15)
+            if vc[lvl, lag]:
+                assert np.isclose(vm[lvl, lag], (-1.0) ** (lag * iv[lvl]) * v2, rtol=1e-12)
+                checked += 1
+    assert checked > 10
+    # MSD from the same position samples is unchanged by the extra correlations
+    sc, sm, _ = c.result("msd")
+    assert np.array_equal(sc, mc)
