@@ -243,81 +243,115 @@ __global__ __launch_bounds__(256) void build_all_pairs_kernel(uint4 const* __res
 
 struct nb16_state {
     uint4 ri;
-    unsigned own, ti, c, cap;
-    uint16_t* list;      // the system's lists (uniform over the workgroup)
-    unsigned base;       // element index of entry 0 of this particle (lane block)
+    unsigned own, ti;
+    unsigned c2, last2;  // twice the count; twice the last slot (cap - 1)
+    char* list;          // the workgroup's list base (uniform)
+    unsigned base2;      // byte offset of entry 0 of this particle (lane block) from list
 };
 
 // 2-byte stores straight to the blocked list: a lane fills one 32-byte block
 // at a time and a wave's blocks share 2 KiB, so the lines are complete in L2
 // before they are written back (no staging buffer, hence more workgroups per
-// CU).  The address is a 32-bit element index from the uniform base.  Entries
-// past the capacity overwrite the particle's last slot instead of branching:
-// the count still exceeds the capacity, and the caller rebuilds larger lists.
+// CU).  Byte offset of entry c: base2 + 2048 (c / 16) + 2 (c % 16), formed
+// from c2 = 2c as ((c2 & ~31) << 6) + base2, | (c2 & 31) (base2 is a multiple
+// of 32).  Entries past the capacity overwrite the particle's last slot
+// instead of branching: the count still exceeds the capacity, and the caller
+// rebuilds larger lists.
 __device__ __forceinline__ void append16(nb16_state& s, unsigned entry) {
-    unsigned const c = min(s.c, s.cap - 1u);
-    s.list[s.base + (c / NB16_BLOCK) * (WAVE * NB16_BLOCK) + c % NB16_BLOCK] = uint16_t(entry);
-    ++s.c;
+    unsigned const c2 = min(s.c2, s.last2);
+    unsigned const o = (((c2 & ~31u) << 6) + s.base2) | (c2 & 31u);
+    *reinterpret_cast<uint16_t*>(s.list + o) = uint16_t(entry);
+    s.c2 += 2;
 }
 
-// the list entry is the record's byte offset in the staged union
-template <bool MULTI, bool ISO, bool SELF>
+// the list entry is the record's byte offset in the full-record union (16
+// bytes per record, the layout the force kernel stages)
+
+// non-cubic box: FP32 squared distance of the scaled fixed-point differences
+template <bool MULTI, bool SELF>
 __device__ __forceinline__ void test16(nb16_state& s, uint4 const rj, unsigned slot, neighbour_params const& p,
-                                       float const* s_rr, float rr_cut0) {
+                                       uint32_t const* s_rr, uint32_t rr0) {
 #pragma clang fp contract(off)
-    bool inside;
-    if (ISO) {
-        float const dx = float(int32_t(s.ri.x - rj.x));
-        float const dy = float(int32_t(s.ri.y - rj.y));
-        float const dz = float(int32_t(s.ri.z - rj.z));
-        float rq = dx * dx;
-        rq = __builtin_fmaf(dy, dy, rq);
-        rq = __builtin_fmaf(dz, dz, rq);
-        inside = rq <= (MULTI ? s_rr[s.ti + (rj.w & 3u)] : rr_cut0);
-    } else {
-        float const dx = float(int32_t(s.ri.x - rj.x)) * p.q[0];
-        float const dy = float(int32_t(s.ri.y - rj.y)) * p.q[1];
-        float const dz = float(int32_t(s.ri.z - rj.z)) * p.q[2];
-        float rr = dx * dx;
-        rr = rr + dy * dy;
-        rr = rr + dz * dz;
-        inside = rr <= (MULTI ? s_rr[s.ti + (rj.w & 3u)] : rr_cut0);
-    }
+    float const dx = float(int32_t(s.ri.x - rj.x)) * p.q[0];
+    float const dy = float(int32_t(s.ri.y - rj.y)) * p.q[1];
+    float const dz = float(int32_t(s.ri.z - rj.z)) * p.q[2];
+    float rr = dx * dx;
+    rr = rr + dy * dy;
+    rr = rr + dz * dz;
+    bool const inside = rr <= __uint_as_float(MULTI ? s_rr[s.ti + (rj.w & 3u)] : rr0);
+    if ((!SELF || slot != s.own) && inside) append16(s, slot * 16u);
+}
+
+// cubic box, packed records: exact integer squared distance of the 16-bit
+// differences of the top halves (wrapped, hence the minimum image), at most
+// 3 * 2^30 and compared unsigned; s.ri.x and s.ri.y hold the particle's own
+// packed record
+typedef short short2v __attribute__((ext_vector_type(2)));
+typedef unsigned short ushort2v __attribute__((ext_vector_type(2)));
+template <bool MULTI, bool SELF>
+__device__ __forceinline__ void test16p(nb16_state& s, uint2 const rj, unsigned slot, uint32_t const* s_rr,
+                                        uint32_t rr0) {
+    ushort2v const dxy = __builtin_bit_cast(ushort2v, s.ri.x) - __builtin_bit_cast(ushort2v, rj.x);
+    int const dz = short(s.ri.y - rj.y);
+    short2v const d = __builtin_bit_cast(short2v, dxy);
+    uint32_t const rq = uint32_t(__builtin_amdgcn_sdot2(d, d, dz * dz, false));
+    bool const inside = rq <= (MULTI ? s_rr[s.ti + (rj.y >> 16)] : rr0);
     if ((!SELF || slot != s.own) && inside) append16(s, slot * 16u);
 }
 
 // candidates [t, e) of one row, four records in flight
-template <bool MULTI, bool ISO, bool SELF>
+template <bool MULTI, bool SELF>
 __device__ __forceinline__ void scan16(nb16_state& s, unsigned t, unsigned const e, uint4 const* s_rec,
-                                       neighbour_params const& p, float const* s_rr, float rr_cut0) {
+                                       neighbour_params const& p, uint32_t const* s_rr, uint32_t rr0) {
     for (; t + 4 <= e; t += 4) {
         uint4 const r0 = s_rec[t], r1 = s_rec[t + 1], r2 = s_rec[t + 2], r3 = s_rec[t + 3];
-        test16<MULTI, ISO, SELF>(s, r0, t, p, s_rr, rr_cut0);
-        test16<MULTI, ISO, SELF>(s, r1, t + 1, p, s_rr, rr_cut0);
-        test16<MULTI, ISO, SELF>(s, r2, t + 2, p, s_rr, rr_cut0);
-        test16<MULTI, ISO, SELF>(s, r3, t + 3, p, s_rr, rr_cut0);
+        test16<MULTI, SELF>(s, r0, t, p, s_rr, rr0);
+        test16<MULTI, SELF>(s, r1, t + 1, p, s_rr, rr0);
+        test16<MULTI, SELF>(s, r2, t + 2, p, s_rr, rr0);
+        test16<MULTI, SELF>(s, r3, t + 3, p, s_rr, rr0);
     }
-    for (; t < e; ++t) test16<MULTI, ISO, SELF>(s, s_rec[t], t, p, s_rr, rr_cut0);
+    for (; t < e; ++t) test16<MULTI, SELF>(s, s_rec[t], t, p, s_rr, rr0);
+}
+template <bool MULTI, bool SELF>
+__device__ __forceinline__ void scan16(nb16_state& s, unsigned t, unsigned const e, uint2 const* s_rec,
+                                       neighbour_params const&, uint32_t const* s_rr, uint32_t rr0) {
+    for (; t + 4 <= e; t += 4) {
+        uint2 const r0 = s_rec[t], r1 = s_rec[t + 1], r2 = s_rec[t + 2], r3 = s_rec[t + 3];
+        test16p<MULTI, SELF>(s, r0, t, s_rr, rr0);
+        test16p<MULTI, SELF>(s, r1, t + 1, s_rr, rr0);
+        test16p<MULTI, SELF>(s, r2, t + 2, s_rr, rr0);
+        test16p<MULTI, SELF>(s, r3, t + 3, s_rr, rr0);
+    }
+    for (; t < e; ++t) test16p<MULTI, SELF>(s, s_rec[t], t, s_rr, rr0);
 }
 
 // One cell segment of the staged list build (the whole workgroup calls it);
-// shared by the single-system kernel and the ensemble kernel.  stats[0]
-// receives the largest count, stats[1] the staging size U + 1.
+// shared by the single-system kernel and the ensemble kernel.  The union is
+// staged as packed 16-bit records in a cubic box (8 bytes per record of the
+// dynamic LDS), else as full records (16 bytes).  stats[0] receives the
+// largest count, stats[1] the staging size U + 1.
 template <bool MULTI, bool ISO>
 __device__ __forceinline__ void staged_build_segment(
     uint4 const* __restrict__ posq, uint32_t const* __restrict__ off, seg_params const& sp, unsigned seg,
     neighbour_params const& p, unsigned cap, uint16_t* __restrict__ list, uint32_t* __restrict__ count,
-    uint32_t* __restrict__ stats, uint4* s_rec, seg_stage& st, unsigned* s_bound, float* s_rr)
+    uint32_t* __restrict__ stats, uint4* s_lds, seg_stage& st, unsigned* s_bound, uint32_t* s_rr)
 {
+    using rec_t = typename std::conditional<ISO, uint2, uint4>::type;
+    rec_t* const s_rec = reinterpret_cast<rec_t*>(s_lds);
     if (threadIdx.x < MAX_TYPES * MAX_TYPES)
-        s_rr[threadIdx.x] = ISO ? p.rr_cut_skin_q[threadIdx.x] : p.rr_cut_skin[threadIdx.x];
+        s_rr[threadIdx.x] = ISO ? p.rr16[threadIdx.x] : __float_as_uint(p.rr_cut_skin[threadIdx.x]);
     unsigned const U = stage_union(sp, seg, off, posq, s_rec, st, s_bound);
     if (threadIdx.x == 0) atomicMax(&stats[1], U + 1);
     if (U + 1 > sp.lds_cap) return;   // uniform: the caller enlarges the staging and rebuilds
 
     unsigned const nx = p.grid.ncell[0];
-    float const rr_cut0 = ISO ? p.rr_cut_skin_q[0] : p.rr_cut_skin[0];
+    uint32_t const rr0 = ISO ? p.rr16[0] : __float_as_uint(p.rr_cut_skin[0]);
     unsigned const hh = st.q.h + 2, T = st.own_pre[st.q.h];
+    // the segment's particles follow the first own row in memory (its rows
+    // y0 .. y0 + h - 1 do not wrap), so the list offsets from the lane block
+    // of its first particle fit 32 bits (particles of one z-plane of cells)
+    unsigned const i0 = st.own_begin[0] & ~(WAVE - 1);
+    char* const wlist = reinterpret_cast<char*>(list + size_t(i0) * cap);
     unsigned cmax = 0;
     for (unsigned t = threadIdx.x; t < T; t += blockDim.x) {
         unsigned j, i;
@@ -325,13 +359,19 @@ __device__ __forceinline__ void staged_build_segment(
         unsigned const self = hh + j + 1;   // union row of the particle's own (y, z) row
         nb16_state s;
         s.own = s_bound[self * SEG_BOUND + 1] + (t - st.own_pre[j]);
-        s.ri = s_rec[s.own];
-        s.ti = (s.ri.w & 3u) * p.ntype;
-        s.c = 0;
-        s.cap = cap;
-        s.list = list;
-        s.base = (i & ~(WAVE - 1)) * cap + (i & (WAVE - 1)) * NB16_BLOCK;
-        unsigned const k = cell_component(s.ri.x, nx) - st.q.x0;   // 0 .. w - 1
+        uint4 const full = posq[i];          // = the staged record s.own, in full
+        if (ISO) {
+            uint2 const pk = pack_rec16(full);
+            s.ri = make_uint4(pk.x, pk.y, 0u, 0u);
+        } else {
+            s.ri = full;
+        }
+        s.ti = (full.w & 3u) * p.ntype;
+        s.c2 = 0;
+        s.last2 = 2 * (cap - 1);
+        s.list = wlist;
+        s.base2 = 2 * ((i & ~(WAVE - 1)) - i0) * cap + (i & (WAVE - 1)) * (2 * NB16_BLOCK);
+        unsigned const k = cell_component(full.x, nx) - st.q.x0;   // 0 .. w - 1
         // the 9 rows (dz, dy) in lexicographic order; only the own row holds
         // the particle itself
         for (unsigned dz = 0; dz < 3; ++dz)
@@ -340,14 +380,14 @@ __device__ __forceinline__ void staged_build_segment(
                 unsigned const b = s_bound[ur * SEG_BOUND + k];
                 unsigned const e = s_bound[ur * SEG_BOUND + k + 3];
                 if (ur == self)
-                    scan16<MULTI, ISO, true>(s, b, e, s_rec, p, s_rr, rr_cut0);
+                    scan16<MULTI, true>(s, b, e, s_rec, p, s_rr, rr0);
                 else
-                    scan16<MULTI, ISO, false>(s, b, e, s_rec, p, s_rr, rr_cut0);
+                    scan16<MULTI, false>(s, b, e, s_rec, p, s_rr, rr0);
             }
-        unsigned const c = s.c;
+        unsigned const c = s.c2 / 2;
         // pad with the sentinel to a multiple of 8 (cap is a multiple of 16)
         if (c <= cap)
-            while (s.c % 8) append16(s, U * 16u);
+            while (s.c2 % 16) append16(s, U * 16u);
         count[i] = c;
         cmax = max(cmax, c);
     }
@@ -363,7 +403,7 @@ __global__ __launch_bounds__(SEG_THREADS_MAX) void build_staged_kernel(
     extern __shared__ uint4 s_rec[];
     __shared__ seg_stage st;
     __shared__ unsigned s_bound[SEG_NR * SEG_BOUND];
-    __shared__ float s_rr[MAX_TYPES * MAX_TYPES];
+    __shared__ uint32_t s_rr[MAX_TYPES * MAX_TYPES];
 
     unsigned const seg = seg_of_block(sp, blockIdx.x);
     if (seg >= sp.nseg) return;   // the whole workgroup
@@ -482,7 +522,7 @@ __global__ __launch_bounds__(SEG_THREADS_MAX) void ens_build_kernel(
     extern __shared__ uint4 s_rec[];
     __shared__ seg_stage st;
     __shared__ unsigned s_bound[SEG_NR * SEG_BOUND];
-    __shared__ float s_rr[MAX_TYPES * MAX_TYPES];
+    __shared__ uint32_t s_rr[MAX_TYPES * MAX_TYPES];
     uint32_t const e = rseg[blockIdx.x];
     unsigned const sys = e >> 16, seg = e & 0xffffu;
     ens_desc const& d = descs[sys];
@@ -564,7 +604,7 @@ void launch_build_neighbours_staged(uint4 const* posq, uint32_t const* offset, s
                                     neighbour_params const& p, unsigned cap, uint16_t* list, uint32_t* count,
                                     uint32_t* stats, hipStream_t s) {
     if (!sp.nseg) return;
-    size_t const lds = size_t(sp.lds_cap) * sizeof(uint4);
+    size_t const lds = size_t(sp.lds_cap) * (p.iso ? sizeof(uint2) : sizeof(uint4));
     unsigned const grid = SEG_XCD * sp.chunk;
     if (p.ntype > 1) {
         if (p.iso)
@@ -639,7 +679,7 @@ void launch_ens_build(uint4 const* posq, uint32_t const* offsets, ens_desc const
                       unsigned nrseg, unsigned threads, unsigned lds_records, bool multi, uint16_t* list,
                       uint32_t* count, uint32_t* stats, hipStream_t s) {
     if (!nrseg) return;
-    size_t const lds = size_t(lds_records) * sizeof(uint4);
+    size_t const lds = size_t(lds_records) * sizeof(uint2);   // cubic boxes: packed records
     if (multi)
         ens_build_kernel<true><<<nrseg, threads, lds, s>>>(posq, offsets, descs, rseg, list, count, stats);
     else

@@ -177,6 +177,7 @@ void ensemble::prepare(sys_host& h) {
         // the single-system rule: (rc_ab + skin)^2, and in fixed-point units
         d.np.rr_cut_skin[ab] = float(std::pow(h.pot->r_cut[ab] + h.skin, 2));
         d.np.rr_cut_skin_q[ab] = float(double(d.np.rr_cut_skin[ab]) / (double(d.np.q[0]) * double(d.np.q[0])));
+        d.np.rr16[ab] = list_threshold16(h.pot->r_cut[ab] + h.skin, b.L[0]);
     }
     split_ff(h.dt, d.ip.dt_hi, d.ip.dt_lo);
     d.ip.half_dt = float(h.dt / 2);

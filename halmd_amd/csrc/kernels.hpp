@@ -68,7 +68,20 @@ struct neighbour_params {
     // rq = fma(dz, dz, fma(dy, dy, dx * dx)) <= float(rr_cut_skin / q^2)
     unsigned iso;
     float rr_cut_skin_q[MAX_TYPES * MAX_TYPES];
+    // cubic box, cell-segment build: integer predicate on the top 16 bits of
+    // each fixed-point coordinate, dx^2 + dy^2 + dz^2 <= rr16 with d the
+    // wrapped 16-bit difference (exact); see list_threshold16
+    uint32_t rr16[MAX_TYPES * MAX_TYPES];
 };
+// rr16 for the distance r_cut + r_skin in a cubic box of edge L: the 16-bit
+// differences are within 1 of the exact differences in units of L / 2^16, so
+// the 16-bit distance is within sqrt(3) of the exact one, and every pair at an
+// exact distance <= r_cut + r_skin passes (pairs up to 2 sqrt(3) units beyond
+// may pass as well); the margin over sqrt(3) absorbs the FP64 rounding here
+inline uint32_t list_threshold16(double r_cut_skin, double L) {
+    double const r = r_cut_skin * 65536.0 / L + 1.7330508075688772;
+    return uint32_t(r * r);
+}
 // full neighbour lists, blocked layout: entry k of particle i at
 // (i / 64) * 64 * cap + (k / NB_BLOCK) * 64 * NB_BLOCK + (i % 64) * NB_BLOCK + k % NB_BLOCK
 // (cap a multiple of NB_BLOCK); *max_count receives the largest per-particle

@@ -593,9 +593,11 @@ neighbour::neighbour(particle& p_, binning* bin_, max_displacement& disp_, box c
         rc_max = std::max(rc_max, r_cut[i]);
     }
     params.iso = params.q[0] == params.q[1] && params.q[1] == params.q[2];
-    for (size_t i = 0; i < r_cut.size(); ++i)
+    for (size_t i = 0; i < r_cut.size(); ++i) {
         params.rr_cut_skin_q[i] =
             float(double(params.rr_cut_skin[i]) / (double(params.q[0]) * double(params.q[0])));
+        params.rr16[i] = list_threshold16(r_cut[i] + r_skin, b.L[0]);
+    }
     // initial capacity from the mean number of neighbours within rc + skin at
     // the given occupancy (from_binning.cpp:96-119), rounded to 8; grown on demand
     double const rho = double(p.n) / b.volume();

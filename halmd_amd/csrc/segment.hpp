@@ -49,10 +49,21 @@ __device__ __forceinline__ unsigned wave_inclusive_sum(unsigned x) {
     return x;
 }
 
+// Staged record formats: the full fixed-point record (pair forces; non-cubic
+// list build), or for the cubic-box list build the top 16 bits of the x and y
+// coordinates in one word (x low) and of z with the species above them in the
+// other.
+__device__ __forceinline__ uint2 pack_rec16(uint4 const r) {
+    return make_uint2((r.x >> 16) | (r.y & 0xffff0000u), (r.z >> 16) | ((r.w & 3u) << 16));
+}
+__device__ __forceinline__ void put_rec(uint4* s, unsigned k, uint4 const r) { s[k] = r; }
+__device__ __forceinline__ void put_rec(uint2* s, unsigned k, uint4 const r) { s[k] = pack_rec16(r); }
+
 // Called by the whole workgroup (at least 64 threads).  Fills st (and, if
 // bound is given, the LDS start of each of the w + 2 cells of every union row:
 // bound[ur * SEG_BOUND + k], k = 0 .. w + 2), then, if U + 1 <= sp.lds_cap,
-// copies the union into s_rec[0 .. U) and a sentinel record into s_rec[U]:
+// copies the union into s_rec[0 .. U) (in the format of Rec) and a sentinel
+// record into s_rec[U]:
 // fixed-point y and z half a box from the centre of the segment's rows and
 // plane, hence at least one cell width (>= rc + skin) from each of its
 // particles in both (h <= ny - 2, nz >= 3).  Returns the union size U; every
@@ -62,8 +73,9 @@ __device__ __forceinline__ unsigned wave_inclusive_sum(unsigned x) {
 // lanes 32 .. 32 + h - 1 the own rows) and forms both prefix sums by a wave
 // scan; then wave v copies union rows v, v + W, ... (W waves), each in chunks
 // of four 64-record loads per lane issued before their stores.
+template <class Rec>
 __device__ __forceinline__ unsigned stage_union(seg_params const& sp, unsigned seg, uint32_t const* __restrict__ off,
-                                                uint4 const* __restrict__ posq, uint4* s_rec, seg_stage& st,
+                                                uint4 const* __restrict__ posq, Rec* s_rec, seg_stage& st,
                                                 unsigned* bound) {
     unsigned const tid = threadIdx.x, lane = tid % WAVE;
     if (tid < WAVE) {
@@ -111,24 +123,26 @@ __device__ __forceinline__ unsigned stage_union(seg_params const& sp, unsigned s
             unsigned const ny = sp.grid.ncell[1], nz = sp.grid.ncell[2];
             uint32_t const cy = uint32_t(((2ull * q.y0 + q.h) << 31) / ny);   // (y0 + h/2) 2^32 / ny
             uint32_t const cz = uint32_t(((2ull * q.z + 1) << 31) / nz);
-            s_rec[U] = make_uint4(0u, cy + 0x80000000u, cz + 0x80000000u, 0u);
+            put_rec(s_rec, U, make_uint4(0u, cy + 0x80000000u, cz + 0x80000000u, 0u));
         }
         unsigned const W = blockDim.x / WAVE;
         for (unsigned r = tid / WAVE; r < nr; r += W) {   // wave-uniform
             unsigned const b = st.base[r], g0 = st.g[r][0], n0 = st.n[r][0], g1 = st.g[r][1];
             unsigned const n = n0 + st.n[r][1];
             for (unsigned u0 = 0; u0 < n; u0 += 4 * WAVE) {
-                // slots at or past n read record 0 and store nothing
+                // slots at or past n read record 0 (one cached line) and
+                // store nothing; the four loads are unconditional, so that
+                // they issue back to back
                 auto src = [&](unsigned u) { return u < n ? (u < n0 ? g0 + u : g1 + (u - n0)) : 0u; };
                 unsigned const u = u0 + lane;
                 uint4 const a0 = posq[src(u)];
-                uint4 const a1 = u0 + WAVE < n ? posq[src(u + WAVE)] : a0;
-                uint4 const a2 = u0 + 2 * WAVE < n ? posq[src(u + 2 * WAVE)] : a0;
-                uint4 const a3 = u0 + 3 * WAVE < n ? posq[src(u + 3 * WAVE)] : a0;
-                if (u < n) s_rec[b + u] = a0;
-                if (u + WAVE < n) s_rec[b + u + WAVE] = a1;
-                if (u + 2 * WAVE < n) s_rec[b + u + 2 * WAVE] = a2;
-                if (u + 3 * WAVE < n) s_rec[b + u + 3 * WAVE] = a3;
+                uint4 const a1 = posq[src(u + WAVE)];
+                uint4 const a2 = posq[src(u + 2 * WAVE)];
+                uint4 const a3 = posq[src(u + 3 * WAVE)];
+                if (u < n) put_rec(s_rec, b + u, a0);
+                if (u + WAVE < n) put_rec(s_rec, b + u + WAVE, a1);
+                if (u + 2 * WAVE < n) put_rec(s_rec, b + u + 2 * WAVE, a2);
+                if (u + 3 * WAVE < n) put_rec(s_rec, b + u + 3 * WAVE, a3);
             }
         }
     }

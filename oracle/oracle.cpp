@@ -833,6 +833,60 @@ int oracle_fixed_neighbour_list(int n, uint32_t const* u, unsigned const* specie
     return maxc;
 }
 
+// Cubic-box rule of the cell-segment list build: the top 16 bits of each
+// fixed-point coordinate, their wrapped 16-bit differences d (minimum image
+// in units of L / 2^16) and the exact integer dx^2 + dy^2 + dz^2 against
+// floor((r_cut_skin 2^16 / L + sqrt(3) + 0.001)^2): each d is within 1 of the
+// exact difference in those units, so every pair within r_cut_skin of exact
+// distance passes.  Same traversal (27 cells, dz, dy, dx; particles in memory
+// order within a cell) and output as oracle_fixed_neighbour_list; r_cut_skin is
+// ntype x ntype (FP64).  No counterpart in the reference, whose rule is the
+// FP32 one of oracle_fixed_neighbour_list.
+int oracle_fixed_neighbour_list16(int n, uint32_t const* u, unsigned const* species, unsigned const* ncell, double L,
+                                  int ntype, double const* r_cut_skin, int cap, unsigned* counts, unsigned* list)
+{
+    size_t ncells = size_t(ncell[0]) * ncell[1] * ncell[2];
+    std::vector<unsigned> cell(n);
+    oracle_fixed_cell_index(n, u, ncell, cell.data());
+    std::vector<std::vector<unsigned>> cells(ncells);
+    for (int i = 0; i < n; ++i) cells[cell[i]].push_back(unsigned(i));
+    std::vector<uint64_t> thr(size_t(ntype) * ntype);
+    for (size_t k = 0; k < thr.size(); ++k) {
+        double const r = r_cut_skin[k] * 65536.0 / L + std::sqrt(3.0) + 0.001;
+        thr[k] = uint64_t(std::floor(r * r));
+    }
+    int maxc = 0;
+    int const nx = int(ncell[0]), ny = int(ncell[1]), nz = int(ncell[2]);
+    for (int i = 0; i < n; ++i) {
+        uint32_t const* ui = u + 3 * size_t(i);
+        unsigned const ti = species[i];
+        unsigned c = cell[i];
+        int cx = int(c % ncell[0]), cy = int((c / ncell[0]) % ncell[1]), cz = int(c / ncell[0] / ncell[1]);
+        unsigned count = 0;
+        for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+            auto const& cl = cells[(size_t((cz + nz + dz) % nz) * ny + (cy + ny + dy) % ny) * nx + (cx + nx + dx) % nx];
+            for (unsigned j : cl) {
+                if (j == unsigned(i)) continue;
+                uint32_t const* uj = u + 3 * size_t(j);
+                uint64_t rq = 0;
+                for (int k = 0; k < 3; ++k) {
+                    int64_t const d = int16_t(uint16_t((ui[k] >> 16) - (uj[k] >> 16)));
+                    rq += uint64_t(d * d);
+                }
+                if (rq <= thr[ti * ntype + species[j]]) {
+                    if (int(count) < cap) list[size_t(i) * cap + count] = j;
+                    count++;
+                }
+            }
+        }
+        counts[i] = count;
+        maxc = std::max(maxc, int(count));
+    }
+    return maxc;
+}
+
 // Hilbert code rule of the device sort (halmd/mdsim/gpu/sorts/hilbert_kernel.cu:38-69):
 // r / L per component in FP32, then the map of sorts/hilbert_kernel.hpp in FP32.
 void oracle_fp32_hilbert_code(int n, float const* pos4, float const* L, unsigned depth, unsigned* code)

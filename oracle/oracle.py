@@ -53,6 +53,7 @@ def lib():
             "oracle_fixed_point": (None, [ctypes.c_int, _d, _d, _u]),
             "oracle_fixed_cell_index": (None, [ctypes.c_int, _u, _u, _u]),
             "oracle_fixed_neighbour_list": (ctypes.c_int, [ctypes.c_int, _u, _u, _u, _f, ctypes.c_int, _f, ctypes.c_int, _u, _u]),
+            "oracle_fixed_neighbour_list16": (ctypes.c_int, [ctypes.c_int, _u, _u, _u, ctypes.c_double, ctypes.c_int, _d, ctypes.c_int, _u, _u]),
             "oracle_all_pairs_neighbour_list": (ctypes.c_int, [ctypes.c_int, _u, _u, _f, ctypes.c_int, _f, ctypes.c_int, _u, _u]),
             "oracle_fp32_hilbert_code": (None, [ctypes.c_int, _f, _f, ctypes.c_uint, _u]),
             "oracle_hilbert_depth": (ctypes.c_uint, [_d]),
@@ -237,6 +238,25 @@ def fixed_neighbour_list(u, species, ncell, L, rr_cut_skin, cap):
     lst = np.zeros(n * cap, dtype=np.uint32)
     m = lib().oracle_fixed_neighbour_list(n, _p(u, _u), _p(sp, _u), _p(nc, _u), _p(q, _f), ntype, _p(rr, _f), cap,
                                           _p(counts, _u), _p(lst, _u))
+    return counts, lst.reshape(n, cap), m
+
+
+def staged_neighbour_list(u, species, ncell, L, r_cut_skin, cap):
+    """Lists of the cell-segment build: the 16-bit integer rule in a cubic box,
+    the FP32 rule of fixed_neighbour_list otherwise.  r_cut_skin: ntype x ntype
+    distances rc + skin (FP64).  (counts, lists[n, cap], max count)."""
+    L = np.broadcast_to(np.asarray(L, dtype=np.float64), (3,))
+    rcs = np.ascontiguousarray(np.atleast_2d(r_cut_skin), dtype=np.float64)
+    if not (L[0] == L[1] == L[2]):
+        return fixed_neighbour_list(u, species, ncell, L, (rcs ** 2).astype(np.float32), cap)
+    u = np.ascontiguousarray(u, dtype=np.uint32)
+    sp = np.ascontiguousarray(species, dtype=np.uint32)
+    nc = np.ascontiguousarray(ncell, dtype=np.uint32)
+    n = len(u)
+    counts = np.zeros(n, dtype=np.uint32)
+    lst = np.zeros(n * cap, dtype=np.uint32)
+    m = lib().oracle_fixed_neighbour_list16(n, _p(u, _u), _p(sp, _u), _p(nc, _u), float(L[0]), rcs.shape[0],
+                                            _p(rcs, _d), cap, _p(counts, _u), _p(lst, _u))
     return counts, lst.reshape(n, cap), m
 
 

@@ -69,14 +69,14 @@ def test_neighbour_lists_bit_exact(oracle, kind):
         w = workloads.kob_andersen(8788, density=1.2, jitter=0.15, seed=12)
     w.sampler.run(37)    # positions written by the integrate kernel
     ncell = w.neighbour.binning.ncell
-    rcs = ((w.potential.r_cut + w.skin) ** 2).astype(np.float32)
+    rcs = w.potential.r_cut + w.skin
     # rebuild from the current positions (which reorders the particles into
     # cell order), then compare with the rule on them in that memory order
     w.neighbour.update()
     u = w.particle.data["position_fixed"]
     species = w.particle.data["species"]
     counts, entries = w.neighbour.lists()
-    ocounts, olists, m = oracle.fixed_neighbour_list(u, species, ncell, w.length, rcs, entries.shape[1])
+    ocounts, olists, m = oracle.staged_neighbour_list(u, species, ncell, w.length, rcs, entries.shape[1])
     assert m <= entries.shape[1]
     assert np.array_equal(counts, ocounts)
     for i in range(len(counts)):
@@ -101,8 +101,8 @@ def test_neighbour_list_overflow_grows(oracle):
     counts, entries = nb.lists()
     assert nb.size >= counts.max() > small * 0.5
     u = p.data["position_fixed"]
-    ocounts, olists, _ = oracle.fixed_neighbour_list(u, p.data["species"], nb.binning.ncell, L,
-                                                     np.array([[2.8 ** 2]], np.float32), entries.shape[1])
+    ocounts, olists, _ = oracle.staged_neighbour_list(u, p.data["species"], nb.binning.ncell, L, [[2.8]],
+                                                      entries.shape[1])
     assert np.array_equal(counts, ocounts)
     assert np.array_equal(entries, olists)
 
@@ -154,8 +154,8 @@ def test_autotune_keeps_parity(oracle):
     w.neighbour.update()
     counts, entries = w.neighbour.lists()
     u = w.particle.data["position_fixed"]
-    ocounts, olists, _ = oracle.fixed_neighbour_list(u, w.particle.data["species"], w.neighbour.binning.ncell,
-                                                     w.length, np.array([[2.8 ** 2]], np.float32), entries.shape[1])
+    ocounts, olists, _ = oracle.staged_neighbour_list(u, w.particle.data["species"], w.neighbour.binning.ncell,
+                                                      w.length, [[2.8]], entries.shape[1])
     assert np.array_equal(counts, ocounts)
     assert np.array_equal(entries, olists)
 
@@ -212,8 +212,8 @@ def test_clustered_configuration_restages(oracle, fraction):
     f = p.ordered("force")
     counts, entries = nb.lists()
     u = p.data["position_fixed"]
-    ocounts, olists, _ = oracle.fixed_neighbour_list(u, p.data["species"], nb.binning.ncell, L,
-                                                     np.array([[2.8 ** 2]], np.float32), entries.shape[1])
+    ocounts, olists, _ = oracle.staged_neighbour_list(u, p.data["species"], nb.binning.ncell, L, [[2.8]],
+                                                      entries.shape[1])
     assert np.array_equal(counts, ocounts)
     assert np.array_equal(entries, olists)
     opot = oracle.Potential(oracle.POT_LJ, "shifted", 1.0, 1.0, 2.5)

@@ -530,3 +530,50 @@ def test_mqd_ballistic_and_vacf_alternating(oracle):
     # MSD from the same position samples is unchanged by the extra correlations
     sc, sm, _ = c.result("msd")
     assert np.array_equal(sc, mc)
+
+
+@pytest.mark.parametrize("n,rho,L_override", [(2048, 0.8442, None), (1500, 1.2, None), (400, 0.9, 8.5)])
+def test_staged_list_rule16_contains_the_sphere(oracle, n, rho, L_override):
+    # cubic-box rule of the cell-segment build (16-bit integer distance):
+    # checked against exact integer geometry, not a restatement of the formula.
+    # (1) every pair whose exact fixed-point distance is within rc + skin is in
+    # the list; (2) no listed pair is farther than rc + skin + 2 (sqrt 3 +
+    # 0.001) units of L / 2^16; (3) the set equals the predicate evaluated on all
+    # pairs by brute force (the cell traversal misses no cell); (4) symmetric.
+    # The small box (3 cells per side) exercises the wrapped 16-bit differences.
+    pos, L, species, _ = _liquid(oracle, n, rho, 9, species_frac=0.3)
+    if L_override is not None:
+        pos = pos * (L_override / L[0])
+        L = np.full(3, L_override)
+    pot = _ka_potential(oracle)
+    skin = 0.3
+    rcs = pot.r_cut + skin
+    ncell = np.floor(L / np.max(rcs)).astype(np.uint32)
+    assert ncell.min() >= 3
+    u = oracle.fixed_point(pos, L)
+    counts, lst, m = oracle.staged_neighbour_list(u, species, ncell, L, rcs, 512)
+    assert m <= 512
+    got = np.zeros((n, n), bool)
+    for i in range(n):
+        got[i, lst[i, :counts[i]]] = True
+        assert len(np.unique(lst[i, :counts[i]])) == counts[i]
+    assert np.array_equal(got, got.T)
+    # exact distances of the fixed-point positions (minimum image), FP64
+    D = (u[:, None, :] - u[None, :, :]).astype(np.int32).astype(np.float64) * (L[0] / 2 ** 32)
+    r = np.sqrt(np.sum(D * D, -1))
+    cut = rcs[species[:, None], species[None, :]]
+    np.fill_diagonal(r, np.inf)
+    assert np.all(got[r <= cut])
+    unit = L[0] / 65536
+    assert np.all(r[got] <= cut[got] + 2 * (np.sqrt(3) + 0.001) * unit)
+    # brute force over all pairs with numpy's integer arithmetic
+    h = (u >> 16).astype(np.int64)
+    d = (h[:, None, :] - h[None, :, :]) % 65536
+    d = np.where(d >= 32768, d - 65536, d)
+    rq = np.sum(d * d, -1)
+    thr = np.floor((rcs * 65536 / L[0] + np.sqrt(3) + 0.001) ** 2)
+    expect = rq <= thr[species[:, None], species[None, :]]
+    np.fill_diagonal(expect, False)
+    assert np.array_equal(got, expect)
+    # the margin is thin: the list exceeds the exact sphere by well under 1 %
+    assert got.sum() < 1.01 * np.sum(r <= cut)
