@@ -344,9 +344,41 @@ __device__ __forceinline__ void staged_build_segment(
     if (threadIdx.x == 0) atomicMax(&stats[1], U + 1);
     if (U + 1 > sp.lds_cap) return;   // uniform: the caller enlarges the staging and rebuilds
 
-    unsigned const nx = p.grid.ncell[0];
+    unsigned const nx = p.grid.ncell[0], ny = p.grid.ncell[1], nz = p.grid.ncell[2];
     uint32_t const rr0 = ISO ? p.rr16[0] : __float_as_uint(p.rr_cut_skin[0]);
     unsigned const hh = st.q.h + 2, T = st.own_pre[st.q.h];
+    // Cubic box: faces of the union's cells in 16-bit units, to skip the cells
+    // of a row that no candidate of the rule can reach.  Face f of an axis
+    // with n cells, below cell c = f mod n, lies at lo = ceil(c 2^32 / n), the
+    // lowest fixed-point coordinate of cell c: its word holds lo >> 16 (the
+    // lowest top half inside c) above (lo - 1) >> 16 (the highest below c).
+    // x faces of union cells x0 - 1 .. x0 + w: 0 .. w + 2; y faces of rows
+    // y0 - 1 .. y0 + h: w + 3 ..; z faces below and above the plane: last two.
+    __shared__ uint32_t s_face[(SEG_MAX + 3) + (SEG_RMAX + 3) + 2];
+    unsigned const fx = st.q.w + 3, fy = st.q.h + 3;
+    if (ISO) {
+        for (unsigned f = threadIdx.x; f < fx + fy + 2; f += blockDim.x) {
+            unsigned c, n;
+            if (f < fx) {
+                c = st.q.x0 + nx - 1 + f;
+                n = nx;
+            } else if (f < fx + fy) {
+                c = st.q.y0 + ny - 1 + (f - fx);
+                n = ny;
+            } else {
+                c = st.q.z + (f - fx - fy);
+                n = nz;
+            }
+            c %= n;
+            uint32_t const lo = uint32_t(((uint64_t(c) << 32) + n - 1) / n);
+            s_face[f] = (lo & 0xffff0000u) | (((lo - 1u) >> 16) & 0xffffu);
+        }
+        __syncthreads();
+    }
+    // largest threshold over the species pairs (the skip test is conservative)
+    uint32_t rr_max = rr0;
+    if (MULTI)
+        for (unsigned ab = 1; ab < p.ntype * p.ntype; ++ab) rr_max = max(rr_max, s_rr[ab]);
     // the segment's particles follow the first own row in memory (its rows
     // y0 .. y0 + h - 1 do not wrap), so the list offsets from the lane block
     // of its first particle fit 32 bits (particles of one z-plane of cells)
@@ -372,13 +404,48 @@ __device__ __forceinline__ void staged_build_segment(
         s.list = wlist;
         s.base2 = 2 * ((i & ~(WAVE - 1)) - i0) * cap + (i & (WAVE - 1)) * (2 * NB16_BLOCK);
         unsigned const k = cell_component(full.x, nx) - st.q.x0;   // 0 .. w - 1
+        // Cubic box: distances (16-bit units) from the particle's top halves
+        // to the faces of its cell, each a lower bound of |d| of the rule for
+        // every candidate beyond that face.  They hold for the minimum image
+        // when the candidate lies within two cells (< L / 2), i.e. with at
+        // least 5 cells along the axis; with fewer the bound is 0.
+        unsigned axl = 0, axh = 0, ayl = 0, ayh = 0, azl = 0, azh = 0;
+        if (ISO) {
+            unsigned const hx = s.ri.x & 0xffffu, hy = s.ri.x >> 16, hz = s.ri.y & 0xffffu;
+            if (nx >= 5) {
+                axl = (hx - (s_face[k + 1] & 0xffffu)) & 0xffffu;
+                axh = ((s_face[k + 2] >> 16) - hx) & 0xffffu;
+            }
+            if (ny >= 5) {
+                ayl = (hy - (s_face[fx + j + 1] & 0xffffu)) & 0xffffu;
+                ayh = ((s_face[fx + j + 2] >> 16) - hy) & 0xffffu;
+            }
+            if (nz >= 5) {
+                azl = (hz - (s_face[fx + fy] & 0xffffu)) & 0xffffu;
+                azh = ((s_face[fx + fy + 1] >> 16) - hz) & 0xffffu;
+            }
+        }
         // the 9 rows (dz, dy) in lexicographic order; only the own row holds
         // the particle itself
         for (unsigned dz = 0; dz < 3; ++dz)
             for (unsigned dy = 0; dy < 3; ++dy) {
                 unsigned const ur = dz * hh + j + dy;
-                unsigned const b = s_bound[ur * SEG_BOUND + k];
-                unsigned const e = s_bound[ur * SEG_BOUND + k + 3];
+                unsigned kb = k, ke = k + 3;
+                if (ISO) {
+                    // skip the row's end cells (or the whole row) when even
+                    // their nearest face is beyond every threshold
+                    unsigned const my = dy == 0 ? ayl : (dy == 2 ? ayh : 0u);
+                    unsigned const mz = dz == 0 ? azl : (dz == 2 ? azh : 0u);
+                    unsigned const ryz = my * my + mz * mz;
+                    if (ryz > rr_max) {
+                        ke = kb;
+                    } else {
+                        if (axl * axl + ryz > rr_max) kb = k + 1;
+                        if (axh * axh + ryz > rr_max) ke = k + 2;
+                    }
+                }
+                unsigned const b = s_bound[ur * SEG_BOUND + kb];
+                unsigned const e = kb == ke ? b : s_bound[ur * SEG_BOUND + ke];
                 if (ur == self)
                     scan16<MULTI, true>(s, b, e, s_rec, p, s_rr, rr0);
                 else

@@ -235,6 +235,25 @@ HALMD_API int halmd_mdsim_potentials_pair_morse_create(unsigned nt1, unsigned nt
         *out = handle<halmd_potential>(new potential(POTENTIAL_MORSE, nt1, nt2, eps, sigma, r_min, distortion));
     });
 }
+HALMD_API int halmd_mdsim_potentials_pair_power_law_create(unsigned nt1, unsigned nt2, const double* eps,
+                                                           const double* sigma, const unsigned* index,
+                                                           halmd_potential** out) {
+    return guard([&] {
+        nonnull(eps, "epsilon"); nonnull(sigma, "sigma"); nonnull(index, "index"); nonnull(out, "out");
+        *out = handle<halmd_potential>(
+            new potential(POTENTIAL_POWER_LAW, nt1, nt2, eps, sigma, nullptr, nullptr, index, nullptr));
+    });
+}
+HALMD_API int halmd_mdsim_potentials_pair_mie_create(unsigned nt1, unsigned nt2, const double* eps,
+                                                     const double* sigma, const unsigned* index_m,
+                                                     const unsigned* index_n, halmd_potential** out) {
+    return guard([&] {
+        nonnull(eps, "epsilon"); nonnull(sigma, "sigma"); nonnull(index_m, "index_m");
+        nonnull(index_n, "index_n"); nonnull(out, "out");
+        *out = handle<halmd_potential>(
+            new potential(POTENTIAL_MIE, nt1, nt2, eps, sigma, nullptr, nullptr, index_m, index_n));
+    });
+}
 HALMD_API int halmd_mdsim_potentials_pair_truncate(halmd_potential* p, const char* truncation, const double* cutoff,
                                                    double smoothing) {
     return guard([&] {

@@ -59,10 +59,30 @@ __device__ __forceinline__ float separation_scale(float x, float q_hi, float q_l
     return __builtin_fmaf(x, q_hi, x * q_lo);
 }
 
+// x^(k/2) for an integer k: binary exponentiation, times sqrt(x) for odd k
+__device__ __forceinline__ float pow_half(float x, unsigned k) {
+    float r = (k & 1u) ? sqrtf(x) : 1.0f;
+    for (unsigned e = k >> 1; e; e >>= 1) {
+        if (e & 1u) r *= x;
+        x *= x;
+    }
+    return r;
+}
+
 template <int KIND, int TRUNC>
 __device__ __forceinline__ void pair_eval(float rr, float morse_d, pair_param const& pp, float rri_smooth,
                                           float& fval, float& en) {
-    if (KIND == POTENTIAL_LENNARD_JONES) {
+    if (KIND == POTENTIAL_POWER_LAW) {
+        // power law (power_law_kernel.hpp:46-60) and Mie (mie_kernel.hpp:48-62)
+        // in one form: U = c_m (sigma/r)^m - c_n (sigma/r)^n,
+        // fval = (m c_m (sigma/r)^m - n c_n (sigma/r)^n) / r^2 (c_n = n = 0 for the power law)
+        float const rri = pp.p[2] * rcp_nr(rr);
+        unsigned const m = unsigned(pp.p_lo[0]), n = unsigned(pp.p_lo[1]);
+        float const a = pp.p[0] * pow_half(rri, m);
+        float const b = pp.p[1] * pow_half(rri, n);
+        fval = (pp.p_lo[0] * a - pp.p_lo[1] * b) * rcp_nr(rr);
+        en = a - b;
+    } else if (KIND == POTENTIAL_LENNARD_JONES) {
         // rri = sigma^2 / rr; fval = 48 eps rri ri6 (ri6 - 1/2) / sigma^2; U = 4 eps ri6 (ri6 - 1)
         float const rri = pp.p[2] * rcp_nr(rr);
         float const ri6 = rri * rri * rri;
@@ -624,6 +644,8 @@ void launch_pair_force(uint4 const* posq, unsigned n, uint32_t const* list, uint
         dispatch_trunc<POTENTIAL_LENNARD_JONES>(a);
     else if (p.kind == POTENTIAL_MORSE)
         dispatch_trunc<POTENTIAL_MORSE>(a);
+    else if (p.kind == POTENTIAL_POWER_LAW)
+        dispatch_trunc<POTENTIAL_POWER_LAW>(a);
     else
         throw error(HALMD_ERR_INVALID_ARGUMENT, "unknown potential");
     HIP_CHECK(hipGetLastError());
@@ -638,6 +660,8 @@ void launch_pair_force_staged(uint4 const* posq, uint32_t const* offset, seg_par
         staged_trunc<POTENTIAL_LENNARD_JONES>(a);
     else if (p.kind == POTENTIAL_MORSE)
         staged_trunc<POTENTIAL_MORSE>(a);
+    else if (p.kind == POTENTIAL_POWER_LAW)
+        staged_trunc<POTENTIAL_POWER_LAW>(a);
     else
         throw error(HALMD_ERR_INVALID_ARGUMENT, "unknown potential");
     HIP_CHECK(hipGetLastError());

@@ -315,10 +315,10 @@ void particle::rearrange(uint32_t const* perm) {
 // potential
 // ---------------------------------------------------------------------------
 potential::potential(int kind_, unsigned nt1, unsigned nt2, double const* eps, double const* sig, double const* rmin,
-                     double const* dist)
+                     double const* dist, unsigned const* im, unsigned const* in)
     : kind(kind_), ntype1(nt1), ntype2(nt2)
 {
-    require(kind == POTENTIAL_LENNARD_JONES || kind == POTENTIAL_MORSE, "potential: unknown kind");
+    require(kind >= POTENTIAL_LENNARD_JONES && kind <= POTENTIAL_MIE, "potential: unknown kind");
     require(nt1 >= 1 && nt1 <= MAX_TYPES && nt2 >= 1 && nt2 <= MAX_TYPES, "potential: 1 <= ntype <= 4");
     size_t const n2 = size_t(nt1) * nt2;
     epsilon.assign(eps, eps + n2);
@@ -333,6 +333,38 @@ potential::potential(int kind_, unsigned nt1, unsigned nt2, double const* eps, d
             require(distortion[i] > 0 && 2 * distortion[i] * distortion[i] - 1 != 0, "morse: invalid distortion");
         }
     }
+    index_m.assign(n2, 0);
+    index_n.assign(n2, 0);
+    epsilon_c = epsilon;
+    if (kind == POTENTIAL_POWER_LAW) {
+        // host/potentials/pair/power_law.cpp
+        require(im != nullptr, "power_law: index required");
+        for (size_t i = 0; i < n2; ++i) {
+            require(im[i] >= 1 && im[i] <= 64, "power_law: index must be in 1 .. 64");
+            index_m[i] = im[i];
+        }
+    } else if (kind == POTENTIAL_MIE) {
+        // host/potentials/pair/mie.cpp:72-92: even indices m > n, epsilon C(m, n)
+        require(im != nullptr && in != nullptr, "mie: index_m and index_n required");
+        for (size_t i = 0; i < n2; ++i) {
+            require(im[i] % 2 == 0 && in[i] % 2 == 0, "mie: power law indices of potential must be even");
+            require(im[i] > in[i] && in[i] >= 2 && im[i] <= 64, "mie: indices need 2 <= n < m <= 64");
+            index_m[i] = im[i];
+            index_n[i] = in[i];
+            double const m = im[i], n = in[i];
+            epsilon_c[i] = epsilon[i] * (m / (m - n) * std::pow(m / n, n / (m - n)));
+        }
+    }
+}
+
+// x^k for an integer k, by binary exponentiation
+static double pow_uint(double x, unsigned k) {
+    double r = 1;
+    for (; k; k >>= 1) {
+        if (k & 1) r *= x;
+        x *= x;
+    }
+    return r;
 }
 
 void potential::base(double rr, unsigned ab, double& fval, double& en) const {
@@ -343,6 +375,25 @@ void potential::base(double rr, unsigned ab, double& fval, double& en) const {
         double const e = epsilon[ab] * r6i;
         fval = 48 * rri * e * (r6i - 0.5) / s2;
         en = 4 * e * (r6i - 1);
+    } else if (kind == POTENTIAL_POWER_LAW) {
+        // power_law.hpp:183-196: eps (sigma/r)^n, fval = n U / r^2
+        unsigned const n = index_m[ab];
+        double const rri = sigma[ab] * sigma[ab] / rr;
+        double rni = pow_uint(rri, n / 2);
+        if (n % 2) rni *= std::sqrt(rri);
+        double const e = epsilon[ab] * rni;
+        fval = n * e / rr;
+        en = e;
+    } else if (kind == POTENTIAL_MIE) {
+        // mie.hpp:75-85: eps C (sigma/r)^n [(sigma/r)^(m-n) - 1]
+        unsigned const m2 = index_m[ab] / 2, n2 = index_n[ab] / 2;
+        double const s2 = sigma[ab] * sigma[ab];
+        double const rri = s2 / rr;
+        double const rni = pow_uint(rri, n2);
+        double const rmni = (m2 - n2 == n2) ? rni : pow_uint(rri, m2 - n2);
+        double const e = epsilon_c[ab] * rni;
+        fval = 2 * rri * e * (m2 * rmni - n2) / s2;
+        en = e * (rmni - 1);
     } else {
         double const B = distortion[ab];
         double const rsb = std::sqrt(rr) / sigma[ab] / B;
@@ -414,7 +465,7 @@ force_params potential::device_params(box const& b) const {
     require(ntype1 == ntype2, "potential: ntype1 != ntype2 is not supported by the device kernels");
     force_params P;
     std::memset(&P, 0, sizeof(P));
-    P.kind = kind;
+    P.kind = kind == POTENTIAL_MIE ? POTENTIAL_POWER_LAW : kind;   // one device form
     P.trunc = trunc;
     P.ntype = ntype2;
     P.rri_smooth = trunc == TRUNC_SMOOTH_R4 ? float(1 / (h * h)) : 0.0f;
@@ -437,6 +488,14 @@ force_params potential::device_params(box const& b) const {
                     split_ff(q * (48 * eps / (sig * sig)), P.q_p0[0], P.q_p0[1]);
                     split_ff(q * q * (48 * eps / (sig * sig)), P.q2_p0[0], P.q2_p0[1]);
                 }
+            } else if (kind == POTENTIAL_POWER_LAW || kind == POTENTIAL_MIE) {
+                bool const mie = kind == POTENTIAL_MIE;
+                pp.p[0] = float(epsilon_c[ab]);
+                pp.p[1] = mie ? float(epsilon_c[ab]) : 0.0f;
+                pp.p[2] = float(sig * sig);
+                pp.p[3] = 0;
+                pp.p_lo[0] = float(index_m[ab]);
+                pp.p_lo[1] = mie ? float(index_n[ab]) : 0.0f;
             } else {
                 double const B = distortion[ab];
                 double const A = 2 * B * B - 1;

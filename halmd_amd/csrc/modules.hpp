@@ -140,13 +140,18 @@ private:
 class potential {
 public:
     // kind: POTENTIAL_*; parameter matrices are ntype x ntype row-major
+    // index_m: the power law index (POTENTIAL_POWER_LAW) or the Mie repulsion
+    // index; index_n: the Mie attraction index
     potential(int kind, unsigned ntype1, unsigned ntype2, double const* epsilon, double const* sigma,
-              double const* r_min, double const* distortion);
+              double const* r_min, double const* distortion, unsigned const* index_m = nullptr,
+              unsigned const* index_n = nullptr);
     // apply a truncation (TRUNC_*); r_cut_sigma in units of sigma
     void truncate(int trunc, double const* r_cut_sigma, double smoothing);
     int kind, trunc = -1;
     unsigned ntype1, ntype2;
     std::vector<double> epsilon, sigma, r_min, distortion, r_cut_sigma, r_cut, rr_cut, en_cut, force_cut;
+    std::vector<unsigned> index_m, index_n;
+    std::vector<double> epsilon_c;   // Mie: epsilon C(m, n)
     double h = 0;
     double r_cut_max() const;
     force_params device_params(box const& b) const;

@@ -1,6 +1,7 @@
-"""halmd.mdsim.potentials.pair: lennard_jones and morse with truncations.
+"""halmd.mdsim.potentials.pair: lennard_jones, morse, power_law and mie with
+truncations.
 
-Mirrors lua/halmd/mdsim/potentials/pair/{lennard_jones,morse}.lua.in and
+Mirrors lua/halmd/mdsim/potentials/pair/{lennard_jones,morse,power_law,mie}.lua.in and
 adapters/truncate.lua.in over include/halmd_hip.h.  The cutoff of a
 truncation is given in units of sigma (r_c = cutoff * sigma), as in the
 reference (potentials/pair/truncations/shifted.hpp constructor).
@@ -11,7 +12,7 @@ import numpy as np
 
 from ..._lib import Handle, call, dptr, f64, out_handle
 
-__all__ = ["lennard_jones", "morse", "TRUNCATIONS"]
+__all__ = ["lennard_jones", "morse", "power_law", "mie", "modified_lennard_jones", "TRUNCATIONS"]
 
 TRUNCATIONS = ("sharp", "shifted", "force_shifted", "smooth_r4")
 
@@ -88,3 +89,58 @@ class morse(_pair_potential):
         self._h = out_handle()
         call("halmd_mdsim_potentials_pair_morse_create", n, n, dptr(eps), dptr(sig), dptr(rmin), dptr(B),
              ctypes.byref(self._h))
+
+
+def _index_matrix(value, n):
+    a = np.atleast_2d(np.asarray(value))
+    if a.shape != (n, n):
+        a = np.broadcast_to(a, (n, n))
+    if np.any(a != np.round(a)) or np.any(a < 0):
+        raise ValueError("power law indices must be non-negative integers")
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+class power_law(_pair_potential):
+    """U(r) = eps (sigma/r)^n with integer index n per species pair
+    (lua/halmd/mdsim/potentials/pair/power_law.lua.in:130-180; default index 12)."""
+
+    description = "power law potential"
+
+    def __init__(self, epsilon=1.0, sigma=1.0, index=12, species=None):
+        super().__init__()
+        n = species or max(_species_matrix(epsilon).shape[0], np.atleast_2d(index).shape[0])
+        eps = _species_matrix(epsilon, n)
+        sig = _species_matrix(sigma, n)
+        idx = _index_matrix(index, n)
+        self.species = n
+        self.epsilon, self.sigma, self.index = eps.copy(), sig.copy(), idx.copy()
+        self._h = out_handle()
+        call("halmd_mdsim_potentials_pair_power_law_create", n, n, dptr(eps), dptr(sig),
+             idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), ctypes.byref(self._h))
+
+
+class mie(_pair_potential):
+    """U(r) = C(m, n) eps [(sigma/r)^m - (sigma/r)^n], C = m/(m-n) (m/n)^(n/(m-n)),
+    even indices m > n (lua/halmd/mdsim/potentials/pair/mie.lua.in:140-200:
+    index_repulsion m, index_attraction n; mie(m=12, n=6) is Lennard-Jones)."""
+
+    description = "Mie potential"
+
+    def __init__(self, epsilon=1.0, sigma=1.0, index_repulsion=12, index_attraction=6, species=None):
+        super().__init__()
+        n = species or max(_species_matrix(epsilon).shape[0], np.atleast_2d(index_repulsion).shape[0])
+        eps = _species_matrix(epsilon, n)
+        sig = _species_matrix(sigma, n)
+        m_ = _index_matrix(index_repulsion, n)
+        n_ = _index_matrix(index_attraction, n)
+        self.species = n
+        self.epsilon, self.sigma = eps.copy(), sig.copy()
+        self.index_repulsion, self.index_attraction = m_.copy(), n_.copy()
+        self._h = out_handle()
+        call("halmd_mdsim_potentials_pair_mie_create", n, n, dptr(eps), dptr(sig),
+             m_.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), n_.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+             ctypes.byref(self._h))
+
+
+# deprecated alias of the reference (lua/halmd/mdsim/potentials/pair/modified_lennard_jones.lua.in:35-39)
+modified_lennard_jones = mie

@@ -127,6 +127,18 @@ HALMD_API int halmd_mdsim_potentials_pair_lennard_jones_create(unsigned ntype1, 
 HALMD_API int halmd_mdsim_potentials_pair_morse_create(unsigned ntype1, unsigned ntype2, const double* epsilon,
                                                        const double* sigma, const double* r_min,
                                                        const double* distortion, halmd_potential** out);
+/* power law eps (sigma/r)^n, index n per species pair (1 .. 64):
+ * lua/halmd/mdsim/potentials/pair/power_law.lua.in, host/potentials/pair/power_law.cpp,
+ * gpu/potentials/pair/power_law_kernel.hpp:46-60 */
+HALMD_API int halmd_mdsim_potentials_pair_power_law_create(unsigned ntype1, unsigned ntype2,
+                                                           const double* epsilon, const double* sigma,
+                                                           const unsigned* index, halmd_potential** out);
+/* Mie eps C(m,n) [(sigma/r)^m - (sigma/r)^n], C(m,n) = m/(m-n) (m/n)^(n/(m-n)), even
+ * indices 2 <= n < m <= 64: lua/halmd/mdsim/potentials/pair/mie.lua.in,
+ * host/potentials/pair/mie.cpp:47-95, gpu/potentials/pair/mie_kernel.hpp:48-62 */
+HALMD_API int halmd_mdsim_potentials_pair_mie_create(unsigned ntype1, unsigned ntype2, const double* epsilon,
+                                                     const double* sigma, const unsigned* index_m,
+                                                     const unsigned* index_n, halmd_potential** out);
 /* potential:truncate{truncation, cutoff = r_c / sigma [, smoothing]};
  * truncation: "sharp", "shifted", "force_shifted", "smooth_r4" */
 HALMD_API int halmd_mdsim_potentials_pair_truncate(halmd_potential* pot, const char* truncation,

@@ -80,16 +80,31 @@ struct box_t {
 // The cutoff is given in units of sigma: r_cut = r_cut_sigma * sigma
 // (shifted.hpp constructor).
 // ---------------------------------------------------------------------------
-enum { POT_LJ = 0, POT_MORSE = 1 };
+enum { POT_LJ = 0, POT_MORSE = 1, POT_POWER_LAW = 2, POT_MIE = 3 };
+
+// x^k for an integer k by binary exponentiation (the reference's
+// numeric/pow.hpp computes the same power by repeated squaring)
+static double pow_int(double x, unsigned k)
+{
+    double r = 1;
+    while (k) {
+        if (k & 1) r *= x;
+        x *= x;
+        k >>= 1;
+    }
+    return r;
+}
 enum { TR_SHARP = 0, TR_SHIFTED = 1, TR_FORCE_SHIFTED = 2, TR_SMOOTH_R4 = 3 };
 
 struct potential_t {
     int kind = POT_LJ, trunc = TR_SHIFTED, ntype = 1;
     std::vector<double> eps, sigma, sigma2, r_min_sigma, B, r_cut_sigma, r_cut, rr_cut, en_cut, force_cut;
+    std::vector<unsigned> index_m, index_n;
     double rri_smooth = 0;
 
     potential_t(int kind_, int trunc_, int ntype_, double const* eps_, double const* sigma_,
-                double const* r_min_, double const* B_, double const* r_cut_sigma_, double h)
+                double const* r_min_, double const* B_, unsigned const* m_, unsigned const* n_,
+                double const* r_cut_sigma_, double h)
         : kind(kind_), trunc(trunc_), ntype(ntype_)
     {
         int const n2 = ntype * ntype;
@@ -98,11 +113,21 @@ struct potential_t {
         sigma2.resize(n2);
         r_min_sigma.assign(n2, 0.0);
         B.assign(n2, 1.0);
+        index_m.assign(n2, 0);
+        index_n.assign(n2, 0);
         for (int i = 0; i < n2; ++i) {
             sigma2[i] = sigma[i] * sigma[i];
             if (kind == POT_MORSE) {
                 r_min_sigma[i] = r_min_[i] / sigma[i];   // morse.cpp:53 element_div(r_min, sigma)
                 B[i] = B_[i];
+            }
+            if (kind == POT_POWER_LAW) index_m[i] = m_[i];
+            if (kind == POT_MIE) {
+                // host/potentials/pair/mie.cpp:85-92: epsilon C(m, n)
+                index_m[i] = m_[i];
+                index_n[i] = n_[i];
+                double const m = m_[i], n = n_[i];
+                eps[i] *= m / (m - n) * std::pow(m / n, n / (m - n));
             }
         }
         r_cut_sigma.assign(r_cut_sigma_, r_cut_sigma_ + n2);
@@ -128,6 +153,24 @@ struct potential_t {
             double eps_r6i = eps[ab] * r6i;
             fval = 48 * rri * eps_r6i * (r6i - 0.5) / s2;
             en = 4 * eps_r6i * (r6i - 1);
+        } else if (kind == POT_POWER_LAW) {
+            // host/potentials/pair/power_law.hpp:183-196
+            unsigned const n = index_m[ab];
+            double rri = sigma2[ab] / rr;
+            double rni = pow_int(rri, n / 2);
+            if (n % 2) rni *= std::sqrt(rri);
+            double eps_rni = eps[ab] * rni;
+            fval = n * eps_rni / rr;
+            en = eps_rni;
+        } else if (kind == POT_MIE) {
+            // host/potentials/pair/mie.hpp:75-85 (eps holds epsilon C(m, n))
+            unsigned const m_2 = index_m[ab] / 2, n_2 = index_n[ab] / 2;
+            double rri = sigma2[ab] / rr;
+            double rni = pow_int(rri, n_2);
+            double rmni = (m_2 - n_2 == n_2) ? rni : pow_int(rri, m_2 - n_2);
+            double eps_rni = eps[ab] * rni;
+            fval = 2 * rri * eps_rni * (m_2 * rmni - n_2) / sigma2[ab];
+            en = eps_rni * (rmni - 1);
         } else {
             double b = B[ab];
             double r_sigma_B = std::sqrt(rr) / sigma[ab] / b;
@@ -561,10 +604,13 @@ extern "C" {
 // ===========================================================================
 // Potentials
 // ===========================================================================
+// index_m: power law index n (POT_POWER_LAW) or Mie repulsion index m; index_n:
+// Mie attraction index n (both ignored by the other kinds)
 void* oracle_potential_create(int kind, int trunc, int ntype, double const* eps, double const* sigma,
-                              double const* r_min, double const* B, double const* r_cut_sigma, double h)
+                              double const* r_min, double const* B, unsigned const* index_m, unsigned const* index_n,
+                              double const* r_cut_sigma, double h)
 {
-    return new potential_t(kind, trunc, ntype, eps, sigma, r_min, B, r_cut_sigma, h);
+    return new potential_t(kind, trunc, ntype, eps, sigma, r_min, B, index_m, index_n, r_cut_sigma, h);
 }
 void oracle_potential_destroy(void* p) { delete static_cast<potential_t*>(p); }
 

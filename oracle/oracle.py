@@ -24,7 +24,7 @@ _i = ctypes.POINTER(ctypes.c_int)
 _l = ctypes.POINTER(ctypes.c_long)
 _vp = ctypes.c_void_p
 
-POT_LJ, POT_MORSE = 0, 1
+POT_LJ, POT_MORSE, POT_POWER_LAW, POT_MIE = 0, 1, 2, 3
 TRUNCATIONS = {"sharp": 0, "shifted": 1, "force_shifted": 2, "smooth_r4": 3}
 INT_VERLET, INT_NHC = 0, 1
 
@@ -41,7 +41,8 @@ def lib():
             build()
         L = ctypes.CDLL(_LIB_PATH)
         sig = {
-            "oracle_potential_create": (_vp, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _d, _d, _d, _d, _d, ctypes.c_double]),
+            "oracle_potential_create": (_vp, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _d, _d, _d, _d, _u, _u, _d,
+                                              ctypes.c_double]),
             "oracle_potential_destroy": (None, [_vp]),
             "oracle_potential_eval": (None, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _d, _d, _d]),
             "oracle_potential_cutoffs": (None, [_vp, _d, _d, _d]),
@@ -97,18 +98,25 @@ def _dd(a):
 class Potential:
     """Pair potential with truncation (host semantics, FP64)."""
 
-    def __init__(self, kind, truncation, epsilon, sigma, r_cut_sigma, r_min=None, distortion=None, h=0.005):
+    def __init__(self, kind, truncation, epsilon, sigma, r_cut_sigma, r_min=None, distortion=None, h=0.005,
+                 index=None, index_m=None, index_n=None):
+        """index: power law index n (POT_POWER_LAW); index_m, index_n: Mie indices (POT_MIE)."""
         eps = np.atleast_2d(_dd(epsilon))
         ntype = eps.shape[0]
         sig = _dd(np.broadcast_to(np.atleast_2d(sigma), (ntype, ntype)))
         rcs = _dd(np.broadcast_to(np.atleast_2d(r_cut_sigma), (ntype, ntype)))
         rmin = _dd(np.broadcast_to(np.atleast_2d(r_min if r_min is not None else 1.0), (ntype, ntype)))
         B = _dd(np.broadcast_to(np.atleast_2d(distortion if distortion is not None else 1.0), (ntype, ntype)))
+        im = index if index is not None else index_m
+        m = np.ascontiguousarray(np.broadcast_to(np.atleast_2d(im if im is not None else 0), (ntype, ntype)),
+                                 dtype=np.uint32)
+        n = np.ascontiguousarray(np.broadcast_to(np.atleast_2d(index_n if index_n is not None else 0),
+                                                 (ntype, ntype)), dtype=np.uint32)
         self.ntype = ntype
         self.kind = kind
-        self._keep = (eps, sig, rcs, rmin, B)
+        self._keep = (eps, sig, rcs, rmin, B, m, n)
         self.h = lib().oracle_potential_create(kind, TRUNCATIONS[truncation], ntype, _p(eps, _d), _p(sig, _d),
-                                               _p(rmin, _d), _p(B, _d), _p(rcs, _d), h)
+                                               _p(rmin, _d), _p(B, _d), _p(m, _u), _p(n, _u), _p(rcs, _d), h)
         rc = np.zeros(ntype * ntype)
         ec = np.zeros(ntype * ntype)
         fc = np.zeros(ntype * ntype)

@@ -84,6 +84,38 @@ def test_host_morse_matches_golden_tables():
             assert (u[0] == uref) if uref == 0 else abs(u[0] - uref) <= tol * abs(uref)
 
 
+def test_host_power_law_and_mie_match_golden_tables():
+    from halmd_amd.mdsim.potentials import pair
+    g = json.load(open(os.path.join(GOLDEN, "potential_tables.json")))
+    pl = g["power_law"]
+    p = pair.power_law(epsilon=pl["epsilon"], sigma=pl["sigma"], index=pl["index"])
+    p.truncate("shifted", cutoff=pl["r_cut_sigma"])
+    mi = g["mie"]
+    m = pair.mie(epsilon=mi["epsilon"], sigma=mi["sigma"], index_repulsion=mi["index_m"],
+                 index_attraction=mi["index_n"])
+    m.truncate("shifted", cutoff=mi["r_cut_sigma"])
+    for pot, tables, tol_of in ((p, pl["tables"], lambda a, b: (1 + pl["index"][a][b] // 2)),
+                                (m, mi["tables"], lambda a, b: mi["tolerance_eps"])):
+        for key, (a, b) in (("aa", (0, 0)), ("ab", (0, 1)), ("bb", (1, 1))):
+            tol = tol_of(a, b) * np.finfo(float).eps
+            for r, fref, uref in tables[key]:
+                f, u = pot(r * r, a, b)
+                for x, ref in ((f[0], fref), (u[0], uref)):
+                    assert (x == ref) if ref == 0 else abs(x - ref) <= tol * abs(ref), (key, r, x, ref)
+
+
+def test_mie_rejects_odd_or_inverted_indices():
+    from halmd_amd.mdsim.potentials import pair
+    with pytest.raises(_lib.HalmdError):
+        pair.mie(index_repulsion=12, index_attraction=5)
+    with pytest.raises(_lib.HalmdError):
+        pair.mie(index_repulsion=6, index_attraction=8)
+    with pytest.raises(_lib.HalmdError):
+        pair.power_law(index=0)
+    # modified_lennard_jones is the reference's deprecated name of mie
+    assert pair.modified_lennard_jones is pair.mie
+
+
 def test_wavevector_shells_match_oracle(oracle):
     from halmd_amd import mdsim
     from halmd_amd.observables.utility import wavevector

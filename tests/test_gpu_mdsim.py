@@ -288,8 +288,31 @@ def test_morse_forces_match_fp64(oracle):
     assert abs(u.sum() - uref.sum()) / abs(uref.sum()) < 1e-5
 
 
+@pytest.mark.parametrize("kind", ["mie", "power_law"])
+def test_mie_and_power_law_forces_match_fp64(oracle, kind):
+    # a thermalised fluid: Mie 10-4 (even indices), soft-sphere power law 9
+    # (odd index, the sqrt branch), both shifted at 2.5 sigma
+    if kind == "mie":
+        pot_d = mdsim.potentials.pair.mie(index_repulsion=10, index_attraction=4).truncate("shifted", cutoff=2.5)
+        pot = oracle.Potential(oracle.POT_MIE, "shifted", 1.0, 1.0, 2.5, index_m=10, index_n=4)
+    else:
+        pot_d = mdsim.potentials.pair.power_law(index=9).truncate("shifted", cutoff=2.5)
+        pot = oracle.Potential(oracle.POT_POWER_LAW, "shifted", 1.0, 1.0, 2.5, index=9)
+    w = workloads.fluid(4000, 0.85, 1.0, pot_d, jitter=0.05, seed=29, timestep=0.002)
+    pos, species = _thermalised(w)
+    fref, uref, _ = oracle.pair_force(pos, species, w.length, pot, skin=0.3)
+    f = w.particle.data["force"]
+    print(f"{kind}: max|dF|/rms = {_force_error(f, fref):.3e}, median rel = {_median_rel(f, fref):.3e}")
+    assert _force_error(f, fref) < FORCE_TOL
+    assert _median_rel(f, fref) < MEDIAN_TOL
+    u = w.particle.data["potential_energy"]
+    assert abs(u.sum() - uref.sum()) / abs(uref.sum()) < 1e-5
+
+
 @pytest.mark.parametrize("kind,truncation", [("lj", "shifted"), ("lj", "force_shifted"), ("lj", "smooth_r4"),
-                                             ("ka", "shifted"), ("morse", "shifted")])
+                                             ("ka", "shifted"), ("morse", "shifted"), ("power_law", "shifted"),
+                                             ("power_law_odd", "force_shifted"), ("mie", "shifted"),
+                                             ("mie", "smooth_r4")])
 def test_pair_table(oracle, kind, truncation):
     # isolated pairs at known separations (the reference's neighbour_chain
     # idea, test/unit/mdsim/potentials/pair/gpu/neighbour_chain.hpp): the force
@@ -304,6 +327,20 @@ def test_pair_table(oracle, kind, truncation):
         pot_d = mdsim.potentials.pair.morse(epsilon=1.0, sigma=1.0 / 6, r_min=1.0, distortion=1.0)
         pot = oracle.Potential(oracle.POT_MORSE, truncation, 1.0, 1.0 / 6, 15.0, r_min=1.0, distortion=1.0)
         cut, r_lo, nsp = 15.0, 0.75, 1
+    elif kind in ("power_law", "power_law_odd"):
+        # binary mixture with species-dependent indices (the odd index takes
+        # the sqrt branch of the power)
+        idx = [[12, 9], [9, 6]] if kind == "power_law" else [[7, 9], [9, 5]]
+        eps, sig = [[1.0, 0.7], [0.7, 0.5]], [[1.0, 1.1], [1.1, 1.2]]
+        pot_d = mdsim.potentials.pair.power_law(epsilon=eps, sigma=sig, index=idx)
+        pot = oracle.Potential(oracle.POT_POWER_LAW, truncation, eps, sig, 2.5, index=idx)
+        cut, r_lo, nsp = 2.5, 0.8, 2
+    elif kind == "mie":
+        im, in_ = [[12, 10], [10, 8]], [[6, 4], [4, 2]]
+        eps, sig = [[1.0, 0.7], [0.7, 0.5]], [[1.0, 1.1], [1.1, 1.2]]
+        pot_d = mdsim.potentials.pair.mie(epsilon=eps, sigma=sig, index_repulsion=im, index_attraction=in_)
+        pot = oracle.Potential(oracle.POT_MIE, truncation, eps, sig, 2.5, index_m=im, index_n=in_, h=0.005)
+        cut, r_lo, nsp = 2.5, 0.85, 2
     elif kind == "ka":
         pot_d = mdsim.potentials.pair.lennard_jones(epsilon=workloads.KA_EPSILON, sigma=workloads.KA_SIGMA)
         pot = oracle.Potential(oracle.POT_LJ, truncation, workloads.KA_EPSILON, workloads.KA_SIGMA, 2.5)

@@ -50,6 +50,50 @@ def test_morse_tables(oracle):
             assert _close_fraction(u[0], uref, tol), (pair, r, u[0], uref)
 
 
+def test_power_law_tables(oracle):
+    g = json.load(open(os.path.join(GOLDEN, "potential_tables.json")))["power_law"]
+    pot = oracle.Potential(oracle.POT_POWER_LAW, g["truncation"], g["epsilon"], g["sigma"], g["r_cut_sigma"],
+                           index=g["index"])
+    for pair, (a, b) in (("aa", (0, 0)), ("ab", (0, 1)), ("bb", (1, 1))):
+        tol = (1 + g["index"][a][b] // 2) * EPS
+        for r, fref, uref in g["tables"][pair]:
+            f, u = pot(r * r, a, b)
+            assert _close_fraction(f[0], fref, tol), (pair, r, f[0], fref)
+            assert _close_fraction(u[0], uref, tol), (pair, r, u[0], uref)
+
+
+def test_mie_tables(oracle):
+    g = json.load(open(os.path.join(GOLDEN, "potential_tables.json")))["mie"]
+    pot = oracle.Potential(oracle.POT_MIE, g["truncation"], g["epsilon"], g["sigma"], g["r_cut_sigma"],
+                           index_m=g["index_m"], index_n=g["index_n"])
+    tol = g["tolerance_eps"] * EPS
+    for pair, (a, b) in (("aa", (0, 0)), ("ab", (0, 1)), ("bb", (1, 1))):
+        for r, fref, uref in g["tables"][pair]:
+            f, u = pot(r * r, a, b)
+            assert _close_fraction(f[0], fref, tol), (pair, r, f[0], fref)
+            assert _close_fraction(u[0], uref, tol), (pair, r, u[0], uref)
+
+
+@pytest.mark.parametrize("kind,args", [("power_law", dict(index=7)), ("mie", dict(index_m=10, index_n=4))])
+def test_power_potentials_derivative_and_mie_limit(oracle, kind, args):
+    # fval = -U'(r)/r by central differences (odd power law index: the sqrt
+    # branch); Mie with m = 12, n = 6 is Lennard-Jones (C(12, 6) = 4)
+    k = oracle.POT_POWER_LAW if kind == "power_law" else oracle.POT_MIE
+    pot = oracle.Potential(k, "sharp", 0.7, 1.3, 3.0, **args)
+    for r in (0.9, 1.1, 1.5, 2.5, 3.5):
+        h = 1e-6
+        _, up = pot((r + h) ** 2, 0, 0)
+        _, um = pot((r - h) ** 2, 0, 0)
+        f, _ = pot(r * r, 0, 0)
+        assert abs(f[0] - (-(up[0] - um[0]) / (2 * h) / r)) < 1e-6 * max(1.0, abs(f[0]))
+    mie = oracle.Potential(oracle.POT_MIE, "shifted", 0.7, 1.3, 2.5, index_m=12, index_n=6)
+    lj = oracle.Potential(oracle.POT_LJ, "shifted", 0.7, 1.3, 2.5)
+    rr = np.linspace(1.0, 3.2, 23) ** 2
+    fm, um = mie(rr, 0, 0)
+    fl, ul = lj(rr, 0, 0)
+    assert np.allclose(fm, fl, rtol=1e-13) and np.allclose(um, ul, rtol=1e-12, atol=1e-15)
+
+
 def test_morse_derivative(oracle):
     # fval = -U'(r)/r by central differences (the check the reference's
     # compute_potential_reference.py makes), for the Morse parameters of the
