@@ -551,6 +551,27 @@ HALMD_API int halmd_mdsim_core_step(const halmd_core* c, uint64_t* step) {
 }
 
 // ---- observables ------------------------------------------------------------------------------
+HALMD_API int halmd_observables_phase_space_create(halmd_particle* p, const halmd_box* b, halmd_phase_space** out) {
+    return guard([&] {
+        nonnull(p, "particle"); nonnull(b, "box"); nonnull(out, "out");
+        *out = handle<halmd_phase_space>(new phase_space(*as<particle>(p), *as<box>(b)));
+    });
+}
+HALMD_API int halmd_observables_phase_space_acquire(halmd_phase_space* h, const char* name, void* out) {
+    return guard([&] {
+        nonnull(name, "name");
+        as<phase_space>(h)->acquire(name, out);
+    });
+}
+HALMD_API int halmd_observables_phase_space_set(halmd_phase_space* h, const char* name, const void* in) {
+    return guard([&] {
+        nonnull(name, "name");
+        as<phase_space>(h)->set(name, in);
+    });
+}
+HALMD_API int halmd_observables_phase_space_destroy(halmd_phase_space* h) {
+    return guard([&] { delete reinterpret_cast<phase_space*>(h); });
+}
 HALMD_API int halmd_observables_thermodynamics_create(halmd_particle* p, const halmd_box* b, halmd_integrator* i,
                                                       halmd_thermodynamics** out) {
     return guard([&] {

@@ -1434,6 +1434,91 @@ void blocking_scheme::process(size_t level) {
 }
 
 // ---------------------------------------------------------------------------
+// phase_space (observables/gpu/phase_space.cpp, phase_space_kernel.cu:38-90)
+// ---------------------------------------------------------------------------
+phase_space::phase_space(particle& p_, box const& b_) : p(p_), b(b_) { p.attach_box(b); }
+
+void phase_space::reverse_ids() {
+    rid_.resize(p.n);
+    p.get_reverse_id(rid_.data());
+}
+
+void phase_space::acquire(std::string const& name, void* out) {
+    require(out != nullptr, "phase_space: null output");
+    size_t const n = p.n;
+    if (name == "position" || name == "velocity") {
+        // device gather in ID order (positions extended by their images)
+        sample_.resize(3 * n);
+        if (name == "position") {
+            launch_sample_position(p.pos, p.image, p.id, p.n, b.L, sample_, p.stream);
+        } else {
+            p.settle_velocity();
+            launch_sample_velocity(p.vel, p.id, p.n, sample_, p.stream);
+        }
+        sample_.download(static_cast<double*>(out), 3 * n, p.stream);
+        p.sync();
+    } else if (name == "species" || name == "mass") {
+        reverse_ids();
+        if (name == "species") {
+            std::vector<unsigned> m(n);
+            p.get_species(m.data());
+            auto* o = static_cast<unsigned*>(out);
+            for (size_t k = 0; k < n; ++k) o[k] = m[rid_[k]];
+        } else {
+            std::vector<double> m(n);
+            p.get_mass(m.data());
+            auto* o = static_cast<double*>(out);
+            for (size_t k = 0; k < n; ++k) o[k] = m[rid_[k]];
+        }
+    } else {
+        throw error(HALMD_ERR_INVALID_ARGUMENT, "phase_space: unknown sample '" + name + "'");
+    }
+}
+
+void phase_space::set(std::string const& name, void const* in) {
+    require(in != nullptr, "phase_space: null input");
+    size_t const n = p.n;
+    reverse_ids();
+    if (name == "position") {
+        // reduce into the box (r - rint(r / L) L) and keep the images, in memory order
+        auto const* r = static_cast<double const*>(in);
+        std::vector<double> rm(3 * n);
+        std::vector<int> im(3 * n);
+        for (size_t k = 0; k < n; ++k) {
+            size_t const i = rid_[k];
+            for (int d = 0; d < 3; ++d) {
+                double const x = r[3 * k + d];
+                require(std::isfinite(x), "phase_space: position not finite");
+                double const img = std::rint(x / b.L[d]);
+                require(std::fabs(img) < 2e9, "phase_space: position too far outside the box");
+                rm[3 * i + d] = x - img * b.L[d];
+                im[3 * i + d] = int(img);
+            }
+        }
+        p.set_position(rm.data());
+        p.set_image(im.data());
+    } else if (name == "velocity") {
+        auto const* v = static_cast<double const*>(in);
+        std::vector<double> vm(3 * n);
+        for (size_t k = 0; k < n; ++k)
+            for (int d = 0; d < 3; ++d) vm[3 * size_t(rid_[k]) + d] = v[3 * k + d];
+        p.set_velocity(vm.data());
+    } else if (name == "species") {
+        auto const* sp = static_cast<unsigned const*>(in);
+        std::vector<unsigned> sm(n);
+        for (size_t k = 0; k < n; ++k) sm[rid_[k]] = sp[k];
+        p.set_species(sm.data());
+    } else if (name == "mass") {
+        auto const* m = static_cast<double const*>(in);
+        std::vector<double> mm(n);
+        for (size_t k = 0; k < n; ++k) mm[rid_[k]] = m[k];
+        p.set_mass(mm.data());
+    } else {
+        throw error(HALMD_ERR_INVALID_ARGUMENT, "phase_space: unknown sample '" + name + "'");
+    }
+}
+
+// ---------------------------------------------------------------------------
 // sampler (halmd/observables/sampler.cpp:42-130)
 // ---------------------------------------------------------------------------
 sampler::sampler(core& c_) : c(c_) {}

@@ -380,6 +380,26 @@ public:
     void stress_tensor(double* out);
 };
 
+// Phase space sampler (observables/gpu/phase_space.cpp,
+// lua/halmd/observables/phase_space.lua.in): particle data in particle-ID
+// order.  acquire: "position" (extended by the image, r + image L; FP64 from
+// the float-float state), "velocity" (FP64), "species", "mass".  set: the same
+// names; positions are reduced into the box (image = rint(r / L), as
+// box_kernel::reduce_periodic) and their images stored.
+class phase_space {
+public:
+    phase_space(particle& p, box const& b);
+    particle& p;
+    box const& b;
+    // out / in: n x 3 doubles (position, velocity), n unsigned (species) or n doubles (mass)
+    void acquire(std::string const& name, void* out);
+    void set(std::string const& name, void const* in);
+private:
+    dbuf<double> sample_;
+    std::vector<unsigned> rid_;
+    void reverse_ids();
+};
+
 class wavevector {
 public:
     // shells of lattice wavevectors with |q| within tolerance of each wavenumber

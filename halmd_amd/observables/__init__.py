@@ -1,4 +1,4 @@
-"""halmd.observables: sampler, thermodynamics, density_mode, utility, dynamics.
+"""halmd.observables: sampler, thermodynamics, density_mode, phase_space, utility, dynamics.
 
 Mirrors lua/halmd/observables/*.lua.in over include/halmd_hip.h.
 """
@@ -8,7 +8,58 @@ import numpy as np
 
 from .._lib import Handle, call, dptr, out_handle
 
-__all__ = ["sampler", "thermodynamics", "density_mode"]
+__all__ = ["sampler", "thermodynamics", "density_mode", "phase_space"]
+
+
+class phase_space(Handle):
+    """Particle samples in particle-ID order (lua/halmd/observables/phase_space.lua.in:165-260,
+    observables/gpu/phase_space.cpp).  acquire(name) for name in position
+    (extended by the periodic image, r + image L), velocity, species, mass;
+    set({name: array}) writes them back (positions reduced into the box, their
+    images kept)."""
+
+    _destroy = "halmd_observables_phase_space_destroy"
+    _SHAPES = {"position": (3, np.float64), "velocity": (3, np.float64), "species": (1, np.uint32),
+               "mass": (1, np.float64)}
+
+    def __init__(self, box, particle):
+        super().__init__()
+        self._deps = (box, particle)
+        self.box, self.particle = box, particle
+        self._h = out_handle()
+        call("halmd_observables_phase_space_create", particle.handle, box.handle, ctypes.byref(self._h))
+
+    def _shape(self, name):
+        if name not in self._SHAPES:
+            raise ValueError(f"phase_space: unknown sample {name!r}")
+        d, t = self._SHAPES[name]
+        n = self.particle.nparticle
+        return ((n, d) if d > 1 else (n,)), t
+
+    def acquire(self, name):
+        shape, t = self._shape(name)
+        out = np.empty(shape, dtype=t)
+        call("halmd_observables_phase_space_acquire", self.handle, name.encode(), out.ctypes.data_as(ctypes.c_void_p))
+        return out
+
+    def acquire_position(self):
+        return self.acquire("position")
+
+    def acquire_velocity(self):
+        return self.acquire("velocity")
+
+    def acquire_species(self):
+        return self.acquire("species")
+
+    def acquire_mass(self):
+        return self.acquire("mass")
+
+    def set(self, samples):
+        """samples: {name: array in particle-ID order} (phase_space.lua.in:116-130)."""
+        for name, value in samples.items():
+            shape, t = self._shape(name)
+            a = np.ascontiguousarray(np.broadcast_to(np.asarray(value, dtype=t), shape))
+            call("halmd_observables_phase_space_set", self.handle, name.encode(), a.ctypes.data_as(ctypes.c_void_p))
 
 
 class thermodynamics(Handle):

@@ -57,6 +57,7 @@ typedef struct halmd_blocking_scheme_s halmd_blocking_scheme;
 typedef struct halmd_sampler_s halmd_sampler;
 typedef struct halmd_ensemble_s halmd_ensemble;
 typedef struct halmd_server_s halmd_server;
+typedef struct halmd_phase_space_s halmd_phase_space;
 
 /* ---- library ---------------------------------------------------------- */
 HALMD_API const char* halmd_last_error(void);
@@ -270,6 +271,16 @@ HALMD_API int halmd_mdsim_core_step(const halmd_core* c, uint64_t* step);
 
 /* ---- observables.thermodynamics: halmd/observables/gpu/thermodynamics.cpp:67-191,
  *      halmd/observables/thermodynamics.hpp:80-90 (mean values per particle) -- */
+/* ---- observables.phase_space: lua/halmd/observables/phase_space.lua.in:165-260,
+ *      observables/gpu/phase_space.cpp (acquire / set), phase_space_kernel.cu:38-90 ----
+ * Samples in particle-ID order.  name: "position" (n x 3 doubles, extended by
+ * the periodic image: r + image L), "velocity" (n x 3 doubles), "species" (n
+ * unsigned), "mass" (n doubles).  set("position") reduces each position into
+ * the box (image = rint(r / L)) and stores the image. */
+HALMD_API int halmd_observables_phase_space_create(halmd_particle* p, const halmd_box* box, halmd_phase_space** out);
+HALMD_API int halmd_observables_phase_space_acquire(halmd_phase_space* ps, const char* name, void* out);
+HALMD_API int halmd_observables_phase_space_set(halmd_phase_space* ps, const char* name, const void* in);
+HALMD_API int halmd_observables_phase_space_destroy(halmd_phase_space* ps);
 HALMD_API int halmd_observables_thermodynamics_create(halmd_particle* p, const halmd_box* box,
                                                       halmd_integrator* integrator, halmd_thermodynamics** out);
 HALMD_API int halmd_observables_thermodynamics_destroy(halmd_thermodynamics* t);

@@ -134,3 +134,36 @@ def test_sampler_density_mode_slot():
     steps, rho = w.sampler.density_mode_records(k)
     assert list(steps) == [0, 10, 20, 30]
     assert np.allclose(rho[-1], dm.acquire())
+
+
+def test_phase_space_acquire_and_set():
+    w = workloads.lennard_jones(4000, density=0.8442, temperature=1.5, seed=74, sort_interval=25)
+    w.sampler.run(120)   # particles sorted (memory order != ID order) and crossing the box
+    ps = observables.phase_space(w.box, w.particle)
+    L = w.length
+    img0 = w.particle.ordered("image")
+    assert np.any(img0 != 0)
+    assert not np.array_equal(w.particle.data["id"], np.arange(4000))
+    r = ps.acquire_position()
+    assert np.array_equal(r, w.particle.ordered("position") + img0 * L)
+    v = ps.acquire_velocity()
+    assert np.array_equal(v, w.particle.ordered("velocity"))
+    assert np.array_equal(ps.acquire_species(), w.particle.ordered("species"))
+    assert np.array_equal(ps.acquire_mass(), w.particle.ordered("mass"))
+    en_pot, en_kin = w.thermodynamics.en_pot(), w.thermodynamics.en_kin()
+    # set: the same physical positions shifted by whole boxes come back as
+    # positions in the box with the shifts in the images; doubled velocities
+    shift = np.random.default_rng(3).integers(-3, 4, r.shape)
+    ps.set({"position": r + shift * L, "velocity": 2 * v})
+    assert np.array_equal(w.particle.ordered("image"), img0 + shift)
+    assert np.allclose(ps.acquire_position(), r + shift * L, rtol=0, atol=1e-11)
+    inbox = w.particle.data["position"]
+    assert np.all(np.abs(inbox) <= L / 2 * (1 + 1e-12))
+    assert np.isclose(w.thermodynamics.en_pot(), en_pot, rtol=1e-6)
+    assert np.isclose(w.thermodynamics.en_kin(), 4 * en_kin, rtol=1e-12)
+    # species and mass round trip in ID order
+    sp = np.zeros(4000, np.uint32)
+    ps.set({"mass": np.full(4000, 2.0), "species": sp})
+    assert np.all(ps.acquire_mass() == 2.0) and np.all(ps.acquire_species() == 0)
+    with pytest.raises(Exception):
+        ps.acquire("bogus")
