@@ -57,13 +57,20 @@ def build_workload(seed):
 
 
 def kernel_times(w, steps):
-    """Average duration per launch (ms) of each kernel class over `steps` steps."""
+    """Average duration per launch (ms) of each kernel class over `steps` steps.
+
+    The pass waits for each step's neighbour check before the force launch, so
+    that every timed force launch does the full work (in the timed region a
+    force launch may start before the check is back and return at once when the
+    lists turn out stale; the module then rebuilds and launches again)."""
     from halmd_amd import _lib
     mods = {"force": w.force, "neighbour": w.neighbour, "binning": w.neighbour.binning,
             "integrate": w.integrator, "finalize": w.integrator, "sort": w.sort}
     for name, m in mods.items():
         _lib.call("halmd_profile_enable", m.handle, name.encode(), 4 * steps + 16)
+    w.particle.set_speculative_force(False)
     w.sampler.run(steps)
+    w.particle.set_speculative_force(True)
     out = {}
     for name, m in mods.items():
         import ctypes

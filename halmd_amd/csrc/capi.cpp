@@ -219,6 +219,9 @@ HALMD_API int halmd_mdsim_particle_rearrange(halmd_particle* hp, const unsigned*
 }
 
 // ---- potentials -------------------------------------------------------------------
+HALMD_API int halmd_mdsim_particle_set_speculative_force(halmd_particle* p, int enable) {
+    return guard([&] { as<particle>(p)->speculative_force = enable != 0; });
+}
 HALMD_API int halmd_mdsim_potentials_pair_lennard_jones_create(unsigned nt1, unsigned nt2, const double* eps,
                                                                const double* sigma, halmd_potential** out) {
     return guard([&] {

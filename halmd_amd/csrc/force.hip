@@ -453,11 +453,26 @@ __device__ __forceinline__ void staged_force_segment(
         }
     }
 }
+// Speculative launch: abort_slots holds the per-wave displacement maxima of
+// the step (DISP_SLOTS words DISP_STRIDE apart, float bits of squared
+// displacements); when their maximum exceeds abort_bits the lists are stale and
+// every wave returns before touching memory (the caller rebuilds and launches
+// again).  Each wave reads the 64 words itself, so the decision is uniform
+// without a barrier.
+__device__ __forceinline__ bool displacement_aborts(uint32_t const* __restrict__ slots, uint32_t bits) {
+    if (!slots) return false;
+    static_assert(DISP_SLOTS == WAVE, "one displacement slot per lane");
+    unsigned m = slots[(threadIdx.x % WAVE) * DISP_STRIDE];
+    for (int o = WAVE / 2; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o, WAVE));
+    return m > bits;
+}
+
 template <int KIND, int TRUNC, bool AUX, bool ISO, bool MULTI>
 __global__ __launch_bounds__(SEG_THREADS_MAX) void pair_force_staged_kernel(
     uint4 const* __restrict__ posq, uint32_t const* __restrict__ off, seg_params sp,
     uint16_t const* __restrict__ list, uint32_t const* __restrict__ count, unsigned cap, force_params P,
-    bool force_zero, float aux_weight, float4* __restrict__ force, float* __restrict__ stress, unsigned stride)
+    bool force_zero, float aux_weight, float4* __restrict__ force, float* __restrict__ stress, unsigned stride,
+    uint32_t const* __restrict__ abort_slots, uint32_t abort_bits)
 {
     extern __shared__ uint4 s_rec[];
     __shared__ seg_stage st;
@@ -465,6 +480,7 @@ __global__ __launch_bounds__(SEG_THREADS_MAX) void pair_force_staged_kernel(
 
     unsigned const seg = seg_of_block(sp, blockIdx.x);
     if (seg >= sp.nseg) return;   // the whole workgroup
+    if (displacement_aborts(abort_slots, abort_bits)) return;   // the whole workgroup
     staged_force_segment<KIND, TRUNC, AUX, ISO, MULTI>(posq, off, sp, seg, list, count, cap, P, force_zero,
                                                        aux_weight, force, stress, stride, s_rec, st, s_pair);
 }
@@ -599,6 +615,8 @@ struct staged_launch {
     float4* force;
     float* stress;
     unsigned stride;
+    uint32_t const* abort_slots;
+    uint32_t abort_bits;
     hipStream_t s;
 };
 
@@ -607,7 +625,8 @@ void staged_go(staged_launch const& a) {
     size_t const lds = size_t(a.sp.lds_cap) * sizeof(uint4);
     unsigned const grid = SEG_XCD * a.sp.chunk;
     pair_force_staged_kernel<KIND, TRUNC, AUX, ISO, MULTI><<<grid, a.sp.threads, lds, a.s>>>(
-        a.posq, a.off, a.sp, a.list, a.count, a.cap, a.p, a.force_zero, a.w, a.force, a.stress, a.stride);
+        a.posq, a.off, a.sp, a.list, a.count, a.cap, a.p, a.force_zero, a.w, a.force, a.stress, a.stride,
+        a.abort_slots, a.abort_bits);
 }
 
 template <int KIND, int TRUNC>
@@ -653,9 +672,11 @@ void launch_pair_force(uint4 const* posq, unsigned n, uint32_t const* list, uint
 
 void launch_pair_force_staged(uint4 const* posq, uint32_t const* offset, seg_params const& sp, uint16_t const* list,
                               uint32_t const* count, unsigned cap, force_params const& p, bool aux, bool force_zero,
-                              float aux_weight, float4* force, float* stress, unsigned stride, hipStream_t s) {
+                              float aux_weight, float4* force, float* stress, unsigned stride, hipStream_t s,
+                              uint32_t const* abort_slots, uint32_t abort_bits) {
     if (!sp.nseg) return;
-    staged_launch const a{posq, offset, sp, list, count, cap, p, aux, force_zero, aux_weight, force, stress, stride, s};
+    staged_launch const a{posq,  offset, sp,     list,   count,       cap,        p, aux, force_zero,
+                          aux_weight, force, stress, stride, abort_slots, abort_bits, s};
     if (p.kind == POTENTIAL_LENNARD_JONES)
         staged_trunc<POTENTIAL_LENNARD_JONES>(a);
     else if (p.kind == POTENTIAL_MORSE)

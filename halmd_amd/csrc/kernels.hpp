@@ -231,10 +231,13 @@ void launch_pair_force(uint4 const* posq, unsigned n, uint32_t const* list, uint
                        force_params const& p, bool aux, bool force_zero, float aux_weight, float4* force,
                        float* stress, unsigned stride, hipStream_t s);
 
-// the same over 16-bit slot lists of cell segments (particles in cell order)
+// the same over 16-bit slot lists of cell segments (particles in cell order);
+// abort_slots (optional): a speculative launch that does nothing when the
+// maximum of the DISP_SLOTS displacement words there exceeds abort_bits
 void launch_pair_force_staged(uint4 const* posq, uint32_t const* offset, seg_params const& sp, uint16_t const* list,
                               uint32_t const* count, unsigned cap, force_params const& p, bool aux, bool force_zero,
-                              float aux_weight, float4* force, float* stress, unsigned stride, hipStream_t s);
+                              float aux_weight, float4* force, float* stress, unsigned stride, hipStream_t s,
+                              uint32_t const* abort_slots = nullptr, uint32_t abort_bits = 0);
 
 // ---------------------------------------------------------------------------
 // integrators (integrate.hip)

@@ -5,6 +5,7 @@
 #include <cstdlib>
 #include <array>
 #include <cmath>
+#include <limits>
 #include <map>
 #include <numeric>
 
@@ -32,11 +33,13 @@ void kernel_timer::start(hipStream_t s) {
     if (next_ + 2 <= events_.size()) HIP_CHECK(hipEventRecord(events_[next_], s));
 }
 
-void kernel_timer::stop(hipStream_t s) {
+bool kernel_timer::stop(hipStream_t s) {
     if (next_ + 2 <= events_.size()) {
         HIP_CHECK(hipEventRecord(events_[next_ + 1], s));
         next_ += 2;
+        return true;
     }
+    return false;
 }
 
 std::pair<double, size_t> kernel_timer::collect() {
@@ -284,12 +287,15 @@ void particle::update_force(bool need_aux) {
         if (aux_enabled) HIP_CHECK(hipMemsetAsync(stress, 0, stress.bytes(), stream));
     } else {
         // every neighbour list first (a rebuild may reorder the particles),
-        // then the forces
-        for (force_module* f : forces) f->prepare();
-        bool first = true;
-        for (force_module* f : forces) {
-            f->apply(first);
-            first = false;
+        // then the forces; a single force module may launch before its
+        // neighbour check is back
+        if (!speculative_force || forces.size() != 1 || !forces[0]->apply_speculative()) {
+            for (force_module* f : forces) f->prepare();
+            bool first = true;
+            for (force_module* f : forces) {
+                f->apply(first);
+                first = false;
+            }
         }
     }
     force_valid = true;
@@ -583,7 +589,17 @@ max_displacement::max_displacement(particle& p_, box const& b_) : p(p_), b(b_) {
     r0.resize(p.n);
     d_max2.resize(2 * DISP_SET);
     h_max2.resize(DISP_SLOTS);
+    HIP_CHECK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+    HIP_CHECK(hipEventCreateWithFlags(&ready_, hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&copied_, hipEventDisableTiming));
     zero();
+}
+
+max_displacement::~max_displacement() {
+    if (copied_) (void)hipEventSynchronize(copied_);
+    if (ready_) (void)hipEventDestroy(ready_);
+    if (copied_) (void)hipEventDestroy(copied_);
+    if (side_) (void)hipStreamDestroy(side_);
 }
 
 void max_displacement::zero() {
@@ -615,8 +631,12 @@ double max_displacement::compute() {
     HIP_CHECK(hipMemcpy2DAsync(h_max2.get(), sizeof(uint32_t), cur, DISP_STRIDE * sizeof(uint32_t), sizeof(uint32_t),
                                DISP_SLOTS, hipMemcpyDeviceToHost, s));
     p.sync();
+    return finish(h_max2.get());
+}
+
+double max_displacement::finish(uint32_t const* words) {
     // the words hold float bits of non-negative values: integer order is float order
-    uint32_t const bits = *std::max_element(h_max2.get(), h_max2.get() + DISP_SLOTS);
+    uint32_t const bits = *std::max_element(words, words + DISP_SLOTS);
     float m2;
     std::memcpy(&m2, &bits, 4);
     value = std::sqrt(double(m2));
@@ -624,6 +644,25 @@ double max_displacement::compute() {
     slot ^= 1u;   // the next fused step writes the slot cleared by this one
     tracked_version = 0;
     return value;
+}
+
+bool max_displacement::speculation_ready() const {
+    return computed_version != p.position_version && tracked_version == p.position_version;
+}
+
+void max_displacement::begin_readback() {
+    // the copy waits for the work already queued on the particle stream (the
+    // integrate kernel that wrote the words), not for what follows it
+    HIP_CHECK(hipEventRecord(ready_, p.stream));
+    HIP_CHECK(hipStreamWaitEvent(side_, ready_, 0));
+    HIP_CHECK(hipMemcpy2DAsync(h_max2.get(), sizeof(uint32_t), current_slots(), DISP_STRIDE * sizeof(uint32_t),
+                               sizeof(uint32_t), DISP_SLOTS, hipMemcpyDeviceToHost, side_));
+    HIP_CHECK(hipEventRecord(copied_, side_));
+}
+
+double max_displacement::finish_readback() {
+    HIP_CHECK(hipEventSynchronize(copied_));
+    return finish(h_max2.get());
 }
 
 // ---------------------------------------------------------------------------
@@ -634,6 +673,16 @@ neighbour::neighbour(particle& p_, binning* bin_, max_displacement& disp_, box c
                      unsigned nt1, unsigned nt2, double skin, double occupancy_)
     : p(p_), bin(bin_), disp(disp_), b(b_), r_skin(skin), occupancy(occupancy_)
 {
+    {
+        // keep_bits: the largest float m2 with sqrt(double(m2)) <= skin / 2,
+        // the rule of require_current, found by stepping across the boundary
+        double const half = skin / 2;
+        auto keeps = [half](float m2) { return std::sqrt(double(m2)) <= half; };
+        float f = float(half * half);
+        while (keeps(f) && f < std::numeric_limits<float>::max()) f = std::nextafter(f, HUGE_VALF);
+        while (!keeps(f) && f > 0.0f) f = std::nextafter(f, 0.0f);
+        std::memcpy(&keep_bits, &f, 4);
+    }
     p.attach_box(b);
     require(nt1 == nt2 && nt1 == p.nspecies, "neighbour: cutoff matrix must be nspecies x nspecies");
     require(occupancy > 0 && occupancy <= 1, "neighbour: occupancy must be in (0, 1]");
@@ -913,17 +962,35 @@ unsigned pair_trunc::tune() {
 
 void pair_trunc::apply(bool force_zero) {
     nb.require_current();
+    launch(force_zero, nullptr, 0);
+}
+
+bool pair_trunc::launch(bool force_zero, uint32_t const* abort_slots, uint32_t abort_bits) {
     bool const aux = p.aux_enabled;
     // a particle interacting with itself counts each pair twice (pair_trunc.hpp:291-294)
     float const w = float(aux_weight / 2);
     timer.start(p.stream);
     if (nb.staged)
         launch_pair_force_staged(p.posq, nb.bin->offset, nb.sp_force, nb.list16, nb.count, nb.cap, params, aux,
-                                 force_zero, w, p.force, p.stress, p.n_pad, p.stream);
+                                 force_zero, w, p.force, p.stress, p.n_pad, p.stream, abort_slots, abort_bits);
     else
         launch_pair_force(p.posq, p.n, nb.list, nb.count, nb.cap, params, aux, force_zero, w, p.force, p.stress,
                           p.n_pad, p.stream);
-    timer.stop(p.stream);
+    return timer.stop(p.stream);
+}
+
+bool pair_trunc::apply_speculative() {
+    if (!nb.staged || !nb.current_but_displacement() || !nb.disp.speculation_ready()) return false;
+    nb.disp.begin_readback();
+    // the kernel reads the same displacement words and does nothing when they
+    // exceed keep_bits, exactly when the readback below asks for a rebuild
+    bool const timed = launch(true, nb.disp.current_slots(), nb.keep_bits);
+    if (nb.disp.finish_readback() > nb.r_skin / 2) {
+        if (timed) timer.drop_last();   // that launch did no work
+        nb.require_current();           // the displacement is known: rebuilds
+        launch(true, nullptr, 0);
+    }
+    return true;
 }
 
 // ---------------------------------------------------------------------------

@@ -42,7 +42,10 @@ public:
     void enable(size_t max_launches);
     void disable();
     void start(hipStream_t s);
-    void stop(hipStream_t s);
+    // true when the launch was recorded
+    bool stop(hipStream_t s);
+    // forget the last recorded launch (one that did no work)
+    void drop_last() { if (next_ >= 2) next_ -= 2; }
     // synchronises, returns (total ms, launches) and resets
     std::pair<double, size_t> collect();
     bool enabled() const { return !events_.empty(); }
@@ -70,6 +73,9 @@ public:
 
     unsigned n, n_pad, nspecies;
     quant_params qp;
+    // a single force module may launch before its neighbour check is back
+    // (results are the same either way)
+    bool speculative_force = true;
     bool has_box = false;
     double box_length[3] = {0, 0, 0};
     hipStream_t stream;
@@ -187,6 +193,7 @@ public:
 class max_displacement {
 public:
     max_displacement(particle& p, box const& b);
+    ~max_displacement();
     particle& p;
     box const& b;
     dbuf<float4> r0;
@@ -201,6 +208,19 @@ public:
     // pointers for the integrate kernel; marks the resulting version as tracked
     uint32_t* fuse_slot();
     uint32_t* next_slot();
+    // Asynchronous form of compute() for positions whose maxima the integrate
+    // kernel wrote (speculation_ready): begin_readback copies the words on a
+    // side stream once the particle stream has reached this point, so the
+    // particle stream goes on at once; finish_readback waits for the copy and
+    // keeps the same state as compute().
+    bool speculation_ready() const;
+    uint32_t const* current_slots() const { return d_max2.get() + slot * DISP_SET; }
+    void begin_readback();
+    double finish_readback();
+private:
+    hipStream_t side_ = nullptr;
+    hipEvent_t ready_ = nullptr, copied_ = nullptr;
+    double finish(uint32_t const* words);
 };
 
 class hilbert;
@@ -242,6 +262,12 @@ public:
     std::vector<std::function<void()>> on_prepend_update, on_append_update;
     // lazy access: rebuild when reordered or max displacement > skin / 2
     void require_current();
+    // the lists fit the particle order; only the displacement is unchecked
+    bool current_but_displacement() const { return built && built_order == p.order_version; }
+    // largest squared-displacement float bits (as the integrate kernel stores
+    // them) that keep the lists: sqrt(double(m2)) <= skin / 2 exactly when
+    // bits(m2) <= keep_bits
+    uint32_t keep_bits = 0;
     void update();
     void get_lists(unsigned* counts, unsigned* entries, unsigned stride);
     kernel_timer timer;
@@ -253,6 +279,9 @@ public:
     // bring inputs up to date (neighbour lists; may reorder the particles)
     virtual void prepare() {}
     virtual void apply(bool force_zero) = 0;
+    // as prepare() + apply(true) for the only force module of a particle,
+    // without waiting for the neighbour check first; false: not possible now
+    virtual bool apply_speculative() { return false; }
 };
 
 class pair_trunc : public force_module {
@@ -267,11 +296,18 @@ public:
     force_params params;
     void prepare() override;
     void apply(bool force_zero) override;
+    // the only force module of its particle: launch the force before the
+    // displacement check has come back (the kernel itself skips the work when
+    // the lists are stale), then rebuild and launch again if they were; false
+    // when the lists or displacement do not allow it (nothing done)
+    bool apply_speculative() override;
     // pick the cell-segment width and workgroup size of the neighbour lists
     // by timing list builds and force evaluations around the current choice;
     // returns the width kept (0 if the lists are not cell-segment lists)
     unsigned tune();
     kernel_timer timer;
+private:
+    bool launch(bool force_zero, uint32_t const* abort_slots, uint32_t abort_bits);
 };
 
 class integrator {

@@ -123,6 +123,11 @@ class particle(Handle):
     def aux_enable(self):
         call("halmd_mdsim_particle_aux_enable", self.handle)
 
+    def set_speculative_force(self, enable=True):
+        """Launch the force before the neighbour check is back (default on;
+        results are the same either way)."""
+        call("halmd_mdsim_particle_set_speculative_force", self.handle, int(bool(enable)))
+
     def rearrange(self, index):
         idx = np.ascontiguousarray(index, dtype=np.uint32)
         call("halmd_mdsim_particle_rearrange", self.handle, uptr(idx))

@@ -120,6 +120,14 @@ HALMD_API int halmd_mdsim_particle_aux_enable(halmd_particle* p);
 /* particle:rearrange(index) (particle.cpp:173-218); index[new] = old */
 HALMD_API int halmd_mdsim_particle_rearrange(halmd_particle* p, const unsigned* index);
 
+/* No counterpart in the reference (which waits for the neighbour-list check
+ * before each force evaluation, from_binning.cpp:162-282): with one force
+ * module, launch the force right after the integrate step while the
+ * displacement check is read back; the kernel does nothing when the lists are
+ * stale, and the module then rebuilds and launches again.  Results are the
+ * same either way; on by default, enable = 0 waits for the check. */
+HALMD_API int halmd_mdsim_particle_set_speculative_force(halmd_particle* p, int enable);
+
 /* ---- mdsim.potentials.pair.*: lennard_jones.cpp:45-71, morse.cpp:41-61;
  *      truncations: lua/halmd/mdsim/potentials/pair/adapters/truncate.lua.in -- */
 HALMD_API int halmd_mdsim_potentials_pair_lennard_jones_create(unsigned ntype1, unsigned ntype2,

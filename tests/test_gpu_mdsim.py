@@ -413,6 +413,22 @@ def test_nve_trajectory_matches_fp64(oracle):
     assert np.max(np.abs(vel - ref["velocity"])) < 1e-5
 
 
+def test_speculative_force_is_bitwise_identical():
+    # the force launched before the displacement check comes back (and
+    # skipped on the device when the lists are stale) gives the trajectory of
+    # the waiting schedule bit for bit, across many list rebuilds
+    runs = []
+    for spec in (True, False):
+        w = workloads.lennard_jones(6912, density=0.8442, temperature=1.5, seed=81, timestep=0.005)
+        w.particle.set_speculative_force(spec)
+        w.sampler.run(300)
+        runs.append((w.particle.ordered("position"), w.particle.ordered("velocity"),
+                     w.thermodynamics.en_pot(), w.neighbour.rebuilds))
+    assert runs[0][3] == runs[1][3] and runs[0][3] >= 10
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+    assert runs[0][2] == runs[1][2]
+
+
 def test_deferred_finalize_is_bitwise_identical():
     # the finalize kick folded into the next integrate pass computes exactly
     # what the separate pass computes, also across rebuilds, sorts and samples
