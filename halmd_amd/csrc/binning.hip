@@ -241,27 +241,38 @@ __global__ __launch_bounds__(256) void build_all_pairs_kernel(uint4 const* __res
 
 // --- cell segments: 16-bit slots into the LDS-staged neighbourhood union ----
 
+// Byte offset of entry c of a lane from the workgroup's list base: H + 2048
+// (c / 16) + 32 lane + 2 (c % 16), with H (a multiple of 2048) the lane's
+// wave block row 0.  The counter q keeps it with the lane field (bits 5..10)
+// set to ones, q = H + 2048 (c / 16) + 0x7e0 + 2 (c % 16): adding 2 then
+// carries from the entry field straight into the block field, and one bit
+// field insert of the lane bits gives the address.
+constexpr uint32_t NB16_LANE_BITS = 0x7e0u;
+
 struct nb16_state {
     uint4 ri;
     unsigned own, ti;
-    unsigned c2, last2;  // twice the count; twice the last slot (cap - 1)
-    char* list;          // the workgroup's list base (uniform)
-    unsigned base2;      // byte offset of entry 0 of this particle (lane block) from list
+    unsigned q, q0, qmax;   // counter; its value at c = 0 and at c = cap - 1
+    unsigned lane32;        // 32 (lane % 64)
+    char* list;             // the workgroup's list base (uniform)
 };
+
+__device__ __forceinline__ unsigned nb16_count(nb16_state const& s) {
+    return (((s.q >> 11) - (s.q0 >> 11)) << 4) + ((s.q & 31u) >> 1);
+}
 
 // 2-byte stores straight to the blocked list: a lane fills one 32-byte block
 // at a time and a wave's blocks share 2 KiB, so the lines are complete in L2
 // before they are written back (no staging buffer, hence more workgroups per
-// CU).  Byte offset of entry c: base2 + 2048 (c / 16) + 2 (c % 16), formed
-// from c2 = 2c as ((c2 & ~31) << 6) + base2, | (c2 & 31) (base2 is a multiple
-// of 32).  Entries past the capacity overwrite the particle's last slot
-// instead of branching: the count still exceeds the capacity, and the caller
-// rebuilds larger lists.
+// CU; gathering entries in LDS and storing whole blocks measured 6% slower).
+// Entries past the capacity overwrite the particle's last slot instead of
+// branching: the count still exceeds the capacity, and the caller rebuilds
+// larger lists.
 __device__ __forceinline__ void append16(nb16_state& s, unsigned entry) {
-    unsigned const c2 = min(s.c2, s.last2);
-    unsigned const o = (((c2 & ~31u) << 6) + s.base2) | (c2 & 31u);
+    unsigned const qa = min(s.q, s.qmax);
+    unsigned const o = (s.lane32 & NB16_LANE_BITS) | (qa & ~NB16_LANE_BITS);
     *reinterpret_cast<uint16_t*>(s.list + o) = uint16_t(entry);
-    s.c2 += 2;
+    s.q = (s.q + 2u) | NB16_LANE_BITS;
 }
 
 // the list entry is the record's byte offset in the full-record union (16
@@ -344,41 +355,9 @@ __device__ __forceinline__ void staged_build_segment(
     if (threadIdx.x == 0) atomicMax(&stats[1], U + 1);
     if (U + 1 > sp.lds_cap) return;   // uniform: the caller enlarges the staging and rebuilds
 
-    unsigned const nx = p.grid.ncell[0], ny = p.grid.ncell[1], nz = p.grid.ncell[2];
+    unsigned const nx = p.grid.ncell[0];
     uint32_t const rr0 = ISO ? p.rr16[0] : __float_as_uint(p.rr_cut_skin[0]);
     unsigned const hh = st.q.h + 2, T = st.own_pre[st.q.h];
-    // Cubic box: faces of the union's cells in 16-bit units, to skip the cells
-    // of a row that no candidate of the rule can reach.  Face f of an axis
-    // with n cells, below cell c = f mod n, lies at lo = ceil(c 2^32 / n), the
-    // lowest fixed-point coordinate of cell c: its word holds lo >> 16 (the
-    // lowest top half inside c) above (lo - 1) >> 16 (the highest below c).
-    // x faces of union cells x0 - 1 .. x0 + w: 0 .. w + 2; y faces of rows
-    // y0 - 1 .. y0 + h: w + 3 ..; z faces below and above the plane: last two.
-    __shared__ uint32_t s_face[(SEG_MAX + 3) + (SEG_RMAX + 3) + 2];
-    unsigned const fx = st.q.w + 3, fy = st.q.h + 3;
-    if (ISO) {
-        for (unsigned f = threadIdx.x; f < fx + fy + 2; f += blockDim.x) {
-            unsigned c, n;
-            if (f < fx) {
-                c = st.q.x0 + nx - 1 + f;
-                n = nx;
-            } else if (f < fx + fy) {
-                c = st.q.y0 + ny - 1 + (f - fx);
-                n = ny;
-            } else {
-                c = st.q.z + (f - fx - fy);
-                n = nz;
-            }
-            c %= n;
-            uint32_t const lo = uint32_t(((uint64_t(c) << 32) + n - 1) / n);
-            s_face[f] = (lo & 0xffff0000u) | (((lo - 1u) >> 16) & 0xffffu);
-        }
-        __syncthreads();
-    }
-    // largest threshold over the species pairs (the skip test is conservative)
-    uint32_t rr_max = rr0;
-    if (MULTI)
-        for (unsigned ab = 1; ab < p.ntype * p.ntype; ++ab) rr_max = max(rr_max, s_rr[ab]);
     // the segment's particles follow the first own row in memory (its rows
     // y0 .. y0 + h - 1 do not wrap), so the list offsets from the lane block
     // of its first particle fit 32 bits (particles of one z-plane of cells)
@@ -399,62 +378,29 @@ __device__ __forceinline__ void staged_build_segment(
             s.ri = full;
         }
         s.ti = (full.w & 3u) * p.ntype;
-        s.c2 = 0;
-        s.last2 = 2 * (cap - 1);
+        unsigned const H = 2 * ((i & ~(WAVE - 1)) - i0) * cap;   // multiple of 2048
+        s.q0 = H | NB16_LANE_BITS;
+        s.q = s.q0;
+        s.qmax = s.q0 + (((cap - 1) >> 4) << 11) + (((cap - 1) & 15u) << 1);
+        s.lane32 = (i & (WAVE - 1)) * (2 * NB16_BLOCK);
         s.list = wlist;
-        s.base2 = 2 * ((i & ~(WAVE - 1)) - i0) * cap + (i & (WAVE - 1)) * (2 * NB16_BLOCK);
         unsigned const k = cell_component(full.x, nx) - st.q.x0;   // 0 .. w - 1
-        // Cubic box: distances (16-bit units) from the particle's top halves
-        // to the faces of its cell, each a lower bound of |d| of the rule for
-        // every candidate beyond that face.  They hold for the minimum image
-        // when the candidate lies within two cells (< L / 2), i.e. with at
-        // least 5 cells along the axis; with fewer the bound is 0.
-        unsigned axl = 0, axh = 0, ayl = 0, ayh = 0, azl = 0, azh = 0;
-        if (ISO) {
-            unsigned const hx = s.ri.x & 0xffffu, hy = s.ri.x >> 16, hz = s.ri.y & 0xffffu;
-            if (nx >= 5) {
-                axl = (hx - (s_face[k + 1] & 0xffffu)) & 0xffffu;
-                axh = ((s_face[k + 2] >> 16) - hx) & 0xffffu;
-            }
-            if (ny >= 5) {
-                ayl = (hy - (s_face[fx + j + 1] & 0xffffu)) & 0xffffu;
-                ayh = ((s_face[fx + j + 2] >> 16) - hy) & 0xffffu;
-            }
-            if (nz >= 5) {
-                azl = (hz - (s_face[fx + fy] & 0xffffu)) & 0xffffu;
-                azh = ((s_face[fx + fy + 1] >> 16) - hz) & 0xffffu;
-            }
-        }
         // the 9 rows (dz, dy) in lexicographic order; only the own row holds
         // the particle itself
         for (unsigned dz = 0; dz < 3; ++dz)
             for (unsigned dy = 0; dy < 3; ++dy) {
                 unsigned const ur = dz * hh + j + dy;
-                unsigned kb = k, ke = k + 3;
-                if (ISO) {
-                    // skip the row's end cells (or the whole row) when even
-                    // their nearest face is beyond every threshold
-                    unsigned const my = dy == 0 ? ayl : (dy == 2 ? ayh : 0u);
-                    unsigned const mz = dz == 0 ? azl : (dz == 2 ? azh : 0u);
-                    unsigned const ryz = my * my + mz * mz;
-                    if (ryz > rr_max) {
-                        ke = kb;
-                    } else {
-                        if (axl * axl + ryz > rr_max) kb = k + 1;
-                        if (axh * axh + ryz > rr_max) ke = k + 2;
-                    }
-                }
-                unsigned const b = s_bound[ur * SEG_BOUND + kb];
-                unsigned const e = kb == ke ? b : s_bound[ur * SEG_BOUND + ke];
+                unsigned const b = s_bound[ur * SEG_BOUND + k];
+                unsigned const e = s_bound[ur * SEG_BOUND + k + 3];
                 if (ur == self)
                     scan16<MULTI, true>(s, b, e, s_rec, p, s_rr, rr0);
                 else
                     scan16<MULTI, false>(s, b, e, s_rec, p, s_rr, rr0);
             }
-        unsigned const c = s.c2 / 2;
+        unsigned const c = nb16_count(s);
         // pad with the sentinel to a multiple of 8 (cap is a multiple of 16)
         if (c <= cap)
-            while (s.c2 % 16) append16(s, U * 16u);
+            while (s.q & 15u) append16(s, U * 16u);
         count[i] = c;
         cmax = max(cmax, c);
     }
