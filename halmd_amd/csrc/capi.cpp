@@ -257,6 +257,12 @@ HALMD_API int halmd_mdsim_potentials_pair_mie_create(unsigned nt1, unsigned nt2,
             new potential(POTENTIAL_MIE, nt1, nt2, eps, sigma, nullptr, nullptr, index_m, index_n));
     });
 }
+HALMD_API int halmd_mdsim_potentials_pair_modify_hard_core(halmd_potential* p, const double* r_core_sigma) {
+    return guard([&] {
+        nonnull(r_core_sigma, "r_core_sigma");
+        as<potential>(p)->modify_hard_core(r_core_sigma);
+    });
+}
 HALMD_API int halmd_mdsim_potentials_pair_truncate(halmd_potential* p, const char* truncation, const double* cutoff,
                                                    double smoothing) {
     return guard([&] {

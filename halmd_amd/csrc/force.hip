@@ -75,13 +75,27 @@ __device__ __forceinline__ void pair_eval(float rr, float morse_d, pair_param co
     if (KIND == POTENTIAL_POWER_LAW) {
         // power law (power_law_kernel.hpp:46-60) and Mie (mie_kernel.hpp:48-62)
         // in one form: U = c_m (sigma/r)^m - c_n (sigma/r)^n,
-        // fval = (m c_m (sigma/r)^m - n c_n (sigma/r)^n) / r^2 (c_n = n = 0 for the power law)
-        float const rri = pp.p[2] * rcp_nr(rr);
+        // fval = (m c_m (sigma/r)^m - n c_n (sigma/r)^n) / r^2 (c_n = n = 0 for the power law),
+        // With a hard core r_core = p[3] (adapters/hard_core_kernel.hpp:35-44)
+        // the argument is r_s = r - r_core, and fval = (...) / (r_s r).
+        float rri, inv;
+        if (pp.p[3] != 0.0f) {
+            float const r = sqrtf(rr);
+            float const rs = r - pp.p[3];
+            rri = pp.p[2] * rcp_nr(rs * rs);
+            inv = rcp_nr(rs * r);
+        } else {
+            inv = rcp_nr(rr);
+            rri = pp.p[2] * inv;
+        }
+        // factored as the reference's Mie kernel, (sigma/r)^n [c_m m (sigma/r)^(m-n) - c_n n]:
+        // near the zero of the force the difference is of a rounded and an
+        // exact term, not of two rounded large ones
         unsigned const m = unsigned(pp.p_lo[0]), n = unsigned(pp.p_lo[1]);
-        float const a = pp.p[0] * pow_half(rri, m);
-        float const b = pp.p[1] * pow_half(rri, n);
-        fval = (pp.p_lo[0] * a - pp.p_lo[1] * b) * rcp_nr(rr);
-        en = a - b;
+        float const sn = pow_half(rri, n);
+        float const smn = pow_half(rri, m - n);
+        fval = sn * (pp.p_lo[0] * pp.p[0] * smn - pp.p_lo[1] * pp.p[1]) * inv;
+        en = sn * (pp.p[0] * smn - pp.p[1]);
     } else if (KIND == POTENTIAL_LENNARD_JONES) {
         // rri = sigma^2 / rr; fval = 48 eps rri ri6 (ri6 - 1/2) / sigma^2; U = 4 eps ri6 (ri6 - 1)
         float const rri = pp.p[2] * rcp_nr(rr);

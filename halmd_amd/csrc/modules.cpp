@@ -341,6 +341,7 @@ potential::potential(int kind_, unsigned nt1, unsigned nt2, double const* eps, d
     }
     index_m.assign(n2, 0);
     index_n.assign(n2, 0);
+    r_core.assign(n2, 0.0);
     epsilon_c = epsilon;
     if (kind == POTENTIAL_POWER_LAW) {
         // host/potentials/pair/power_law.cpp
@@ -373,7 +374,29 @@ static double pow_uint(double x, unsigned k) {
     return r;
 }
 
+void potential::modify_hard_core(double const* rcs) {
+    require(trunc < 0, "hard_core: modify the potential before truncating it");
+    require(kind != POTENTIAL_MORSE, "hard_core: Lennard-Jones, power law or Mie potentials");
+    for (size_t i = 0; i < r_core.size(); ++i) {
+        require(rcs[i] >= 0 && std::isfinite(rcs[i]), "hard_core: radius must be non-negative");
+        r_core[i] = rcs[i] * sigma[i];   // adapters/hard_core.hpp: element_prod(core, sigma)
+    }
+}
+
+// through the hard core when there is one (adapters/hard_core.hpp:55-63):
+// U(r - r_core), fval scaled by r_s / r
 void potential::base(double rr, unsigned ab, double& fval, double& en) const {
+    if (r_core[ab] == 0) {
+        base_plain(rr, ab, fval, en);
+        return;
+    }
+    double const r = std::sqrt(rr);
+    double const rs = r - r_core[ab];
+    base_plain(rs * rs, ab, fval, en);
+    fval *= rs / r;
+}
+
+void potential::base_plain(double rr, unsigned ab, double& fval, double& en) const {
     if (kind == POTENTIAL_LENNARD_JONES) {
         double const s2 = sigma[ab] * sigma[ab];
         double const rri = s2 / rr;
@@ -471,7 +494,10 @@ force_params potential::device_params(box const& b) const {
     require(ntype1 == ntype2, "potential: ntype1 != ntype2 is not supported by the device kernels");
     force_params P;
     std::memset(&P, 0, sizeof(P));
-    P.kind = kind == POTENTIAL_MIE ? POTENTIAL_POWER_LAW : kind;   // one device form
+    // power law and Mie share one device form, and so does any potential with
+    // a hard core (Lennard-Jones as Mie 12-6 with C = 4)
+    bool const core = std::any_of(r_core.begin(), r_core.end(), [](double x) { return x != 0; });
+    P.kind = (kind == POTENTIAL_MIE || core) ? POTENTIAL_POWER_LAW : kind;
     P.trunc = trunc;
     P.ntype = ntype2;
     P.rri_smooth = trunc == TRUNC_SMOOTH_R4 ? float(1 / (h * h)) : 0.0f;
@@ -484,7 +510,15 @@ force_params potential::device_params(box const& b) const {
             unsigned const ab = a * ntype2 + c;
             pair_param& pp = P.pair[ab];
             double const eps = epsilon[ab], sig = sigma[ab];
-            if (kind == POTENTIAL_LENNARD_JONES) {
+            if (P.kind == POTENTIAL_POWER_LAW) {
+                bool const lj = kind == POTENTIAL_LENNARD_JONES, mie = kind == POTENTIAL_MIE;
+                pp.p[0] = float(lj ? 4 * eps : epsilon_c[ab]);
+                pp.p[1] = (lj || mie) ? float(lj ? 4 * eps : epsilon_c[ab]) : 0.0f;
+                pp.p[2] = float(sig * sig);
+                pp.p[3] = float(r_core[ab]);
+                pp.p_lo[0] = float(lj ? 12u : index_m[ab]);
+                pp.p_lo[1] = float(lj ? 6u : (mie ? index_n[ab] : 0u));
+            } else if (kind == POTENTIAL_LENNARD_JONES) {
                 pp.p[0] = float(48 * eps / (sig * sig));
                 pp.p[1] = float(4 * eps);
                 pp.p[2] = float(sig * sig);
@@ -494,14 +528,6 @@ force_params potential::device_params(box const& b) const {
                     split_ff(q * (48 * eps / (sig * sig)), P.q_p0[0], P.q_p0[1]);
                     split_ff(q * q * (48 * eps / (sig * sig)), P.q2_p0[0], P.q2_p0[1]);
                 }
-            } else if (kind == POTENTIAL_POWER_LAW || kind == POTENTIAL_MIE) {
-                bool const mie = kind == POTENTIAL_MIE;
-                pp.p[0] = float(epsilon_c[ab]);
-                pp.p[1] = mie ? float(epsilon_c[ab]) : 0.0f;
-                pp.p[2] = float(sig * sig);
-                pp.p[3] = 0;
-                pp.p_lo[0] = float(index_m[ab]);
-                pp.p_lo[1] = mie ? float(index_n[ab]) : 0.0f;
             } else {
                 double const B = distortion[ab];
                 double const A = 2 * B * B - 1;

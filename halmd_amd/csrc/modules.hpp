@@ -151,6 +151,9 @@ public:
     potential(int kind, unsigned ntype1, unsigned ntype2, double const* epsilon, double const* sigma,
               double const* r_min, double const* distortion, unsigned const* index_m = nullptr,
               unsigned const* index_n = nullptr);
+    // hard core (adapters/hard_core.hpp): U(r - r_core) with r_core =
+    // r_core_sigma sigma per pair; before truncate (Lennard-Jones, power law, Mie)
+    void modify_hard_core(double const* r_core_sigma);
     // apply a truncation (TRUNC_*); r_cut_sigma in units of sigma
     void truncate(int trunc, double const* r_cut_sigma, double smoothing);
     int kind, trunc = -1;
@@ -158,6 +161,7 @@ public:
     std::vector<double> epsilon, sigma, r_min, distortion, r_cut_sigma, r_cut, rr_cut, en_cut, force_cut;
     std::vector<unsigned> index_m, index_n;
     std::vector<double> epsilon_c;   // Mie: epsilon C(m, n)
+    std::vector<double> r_core;      // hard core radius (absolute), 0 without
     double h = 0;
     double r_cut_max() const;
     force_params device_params(box const& b) const;
@@ -165,6 +169,7 @@ public:
     void eval(double rr, unsigned a, unsigned b, double& fval, double& en) const;
 private:
     void base(double rr, unsigned ab, double& fval, double& en) const;
+    void base_plain(double rr, unsigned ab, double& fval, double& en) const;
 };
 
 class binning {

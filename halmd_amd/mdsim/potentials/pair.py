@@ -28,6 +28,20 @@ class _pair_potential(Handle):
     _destroy = "halmd_mdsim_potentials_pair_destroy"
     description = "pair potential"
 
+    def modify(self, modification, radius=None):
+        """potential:modify{"hard_core", radius = r_core / sigma}: U(r) -> U(r - r_core)
+        (lua/halmd/mdsim/potentials/pair/adapters/modify.lua.in:38-64); before truncate."""
+        if isinstance(modification, (tuple, list)):
+            modification, radius = modification[0], modification[1] if len(modification) > 1 else radius
+        if modification != "hard_core":
+            raise ValueError(f"unknown modification {modification!r}")
+        if radius is None:
+            raise ValueError("hard_core: radius is required")
+        rc = _species_matrix(radius, self.species)
+        call("halmd_mdsim_potentials_pair_modify_hard_core", self.handle, dptr(rc))
+        self.r_core_sigma = rc.copy()
+        return self
+
     def truncate(self, truncation="shifted", cutoff=None, smoothing=0.005):
         """potential:truncate{truncation, cutoff = r_c / sigma [, smoothing = h]}."""
         if isinstance(truncation, (tuple, list)):

@@ -104,14 +104,15 @@ def batched_specs(count=256, seed=2024, n_range=(500, 32000), density_range=(0.7
     return out
 
 
-def spec_potential(name):
+def spec_potential(name, truncation="shifted", smoothing=0.005):
     """The single-system potential object of an ensemble potential name."""
     from .ensemble import POTENTIALS
     kind, eps, sigma, rc, r_min, dist = POTENTIALS[name]
     if kind == 0:
-        return mdsim.potentials.pair.lennard_jones(epsilon=eps, sigma=sigma).truncate("shifted", cutoff=rc)
+        return mdsim.potentials.pair.lennard_jones(epsilon=eps, sigma=sigma).truncate(truncation, cutoff=rc,
+                                                                                      smoothing=smoothing)
     return mdsim.potentials.pair.morse(epsilon=eps, sigma=sigma, r_min=r_min, distortion=dist).truncate(
-        "shifted", cutoff=rc)
+        truncation, cutoff=rc, smoothing=smoothing)
 
 
 def ka_species(n, seed, fraction_b=0.2):
@@ -128,8 +129,9 @@ def spec_species(spec):
 
 def single_from_spec(spec, timestep=0.003, skin=0.3):
     """The system of `spec` alone, through the per-system modules."""
-    return fluid(spec["n"], spec["density"], spec["temperature"], spec_potential(spec["potential"]), skin=skin,
-                 timestep=timestep, seed=spec["seed"], species=spec_species(spec))
+    pot = spec_potential(spec["potential"], spec.get("truncation", "shifted"), spec.get("smoothing", 0.005))
+    return fluid(spec["n"], spec["density"], spec["temperature"], pot, skin=skin, timestep=timestep,
+                 seed=spec["seed"], species=spec_species(spec))
 
 
 def ensemble_from_specs(specs, timestep=0.003, skin=0.3):
@@ -138,7 +140,8 @@ def ensemble_from_specs(specs, timestep=0.003, skin=0.3):
     e = ensemble()
     for sp in specs:
         L, r0, v0, _ = system_inputs(sp["n"], sp["density"], sp["temperature"], sp["seed"])
-        e.add(r0, v0, L[0], potential=sp["potential"], timestep=timestep, skin=skin, species=spec_species(sp))
+        e.add(r0, v0, L[0], potential=sp["potential"], truncation=sp.get("truncation", "shifted"),
+              smoothing=sp.get("smoothing", 0.005), timestep=timestep, skin=skin, species=spec_species(sp))
     e.commit()
     return e
 

@@ -148,6 +148,11 @@ HALMD_API int halmd_mdsim_potentials_pair_power_law_create(unsigned ntype1, unsi
 HALMD_API int halmd_mdsim_potentials_pair_mie_create(unsigned ntype1, unsigned ntype2, const double* epsilon,
                                                      const double* sigma, const unsigned* index_m,
                                                      const unsigned* index_n, halmd_potential** out);
+/* potential:modify{"hard_core", radius = r_core / sigma} (ntype x ntype radii):
+ * U(r) -> U(r - r_core), before truncate; Lennard-Jones, power law, Mie.
+ * lua/halmd/mdsim/potentials/pair/adapters/modify.lua.in:38-64,
+ * host/potentials/pair/adapters/hard_core.hpp:35-63, gpu/.../adapters/hard_core_kernel.hpp */
+HALMD_API int halmd_mdsim_potentials_pair_modify_hard_core(halmd_potential* pot, const double* r_core_sigma);
 /* potential:truncate{truncation, cutoff = r_c / sigma [, smoothing]};
  * truncation: "sharp", "shifted", "force_shifted", "smooth_r4" */
 HALMD_API int halmd_mdsim_potentials_pair_truncate(halmd_potential* pot, const char* truncation,

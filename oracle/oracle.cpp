@@ -100,11 +100,12 @@ struct potential_t {
     int kind = POT_LJ, trunc = TR_SHIFTED, ntype = 1;
     std::vector<double> eps, sigma, sigma2, r_min_sigma, B, r_cut_sigma, r_cut, rr_cut, en_cut, force_cut;
     std::vector<unsigned> index_m, index_n;
+    std::vector<double> r_core;   // hard core (adapters/hard_core.hpp), absolute
     double rri_smooth = 0;
 
     potential_t(int kind_, int trunc_, int ntype_, double const* eps_, double const* sigma_,
                 double const* r_min_, double const* B_, unsigned const* m_, unsigned const* n_,
-                double const* r_cut_sigma_, double h)
+                double const* r_core_sigma_, double const* r_cut_sigma_, double h)
         : kind(kind_), trunc(trunc_), ntype(ntype_)
     {
         int const n2 = ntype * ntype;
@@ -115,8 +116,10 @@ struct potential_t {
         B.assign(n2, 1.0);
         index_m.assign(n2, 0);
         index_n.assign(n2, 0);
+        r_core.assign(n2, 0.0);
         for (int i = 0; i < n2; ++i) {
             sigma2[i] = sigma[i] * sigma[i];
+            if (r_core_sigma_) r_core[i] = r_core_sigma_[i] * sigma[i];   // hard_core.hpp: element_prod(core, sigma)
             if (kind == POT_MORSE) {
                 r_min_sigma[i] = r_min_[i] / sigma[i];   // morse.cpp:53 element_div(r_min, sigma)
                 B[i] = B_[i];
@@ -145,7 +148,20 @@ struct potential_t {
         }
     }
 
+    // the potential at r, through the hard-core adapter when it has one
+    // (host/potentials/pair/adapters/hard_core.hpp:55-63: U(r - r_core), fval scaled by r_s / r)
     void base(double rr, int ab, double& fval, double& en) const {
+        if (r_core[ab] == 0) {
+            base_plain(rr, ab, fval, en);
+            return;
+        }
+        double const r = std::sqrt(rr);
+        double const r_s = r - r_core[ab];
+        base_plain(r_s * r_s, ab, fval, en);
+        fval *= r_s / r;
+    }
+
+    void base_plain(double rr, int ab, double& fval, double& en) const {
         if (kind == POT_LJ) {
             double s2 = sigma2[ab];
             double rri = s2 / rr;
@@ -606,11 +622,12 @@ extern "C" {
 // ===========================================================================
 // index_m: power law index n (POT_POWER_LAW) or Mie repulsion index m; index_n:
 // Mie attraction index n (both ignored by the other kinds)
+// r_core_sigma: hard-core radius in units of sigma per pair (null: none)
 void* oracle_potential_create(int kind, int trunc, int ntype, double const* eps, double const* sigma,
                               double const* r_min, double const* B, unsigned const* index_m, unsigned const* index_n,
-                              double const* r_cut_sigma, double h)
+                              double const* r_core_sigma, double const* r_cut_sigma, double h)
 {
-    return new potential_t(kind, trunc, ntype, eps, sigma, r_min, B, index_m, index_n, r_cut_sigma, h);
+    return new potential_t(kind, trunc, ntype, eps, sigma, r_min, B, index_m, index_n, r_core_sigma, r_cut_sigma, h);
 }
 void oracle_potential_destroy(void* p) { delete static_cast<potential_t*>(p); }
 

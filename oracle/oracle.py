@@ -41,8 +41,8 @@ def lib():
             build()
         L = ctypes.CDLL(_LIB_PATH)
         sig = {
-            "oracle_potential_create": (_vp, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _d, _d, _d, _d, _u, _u, _d,
-                                              ctypes.c_double]),
+            "oracle_potential_create": (_vp, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _d, _d, _d, _d, _u, _u, _vp,
+                                              _d, ctypes.c_double]),
             "oracle_potential_destroy": (None, [_vp]),
             "oracle_potential_eval": (None, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _d, _d, _d]),
             "oracle_potential_cutoffs": (None, [_vp, _d, _d, _d]),
@@ -99,8 +99,9 @@ class Potential:
     """Pair potential with truncation (host semantics, FP64)."""
 
     def __init__(self, kind, truncation, epsilon, sigma, r_cut_sigma, r_min=None, distortion=None, h=0.005,
-                 index=None, index_m=None, index_n=None):
-        """index: power law index n (POT_POWER_LAW); index_m, index_n: Mie indices (POT_MIE)."""
+                 index=None, index_m=None, index_n=None, r_core_sigma=None):
+        """index: power law index n (POT_POWER_LAW); index_m, index_n: Mie indices (POT_MIE);
+        r_core_sigma: hard-core radius / sigma (adapters/hard_core)."""
         eps = np.atleast_2d(_dd(epsilon))
         ntype = eps.shape[0]
         sig = _dd(np.broadcast_to(np.atleast_2d(sigma), (ntype, ntype)))
@@ -112,11 +113,14 @@ class Potential:
                                  dtype=np.uint32)
         n = np.ascontiguousarray(np.broadcast_to(np.atleast_2d(index_n if index_n is not None else 0),
                                                  (ntype, ntype)), dtype=np.uint32)
+        core = None if r_core_sigma is None else _dd(np.broadcast_to(np.atleast_2d(r_core_sigma), (ntype, ntype)))
         self.ntype = ntype
         self.kind = kind
-        self._keep = (eps, sig, rcs, rmin, B, m, n)
+        self._keep = (eps, sig, rcs, rmin, B, m, n, core)
         self.h = lib().oracle_potential_create(kind, TRUNCATIONS[truncation], ntype, _p(eps, _d), _p(sig, _d),
-                                               _p(rmin, _d), _p(B, _d), _p(m, _u), _p(n, _u), _p(rcs, _d), h)
+                                               _p(rmin, _d), _p(B, _d), _p(m, _u), _p(n, _u),
+                                               None if core is None else core.ctypes.data_as(ctypes.c_void_p),
+                                               _p(rcs, _d), h)
         rc = np.zeros(ntype * ntype)
         ec = np.zeros(ntype * ntype)
         fc = np.zeros(ntype * ntype)

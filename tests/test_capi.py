@@ -104,6 +104,25 @@ def test_host_power_law_and_mie_match_golden_tables():
                     assert (x == ref) if ref == 0 else abs(x - ref) <= tol * abs(ref), (key, r, x, ref)
 
 
+def test_host_hard_core_matches_golden_table():
+    from halmd_amd.mdsim.potentials import pair
+    g = json.load(open(os.path.join(GOLDEN, "potential_tables.json")))["power_law_hard_core"]
+    p = pair.power_law(epsilon=g["epsilon"], sigma=g["sigma"], index=g["index"])
+    p.modify("hard_core", radius=g["r_core_sigma"]).truncate("shifted", cutoff=g["r_cut_sigma"])
+    for key, (a, b) in (("aa", (0, 0)), ("ab", (0, 1)), ("bb", (1, 1))):
+        for r, fref, uref in g["tables"][key]:
+            rs = r / g["sigma"][a][b]
+            tol = np.finfo(float).eps * g["index"][a][b] * (1 + rs / (rs - g["r_core_sigma"][a][b]))
+            f, u = p(r * r, a, b)
+            for x, ref in ((f[0], fref), (u[0], uref)):
+                assert (x == ref) if ref == 0 else abs(x - ref) <= tol * abs(ref), (key, r, x, ref)
+    # modify comes before truncate, and Morse has no hard-core form
+    with pytest.raises(_lib.HalmdError):
+        pair.lennard_jones().truncate("shifted", cutoff=2.5).modify("hard_core", radius=0.3)
+    with pytest.raises(_lib.HalmdError):
+        pair.morse().modify("hard_core", radius=0.3)
+
+
 def test_mie_rejects_odd_or_inverted_indices():
     from halmd_amd.mdsim.potentials import pair
     with pytest.raises(_lib.HalmdError):

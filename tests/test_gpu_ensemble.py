@@ -52,6 +52,27 @@ def test_each_system_same_alone_and_in_the_mix():
             assert e.rebuilds(k) == w.neighbour.rebuilds
 
 
+def test_truncations_same_alone_and_in_the_mix():
+    # every truncation in batched mode, smooth_r4 included, with the systems of
+    # different truncations in one ensemble: bitwise the single path
+    specs = [dict(n=4000, density=0.8442, temperature=1.2, potential="lj", seed=21, truncation="smooth_r4"),
+             dict(n=3000, density=1.0, temperature=0.8, potential="lj", seed=22, truncation="force_shifted"),
+             dict(n=2500, density=0.9, temperature=1.0, potential="morse", seed=23, truncation="smooth_r4"),
+             dict(n=5000, density=0.9, temperature=1.0, potential="lj", seed=24, truncation="sharp"),
+             dict(n=4000, density=1.2, temperature=0.8, potential="ka", seed=25, truncation="smooth_r4")]
+    steps = 120
+    e = workloads.ensemble_from_specs(specs, timestep=DT)
+    e.step(steps)
+    en_pot, _ = e.energies()
+    for k, spec in enumerate(specs):
+        r_e, v_e, im_e = e.state(k)
+        r_a, v_a, im_a, w = _alone(spec, steps)
+        assert np.array_equal(r_e, r_a), (k, spec["truncation"])
+        assert np.array_equal(v_e, v_a) and np.array_equal(im_e, im_a)
+        u = w.thermodynamics.en_pot()
+        assert abs(en_pot[k] - u) < 1e-5 * max(1.0, abs(u))
+
+
 def test_ensemble_energies_match_single_path():
     e = workloads.ensemble_from_specs(SPECS, timestep=DT)
     e.step(40)

@@ -312,7 +312,8 @@ def test_mie_and_power_law_forces_match_fp64(oracle, kind):
 @pytest.mark.parametrize("kind,truncation", [("lj", "shifted"), ("lj", "force_shifted"), ("lj", "smooth_r4"),
                                              ("ka", "shifted"), ("morse", "shifted"), ("power_law", "shifted"),
                                              ("power_law_odd", "force_shifted"), ("mie", "shifted"),
-                                             ("mie", "smooth_r4")])
+                                             ("mie", "smooth_r4"), ("power_law_hard_core", "shifted"),
+                                             ("lj_hard_core", "force_shifted")])
 def test_pair_table(oracle, kind, truncation):
     # isolated pairs at known separations (the reference's neighbour_chain
     # idea, test/unit/mdsim/potentials/pair/gpu/neighbour_chain.hpp): the force
@@ -335,6 +336,16 @@ def test_pair_table(oracle, kind, truncation):
         pot_d = mdsim.potentials.pair.power_law(epsilon=eps, sigma=sig, index=idx)
         pot = oracle.Potential(oracle.POT_POWER_LAW, truncation, eps, sig, 2.5, index=idx)
         cut, r_lo, nsp = 2.5, 0.8, 2
+    elif kind == "power_law_hard_core":
+        idx, core = [[6, 12], [12, 7]], [[0.375, 0.3], [0.3, 0.25]]
+        eps, sig = [[1.0, 0.7], [0.7, 0.5]], [[1.0, 1.1], [1.1, 1.2]]
+        pot_d = mdsim.potentials.pair.power_law(epsilon=eps, sigma=sig, index=idx).modify("hard_core", radius=core)
+        pot = oracle.Potential(oracle.POT_POWER_LAW, truncation, eps, sig, 2.5, index=idx, r_core_sigma=core)
+        cut, r_lo, nsp = 2.5, 0.95, 2
+    elif kind == "lj_hard_core":
+        pot_d = mdsim.potentials.pair.lennard_jones(epsilon=1.0, sigma=1.0).modify("hard_core", radius=0.3)
+        pot = oracle.Potential(oracle.POT_LJ, truncation, 1.0, 1.0, 2.5, r_core_sigma=0.3)
+        cut, r_lo, nsp = 2.5, 1.05, 1
     elif kind == "mie":
         im, in_ = [[12, 10], [10, 8]], [[6, 4], [4, 2]]
         eps, sig = [[1.0, 0.7], [0.7, 0.5]], [[1.0, 1.1], [1.1, 1.2]]

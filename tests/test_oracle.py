@@ -74,6 +74,35 @@ def test_mie_tables(oracle):
             assert _close_fraction(u[0], uref, tol), (pair, r, u[0], uref)
 
 
+def test_power_law_hard_core_tables(oracle):
+    g = json.load(open(os.path.join(GOLDEN, "potential_tables.json")))["power_law_hard_core"]
+    pot = oracle.Potential(oracle.POT_POWER_LAW, g["truncation"], g["epsilon"], g["sigma"], g["r_cut_sigma"],
+                           index=g["index"], r_core_sigma=g["r_core_sigma"])
+    for pair, (a, b) in (("aa", (0, 0)), ("ab", (0, 1)), ("bb", (1, 1))):
+        for r, fref, uref in g["tables"][pair]:
+            rs = r / g["sigma"][a][b]
+            tol = EPS * g["index"][a][b] * (1 + rs / (rs - g["r_core_sigma"][a][b]))
+            f, u = pot(r * r, a, b)
+            assert _close_fraction(f[0], fref, tol), (pair, r, f[0], fref)
+            assert _close_fraction(u[0], uref, tol), (pair, r, u[0], uref)
+
+
+def test_hard_core_lennard_jones_is_shifted_argument(oracle):
+    # hard core over Lennard-Jones: U(r) = U_LJ(r - r_core), -U'(r)/r by
+    # central differences
+    hc = oracle.Potential(oracle.POT_LJ, "sharp", 1.0, 1.0, 3.0, r_core_sigma=0.4)
+    lj = oracle.Potential(oracle.POT_LJ, "sharp", 1.0, 1.0, 3.0)
+    for r in (1.1, 1.3, 1.6, 2.2, 2.9):
+        _, u = hc(r * r, 0, 0)
+        _, ul = lj((r - 0.4) ** 2, 0, 0)
+        assert u[0] == ul[0]
+        h = 1e-6
+        _, up = hc((r + h) ** 2, 0, 0)
+        _, um = hc((r - h) ** 2, 0, 0)
+        f, _ = hc(r * r, 0, 0)
+        assert abs(f[0] - (-(up[0] - um[0]) / (2 * h) / r)) < 1e-6 * max(1.0, abs(f[0]))
+
+
 @pytest.mark.parametrize("kind,args", [("power_law", dict(index=7)), ("mie", dict(index_m=10, index_n=4))])
 def test_power_potentials_derivative_and_mie_limit(oracle, kind, args):
     # fval = -U'(r)/r by central differences (odd power law index: the sqrt
