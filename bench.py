@@ -375,12 +375,23 @@ def main():
     f_ms, f_n = kt["force"]
     f_avg = f_ms / max(f_n, 1)
     # algorithmic bytes per particle for the force kernel (fixed definition of
-    # unique compulsory traffic with float-float state): r_i (hi+lo, 24 B),
-    # (n_nb + 1) list words of 4 B (entries + count), force (12 B)
-    force_bytes = N * (24 + 4 * (n_nb + 1) + 12)
-    achieved = force_bytes / (f_avg * 1e-3) / 1e9
+    # unique compulsory traffic with float-float state)
     staged = w.neighbour.entry_bytes == 2
-    kname = ("pair_force_staged_kernel<0, 1, false, true, false>" if staged
+    # the Verlet update ran inside the force kernel (all but the steps after a
+    # sample, whose force was already evaluated)
+    fused = kt["integrate"][1] * 10 < kt["force"][1]
+    if fused:
+        # force kernel that integrates each particle as it finishes it: r_i
+        # (hi+lo, 24 B) and (n_nb + 1) list words of 4 B for the force, r and v
+        # read and written (96 B), r0 for the displacement (12 B); the force
+        # itself never goes to memory
+        force_bytes = N * (24 + 4 * (n_nb + 1) + 96 + 12)
+    else:
+        # (unique compulsory traffic with float-float state): r_i (hi+lo, 24 B),
+        # (n_nb + 1) list words of 4 B (entries + count), force (12 B)
+        force_bytes = N * (24 + 4 * (n_nb + 1) + 12)
+    achieved = force_bytes / (f_avg * 1e-3) / 1e9
+    kname = ("pair_force_staged_kernel<0, 1, false, true, false, %s>" % ("true" if fused else "false") if staged
              else "pair_force_kernel<0, 1, false>")
     prof = counter_summary(kname)
     traffic = prof.get("traffic_bytes_per_launch")

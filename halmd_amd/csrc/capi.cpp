@@ -162,6 +162,7 @@ HALMD_API int halmd_mdsim_particle_get_position_fixed(halmd_particle* hp, uint32
         nonnull(u, "u");
         auto* p = as<particle>(hp);
         require(p->has_box, "particle: no box attached yet");
+        p->observe();
         std::vector<uint4> h(p->n);
         p->posq.download(h.data(), p->n, p->stream);
         p->sync();
@@ -221,6 +222,9 @@ HALMD_API int halmd_mdsim_particle_rearrange(halmd_particle* hp, const unsigned*
 // ---- potentials -------------------------------------------------------------------
 HALMD_API int halmd_mdsim_particle_set_speculative_force(halmd_particle* p, int enable) {
     return guard([&] { as<particle>(p)->speculative_force = enable != 0; });
+}
+HALMD_API int halmd_mdsim_particle_set_fused_step(halmd_particle* p, int enable) {
+    return guard([&] { as<particle>(p)->fused_step = enable != 0; });
 }
 HALMD_API int halmd_mdsim_potentials_pair_lennard_jones_create(unsigned nt1, unsigned nt2, const double* eps,
                                                                const double* sigma, halmd_potential** out) {

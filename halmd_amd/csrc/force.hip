@@ -18,6 +18,7 @@
 // dsfloat accumulator).  Energy and stress (aux) are FP32 per particle
 // (compensated the same way in the cell-segment kernel); the reductions over
 // particles are FP64.
+#include "integrate_device.hpp"
 #include "segment.hpp"
 
 namespace halmd {
@@ -268,13 +269,17 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
 // P); shared by the single-system kernel and the ensemble kernel.
 constexpr unsigned SPLIT_MAX = 8;
 
-template <int KIND, int TRUNC, bool AUX, bool ISO, bool MULTI>
+// FUSE (no aux): each particle is integrated (fused_integrate) as soon as its
+// force is final, instead of having its force written.
+template <int KIND, int TRUNC, bool AUX, bool ISO, bool MULTI, bool FUSE = false>
 __device__ __forceinline__ void staged_force_segment(
     uint4 const* __restrict__ posq, uint32_t const* __restrict__ off, seg_params const& sp, unsigned seg,
     uint16_t const* __restrict__ list, uint32_t const* __restrict__ count, unsigned cap, force_params const& P,
     bool force_zero, float aux_weight, float4* __restrict__ force, float* __restrict__ stress, unsigned stride,
-    uint4* s_rec, seg_stage& st, pair_param* s_pair)
+    uint4* s_rec, seg_stage& st, pair_param* s_pair, fused_integrate const* __restrict__ F = nullptr)
 {
+    static_assert(!(FUSE && AUX), "the fused step has no aux path");
+    float d2max = 0.0f;   // FUSE: the lane's largest squared displacement
     constexpr bool LJQ =
         ISO && KIND == POTENTIAL_LENNARD_JONES && (TRUNC == TRUNC_SHARP || TRUNC == TRUNC_SHIFTED);
     constexpr bool P0_OUT = LJQ && !MULTI;   // 48 eps / sigma^2 applied per particle
@@ -452,6 +457,11 @@ __device__ __forceinline__ void staged_force_segment(
             sc[0] = aux_weight * s0; sc[1] = aux_weight * s1; sc[2] = aux_weight * s2;
             sc[3] = aux_weight * s3; sc[4] = aux_weight * s4; sc[5] = aux_weight * s5;
         }
+        if (FUSE) {
+            d2max = fmaxf(d2max, integrate_particle_f(F->pos, F->vel, f, F->image, F->posq_out, i, F->ip,
+                                                      F->pre_half_dt, nullptr, F->r0));
+            continue;
+        }
         if (!force_zero) {
             float4 const old = force[i];
             f.x += old.x; f.y += old.y; f.z += old.z; f.w += old.w;
@@ -465,6 +475,11 @@ __device__ __forceinline__ void staged_force_segment(
 #pragma unroll
             for (int c = 0; c < 6; ++c) stress[c * size_t(stride) + i] = sc[c];
         }
+    }
+    if (FUSE) {
+        // every lane of every wave reaches this point (the loop is wave-uniform)
+        wave_max_to(d2max, disp_slot(F->disp_write));
+        if (blockIdx.x == 0 && threadIdx.x < DISP_SLOTS) F->disp_clear[threadIdx.x * DISP_STRIDE] = 0u;
     }
 }
 // Speculative launch: abort_slots holds the per-wave displacement maxima of
@@ -481,22 +496,23 @@ __device__ __forceinline__ bool displacement_aborts(uint32_t const* __restrict__
     return m > bits;
 }
 
-template <int KIND, int TRUNC, bool AUX, bool ISO, bool MULTI>
+template <int KIND, int TRUNC, bool AUX, bool ISO, bool MULTI, bool FUSE>
 __global__ __launch_bounds__(SEG_THREADS_MAX) void pair_force_staged_kernel(
     uint4 const* __restrict__ posq, uint32_t const* __restrict__ off, seg_params sp,
     uint16_t const* __restrict__ list, uint32_t const* __restrict__ count, unsigned cap, force_params P,
     bool force_zero, float aux_weight, float4* __restrict__ force, float* __restrict__ stress, unsigned stride,
-    uint32_t const* __restrict__ abort_slots, uint32_t abort_bits)
+    uint32_t const* __restrict__ abort_slots, uint32_t abort_bits, fused_integrate F)
 {
     extern __shared__ uint4 s_rec[];
     __shared__ seg_stage st;
     __shared__ pair_param s_pair[MAX_TYPES * MAX_TYPES];
 
-    unsigned const seg = seg_of_block(sp, blockIdx.x);
+    unsigned const seg = seg_of_block(sp, blockIdx.x);   // workgroup 0 has segment 0
     if (seg >= sp.nseg) return;   // the whole workgroup
     if (displacement_aborts(abort_slots, abort_bits)) return;   // the whole workgroup
-    staged_force_segment<KIND, TRUNC, AUX, ISO, MULTI>(posq, off, sp, seg, list, count, cap, P, force_zero,
-                                                       aux_weight, force, stress, stride, s_rec, st, s_pair);
+    staged_force_segment<KIND, TRUNC, AUX, ISO, MULTI, FUSE>(posq, off, sp, seg, list, count, cap, P, force_zero,
+                                                             aux_weight, force, stress, stride, s_rec, st, s_pair,
+                                                             &F);
 }
 
 
@@ -631,6 +647,7 @@ struct staged_launch {
     unsigned stride;
     uint32_t const* abort_slots;
     uint32_t abort_bits;
+    fused_integrate const* fuse;
     hipStream_t s;
 };
 
@@ -638,9 +655,17 @@ template <int KIND, int TRUNC, bool AUX, bool ISO, bool MULTI>
 void staged_go(staged_launch const& a) {
     size_t const lds = size_t(a.sp.lds_cap) * sizeof(uint4);
     unsigned const grid = SEG_XCD * a.sp.chunk;
-    pair_force_staged_kernel<KIND, TRUNC, AUX, ISO, MULTI><<<grid, a.sp.threads, lds, a.s>>>(
+    if constexpr (!AUX) {
+        if (a.fuse) {
+            pair_force_staged_kernel<KIND, TRUNC, false, ISO, MULTI, true><<<grid, a.sp.threads, lds, a.s>>>(
+                a.posq, a.off, a.sp, a.list, a.count, a.cap, a.p, true, a.w, a.force, a.stress, a.stride,
+                a.abort_slots, a.abort_bits, *a.fuse);
+            return;
+        }
+    }
+    pair_force_staged_kernel<KIND, TRUNC, AUX, ISO, MULTI, false><<<grid, a.sp.threads, lds, a.s>>>(
         a.posq, a.off, a.sp, a.list, a.count, a.cap, a.p, a.force_zero, a.w, a.force, a.stress, a.stride,
-        a.abort_slots, a.abort_bits);
+        a.abort_slots, a.abort_bits, fused_integrate{});
 }
 
 template <int KIND, int TRUNC>
@@ -687,10 +712,11 @@ void launch_pair_force(uint4 const* posq, unsigned n, uint32_t const* list, uint
 void launch_pair_force_staged(uint4 const* posq, uint32_t const* offset, seg_params const& sp, uint16_t const* list,
                               uint32_t const* count, unsigned cap, force_params const& p, bool aux, bool force_zero,
                               float aux_weight, float4* force, float* stress, unsigned stride, hipStream_t s,
-                              uint32_t const* abort_slots, uint32_t abort_bits) {
+                              uint32_t const* abort_slots, uint32_t abort_bits, fused_integrate const* fuse) {
     if (!sp.nseg) return;
-    staged_launch const a{posq,  offset, sp,     list,   count,       cap,        p, aux, force_zero,
-                          aux_weight, force, stress, stride, abort_slots, abort_bits, s};
+    if (fuse && aux) throw error(HALMD_ERR_INVALID_ARGUMENT, "pair force: the fused step has no aux path");
+    staged_launch const a{posq,       offset, sp,     list,   count,       cap,        p,    aux, force_zero,
+                          aux_weight, force,  stress, stride, abort_slots, abort_bits, fuse, s};
     if (p.kind == POTENTIAL_LENNARD_JONES)
         staged_trunc<POTENTIAL_LENNARD_JONES>(a);
     else if (p.kind == POTENTIAL_MORSE)

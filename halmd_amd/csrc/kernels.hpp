@@ -231,13 +231,40 @@ void launch_pair_force(uint4 const* posq, unsigned n, uint32_t const* list, uint
                        force_params const& p, bool aux, bool force_zero, float aux_weight, float4* force,
                        float* stress, unsigned stride, hipStream_t s);
 
+// velocity-Verlet step parameters (integrate.hip, and the fused force kernel)
+struct integrate_params {
+    float dt_hi, dt_lo, half_dt;
+    float L_hi[3], L_lo[3], invL[3];
+    quant_params qp;
+};
+
+// The velocity-Verlet step of each particle, done by the force kernel as soon
+// as the particle's force is summed (instead of writing the force): the
+// deferred finalize kick pre_half_dt and the half kick with that force, the
+// drift, wrap and image, the fixed-point snapshot into posq_out (not the posq
+// the kernel stages from), and the displacement from r0 folded into the slot
+// set disp_write; disp_clear is zeroed for the update after.
+struct fused_integrate {
+    float4* pos;
+    float4* vel;
+    int4* image;
+    uint4* posq_out;
+    float4 const* r0;
+    uint32_t* disp_write;
+    uint32_t* disp_clear;
+    integrate_params ip;
+    float pre_half_dt;
+};
+
 // the same over 16-bit slot lists of cell segments (particles in cell order);
 // abort_slots (optional): a speculative launch that does nothing when the
-// maximum of the DISP_SLOTS displacement words there exceeds abort_bits
+// maximum of the DISP_SLOTS displacement words there exceeds abort_bits;
+// fuse (optional, no aux): integrate each particle instead of writing its force
 void launch_pair_force_staged(uint4 const* posq, uint32_t const* offset, seg_params const& sp, uint16_t const* list,
                               uint32_t const* count, unsigned cap, force_params const& p, bool aux, bool force_zero,
                               float aux_weight, float4* force, float* stress, unsigned stride, hipStream_t s,
-                              uint32_t const* abort_slots = nullptr, uint32_t abort_bits = 0);
+                              uint32_t const* abort_slots = nullptr, uint32_t abort_bits = 0,
+                              fused_integrate const* fuse = nullptr);
 
 // ---------------------------------------------------------------------------
 // integrators (integrate.hip)
@@ -249,11 +276,6 @@ void launch_pair_force_staged(uint4 const* posq, uint32_t const* offset, seg_par
 constexpr unsigned DISP_SLOTS = 64;
 constexpr unsigned DISP_STRIDE = 64;
 constexpr unsigned DISP_SET = DISP_SLOTS * DISP_STRIDE;   // words per slot set
-struct integrate_params {
-    float dt_hi, dt_lo, half_dt;
-    float L_hi[3], L_lo[3], invL[3];
-    quant_params qp;
-};
 // kick-drift-wrap and fixed-point snapshot; optional deferred finalize kick
 // of the previous step (pre_half_dt != 0: v += F dt/2m first, the same
 // arithmetic as the finalize kernel), optional velocity pre-scale (*scale,

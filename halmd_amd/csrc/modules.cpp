@@ -83,6 +83,7 @@ particle::particle(unsigned n_, unsigned nspecies_) : n(n_), nspecies(nspecies_)
     vel.resize(2 * size_t(n_pad));
     posq.resize(n_pad);
     posq_alt.resize(n_pad);
+    posq_next.resize(n_pad);
     force.resize(n_pad);
     stress.resize(6 * size_t(n_pad));
     image.resize(n_pad);
@@ -167,6 +168,10 @@ void particle::upload_hilo(dbuf<float4>& b, double const* in, bool) {
 
 void particle::settle_velocity() {
     if (pending_half_dt == 0) return;
+    // the kick is owed with the force of the current positions: evaluate it
+    // first if it is not yet (with en_pot and stress if they were requested
+    // for it, as a step that evaluates its force at once does)
+    if (!force_valid) evaluate_owed_force();
     launch_verlet_finalize(vel, force, n, pending_half_dt, stream);
     pending_half_dt = 0;
 }
@@ -177,13 +182,17 @@ void particle::set_position(double const* r) {
     requantize();
     positions_changed();
 }
-void particle::get_position(double* r) { download_hilo(pos, r); }
+void particle::get_position(double* r) {
+    observe();
+    download_hilo(pos, r);
+}
 void particle::set_velocity(double const* v) {
     pending_half_dt = 0;   // overwritten
     upload_hilo(vel, v, false);
     velocities_changed();
 }
 void particle::get_velocity(double* v) {
+    observe();
     settle_velocity();
     download_hilo(vel, v);
 }
@@ -206,6 +215,7 @@ void particle::set_species(unsigned const* s) {
 }
 
 void particle::get_species(unsigned* s) {
+    observe();
     host_tmp_.resize(2 * size_t(n));
     pos.download(host_tmp_.data(), host_tmp_.size(), stream);
     sync();
@@ -227,6 +237,7 @@ void particle::set_mass(double const* m) {
 }
 
 void particle::get_mass(double* m) {
+    observe();
     host_tmp_.resize(2 * size_t(n));
     vel.download(host_tmp_.data(), host_tmp_.size(), stream);
     sync();
@@ -241,6 +252,7 @@ void particle::set_image(int const* im) {
 }
 
 void particle::get_image(int* im) {
+    observe();
     std::vector<int4> h(n);
     image.download(h.data(), n, stream);
     sync();
@@ -272,39 +284,72 @@ void particle::get_stress_pot(double* s) {
         for (int c = 0; c < 6; ++c) s[6 * i + c] = h[c * size_t(n_pad) + i];
 }
 
-void particle::get_id(unsigned* v) { id.download(v, n, stream); sync(); }
-void particle::get_reverse_id(unsigned* v) { reverse_id.download(v, n, stream); sync(); }
+void particle::get_id(unsigned* v) {
+    observe();
+    id.download(v, n, stream);
+    sync();
+}
+void particle::get_reverse_id(unsigned* v) {
+    observe();
+    reverse_id.download(v, n, stream);
+    sync();
+}
 
 // force protocol of halmd/mdsim/gpu/particle.cpp:221-253: the first force
 // module overwrites, later ones accumulate; aux variables are computed only
 // when requested
 void particle::update_force(bool need_aux) {
     if (force_valid && (!need_aux || aux_valid)) return;
-    settle_velocity();   // a pending kick refers to the force about to be replaced
+    // a pending kick refers to the plain force of the current positions:
+    // apply it before that force is replaced by one with aux (settle_velocity
+    // evaluates it if needed), so that sampling does not change the trajectory
+    settle_velocity();
+    if (!force_valid && force_owed) evaluate_owed_force();
+    if (force_valid && (!need_aux || aux_valid)) return;
     aux_enabled = aux_enabled || need_aux;
-    if (forces.empty()) {
-        HIP_CHECK(hipMemsetAsync(force, 0, force.bytes(), stream));
-        if (aux_enabled) HIP_CHECK(hipMemsetAsync(stress, 0, stress.bytes(), stream));
-    } else {
-        // every neighbour list first (a rebuild may reorder the particles),
-        // then the forces; a single force module may launch before its
-        // neighbour check is back
-        if (!speculative_force || forces.size() != 1 || !forces[0]->apply_speculative()) {
-            for (force_module* f : forces) f->prepare();
-            bool first = true;
-            for (force_module* f : forces) {
-                f->apply(first);
-                first = false;
-            }
-        }
-    }
+    evaluate_forces();
     force_valid = true;
+    force_owed = false;
+    owed_aux = false;
     aux_valid = aux_enabled;
     aux_enabled = false;
 }
 
+void particle::evaluate_owed_force() {
+    bool const later = aux_enabled;   // a request made after the step, for a later force
+    aux_enabled = force_owed && owed_aux;
+    evaluate_forces();
+    force_valid = true;
+    aux_valid = aux_enabled;
+    force_owed = false;
+    owed_aux = false;
+    aux_enabled = later;
+}
+
+void particle::evaluate_forces() {
+    if (forces.empty()) {
+        HIP_CHECK(hipMemsetAsync(force, 0, force.bytes(), stream));
+        if (aux_enabled) HIP_CHECK(hipMemsetAsync(stress, 0, stress.bytes(), stream));
+        return;
+    }
+    // every neighbour list first (a rebuild may reorder the particles), then
+    // the forces; a single force module may launch before its neighbour check
+    // is back
+    if (!speculative_force || forces.size() != 1 || !forces[0]->apply_speculative()) {
+        for (force_module* f : forces) f->prepare();
+        bool first = true;
+        for (force_module* f : forces) {
+            f->apply(first);
+            first = false;
+        }
+    }
+}
+
 void particle::rearrange(uint32_t const* perm) {
-    settle_velocity();
+    // a kick owed with a force not yet evaluated stays owed (the force will be
+    // evaluated in the new order); one whose force is in memory is applied,
+    // since the force array is not permuted
+    if (force_valid) settle_velocity();
     launch_rearrange(pos, vel, image, id, posq, perm, n, pos_alt, vel_alt, image_alt, id_alt, posq_alt, stream);
     std::swap(posq, posq_alt);
     std::swap(pos, pos_alt);
@@ -613,7 +658,7 @@ void binning::cell_lists(unsigned* begin_out, unsigned* end_out, unsigned* index
 max_displacement::max_displacement(particle& p_, box const& b_) : p(p_), b(b_) {
     p.attach_box(b);
     r0.resize(p.n);
-    d_max2.resize(2 * DISP_SET);
+    d_max2.resize(3 * DISP_SET);
     h_max2.resize(DISP_SLOTS);
     HIP_CHECK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
     HIP_CHECK(hipEventCreateWithFlags(&ready_, hipEventDisableTiming));
@@ -630,32 +675,34 @@ max_displacement::~max_displacement() {
 
 void max_displacement::zero() {
     launch_copy_hi(p.pos, r0, p.n, p.stream);
-    launch_fill_u32(d_max2, 0u, 2 * DISP_SET, p.stream);
-    slot = 0;
+    launch_fill_u32(d_max2, 0u, 3 * DISP_SET, p.stream);
+    updates = 0;
     tracked_version = p.position_version;
     computed_version = p.position_version;
     value = 0;
 }
 
-uint32_t* max_displacement::fuse_slot() {
-    // the integrate kernel folds the new positions into the current slot and
-    // clears the other one, which the next step will use
+void max_displacement::begin_update(uint32_t*& write, uint32_t*& clear) {
+    write = d_max2.get() + updates % 3 * DISP_SET;
+    clear = d_max2.get() + (updates + 1) % 3 * DISP_SET;
+    ++updates;
     tracked_version = p.position_version + 1;
-    return d_max2.get() + slot * DISP_SET;
 }
-
-uint32_t* max_displacement::next_slot() { return d_max2.get() + (slot ^ 1u) * DISP_SET; }
 
 double max_displacement::compute() {
     if (computed_version == p.position_version) return value;
     hipStream_t s = p.stream;
-    uint32_t* const cur = d_max2.get() + slot * DISP_SET;
     if (tracked_version != p.position_version) {
-        launch_fill_u32(cur, 0u, DISP_SET, s);
-        launch_max_displacement(p.pos, r0, p.n, b.Lf, b.invL, cur, s);
+        // positions changed without an update that tracks them: a separate pass
+        uint32_t *write, *clear;
+        begin_update(write, clear);
+        tracked_version = p.position_version;
+        launch_fill_u32(write, 0u, DISP_SET, s);
+        launch_fill_u32(clear, 0u, DISP_SET, s);
+        launch_max_displacement(p.pos, r0, p.n, b.Lf, b.invL, write, s);
     }
-    HIP_CHECK(hipMemcpy2DAsync(h_max2.get(), sizeof(uint32_t), cur, DISP_STRIDE * sizeof(uint32_t), sizeof(uint32_t),
-                               DISP_SLOTS, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpy2DAsync(h_max2.get(), sizeof(uint32_t), current_slots(), DISP_STRIDE * sizeof(uint32_t),
+                               sizeof(uint32_t), DISP_SLOTS, hipMemcpyDeviceToHost, s));
     p.sync();
     return finish(h_max2.get());
 }
@@ -667,8 +714,6 @@ double max_displacement::finish(uint32_t const* words) {
     std::memcpy(&m2, &bits, 4);
     value = std::sqrt(double(m2));
     computed_version = p.position_version;
-    slot ^= 1u;   // the next fused step writes the slot cleared by this one
-    tracked_version = 0;
     return value;
 }
 
@@ -991,18 +1036,58 @@ void pair_trunc::apply(bool force_zero) {
     launch(force_zero, nullptr, 0);
 }
 
-bool pair_trunc::launch(bool force_zero, uint32_t const* abort_slots, uint32_t abort_bits) {
-    bool const aux = p.aux_enabled;
+bool pair_trunc::launch(bool force_zero, uint32_t const* abort_slots, uint32_t abort_bits,
+                        fused_integrate const* fuse) {
+    bool const aux = p.aux_enabled && !fuse;   // the fused step evaluates the plain force
     // a particle interacting with itself counts each pair twice (pair_trunc.hpp:291-294)
     float const w = float(aux_weight / 2);
     timer.start(p.stream);
     if (nb.staged)
         launch_pair_force_staged(p.posq, nb.bin->offset, nb.sp_force, nb.list16, nb.count, nb.cap, params, aux,
-                                 force_zero, w, p.force, p.stress, p.n_pad, p.stream, abort_slots, abort_bits);
+                                 force_zero, w, p.force, p.stress, p.n_pad, p.stream, abort_slots, abort_bits, fuse);
     else
         launch_pair_force(p.posq, p.n, nb.list, nb.count, nb.cap, params, aux, force_zero, w, p.force, p.stress,
                           p.n_pad, p.stream);
     return timer.stop(p.stream);
+}
+
+bool pair_trunc::integrate_fused(verlet& v) {
+    if (!nb.staged || v.fused_disp != &nb.disp) return false;
+    max_displacement& d = nb.disp;
+    bool const spec = p.speculative_force && nb.current_but_displacement() && d.speculation_ready();
+    if (!spec) nb.require_current();   // lists for the current positions (may rebuild and reorder)
+    float const pre = p.pending_half_dt;
+    auto args = [&]() {
+        // (after a rebuild the particle arrays are other buffers)
+        fused_integrate F{};
+        F.pos = p.pos;
+        F.vel = p.vel;
+        F.image = p.image;
+        F.posq_out = p.posq_next;
+        F.r0 = d.r0;
+        d.begin_update(F.disp_write, F.disp_clear);
+        F.ip = v.ip;
+        F.pre_half_dt = pre;
+        return F;
+    };
+    uint32_t const* const cur = d.current_slots();   // before begin_update moves it
+    if (spec) d.begin_readback();
+    fused_integrate F = args();
+    bool const timed = launch(true, spec ? cur : nullptr, nb.keep_bits, &F);   // timed as the force
+    if (spec && d.finish_readback() > nb.r_skin / 2) {
+        // the launch found its lists stale and did nothing
+        if (timed) timer.drop_last();
+        nb.require_current();
+        F = args();
+        launch(true, nullptr, 0, &F);
+    }
+    std::swap(p.posq, p.posq_next);
+    p.pending_half_dt = 0;
+    p.force_owed = false;
+    p.owed_aux = false;
+    p.positions_changed();
+    p.velocities_changed();
+    return true;
 }
 
 bool pair_trunc::apply_speculative() {
@@ -1037,16 +1122,26 @@ void verlet::set_timestep(double dt_) {
 }
 
 void verlet::integrate_kernel(double const* scale) {
-    p.update_force();
+    // plain NVE with the force of the current positions not yet evaluated and
+    // a single pair force: the force kernel integrates each particle as it
+    // finishes it (no separate integrate pass, no force array round trip)
+    // (the owed force of the previous step is the plain one even when en_pot or
+    // the stress was requested for the next evaluation, as the step that
+    // evaluates its force at once computes it before any such request)
+    if (!scale && p.fused_step && fused_disp && !p.force_valid && p.forces.size() == 1 &&
+        (p.force_owed ? !p.owed_aux : !p.aux_enabled))
+        if (p.forces[0]->integrate_fused(*this)) return;
+    if (p.force_owed && !p.force_valid)
+        p.evaluate_owed_force();
+    else
+        p.update_force();
     float4 const* r0 = nullptr;
     uint32_t* slot = nullptr;
     uint32_t* clear = nullptr;
-    if (fused_disp && fused_disp->computed_version == p.position_version) {
-        // positions are the ones the displacement was last evaluated for:
-        // track the next step inside the integrate kernel
+    if (fused_disp) {
+        // track the displacement of the new positions inside the integrate kernel
         r0 = fused_disp->r0;
-        slot = fused_disp->fuse_slot();
-        clear = fused_disp->next_slot();
+        fused_disp->begin_update(slot, clear);
     }
     float const pre = p.take_pending_kick();
     timer_integrate.start(p.stream);
@@ -1059,10 +1154,18 @@ void verlet::integrate_kernel(double const* scale) {
 void verlet::integrate() { integrate_kernel(nullptr); }
 
 void verlet::finalize() {
-    p.update_force();
     if (defer_finalize) {
+        // owed with the force of the current positions, evaluated when first
+        // needed (by the next step's force kernel, or a reader)
         p.pending_half_dt = ip.half_dt;
+        if (!p.force_valid) {
+            // en_pot and stress requested before this step are for this force
+            p.force_owed = true;
+            p.owed_aux = p.aux_enabled;
+            p.aux_enabled = false;
+        }
     } else {
+        p.update_force();
         timer_finalize.start(p.stream);
         launch_verlet_finalize(p.vel, p.force, p.n, ip.half_dt, p.stream);
         timer_finalize.stop(p.stream);
@@ -1366,6 +1469,7 @@ density_mode::density_mode(particle& p_, wavevector const& wv_) : p(p_), wv(wv_)
 }
 
 std::shared_ptr<std::vector<std::complex<double>>> density_mode::acquire() {
+    p.observe();   // the particle order of a step that evaluates its force at once (summation order)
     if (last && version == p.position_version) return last;
     unsigned const nq = unsigned(wv.q.size() / 3);
     timer.start(p.stream);

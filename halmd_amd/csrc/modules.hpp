@@ -65,6 +65,7 @@ public:
 };
 
 class force_module;
+class verlet;
 
 class particle {
 public:
@@ -76,12 +77,16 @@ public:
     // a single force module may launch before its neighbour check is back
     // (results are the same either way)
     bool speculative_force = true;
+    // plain NVE Verlet with a single pair force: the force kernel integrates
+    // each particle as it finishes it (results are the same either way)
+    bool fused_step = true;
     bool has_box = false;
     double box_length[3] = {0, 0, 0};
     hipStream_t stream;
 
     dbuf<float4> pos, vel, force;     // pos/vel: 2 float4 per particle (hi, lo)
     dbuf<uint4> posq, posq_alt;       // fixed-point snapshot of pos (written with every position update)
+    dbuf<uint4> posq_next;            // the snapshot a fused force-and-integrate launch writes
     dbuf<float> stress;               // 6 x n_pad, column-major
     dbuf<int4> image;
     dbuf<uint32_t> id, reverse_id;
@@ -115,10 +120,11 @@ public:
 
     void positions_changed() { ++position_version; force_valid = false; aux_valid = false; }
     // Deferred half kick of the plain Verlet finalize, v += F dt / 2m with the
-    // force of the current positions.  The next integrate kernel applies it
-    // together with its own half kick (the same arithmetic, one pass fewer);
-    // any reader of the velocities, or of the force it refers to, settles it
-    // first.  0 = none pending.
+    // force of the current positions (evaluated when first needed, so that the
+    // next step's force kernel can integrate as it goes).  The next integrate
+    // applies it together with its own half kick (the same arithmetic, one
+    // pass fewer); any reader of the velocities, or of the force it refers to,
+    // settles it first.  0 = none pending.
     float pending_half_dt = 0;
     void settle_velocity();
     float take_pending_kick() { float const h = pending_half_dt; pending_half_dt = 0; return h; }
@@ -133,6 +139,18 @@ public:
     // make the force (and en_pot / stress if need_aux or aux_enable()d) valid
     // for the current positions (particle.cpp:221-253)
     void update_force(bool need_aux = false);
+    // evaluate every force module at the current positions (lists first)
+    void evaluate_forces();
+    // the owed force (with en_pot and stress if owed_aux), marked valid; an
+    // aux request made after the step stays for a later evaluation
+    void evaluate_owed_force();
+    // The plain Verlet step leaves the force of its new positions to be
+    // evaluated when first needed (force_owed); the state getters and order-
+    // dependent observables first evaluate it, so that they see what a step
+    // that evaluates its force at once leaves (the particle order after any
+    // list rebuild included).
+    bool force_owed = false, owed_aux = false;
+    void observe() { if (force_owed) update_force(); }
     // reorder by a device permutation perm[new] = old
     void rearrange(uint32_t const* perm);
     void sync();
@@ -202,24 +220,29 @@ public:
     particle& p;
     box const& b;
     dbuf<float4> r0;
-    dbuf<uint32_t> d_max2;            // two slot sets (DISP_SET words): current step and next step
+    // three slot sets (DISP_SET words each) in rotation: position update u
+    // writes the maxima of the new positions into set u % 3 (zeroed by update
+    // u - 1) and zeroes set (u + 1) % 3, while set (u - 1) % 3, that of the
+    // positions it starts from, may still be read (a speculative force
+    // launch's check, the copy to the host)
+    dbuf<uint32_t> d_max2;
     hbuf<uint32_t> h_max2;
-    unsigned slot = 0;
-    uint64_t tracked_version = 0;     // position version whose max is in slot (fused in integrate)
+    unsigned updates = 0;
+    uint64_t tracked_version = 0;     // position version whose maxima the last update wrote
     uint64_t computed_version = 0;
     double value = 0;
     void zero();
     double compute();
-    // pointers for the integrate kernel; marks the resulting version as tracked
-    uint32_t* fuse_slot();
-    uint32_t* next_slot();
+    // the set the next position update writes and the set it zeroes; marks
+    // the resulting position version as tracked
+    void begin_update(uint32_t*& write, uint32_t*& clear);
     // Asynchronous form of compute() for positions whose maxima the integrate
     // kernel wrote (speculation_ready): begin_readback copies the words on a
     // side stream once the particle stream has reached this point, so the
     // particle stream goes on at once; finish_readback waits for the copy and
     // keeps the same state as compute().
     bool speculation_ready() const;
-    uint32_t const* current_slots() const { return d_max2.get() + slot * DISP_SET; }
+    uint32_t const* current_slots() const { return d_max2.get() + (updates + 2) % 3 * DISP_SET; }
     void begin_readback();
     double finish_readback();
 private:
@@ -287,6 +310,9 @@ public:
     // as prepare() + apply(true) for the only force module of a particle,
     // without waiting for the neighbour check first; false: not possible now
     virtual bool apply_speculative() { return false; }
+    // the force of the current positions and the Verlet step with it in one
+    // launch (plain NVE, the only force module); false: not possible now
+    virtual bool integrate_fused(verlet&) { return false; }
 };
 
 class pair_trunc : public force_module {
@@ -306,13 +332,15 @@ public:
     // the lists are stale), then rebuild and launch again if they were; false
     // when the lists or displacement do not allow it (nothing done)
     bool apply_speculative() override;
+    bool integrate_fused(verlet& v) override;
     // pick the cell-segment width and workgroup size of the neighbour lists
     // by timing list builds and force evaluations around the current choice;
     // returns the width kept (0 if the lists are not cell-segment lists)
     unsigned tune();
     kernel_timer timer;
 private:
-    bool launch(bool force_zero, uint32_t const* abort_slots, uint32_t abort_bits);
+    bool launch(bool force_zero, uint32_t const* abort_slots, uint32_t abort_bits,
+                fused_integrate const* fuse = nullptr);
 };
 
 class integrator {

@@ -128,6 +128,12 @@ class particle(Handle):
         results are the same either way)."""
         call("halmd_mdsim_particle_set_speculative_force", self.handle, int(bool(enable)))
 
+    def set_fused_step(self, enable=True):
+        """Plain NVE Verlet with one pair force: integrate each particle in the
+        force kernel as soon as its force is summed (default on; results are
+        the same either way)."""
+        call("halmd_mdsim_particle_set_fused_step", self.handle, int(bool(enable)))
+
     def rearrange(self, index):
         idx = np.ascontiguousarray(index, dtype=np.uint32)
         call("halmd_mdsim_particle_rearrange", self.handle, uptr(idx))

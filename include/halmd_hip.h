@@ -128,6 +128,14 @@ HALMD_API int halmd_mdsim_particle_rearrange(halmd_particle* p, const unsigned* 
  * same either way; on by default, enable = 0 waits for the check. */
 HALMD_API int halmd_mdsim_particle_set_speculative_force(halmd_particle* p, int enable);
 
+/* No counterpart in the reference (separate force and integrate kernels,
+ * verlet.cpp:64-121): plain NVE Verlet with one pair force evaluates the force
+ * of the current positions and performs the Verlet step of each particle in
+ * one kernel, as soon as the particle's force is summed (the force is then
+ * evaluated lazily: at the next step, or when read).  Results are the same
+ * either way; on by default, enable = 0 uses separate kernels. */
+HALMD_API int halmd_mdsim_particle_set_fused_step(halmd_particle* p, int enable);
+
 /* ---- mdsim.potentials.pair.*: lennard_jones.cpp:45-71, morse.cpp:41-61;
  *      truncations: lua/halmd/mdsim/potentials/pair/adapters/truncate.lua.in -- */
 HALMD_API int halmd_mdsim_potentials_pair_lennard_jones_create(unsigned ntype1, unsigned ntype2,

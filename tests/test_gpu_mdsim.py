@@ -440,6 +440,28 @@ def test_speculative_force_is_bitwise_identical():
     assert runs[0][2] == runs[1][2]
 
 
+@pytest.mark.parametrize("kind", ["lj", "ka"])
+def test_fused_step_is_bitwise_identical(kind):
+    # the force kernel that integrates each particle as it finishes it gives
+    # the trajectory of separate force and integrate kernels bit for bit,
+    # across rebuilds, Hilbert sorts and samples taken in between
+    runs = []
+    for fused in (True, False):
+        if kind == "lj":
+            w = workloads.lennard_jones(6912, density=0.8442, temperature=1.5, seed=82, sort_interval=40)
+        else:
+            w = workloads.kob_andersen(8788, density=1.2, temperature=0.9, seed=83, sort_interval=40)
+        w.particle.set_fused_step(fused)
+        s = w.sampler.on_sample_thermodynamics(w.thermodynamics, 25)
+        w.sampler.run(260)
+        runs.append((w.particle.ordered("position"), w.particle.ordered("velocity"), w.particle.ordered("image"),
+                     w.particle.ordered("force"), w.sampler.thermodynamics_records(s), w.neighbour.rebuilds))
+    a, b = runs
+    for k in range(5):
+        assert np.array_equal(a[k], b[k]), k
+    assert a[5] == b[5] > 5
+
+
 def test_deferred_finalize_is_bitwise_identical():
     # the finalize kick folded into the next integrate pass computes exactly
     # what the separate pass computes, also across rebuilds, sorts and samples
