@@ -10,9 +10,8 @@
 // Precision: the separation is the exact integer difference of the fixed-point
 // coordinates (minimum image by 32-bit wrap-around), so r_ij carries ~1e-7 of
 // its own magnitude instead of ~1 ulp of the box coordinate that the
-// reference's high-word difference has.  For the Morse potential, whose
-// exponent argument amplifies the error of r by r_min / sigma, r is formed as a
-// double word from the exact integers.  Pair potentials are FP32; the
+// reference's high-word difference has.  Pair potentials are FP32 (the Morse
+// exponent argument too, as in the reference, with r = sqrt(r^2)); the
 // per-particle force sum is compensated: plain FP32 over each group of 8 pair
 // forces, Kahan summation over the groups (the counterpart of the reference's
 // dsfloat accumulator).  Energy and stress (aux) are FP32 per particle
@@ -24,18 +23,11 @@
 namespace halmd {
 namespace {
 
-// Morse exponent argument r_min/(sigma B) - r/(sigma B) from the exact integer
-// separation (double-word arithmetic throughout; no FP64)
-__device__ __forceinline__ float morse_argument(int32_t Dx, int32_t Dy, int32_t Dz, force_params const& P,
-                                                pair_param const& pp) {
-    ff const x = ff_mul(ff_from_int(Dx), ff{P.q[0], P.q_lo[0]});
-    ff const y = ff_mul(ff_from_int(Dy), ff{P.q[1], P.q_lo[1]});
-    ff const z = ff_mul(ff_from_int(Dz), ff{P.q[2], P.q_lo[2]});
-    ff const rr = ff_add(ff_add(ff_mul(x, x), ff_mul(y, y)), ff_mul(z, z));
-    ff const r = ff_sqrt(rr);
-    ff const t = ff_mul(r, ff{pp.p[1], pp.p_lo[1]});
-    ff const a = ff_add(ff{pp.p[2], pp.p_lo[2]}, ff_neg(t));
-    return a.hi + a.lo;
+// Morse exponent argument r_min/(sigma B) - r/(sigma B), in FP32 as the
+// reference computes it (morse_kernel.hpp:46-66), from the squared distance:
+// one square root and one fused multiply-add
+__device__ __forceinline__ float morse_argument(float rr, pair_param const& pp) {
+    return __builtin_fmaf(-sqrtf(rr), pp.p[1], pp.p[2]);
 }
 
 // 1/x: hardware reciprocal (~1 ulp, not symmetric about the exact value) plus
@@ -179,7 +171,7 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
         float const rr = dx * dx + dy * dy + dz * dz;
         pair_param const& pp = s_pair[ti + (qj.w & 3u)];
         if (!(rr < pp.rr_cut)) return;
-        float const md = KIND == POTENTIAL_MORSE ? morse_argument(Dx, Dy, Dz, P, pp) : 0.0f;
+        float const md = KIND == POTENTIAL_MORSE ? morse_argument(rr, pp) : 0.0f;
         float fval, u;
         pair_eval<KIND, TRUNC>(rr, md, pp, P.rri_smooth, fval, u);
         bx += fval * dx;
@@ -353,7 +345,7 @@ __device__ __forceinline__ void staged_force_segment(
                     rr = dx * dx + dy * dy + dz * dz;
                     inside = rr < pp.rr_cut;
                 }
-                float const md = KIND == POTENTIAL_MORSE ? morse_argument(Dx, Dy, Dz, P, pp) : 0.0f;
+                float const md = KIND == POTENTIAL_MORSE ? morse_argument(rr, pp) : 0.0f;
                 pair_eval<KIND, TRUNC>(rr, md, pp, P.rri_smooth, fval, u);
             }
             fval = inside ? fval : 0.0f;
@@ -563,9 +555,7 @@ __global__ __launch_bounds__(256) void ens_force_small_kernel(
             float const rq = dx * dx + dy * dy + dz * dz;
             bool const inside = rq < pp.rr_cut_q;
             float const rr = __builtin_fmaf(rq, P.q2[0], rq * P.q2[1]);
-            float const md = KIND == POTENTIAL_MORSE
-                ? morse_argument(int32_t(qi.x - qj.x), int32_t(qi.y - qj.y), int32_t(qi.z - qj.z), P, pp)
-                : 0.0f;
+            float const md = KIND == POTENTIAL_MORSE ? morse_argument(rr, pp) : 0.0f;
             float fval, u;
             pair_eval<KIND, TRUNC>(rr, md, pp, P.rri_smooth, fval, u);
             fval = inside ? fval : 0.0f;

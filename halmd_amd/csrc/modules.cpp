@@ -577,8 +577,8 @@ force_params potential::device_params(box const& b) const {
                 double const B = distortion[ab];
                 double const A = 2 * B * B - 1;
                 pp.p[0] = float(eps / A);
-                split_ff(1 / (sig * B), pp.p[1], pp.p_lo[1]);
-                split_ff(r_min[ab] / sig / B, pp.p[2], pp.p_lo[2]);
+                pp.p[1] = float(1 / (sig * B));
+                pp.p[2] = float(r_min[ab] / sig / B);
                 pp.p[3] = float(A);
             }
             pp.rr_cut = float(rr_cut[ab]);
