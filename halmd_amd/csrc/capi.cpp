@@ -453,6 +453,29 @@ HALMD_API int halmd_mdsim_forces_pair_trunc_apply(halmd_force* hf) {
     });
 }
 
+HALMD_API int halmd_mdsim_forces_pair_full_create(const halmd_potential* pot, halmd_particle* p1,
+                                                  halmd_particle* p2, const halmd_box* b, double w,
+                                                  halmd_force** out) {
+    return guard([&] {
+        nonnull(out, "out");
+        *out = handle<halmd_force>(new pair_full(*as<potential>(pot), *as<particle>(p1), *as<particle>(p2),
+                                                 *as<box>(b), w));
+    });
+}
+HALMD_API int halmd_mdsim_forces_pair_full_destroy(halmd_force* f) {
+    return guard([&] { delete reinterpret_cast<pair_full*>(f); });
+}
+HALMD_API int halmd_mdsim_forces_pair_full_apply(halmd_force* hf) {
+    return guard([&] {
+        auto* f = as<pair_full>(hf);
+        f->apply(true);
+        f->p1.force_valid = true;
+        f->p1.aux_valid = f->p1.aux_enabled;
+        f->p1.aux_enabled = false;
+        f->p1.sync();
+    });
+}
+
 // ---- integrators -------------------------------------------------------------------------
 HALMD_API int halmd_mdsim_integrators_verlet_create(halmd_particle* p, const halmd_box* b, double dt,
                                                     halmd_integrator** out) {

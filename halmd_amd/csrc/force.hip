@@ -20,6 +20,8 @@
 #include "integrate_device.hpp"
 #include "segment.hpp"
 
+#include <algorithm>
+
 namespace halmd {
 namespace {
 
@@ -681,6 +683,167 @@ void staged_trunc(staged_launch const& a) {
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Untruncated forces over all pairs (pair_full_kernel::compute,
+// halmd/mdsim/gpu/forces/pair_full_kernel.cuh:49-130): the force on particle i
+// of instance 1 from every particle j of instance 2 (j != i when both are the
+// same instance), minimum image by the 32-bit wrap-around of the fixed-point
+// coordinates.
+//
+// Grid: x over blocks of FULL_TILE particles i (one per lane), y over `parts`
+// contiguous ranges of j, each a multiple of FULL_TILE long, so that small
+// systems still occupy the whole chip.  Each workgroup stages its j range in
+// tiles of FULL_TILE records through LDS and every lane sums over the tile:
+// FP32 over groups of 8 pairs, Kahan over the groups.  Part y writes its sums
+// (force x, y, z, en_pot, 6 stress components) to partial[(y * 10 + c) * n1 + i];
+// pair_full_reduce_kernel adds the parts in order (Kahan), scales the aux
+// sums by aux_weight and adds the stored values unless force_zero.  Both the
+// split and the order depend only on (n1, n2).
+constexpr unsigned FULL_TILE = 256;
+constexpr unsigned FULL_COMPONENTS = PAIR_FULL_COMPONENTS;
+
+template <int KIND, bool AUX>
+__global__ __launch_bounds__(FULL_TILE) void pair_full_kernel(
+    uint4 const* __restrict__ posq1, unsigned n1, uint4 const* __restrict__ posq2, unsigned n2, unsigned same,
+    unsigned chunk, force_params P, float* __restrict__ partial)
+{
+    __shared__ pair_param s_pair[MAX_TYPES * MAX_TYPES];
+    __shared__ uint4 s_q[FULL_TILE];
+    if (threadIdx.x < MAX_TYPES * MAX_TYPES) s_pair[threadIdx.x] = P.pair[threadIdx.x];
+
+    unsigned const i = blockIdx.x * FULL_TILE + threadIdx.x;
+    bool const active = i < n1;
+    uint4 const qi = posq1[active ? i : 0];
+    unsigned const ti = (qi.w & 3u) * P.ntype;
+    unsigned const j0 = blockIdx.y * chunk, j1 = min(n2, j0 + chunk);
+
+    float fx = 0, fy = 0, fz = 0, cx = 0, cy = 0, cz = 0;
+    float bx = 0, by = 0, bz = 0;
+    float en = 0, ce = 0, be = 0;
+    float st[6] = {0, 0, 0, 0, 0, 0}, cs[6] = {0, 0, 0, 0, 0, 0}, bs[6] = {0, 0, 0, 0, 0, 0};
+
+    auto pair = [&](uint4 const qj) {
+        int32_t const Dx = int32_t(qi.x - qj.x), Dy = int32_t(qi.y - qj.y), Dz = int32_t(qi.z - qj.z);
+        float const dx = separation(Dx, P.q[0], P.q_lo[0]);
+        float const dy = separation(Dy, P.q[1], P.q_lo[1]);
+        float const dz = separation(Dz, P.q[2], P.q_lo[2]);
+        float const rr = dx * dx + dy * dy + dz * dz;
+        pair_param const& pp = s_pair[ti + (qj.w & 3u)];
+        float const md = KIND == POTENTIAL_MORSE ? morse_argument(rr, pp) : 0.0f;
+        float fval, u;
+        pair_eval<KIND, TRUNC_SHARP>(rr, md, pp, 0.0f, fval, u);
+        bx += fval * dx;
+        by += fval * dy;
+        bz += fval * dz;
+        if (AUX) {
+            be += u;
+            bs[0] += fval * dx * dx;
+            bs[1] += fval * dy * dy;
+            bs[2] += fval * dz * dz;
+            bs[3] += fval * dx * dy;
+            bs[4] += fval * dx * dz;
+            bs[5] += fval * dy * dz;
+        }
+    };
+    auto fold = [&]() {
+        kahan_add(fx, cx, bx);
+        kahan_add(fy, cy, by);
+        kahan_add(fz, cz, bz);
+        bx = by = bz = 0;
+        if (AUX) {
+            kahan_add(en, ce, be);
+            be = 0;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) { kahan_add(st[c], cs[c], bs[c]); bs[c] = 0; }
+        }
+    };
+
+    for (unsigned jt = j0; jt < j1; jt += FULL_TILE) {
+        unsigned const m = min(FULL_TILE, j1 - jt);
+        __syncthreads();   // the previous tile is consumed (and s_pair written, first time)
+        if (threadIdx.x < m) s_q[threadIdx.x] = posq2[jt + threadIdx.x];
+        __syncthreads();
+        if (!active) continue;
+        // the tile holding j = i (same instance) takes the per-pair test
+        bool const self = same && i >= jt && i < jt + m;
+        unsigned k = 0;
+        for (; k + 8 <= m; k += 8) {
+            if (self) {
+#pragma unroll
+                for (unsigned t = 0; t < 8; ++t)
+                    if (jt + k + t != i) pair(s_q[k + t]);
+            } else {
+#pragma unroll
+                for (unsigned t = 0; t < 8; ++t) pair(s_q[k + t]);
+            }
+            fold();
+        }
+        if (k < m) {
+            for (; k < m; ++k)
+                if (!self || jt + k != i) pair(s_q[k]);
+            fold();
+        }
+    }
+    if (!active) return;
+    size_t const base = size_t(blockIdx.y) * FULL_COMPONENTS * n1 + i;
+    partial[base] = fx - cx;
+    partial[base + n1] = fy - cy;
+    partial[base + 2 * size_t(n1)] = fz - cz;
+    if (AUX) {
+        partial[base + 3 * size_t(n1)] = en - ce;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) partial[base + (4 + c) * size_t(n1)] = st[c] - cs[c];
+    }
+}
+
+template <bool AUX>
+__global__ __launch_bounds__(256) void pair_full_reduce_kernel(
+    float const* __restrict__ partial, unsigned n1, unsigned parts, bool force_zero, float aux_weight,
+    float4* __restrict__ force, float* __restrict__ stress, unsigned stride)
+{
+    unsigned const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n1) return;
+    constexpr unsigned NC = AUX ? FULL_COMPONENTS : 3;
+    float s[NC], c[NC];
+#pragma unroll
+    for (unsigned k = 0; k < NC; ++k) s[k] = c[k] = 0;
+    for (unsigned y = 0; y < parts; ++y) {
+        size_t const base = size_t(y) * FULL_COMPONENTS * n1 + i;
+#pragma unroll
+        for (unsigned k = 0; k < NC; ++k) kahan_add(s[k], c[k], partial[base + k * size_t(n1)]);
+    }
+    float4 f = make_float4(s[0] - c[0], s[1] - c[1], s[2] - c[2], 0.0f);
+    float sv[6] = {0, 0, 0, 0, 0, 0};
+    if (AUX) {
+        f.w = aux_weight * (s[3] - c[3]);
+#pragma unroll
+        for (unsigned k = 0; k < 6; ++k) sv[k] = aux_weight * (s[4 + k] - c[4 + k]);
+    }
+    if (!force_zero) {
+        float4 const old = force[i];
+        f.x += old.x; f.y += old.y; f.z += old.z; f.w += old.w;
+        if (AUX) {
+#pragma unroll
+            for (unsigned k = 0; k < 6; ++k) sv[k] += stress[k * size_t(stride) + i];
+        }
+    }
+    force[i] = f;
+    if (AUX) {
+#pragma unroll
+        for (unsigned k = 0; k < 6; ++k) stress[k * size_t(stride) + i] = sv[k];
+    }
+}
+
+template <int KIND>
+void pair_full_go(uint4 const* posq1, unsigned n1, uint4 const* posq2, unsigned n2, bool same, unsigned parts,
+                  unsigned chunk, force_params const& p, bool aux, float* partial, hipStream_t s) {
+    dim3 const grid((n1 + FULL_TILE - 1) / FULL_TILE, parts);
+    if (aux)
+        pair_full_kernel<KIND, true><<<grid, FULL_TILE, 0, s>>>(posq1, n1, posq2, n2, same, chunk, p, partial);
+    else
+        pair_full_kernel<KIND, false><<<grid, FULL_TILE, 0, s>>>(posq1, n1, posq2, n2, same, chunk, p, partial);
+}
 } // namespace
 
 void launch_pair_force(uint4 const* posq, unsigned n, uint32_t const* list, uint32_t const* count, unsigned cap,
@@ -696,6 +859,50 @@ void launch_pair_force(uint4 const* posq, unsigned n, uint32_t const* list, uint
         dispatch_trunc<POTENTIAL_POWER_LAW>(a);
     else
         throw error(HALMD_ERR_INVALID_ARGUMENT, "unknown potential");
+    HIP_CHECK(hipGetLastError());
+}
+
+namespace {
+// the j ranges of launch_pair_full: chunk (a multiple of FULL_TILE) and their
+// number; about 512 workgroups (2048 waves) for small n1
+void pair_full_split(unsigned n1, unsigned n2, unsigned& chunk, unsigned& parts) {
+    unsigned const blocks = std::max(1u, (n1 + FULL_TILE - 1) / FULL_TILE);
+    unsigned const tiles = (n2 + FULL_TILE - 1) / FULL_TILE;
+    unsigned const want = std::max(1u, std::min({(512 + blocks - 1) / blocks, tiles, PAIR_FULL_MAX_PARTS}));
+    chunk = ((tiles + want - 1) / want) * FULL_TILE;
+    parts = n2 ? (n2 + chunk - 1) / chunk : 0;
+}
+} // namespace
+
+size_t pair_full_partial_floats(unsigned n1, unsigned n2) {
+    unsigned chunk, parts;
+    pair_full_split(n1, n2, chunk, parts);
+    return size_t(parts) * PAIR_FULL_COMPONENTS * n1;
+}
+
+void launch_pair_full(uint4 const* posq1, unsigned n1, uint4 const* posq2, unsigned n2, bool same,
+                      force_params const& p, bool aux, bool force_zero, float aux_weight, float4* force,
+                      float* stress, unsigned stride, float* partial, hipStream_t s) {
+    if (!n1) return;
+    if (p.trunc != TRUNC_SHARP) throw error(HALMD_ERR_INVALID_ARGUMENT, "pair_full: untruncated parameters expected");
+    unsigned chunk, np;
+    pair_full_split(n1, n2, chunk, np);
+    if (np) {
+        if (p.kind == POTENTIAL_LENNARD_JONES)
+            pair_full_go<POTENTIAL_LENNARD_JONES>(posq1, n1, posq2, n2, same, np, chunk, p, aux, partial, s);
+        else if (p.kind == POTENTIAL_MORSE)
+            pair_full_go<POTENTIAL_MORSE>(posq1, n1, posq2, n2, same, np, chunk, p, aux, partial, s);
+        else if (p.kind == POTENTIAL_POWER_LAW)
+            pair_full_go<POTENTIAL_POWER_LAW>(posq1, n1, posq2, n2, same, np, chunk, p, aux, partial, s);
+        else
+            throw error(HALMD_ERR_INVALID_ARGUMENT, "unknown potential");
+    }
+    // with no partner particles the sums are zero: reduce over no parts
+    unsigned const blocks = (n1 + 255) / 256;
+    if (aux)
+        pair_full_reduce_kernel<true><<<blocks, 256, 0, s>>>(partial, n1, np, force_zero, aux_weight, force, stress, stride);
+    else
+        pair_full_reduce_kernel<false><<<blocks, 256, 0, s>>>(partial, n1, np, force_zero, aux_weight, force, stress, stride);
     HIP_CHECK(hipGetLastError());
 }
 

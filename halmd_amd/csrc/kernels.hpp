@@ -230,6 +230,17 @@ void launch_pair_force(uint4 const* posq, unsigned n, uint32_t const* list, uint
                        force_params const& p, bool aux, bool force_zero, float aux_weight, float4* force,
                        float* stress, unsigned stride, hipStream_t s);
 
+// untruncated forces over all pairs of two particle instances (same: one
+// instance, j != i); p as from potential::device_params for pair_full (sharp,
+// infinite cutoff).  partial: pair_full_partial_floats(n1, n2) floats of
+// scratch, at most PAIR_FULL_MAX_PARTS * PAIR_FULL_COMPONENTS * n1.
+constexpr unsigned PAIR_FULL_MAX_PARTS = 64;
+constexpr unsigned PAIR_FULL_COMPONENTS = 10;
+size_t pair_full_partial_floats(unsigned n1, unsigned n2);
+void launch_pair_full(uint4 const* posq1, unsigned n1, uint4 const* posq2, unsigned n2, bool same,
+                      force_params const& p, bool aux, bool force_zero, float aux_weight, float4* force,
+                      float* stress, unsigned stride, float* partial, hipStream_t s);
+
 // velocity-Verlet step parameters (integrate.hip, and the fused force kernel)
 struct integrate_params {
     float dt_hi, dt_lo, half_dt;

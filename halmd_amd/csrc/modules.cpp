@@ -299,6 +299,9 @@ void particle::get_reverse_id(unsigned* v) {
 // module overwrites, later ones accumulate; aux variables are computed only
 // when requested
 void particle::update_force(bool need_aux) {
+    for (force_module* f : forces) {
+        if (f->inputs_changed()) { force_valid = aux_valid = false; break; }
+    }
     if (force_valid && (!need_aux || aux_valid)) return;
     // a pending kick refers to the plain force of the current positions:
     // apply it before that force is replaced by one with aux (settle_velocity
@@ -534,8 +537,11 @@ void potential::eval(double rr, unsigned a, unsigned b, double& fval, double& en
     }
 }
 
-force_params potential::device_params(box const& b) const {
-    require(trunc >= 0, "potential: no truncation applied");
+force_params potential::device_params(box const& b, bool full) const {
+    if (full)
+        require(trunc < 0, "pair_full: the potential must not be truncated");
+    else
+        require(trunc >= 0, "potential: no truncation applied");
     require(ntype1 == ntype2, "potential: ntype1 != ntype2 is not supported by the device kernels");
     force_params P;
     std::memset(&P, 0, sizeof(P));
@@ -543,7 +549,7 @@ force_params potential::device_params(box const& b) const {
     // a hard core (Lennard-Jones as Mie 12-6 with C = 4)
     bool const core = std::any_of(r_core.begin(), r_core.end(), [](double x) { return x != 0; });
     P.kind = (kind == POTENTIAL_MIE || core) ? POTENTIAL_POWER_LAW : kind;
-    P.trunc = trunc;
+    P.trunc = full ? TRUNC_SHARP : trunc;
     P.ntype = ntype2;
     P.rri_smooth = trunc == TRUNC_SMOOTH_R4 ? float(1 / (h * h)) : 0.0f;
     for (int k = 0; k < 3; ++k) { P.q[k] = b.q_hi[k]; P.q_lo[k] = b.q_lo[k]; }
@@ -580,6 +586,11 @@ force_params potential::device_params(box const& b) const {
                 pp.p[1] = float(1 / (sig * B));
                 pp.p[2] = float(r_min[ab] / sig / B);
                 pp.p[3] = float(A);
+            }
+            if (full) {
+                pp.rr_cut = pp.r_cut = pp.rr_cut_q = INFINITY;
+                pp.en_cut = pp.force_cut = 0;
+                continue;
             }
             pp.rr_cut = float(rr_cut[ab]);
             pp.r_cut = float(r_cut[ab]);
@@ -985,6 +996,51 @@ pair_trunc::~pair_trunc() {
 }
 
 void pair_trunc::prepare() { nb.require_current(); }
+
+// ---------------------------------------------------------------------------
+pair_full::pair_full(potential const& pot_, particle& p1_, particle& p2_, box const& b_, double w)
+    : pot(pot_), p1(p1_), p2(p2_), b(b_), aux_weight(w)
+{
+    p1.attach_box(b);
+    p2.attach_box(b);
+    // pair_full.hpp:139-141
+    require(std::min(pot.ntype1, pot.ntype2) >= std::max(p1.nspecies, p2.nspecies),
+            "pair_full: size of potential coefficients less than number of particle species");
+    require(pot.ntype1 == pot.ntype2 && pot.ntype2 == p2.nspecies,
+            "pair_full: potential shape != nspecies of the second instance");
+    params = pot.device_params(b, true);
+    partial_.resize(std::max<size_t>(1, pair_full_partial_floats(p1.n, p2.n)));
+    if (&p1 != &p2) HIP_CHECK(hipEventCreateWithFlags(&p2_ready_, hipEventDisableTiming));
+    p1.forces.push_back(this);
+    p1.force_valid = p1.aux_valid = false;
+}
+
+pair_full::~pair_full() {
+    auto& v = p1.forces;
+    v.erase(std::remove(v.begin(), v.end(), static_cast<force_module*>(this)), v.end());
+    p1.force_valid = p1.aux_valid = false;
+    if (p2_ready_) (void)hipEventDestroy(p2_ready_);
+}
+
+void pair_full::prepare() {}
+
+bool pair_full::inputs_changed() const { return &p1 != &p2 && p2.position_version != p2_version_; }
+
+void pair_full::apply(bool force_zero) {
+    bool const aux = p1.aux_enabled;
+    // a particle interacting with itself counts each pair twice (pair_full.hpp:262-265)
+    float const w = float(&p1 == &p2 ? aux_weight / 2 : aux_weight);
+    if (&p1 != &p2) {
+        // the positions of the second instance as its own stream leaves them
+        HIP_CHECK(hipEventRecord(p2_ready_, p2.stream));
+        HIP_CHECK(hipStreamWaitEvent(p1.stream, p2_ready_, 0));
+        p2_version_ = p2.position_version;
+    }
+    timer.start(p1.stream);
+    launch_pair_full(p1.posq, p1.n, p2.posq, p2.n, &p1 == &p2, params, aux, force_zero, w, p1.force, p1.stress,
+                     p1.n_pad, partial_, p1.stream);
+    timer.stop(p1.stream);
+}
 
 unsigned pair_trunc::tune() {
     if (!nb.staged) return 0;

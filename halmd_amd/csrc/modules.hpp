@@ -182,7 +182,9 @@ public:
     std::vector<double> r_core;      // hard core radius (absolute), 0 without
     double h = 0;
     double r_cut_max() const;
-    force_params device_params(box const& b) const;
+    // full: the untruncated potential for pair_full (sharp, infinite cutoff;
+    // the potential must not be truncated)
+    force_params device_params(box const& b, bool full = false) const;
     // host evaluation in FP64 (diagnostics and tables)
     void eval(double rr, unsigned a, unsigned b, double& fval, double& en) const;
 private:
@@ -313,6 +315,32 @@ public:
     // the force of the current positions and the Verlet step with it in one
     // launch (plain NVE, the only force module); false: not possible now
     virtual bool integrate_fused(verlet&) { return false; }
+    // the force depends on inputs other than its particle's positions that
+    // changed since the last apply (pair_full: the positions of its second
+    // instance)
+    virtual bool inputs_changed() const { return false; }
+};
+
+// untruncated pair force over all pairs (halmd/mdsim/gpu/forces/pair_full.hpp:
+// 116-313): the force on p1 from every particle of p2 (p1 itself: j != i)
+class pair_full : public force_module {
+public:
+    pair_full(potential const& pot, particle& p1, particle& p2, box const& b, double aux_weight);
+    ~pair_full() override;
+    potential const& pot;
+    particle& p1;
+    particle& p2;
+    box const& b;
+    double aux_weight;
+    force_params params;
+    void prepare() override;
+    void apply(bool force_zero) override;
+    bool inputs_changed() const override;
+    kernel_timer timer;
+private:
+    dbuf<float> partial_;
+    uint64_t p2_version_ = 0;
+    hipEvent_t p2_ready_ = nullptr;
 };
 
 class pair_trunc : public force_module {

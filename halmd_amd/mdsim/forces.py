@@ -1,13 +1,13 @@
-"""halmd.mdsim.forces: pair_trunc and the pair convenience constructor.
+"""halmd.mdsim.forces: pair_trunc, pair_full and the pair convenience constructor.
 
-Mirrors lua/halmd/mdsim/forces/{pair,pair_trunc}.lua.in over include/halmd_hip.h.
+Mirrors lua/halmd/mdsim/forces/{pair,pair_trunc,pair_full}.lua.in over include/halmd_hip.h.
 """
 import ctypes
 
 from .._lib import Handle, call, out_handle
 from . import neighbour as _neighbour
 
-__all__ = ["pair_trunc", "pair"]
+__all__ = ["pair_trunc", "pair_full", "pair"]
 
 
 class pair_trunc(Handle):
@@ -47,6 +47,36 @@ class pair_trunc(Handle):
         return v.value
 
 
+class pair_full(Handle):
+    """Untruncated pair force over all pairs (lua/halmd/mdsim/forces/pair_full.lua.in;
+    gpu/forces/pair_full.hpp:116-313).  particle is one instance or a pair
+    (particle1, particle2): the force on particle1 from every particle of
+    particle2.  The potential must not be truncated."""
+
+    _destroy = "halmd_mdsim_forces_pair_full_destroy"
+
+    def __init__(self, box, particle, potential, weight=1.0):
+        super().__init__()
+        if not isinstance(particle, (tuple, list)):
+            particle = (particle, particle)
+        if len(particle) != 2:
+            raise ValueError("bad argument 'particle'")
+        if getattr(potential, "r_cut", None) is not None:
+            raise ValueError("pair_full: the potential is truncated (use pair_trunc)")
+        self.potential = potential
+        self.particle = particle[0]
+        self._deps = (box, particle[1])
+        self._h = out_handle()
+        call("halmd_mdsim_forces_pair_full_create", potential.handle, particle[0].handle, particle[1].handle,
+             box.handle, float(weight), ctypes.byref(self._h))
+
+    def apply(self):
+        call("halmd_mdsim_forces_pair_full_apply", self.handle)
+
+
 def pair(box, particle, potential, neighbour=None, weight=1.0):
-    """forces.pair{box, particle, potential, neighbour}: truncated potentials use pair_trunc."""
+    """forces.pair{box, particle, potential, neighbour} (lua/halmd/mdsim/forces/pair.lua.in):
+    truncated potentials use pair_trunc, untruncated ones pair_full."""
+    if getattr(potential, "r_cut", None) is None:
+        return pair_full(box=box, particle=particle, potential=potential, weight=weight)
     return pair_trunc(box=box, particle=particle, potential=potential, neighbour=neighbour, weight=weight)

@@ -245,6 +245,20 @@ HALMD_API int halmd_mdsim_forces_pair_trunc_tune(halmd_force* f, unsigned* cells
 /* force:apply() (pair_trunc.hpp:193-215): evaluate now (overwrites the force) */
 HALMD_API int halmd_mdsim_forces_pair_trunc_apply(halmd_force* f);
 
+/* ---- mdsim.forces.pair_full: halmd/mdsim/gpu/forces/pair_full.hpp:116-313,
+ *      lua/halmd/mdsim/forces/pair_full.lua.in --------------------------------
+ * pair_full(potential, particle1, particle2, box, weight): the untruncated force
+ * on particle1 from every particle of particle2 (p2 == p1: all pairs j != i).
+ * The potential must not be truncated (forces.pair selects pair_full for such
+ * potentials, lua/halmd/mdsim/forces/pair.lua.in).  Both instances belong to
+ * the same box; a change of p2's positions invalidates p1's force. */
+HALMD_API int halmd_mdsim_forces_pair_full_create(const halmd_potential* pot, halmd_particle* p1,
+                                                  halmd_particle* p2, const halmd_box* box, double aux_weight,
+                                                  halmd_force** out);
+HALMD_API int halmd_mdsim_forces_pair_full_destroy(halmd_force* f);
+/* force:apply() (pair_full.hpp:176-196): evaluate now (overwrites the force) */
+HALMD_API int halmd_mdsim_forces_pair_full_apply(halmd_force* f);
+
 /* ---- mdsim.integrators.*: verlet.cpp:124-155, verlet_nvt_hoover.cpp:270-313,
  *      verlet_nvt_andersen.cpp:146-183 -------------------------------------- */
 HALMD_API int halmd_mdsim_integrators_verlet_create(halmd_particle* p, const halmd_box* box, double timestep,
