@@ -187,6 +187,47 @@ def batched(args):
     print(json.dumps(out), flush=True)
 
 
+TINY_H100 = 4.23e7   # particle-steps/s, KA "tiny" N=4096 on H100, unrolled force loop (23.6 ns)
+
+
+def tiny(args):
+    """The reference's "tiny" variant (doc/benchmarks.rst.in:109-127): Kob-Andersen
+    N=4096, rho=1.2, r_c=2.5 sigma shifted, skin 0.3, NVE dt=0.001, neighbour
+    lists built from all particle pairs (no binning), no sorting."""
+    from halmd_amd import workloads
+    n = 4096
+    w = workloads.kob_andersen(n, density=1.2, temperature=0.7, timestep=0.001, seed=args.seed, binning=False)
+    assert w.neighbour.algorithm == "from_particle"
+    w.sampler.run(args.warmup)
+    ms = w.sampler.run_timed(args.steps)
+    value = n * args.steps / (ms / 1e3)
+    log(f"tiny: {ms / args.steps * 1e3:.2f} us/step, {ms / args.steps * 1e6 / n:.2f} ns per step and particle, "
+        f"{value:.4e} particle-steps/s; {w.neighbour.rebuilds} list builds")
+    out = {
+        "metric": "particle-steps/s, Kob-Andersen N=4096 (tiny)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / TINY_H100,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "Kob-Andersen 80:20 N=4096, rho=1.2, T=0.7, r_c=2.5 sigma shifted, skin 0.3, NVE "
+                        "dt=0.001, neighbour lists from all pairs (from_particle), no sorting",
+            "nparticle": n,
+            "ns_per_step_and_particle": ms / args.steps * 1e6 / n,
+            "neighbour_rebuilds": int(w.neighbour.rebuilds),
+            "baseline": "H100, unrolled force loop, 23.6 ns per step and particle (4.23e7 particle-steps/s)",
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def serving(args):
     """Served job stream: Poisson arrivals of LJ / Kob-Andersen / WCA jobs (N
     log-normal, median 2e4, clipped to [1e3, 2e5]; rho in [0.8, 1.2]; T in
@@ -326,9 +367,10 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default cell-segment shape")
     ap.add_argument("--segments", default=None,
                     help="cell-segment shape 'cells,rows,threads' instead of autotuning (profiling runs)")
-    ap.add_argument("--workload", default="lj", choices=["lj", "batched", "serving"],
+    ap.add_argument("--workload", default="lj", choices=["lj", "batched", "serving", "tiny"],
                     help="lj: LJ fluid N = 2^20 (default); batched: ensemble of independent systems; "
-                         "serving: Poisson job stream through the continuous-batching server")
+                         "serving: Poisson job stream through the continuous-batching server; "
+                         "tiny: Kob-Andersen N = 4096 with all-pairs neighbour lists")
     ap.add_argument("--rate", type=float, default=50.0, help="serving: job arrivals per second")
     ap.add_argument("--duration", type=float, default=10.0, help="serving: seconds of arrivals")
     ap.add_argument("--step-scale", type=float, default=1.0 / 30, help="serving: factor on the job step counts")
@@ -350,6 +392,8 @@ def main():
         return batched(args)
     if args.workload == "serving":
         return serving(args)
+    if args.workload == "tiny":
+        return tiny(args)
     arch, cus, mem = halmd_amd.device_info()
     if args.calibration_copy_mib:
         calibration_copy(args.calibration_copy_mib << 20)

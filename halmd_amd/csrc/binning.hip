@@ -192,51 +192,68 @@ __global__ __launch_bounds__(256) void build_neighbours_kernel(
     if ((threadIdx.x % WAVE) == 0 && m > 0) atomicMax(max_count, m);
 }
 
-// --- all pairs (from_particle): tiles of 256 records in LDS, ascending j ----
-
+// --- all pairs (from_particle): one wave per particle, ascending j ----------
+// Lane l of the wave of particle i tests j = base + l for base = 0, 64, ...;
+// a ballot of the hits gives each its rank, so the list holds the j in
+// ascending order (the order of a sequential scan) while every step of the
+// wave covers 64 candidates.  Entry k goes to the blocked layout of the other
+// builds, (i / 64) 64 cap + (k / 16) 1024 + (i % 64) 16 + k % 16; entries past
+// the capacity are dropped and the count still says how many there were.
+//
+// With cond (a device-side rebuild decision): the kernel does nothing unless
+// the displacement words cond.slots exceed cond.keep_bits; when it rebuilds,
+// it also sets the reference positions r0[i] = hi(pos[i]) of the displacement
+// check and workgroup 0 counts the rebuild.
 template <bool ISO>
 __global__ __launch_bounds__(256) void build_all_pairs_kernel(uint4 const* __restrict__ posq, unsigned n,
                                                               neighbour_params p, unsigned cap,
                                                               uint32_t* __restrict__ list,
                                                               uint32_t* __restrict__ count,
-                                                              uint32_t* __restrict__ max_count) {
-    __shared__ uint4 s_tile[256];
+                                                              uint32_t* __restrict__ max_count,
+                                                              rebuild_condition cond) {
     __shared__ float s_rr[MAX_TYPES * MAX_TYPES];
-    __shared__ uint32_t s_buf[256 * NB_STAGE];
+    __shared__ unsigned s_max[256 / WAVE];
+    if (cond.slots) {
+        if (!displacement_aborts(cond.slots, cond.keep_bits)) return;   // the whole workgroup
+        unsigned const w0 = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+        if (threadIdx.x % WAVE == 0 && w0 < n) cond.r0[w0] = cond.pos[2 * size_t(w0)];
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(cond.rebuilt, 1u);
+    }
     if (threadIdx.x < MAX_TYPES * MAX_TYPES)
         s_rr[threadIdx.x] = ISO ? p.rr_cut_skin_q[threadIdx.x] : p.rr_cut_skin[threadIdx.x];
-    unsigned const i = blockIdx.x * blockDim.x + threadIdx.x;
-    nb_state s;
-    s.c = 0;
-    s.i = i;
-    s.cap = cap;
-    s.ri = i < n ? posq[i] : make_uint4(0, 0, 0, 0);
-    s.ti = (s.ri.w & 3u) * p.ntype;
-    s.out = list + size_t(i & ~(WAVE - 1)) * cap + (i & (WAVE - 1)) * NB_BLOCK;
-    s.buf = s_buf + threadIdx.x * NB_STAGE;
-    for (unsigned base = 0; base < n; base += 256) {
-        __syncthreads();
-        if (base + threadIdx.x < n) s_tile[threadIdx.x] = posq[base + threadIdx.x];
-        __syncthreads();
-        if (i >= n) continue;
-        unsigned const m = min(256u, n - base);
-        for (unsigned t = 0; t < m; ++t) {
-            uint4 const rj = s_tile[t];
-            unsigned const j = base + t;
-            if (j != i && pair_inside<ISO>(s.ri, rj, p, s_rr[s.ti + (rj.w & 3u)])) {
-                s.buf[s.c % NB_BLOCK] = j;
-                if (s.c % NB_BLOCK == NB_BLOCK - 1) flush_block(s, s.c / NB_BLOCK);
-                ++s.c;
-            }
-        }
-    }
+    __syncthreads();
+    unsigned const lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    unsigned const i = blockIdx.x * (blockDim.x / WAVE) + w;   // wave-uniform
+    unsigned c = 0;
     if (i < n) {
-        if (s.c % NB_BLOCK) flush_block(s, s.c / NB_BLOCK);
-        count[i] = s.c;
+        uint4 const ri = posq[i];
+        unsigned const ti = (ri.w & 3u) * p.ntype;
+        uint32_t* const out = list + size_t(i & ~(WAVE - 1)) * cap + (i & (WAVE - 1)) * NB_BLOCK;
+        uint64_t const below = (uint64_t(1) << lane) - 1u;
+        for (unsigned base = 0; base < n; base += WAVE) {
+            unsigned const j = base + lane;
+            bool hit = false;
+            if (j < n) {
+                uint4 const rj = posq[j];
+                hit = j != i && pair_inside<ISO>(ri, rj, p, s_rr[ti + (rj.w & 3u)]);
+            }
+            uint64_t const m = __ballot(hit);
+            if (hit) {
+                unsigned const k = c + unsigned(__popcll(m & below));
+                if (k < cap) out[size_t(k / NB_BLOCK) * (NB_BLOCK * WAVE) + k % NB_BLOCK] = j;
+            }
+            c += unsigned(__popcll(m));
+        }
+        if (lane == 0) count[i] = c;
     }
-    unsigned m = i < n ? s.c : 0u;
-    for (int off = WAVE / 2; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off, WAVE));
-    if ((threadIdx.x % WAVE) == 0 && m > 0) atomicMax(max_count, m);
+    // largest count of the workgroup -> one atomic
+    if (lane == 0) s_max[w] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned m = 0;
+        for (unsigned v = 0; v < blockDim.x / WAVE; ++v) m = max(m, s_max[v]);
+        if (m > 0) atomicMax(max_count, m);
+    }
 }
 
 // --- cell segments: 16-bit slots into the LDS-staged neighbourhood union ----
@@ -639,12 +656,16 @@ void launch_build_neighbours_staged(uint4 const* posq, uint32_t const* offset, s
 namespace halmd {
 
 void launch_build_neighbours_all_pairs(uint4 const* posq, unsigned n, neighbour_params const& p, unsigned cap,
-                                       uint32_t* list, uint32_t* count, uint32_t* max_count, hipStream_t s) {
+                                       uint32_t* list, uint32_t* count, uint32_t* max_count, hipStream_t s,
+                                       rebuild_condition const& cond) {
     if (!n) return;
+    if (cond.slots && !(cond.pos && cond.r0 && cond.rebuilt))
+        throw error(HALMD_ERR_INVALID_ARGUMENT, "all-pairs build: incomplete rebuild condition");
+    unsigned const grid = div_up(n, 256 / WAVE);
     if (p.iso)
-        build_all_pairs_kernel<true><<<div_up(n, 256), 256, 0, s>>>(posq, n, p, cap, list, count, max_count);
+        build_all_pairs_kernel<true><<<grid, 256, 0, s>>>(posq, n, p, cap, list, count, max_count, cond);
     else
-        build_all_pairs_kernel<false><<<div_up(n, 256), 256, 0, s>>>(posq, n, p, cap, list, count, max_count);
+        build_all_pairs_kernel<false><<<grid, 256, 0, s>>>(posq, n, p, cap, list, count, max_count, cond);
     HIP_CHECK(hipGetLastError());
 }
 

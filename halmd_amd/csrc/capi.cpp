@@ -411,7 +411,7 @@ HALMD_API int halmd_mdsim_neighbours_from_binning_set_segments(halmd_neighbour* 
     });
 }
 HALMD_API int halmd_mdsim_neighbours_from_binning_rebuilds(const halmd_neighbour* nb, uint64_t* count) {
-    return guard([&] { *count = as<neighbour>(nb)->rebuilds; });
+    return guard([&] { *count = as<neighbour>(const_cast<halmd_neighbour*>(nb))->rebuild_count(); });
 }
 HALMD_API int halmd_mdsim_neighbours_from_binning_on_prepend_update_sort(halmd_neighbour* hnb, halmd_sort* hs) {
     return guard([&] {

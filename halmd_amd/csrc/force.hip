@@ -235,6 +235,94 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
     }
 }
 
+// Small systems: T lanes per particle (T divides the wave), so that a few
+// thousand particles still fill the chip.  Lane u of particle i takes the
+// entries k = u, u + T, u + 2T, ... (the T lanes of a particle read T
+// consecutive words of a list block), sums them in FP32, and a butterfly over
+// the T lanes adds the partial sums in a fixed order (a log2 T deep tree);
+// lane u = 0 writes the result.
+template <int KIND, int TRUNC, bool AUX, unsigned T>
+__global__ __launch_bounds__(256) void pair_force_split_kernel(
+    uint4 const* __restrict__ posq, unsigned n, uint32_t const* __restrict__ list, uint32_t const* __restrict__ count,
+    unsigned cap, force_params P, bool force_zero, float aux_weight, float4* __restrict__ force,
+    float* __restrict__ stress, unsigned stride)
+{
+    __shared__ pair_param s_pair[MAX_TYPES * MAX_TYPES];
+    if (threadIdx.x < MAX_TYPES * MAX_TYPES) s_pair[threadIdx.x] = P.pair[threadIdx.x];
+    __syncthreads();
+
+    unsigned const t = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned const i = t / T, u = t % T;
+    // the T lanes of a particle are one aligned group of a wave: all of them
+    // return here or none
+    if (i >= n) return;
+
+    uint4 const qi = posq[i];
+    unsigned const ti = (qi.w & 3u) * P.ntype;
+    unsigned const cnt = count[i];
+    uint32_t const* lp = list + size_t(i & ~(WAVE - 1)) * cap + (i & (WAVE - 1)) * NB_BLOCK;
+
+    float fx = 0, fy = 0, fz = 0;
+    float en = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
+    for (unsigned k = u; k < cnt; k += T) {
+        uint4 const qj = posq[lp[size_t(k / NB_BLOCK) * (NB_BLOCK * WAVE) + k % NB_BLOCK]];
+        int32_t const Dx = int32_t(qi.x - qj.x), Dy = int32_t(qi.y - qj.y), Dz = int32_t(qi.z - qj.z);
+        float const dx = separation(Dx, P.q[0], P.q_lo[0]);
+        float const dy = separation(Dy, P.q[1], P.q_lo[1]);
+        float const dz = separation(Dz, P.q[2], P.q_lo[2]);
+        float const rr = dx * dx + dy * dy + dz * dz;
+        pair_param const& pp = s_pair[ti + (qj.w & 3u)];
+        if (!(rr < pp.rr_cut)) continue;
+        float const md = KIND == POTENTIAL_MORSE ? morse_argument(rr, pp) : 0.0f;
+        float fval, e;
+        pair_eval<KIND, TRUNC>(rr, md, pp, P.rri_smooth, fval, e);
+        fx += fval * dx;
+        fy += fval * dy;
+        fz += fval * dz;
+        if (AUX) {
+            en += e;
+            s0 += fval * dx * dx;
+            s1 += fval * dy * dy;
+            s2 += fval * dz * dz;
+            s3 += fval * dx * dy;
+            s4 += fval * dx * dz;
+            s5 += fval * dy * dz;
+        }
+    }
+#pragma unroll
+    for (unsigned d = T / 2; d > 0; d >>= 1) {
+        fx += __shfl_xor(fx, d, T);
+        fy += __shfl_xor(fy, d, T);
+        fz += __shfl_xor(fz, d, T);
+        if (AUX) {
+            en += __shfl_xor(en, d, T);
+            s0 += __shfl_xor(s0, d, T);
+            s1 += __shfl_xor(s1, d, T);
+            s2 += __shfl_xor(s2, d, T);
+            s3 += __shfl_xor(s3, d, T);
+            s4 += __shfl_xor(s4, d, T);
+            s5 += __shfl_xor(s5, d, T);
+        }
+    }
+    if (u != 0) return;
+    float4 f = make_float4(fx, fy, fz, AUX ? aux_weight * en : 0.0f);
+    float st[6] = {aux_weight * s0, aux_weight * s1, aux_weight * s2, aux_weight * s3, aux_weight * s4,
+                   aux_weight * s5};
+    if (!force_zero) {
+        float4 const old = force[i];
+        f.x += old.x; f.y += old.y; f.z += old.z; f.w += old.w;
+        if (AUX) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) st[c] += stress[c * size_t(stride) + i];
+        }
+    }
+    force[i] = f;
+    if (AUX) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) stress[c * size_t(stride) + i] = st[c];
+    }
+}
+
 // Cell-segment variant: the workgroup stages its segment's neighbourhood union
 // (current fixed-point positions) in LDS, then every lane walks the 16-bit slot
 // list of one particle of the segment, padded to a multiple of 8 with a
@@ -482,13 +570,6 @@ __device__ __forceinline__ void staged_force_segment(
 // every wave returns before touching memory (the caller rebuilds and launches
 // again).  Each wave reads the 64 words itself, so the decision is uniform
 // without a barrier.
-__device__ __forceinline__ bool displacement_aborts(uint32_t const* __restrict__ slots, uint32_t bits) {
-    if (!slots) return false;
-    static_assert(DISP_SLOTS == WAVE, "one displacement slot per lane");
-    unsigned m = slots[(threadIdx.x % WAVE) * DISP_STRIDE];
-    for (int o = WAVE / 2; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o, WAVE));
-    return m > bits;
-}
 
 template <int KIND, int TRUNC, bool AUX, bool ISO, bool MULTI, bool FUSE>
 __global__ __launch_bounds__(SEG_THREADS_MAX) void pair_force_staged_kernel(
@@ -587,6 +668,8 @@ __global__ __launch_bounds__(256) void ens_force_small_kernel(
     }
 }
 
+constexpr unsigned PAIR_SPLIT16_BELOW = 16384, PAIR_SPLIT4_BELOW = 65536;
+
 struct force_launch {
     uint4 const* posq;
     unsigned n;
@@ -602,8 +685,22 @@ struct force_launch {
     hipStream_t s;
 };
 
+template <int KIND, int TRUNC, unsigned T>
+void dispatch_split(force_launch const& a) {
+    unsigned const grid = unsigned(div_up(size_t(a.n) * T, 256));
+    if (a.aux)
+        pair_force_split_kernel<KIND, TRUNC, true, T><<<grid, 256, 0, a.s>>>(
+            a.posq, a.n, a.list, a.count, a.cap, a.p, a.force_zero, a.w, a.force, a.stress, a.stride);
+    else
+        pair_force_split_kernel<KIND, TRUNC, false, T><<<grid, 256, 0, a.s>>>(
+            a.posq, a.n, a.list, a.count, a.cap, a.p, a.force_zero, a.w, a.force, a.stress, a.stride);
+}
+
 template <int KIND, int TRUNC>
 void dispatch_aux(force_launch const& a) {
+    // lanes per particle from the particle count alone: at least ~1000 waves
+    if (a.n < PAIR_SPLIT16_BELOW) return dispatch_split<KIND, TRUNC, 16>(a);
+    if (a.n < PAIR_SPLIT4_BELOW) return dispatch_split<KIND, TRUNC, 4>(a);
     unsigned const grid = div_up(a.n, 256);
     if (a.aux)
         pair_force_kernel<KIND, TRUNC, true><<<grid, 256, 0, a.s>>>(a.posq, a.n, a.list, a.count, a.cap, a.p,

@@ -95,10 +95,23 @@ void launch_build_neighbours(uint4 const* posq, unsigned n, uint32_t const* offs
                              neighbour_params const& p, unsigned cap, uint32_t* list, uint32_t* count,
                              uint32_t* max_count, hipStream_t s);
 
+// A rebuild decided on the device: rebuild only if the largest of the
+// displacement words at slots (DISP_SLOTS words, DISP_STRIDE apart) exceeds
+// keep_bits, and then also set r0[i] = hi(pos[i]) and add 1 to *rebuilt.
+// slots == nullptr: rebuild unconditionally (the other fields unused).
+struct rebuild_condition {
+    uint32_t const* slots = nullptr;
+    uint32_t keep_bits = 0;
+    float4 const* pos = nullptr;
+    float4* r0 = nullptr;
+    uint32_t* rebuilt = nullptr;
+};
+
 // lists from all particle pairs (small boxes): ascending j, the same predicate
 // and list layout as launch_build_neighbours
 void launch_build_neighbours_all_pairs(uint4 const* posq, unsigned n, neighbour_params const& p, unsigned cap,
-                                       uint32_t* list, uint32_t* count, uint32_t* max_count, hipStream_t s);
+                                       uint32_t* list, uint32_t* count, uint32_t* max_count, hipStream_t s,
+                                       rebuild_condition const& cond = rebuild_condition());
 
 // ---------------------------------------------------------------------------
 // cell segments (binning.hip, force.hip): particles stored in cell order, one

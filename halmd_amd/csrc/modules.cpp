@@ -751,6 +751,10 @@ double max_displacement::finish_readback() {
 // neighbour lists from binning (halmd/mdsim/gpu/neighbours/from_binning.cpp)
 // or from all particle pairs (halmd/mdsim/gpu/neighbours/from_particle.cpp)
 // ---------------------------------------------------------------------------
+// all-pairs lists get room for every particle (and so never overflow) up to
+// this size: 256 MiB, 8192 particles
+constexpr size_t FULL_CAP_BYTES = size_t(256) << 20;
+
 neighbour::neighbour(particle& p_, binning* bin_, max_displacement& disp_, box const& b_, double const* rc,
                      unsigned nt1, unsigned nt2, double skin, double occupancy_)
     : p(p_), bin(bin_), disp(disp_), b(b_), r_skin(skin), occupancy(occupancy_)
@@ -819,12 +823,35 @@ neighbour::neighbour(particle& p_, binning* bin_, max_displacement& disp_, box c
         set_segments(S, R, SEG_THREADS_MAX);
     } else {
         cap = unsigned(round_up(size_t(std::ceil(nmean / occupancy)) + NB_BLOCK, NB_BLOCK));
+        // all-pairs lists: room for every particle when that stays small
+        size_t const full = round_up(p.n, NB_BLOCK);
+        if (!bin && size_t(round_up(p.n, WAVE)) * full * sizeof(uint32_t) <= FULL_CAP_BYTES) {
+            cap = unsigned(std::max<size_t>(cap, full));
+            full_cap = true;
+            d_rebuilt.resize(1);
+            launch_fill_u32(d_rebuilt, 0u, 1, p.stream);
+        }
         list.resize(size_t(round_up(p.n, WAVE)) * cap);
     }
+    ++disp.users;
 }
 
 neighbour::~neighbour() {
     if (p.order_owner == this) p.order_owner = nullptr;
+    --disp.users;
+}
+
+bool neighbour::device_decides() const {
+    return full_cap && p.speculative_force && !staged && !bin && built && built_order == p.order_version &&
+           on_prepend_update.empty() && on_append_update.empty() && disp.users == 1 && disp.speculation_ready();
+}
+
+uint64_t neighbour::rebuild_count() {
+    if (!full_cap) return rebuilds;
+    uint32_t h = 0;
+    d_rebuilt.download(&h, 1, p.stream);
+    p.sync();
+    return rebuilds + h;
 }
 
 void segment_shape(double occupancy, cell_grid const& g, double target, unsigned& S, unsigned& R) {
@@ -858,6 +885,18 @@ void neighbour::plan_segments(unsigned S, unsigned R) {
 }
 
 void neighbour::require_current() {
+    if (built && built_order == p.order_version && device_checked_version == p.position_version) return;
+    if (device_decides()) {
+        rebuild_condition c;
+        c.slots = disp.current_slots();
+        c.keep_bits = keep_bits;
+        c.pos = p.pos;
+        c.r0 = disp.r0;
+        c.rebuilt = d_rebuilt;
+        launch_build_neighbours_all_pairs(p.posq, p.n, params, cap, list, count, d_maxcount, p.stream, c);
+        device_checked_version = p.position_version;
+        return;
+    }
     if (built && built_order == p.order_version && disp.compute() <= r_skin / 2) return;
     for (auto& f : on_prepend_update) f();
     update();

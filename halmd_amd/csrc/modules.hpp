@@ -230,6 +230,7 @@ public:
     dbuf<uint32_t> d_max2;
     hbuf<uint32_t> h_max2;
     unsigned updates = 0;
+    unsigned users = 0;               // neighbour modules that check against r0
     uint64_t tracked_version = 0;     // position version whose maxima the last update wrote
     uint64_t computed_version = 0;
     double value = 0;
@@ -288,7 +289,18 @@ public:
     double mean_occupancy() const;
     uint64_t built_order = 0;
     bool built = false;
-    uint64_t rebuilds = 0;
+    uint64_t rebuilds = 0;            // host-side builds; rebuild_count() adds the device-side ones
+    uint64_t rebuild_count();
+    // All-pairs lists whose capacity holds every particle (no overflow) can
+    // rebuild without the host: the build kernel reads the displacement words
+    // the integrate kernel wrote and does nothing unless they exceed keep_bits
+    // (it then also resets r0).  Used when this module alone checks against
+    // its displacement, with no update hooks, and particle.speculative_force
+    // is set; the lists and rebuild steps are the same as with the host check.
+    bool full_cap = false;
+    dbuf<uint32_t> d_rebuilt;
+    uint64_t device_checked_version = 0;   // position version the device decided for
+    bool device_decides() const;
     std::vector<std::function<void()>> on_prepend_update, on_append_update;
     // lazy access: rebuild when reordered or max displacement > skin / 2
     void require_current();

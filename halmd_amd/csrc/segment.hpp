@@ -38,6 +38,17 @@ __device__ __forceinline__ void seg_own(seg_stage const& st, unsigned t, unsigne
     i = st.own_begin[r] + (t - st.own_pre[r]);
 }
 
+// the largest of the DISP_SLOTS displacement words at slots exceeds bits
+// (false without slots).  Each wave reads the 64 words itself, so the
+// decision is uniform without a barrier.
+__device__ __forceinline__ bool displacement_aborts(uint32_t const* __restrict__ slots, uint32_t bits) {
+    if (!slots) return false;
+    static_assert(DISP_SLOTS == WAVE, "one displacement slot per lane");
+    unsigned m = slots[(threadIdx.x % WAVE) * DISP_STRIDE];
+    for (int o = WAVE / 2; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o, WAVE));
+    return m > bits;
+}
+
 // inclusive sum over the lanes of a wave
 __device__ __forceinline__ unsigned wave_inclusive_sum(unsigned x) {
     unsigned const lane = threadIdx.x % WAVE;

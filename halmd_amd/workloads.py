@@ -33,8 +33,9 @@ def system_inputs(n, density, temperature, seed=1, jitter=0.0):
 
 def fluid(nparticle, density, temperature, potential, skin=0.3, timestep=0.005, seed=1, species=None,
           integrator="verlet", resonance_frequency=5.0, collision_rate=1.0, sort_interval=0, hilbert_on_rebuild=False,
-          jitter=0.0):
-    """fcc lattice at `density`, Maxwell-Boltzmann velocities at `temperature`."""
+          jitter=0.0, binning=None):
+    """fcc lattice at `density`, Maxwell-Boltzmann velocities at `temperature`;
+    binning=False: neighbour lists from all particle pairs (from_particle)."""
     n = int(nparticle)
     L, r0, v0, a = system_inputs(n, density, temperature, seed, jitter)
     nspecies = 1 if species is None else int(np.max(species)) + 1
@@ -46,7 +47,7 @@ def fluid(nparticle, density, temperature, potential, skin=0.3, timestep=0.005, 
     sp = np.zeros(n, dtype=np.uint32) if species is None else np.asarray(species, dtype=np.uint32)
     if species is not None:
         particle.data["species"] = sp
-    neighbour = mdsim.neighbour(particle=particle, box=box, r_cut=potential.r_cut, skin=skin,
+    neighbour = mdsim.neighbour(particle=particle, box=box, r_cut=potential.r_cut, skin=skin, binning=binning,
                                 disable_sorting=not hilbert_on_rebuild)
     force = mdsim.forces.pair_trunc(box=box, particle=particle, potential=potential, neighbour=neighbour)
     if integrator == "verlet":

@@ -440,6 +440,33 @@ def test_speculative_force_is_bitwise_identical():
     assert runs[0][2] == runs[1][2]
 
 
+def test_device_decided_rebuild_is_bitwise_identical(oracle):
+    # all-pairs lists that rebuild on the device (no host displacement check)
+    # give the trajectory, the rebuild count and the lists of the host-checked
+    # schedule bit for bit; the lists stay the oracle's
+    runs = []
+    for device in (True, False):
+        w = workloads.kob_andersen(2048, density=1.2, temperature=1.0, timestep=0.002, seed=17, binning=False)
+        assert w.neighbour.algorithm == "from_particle"
+        w.particle.set_speculative_force(device)
+        w.sampler.run(400)
+        counts, entries = w.neighbour.lists()
+        runs.append((w.particle.ordered("position"), w.particle.ordered("velocity"), w.neighbour.rebuilds,
+                     counts, entries, w.particle.data["position_fixed"], w.particle.data["species"]))
+    assert runs[0][2] == runs[1][2] and runs[0][2] >= 10
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+    assert np.array_equal(runs[0][3], runs[1][3]) and np.array_equal(runs[0][4], runs[1][4])
+    # the lists were built at the positions of the last rebuild; at the current
+    # positions every pair within its cutoff must be listed
+    u, sp = runs[0][5], runs[0][6]
+    L = w.length
+    rc = 2.5 * np.array([[1.0, 0.8], [0.8, 0.88]])   # the KA cutoffs 2.5 sigma_ab
+    ocounts, olists, _ = oracle.all_pairs_neighbour_list(u, sp, L, (rc ** 2).astype(np.float32),
+                                                         runs[0][4].shape[1])
+    for i in range(0, len(u), 97):
+        assert set(olists[i, :ocounts[i]]) <= set(runs[0][4][i, :runs[0][3][i]])
+
+
 @pytest.mark.parametrize("kind", ["lj", "ka"])
 def test_fused_step_is_bitwise_identical(kind):
     # the force kernel that integrates each particle as it finishes it gives
