@@ -241,11 +241,15 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
 // consecutive words of a list block), sums them in FP32, and a butterfly over
 // the T lanes adds the partial sums in a fixed order (a log2 T deep tree);
 // lane u = 0 writes the result.
-template <int KIND, int TRUNC, bool AUX, unsigned T>
+//
+// FUSE: lane u = 0 then makes the Verlet step of particle i with its force (as
+// the cell-segment kernel does), writing the new fixed-point record to
+// F.posq_out and the wave's largest squared displacement to F.disp_write.
+template <int KIND, int TRUNC, bool AUX, unsigned T, bool FUSE>
 __global__ __launch_bounds__(256) void pair_force_split_kernel(
     uint4 const* __restrict__ posq, unsigned n, uint32_t const* __restrict__ list, uint32_t const* __restrict__ count,
     unsigned cap, force_params P, bool force_zero, float aux_weight, float4* __restrict__ force,
-    float* __restrict__ stress, unsigned stride)
+    float* __restrict__ stress, unsigned stride, fused_integrate F)
 {
     __shared__ pair_param s_pair[MAX_TYPES * MAX_TYPES];
     if (threadIdx.x < MAX_TYPES * MAX_TYPES) s_pair[threadIdx.x] = P.pair[threadIdx.x];
@@ -253,13 +257,12 @@ __global__ __launch_bounds__(256) void pair_force_split_kernel(
 
     unsigned const t = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned const i = t / T, u = t % T;
-    // the T lanes of a particle are one aligned group of a wave: all of them
-    // return here or none
-    if (i >= n) return;
-
-    uint4 const qi = posq[i];
+    // lanes past the last particle stay (with no entries) for the wave-wide
+    // reductions; the T lanes of a particle are one aligned group of a wave
+    bool const live = i < n;
+    uint4 const qi = posq[live ? i : 0u];
     unsigned const ti = (qi.w & 3u) * P.ntype;
-    unsigned const cnt = count[i];
+    unsigned const cnt = live ? count[i] : 0u;
     uint32_t const* lp = list + size_t(i & ~(WAVE - 1)) * cap + (i & (WAVE - 1)) * NB_BLOCK;
 
     float fx = 0, fy = 0, fz = 0;
@@ -304,7 +307,16 @@ __global__ __launch_bounds__(256) void pair_force_split_kernel(
             s5 += __shfl_xor(s5, d, T);
         }
     }
-    if (u != 0) return;
+    if (FUSE) {
+        float d2 = 0.0f;
+        if (live && u == 0)
+            d2 = integrate_particle_f(F.pos, F.vel, make_float4(fx, fy, fz, 0.0f), F.image, F.posq_out, i, F.ip,
+                                      F.pre_half_dt, nullptr, F.r0);
+        wave_max_to(d2, disp_slot(F.disp_write));   // every lane of the wave
+        if (blockIdx.x == 0 && threadIdx.x < DISP_SLOTS) F.disp_clear[threadIdx.x * DISP_STRIDE] = 0u;
+        return;
+    }
+    if (!live || u != 0) return;
     float4 f = make_float4(fx, fy, fz, AUX ? aux_weight * en : 0.0f);
     float st[6] = {aux_weight * s0, aux_weight * s1, aux_weight * s2, aux_weight * s3, aux_weight * s4,
                    aux_weight * s5};
@@ -683,17 +695,22 @@ struct force_launch {
     float* stress;
     unsigned stride;
     hipStream_t s;
+    fused_integrate const* fuse;
 };
 
 template <int KIND, int TRUNC, unsigned T>
 void dispatch_split(force_launch const& a) {
     unsigned const grid = unsigned(div_up(size_t(a.n) * T, 256));
-    if (a.aux)
-        pair_force_split_kernel<KIND, TRUNC, true, T><<<grid, 256, 0, a.s>>>(
-            a.posq, a.n, a.list, a.count, a.cap, a.p, a.force_zero, a.w, a.force, a.stress, a.stride);
+    fused_integrate const F = a.fuse ? *a.fuse : fused_integrate{};
+    if (a.fuse)
+        pair_force_split_kernel<KIND, TRUNC, false, T, true><<<grid, 256, 0, a.s>>>(
+            a.posq, a.n, a.list, a.count, a.cap, a.p, a.force_zero, a.w, a.force, a.stress, a.stride, F);
+    else if (a.aux)
+        pair_force_split_kernel<KIND, TRUNC, true, T, false><<<grid, 256, 0, a.s>>>(
+            a.posq, a.n, a.list, a.count, a.cap, a.p, a.force_zero, a.w, a.force, a.stress, a.stride, F);
     else
-        pair_force_split_kernel<KIND, TRUNC, false, T><<<grid, 256, 0, a.s>>>(
-            a.posq, a.n, a.list, a.count, a.cap, a.p, a.force_zero, a.w, a.force, a.stress, a.stride);
+        pair_force_split_kernel<KIND, TRUNC, false, T, false><<<grid, 256, 0, a.s>>>(
+            a.posq, a.n, a.list, a.count, a.cap, a.p, a.force_zero, a.w, a.force, a.stress, a.stride, F);
 }
 
 template <int KIND, int TRUNC>
@@ -701,6 +718,7 @@ void dispatch_aux(force_launch const& a) {
     // lanes per particle from the particle count alone: at least ~1000 waves
     if (a.n < PAIR_SPLIT16_BELOW) return dispatch_split<KIND, TRUNC, 16>(a);
     if (a.n < PAIR_SPLIT4_BELOW) return dispatch_split<KIND, TRUNC, 4>(a);
+    if (a.fuse) throw error(HALMD_ERR_INVALID_ARGUMENT, "pair force: no fused step for this many particles");
     unsigned const grid = div_up(a.n, 256);
     if (a.aux)
         pair_force_kernel<KIND, TRUNC, true><<<grid, 256, 0, a.s>>>(a.posq, a.n, a.list, a.count, a.cap, a.p,
@@ -943,11 +961,14 @@ void pair_full_go(uint4 const* posq1, unsigned n1, uint4 const* posq2, unsigned 
 }
 } // namespace
 
+bool pair_force_fusable(unsigned n) { return n < PAIR_SPLIT4_BELOW; }
+
 void launch_pair_force(uint4 const* posq, unsigned n, uint32_t const* list, uint32_t const* count, unsigned cap,
                        force_params const& p, bool aux, bool force_zero, float aux_weight, float4* force,
-                       float* stress, unsigned stride, hipStream_t s) {
+                       float* stress, unsigned stride, hipStream_t s, fused_integrate const* fuse) {
     if (!n) return;
-    force_launch const a{posq, n, list, count, cap, p, aux, force_zero, aux_weight, force, stress, stride, s};
+    if (fuse && aux) throw error(HALMD_ERR_INVALID_ARGUMENT, "pair force: the fused step has no aux path");
+    force_launch const a{posq, n, list, count, cap, p, aux, force_zero, aux_weight, force, stress, stride, s, fuse};
     if (p.kind == POTENTIAL_LENNARD_JONES)
         dispatch_trunc<POTENTIAL_LENNARD_JONES>(a);
     else if (p.kind == POTENTIAL_MORSE)

@@ -238,10 +238,14 @@ struct force_params {
     float q_p0[2], q2_p0[2];   // q p[0] and q^2 p[0] of pair 0 as double words (cubic boxes, one species)
     pair_param pair[MAX_TYPES * MAX_TYPES];
 };
-// force_zero: overwrite force/en_pot/stress, else add to the stored values
+// force_zero: overwrite force/en_pot/stress, else add to the stored values.
+// fuse: the force kernel makes the Verlet step of each particle with its force
+// (no aux; only for pair_force_fusable(n) particles)
+struct fused_integrate;
+bool pair_force_fusable(unsigned n);
 void launch_pair_force(uint4 const* posq, unsigned n, uint32_t const* list, uint32_t const* count, unsigned cap,
                        force_params const& p, bool aux, bool force_zero, float aux_weight, float4* force,
-                       float* stress, unsigned stride, hipStream_t s);
+                       float* stress, unsigned stride, hipStream_t s, fused_integrate const* fuse = nullptr);
 
 // untruncated forces over all pairs of two particle instances (same: one
 // instance, j != i); p as from potential::device_params for pair_full (sharp,

@@ -1142,14 +1142,16 @@ bool pair_trunc::launch(bool force_zero, uint32_t const* abort_slots, uint32_t a
                                  force_zero, w, p.force, p.stress, p.n_pad, p.stream, abort_slots, abort_bits, fuse);
     else
         launch_pair_force(p.posq, p.n, nb.list, nb.count, nb.cap, params, aux, force_zero, w, p.force, p.stress,
-                          p.n_pad, p.stream);
+                          p.n_pad, p.stream, fuse);
     return timer.stop(p.stream);
 }
 
 bool pair_trunc::integrate_fused(verlet& v) {
-    if (!nb.staged || v.fused_disp != &nb.disp) return false;
+    if (v.fused_disp != &nb.disp) return false;
+    if (!nb.staged && !pair_force_fusable(p.n)) return false;
     max_displacement& d = nb.disp;
-    bool const spec = p.speculative_force && nb.current_but_displacement() && d.speculation_ready();
+    // (global-index lists: the check, on the host or on the device, comes first)
+    bool const spec = nb.staged && p.speculative_force && nb.current_but_displacement() && d.speculation_ready();
     if (!spec) nb.require_current();   // lists for the current positions (may rebuild and reorder)
     float const pre = p.pending_half_dt;
     auto args = [&]() {

@@ -467,7 +467,7 @@ def test_device_decided_rebuild_is_bitwise_identical(oracle):
         assert set(olists[i, :ocounts[i]]) <= set(runs[0][4][i, :runs[0][3][i]])
 
 
-@pytest.mark.parametrize("kind", ["lj", "ka"])
+@pytest.mark.parametrize("kind", ["lj", "ka", "ka_all_pairs"])
 def test_fused_step_is_bitwise_identical(kind):
     # the force kernel that integrates each particle as it finishes it gives
     # the trajectory of separate force and integrate kernels bit for bit,
@@ -476,6 +476,9 @@ def test_fused_step_is_bitwise_identical(kind):
     for fused in (True, False):
         if kind == "lj":
             w = workloads.lennard_jones(6912, density=0.8442, temperature=1.5, seed=82, sort_interval=40)
+        elif kind == "ka_all_pairs":
+            # all-pairs lists (split-lane force kernel, rebuilds decided on the device)
+            w = workloads.kob_andersen(3000, density=1.2, temperature=0.9, seed=84, binning=False)
         else:
             w = workloads.kob_andersen(8788, density=1.2, temperature=0.9, seed=83, sort_interval=40)
         w.particle.set_fused_step(fused)
