@@ -560,8 +560,10 @@ __global__ __launch_bounds__(SEG_THREADS_MAX) void ens_build_kernel(
                                       list + d.list_off, count + d.base, stats + 2 * sys, s_rec, st, s_bound, s_rr);
 }
 
-// all-pairs lists of one small system per workgroup (the system in LDS);
-// entries are particle indices of the system, ascending
+// all-pairs lists of small systems: workgroup (x, y) builds the lists of
+// particles y ENS_SMALL_P .. of system rsmall[x], with the whole system staged
+// in LDS; one wave per particle, lane l testing j = base + l, a ballot giving
+// each hit its rank (entries are particle indices of the system, ascending)
 __global__ __launch_bounds__(256) void ens_build_small_kernel(uint4 const* __restrict__ posq,
                                                               ens_desc const* __restrict__ descs,
                                                               uint32_t const* __restrict__ rsmall,
@@ -571,27 +573,36 @@ __global__ __launch_bounds__(256) void ens_build_small_kernel(uint4 const* __res
     extern __shared__ uint4 s_rec[];
     unsigned const sys = rsmall[blockIdx.x];
     ens_desc const& d = descs[sys];
-    for (unsigned t = threadIdx.x; t < d.n; t += blockDim.x) s_rec[t] = posq[d.base + t];
+    unsigned const i0 = blockIdx.y * ENS_SMALL_P;
+    if (i0 >= d.n) return;   // the whole workgroup
+    unsigned const n = d.n, cap = d.cap;
+    for (unsigned t = threadIdx.x; t < n; t += blockDim.x) s_rec[t] = posq[d.base + t];
     __syncthreads();
     float const cut = d.np.rr_cut_skin_q[0];
     uint16_t* const L = list + d.list_off;
+    unsigned const lane = threadIdx.x % WAVE, W = blockDim.x / WAVE;
+    uint64_t const below = (uint64_t(1) << lane) - 1u;
     unsigned cmax = 0;
-    for (unsigned i = threadIdx.x; i < d.n; i += blockDim.x) {
+    unsigned const i1 = min(n, i0 + ENS_SMALL_P);
+    for (unsigned i = i0 + threadIdx.x / WAVE; i < i1; i += W) {   // wave-uniform
         uint4 const ri = s_rec[i];
-        uint16_t* const out = L + size_t(i & ~(WAVE - 1)) * d.cap + (i & (WAVE - 1)) * NB16_BLOCK;
+        uint16_t* const out = L + size_t(i & ~(WAVE - 1)) * cap + (i & (WAVE - 1)) * NB16_BLOCK;
         unsigned c = 0;
-        for (unsigned j = 0; j < d.n; ++j) {
-            if (j != i && pair_inside<true>(ri, s_rec[j], d.np, cut)) {
-                if (c < d.cap) out[size_t(c / NB16_BLOCK) * (WAVE * NB16_BLOCK) + c % NB16_BLOCK] = uint16_t(j);
-                ++c;
+        for (unsigned base = 0; base < n; base += WAVE) {
+            unsigned const j = base + lane;
+            bool const hit = j < n && j != i && pair_inside<true>(ri, s_rec[j < n ? j : i], d.np, cut);
+            uint64_t const m = __ballot(hit);
+            if (hit) {
+                unsigned const cc = c + unsigned(__popcll(m & below));
+                if (cc < cap) out[size_t(cc / NB16_BLOCK) * (WAVE * NB16_BLOCK) + cc % NB16_BLOCK] = uint16_t(j);
             }
+            c += unsigned(__popcll(m));
         }
-        count[d.base + i] = c;
+        if (lane == 0) count[d.base + i] = c;
         cmax = max(cmax, c);
     }
-    for (int o = WAVE / 2; o > 0; o >>= 1) cmax = max(cmax, (unsigned)__shfl_xor((int)cmax, o, WAVE));
-    if ((threadIdx.x % WAVE) == 0 && cmax > 0) atomicMax(&stats[2 * sys], cmax);
-    if (threadIdx.x == 0) atomicMax(&stats[2 * sys + 1], d.n + 1);
+    if (lane == 0 && cmax > 0) atomicMax(&stats[2 * sys], cmax);
+    if (blockIdx.y == 0 && threadIdx.x == 0) atomicMax(&stats[2 * sys + 1], n + 1);
 }
 
 } // namespace
@@ -724,8 +735,9 @@ void launch_ens_build(uint4 const* posq, uint32_t const* offsets, ens_desc const
 void launch_ens_build_small(uint4 const* posq, ens_desc const* descs, uint32_t const* rsmall, unsigned nsmall,
                             unsigned max_n, uint16_t* list, uint32_t* count, uint32_t* stats, hipStream_t s) {
     if (!nsmall) return;
-    ens_build_small_kernel<<<nsmall, 256, size_t(max_n) * sizeof(uint4), s>>>(posq, descs, rsmall, list, count,
-                                                                              stats);
+    dim3 const grid(nsmall, div_up(max_n, ENS_SMALL_P));
+    ens_build_small_kernel<<<grid, 256, size_t(max_n) * sizeof(uint4), s>>>(posq, descs, rsmall, list, count,
+                                                                            stats);
     HIP_CHECK(hipGetLastError());
 }
 

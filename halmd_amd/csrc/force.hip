@@ -622,61 +622,73 @@ __global__ __launch_bounds__(SEG_THREADS_MAX) void ens_force_kernel(
         force + d.base, AUX ? stress + d.base : nullptr, stride, s_rec, st, s_pair);
 }
 
-// all-pairs systems: one workgroup per system, the system in LDS; list
-// entries are particle indices (the arithmetic of the cubic-box path)
+// all-pairs systems: workgroup (x, y) takes particles y ENS_SMALL_P .. of
+// system fsmall[x], with the whole system staged in LDS, 16 lanes per
+// particle (lane u the entries k = u mod 16, then a butterfly over the 16
+// lanes); list entries are particle indices (the arithmetic of the cubic-box
+// path)
 template <int KIND, int TRUNC, bool AUX>
 __global__ __launch_bounds__(256) void ens_force_small_kernel(
     uint4 const* __restrict__ posq, ens_desc const* __restrict__ descs, uint32_t const* __restrict__ fsmall,
     uint16_t const* __restrict__ list, uint32_t const* __restrict__ count, float4* __restrict__ force,
     float* __restrict__ stress, unsigned stride)
 {
+    constexpr unsigned T = NB16_BLOCK;   // lanes per particle: lane u reads slot u of each list block
+    static_assert(256 / T == ENS_SMALL_P, "one particle per group of T lanes");
     extern __shared__ uint4 s_rec[];
     ens_desc const& d = descs[fsmall[blockIdx.x]];
+    unsigned const i0 = blockIdx.y * ENS_SMALL_P;
+    if (i0 >= d.n) return;   // the whole workgroup
     for (unsigned t = threadIdx.x; t < d.n; t += blockDim.x) s_rec[t] = posq[d.base + t];
     __syncthreads();
     force_params const& P = d.P;
     pair_param const& pp = P.pair[0];
-    uint16_t const* const L = list + d.list_off;
-    for (unsigned i = threadIdx.x; i < d.n; i += blockDim.x) {
-        uint4 const qi = s_rec[i];
-        unsigned const cnt = count[d.base + i];
-        uint16_t const* const lp = L + size_t(i & ~(WAVE - 1)) * d.cap + (i & (WAVE - 1)) * NB16_BLOCK;
-        float fx = 0, fy = 0, fz = 0, cx = 0, cy = 0, cz = 0, bx = 0, by = 0, bz = 0;
-        float en = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
-        for (unsigned k = 0; k < cnt; ++k) {
-            uint4 const qj = s_rec[lp[size_t(k / NB16_BLOCK) * (WAVE * NB16_BLOCK) + k % NB16_BLOCK]];
-            float const dx = float(int32_t(qi.x - qj.x)), dy = float(int32_t(qi.y - qj.y)),
-                        dz = float(int32_t(qi.z - qj.z));
-            float const rq = dx * dx + dy * dy + dz * dz;
-            bool const inside = rq < pp.rr_cut_q;
-            float const rr = __builtin_fmaf(rq, P.q2[0], rq * P.q2[1]);
-            float const md = KIND == POTENTIAL_MORSE ? morse_argument(rr, pp) : 0.0f;
-            float fval, u;
-            pair_eval<KIND, TRUNC>(rr, md, pp, P.rri_smooth, fval, u);
-            fval = inside ? fval : 0.0f;
-            bx += fval * dx;
-            by += fval * dy;
-            bz += fval * dz;
-            if (AUX) {
-                en += inside ? u : 0.0f;
-                s0 += fval * dx * dx; s1 += fval * dy * dy; s2 += fval * dz * dz;
-                s3 += fval * dx * dy; s4 += fval * dx * dz; s5 += fval * dy * dz;
-            }
-            if (k % NB16_BLOCK == NB16_BLOCK - 1 || k + 1 == cnt) {
-                kahan_add(fx, cx, bx);
-                kahan_add(fy, cy, by);
-                kahan_add(fz, cz, bz);
-                bx = by = bz = 0;
-            }
-        }
-        float const qh = P.q[0], ql = P.q_lo[0];
-        force[d.base + i] = make_float4(separation_scale(fx - cx, qh, ql), separation_scale(fy - cy, qh, ql),
-                                        separation_scale(fz - cz, qh, ql), AUX ? 0.5f * en : 0.0f);
+    unsigned const i = i0 + threadIdx.x / T, u = threadIdx.x % T;
+    bool const live = i < d.n;
+    uint4 const qi = s_rec[live ? i : i0];
+    unsigned const cnt = live ? count[d.base + i] : 0u;
+    uint16_t const* const lp = d.list_off + list + size_t(i & ~(WAVE - 1)) * d.cap + (i & (WAVE - 1)) * NB16_BLOCK + u;
+    float fx = 0, fy = 0, fz = 0;
+    float en = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
+    for (unsigned k = u; k < cnt; k += T) {
+        uint4 const qj = s_rec[lp[size_t(k / NB16_BLOCK) * (WAVE * NB16_BLOCK)]];
+        float const dx = float(int32_t(qi.x - qj.x)), dy = float(int32_t(qi.y - qj.y)),
+                    dz = float(int32_t(qi.z - qj.z));
+        float const rq = dx * dx + dy * dy + dz * dz;
+        bool const inside = rq < pp.rr_cut_q;
+        float const rr = __builtin_fmaf(rq, P.q2[0], rq * P.q2[1]);
+        float const md = KIND == POTENTIAL_MORSE ? morse_argument(rr, pp) : 0.0f;
+        float fval, e;
+        pair_eval<KIND, TRUNC>(rr, md, pp, P.rri_smooth, fval, e);
+        fval = inside ? fval : 0.0f;
+        fx += fval * dx;
+        fy += fval * dy;
+        fz += fval * dz;
         if (AUX) {
-            float const sv[6] = {s0, s1, s2, s3, s4, s5};
-            for (int c = 0; c < 6; ++c)
-                stress[c * size_t(stride) + d.base + i] = __builtin_fmaf(sv[c], 0.5f * P.q2[0], sv[c] * (0.5f * P.q2[1]));
+            en += inside ? e : 0.0f;
+            s0 += fval * dx * dx; s1 += fval * dy * dy; s2 += fval * dz * dz;
+            s3 += fval * dx * dy; s4 += fval * dx * dz; s5 += fval * dy * dz;
         }
+    }
+#pragma unroll
+    for (unsigned o = T / 2; o > 0; o >>= 1) {
+        fx += __shfl_xor(fx, o, T);
+        fy += __shfl_xor(fy, o, T);
+        fz += __shfl_xor(fz, o, T);
+        if (AUX) {
+            en += __shfl_xor(en, o, T);
+            s0 += __shfl_xor(s0, o, T); s1 += __shfl_xor(s1, o, T); s2 += __shfl_xor(s2, o, T);
+            s3 += __shfl_xor(s3, o, T); s4 += __shfl_xor(s4, o, T); s5 += __shfl_xor(s5, o, T);
+        }
+    }
+    if (!live || u != 0) return;
+    float const qh = P.q[0], ql = P.q_lo[0];
+    force[d.base + i] = make_float4(separation_scale(fx, qh, ql), separation_scale(fy, qh, ql),
+                                    separation_scale(fz, qh, ql), AUX ? 0.5f * en : 0.0f);
+    if (AUX) {
+        float const sv[6] = {s0, s1, s2, s3, s4, s5};
+        for (int c = 0; c < 6; ++c)
+            stress[c * size_t(stride) + d.base + i] = __builtin_fmaf(sv[c], 0.5f * P.q2[0], sv[c] * (0.5f * P.q2[1]));
     }
 }
 
@@ -1065,8 +1077,9 @@ void ens_force_go(bool multi, uint4 const* posq, uint32_t const* offsets, ens_de
                                                                                  count, force, stress, stride);
     }
     if (nsmall)
-        ens_force_small_kernel<KIND, TRUNC, AUX><<<nsmall, 256, size_t(small_max_n) * sizeof(uint4), s>>>(
-            posq, descs, fsmall, list, count, force, stress, stride);
+        ens_force_small_kernel<KIND, TRUNC, AUX>
+            <<<dim3(nsmall, div_up(small_max_n, ENS_SMALL_P)), 256, size_t(small_max_n) * sizeof(uint4), s>>>(
+                posq, descs, fsmall, list, count, force, stress, stride);
 }
 
 template <int KIND, int TRUNC>

@@ -377,8 +377,12 @@ void launch_ens_move(uint4* dst, uint4 const* src, ens_move const* moves, unsign
                      hipStream_t s);
 void launch_ens_r0(float4 const* pos, float4* r0, ens_desc const* descs, uint32_t const* rblk_sys, unsigned nrblk,
                    hipStream_t s);
+// particles per workgroup of the small-system (all-pairs) build and force
+constexpr unsigned ENS_SMALL_P = 16;
+
 // list builds: rseg (s << 16) | segment for cell systems; rsmall the
-// all-pairs systems (one workgroup each); stats[2 s .. 2 s + 1].  multi: cutoffs
+// all-pairs systems (a grid of workgroups each, ENS_SMALL_P particles per
+// workgroup); stats[2 s .. 2 s + 1].  multi: cutoffs
 // by species pair (the same lists as the one-species rule when ntype is 1)
 void launch_ens_build(uint4 const* posq, uint32_t const* offsets, ens_desc const* descs, uint32_t const* rseg,
                       unsigned nrseg, unsigned threads, unsigned lds_records, bool multi, uint16_t* list,
