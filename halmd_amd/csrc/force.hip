@@ -692,7 +692,7 @@ __global__ __launch_bounds__(256) void ens_force_small_kernel(
     }
 }
 
-constexpr unsigned PAIR_SPLIT16_BELOW = 16384, PAIR_SPLIT4_BELOW = 65536;
+constexpr unsigned PAIR_SPLIT32_BELOW = 8192, PAIR_SPLIT16_BELOW = 16384, PAIR_SPLIT4_BELOW = 65536;
 
 struct force_launch {
     uint4 const* posq;
@@ -728,6 +728,7 @@ void dispatch_split(force_launch const& a) {
 template <int KIND, int TRUNC>
 void dispatch_aux(force_launch const& a) {
     // lanes per particle from the particle count alone: at least ~1000 waves
+    if (a.n < PAIR_SPLIT32_BELOW) return dispatch_split<KIND, TRUNC, 32>(a);
     if (a.n < PAIR_SPLIT16_BELOW) return dispatch_split<KIND, TRUNC, 16>(a);
     if (a.n < PAIR_SPLIT4_BELOW) return dispatch_split<KIND, TRUNC, 4>(a);
     if (a.fuse) throw error(HALMD_ERR_INVALID_ARGUMENT, "pair force: no fused step for this many particles");
