@@ -192,18 +192,23 @@ __global__ __launch_bounds__(256) void build_neighbours_kernel(
     if ((threadIdx.x % WAVE) == 0 && m > 0) atomicMax(max_count, m);
 }
 
-// --- all pairs (from_particle): one wave per particle, ascending j ----------
-// Lane l of the wave of particle i tests j = base + l for base = 0, 64, ...;
-// a ballot of the hits gives each its rank, so the list holds the j in
-// ascending order (the order of a sequential scan) while every step of the
-// wave covers 64 candidates.  Entry k goes to the blocked layout of the other
-// builds, (i / 64) 64 cap + (k / 16) 1024 + (i % 64) 16 + k % 16; entries past
-// the capacity are dropped and the count still says how many there were.
+// --- all pairs (from_particle): waves over particles, ascending j -----------
+// The workgroup stages the records in tiles of 256 through LDS; each of its
+// waves takes AP_PPW particles, and for each lane l tests j = base + l against
+// them.  A ballot of the hits gives each its rank, so every list holds the j
+// in ascending order (the order of a sequential scan) while every step of a
+// wave covers 64 candidates, and each tile read from L2 serves 4 AP_PPW
+// particles.  Entry k goes to the blocked layout of the other builds,
+// (i / 64) 64 cap + (k / 16) 1024 + (i % 64) 16 + k % 16; entries past the
+// capacity are dropped and the count still says how many there were.
 //
 // With cond (a device-side rebuild decision): the kernel does nothing unless
 // the displacement words cond.slots exceed cond.keep_bits; when it rebuilds,
 // it also sets the reference positions r0[i] = hi(pos[i]) of the displacement
 // check and workgroup 0 counts the rebuild.
+constexpr unsigned AP_PPW = 2;                        // particles per wave
+constexpr unsigned AP_PER_GROUP = 256 / WAVE * AP_PPW;   // particles per 256-thread workgroup
+
 template <bool ISO>
 __global__ __launch_bounds__(256) void build_all_pairs_kernel(uint4 const* __restrict__ posq, unsigned n,
                                                               neighbour_params p, unsigned cap,
@@ -212,47 +217,67 @@ __global__ __launch_bounds__(256) void build_all_pairs_kernel(uint4 const* __res
                                                               uint32_t* __restrict__ max_count,
                                                               rebuild_condition cond) {
     __shared__ float s_rr[MAX_TYPES * MAX_TYPES];
+    __shared__ uint4 s_tile[256];
     __shared__ unsigned s_max[256 / WAVE];
+    unsigned const lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    unsigned const i0 = blockIdx.x * AP_PER_GROUP;
     if (cond.slots) {
         if (!displacement_aborts(cond.slots, cond.keep_bits)) return;   // the whole workgroup
-        unsigned const w0 = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
-        if (threadIdx.x % WAVE == 0 && w0 < n) cond.r0[w0] = cond.pos[2 * size_t(w0)];
+        if (threadIdx.x < AP_PER_GROUP && i0 + threadIdx.x < n)
+            cond.r0[i0 + threadIdx.x] = cond.pos[2 * size_t(i0 + threadIdx.x)];
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(cond.rebuilt, 1u);
     }
     if (threadIdx.x < MAX_TYPES * MAX_TYPES)
         s_rr[threadIdx.x] = ISO ? p.rr_cut_skin_q[threadIdx.x] : p.rr_cut_skin[threadIdx.x];
-    __syncthreads();
-    unsigned const lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
-    unsigned const i = blockIdx.x * (blockDim.x / WAVE) + w;   // wave-uniform
-    unsigned c = 0;
-    if (i < n) {
-        uint4 const ri = posq[i];
-        unsigned const ti = (ri.w & 3u) * p.ntype;
-        uint32_t* const out = list + size_t(i & ~(WAVE - 1)) * cap + (i & (WAVE - 1)) * NB_BLOCK;
-        uint64_t const below = (uint64_t(1) << lane) - 1u;
-        for (unsigned base = 0; base < n; base += WAVE) {
-            unsigned const j = base + lane;
-            bool hit = false;
-            if (j < n) {
-                uint4 const rj = posq[j];
-                hit = j != i && pair_inside<ISO>(ri, rj, p, s_rr[ti + (rj.w & 3u)]);
+    uint64_t const below = (uint64_t(1) << lane) - 1u;
+    unsigned ii[AP_PPW], ti[AP_PPW], c[AP_PPW];
+    uint4 ri[AP_PPW];
+#pragma unroll
+    for (unsigned q = 0; q < AP_PPW; ++q) {
+        ii[q] = i0 + w * AP_PPW + q;   // wave-uniform
+        ri[q] = posq[ii[q] < n ? ii[q] : 0u];
+        ti[q] = (ri[q].w & 3u) * p.ntype;
+        c[q] = 0;
+    }
+    for (unsigned base = 0; base < n; base += 256) {
+        __syncthreads();   // the previous tile is consumed (and s_rr written, first time)
+        if (base + threadIdx.x < n) s_tile[threadIdx.x] = posq[base + threadIdx.x];
+        __syncthreads();
+        unsigned const m = min(256u, n - base);
+        for (unsigned k = 0; k < m; k += WAVE) {
+            bool const inb = k + lane < m;
+            unsigned const j = base + k + lane;
+            uint4 const rj = s_tile[inb ? k + lane : 0u];
+#pragma unroll
+            for (unsigned q = 0; q < AP_PPW; ++q) {
+                bool const hit = inb && ii[q] < n && j != ii[q] &&
+                                 pair_inside<ISO>(ri[q], rj, p, s_rr[ti[q] + (rj.w & 3u)]);
+                uint64_t const mm = __ballot(hit);
+                if (hit) {
+                    unsigned const cc = c[q] + unsigned(__popcll(mm & below));
+                    if (cc < cap)
+                        list[size_t(ii[q] & ~(WAVE - 1)) * cap + (ii[q] & (WAVE - 1)) * NB_BLOCK +
+                             size_t(cc / NB_BLOCK) * (NB_BLOCK * WAVE) + cc % NB_BLOCK] = j;
+                }
+                c[q] += unsigned(__popcll(mm));
             }
-            uint64_t const m = __ballot(hit);
-            if (hit) {
-                unsigned const k = c + unsigned(__popcll(m & below));
-                if (k < cap) out[size_t(k / NB_BLOCK) * (NB_BLOCK * WAVE) + k % NB_BLOCK] = j;
-            }
-            c += unsigned(__popcll(m));
         }
-        if (lane == 0) count[i] = c;
+    }
+    unsigned cm = 0;
+#pragma unroll
+    for (unsigned q = 0; q < AP_PPW; ++q) {
+        if (ii[q] < n) {
+            if (lane == 0) count[ii[q]] = c[q];
+            cm = max(cm, c[q]);
+        }
     }
     // largest count of the workgroup -> one atomic
-    if (lane == 0) s_max[w] = c;
+    if (lane == 0) s_max[w] = cm;
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned m = 0;
-        for (unsigned v = 0; v < blockDim.x / WAVE; ++v) m = max(m, s_max[v]);
-        if (m > 0) atomicMax(max_count, m);
+        unsigned mx = 0;
+        for (unsigned v = 0; v < blockDim.x / WAVE; ++v) mx = max(mx, s_max[v]);
+        if (mx > 0) atomicMax(max_count, mx);
     }
 }
 
@@ -672,7 +697,7 @@ void launch_build_neighbours_all_pairs(uint4 const* posq, unsigned n, neighbour_
     if (!n) return;
     if (cond.slots && !(cond.pos && cond.r0 && cond.rebuilt))
         throw error(HALMD_ERR_INVALID_ARGUMENT, "all-pairs build: incomplete rebuild condition");
-    unsigned const grid = div_up(n, 256 / WAVE);
+    unsigned const grid = div_up(n, AP_PER_GROUP);
     if (p.iso)
         build_all_pairs_kernel<true><<<grid, 256, 0, s>>>(posq, n, p, cap, list, count, max_count, cond);
     else
