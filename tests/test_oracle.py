@@ -650,3 +650,36 @@ def test_staged_list_rule16_contains_the_sphere(oracle, n, rho, L_override):
     assert np.array_equal(got, expect)
     # the margin is thin: the list exceeds the exact sphere by well under 1 %
     assert got.sum() < 1.01 * np.sum(r <= cut)
+
+
+def test_untruncated_brute_force_closed_forms(oracle):
+    # the reference for pair_full: a sharp cutoff beyond every minimum-image
+    # distance leaves the potential untruncated.  Two particles: the
+    # Lennard-Jones closed form F = 24 eps (2 (s/r)^12 - (s/r)^6) / r, U/2 each,
+    # the virial trace F r / 2 each; three particles: the sum of the pair terms
+    L = np.array([40.0, 40.0, 40.0])
+    pot = oracle.Potential(oracle.POT_LJ, "sharp", 1.0, 1.0, 1e4)
+    for r in (0.95, 1.12, 1.7, 3.9, 12.0):
+        pos = np.array([[0.0, 0.0, 0.0], [r, 0.0, 0.0]])
+        f, u, s = oracle.pair_force_brute(pos, np.zeros(2, np.uint32), L, pot)
+        sr6 = r ** -6
+        fmag = 24 * (2 * sr6 * sr6 - sr6) / r
+        uu = 4 * (sr6 * sr6 - sr6)
+        assert np.allclose(f[0], [-fmag, 0, 0], rtol=1e-13, atol=1e-15)
+        assert np.allclose(f[1], [fmag, 0, 0], rtol=1e-13, atol=1e-15)
+        assert np.allclose(u, [uu / 2, uu / 2], rtol=1e-13, atol=1e-16)
+        assert np.allclose(s[:, 0], [fmag * r / 2, fmag * r / 2], rtol=1e-13, atol=1e-16)
+    # minimum image across the boundary: 39.0 apart in a box of 40 is 1.0 apart
+    pos = np.array([[-19.5, 0.0, 0.0], [19.5, 0.0, 0.0]])
+    f, u, _ = oracle.pair_force_brute(pos, np.zeros(2, np.uint32), L, pot)
+    assert np.allclose(f[0], [24.0, 0.0, 0.0], rtol=1e-13) and np.allclose(u, [0.0, 0.0], atol=1e-14)
+    rng = np.random.default_rng(8)
+    pos = rng.uniform(-3, 3, (3, 3))
+    f3, u3, _ = oracle.pair_force_brute(pos, np.zeros(3, np.uint32), L, pot)
+    fsum = np.zeros((3, 3))
+    usum = np.zeros(3)
+    for a, b in ((0, 1), (0, 2), (1, 2)):
+        fp, up, _ = oracle.pair_force_brute(pos[[a, b]], np.zeros(2, np.uint32), L, pot)
+        fsum[[a, b]] += fp
+        usum[[a, b]] += up
+    assert np.allclose(f3, fsum, rtol=1e-12, atol=1e-12) and np.allclose(u3, usum, rtol=1e-12, atol=1e-14)
