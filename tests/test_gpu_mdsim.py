@@ -467,6 +467,30 @@ def test_device_decided_rebuild_is_bitwise_identical(oracle):
         assert set(olists[i, :ocounts[i]]) <= set(runs[0][4][i, :runs[0][3][i]])
 
 
+def test_device_decided_rebuild_after_host_position_change(oracle):
+    # positions set from the host between device-decided steps: the host check
+    # takes over (against the reference positions the device last set), and the
+    # forces before and after further steps are those of the FP64 oracle
+    def run(steps):
+        w = workloads.kob_andersen(2048, density=1.2, temperature=1.0, timestep=0.002, seed=19, binning=False)
+        w.sampler.run(steps)
+        return w
+    # positions of the same trajectory 60 steps on (a liquid configuration,
+    # displaced past half the skin), set into the run at step 120
+    pos = run(180).particle.data["position"]
+    w = run(120)
+    L = w.length
+    w.particle.data["position"] = pos
+    species = w.particle.data["species"]
+    pot = oracle.Potential(oracle.POT_LJ, "shifted", workloads.KA_EPSILON, workloads.KA_SIGMA, 2.5)
+    fref, _, _ = oracle.pair_force(pos, species, L, pot, skin=0.3)
+    assert _force_error(w.particle.data["force"], fref) < FORCE_TOL
+    w.sampler.run(60)
+    pos = w.particle.data["position"]
+    fref, _, _ = oracle.pair_force(pos, species, L, pot, skin=0.3)
+    assert _force_error(w.particle.data["force"], fref) < FORCE_TOL
+
+
 @pytest.mark.parametrize("kind", ["lj", "ka", "ka_all_pairs"])
 def test_fused_step_is_bitwise_identical(kind):
     # the force kernel that integrates each particle as it finishes it gives
