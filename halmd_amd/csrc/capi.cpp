@@ -902,6 +902,9 @@ HALMD_API int halmd_ensemble_state(halmd_ensemble* e, unsigned system, double* p
                                    int* image) {
     return guard([&] { as<ensemble>(e)->get_state(system, position, velocity, image); });
 }
+HALMD_API int halmd_ensemble_force(halmd_ensemble* e, unsigned system, double* force) {
+    return guard([&] { nonnull(force, "force"); as<ensemble>(e)->get_force(system, force); });
+}
 HALMD_API int halmd_ensemble_rebuilds(const halmd_ensemble* e, unsigned system, uint64_t* count) {
     return guard([&] { nonnull(count, "count"); *count = as<ensemble>(e)->rebuilds(system); });
 }
@@ -989,6 +992,16 @@ HALMD_API int halmd_server_job_info(const halmd_server* s, uint64_t id, int* sta
         if (status) *status = j.status;
         if (times) { times[0] = j.req.arrival; times[1] = j.t_admit; times[2] = j.t_done; }
         if (energies) { energies[0] = j.en_pot; energies[1] = j.en_kin; }
+    });
+}
+HALMD_API int halmd_server_job_error(const halmd_server* s, uint64_t id, char* message, size_t len) {
+    return guard([&] {
+        nonnull(message, "message");
+        require(len > 0, "server: empty message buffer");
+        auto const& j = as<server>(s)->get(id);
+        size_t const k = std::min(len - 1, j.error.size());
+        std::memcpy(message, j.error.data(), k);
+        message[k] = 0;
     });
 }
 HALMD_API int halmd_server_job_levels(const halmd_server* s, uint64_t id, unsigned* levels, unsigned* block_size,

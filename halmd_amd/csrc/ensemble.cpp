@@ -83,17 +83,16 @@ size_t ensemble::footprint(unsigned n, double length, double r_cut_max, double s
     return size_t(round_up(n, ENS_BLOCK)) * (330 + 2 * cap) + 4 * (ncell * ncell * ncell + 1);
 }
 
-unsigned ensemble::add(unsigned n, double length, int kind, int trunc, unsigned ntype, double const* epsilon,
-                       double const* sigma, double const* r_cut_sigma, double const* r_min, double const* distortion,
-                       double smoothing, double timestep, double skin, double const* position, double const* velocity,
-                       unsigned const* species) {
+std::unique_ptr<ensemble::sys_host> ensemble::make_system(unsigned n, double length, int kind, int trunc,
+                                                          unsigned ntype, double const* epsilon, double const* sigma,
+                                                          double const* r_cut_sigma, double const* r_min,
+                                                          double const* distortion, double smoothing,
+                                                          double timestep, double skin, unsigned const* species) {
     require(n >= 2 && n < 65536 * 256, "ensemble: particle count out of range");
     require(length > 0 && timestep > 0 && skin >= 0, "ensemble: invalid box, timestep or skin");
     require(ntype >= 1 && ntype <= MAX_TYPES, "ensemble: invalid number of species");
     require(ntype == 1 || kind == POTENTIAL_LENNARD_JONES, "ensemble: several species need Lennard-Jones");
-    require(sys_.size() < 65535, "ensemble: at most 65535 systems");
     auto h = std::make_unique<sys_host>();
-    h->uid = next_uid_++;
     h->n = n;
     h->ntype = ntype;
     h->L = length;
@@ -107,17 +106,39 @@ unsigned ensemble::add(unsigned n, double length, int kind, int trunc, unsigned 
     h->pot->truncate(trunc, r_cut_sigma, smoothing);
     h->r_cut_max = h->pot->r_cut_max();
     require(h->r_cut_max + skin < length / 2, "ensemble: the box must exceed twice rc + skin (minimum image)");
+    if (species)
+        for (unsigned i = 0; i < n; ++i) require(species[i] < ntype, "ensemble: species out of range");
+    return h;
+}
+
+void ensemble::validate(unsigned n, double length, int kind, int trunc, unsigned ntype, double const* epsilon,
+                        double const* sigma, double const* r_cut_sigma, double const* r_min, double const* distortion,
+                        double smoothing, double timestep, double skin, unsigned const* species) {
+    auto h = make_system(n, length, kind, trunc, ntype, epsilon, sigma, r_cut_sigma, r_min, distortion, smoothing,
+                         timestep, skin, species);
+    prepare(*h);   // the layout checks of the first placement
+}
+
+unsigned ensemble::add(unsigned n, double length, int kind, int trunc, unsigned ntype, double const* epsilon,
+                       double const* sigma, double const* r_cut_sigma, double const* r_min, double const* distortion,
+                       double smoothing, double timestep, double skin, double const* position, double const* velocity,
+                       unsigned const* species) {
+    require(sys_.size() < 65535, "ensemble: at most 65535 systems");
+    auto h = make_system(n, length, kind, trunc, ntype, epsilon, sigma, r_cut_sigma, r_min, distortion, smoothing,
+                         timestep, skin, species);
+    prepare(*h);   // fails here, not at the placement, for a system the layout cannot hold
+    h->uid = next_uid_++;
     h->r.assign(position, position + 3 * size_t(n));
     h->v.assign(velocity, velocity + 3 * size_t(n));
     h->species.assign(n, 0u);
-    if (species) {
-        for (unsigned i = 0; i < n; ++i) {
-            require(species[i] < ntype, "ensemble: species out of range");
-            h->species[i] = species[i];
-        }
-    }
+    if (species) h->species.assign(species, species + n);
     sys_.push_back(std::move(h));
     return unsigned(sys_.size() - 1);
+}
+
+void ensemble::drop_unplaced(unsigned s) {
+    require(s < sys_.size() && !sys_[s]->on_device, "ensemble: only a system not yet placed can be dropped");
+    sys_.erase(sys_.begin() + s);
 }
 
 void ensemble::retire(unsigned s) {
@@ -601,6 +622,25 @@ void ensemble::thermo(double* out) {
                       thermo_, stream);
     thermo_.download(out, 4 * sys_.size(), stream);
     HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+void ensemble::get_force(unsigned s, double* force) {
+    require(committed_ && s < sys_.size() && sys_[s]->on_device, "ensemble: system index out of range");
+    apply();
+    auto& h = *sys_[s];
+    size_t const n = h.n;
+    std::vector<float4> hf(n);
+    std::vector<uint32_t> hid(n);
+    HIP_CHECK(hipMemcpyAsync(hf.data(), force_.get() + h.base, n * sizeof(float4), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipMemcpyAsync(hid.data(), id_.get() + h.base, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    for (size_t i = 0; i < n; ++i) {
+        size_t const t = hid[i];
+        require(t < n, "ensemble: corrupt particle ids");
+        force[3 * t] = hf[i].x;
+        force[3 * t + 1] = hf[i].y;
+        force[3 * t + 2] = hf[i].z;
+    }
 }
 
 void ensemble::get_state(unsigned s, double* position, double* velocity, int* image) {

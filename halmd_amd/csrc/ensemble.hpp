@@ -37,6 +37,14 @@ public:
                  double const* sigma, double const* r_cut_sigma, double const* r_min, double const* distortion,
                  double smoothing, double timestep, double skin, double const* position, double const* velocity,
                  unsigned const* species);
+    // the checks add() and the first placement make, without adding: throws
+    // std::invalid_argument / logic_error for a system the ensemble cannot hold
+    void validate(unsigned n, double length, int kind, int trunc, unsigned ntype, double const* epsilon,
+                  double const* sigma, double const* r_cut_sigma, double const* r_min, double const* distortion,
+                  double smoothing, double timestep, double skin, unsigned const* species);
+    // removes a system added since the last apply() (indices of later systems
+    // decrease by one)
+    void drop_unplaced(unsigned sys);
     // marks system `sys` for removal at the next apply()
     void retire(unsigned sys);
     // places added systems and drops retired ones (the first call places all)
@@ -48,6 +56,9 @@ public:
     // state of one system in particle-ID order: positions (in the box),
     // velocities, images (each n x 3; null to skip)
     void get_state(unsigned sys, double* position, double* velocity, int* image);
+    // pair force on each particle of one system at its current positions
+    // (particle-ID order, n x 3)
+    void get_force(unsigned sys, double* force);
     unsigned size() const { return unsigned(sys_.size()); }
     uint64_t rebuilds(unsigned sys) const;
     uint64_t uid(unsigned sys) const;
@@ -102,6 +113,10 @@ private:
     dbuf<uint32_t> fclass_tables_, fsub_tables_;
 
     void prepare(sys_host& h);
+    std::unique_ptr<sys_host> make_system(unsigned n, double length, int kind, int trunc, unsigned ntype,
+                                          double const* epsilon, double const* sigma, double const* r_cut_sigma,
+                                          double const* r_min, double const* distortion, double smoothing,
+                                          double timestep, double skin, unsigned const* species);
     void place_all();
     size_t layout_lists();
     void upload_descs();

@@ -895,6 +895,11 @@ void neighbour::require_current() {
         c.rebuilt = d_rebuilt;
         launch_build_neighbours_all_pairs(p.posq, p.n, params, cap, list, count, d_maxcount, p.stream, c);
         device_checked_version = p.position_version;
+        // the kernel may have reset r0: the words the integrate kernel wrote
+        // measure against the previous r0, so a later compute() of this
+        // position version makes a fresh pass against the r0 now in memory
+        disp.tracked_version = 0;
+        disp.computed_version = 0;
         return;
     }
     if (built && built_order == p.order_version && disp.compute() <= r_skin / 2) return;

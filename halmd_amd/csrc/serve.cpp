@@ -20,8 +20,7 @@ job_correlator::job_correlator(unsigned n, double length, double dt, uint64_t re
     : timestep(dt), n_(n), block_size_(bsz), separation_(sep), s_(stream), dcache_(dcache), hcache_(hcache),
       wv(std::move(shells))
 {
-    require(block_size_ >= 2, "correlator: minimum block size is 2");
-    require(shift != 0 && shift < block_size_, "correlator: coarse-graining shift must be in 1 .. block size - 1");
+    check(block_size_, shift);
     require(resolution_steps > 0, "correlator: resolution below one step");
     for (int k = 0; k < 3; ++k) L_[k] = length;
     for (uint64_t s = resolution_steps; s <= max_steps; s *= block_size_) {
@@ -45,6 +44,11 @@ job_correlator::job_correlator(unsigned n, double length, double dt, uint64_t re
         isf.resize(interval.size() * block_size_ * wv->shell.size());
         HIP_CHECK(hipStreamSynchronize(s_));
     }
+}
+
+void job_correlator::check(unsigned block_size, unsigned shift) {
+    require(block_size >= 2, "correlator: minimum block size is 2");
+    require(shift != 0 && shift < block_size, "correlator: coarse-graining shift must be in 1 .. block size - 1");
 }
 
 job_correlator::~job_correlator() {
@@ -262,6 +266,18 @@ uint64_t server::submit(job_request r) {
             "server: pair parameters are ntype x ntype");
     require(r.species.empty() || r.species.size() == r.n, "server: one species per particle");
     require(r.steps >= 1, "server: a job runs at least one step");
+    // every check the ensemble makes when the job is added and placed, now:
+    // a job that cannot run is refused here instead of blocking the queue
+    {
+        std::vector<double> ones(nn, 1.0);
+        ens.validate(r.n, r.length, r.kind, r.trunc, r.ntype, r.epsilon.data(), r.sigma.data(), r.r_cut_sigma.data(),
+                     ones.data(), ones.data(), 0.005, r.timestep, r.skin, r.species.empty() ? nullptr : r.species.data());
+        job_correlator::check(r.block_size, r.shift);
+        if (!r.wavenumber.empty()) {
+            for (double k : r.wavenumber) require(k > 0 && std::isfinite(k), "server: wavenumbers must be positive");
+            require(r.tolerance >= 0 && r.max_count >= 1, "server: invalid wavevector tolerance or count");
+        }
+    }
     auto j = std::make_unique<job>();
     j->id = next_id_++;
     j->footprint = job_footprint(r);
@@ -286,7 +302,7 @@ server::job const& server::get(uint64_t id) const {
 void server::release(uint64_t id) {
     auto it = jobs_.find(id);
     require(it != jobs_.end(), "server: unknown job");
-    require(it->second->status == 2, "server: only finished jobs are released");
+    require(it->second->status >= 2, "server: only finished or failed jobs are released");
     jobs_.erase(it);
 }
 
@@ -311,19 +327,31 @@ unsigned server::tick() {
         job& j = *queue_.front();
         if (used_ + j.footprint > budget_ && !active_.empty()) break;
         auto& r = j.req;
-        auto shells = take_shells(j);   // first: an invalid wavenumber set fails before the job is placed
-        std::vector<double> ones(size_t(r.ntype) * r.ntype, 1.0);
-        unsigned const idx =
-            ens.add(r.n, r.length, r.kind, r.trunc, r.ntype, r.epsilon.data(), r.sigma.data(), r.r_cut_sigma.data(),
-                    ones.data(), ones.data(), 0.005, r.timestep, r.skin, r.position.data(), r.velocity.data(),
-                    r.species.empty() ? nullptr : r.species.data());
-        j.uid = ens.uid(idx);
+        int added = -1;
+        try {
+            auto shells = take_shells(j);   // first: an invalid wavenumber set fails before the job is placed
+            std::vector<double> ones(size_t(r.ntype) * r.ntype, 1.0);
+            added = int(ens.add(r.n, r.length, r.kind, r.trunc, r.ntype, r.epsilon.data(), r.sigma.data(),
+                                r.r_cut_sigma.data(), ones.data(), ones.data(), 0.005, r.timestep, r.skin,
+                                r.position.data(), r.velocity.data(), r.species.empty() ? nullptr : r.species.data()));
+            uint64_t const steps = (r.steps + tick_ - 1) / tick_ * tick_;
+            j.corr = std::make_unique<job_correlator>(r.n, r.length, r.timestep, tick_, steps, r.block_size,
+                                                      r.shift, r.separation, std::move(shells), ens.stream, dcache_,
+                                                      hcache_);
+        } catch (std::exception const& e) {
+            // this job alone fails; the queue goes on
+            if (added >= 0) ens.drop_unplaced(unsigned(added));
+            j.corr.reset();
+            j.status = 3;
+            j.error = e.what();
+            j.t_done = now();
+            queue_.pop_front();
+            continue;
+        }
+        j.uid = ens.uid(unsigned(added));
         r.position.clear(); r.position.shrink_to_fit();
         r.velocity.clear(); r.velocity.shrink_to_fit();
         r.species.clear(); r.species.shrink_to_fit();
-        uint64_t const steps = (r.steps + tick_ - 1) / tick_ * tick_;
-        j.corr = std::make_unique<job_correlator>(r.n, r.length, r.timestep, tick_, steps, r.block_size, r.shift,
-                                                  r.separation, std::move(shells), ens.stream, dcache_, hcache_);
         j.status = 1;
         j.t_admit = now();
         used_ += j.footprint;

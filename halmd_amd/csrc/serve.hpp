@@ -23,6 +23,7 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <string>
 #include <thread>
 
 namespace halmd {
@@ -59,6 +60,8 @@ using host_cache = buffer_cache<hbuf<double>>;
 // into FP64 accumulators after a stream synchronisation.
 class job_correlator {
 public:
+    // the parameter checks of the constructor (blocking_scheme.cpp:52-99)
+    static void check(unsigned block_size, unsigned shift);
     // shells: the ISF wavevectors (null for none); buffers come from and
     // return to the caches (which outlive the correlator)
     job_correlator(unsigned n, double length, double timestep, uint64_t resolution_steps, uint64_t max_steps,
@@ -142,7 +145,8 @@ public:
     struct job {
         uint64_t id = 0;
         job_request req;
-        int status = 0;   // 0 queued, 1 running, 2 done
+        int status = 0;   // 0 queued, 1 running, 2 done, 3 failed at admission (error says why)
+        std::string error;
         uint64_t step = 0, uid = 0;
         size_t footprint = 0;
         double t_admit = 0, t_done = 0;

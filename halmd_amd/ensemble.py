@@ -113,6 +113,12 @@ class ensemble(Handle):
         call("halmd_ensemble_state", self.handle, int(k), dptr(r), dptr(v), iptr(im))
         return r, v, im
 
+    def force(self, k):
+        """Pair force on each particle of system k at its current positions (particle-ID order)."""
+        f = np.zeros((self.n[k], 3))
+        call("halmd_ensemble_force", self.handle, int(k), dptr(f))
+        return f
+
     def rebuilds(self, k):
         c = ctypes.c_uint64()
         call("halmd_ensemble_rebuilds", self.handle, int(k), ctypes.byref(c))

@@ -15,7 +15,7 @@ from .ensemble import potential_matrices
 
 __all__ = ["server"]
 
-STATUS = {0: "queued", 1: "running", 2: "done"}
+STATUS = {0: "queued", 1: "running", 2: "done", 3: "failed"}
 
 
 class server(Handle):
@@ -76,6 +76,12 @@ class server(Handle):
         e = np.zeros(2)
         call("halmd_server_job_info", self.handle, int(jid), ctypes.byref(st), dptr(t), dptr(e))
         return dict(status=STATUS[st.value], arrival=t[0], admitted=t[1], done=t[2], en_pot=e[0], en_kin=e[1])
+
+    def error(self, jid):
+        """Why a job failed at admission ('' otherwise)."""
+        buf = ctypes.create_string_buffer(1024)
+        call("halmd_server_job_error", self.handle, int(jid), buf, ctypes.c_size_t(len(buf)))
+        return buf.value.decode()
 
     def _levels(self, jid):
         lv, bs, ns = ctypes.c_uint(), ctypes.c_uint(), ctypes.c_uint()

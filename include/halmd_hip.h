@@ -433,6 +433,9 @@ HALMD_API int halmd_ensemble_thermo(halmd_ensemble* e, double* out);
  * box, velocities, periodic images */
 HALMD_API int halmd_ensemble_state(halmd_ensemble* e, unsigned system, double* position, double* velocity,
                                    int* image);
+/* pair force on each particle of one system at its current positions
+ * (particle-ID order, n x 3) */
+HALMD_API int halmd_ensemble_force(halmd_ensemble* e, unsigned system, double* force);
 HALMD_API int halmd_ensemble_rebuilds(const halmd_ensemble* e, unsigned system, uint64_t* count);
 
 /* ---- serving: continuous batching of MD jobs over one ensemble -----------
@@ -465,10 +468,15 @@ HALMD_API int halmd_server_stats(const halmd_server* s, unsigned* active, unsign
 /* host seconds spent over all ticks in: admission and placement, steps,
  * correlator samples, completion (4 doubles) */
 HALMD_API int halmd_server_phases(const halmd_server* s, double* seconds);
-/* status 0 queued, 1 running, 2 done; times: arrival, admission, completion
- * (server clock); energies: en_pot, en_kin per particle at completion */
+/* status 0 queued, 1 running, 2 done, 3 failed at admission; times: arrival,
+ * admission, completion or failure (server clock); energies: en_pot, en_kin
+ * per particle at completion.  submit() refuses a job that the ensemble could
+ * not hold (the checks of halmd_ensemble_add and of its placement); a job that
+ * still fails when it is admitted fails alone and the queue goes on. */
 HALMD_API int halmd_server_job_info(const halmd_server* s, uint64_t id, int* status, double* times,
                                     double* energies);
+/* the reason a job failed (empty otherwise), as a NUL-terminated string */
+HALMD_API int halmd_server_job_error(const halmd_server* s, uint64_t id, char* message, size_t len);
 HALMD_API int halmd_server_job_levels(const halmd_server* s, uint64_t id, unsigned* levels, unsigned* block_size,
                                       unsigned* nshell);
 /* levels x block_size entries: lag time, mean, count (the lag-0 column is 0) */
@@ -478,7 +486,7 @@ HALMD_API int halmd_server_job_isf(const halmd_server* s, uint64_t id, double* m
 /* final state of a finished job submitted with keep_state (particle-ID order) */
 HALMD_API int halmd_server_job_state(const halmd_server* s, uint64_t id, double* position, double* velocity,
                                      int* image);
-/* forgets a finished job */
+/* forgets a finished or failed job */
 HALMD_API int halmd_server_job_release(halmd_server* s, uint64_t id);
 
 /* ---- profiling: per-module kernel timers (HIP events on the stream) ------ */

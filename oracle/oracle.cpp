@@ -477,12 +477,32 @@ struct blocking_scheme_t {
 // compute] -> finalize.
 //   verlet:            halmd/mdsim/host/integrators/verlet.cpp:58-101
 //   verlet_nvt_hoover: halmd/mdsim/host/integrators/verlet_nvt_hoover.cpp:76-205
+//   verlet_nvt_andersen: halmd/mdsim/host/integrators/verlet_nvt_andersen.cpp:96-140
+//     (per-particle collision with probability rate * dt, resampled components
+//     N(0, sqrt(T)) without division by the mass).  The random numbers come from
+//     the device module's counter generator (andersen_uniform below) instead of
+//     the reference's mt19937, so that the same particles collide at the same
+//     steps on both sides; the normal variates are Box-Muller in FP64 from the
+//     same uniforms.
 //   max_displacement:  halmd/mdsim/host/max_displacement.cpp (sqrt of max rr)
 //   rebuild criterion: halmd/mdsim/host/neighbours/from_binning.cpp:84-95
 // The NHC kinetic-energy sum uses m v^2 (the host code assumes unit mass; the
 // device module uses m v^2; the two agree for unit masses).
 // ---------------------------------------------------------------------------
-enum { INT_VERLET = 0, INT_NHC = 1 };
+enum { INT_VERLET = 0, INT_NHC = 1, INT_ANDERSEN = 2 };
+
+// Counter-based uniforms of the Andersen heat bath: splitmix64 finaliser of
+// (seed, step, particle id), top 24 bits mapped to (0, 1] in FP32 (exact)
+inline uint64_t andersen_mix(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+inline float andersen_u01(uint64_t x) { return float((x >> 40) + 1) * (1.0f / 16777216.0f); }
+inline uint64_t andersen_key(uint64_t seed, uint64_t step, uint32_t id) {
+    return andersen_mix(seed ^ andersen_mix(step * 0x632be59bd9b4e019ull + id));
+}
 
 struct sim_t {
     size_t n;
@@ -501,6 +521,10 @@ struct sim_t {
     // NHC state
     double temperature = 0, nu = 0, mass_xi[2] = {0, 0}, xi[2] = {0, 0}, v_xi[2] = {0, 0}, en_nhc = 0;
     double en_kin_target_2 = 0;
+    // Andersen state: seed, collision rate; collisions counted per run
+    uint64_t andersen_seed = 0;
+    double rate = 0;
+    uint64_t collisions = 0;
     // Hilbert sort interval (0 = never): particles are reordered by the FP64
     // Hilbert code, and all per-particle arrays permuted (id tracks the
     // original index).
@@ -563,6 +587,32 @@ struct sim_t {
         v_xi[1] += (mass_xi[0] * v_xi[0] * v_xi[0] - temperature) / mass_xi[1] * dt4;
     }
 
+    // halmd/mdsim/host/integrators/verlet_nvt_andersen.cpp:96-140: a deterministic
+    // half kick unless the particle collides with the heat bath (probability
+    // rate * dt, compared in FP32 as the device module does); then every
+    // component is drawn from N(0, sqrt(T))
+    void andersen_finalize() {
+        float const coll_prob = float(rate * dt);
+        double const sqrt_t = std::sqrt(temperature);
+        double const dt2 = dt / 2;
+        for (size_t i = 0; i < n; ++i) {
+            uint64_t const key = andersen_key(andersen_seed, step, id[i]);
+            double* v = &vel[3 * i];
+            if (andersen_u01(andersen_mix(key)) > coll_prob) {
+                for (int k = 0; k < 3; ++k) v[k] += force[3 * i + k] * dt2 / mass[i];
+            } else {
+                double const u1 = andersen_u01(andersen_mix(key + 1)), u2 = andersen_u01(andersen_mix(key + 2));
+                double const u3 = andersen_u01(andersen_mix(key + 3)), u4 = andersen_u01(andersen_mix(key + 4));
+                double const r1 = sqrt_t * std::sqrt(-2.0 * std::log(u1));
+                double const r2 = sqrt_t * std::sqrt(-2.0 * std::log(u3));
+                v[0] = r1 * std::cos(2 * M_PI * u2);
+                v[1] = r1 * std::sin(2 * M_PI * u2);
+                v[2] = r2 * std::cos(2 * M_PI * u4);
+                ++collisions;
+            }
+        }
+    }
+
     void mdstep() {
         if (!force_valid) update_force();
         // integrate
@@ -582,8 +632,12 @@ struct sim_t {
         // force (lazily evaluated on finalize's read of particle.force())
         update_force();
         // finalize
-        for (size_t i = 0; i < n; ++i)
-            for (int k = 0; k < 3; ++k) vel[3 * i + k] += force[3 * i + k] * dt2 / mass[i];
+        if (integrator == INT_ANDERSEN) {
+            andersen_finalize();
+        } else {
+            for (size_t i = 0; i < n; ++i)
+                for (int k = 0; k < 3; ++k) vel[3 * i + k] += force[3 * i + k] * dt2 / mass[i];
+        }
         if (integrator == INT_NHC) {
             propagate_chain();
             en_nhc = temperature * (3.0 * double(n) * xi[0] + xi[1]);
@@ -1080,6 +1134,21 @@ void oracle_sim_nhc_state(void* h, double* xi, double* v_xi, double* en_nhc)
 }
 
 long oracle_sim_rebuilds(void* h) { return long(static_cast<sim_t*>(h)->rebuilds); }
+
+// Andersen heat bath: temperature, collision rate and seed of the counter
+// generator (the integrator must be INT_ANDERSEN)
+void oracle_sim_set_andersen(void* h, double T, double rate, uint64_t seed)
+{
+    auto* s = static_cast<sim_t*>(h);
+    s->temperature = T;
+    s->rate = rate;
+    s->andersen_seed = seed;
+}
+
+long oracle_sim_collisions(void* h) { return long(static_cast<sim_t*>(h)->collisions); }
+
+// maximum displacement |r - r0| (minimum image) since the last list rebuild
+double oracle_sim_max_displacement(void* h) { return static_cast<sim_t*>(h)->max_displacement(); }
 
 // Wall-clock seconds of `steps` MD steps (force already initialised), for the
 // CPU baseline.

@@ -26,7 +26,7 @@ _vp = ctypes.c_void_p
 
 POT_LJ, POT_MORSE, POT_POWER_LAW, POT_MIE = 0, 1, 2, 3
 TRUNCATIONS = {"sharp": 0, "shifted": 1, "force_shifted": 2, "smooth_r4": 3}
-INT_VERLET, INT_NHC = 0, 1
+INT_VERLET, INT_NHC, INT_ANDERSEN = 0, 1, 2
 
 
 def build():
@@ -63,6 +63,9 @@ def lib():
             "oracle_sim_create": (_vp, [ctypes.c_int, _d, _vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _d, _d, _d, _i]),
             "oracle_sim_destroy": (None, [_vp]),
             "oracle_sim_set_nhc": (None, [_vp, ctypes.c_double, ctypes.c_double]),
+            "oracle_sim_set_andersen": (None, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_uint64]),
+            "oracle_sim_collisions": (ctypes.c_long, [_vp]),
+            "oracle_sim_max_displacement": (ctypes.c_double, [_vp]),
             "oracle_sim_run": (ctypes.c_long, [_vp, ctypes.c_long, ctypes.c_long, ctypes.c_int, _d]),
             "oracle_sim_get": (None, [_vp, _d, _d, _l, _d, _d, _d]),
             "oracle_sim_nhc_state": (None, [_vp, _d, _d, _d]),
@@ -292,7 +295,8 @@ def hilbert_map_f64(x, y, z, depth):
 class Simulation:
     """FP64 host MD (binning + half lists + pair_trunc + verlet / NHC)."""
 
-    def __init__(self, pos, vel, species, mass, L, pot, skin, dt, integrator=INT_VERLET, temperature=None, nu=None):
+    def __init__(self, pos, vel, species, mass, L, pot, skin, dt, integrator=INT_VERLET, temperature=None, nu=None,
+                 rate=None, seed=0):
         self.pot = pot
         pos = _dd(pos)
         vel = _dd(vel)
@@ -304,6 +308,8 @@ class Simulation:
                                          _p(m, _d), _p(sp, _i))
         if integrator == INT_NHC:
             lib().oracle_sim_set_nhc(self.h, temperature, nu)
+        elif integrator == INT_ANDERSEN:
+            lib().oracle_sim_set_andersen(self.h, temperature, rate, seed)
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -332,6 +338,14 @@ class Simulation:
 
     def rebuilds(self):
         return int(lib().oracle_sim_rebuilds(self.h))
+
+    def collisions(self):
+        """Heat-bath collisions so far (Andersen)."""
+        return int(lib().oracle_sim_collisions(self.h))
+
+    def max_displacement(self):
+        """max_i |r_i - r0_i| since the last neighbour-list rebuild (FP64)."""
+        return float(lib().oracle_sim_max_displacement(self.h))
 
 
 def density_mode(pos, q):

@@ -29,7 +29,27 @@ def _alone(spec, steps):
     return pos, vel, img, w
 
 
-def test_each_system_same_alone_and_in_the_mix():
+FORCE_TOL = 1e-5          # max_i |F_i - F_i^ref| / rms(|F^ref|) against the FP64 oracle
+
+
+def _oracle_potential(oracle, spec):
+    from halmd_amd.ensemble import POTENTIALS
+    name = spec["potential"]
+    tr = spec.get("truncation", "shifted")
+    if name == "ka":
+        return oracle.Potential(oracle.POT_LJ, tr, workloads.KA_EPSILON, workloads.KA_SIGMA, 2.5, h=0.005)
+    kind, eps, sigma, rc, r_min, dist = POTENTIALS[name]
+    if kind == 0:
+        return oracle.Potential(oracle.POT_LJ, tr, eps, sigma, rc, h=0.005)
+    return oracle.Potential(oracle.POT_MORSE, tr, eps, sigma, rc, r_min=r_min, distortion=dist, h=0.005)
+
+
+def _force_error(f, fref):
+    rms = np.sqrt(np.mean(np.sum(fref ** 2, axis=1)))
+    return np.max(np.abs(f - fref)) / rms
+
+
+def test_each_system_same_alone_and_in_the_mix(oracle):
     steps = 150
     e = workloads.ensemble_from_specs(SPECS, timestep=DT)
     e.step(steps)
@@ -40,11 +60,19 @@ def test_each_system_same_alone_and_in_the_mix():
         small = int(L[0] / (w.potential.r_cut[0, 0] + 0.3)) < 3
         print(f"system {k} ({spec['potential']}, N={spec['n']}, small={small}): rebuilds {e.rebuilds(k)} vs "
               f"{w.neighbour.rebuilds}; max |dr| {np.max(np.abs((r_e + im_e * L) - (r_a + im_a * L))):.2e}")
+        # every system's forces at its current positions against the FP64 oracle
+        # (the all-pairs path of boxes under 3 cells included)
+        sp = workloads.spec_species(spec)
+        sp = np.zeros(spec["n"], np.int32) if sp is None else sp
+        fref, _, _ = oracle.pair_force(r_e, sp, L, _oracle_potential(oracle, spec), skin=0.3)
+        err = _force_error(e.force(k), fref)
+        print(f"  force vs FP64: {err:.2e}")
+        assert err < FORCE_TOL
         if small:
             # all-pairs lists: the single path's force kernel differs in rounding
             assert np.max(np.abs((r_e + im_e * L) - (r_a + im_a * L))) < 1e-4
             assert np.max(np.abs(v_e - v_a)) < 1e-3
-        else:
+            assert e.rebuilds(k) == w.neighbour.rebuilds
             # the same device code and the same rebuild rule: bitwise
             assert np.array_equal(r_e, r_a)
             assert np.array_equal(v_e, v_a)

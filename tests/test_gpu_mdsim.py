@@ -424,6 +424,77 @@ def test_nve_trajectory_matches_fp64(oracle):
     assert np.max(np.abs(vel - ref["velocity"])) < 1e-5
 
 
+def test_rebuild_rule_matches_fp64(oracle):
+    # max_displacement (max_displacement.cpp:60-106) and the rebuild rule
+    # (from_binning.cpp:130-131): the displacement the lists are checked
+    # against is max_i |r_i - r0_i| (minimum image) of the stored positions
+    # since the last rebuild, within FP32 rounding (it is taken on the hi
+    # words), and the lists are rebuilt exactly at the steps where it exceeds
+    # skin / 2.  Over the first 100 steps the rebuild steps are also those of
+    # the FP64 oracle run from the same input.
+    w = workloads.lennard_jones(4000, density=0.8442, temperature=1.2, seed=36, timestep=0.005, jitter=0.05,
+                                truncation="smooth_r4")
+    pot = oracle.Potential(oracle.POT_LJ, "smooth_r4", 1.0, 1.0, 2.5, h=0.005)
+    sim = oracle.Simulation(w.position, w.velocity, w.species, np.ones(4000), w.length, pot, 0.3, 0.005)
+    L, half = w.length, w.skin / 2
+    band = 5e-6            # FP32 rounding of the hi-word displacement
+    disp = w.neighbour.displacement
+    w.particle.data["force"]                 # lists built at the input positions
+    sim.state()
+    r0 = w.particle.ordered("position")
+    prev, sched, osched = w.neighbour.rebuilds, [], []
+    oprev = sim.rebuilds()
+    assert prev == oprev == 1
+    for step in range(1, 301):
+        w.sampler.run(1)
+        r = w.particle.ordered("position")   # (reading evaluates the owed force: the list check)
+        dr = r - r0
+        dr -= np.rint(dr / L) * L
+        m = np.sqrt(np.max(np.sum(dr ** 2, axis=1)))
+        now = w.neighbour.rebuilds
+        if now > prev:
+            assert now == prev + 1
+            assert m > half or abs(m - half) < band, (step, m)
+            assert disp.compute() == 0.0
+            r0 = r
+            sched.append(step)
+        else:
+            assert m <= half or abs(m - half) < band, (step, m)
+            assert abs(disp.compute() - m) < band, (step, disp.compute(), m)
+        prev = now
+        if step <= 100:
+            sim.run(1)
+            if sim.rebuilds() > oprev:
+                osched.append(step)
+            oprev = sim.rebuilds()
+    print(f"rebuild steps: {sched}")
+    assert len(sched) >= 10
+    assert [s for s in sched if s <= 100] == osched
+
+
+def test_device_rebuild_leaves_displacement_current():
+    # all-pairs lists rebuilt on the device reset r0 there: the displacement
+    # module reports the displacement from the reference positions now in
+    # memory (0 right after a rebuild, below skin / 2 otherwise)
+    w = workloads.kob_andersen(2048, density=1.2, temperature=1.0, timestep=0.002, seed=18, binning=False)
+    assert w.neighbour.algorithm == "from_particle"
+    disp = w.neighbour.displacement
+    w.particle.data["force"]
+    prev, hits = w.neighbour.rebuilds, 0
+    for _ in range(200):
+        w.sampler.run(1)
+        w.particle.data["force"]            # evaluates the owed force: the device decides
+        now = w.neighbour.rebuilds
+        d = disp.compute()
+        if now > prev:
+            assert d == 0.0
+            hits += 1
+        else:
+            assert d <= w.skin / 2
+        prev = now
+    assert hits >= 3
+
+
 def test_speculative_force_is_bitwise_identical():
     # the force launched before the displacement check comes back (and
     # skipped on the device when the lists are stale) gives the trajectory of
@@ -548,7 +619,7 @@ def test_nve_energy_conservation_and_drift(oracle):
     oslope = np.polyfit(orec[late, 0] * 0.005, (orec[:, 1] + orec[:, 2])[late], 1)[0]
     print(f"NVE drift per particle per tau: device {slope:.3e}, FP64 oracle {oslope:.3e}")
     assert abs(slope) < 1e-4
-    assert abs(slope - oslope) < 2e-5
+    assert abs(slope - oslope) < 1e-5
     assert np.max(np.abs(etot[late] - etot[late].mean())) < 2e-3
 
 
@@ -577,21 +648,75 @@ def test_nhc_matches_fp64(oracle):
     assert np.max(np.abs(econs - econs[0])) < 5e-3
 
 
-def test_andersen_thermalises():
+def test_andersen_matches_fp64(oracle):
+    # the heat bath draws from a counter generator of (seed, step, particle id),
+    # which the oracle restates: the same particles collide at the same steps,
+    # and the trajectory (resampled velocities included) is the FP64 one: a
+    # collision set that differed anywhere would leave O(1) velocity differences
+    w = workloads.lennard_jones(4000, density=0.8442, temperature=1.2, seed=34, timestep=0.005, jitter=0.05,
+                                truncation="smooth_r4", integrator="verlet_nvt_andersen", collision_rate=5.0)
+    pot = oracle.Potential(oracle.POT_LJ, "smooth_r4", 1.0, 1.0, 2.5, h=0.005)
+    sim = oracle.Simulation(w.position, w.velocity, w.species, np.ones(4000), w.length, pot, 0.3, 0.005,
+                            integrator=oracle.INT_ANDERSEN, temperature=1.2, rate=5.0, seed=34)
+    steps = 40
+    w.sampler.run(steps)
+    sim.run(steps)
+    ref = sim.state()
+    rid = w.particle.data["reverse_id"]
+    vel = w.particle.data["velocity"][rid]
+    pos = w.particle.data["position"][rid] + w.particle.data["image"][rid] * w.length
+    dv = np.max(np.abs(vel - ref["velocity"]))
+    dr = np.max(np.abs(pos - (ref["position"] + ref["image"] * w.length)))
+    # ~ N steps rate dt = 4000 collisions, all at the oracle's particles and steps
+    print(f"Andersen vs FP64: {sim.collisions()} collisions, max |dv| = {dv:.3e}, max |dr| = {dr:.3e}")
+    assert 3000 < sim.collisions() < 5000
+    assert dv < 1e-5 and dr < 1e-6
+
+
+def test_andersen_ideal_gas_statistics():
+    # test/unit/mdsim/integrators/verlet_nvt_andersen.cpp:101-200 (GPU set-up:
+    # 5000 particles without interactions, box ratios 1 : 2 : 1.01, rho = 0.3,
+    # dt = 0.01, T = 1, collision rate 10, 500 tau sampled every 3 collision
+    # times): centre-of-mass velocity, temperature and specific heat within 4.5
+    # sigma, and the collision count within 4.5 sigma of N steps rate dt
+    import andersen_checks as ac
+    from halmd_amd import observables
     from halmd_amd.mdsim import velocities
-    # heat bath at T = 1 acting on velocities drawn at T = 2
-    w = workloads.lennard_jones(4000, density=0.8442, temperature=1.0, seed=34, timestep=0.005,
-                                integrator="verlet_nvt_andersen", collision_rate=5.0)
-    w.particle.data["velocity"] = velocities.maxwell_boltzmann(4000, 2.0, seed=99)
-    slot = w.sampler.on_sample_thermodynamics(w.thermodynamics, 20)
-    w.sampler.run(3000)
-    rec = w.sampler.thermodynamics_records(slot)
-    temp = 2 * rec[-50:, 2] / 3
-    # single-sample spread of the kinetic temperature is T sqrt(2 / 3N); the mean
-    # of 50 correlated samples stays well inside 4.5 of those
-    sigma = np.sqrt(2.0 / 3 / 4000)
-    print(f"Andersen: mean T = {temp.mean():.4f}")
-    assert abs(temp.mean() - 1.0) < 4.5 * sigma
+    n = 5000
+    L = ac.box_edges(n)
+    box = mdsim.box(length=L)
+    p = mdsim.particle(particles=n)
+    r, _ = positions.fcc(n, L)
+    p.data["position"] = r
+    p.data["velocity"] = velocities.maxwell_boltzmann(n, ac.TEMPERATURE, seed=71)
+    integ = mdsim.integrators.verlet_nvt_andersen(box=box, particle=p, timestep=ac.TIMESTEP,
+                                                  temperature=ac.TEMPERATURE, rate=ac.RATE, seed=72)
+    core = mdsim.core(p)
+    core.set_integrator(integ)
+    thermo = observables.thermodynamics(box=box, particle=p, integrator=integ)
+    sampler = observables.sampler(core)
+    steps = int(np.ceil(500 / ac.TIMESTEP))
+    period = ac.sample_period()
+    temps, vcms = [], []
+    changed = 0
+    for k in range(0, steps, period):
+        if k == 0:
+            # without forces a velocity changes only in a collision: count them
+            # over one step
+            v_before = p.data["velocity"]
+            sampler.run(1)
+            changed = int(np.sum(np.any(p.data["velocity"] != v_before, axis=1)))
+        else:
+            sampler.run(period)
+        temps.append(thermo.temperature())
+        vcms.append(thermo.center_of_mass_velocity())
+    done = 1 + (len(temps) - 1) * period
+    sampler.run(steps - done)
+    ac.check(temps, vcms, thermo.temperature(), thermo.center_of_mass_velocity(), n)
+    # one step's collisions: binomial(N, rate dt)
+    pc = float(np.float32(ac.RATE * ac.TIMESTEP))
+    print(f"collisions in one step: {changed} (expected {n * pc:.0f})")
+    assert abs(changed - n * pc) < 4.5 * np.sqrt(n * pc * (1 - pc))
 
 
 # ----------------------------------------------------------------------------

@@ -121,3 +121,19 @@ def test_admission_waits_for_memory():
     ia, ib = s.info(a), s.info(b)
     assert ib["admitted"] >= ia["done"]
     assert s.stats()["used_bytes"] == 0
+
+
+def test_invalid_job_is_refused_and_the_queue_goes_on():
+    # a Kob-Andersen job of a few hundred particles at rho = 1.2 fits fewer than
+    # 3 cells of rc + skin, which the batched layout holds for one species only:
+    # submit() refuses it, and the valid jobs before and after it run to the end
+    s = server(tick_steps=TICK)
+    a = _submit(s, dict(SPEC_LJ, steps=150), keep_state=False)
+    bad = dict(n=300, density=1.2, temperature=0.8, potential="ka", steps=100, seed=507)
+    with pytest.raises(Exception, match="3 cells"):
+        _submit(s, bad, keep_state=False)
+    b = _submit(s, dict(SPEC_KA, steps=150), keep_state=False)
+    _run_until_done(s, [a, b])
+    assert s.info(a)["status"] == s.info(b)["status"] == "done"
+    assert s.error(a) == ""
+    assert s.stats()["used_bytes"] == 0

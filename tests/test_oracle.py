@@ -492,6 +492,56 @@ def test_nhc_conserved_energy(oracle):
     assert abs(temp.mean() - 1.0) < 0.1
 
 
+@pytest.mark.parametrize("temp,duration", [(1.0, 500), (2.5, 100)])
+def test_andersen_ideal_gas_statistics(oracle, temp, duration):
+    # test/unit/mdsim/integrators/verlet_nvt_andersen.cpp:101-200 (host set-up:
+    # 1500 particles, box ratios 1 : 2 : 1.01, rho = 0.3, dt = 0.01, T = 1,
+    # collision rate 10, a run of 500 tau sampled every 3 collision times);
+    # a second, shorter run at T = 2.5 pins the sqrt(T) scale of the draws
+    import andersen_checks as ac
+    n = 1500
+    L = ac.box_edges(n)
+    rng = np.random.default_rng(61)
+    pos = rng.uniform(-L / 2, L / 2, (n, 3))
+    vel = rng.normal(0, np.sqrt(temp), (n, 3))
+    vel -= vel.mean(0)
+    pot = oracle.Potential(oracle.POT_LJ, "sharp", 0.0, 1.0, 1.0)   # no interactions
+    sim = oracle.Simulation(pos, vel, np.zeros(n, np.int32), np.ones(n), L, pot, 1.0, ac.TIMESTEP,
+                            integrator=oracle.INT_ANDERSEN, temperature=temp, rate=ac.RATE, seed=5)
+    steps = int(np.ceil(duration / ac.TIMESTEP))
+    period = ac.sample_period()
+    temps, vcms = [], []
+    for k in range(0, steps, period):
+        # samples at steps 0, period, 2 period, ... taken after the step (the
+        # reference samples after finalize when i % period == 0)
+        sim.run(1 if k == 0 else period)
+        v = sim.state()["velocity"]
+        temps.append(np.mean(np.sum(v ** 2, axis=1)) / 3)
+        vcms.append(v.mean(0))
+    done = 1 + (len(temps) - 1) * period
+    sim.run(steps - done)
+    v = sim.state()["velocity"]
+    ac.check(temps, vcms, np.mean(np.sum(v ** 2, axis=1)) / 3, v.mean(0), n, collisions=sim.collisions(),
+             particle_steps=n * steps, temp=temp)
+
+
+def test_andersen_without_collisions_is_verlet(oracle):
+    # rate 0: no particle ever collides, and the trajectory is velocity Verlet's
+    n = 500
+    pos, L, species, rng = _liquid(oracle, n, 0.8442, 10, jitter=0.05)
+    vel = rng.normal(0, 1.0, (n, 3))
+    vel -= vel.mean(0)
+    pot = oracle.Potential(oracle.POT_LJ, "shifted", 1.0, 1.0, 2.5)
+    a = oracle.Simulation(pos, vel, species, np.ones(n), L, pot, 0.3, 0.002, integrator=oracle.INT_ANDERSEN,
+                          temperature=1.0, rate=0.0, seed=3)
+    b = oracle.Simulation(pos, vel, species, np.ones(n), L, pot, 0.3, 0.002)
+    a.run(50)
+    b.run(50)
+    assert a.collisions() == 0
+    assert np.array_equal(a.state()["velocity"], b.state()["velocity"])
+    assert np.array_equal(a.state()["position"], b.state()["position"])
+
+
 # ----------------------------------------------------------------------------
 # observables
 # ----------------------------------------------------------------------------
