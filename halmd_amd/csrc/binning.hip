@@ -143,7 +143,7 @@ template <bool ISO>
 __global__ __launch_bounds__(256) void build_neighbours_kernel(
     uint4 const* __restrict__ posq, unsigned n, uint32_t const* __restrict__ offset,
     uint4 const* __restrict__ cell_pos, neighbour_params p, unsigned cap, uint32_t* __restrict__ list,
-    uint32_t* __restrict__ count, uint32_t* __restrict__ max_count)
+    uint32_t* __restrict__ count, uint32_t* __restrict__ max_count, bool self)
 {
     __shared__ float s_rr[MAX_TYPES * MAX_TYPES];
     __shared__ uint32_t s_buf[256 * NB_STAGE];
@@ -156,7 +156,9 @@ __global__ __launch_bounds__(256) void build_neighbours_kernel(
     s.c = 0;
     if (i < n) {
         s.ri = posq[i];
-        s.i = i;
+        // the records of a second particle instance (cell_pos) are never
+        // particle i itself: no index matches
+        s.i = self ? i : 0xffffffffu;
         s.ti = (s.ri.w & 3u) * p.ntype;
         s.cap = cap;
         s.out = list + size_t(i & ~(WAVE - 1)) * cap + (i & (WAVE - 1)) * NB_BLOCK;
@@ -651,14 +653,14 @@ void launch_gather_cell_pos(uint4 const* posq, uint32_t const* vals, unsigned n,
 
 void launch_build_neighbours(uint4 const* posq, unsigned n, uint32_t const* offset, uint4 const* cell_pos,
                              neighbour_params const& p, unsigned cap, uint32_t* list, uint32_t* count,
-                             uint32_t* max_count, hipStream_t s) {
+                             uint32_t* max_count, hipStream_t s, bool self) {
     if (!n) return;
     if (p.iso)
         build_neighbours_kernel<true><<<div_up(n, 256), 256, 0, s>>>(posq, n, offset, cell_pos, p, cap, list, count,
-                                                                     max_count);
+                                                                     max_count, self);
     else
         build_neighbours_kernel<false><<<div_up(n, 256), 256, 0, s>>>(posq, n, offset, cell_pos, p, cap, list, count,
-                                                                      max_count);
+                                                                      max_count, self);
     HIP_CHECK(hipGetLastError());
 }
 

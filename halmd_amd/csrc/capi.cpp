@@ -1,6 +1,7 @@
 // extern "C" boundary (include/halmd_hip.h) over the C++ module layer.
 #include "../../include/halmd_hip.h"
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -346,6 +347,30 @@ HALMD_API int halmd_mdsim_neighbours_from_binning_create(halmd_particle* p, halm
                                                      skin, occupancy));
     });
 }
+HALMD_API int halmd_mdsim_neighbours_from_binning_create2(halmd_particle* p1, halmd_particle* p2,
+                                                          halmd_binning* bin1, halmd_binning* bin2,
+                                                          halmd_max_displacement* disp1,
+                                                          halmd_max_displacement* disp2, const halmd_box* b,
+                                                          const double* r_cut, double skin, double occupancy,
+                                                          halmd_neighbour** out) {
+    return guard([&] {
+        nonnull(r_cut, "r_cut"); nonnull(out, "out");
+        auto* q1 = as<particle>(p1);
+        auto* q2 = as<particle>(p2);
+        if (q1 == q2) {
+            require(bin1 == bin2 && disp1 == disp2, "from_binning: one instance takes one binning and displacement");
+            auto* bb = as<binning>(bin1);
+            auto* dd = as<max_displacement>(disp1);
+            *out = handle<halmd_neighbour>(new neighbour(*q1, bb, *dd, *as<box>(b), r_cut, q1->nspecies,
+                                                         q1->nspecies, skin, occupancy));
+            return;
+        }
+        *out = handle<halmd_neighbour>(new neighbour(*q1, *q2, as<binning>(bin1), as<binning>(bin2),
+                                                     *as<max_displacement>(disp1), *as<max_displacement>(disp2),
+                                                     *as<box>(b), r_cut, q1->nspecies, q2->nspecies, skin,
+                                                     occupancy));
+    });
+}
 HALMD_API int halmd_mdsim_neighbours_from_particle_create(halmd_particle* p, halmd_max_displacement* disp,
                                                           const halmd_box* b, const double* r_cut, double skin,
                                                           double occupancy, halmd_neighbour** out) {
@@ -431,6 +456,22 @@ HALMD_API int halmd_mdsim_forces_pair_trunc_create(const halmd_potential* pot, h
         auto* n = as<neighbour>(nb);
         require(&n->p == pp, "pair_trunc: neighbour list of another particle instance");
         *out = handle<halmd_force>(new pair_trunc(*as<potential>(pot), *pp, *as<box>(b), *n, w));
+    });
+}
+HALMD_API int halmd_mdsim_forces_pair_trunc_create2(const halmd_potential* pot, halmd_particle* p1,
+                                                    halmd_particle* p2, const halmd_box* b, halmd_neighbour* nb,
+                                                    double w, halmd_force** out) {
+    return guard([&] {
+        nonnull(out, "out");
+        auto* q1 = as<particle>(p1);
+        auto* q2 = as<particle>(p2);
+        auto* n = as<neighbour>(nb);
+        if (q1 == q2) {
+            require(&n->p == q1 && !n->two, "pair_trunc: neighbour list of another particle instance");
+            *out = handle<halmd_force>(new pair_trunc(*as<potential>(pot), *q1, *as<box>(b), *n, w));
+        } else {
+            *out = handle<halmd_force>(new pair_trunc(*as<potential>(pot), *q1, *q2, *as<box>(b), *n, w));
+        }
     });
 }
 HALMD_API int halmd_mdsim_forces_pair_trunc_tune(halmd_force* f, unsigned* cells) {
@@ -573,6 +614,17 @@ HALMD_API int halmd_mdsim_core_create(halmd_particle* p, halmd_core** out) {
 HALMD_API int halmd_mdsim_core_destroy(halmd_core* c) { return guard([&] { delete reinterpret_cast<core*>(c); }); }
 HALMD_API int halmd_mdsim_core_set_integrator(halmd_core* c, halmd_integrator* i) {
     return guard([&] { auto* cc = as<core>(c); cc->integ = as<integrator>(i); cc->wired = false; });
+}
+HALMD_API int halmd_mdsim_core_add_integrator(halmd_core* c, halmd_integrator* i) {
+    return guard([&] {
+        auto* cc = as<core>(c);
+        auto* g = as<integrator>(i);
+        require(cc->integ != nullptr, "core: set the first integrator first");
+        require(g != cc->integ && std::find(cc->more.begin(), cc->more.end(), g) == cc->more.end(),
+                "core: integrator already connected");
+        cc->more.push_back(g);
+        cc->wired = false;
+    });
 }
 HALMD_API int halmd_mdsim_core_set_sort(halmd_core* c, halmd_sort* s, unsigned interval) {
     return guard([&] {

@@ -139,9 +139,9 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
 
 template <int KIND, int TRUNC, bool AUX>
 __global__ __launch_bounds__(256) void pair_force_kernel(
-    uint4 const* __restrict__ posq, unsigned n, uint32_t const* __restrict__ list, uint32_t const* __restrict__ count,
-    unsigned cap, force_params P, bool force_zero, float aux_weight, float4* __restrict__ force,
-    float* __restrict__ stress, unsigned stride)
+    uint4 const* __restrict__ posq, uint4 const* __restrict__ posq2, unsigned n, uint32_t const* __restrict__ list,
+    uint32_t const* __restrict__ count, unsigned cap, force_params P, bool force_zero, float aux_weight,
+    float4* __restrict__ force, float* __restrict__ stress, unsigned stride)
 {
     __shared__ pair_param s_pair[MAX_TYPES * MAX_TYPES];
     if (threadIdx.x < MAX_TYPES * MAX_TYPES) s_pair[threadIdx.x] = P.pair[threadIdx.x];
@@ -202,8 +202,8 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
     for (unsigned h = 0; h < full; ++h) {
         uint4 const* const g = group(h);
         uint4 const a = g[0], c = g[1];
-        uint4 const q0 = posq[a.x], q1 = posq[a.y], q2 = posq[a.z], q3 = posq[a.w];
-        uint4 const q4 = posq[c.x], q5 = posq[c.y], q6 = posq[c.z], q7 = posq[c.w];
+        uint4 const q0 = posq2[a.x], q1 = posq2[a.y], q2 = posq2[a.z], q3 = posq2[a.w];
+        uint4 const q4 = posq2[c.x], q5 = posq2[c.y], q6 = posq2[c.z], q7 = posq2[c.w];
         pair(q0); pair(q1); pair(q2); pair(q3);
         pair(q4); pair(q5); pair(q6); pair(q7);
         fold();
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
         uint4 const* const g = group(full);
         uint4 const a = g[0], c = g[1];
         unsigned const js[8] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w};
-        for (unsigned k = 0; k < rest; ++k) pair(posq[js[k]]);
+        for (unsigned k = 0; k < rest; ++k) pair(posq2[js[k]]);
         fold();
     }
     float4 f = make_float4(fx - cx, fy - cy, fz - cz, AUX ? aux_weight * en : 0.0f);
@@ -247,9 +247,9 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
 // F.posq_out and the wave's largest squared displacement to F.disp_write.
 template <int KIND, int TRUNC, bool AUX, unsigned T, bool FUSE>
 __global__ __launch_bounds__(256) void pair_force_split_kernel(
-    uint4 const* __restrict__ posq, unsigned n, uint32_t const* __restrict__ list, uint32_t const* __restrict__ count,
-    unsigned cap, force_params P, bool force_zero, float aux_weight, float4* __restrict__ force,
-    float* __restrict__ stress, unsigned stride, fused_integrate F)
+    uint4 const* __restrict__ posq, uint4 const* __restrict__ posq2, unsigned n, uint32_t const* __restrict__ list,
+    uint32_t const* __restrict__ count, unsigned cap, force_params P, bool force_zero, float aux_weight,
+    float4* __restrict__ force, float* __restrict__ stress, unsigned stride, fused_integrate F)
 {
     __shared__ pair_param s_pair[MAX_TYPES * MAX_TYPES];
     if (threadIdx.x < MAX_TYPES * MAX_TYPES) s_pair[threadIdx.x] = P.pair[threadIdx.x];
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(256) void pair_force_split_kernel(
     float fx = 0, fy = 0, fz = 0;
     float en = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
     for (unsigned k = u; k < cnt; k += T) {
-        uint4 const qj = posq[lp[size_t(k / NB_BLOCK) * (NB_BLOCK * WAVE) + k % NB_BLOCK]];
+        uint4 const qj = posq2[lp[size_t(k / NB_BLOCK) * (NB_BLOCK * WAVE) + k % NB_BLOCK]];
         int32_t const Dx = int32_t(qi.x - qj.x), Dy = int32_t(qi.y - qj.y), Dz = int32_t(qi.z - qj.z);
         float const dx = separation(Dx, P.q[0], P.q_lo[0]);
         float const dy = separation(Dy, P.q[1], P.q_lo[1]);
@@ -696,6 +696,7 @@ constexpr unsigned PAIR_SPLIT32_BELOW = 8192, PAIR_SPLIT16_BELOW = 16384, PAIR_S
 
 struct force_launch {
     uint4 const* posq;
+    uint4 const* posq2;   // neighbour positions (a second particle instance, or posq)
     unsigned n;
     uint32_t const* list;
     uint32_t const* count;
@@ -716,13 +717,13 @@ void dispatch_split(force_launch const& a) {
     fused_integrate const F = a.fuse ? *a.fuse : fused_integrate{};
     if (a.fuse)
         pair_force_split_kernel<KIND, TRUNC, false, T, true><<<grid, 256, 0, a.s>>>(
-            a.posq, a.n, a.list, a.count, a.cap, a.p, a.force_zero, a.w, a.force, a.stress, a.stride, F);
+            a.posq, a.posq2, a.n, a.list, a.count, a.cap, a.p, a.force_zero, a.w, a.force, a.stress, a.stride, F);
     else if (a.aux)
         pair_force_split_kernel<KIND, TRUNC, true, T, false><<<grid, 256, 0, a.s>>>(
-            a.posq, a.n, a.list, a.count, a.cap, a.p, a.force_zero, a.w, a.force, a.stress, a.stride, F);
+            a.posq, a.posq2, a.n, a.list, a.count, a.cap, a.p, a.force_zero, a.w, a.force, a.stress, a.stride, F);
     else
         pair_force_split_kernel<KIND, TRUNC, false, T, false><<<grid, 256, 0, a.s>>>(
-            a.posq, a.n, a.list, a.count, a.cap, a.p, a.force_zero, a.w, a.force, a.stress, a.stride, F);
+            a.posq, a.posq2, a.n, a.list, a.count, a.cap, a.p, a.force_zero, a.w, a.force, a.stress, a.stride, F);
 }
 
 template <int KIND, int TRUNC>
@@ -734,11 +735,13 @@ void dispatch_aux(force_launch const& a) {
     if (a.fuse) throw error(HALMD_ERR_INVALID_ARGUMENT, "pair force: no fused step for this many particles");
     unsigned const grid = div_up(a.n, 256);
     if (a.aux)
-        pair_force_kernel<KIND, TRUNC, true><<<grid, 256, 0, a.s>>>(a.posq, a.n, a.list, a.count, a.cap, a.p,
-                                                                   a.force_zero, a.w, a.force, a.stress, a.stride);
+        pair_force_kernel<KIND, TRUNC, true><<<grid, 256, 0, a.s>>>(a.posq, a.posq2, a.n, a.list, a.count, a.cap,
+                                                                   a.p, a.force_zero, a.w, a.force, a.stress,
+                                                                   a.stride);
     else
-        pair_force_kernel<KIND, TRUNC, false><<<grid, 256, 0, a.s>>>(a.posq, a.n, a.list, a.count, a.cap, a.p,
-                                                                    a.force_zero, a.w, a.force, a.stress, a.stride);
+        pair_force_kernel<KIND, TRUNC, false><<<grid, 256, 0, a.s>>>(a.posq, a.posq2, a.n, a.list, a.count, a.cap,
+                                                                    a.p, a.force_zero, a.w, a.force, a.stress,
+                                                                    a.stride);
 }
 
 template <int KIND>
@@ -978,10 +981,14 @@ bool pair_force_fusable(unsigned n) { return n < PAIR_SPLIT4_BELOW; }
 
 void launch_pair_force(uint4 const* posq, unsigned n, uint32_t const* list, uint32_t const* count, unsigned cap,
                        force_params const& p, bool aux, bool force_zero, float aux_weight, float4* force,
-                       float* stress, unsigned stride, hipStream_t s, fused_integrate const* fuse) {
+                       float* stress, unsigned stride, hipStream_t s, fused_integrate const* fuse,
+                       uint4 const* posq2) {
     if (!n) return;
     if (fuse && aux) throw error(HALMD_ERR_INVALID_ARGUMENT, "pair force: the fused step has no aux path");
-    force_launch const a{posq, n, list, count, cap, p, aux, force_zero, aux_weight, force, stress, stride, s, fuse};
+    if (fuse && posq2 && posq2 != posq)
+        throw error(HALMD_ERR_INVALID_ARGUMENT, "pair force: the fused step needs one particle instance");
+    force_launch const a{posq, posq2 ? posq2 : posq, n, list, count, cap, p, aux, force_zero, aux_weight, force,
+                         stress, stride, s, fuse};
     if (p.kind == POTENTIAL_LENNARD_JONES)
         dispatch_trunc<POTENTIAL_LENNARD_JONES>(a);
     else if (p.kind == POTENTIAL_MORSE)

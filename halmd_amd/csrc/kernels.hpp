@@ -91,9 +91,11 @@ __host__ __device__ inline size_t list_index(size_t i, size_t k, size_t cap) {
     return (i & ~size_t(WAVE - 1)) * cap + (k / NB_BLOCK) * WAVE * NB_BLOCK + (i & (WAVE - 1)) * NB_BLOCK +
            k % NB_BLOCK;
 }
+// self = false: cell_pos and offset are the binning of a second particle
+// instance (no entry is skipped as particle i itself)
 void launch_build_neighbours(uint4 const* posq, unsigned n, uint32_t const* offset, uint4 const* cell_pos,
                              neighbour_params const& p, unsigned cap, uint32_t* list, uint32_t* count,
-                             uint32_t* max_count, hipStream_t s);
+                             uint32_t* max_count, hipStream_t s, bool self = true);
 
 // A rebuild decided on the device: rebuild only if the largest of the
 // displacement words at slots (DISP_SLOTS words, DISP_STRIDE apart) exceeds
@@ -243,9 +245,12 @@ struct force_params {
 // (no aux; only for pair_force_fusable(n) particles)
 struct fused_integrate;
 bool pair_force_fusable(unsigned n);
+// posq2: the positions the list indices refer to (a second particle instance;
+// null: posq)
 void launch_pair_force(uint4 const* posq, unsigned n, uint32_t const* list, uint32_t const* count, unsigned cap,
                        force_params const& p, bool aux, bool force_zero, float aux_weight, float4* force,
-                       float* stress, unsigned stride, hipStream_t s, fused_integrate const* fuse = nullptr);
+                       float* stress, unsigned stride, hipStream_t s, fused_integrate const* fuse = nullptr,
+                       uint4 const* posq2 = nullptr);
 
 // untruncated forces over all pairs of two particle instances (same: one
 // instance, j != i); p as from potential::device_params for pair_full (sharp,

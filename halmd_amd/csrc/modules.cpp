@@ -298,9 +298,19 @@ void particle::get_reverse_id(unsigned* v) {
 // force protocol of halmd/mdsim/gpu/particle.cpp:221-253: the first force
 // module overwrites, later ones accumulate; aux variables are computed only
 // when requested
+bool particle::coupled() const {
+    for (force_module const* f : forces)
+        if (f->couples()) return true;
+    return false;
+}
+
 void particle::update_force(bool need_aux) {
-    for (force_module* f : forces) {
-        if (f->inputs_changed()) { force_valid = aux_valid = false; break; }
+    // (a force held for the step's integrate stays valid while the other
+    // instances move)
+    if (!hold_force) {
+        for (force_module* f : forces) {
+            if (f->inputs_changed()) { force_valid = aux_valid = false; break; }
+        }
     }
     if (force_valid && (!need_aux || aux_valid)) return;
     // a pending kick refers to the plain force of the current positions:
@@ -339,7 +349,17 @@ void particle::evaluate_forces() {
     // the forces; a single force module may launch before its neighbour check
     // is back
     if (!speculative_force || forces.size() != 1 || !forces[0]->apply_speculative()) {
-        for (force_module* f : forces) f->prepare();
+        // the modules that rearrange the particles prepare first; lists of the
+        // others follow the order they leave (a pass repeats if it changed)
+        for (int pass = 0;; ++pass) {
+            require(pass < 4, "particle: the neighbour modules keep rearranging the particles");
+            uint64_t const before = order_version;
+            for (force_module* f : forces)
+                if (f->owns_order()) f->prepare();
+            for (force_module* f : forces)
+                if (!f->owns_order()) f->prepare();
+            if (order_version == before) break;
+        }
         bool first = true;
         for (force_module* f : forces) {
             f->apply(first);
@@ -757,12 +777,35 @@ constexpr size_t FULL_CAP_BYTES = size_t(256) << 20;
 
 neighbour::neighbour(particle& p_, binning* bin_, max_displacement& disp_, box const& b_, double const* rc,
                      unsigned nt1, unsigned nt2, double skin, double occupancy_)
-    : p(p_), bin(bin_), disp(disp_), b(b_), r_skin(skin), occupancy(occupancy_)
+    : p(p_), p2(p_), bin(bin_), bin2(bin_), disp(disp_), b(b_), r_skin(skin), occupancy(occupancy_)
 {
+    require(&disp.p == &p, "neighbour: the displacement module belongs to another particle instance");
+    if (bin) require(&bin->p == &p, "neighbour: the binning module belongs to another particle instance");
+    init(rc, nt1, nt2);
+}
+
+neighbour::neighbour(particle& p1, particle& p2_, binning* bin1, binning* bin2_, max_displacement& disp1,
+                     max_displacement& disp2_, box const& b_, double const* rc, unsigned nt1, unsigned nt2,
+                     double skin, double occupancy_)
+    : p(p1), p2(p2_), bin(bin1), bin2(bin2_), disp(disp1), disp2(&disp2_), b(b_), r_skin(skin),
+      occupancy(occupancy_)
+{
+    require(&p1 != &p2_, "neighbour: two instances expected (use the one-instance form)");
+    require(bin1 && bin2_, "neighbour: lists between two particle instances need a binning of each");
+    require(&bin1->p == &p1 && &bin2_->p == &p2_,
+            "neighbour: 'particle' instances of binning modules do not match with 'particle' argument");
+    require(&disp1.p == &p1 && &disp2_.p == &p2_,
+            "neighbour: 'particle' instances of displacement modules do not match with 'particle' argument");
+    two = true;
+    HIP_CHECK(hipEventCreateWithFlags(&p2_ready_, hipEventDisableTiming));
+    init(rc, nt1, nt2);
+}
+
+void neighbour::init(double const* rc, unsigned nt1, unsigned nt2) {
     {
         // keep_bits: the largest float m2 with sqrt(double(m2)) <= skin / 2,
         // the rule of require_current, found by stepping across the boundary
-        double const half = skin / 2;
+        double const half = r_skin / 2;
         auto keeps = [half](float m2) { return std::sqrt(double(m2)) <= half; };
         float f = float(half * half);
         while (keeps(f) && f < std::numeric_limits<float>::max()) f = std::nextafter(f, HUGE_VALF);
@@ -770,15 +813,19 @@ neighbour::neighbour(particle& p_, binning* bin_, max_displacement& disp_, box c
         std::memcpy(&keep_bits, &f, 4);
     }
     p.attach_box(b);
-    require(nt1 == nt2 && nt1 == p.nspecies, "neighbour: cutoff matrix must be nspecies x nspecies");
+    p2.attach_box(b);
+    require(nt1 == p.nspecies && nt2 == p2.nspecies, "neighbour: cutoff matrix must be nspecies1 x nspecies2");
+    require(two || nt1 == nt2, "neighbour: cutoff matrix must be nspecies x nspecies");
     require(occupancy > 0 && occupancy <= 1, "neighbour: occupancy must be in (0, 1]");
-    if (bin)
+    // the cells searched are those of the listed instance's binning
+    // (from_binning.cpp:169-173: at least 3 per dimension)
+    if (bin2)
         for (int k = 0; k < 3; ++k)
-            require(bin->grid.ncell[k] >= 3,
+            require(bin2->grid.ncell[k] >= 3,
                     "neighbour: the binning needs at least 3 cells per dimension (box too small for rc + skin)");
     r_cut.assign(rc, rc + size_t(nt1) * nt2);
     std::memset(&params, 0, sizeof(params));
-    if (bin) params.grid = bin->grid;
+    if (bin2) params.grid = bin2->grid;
     for (int k = 0; k < 3; ++k) params.q[k] = b.q_hi[k];
     params.ntype = nt2;
     double rc_max = 0;
@@ -794,14 +841,15 @@ neighbour::neighbour(particle& p_, binning* bin_, max_displacement& disp_, box c
     }
     // initial capacity from the mean number of neighbours within rc + skin at
     // the given occupancy (from_binning.cpp:96-119), rounded to 8; grown on demand
-    double const rho = double(p.n) / b.volume();
+    // density of the listed instance (from_binning.cpp:104)
+    double const rho = double(p2.n) / b.volume();
     double const nmean = 4.0 / 3.0 * M_PI * std::pow(rc_max + r_skin, 3) * rho;
     count.resize(p.n);
     d_maxcount.resize(2);
     h_maxcount.resize(2);
     // the first neighbour module of a particle instance keeps it in cell order
     // and uses the cell-segment kernels; the rules need nx >= 3 (S <= nx - 2)
-    staged = bin && p.order_owner == nullptr;
+    staged = bin && !two && p.order_owner == nullptr;
     if (staged) {
         p.order_owner = this;
         cap = unsigned(round_up(size_t(std::ceil(nmean / occupancy)) + NB16_BLOCK, NB16_BLOCK));
@@ -834,11 +882,14 @@ neighbour::neighbour(particle& p_, binning* bin_, max_displacement& disp_, box c
         list.resize(size_t(round_up(p.n, WAVE)) * cap);
     }
     ++disp.users;
+    if (disp2) ++disp2->users;
 }
 
 neighbour::~neighbour() {
     if (p.order_owner == this) p.order_owner = nullptr;
     --disp.users;
+    if (disp2) --disp2->users;
+    if (p2_ready_) (void)hipEventDestroy(p2_ready_);
 }
 
 bool neighbour::device_decides() const {
@@ -885,6 +936,22 @@ void neighbour::plan_segments(unsigned S, unsigned R) {
 }
 
 void neighbour::require_current() {
+    if (two) {
+        // from_binning.cpp:130-139: either instance reordered, or moved more
+        // than skin / 2 since the lists were built
+        if (built && built_order == p.order_version && built_order2 == p2.order_version &&
+            disp.compute() <= r_skin / 2 && disp2->compute() <= r_skin / 2)
+            return;
+        for (auto& f : on_prepend_update) f();
+        update();
+        disp.zero();
+        disp2->zero();
+        built_order = p.order_version;
+        built_order2 = p2.order_version;
+        built = true;
+        for (auto& f : on_append_update) f();
+        return;
+    }
     if (built && built_order == p.order_version && device_checked_version == p.position_version) return;
     if (device_decides()) {
         rebuild_condition c;
@@ -914,13 +981,19 @@ void neighbour::require_current() {
 void neighbour::update() {
     hipStream_t s = p.stream;
     if (!staged) {
-        if (bin) bin->update(true);
+        if (bin2) bin2->update(true);
+        if (two) {
+            // the cell records of p2 (its stream) before the build (this one);
+            // the build finishes before update() returns (the count read below)
+            HIP_CHECK(hipEventRecord(p2_ready_, p2.stream));
+            HIP_CHECK(hipStreamWaitEvent(s, p2_ready_, 0));
+        }
         while (true) {
             launch_fill_u32(d_maxcount, 0u, 1, s);
             timer.start(s);
-            if (bin)
-                launch_build_neighbours(p.posq, p.n, bin->offset, bin->cell_pos, params, cap, list, count, d_maxcount,
-                                        s);
+            if (bin2)
+                launch_build_neighbours(p.posq, p.n, bin2->offset, bin2->cell_pos, params, cap, list, count,
+                                        d_maxcount, s, !two);
             else
                 launch_build_neighbours_all_pairs(p.posq, p.n, params, cap, list, count, d_maxcount, s);
             timer.stop(s);
@@ -1024,11 +1097,29 @@ void neighbour::get_lists(unsigned* counts, unsigned* entries, unsigned stride) 
 // pair_trunc (halmd/mdsim/gpu/forces/pair_trunc.hpp:152-337)
 // ---------------------------------------------------------------------------
 pair_trunc::pair_trunc(potential const& pot_, particle& p_, box const& b_, neighbour& nb_, double w)
-    : pot(pot_), p(p_), b(b_), nb(nb_), aux_weight(w)
+    : pot(pot_), p(p_), p2(p_), b(b_), nb(nb_), aux_weight(w)
 {
     p.attach_box(b);
     require(pot.ntype1 == p.nspecies && pot.ntype2 == p.nspecies, "pair_trunc: potential shape != nspecies");
+    require(&nb.p == &p && !nb.two, "pair_trunc: the neighbour lists belong to other particle instances");
     params = pot.device_params(b);
+    p.forces.push_back(this);
+    p.force_valid = p.aux_valid = false;
+}
+
+pair_trunc::pair_trunc(potential const& pot_, particle& p1_, particle& p2_, box const& b_, neighbour& nb_, double w)
+    : pot(pot_), p(p1_), p2(p2_), b(b_), nb(nb_), aux_weight(w)
+{
+    require(&p1_ != &p2_, "pair_trunc: two instances expected (use the one-instance form)");
+    p.attach_box(b);
+    p2.attach_box(b);
+    // pair_trunc.hpp:169-171 (the device tables are nspecies1 x nspecies2)
+    require(pot.ntype1 == p.nspecies && pot.ntype2 == p2.nspecies,
+            "pair_trunc: potential shape != nspecies of the particle instances");
+    require(&nb.p == &p1_ && &nb.p2 == &p2_ && nb.two,
+            "pair_trunc: the neighbour lists belong to other particle instances");
+    params = pot.device_params(b);
+    HIP_CHECK(hipEventCreateWithFlags(&p2_ready_, hipEventDisableTiming));
     p.forces.push_back(this);
     p.force_valid = p.aux_valid = false;
 }
@@ -1037,7 +1128,10 @@ pair_trunc::~pair_trunc() {
     auto& v = p.forces;
     v.erase(std::remove(v.begin(), v.end(), static_cast<force_module*>(this)), v.end());
     p.force_valid = p.aux_valid = false;
+    if (p2_ready_) (void)hipEventDestroy(p2_ready_);
 }
+
+bool pair_trunc::inputs_changed() const { return &p != &p2 && p2.position_version != p2_version_; }
 
 void pair_trunc::prepare() { nb.require_current(); }
 
@@ -1140,19 +1234,26 @@ bool pair_trunc::launch(bool force_zero, uint32_t const* abort_slots, uint32_t a
                         fused_integrate const* fuse) {
     bool const aux = p.aux_enabled && !fuse;   // the fused step evaluates the plain force
     // a particle interacting with itself counts each pair twice (pair_trunc.hpp:291-294)
-    float const w = float(aux_weight / 2);
+    bool const same = &p == &p2;
+    float const w = float(same ? aux_weight / 2 : aux_weight);
+    if (!same) {
+        // the positions of the second instance as its own stream leaves them
+        HIP_CHECK(hipEventRecord(p2_ready_, p2.stream));
+        HIP_CHECK(hipStreamWaitEvent(p.stream, p2_ready_, 0));
+        p2_version_ = p2.position_version;
+    }
     timer.start(p.stream);
     if (nb.staged)
         launch_pair_force_staged(p.posq, nb.bin->offset, nb.sp_force, nb.list16, nb.count, nb.cap, params, aux,
                                  force_zero, w, p.force, p.stress, p.n_pad, p.stream, abort_slots, abort_bits, fuse);
     else
         launch_pair_force(p.posq, p.n, nb.list, nb.count, nb.cap, params, aux, force_zero, w, p.force, p.stress,
-                          p.n_pad, p.stream, fuse);
+                          p.n_pad, p.stream, fuse, p2.posq);
     return timer.stop(p.stream);
 }
 
 bool pair_trunc::integrate_fused(verlet& v) {
-    if (v.fused_disp != &nb.disp) return false;
+    if (nb.two || v.fused_disp != &nb.disp) return false;
     if (!nb.staged && !pair_force_fusable(p.n)) return false;
     max_displacement& d = nb.disp;
     // (global-index lists: the check, on the host or on the device, comes first)
@@ -1256,7 +1357,9 @@ void verlet::integrate_kernel(double const* scale) {
 void verlet::integrate() { integrate_kernel(nullptr); }
 
 void verlet::finalize() {
-    if (defer_finalize) {
+    // (a coupled instance's force depends on the others' positions: its kick
+    // must use them now, before they move again)
+    if (defer_finalize && !p.coupled()) {
         // owed with the force of the current positions, evaluated when first
         // needed (by the next step's force kernel, or a reader)
         p.pending_half_dt = ip.half_dt;
@@ -1407,18 +1510,42 @@ core::core(particle& p_) : p(p_) {}
 
 void core::mdstep() {
     require(integ != nullptr, "core: no integrator");
+    std::vector<integrator*> all{integ};
+    all.insert(all.end(), more.begin(), more.end());
     if (!wired) {
         // let the integrate kernel track the displacement of the neighbour
         // list that serves this particle's pair force
-        for (force_module* f : p.forces)
-            if (auto* pt = dynamic_cast<pair_trunc*>(f))
-                if (&pt->nb.disp.p == &p) integ->fused_disp = &pt->nb.disp;
+        for (integrator* g : all) {
+            auto* v = dynamic_cast<verlet*>(g);
+            if (!v) continue;
+            for (force_module* f : v->p.forces)
+                if (auto* pt = dynamic_cast<pair_trunc*>(f))
+                    if (&pt->nb.disp.p == &v->p && !pt->nb.two) g->fused_disp = &pt->nb.disp;
+        }
         wired = true;
     }
-    integ->integrate();
+    if (all.size() == 1) {
+        integ->integrate();
+    } else {
+        // several instances: every force of time t first, then every
+        // integrate (on_integrate, core.cpp:42), each with its own force of
+        // time t while the others move
+        std::vector<particle*> ps;
+        for (integrator* g : all)
+            if (auto* v = dynamic_cast<verlet*>(g)) ps.push_back(&v->p);
+        for (particle* q : ps) q->update_force();
+        for (particle* q : ps) q->hold_force = true;
+        try {
+            for (integrator* g : all) g->integrate();
+        } catch (...) {
+            for (particle* q : ps) q->hold_force = false;
+            throw;
+        }
+        for (particle* q : ps) q->hold_force = false;
+    }
     ++step;
     if (sort && sort_interval && step % sort_interval == 0) sort->order();
-    integ->finalize();
+    for (integrator* g : all) g->finalize();
 }
 
 // ---------------------------------------------------------------------------

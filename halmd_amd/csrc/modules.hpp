@@ -151,6 +151,13 @@ public:
     // list rebuild included).
     bool force_owed = false, owed_aux = false;
     void observe() { if (force_owed) update_force(); }
+    // a force module depends on the positions of another particle instance
+    bool coupled() const;
+    // set by a core that steps several coupled instances: the force of the
+    // step's start stays valid while the other instances move (each instance
+    // kicks with the force at time t before any position of time t + dt is
+    // used)
+    bool hold_force = false;
     // reorder by a device permutation perm[new] = old
     void rearrange(uint32_t const* perm);
     void sync();
@@ -266,10 +273,22 @@ public:
     // boxes with fewer than 3 cells of rc + skin in some dimension
     neighbour(particle& p, binning* bin, max_displacement& disp, box const& b, double const* r_cut, unsigned ntype1,
               unsigned ntype2, double skin, double occupancy);
+    // lists of the particles of p1 with neighbours from a second instance p2
+    // (from_binning.cpp:328-339 with particle1 != particle2): global indices
+    // into p2, built from p2's binning bin2; rebuilt when either instance
+    // moved more than skin / 2 or changed its order
+    neighbour(particle& p1, particle& p2, binning* bin1, binning* bin2, max_displacement& disp1,
+              max_displacement& disp2, box const& b, double const* r_cut, unsigned ntype1, unsigned ntype2,
+              double skin, double occupancy);
     ~neighbour();
     particle& p;
+    particle& p2;                     // the instance the list entries index (p for one instance)
     binning* bin;
+    binning* bin2;                    // binning of p2 (bin for one instance)
     max_displacement& disp;
+    max_displacement* disp2 = nullptr;   // displacement of p2 (two instances)
+    bool two = false;
+    uint64_t built_order2 = 0;
     box const& b;
     double r_skin, occupancy;
     std::vector<double> r_cut;
@@ -313,6 +332,9 @@ public:
     void update();
     void get_lists(unsigned* counts, unsigned* entries, unsigned stride);
     kernel_timer timer;
+private:
+    void init(double const* r_cut, unsigned ntype1, unsigned ntype2);
+    hipEvent_t p2_ready_ = nullptr;
 };
 
 class force_module {
@@ -331,6 +353,10 @@ public:
     // changed since the last apply (pair_full: the positions of its second
     // instance)
     virtual bool inputs_changed() const { return false; }
+    // reads the positions of another particle instance
+    virtual bool couples() const { return false; }
+    // rearranges its particle when it prepares (the cell-order neighbour lists)
+    virtual bool owns_order() const { return false; }
 };
 
 // untruncated pair force over all pairs (halmd/mdsim/gpu/forces/pair_full.hpp:
@@ -348,6 +374,7 @@ public:
     void prepare() override;
     void apply(bool force_zero) override;
     bool inputs_changed() const override;
+    bool couples() const override { return &p1 != &p2; }
     kernel_timer timer;
 private:
     dbuf<float> partial_;
@@ -358,9 +385,13 @@ private:
 class pair_trunc : public force_module {
 public:
     pair_trunc(potential const& pot, particle& p, box const& b, neighbour& nb, double aux_weight);
+    // the force on p1 from the particles of p2 (pair_trunc.hpp:150-190 with
+    // particle1 != particle2) through two-instance neighbour lists
+    pair_trunc(potential const& pot, particle& p1, particle& p2, box const& b, neighbour& nb, double aux_weight);
     ~pair_trunc() override;
     potential const& pot;
     particle& p;
+    particle& p2;
     box const& b;
     neighbour& nb;
     double aux_weight;
@@ -373,6 +404,9 @@ public:
     // when the lists or displacement do not allow it (nothing done)
     bool apply_speculative() override;
     bool integrate_fused(verlet& v) override;
+    bool inputs_changed() const override;
+    bool couples() const override { return &p != &p2; }
+    bool owns_order() const override { return nb.staged; }
     // pick the cell-segment width and workgroup size of the neighbour lists
     // by timing list builds and force evaluations around the current choice;
     // returns the width kept (0 if the lists are not cell-segment lists)
@@ -381,6 +415,8 @@ public:
 private:
     bool launch(bool force_zero, uint32_t const* abort_slots, uint32_t abort_bits,
                 fused_integrate const* fuse = nullptr);
+    uint64_t p2_version_ = 0;
+    hipEvent_t p2_ready_ = nullptr;
 };
 
 class integrator {
@@ -457,6 +493,10 @@ public:
     explicit core(particle& p);
     particle& p;
     integrator* integ = nullptr;
+    // integrators of further particle instances (core.cpp:30-63 connects every
+    // integrator to on_integrate / on_finalize): all integrate, then all
+    // finalize
+    std::vector<integrator*> more;
     hilbert* sort = nullptr;
     unsigned sort_interval = 0;       // 0: sort only when the neighbour list asks (rebuild hook)
     uint64_t step = 0;

@@ -207,9 +207,16 @@ class neighbour(Handle):
                  algorithm="shared_mem", disable_sorting=False):
         super().__init__()
         if isinstance(particle, (tuple, list)):
+            if len(particle) != 2:
+                raise ValueError("bad argument 'particle'")
             if particle[0] is not particle[1]:
-                raise ValueError("neighbour lists between two particle instances are not supported")
+                self._init_two(particle, box, r_cut, skin, occupancy, binning, displacement)
+                return
             particle = particle[0]
+        if isinstance(binning, (tuple, list)):
+            binning = binning[0]
+        if isinstance(displacement, (tuple, list)):
+            displacement = displacement[0]
         rc = _matrix(r_cut, particle.nspecies)
         if binning is None:
             binning = globals()["binning"](particle, box, rc, skin=skin)
@@ -238,6 +245,37 @@ class neighbour(Handle):
             from .sorts import hilbert
             self.sort = hilbert(particle=particle, box=box)
             self.on_prepend_update(self.sort)
+
+    def _init_two(self, particle, box, r_cut, skin, occupancy, binning, displacement):
+        """Lists of particle[0]'s particles holding indices of particle[1]
+        (neighbour.lua.in:165-245 with two instances): a binning and a
+        displacement module of each (created when not given).  No Hilbert sort
+        is attached: each instance's order belongs to its own cell-order
+        neighbour module, which rearranges it at every rebuild."""
+        p1, p2 = particle
+        rc = np.ascontiguousarray(np.broadcast_to(np.atleast_2d(np.asarray(r_cut, dtype=np.float64)),
+                                                  (p1.nspecies, p2.nspecies)))
+        if binning is None:
+            binning = (globals()["binning"](p1, box, rc, skin=skin), globals()["binning"](p2, box, rc, skin=skin))
+        if binning is False or min(binning[1].ncell) < 3:
+            raise ValueError("neighbour: lists between two particle instances need at least 3 cells of "
+                             "rc + skin per dimension")
+        if displacement is None:
+            displacement = (max_displacement(p1, box), max_displacement(p2, box))
+        if binning[0].particle is not p1 or binning[1].particle is not p2:
+            raise ValueError("'particle' instances of binning modules do not match with 'particle' argument")
+        self.binning = tuple(binning)
+        self.displacement = tuple(displacement)
+        self.algorithm = "from_binning"
+        self.particle = p1
+        self.particles = (p1, p2)
+        self._deps = (p1, p2, box, self.binning, self.displacement)
+        self._h = out_handle()
+        call("halmd_mdsim_neighbours_from_binning_create2", p1.handle, p2.handle, binning[0].handle,
+             binning[1].handle, displacement[0].handle, displacement[1].handle, box.handle, dptr(rc), float(skin),
+             float(occupancy), ctypes.byref(self._h))
+        self.r_skin = float(skin)
+        self.sort = None
 
     def on_prepend_update(self, sort):
         call("halmd_mdsim_neighbours_from_binning_on_prepend_update_sort", self.handle, sort.handle)
@@ -308,6 +346,13 @@ class core(Handle):
     def set_integrator(self, integrator):
         call("halmd_mdsim_core_set_integrator", self.handle, integrator.handle)
         self.integrator = integrator
+        self.integrators = [integrator]
+
+    def add_integrator(self, integrator):
+        """Integrator of a further particle instance: a step integrates every
+        instance, then finalizes every instance (core.cpp:30-63)."""
+        call("halmd_mdsim_core_add_integrator", self.handle, integrator.handle)
+        self.integrators.append(integrator)
 
     def set_sort(self, sort, interval):
         call("halmd_mdsim_core_set_sort", self.handle, sort.handle if sort else None, int(interval))

@@ -19,22 +19,30 @@ class pair_trunc(Handle):
 
     def __init__(self, box, particle, potential, neighbour=None, weight=1.0):
         super().__init__()
+        pair = None
         if isinstance(particle, (tuple, list)):
+            if len(particle) != 2:
+                raise ValueError("bad argument 'particle'")
             if particle[0] is not particle[1]:
-                raise ValueError("pair forces between two particle instances are not supported")
+                pair = tuple(particle)
             particle = particle[0]
         if getattr(potential, "r_cut", None) is None:
             raise ValueError("pair_trunc: the potential must be truncated first")
         if neighbour is None or isinstance(neighbour, dict):
             kw = dict(neighbour or {})
-            neighbour = _neighbour(particle=particle, box=box, r_cut=potential.r_cut, **kw)
+            neighbour = _neighbour(particle=pair or particle, box=box, r_cut=potential.r_cut, **kw)
         self.neighbour = neighbour
         self.potential = potential
         self.particle = particle
-        self._deps = (box,)
+        self._deps = (box, pair)
         self._h = out_handle()
-        call("halmd_mdsim_forces_pair_trunc_create", potential.handle, particle.handle, box.handle,
-             neighbour.handle, float(weight), ctypes.byref(self._h))
+        if pair:
+            # the force on particle[0] from the particles of particle[1]
+            call("halmd_mdsim_forces_pair_trunc_create2", potential.handle, pair[0].handle, pair[1].handle,
+                 box.handle, neighbour.handle, float(weight), ctypes.byref(self._h))
+        else:
+            call("halmd_mdsim_forces_pair_trunc_create", potential.handle, particle.handle, box.handle,
+                 neighbour.handle, float(weight), ctypes.byref(self._h))
 
     def apply(self):
         call("halmd_mdsim_forces_pair_trunc_apply", self.handle)

@@ -194,6 +194,19 @@ HALMD_API int halmd_mdsim_neighbours_from_binning_create(halmd_particle* p, halm
                                                          halmd_max_displacement* displacement, const halmd_box* box,
                                                          const double* r_cut, double skin, double occupancy,
                                                          halmd_neighbour** out);
+/* the full form of from_binning.cpp:328-339 (lua/halmd/mdsim/neighbour.lua.in:
+ * 165-245, particle = {p1, p2}, binning = {b1, b2}, displacement = {d1, d2}):
+ * lists of the particles of p1 holding particle indices of p2 within
+ * r_cut(species1, species2) + skin, searched in the cells of b2 (at least 3
+ * per dimension); r_cut is nspecies1 x nspecies2.  Rebuilt when p1 or p2
+ * changed order or moved more than skin / 2 (from_binning.cpp:130-139).  With
+ * p1 == p2 (b1 == b2, d1 == d2) it is halmd_mdsim_neighbours_from_binning_create. */
+HALMD_API int halmd_mdsim_neighbours_from_binning_create2(halmd_particle* p1, halmd_particle* p2,
+                                                          halmd_binning* bin1, halmd_binning* bin2,
+                                                          halmd_max_displacement* disp1,
+                                                          halmd_max_displacement* disp2, const halmd_box* box,
+                                                          const double* r_cut, double skin, double occupancy,
+                                                          halmd_neighbour** out);
 HALMD_API int halmd_mdsim_neighbours_from_binning_destroy(halmd_neighbour* nb);
 /* ---- mdsim.neighbours.from_particle: halmd/mdsim/gpu/neighbours/from_particle.cpp:45-200
  *      from_particle(particle, particle, displacement, box, r_cut, skin, occupancy): all
@@ -236,6 +249,15 @@ HALMD_API int halmd_mdsim_neighbours_from_binning_on_prepend_update_sort(halmd_n
 HALMD_API int halmd_mdsim_forces_pair_trunc_create(const halmd_potential* pot, halmd_particle* p,
                                                    const halmd_box* box, halmd_neighbour* nb, double aux_weight,
                                                    halmd_force** out);
+/* pair_trunc(potential, particle1, particle2, box, neighbour, weight) with
+ * particle1 != particle2 (pair_trunc.hpp:150-190, 362-369): the force on p1
+ * from the particles of p2 through lists made by
+ * halmd_mdsim_neighbours_from_binning_create2(p1, p2, ...); the potential is
+ * nspecies1 x nspecies2, the stress and energy carry the full weight (each
+ * pair once per instance).  p1 == p2 is halmd_mdsim_forces_pair_trunc_create. */
+HALMD_API int halmd_mdsim_forces_pair_trunc_create2(const halmd_potential* pot, halmd_particle* p1,
+                                                    halmd_particle* p2, const halmd_box* box, halmd_neighbour* nb,
+                                                    double aux_weight, halmd_force** out);
 HALMD_API int halmd_mdsim_forces_pair_trunc_destroy(halmd_force* f);
 /* autotuner: time list builds and force evaluations for cell-segment widths
  * around the current one and keep the fastest (cells per segment in *cells;
@@ -299,6 +321,11 @@ HALMD_API int halmd_mdsim_sorts_hilbert_depth(const halmd_sort* s, unsigned* dep
 HALMD_API int halmd_mdsim_core_create(halmd_particle* p, halmd_core** out);
 HALMD_API int halmd_mdsim_core_destroy(halmd_core* c);
 HALMD_API int halmd_mdsim_core_set_integrator(halmd_core* c, halmd_integrator* i);
+/* connects the integrator of a further particle instance (every integrator
+ * connects to core.on_integrate / on_finalize, core.cpp:30-63): an MD step
+ * integrates all instances, each with its force of the step's start, then
+ * finalizes all with the forces of the new positions of every instance */
+HALMD_API int halmd_mdsim_core_add_integrator(halmd_core* c, halmd_integrator* i);
 /* sort every `interval` steps (0: only when connected to a neighbour rebuild) */
 HALMD_API int halmd_mdsim_core_set_sort(halmd_core* c, halmd_sort* s, unsigned interval);
 HALMD_API int halmd_mdsim_core_mdstep(halmd_core* c);
