@@ -641,6 +641,7 @@ binning::binning(particle& p_, box const& b_, double const* r_cut, unsigned nt1,
     keys.resize(p.n);
     vals.resize(p.n);
     offset.resize(size_t(ncells) + 1);
+    order_offset.resize(size_t(ncells) + 1);
     cell_pos.resize(p.n);
     ws.reserve(p.n);
 }
@@ -669,6 +670,7 @@ void binning::reorder_particles() {
     timer.start(s);
     p.rearrange(vals);
     launch_iota(vals, p.n, s);        // sorted order is now memory order
+    HIP_CHECK(hipMemcpyAsync(order_offset.get(), offset.get(), order_offset.bytes(), hipMemcpyDeviceToDevice, s));
     timer.stop(s);
     // keys and offsets describe the new order as well
     version = p.position_version;
@@ -1013,7 +1015,7 @@ void neighbour::update() {
     while (true) {
         launch_fill_u32(d_maxcount, 0u, 2, s);
         timer.start(s);
-        launch_build_neighbours_staged(p.posq, bin->offset, sp, params, cap, list16, count, d_maxcount, s);
+        launch_build_neighbours_staged(p.posq, bin->order_offset, sp, params, cap, list16, count, d_maxcount, s);
         timer.stop(s);
         HIP_CHECK(hipMemcpyAsync(h_maxcount.get(), d_maxcount.get(), 8, hipMemcpyDeviceToHost, s));
         p.sync();
@@ -1066,7 +1068,7 @@ void neighbour::get_lists(unsigned* counts, unsigned* entries, unsigned stride) 
     std::vector<uint16_t> h(size_t(round_up(p.n, WAVE)) * cap);
     std::vector<uint32_t> off(size_t(bin->ncells) + 1);
     list16.download(h.data(), h.size(), s);
-    bin->offset.download(off.data(), off.size(), s);
+    bin->order_offset.download(off.data(), off.size(), s);
     p.sync();
     std::vector<uint32_t> slot_to_index;
     for (unsigned seg = 0; seg < sp.nseg; ++seg) {
@@ -1244,7 +1246,7 @@ bool pair_trunc::launch(bool force_zero, uint32_t const* abort_slots, uint32_t a
     }
     timer.start(p.stream);
     if (nb.staged)
-        launch_pair_force_staged(p.posq, nb.bin->offset, nb.sp_force, nb.list16, nb.count, nb.cap, params, aux,
+        launch_pair_force_staged(p.posq, nb.bin->order_offset, nb.sp_force, nb.list16, nb.count, nb.cap, params, aux,
                                  force_zero, w, p.force, p.stress, p.n_pad, p.stream, abort_slots, abort_bits, fuse);
     else
         launch_pair_force(p.posq, p.n, nb.list, nb.count, nb.cap, params, aux, force_zero, w, p.force, p.stress,

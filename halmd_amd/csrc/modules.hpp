@@ -209,6 +209,11 @@ public:
     cell_grid grid;
     unsigned ncells;
     dbuf<uint32_t> keys, vals, offset;   // offset: ncells + 1 entries
+    // the offsets of the memory order left by the last reorder_particles():
+    // cell c holds particles order_offset[c] .. order_offset[c + 1] - 1 (the
+    // cell-ordered lists read these; update() for later positions, by another
+    // module sharing this binning, leaves them)
+    dbuf<uint32_t> order_offset;
     dbuf<uint4> cell_pos;
     radix_sort_workspace ws;
     uint64_t version = 0;             // position version of the last update

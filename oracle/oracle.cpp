@@ -746,13 +746,25 @@ long oracle_neighbour_half(int n, double const* pos, int const* species, double 
     return total;
 }
 
-// Host pair force (half lists, Newton 3) with aux variables.
+void oracle_pair_force_brute(int n, double const* pos, int const* species, double const* L, void const* p,
+                             double* force, double* en_pot, double* stress);
+
+// Host pair force (half lists, Newton 3) with aux variables.  A box with
+// fewer than 3 cells of rc + skin along some axis would visit a neighbour cell
+// twice through the periodic wrap (the reference's host lists assume 3 or more
+// cells); such boxes take the brute-force sum over all pairs instead.
 void oracle_pair_force(int n, double const* pos, int const* species, double const* L, void const* p,
                        double skin, double aux_weight, double* force, double* en_pot, double* stress)
 {
     auto const& pot = *static_cast<potential_t const*>(p);
     box_t box(L);
     binning_t b(box, pot.r_cut_max(), skin);
+    if (std::min({b.ncell[0], b.ncell[1], b.ncell[2]}) < 3) {
+        oracle_pair_force_brute(n, pos, species, L, p, force, en_pot, stress);
+        for (size_t i = 0; i < size_t(n); ++i) en_pot[i] *= aux_weight;
+        for (size_t i = 0; i < 6 * size_t(n); ++i) stress[i] *= aux_weight;
+        return;
+    }
     std::vector<double> ps(pos, pos + 3 * size_t(n));
     std::vector<int> sp(species, species + n);
     b.update(ps, n);
@@ -791,6 +803,52 @@ void oracle_pair_force_brute(int n, double const* pos, int const* species, doubl
             for (int k = 0; k < 6; ++k) stress[6 * i + k] += 0.5 * fval * s[k];
         }
     }
+}
+
+// Pairs whose squared distance lies in a relative band around the cutoff,
+// |rr - rc_ab^2| <= band * rc_ab^2.  The device evaluates the cutoff test in
+// FP32 on the fixed-point separation, so rounding may put such a pair on the
+// other side of the cutoff than FP64 does; a truncation with a force jump at
+// rc (shifted, sharp) then changes the force by that pair's |F|.  jump[i]
+// (zeroed here) receives, for each band pair of i, the magnitude of the pair
+// force at its FP64 distance.  Returns the number of band pairs.
+long oracle_cutoff_band(int n, double const* pos, int const* species, double const* L, void const* p, double band,
+                        double* jump)
+{
+    auto const& pot = *static_cast<potential_t const*>(p);
+    box_t box(L);
+    std::fill(jump, jump + n, 0.0);
+    long count = 0;
+    auto visit = [&](unsigned i, unsigned j) {
+        double r[3];
+        for (int k = 0; k < 3; ++k) r[k] = pos[3 * i + k] - pos[3 * j + k];
+        box.reduce_periodic(r);
+        double const rr = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        int const a = species[i], b = species[j];
+        double const rc2 = pot.rr_cut[a * pot.ntype + b];
+        if (std::abs(rr - rc2) > band * rc2) return;
+        double fval, u;
+        pot.eval(rr, a, b, fval, u);
+        double const f = std::abs(fval) * std::sqrt(rr);
+        jump[i] += f;
+        jump[j] += f;
+        ++count;
+    };
+    double const skin = pot.r_cut_max() * (std::sqrt(1 + band) - 1) + 1e-9;
+    binning_t b(box, pot.r_cut_max(), skin);
+    if (std::min({b.ncell[0], b.ncell[1], b.ncell[2]}) < 3) {
+        for (int i = 0; i < n; ++i)
+            for (int j = i + 1; j < n; ++j) visit(unsigned(i), unsigned(j));
+        return count;
+    }
+    std::vector<double> ps(pos, pos + 3 * size_t(n));
+    std::vector<int> sp(species, species + n);
+    b.update(ps, n);
+    neighbour_t nb(pot, skin);
+    nb.update(b, box, ps, sp, n);
+    for (int i = 0; i < n; ++i)
+        for (unsigned j : nb.list[i]) visit(unsigned(i), j);
+    return count;
 }
 
 // ===========================================================================

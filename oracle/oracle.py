@@ -51,6 +51,7 @@ def lib():
             "oracle_neighbour_half": (ctypes.c_long, [ctypes.c_int, _d, _i, _d, _vp, ctypes.c_double, ctypes.c_long, _l, _u]),
             "oracle_pair_force": (None, [ctypes.c_int, _d, _i, _d, _vp, ctypes.c_double, ctypes.c_double, _d, _d, _d]),
             "oracle_pair_force_brute": (None, [ctypes.c_int, _d, _i, _d, _vp, _d, _d, _d]),
+            "oracle_cutoff_band": (ctypes.c_long, [ctypes.c_int, _d, _i, _d, _vp, ctypes.c_double, _d]),
             "oracle_fixed_point": (None, [ctypes.c_int, _d, _d, _u]),
             "oracle_fixed_cell_index": (None, [ctypes.c_int, _u, _u, _u]),
             "oracle_fixed_neighbour_list": (ctypes.c_int, [ctypes.c_int, _u, _u, _u, _f, ctypes.c_int, _f, ctypes.c_int, _u, _u]),
@@ -202,6 +203,17 @@ def pair_force_brute(pos, species, L, pot):
     s = np.zeros((n, 6))
     lib().oracle_pair_force_brute(n, _p(pos, _d), _p(sp, _i), _p(L, _d), pot.h, _p(f, _d), _p(u, _d), _p(s, _d))
     return f, u, s
+
+
+def cutoff_band(pos, species, L, pot, band=1e-6):
+    """(number of pairs, per-particle sum of |F_pair|) over the pairs whose
+    squared distance is within a relative `band` of the squared cutoff."""
+    pos = _dd(pos)
+    sp = np.ascontiguousarray(species, dtype=np.int32)
+    L = _dd(L)
+    jump = np.zeros(len(pos))
+    cnt = lib().oracle_cutoff_band(len(pos), _p(pos, _d), _p(sp, _i), _p(L, _d), pot.h, float(band), _p(jump, _d))
+    return cnt, jump
 
 
 def fixed_point(pos, L):

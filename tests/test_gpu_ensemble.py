@@ -1,6 +1,7 @@
 """Batched ensembles on the GPU: each system's result is the same alone and in the mix."""
 import numpy as np
 import pytest
+from forcecheck import force_errors
 
 from halmd_amd import workloads
 
@@ -64,8 +65,9 @@ def test_each_system_same_alone_and_in_the_mix(oracle):
         # (the all-pairs path of boxes under 3 cells included)
         sp = workloads.spec_species(spec)
         sp = np.zeros(spec["n"], np.int32) if sp is None else sp
-        fref, _, _ = oracle.pair_force(r_e, sp, L, _oracle_potential(oracle, spec), skin=0.3)
-        err = _force_error(e.force(k), fref)
+        pot = _oracle_potential(oracle, spec)
+        fref, _, _ = oracle.pair_force(r_e, sp, L, pot, skin=0.3)
+        err, _, _ = force_errors(e.force(k), fref, r_e, sp, L, pot, oracle, FORCE_TOL)
         print(f"  force vs FP64: {err:.2e}")
         assert err < FORCE_TOL
         if small:

@@ -215,6 +215,49 @@ def test_force_two_particles_closed_form(oracle):
         assert np.isclose(s[0, 0], 0.5 * fval[0] * r * r, rtol=1e-14)
 
 
+def test_force_small_box_is_brute_force(oracle):
+    # a box under 3 cells of rc + skin: the cell lists would visit a cell
+    # twice through the wrap; the oracle sums over all pairs instead
+    rng = np.random.default_rng(7)
+    n = 400
+    L = np.full(3, (n / 1.1) ** (1 / 3))
+    assert L[0] < 3 * 2.8
+    pos = rng.uniform(-L / 2, L / 2, (n, 3))
+    species = (rng.random(n) < 0.2).astype(np.int32)
+    pot = _ka_potential(oracle)
+    f, u, s = oracle.pair_force(pos, species, L, pot, skin=0.3, aux_weight=0.5)
+    fb, ub, sb = oracle.pair_force_brute(pos, species, L, pot)
+    assert np.array_equal(f, fb)
+    assert np.allclose(u, 0.5 * ub, rtol=1e-15) and np.allclose(s, 0.5 * sb, rtol=1e-15)
+
+
+def test_cutoff_band_pairs(oracle):
+    # closed form: two particles at r_c (1 +- 4e-7) are in a 1e-6 band with
+    # |F(r_c)| = |48 r^-13 - 24 r^-7| each; at r_c (1 - 4e-6) they are not
+    L = np.full(3, 20.0)
+    pot = oracle.Potential(oracle.POT_LJ, "shifted", 1.0, 1.0, 2.5)
+    for r, inside in ((2.5 * (1 - 4e-7), True), (2.5 * (1 + 4e-7), True), (2.5 * (1 - 4e-6), False)):
+        cnt, jump = oracle.cutoff_band(np.array([[0.0, 0, 0], [0, 0, r]]), np.zeros(2, np.int32), L, pot, 1e-6)
+        assert cnt == int(inside)
+        f = abs(48 * r ** -13 - 24 * r ** -7) if inside else 0.0
+        assert np.allclose(jump, [f, f], rtol=1e-12, atol=0)
+    # a liquid: the band pairs are those a brute-force sweep finds, with the
+    # per-species cutoffs of a mixture, across the periodic boundary
+    pos, L, species, _ = _liquid(oracle, 1372, 1.2, 2, species_frac=0.2)
+    pot = _ka_potential(oracle)
+    band = 2e-3
+    cnt, jump = oracle.cutoff_band(pos, species, L, pot, band)
+    rc = np.array([[2.5, 2.0], [2.0, 2.2]])
+    d = pos[:, None, :] - pos[None, :, :]
+    d -= L * np.round(d / L)
+    rr = np.sum(d * d, axis=-1)
+    rc2 = rc[species][:, species] ** 2
+    hit = np.triu(np.abs(rr - rc2) <= band * rc2, 1)
+    assert cnt == hit.sum() and cnt > 10
+    deg = hit.sum(axis=0) + hit.sum(axis=1)
+    assert np.array_equal(jump > 0, deg > 0)
+
+
 # ----------------------------------------------------------------------------
 # binning and neighbour lists
 # ----------------------------------------------------------------------------
