@@ -32,13 +32,16 @@ _SCALARS = {
     "int": ctypes.c_int,
     "size_t": ctypes.c_size_t,
     "uint64_t": ctypes.c_uint64,
+    "int64_t": ctypes.c_int64,
 }
 _POINTERS = {
     "double": ctypes.POINTER(ctypes.c_double),
+    "float": ctypes.POINTER(ctypes.c_float),
     "unsigned": ctypes.POINTER(ctypes.c_uint),
     "int": ctypes.POINTER(ctypes.c_int),
     "size_t": ctypes.POINTER(ctypes.c_size_t),
     "uint64_t": ctypes.POINTER(ctypes.c_uint64),
+    "int64_t": ctypes.POINTER(ctypes.c_int64),
     "uint32_t": ctypes.c_void_p,   # device pointers in the algorithm entries
     "char": ctypes.c_char_p,
     "void": ctypes.c_void_p,
@@ -60,15 +63,18 @@ def _ctype(decl):
         return _POINTERS[base]
     if stars == 2 and base == "void":
         return ctypes.POINTER(ctypes.c_void_p)
+    if stars == 2 and base == "char":
+        return ctypes.POINTER(ctypes.c_char_p)
     raise ValueError(f"unsupported C type: {decl}")
 
 
-def parse_header(path=HEADER_PATH):
-    """Return {name: (restype, [argtypes])} for every HALMD_API function."""
+def parse_header(path=HEADER_PATH, prefix=r"HALMD_API\s+"):
+    """Return {name: (restype, [argtypes])} for every function of the header
+    (declarations that start with `prefix`)."""
     text = open(path).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     sigs = {}
-    for m in re.finditer(r"HALMD_API\s+([\w\s\*]+?)\s*\b(halmd_\w+)\s*\(([^)]*)\)\s*;", text):
+    for m in re.finditer(prefix + r"([\w\s\*]+?)\s*\b(halmd_\w+)\s*\(([^)]*)\)\s*;", text):
         ret, name, params = m.group(1).strip(), m.group(2), m.group(3)
         restype = ctypes.c_char_p if "char" in ret else ctypes.c_int
         args = []

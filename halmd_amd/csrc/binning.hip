@@ -62,24 +62,25 @@ __device__ __forceinline__ bool pair_inside(uint4 const ri, uint4 const rj, neig
 }
 
 __global__ void compute_cell_kernel(uint4 const* __restrict__ posq, unsigned n, cell_grid g,
-                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, unsigned sub_bits) {
     unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint4 const r = posq[i];
-    unsigned const x = cell_component(r.x, g.ncell[0]);
+    // (x sub-bin and cell: floor(u nx 2^b / 2^32) = cell 2^b + sub-bin)
+    unsigned const xs = unsigned((uint64_t(r.x) * (g.ncell[0] << sub_bits)) >> 32);
     unsigned const y = cell_component(r.y, g.ncell[1]);
     unsigned const z = cell_component(r.z, g.ncell[2]);
-    keys[i] = (z * g.ncell[1] + y) * g.ncell[0] + x;
+    keys[i] = (((z * g.ncell[1] + y) * g.ncell[0]) << sub_bits) + xs;
     vals[i] = i;
 }
 
-// offset[c] = first k with keys[k] >= c, for c = 0 .. ncells (k = 0 .. n)
-__global__ void cell_offsets_kernel(uint32_t const* __restrict__ keys, unsigned n, unsigned ncells,
-                                    uint32_t* __restrict__ offset) {
+// offset[c] = first k with (keys[k] >> shift) >= c, for c = 0 .. nbins (k = 0 .. n)
+__global__ void cell_offsets_kernel(uint32_t const* __restrict__ keys, unsigned n, unsigned nbins,
+                                    uint32_t* __restrict__ offset, unsigned shift) {
     unsigned const k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k > n) return;
-    int const prev = k == 0 ? -1 : int(keys[k - 1]);
-    int const cur = k == n ? int(ncells) : int(keys[k]);
+    int const prev = k == 0 ? -1 : int(keys[k - 1] >> shift);
+    int const cur = k == n ? int(nbins) : int(keys[k] >> shift);
     for (int c = prev + 1; c <= cur; ++c) offset[c] = k;
 }
 
@@ -354,6 +355,16 @@ __device__ __forceinline__ void test16p(nb16_state& s, uint2 const rj, unsigned 
     if ((!SELF || slot != s.own) && inside) append16(s, slot * 16u);
 }
 
+// the 16-bit predicate of test16p alone
+template <bool MULTI>
+__device__ __forceinline__ bool pass16(nb16_state const& s, uint2 const rj, uint32_t const* s_rr, uint32_t rr0) {
+    ushort2v const dxy = __builtin_bit_cast(ushort2v, s.ri.x) - __builtin_bit_cast(ushort2v, rj.x);
+    int const dz = short(s.ri.y - rj.y);
+    short2v const d = __builtin_bit_cast(short2v, dxy);
+    uint32_t const rq = uint32_t(__builtin_amdgcn_sdot2(d, d, dz * dz, false));
+    return rq <= (MULTI ? s_rr[s.ti + (rj.y >> 16)] : rr0);
+}
+
 // candidates [t, e) of one row, four records in flight
 template <bool MULTI, bool SELF>
 __device__ __forceinline__ void scan16(nb16_state& s, unsigned t, unsigned const e, uint4 const* s_rec,
@@ -389,13 +400,14 @@ template <bool MULTI, bool ISO>
 __device__ __forceinline__ void staged_build_segment(
     uint4 const* __restrict__ posq, uint32_t const* __restrict__ off, seg_params const& sp, unsigned seg,
     neighbour_params const& p, unsigned cap, uint16_t* __restrict__ list, uint32_t* __restrict__ count,
-    uint32_t* __restrict__ stats, uint4* s_lds, seg_stage& st, unsigned* s_bound, uint32_t* s_rr)
+    uint32_t* __restrict__ stats, uint4* s_lds, seg_stage& st, unsigned* s_bound, uint32_t* s_rr,
+    unsigned bstride = SEG_BOUND)
 {
     using rec_t = typename std::conditional<ISO, uint2, uint4>::type;
     rec_t* const s_rec = reinterpret_cast<rec_t*>(s_lds);
     if (threadIdx.x < MAX_TYPES * MAX_TYPES)
         s_rr[threadIdx.x] = ISO ? p.rr16[threadIdx.x] : __float_as_uint(p.rr_cut_skin[threadIdx.x]);
-    unsigned const U = stage_union(sp, seg, off, posq, s_rec, st, s_bound);
+    unsigned const U = stage_union(sp, seg, off, posq, s_rec, st, s_bound, bstride);
     if (threadIdx.x == 0) atomicMax(&stats[1], U + 1);
     if (U + 1 > sp.lds_cap) return;   // uniform: the caller enlarges the staging and rebuilds
 
@@ -413,7 +425,7 @@ __device__ __forceinline__ void staged_build_segment(
         seg_own(st, t, j, i);
         unsigned const self = hh + j + 1;   // union row of the particle's own (y, z) row
         nb16_state s;
-        s.own = s_bound[self * SEG_BOUND + 1] + (t - st.own_pre[j]);
+        s.own = s_bound[self * bstride + 1] + (t - st.own_pre[j]);
         uint4 const full = posq[i];          // = the staged record s.own, in full
         if (ISO) {
             uint2 const pk = pack_rec16(full);
@@ -434,8 +446,8 @@ __device__ __forceinline__ void staged_build_segment(
         for (unsigned dz = 0; dz < 3; ++dz)
             for (unsigned dy = 0; dy < 3; ++dy) {
                 unsigned const ur = dz * hh + j + dy;
-                unsigned const b = s_bound[ur * SEG_BOUND + k];
-                unsigned const e = s_bound[ur * SEG_BOUND + k + 3];
+                unsigned const b = s_bound[ur * bstride + k];
+                unsigned const e = s_bound[ur * bstride + k + 3];
                 if (ur == self)
                     scan16<MULTI, true>(s, b, e, s_rec, p, s_rr, rr0);
                 else
@@ -459,12 +471,190 @@ __global__ __launch_bounds__(SEG_THREADS_MAX) void build_staged_kernel(
 {
     extern __shared__ uint4 s_rec[];
     __shared__ seg_stage st;
-    __shared__ unsigned s_bound[SEG_NR * SEG_BOUND];
     __shared__ uint32_t s_rr[MAX_TYPES * MAX_TYPES];
 
     unsigned const seg = seg_of_block(sp, blockIdx.x);
     if (seg >= sp.nseg) return;   // the whole workgroup
-    staged_build_segment<MULTI, ISO>(posq, off, sp, seg, p, cap, list, count, stats, s_rec, st, s_bound, s_rr);
+    // the cell starts of the union rows follow the records in the dynamic LDS
+    // (3 (R + 2) rows of S + 3 words), sized to the segment shape
+    unsigned* const s_bound = reinterpret_cast<unsigned*>(reinterpret_cast<char*>(s_rec) +
+                                                          size_t(sp.lds_cap) * (ISO ? sizeof(uint2) : sizeof(uint4)));
+    staged_build_segment<MULTI, ISO>(posq, off, sp, seg, p, cap, list, count, stats, s_rec, st, s_bound, s_rr,
+                                     sp.S + 3);
+}
+
+// --- cell segments with x sub-bins: per-particle candidate windows ---------
+//
+// The particle arrays are in (cell, sub-bin) order: each cell is cut along x
+// into SUB_BINS slabs, sub-bin = (u_x * nx * SUB_BINS) >> 32, and a stable
+// sort puts a cell's particles slab by slab.  A union row is then sorted by
+// sub-bin, and the records of any x interval form one contiguous range of it.
+// For each of its 9 rows (dz, dy) a lane takes the smallest |dy|, |dz| (in
+// 16-bit units) that a particle of that row can have, the largest |dx| that
+// can still pass the 16-bit predicate, and the sub-bins that interval covers:
+// its window, a range of the row's 3-cell scan that holds every candidate that
+// can pass.  The lists are those of staged_build_segment entry for entry (same
+// predicate, same ascending slots per row), over about half the candidates.
+// The own row's window is split at the particle itself.  The windows (at most
+// WIN_MAX per lane) go to LDS, and the lane walks them as one sequence, four
+// candidates per step, so that every lane of a wave is busy until the wave's
+// longest sequence ends (rows are no longer the unit of divergence).
+//
+// Dynamic LDS: lds_cap + 3 packed records (reads run up to 3 records past a
+// window's end), then the sub-bin starts of the union rows (3 (R + 2) rows of
+// (S + 2) SUB_BINS + 1 words), then WIN_MAX words per thread.
+__host__ __device__ inline unsigned windowed_sub_stride(unsigned S) { return (S + 2) * SUB_BINS + 1; }
+
+template <bool MULTI>
+__device__ __forceinline__ void windowed_build_segment(
+    uint4 const* __restrict__ posq, uint32_t const* __restrict__ off, uint32_t const* __restrict__ soff,
+    seg_params const& sp, unsigned seg, neighbour_params const& p, unsigned cap, uint16_t* __restrict__ list,
+    uint32_t* __restrict__ count, uint32_t* __restrict__ stats, uint4* s_lds, seg_stage& st, unsigned* s_bound,
+    uint32_t* s_rr, unsigned* s_yz)
+{
+    uint2* const s_rec = reinterpret_cast<uint2*>(s_lds);
+    if (threadIdx.x < MAX_TYPES * MAX_TYPES) s_rr[threadIdx.x] = p.rr16[threadIdx.x];
+    unsigned const U = stage_union(sp, seg, off, posq, s_rec, st, s_bound);
+    if (threadIdx.x == 0) atomicMax(&stats[1], U + 1);
+    if (U + 1 > sp.lds_cap) return;   // uniform: the caller enlarges the staging and rebuilds
+
+    seg_place const q = st.q;
+    unsigned const nx = sp.grid.ncell[0], ny = sp.grid.ncell[1], nz = sp.grid.ncell[2];
+    unsigned const hh = q.h + 2, nr = st.nr;
+    unsigned const sstride = windowed_sub_stride(sp.S);
+    unsigned* const s_sub = reinterpret_cast<unsigned*>(s_rec + sp.lds_cap + 3);
+    uint32_t* const s_win = s_sub + 3 * (sp.R + 2) * sstride;
+    // LDS start of sub-bin b of relative cell c (cells x0 - 1 .. x0 + w) of
+    // each union row: the cell's start plus the records of its sub-bins below b
+    unsigned const wsub = (q.w + 2) * SUB_BINS;
+    for (unsigned e = threadIdx.x; e < nr * (wsub + 1); e += blockDim.x) {
+        unsigned const ur = e / (wsub + 1), rs = e % (wsub + 1), c = rs / SUB_BINS;
+        unsigned v = s_bound[ur * SEG_BOUND + c];
+        if (c < q.w + 2) {
+            unsigned const sb = (seg_row_start(sp, q, ur) + (q.x0 + nx - 1 + c) % nx) * SUB_BINS;
+            v += soff[sb + rs % SUB_BINS] - soff[sb];
+        }
+        s_sub[ur * sstride + rs] = v;
+    }
+    // 16-bit lower boundaries floor(c 2^16 / n) of the segment's rows y0 .. y0
+    // + h (h + 1 values) and of its plane z and the next; a particle of cell c
+    // has u >> 16 in [floor(c 2^16 / n), floor((c + 1) 2^16 / n)]
+    if (threadIdx.x <= q.h) s_yz[threadIdx.x] = ((q.y0 + threadIdx.x) << 16) / ny;
+    if (threadIdx.x == q.h + 1) s_yz[q.h + 1] = (q.z << 16) / nz;
+    if (threadIdx.x == q.h + 2) s_yz[q.h + 2] = ((q.z + 1) << 16) / nz;
+    __syncthreads();
+
+    uint32_t const rr0 = p.rr16[0];
+    unsigned const T = st.own_pre[q.h];
+    unsigned const nxs = nx * SUB_BINS;
+    unsigned const i0 = st.own_begin[0] & ~(WAVE - 1);
+    char* const wlist = reinterpret_cast<char*>(list + size_t(i0) * cap);
+    unsigned const zlo = s_yz[q.h + 1], zhi = s_yz[q.h + 2];
+    unsigned cmax = 0;
+    for (unsigned t = threadIdx.x; t < T; t += blockDim.x) {
+        unsigned j, i;
+        seg_own(st, t, j, i);
+        unsigned const self = hh + j + 1;
+        nb16_state s;
+        s.own = s_bound[self * SEG_BOUND + 1] + (t - st.own_pre[j]);
+        uint4 const full = posq[i];
+        uint2 const pk = pack_rec16(full);
+        s.ri = make_uint4(pk.x, pk.y, 0u, 0u);
+        s.ti = (full.w & 3u) * p.ntype;
+        unsigned const H = 2 * ((i & ~(WAVE - 1)) - i0) * cap;
+        s.q0 = H | NB16_LANE_BITS;
+        s.q = s.q0;
+        s.qmax = s.q0 + (((cap - 1) >> 4) << 11) + (((cap - 1) & 15u) << 1);
+        s.lane32 = (i & (WAVE - 1)) * (2 * NB16_BLOCK);
+        s.list = wlist;
+
+        uint32_t rrmax = rr0;
+        if (MULTI)
+            for (unsigned b = 0; b < p.ntype; ++b) rrmax = max(rrmax, s_rr[s.ti + b]);
+        unsigned const x16 = full.x >> 16, y16 = full.y >> 16, z16 = full.z >> 16;
+        // smallest |d| to the rows below / above and the planes below / above
+        int const dyl = int(y16 - s_yz[j]), dyh = int(s_yz[j + 1] - y16);
+        int const dzl = int(z16 - zlo), dzh = int(zhi - z16);
+        unsigned const sbi = __umulhi(full.x, nxs);                  // own sub-bin (absolute)
+        int const k = int(sbi / SUB_BINS) - int(q.x0);                // own cell - 1, relative to the row
+        int const rsi = (k + 1) * int(SUB_BINS) + int(sbi % SUB_BINS);
+        unsigned nw = 0;
+        for (unsigned dz = 0; dz < 3; ++dz) {
+            int const mz = dz == 0 ? dzl : (dz == 2 ? dzh : 0);
+            for (unsigned dy = 0; dy < 3; ++dy) {
+                int const my = dy == 0 ? dyl : (dy == 2 ? dyh : 0);
+                int const rem = int(rrmax) - my * my - mz * mz;
+                if (rem < 0) continue;   // no particle of this row can pass
+                unsigned const rx = unsigned(__builtin_sqrtf(float(rem))) + 1u;
+                uint32_t const ulo = ((x16 - rx) & 0xffffu) << 16, uhi = (((x16 + rx) & 0xffffu) << 16) | 0xffffu;
+                int dlo = int(sbi) - int(__umulhi(ulo, nxs));
+                int dhi = int(__umulhi(uhi, nxs)) - int(sbi);
+                dlo += dlo < 0 ? int(nxs) : 0;   // the interval wraps across the box edge
+                dhi += dhi < 0 ? int(nxs) : 0;
+                int const lo = max(rsi - dlo, k * int(SUB_BINS));
+                int const hi = min(rsi + dhi, (k + 3) * int(SUB_BINS) - 1);
+                unsigned const ur = dz * hh + j + dy;
+                unsigned const b = s_sub[ur * sstride + lo], e = s_sub[ur * sstride + hi + 1];
+                if (ur == self) {
+                    if (b < s.own) s_win[nw++ * blockDim.x + threadIdx.x] = b | (s.own << 16);
+                    if (s.own + 1 < e) s_win[nw++ * blockDim.x + threadIdx.x] = (s.own + 1) | (e << 16);
+                } else if (b < e) {
+                    s_win[nw++ * blockDim.x + threadIdx.x] = b | (e << 16);
+                }
+            }
+        }
+        // the windows as one sequence, four candidates per step
+        unsigned r = 0, c0 = 0, ce = 0;
+        if (nw) {
+            uint32_t const wv = s_win[threadIdx.x];
+            c0 = wv & 0xffffu;
+            ce = wv >> 16;
+        }
+        while (r < nw) {
+            uint2 const a0 = s_rec[c0], a1 = s_rec[c0 + 1], a2 = s_rec[c0 + 2], a3 = s_rec[c0 + 3];
+            // the four predicates first (no branches between them), then the appends
+            bool const h0 = pass16<MULTI>(s, a0, s_rr, rr0);
+            bool const h1 = pass16<MULTI>(s, a1, s_rr, rr0) && c0 + 1 < ce;
+            bool const h2 = pass16<MULTI>(s, a2, s_rr, rr0) && c0 + 2 < ce;
+            bool const h3 = pass16<MULTI>(s, a3, s_rr, rr0) && c0 + 3 < ce;
+            if (h0) append16(s, c0 * 16u);
+            if (h1) append16(s, c0 * 16u + 16u);
+            if (h2) append16(s, c0 * 16u + 32u);
+            if (h3) append16(s, c0 * 16u + 48u);
+            c0 += 4;
+            if (c0 >= ce) {
+                ++r;
+                uint32_t const wv = s_win[min(r, WIN_MAX - 1) * blockDim.x + threadIdx.x];
+                c0 = wv & 0xffffu;
+                ce = wv >> 16;
+            }
+        }
+        unsigned const c = nb16_count(s);
+        if (c <= cap)
+            while (s.q & 15u) append16(s, U * 16u);
+        count[i] = c;
+        cmax = max(cmax, c);
+    }
+    for (int o = WAVE / 2; o > 0; o >>= 1) cmax = max(cmax, (unsigned)__shfl_xor((int)cmax, o, WAVE));
+    if ((threadIdx.x % WAVE) == 0 && cmax > 0) atomicMax(&stats[0], cmax);
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(SEG_THREADS_MAX) void build_windowed_kernel(
+    uint4 const* __restrict__ posq, uint32_t const* __restrict__ off, uint32_t const* __restrict__ soff,
+    seg_params sp, neighbour_params p, unsigned cap, uint16_t* __restrict__ list, uint32_t* __restrict__ count,
+    uint32_t* __restrict__ stats)
+{
+    extern __shared__ uint4 s_rec[];
+    __shared__ seg_stage st;
+    __shared__ unsigned s_bound[SEG_NR * SEG_BOUND];
+    __shared__ uint32_t s_rr[MAX_TYPES * MAX_TYPES];
+    __shared__ unsigned s_yz[SEG_RMAX + 3];
+
+    unsigned const seg = seg_of_block(sp, blockIdx.x);
+    if (seg >= sp.nseg) return;   // the whole workgroup
+    windowed_build_segment<MULTI>(posq, off, soff, sp, seg, p, cap, list, count, stats, s_rec, st, s_bound, s_rr,
+                                  s_yz);
 }
 
 // --- ensembles: rebuild of a listed subset of systems --------------------
@@ -634,14 +824,16 @@ __global__ __launch_bounds__(256) void ens_build_small_kernel(uint4 const* __res
 
 } // namespace
 
-void launch_compute_cell(uint4 const* posq, unsigned n, cell_grid g, uint32_t* keys, uint32_t* vals, hipStream_t s) {
+void launch_compute_cell(uint4 const* posq, unsigned n, cell_grid g, uint32_t* keys, uint32_t* vals, hipStream_t s,
+                         unsigned sub_bits) {
     if (!n) return;
-    compute_cell_kernel<<<div_up(n, 256), 256, 0, s>>>(posq, n, g, keys, vals);
+    compute_cell_kernel<<<div_up(n, 256), 256, 0, s>>>(posq, n, g, keys, vals, sub_bits);
     HIP_CHECK(hipGetLastError());
 }
 
-void launch_cell_offsets(uint32_t const* keys, unsigned n, unsigned ncells, uint32_t* offset, hipStream_t s) {
-    cell_offsets_kernel<<<div_up(size_t(n) + 1, 256), 256, 0, s>>>(keys, n, ncells, offset);
+void launch_cell_offsets(uint32_t const* keys, unsigned n, unsigned nbins, uint32_t* offset, hipStream_t s,
+                         unsigned shift) {
+    cell_offsets_kernel<<<div_up(size_t(n) + 1, 256), 256, 0, s>>>(keys, n, nbins, offset, shift);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -670,10 +862,24 @@ namespace halmd {
 
 void launch_build_neighbours_staged(uint4 const* posq, uint32_t const* offset, seg_params const& sp,
                                     neighbour_params const& p, unsigned cap, uint16_t* list, uint32_t* count,
-                                    uint32_t* stats, hipStream_t s) {
+                                    uint32_t* stats, hipStream_t s, uint32_t const* sub_offset) {
     if (!sp.nseg) return;
-    size_t const lds = size_t(sp.lds_cap) * (p.iso ? sizeof(uint2) : sizeof(uint4));
     unsigned const grid = SEG_XCD * sp.chunk;
+    if (sub_offset) {
+        if (!p.iso || sp.grid.ncell[0] < 4)
+            throw error(HALMD_ERR_INVALID_ARGUMENT, "windowed list build: cubic box with 4 or more cells along x");
+        size_t const wl = windowed_build_lds(sp);
+        if (p.ntype > 1)
+            build_windowed_kernel<true><<<grid, sp.threads, wl, s>>>(posq, offset, sub_offset, sp, p, cap, list,
+                                                                    count, stats);
+        else
+            build_windowed_kernel<false><<<grid, sp.threads, wl, s>>>(posq, offset, sub_offset, sp, p, cap, list,
+                                                                     count, stats);
+        HIP_CHECK(hipGetLastError());
+        return;
+    }
+    size_t const lds = size_t(sp.lds_cap) * (p.iso ? sizeof(uint2) : sizeof(uint4)) +
+                       size_t(3) * (sp.R + 2) * (sp.S + 3) * sizeof(unsigned);
     if (p.ntype > 1) {
         if (p.iso)
             build_staged_kernel<true, true><<<grid, sp.threads, lds, s>>>(posq, offset, sp, p, cap, list, count, stats);

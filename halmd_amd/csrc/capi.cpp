@@ -187,6 +187,10 @@ PARTICLE_IO(set_position, const double*, set_position)
 PARTICLE_IO(get_position, double*, get_position)
 PARTICLE_IO(set_velocity, const double*, set_velocity)
 PARTICLE_IO(get_velocity, double*, get_velocity)
+PARTICLE_IO(get_position_parts, float*, get_position_parts)
+PARTICLE_IO(set_position_parts, const float*, set_position_parts)
+PARTICLE_IO(get_velocity_parts, float*, get_velocity_parts)
+PARTICLE_IO(set_velocity_parts, const float*, set_velocity_parts)
 PARTICLE_IO(set_species, const unsigned*, set_species)
 PARTICLE_IO(get_species, unsigned*, get_species)
 PARTICLE_IO(set_mass, const double*, set_mass)
@@ -636,6 +640,9 @@ HALMD_API int halmd_mdsim_core_set_sort(halmd_core* c, halmd_sort* s, unsigned i
 HALMD_API int halmd_mdsim_core_mdstep(halmd_core* c) { return guard([&] { as<core>(c)->mdstep(); }); }
 HALMD_API int halmd_mdsim_core_step(const halmd_core* c, uint64_t* step) {
     return guard([&] { *step = as<core>(c)->step; });
+}
+HALMD_API int halmd_mdsim_core_set_step(halmd_core* c, uint64_t step) {
+    return guard([&] { as<core>(c)->step = step; });
 }
 
 // ---- observables ------------------------------------------------------------------------------

@@ -52,10 +52,18 @@ void launch_quantize(float4 const* pos, unsigned n, quant_params const& q, uint4
 struct cell_grid {
     unsigned ncell[3];
 };
-// cell index per component: (ux * ncell) >> 32, linearised x fastest
-void launch_compute_cell(uint4 const* posq, unsigned n, cell_grid g, uint32_t* keys, uint32_t* vals, hipStream_t s);
-// offset[c] = first sorted index with key >= c, c = 0 .. ncells (ncells + 1 entries)
-void launch_cell_offsets(uint32_t const* sorted_keys, unsigned n, unsigned ncells, uint32_t* offset, hipStream_t s);
+// cell index per component: (ux * ncell) >> 32, linearised x fastest; with
+// sub_bits > 0 the key is (cell << sub_bits) | the x sub-bin of the cell,
+// (ux * ncell << sub_bits) >> 32 mod 2^sub_bits
+void launch_compute_cell(uint4 const* posq, unsigned n, cell_grid g, uint32_t* keys, uint32_t* vals, hipStream_t s,
+                         unsigned sub_bits = 0);
+// offset[c] = first sorted index with (key >> shift) >= c, c = 0 .. nbins
+// (nbins + 1 entries)
+void launch_cell_offsets(uint32_t const* sorted_keys, unsigned n, unsigned nbins, uint32_t* offset, hipStream_t s,
+                         unsigned shift = 0);
+// x sub-bins per cell of the particle order that cell-segment lists use
+constexpr unsigned SUB_BITS = 3;
+constexpr unsigned SUB_BINS = 1u << SUB_BITS;
 // cell_pos[k] = fixed-point position of particle vals[k], w = (vals[k] << 2) | species
 void launch_gather_cell_pos(uint4 const* posq, uint32_t const* sorted_vals, unsigned n, uint4* cell_pos, hipStream_t s);
 
@@ -209,9 +217,19 @@ __host__ __device__ inline size_t list16_index(size_t i, size_t k, size_t cap) {
 // stats[0]: largest count (may exceed cap); stats[1]: largest staging size
 // U + 1 (a segment whose staging exceeds sp.lds_cap builds nothing; the caller
 // retries)
+//
+// With sub_offset (the offsets of the (cell, x sub-bin) order the particles
+// are in, ncells SUB_BINS + 1 entries; cubic box, nx >= 4) each lane scans
+// only the candidate windows its position allows (same lists); the dynamic
+// LDS is then windowed_build_lds() bytes.
 void launch_build_neighbours_staged(uint4 const* posq, uint32_t const* offset, seg_params const& sp,
                                     neighbour_params const& p, unsigned cap, uint16_t* list, uint32_t* count,
-                                    uint32_t* stats, hipStream_t s);
+                                    uint32_t* stats, hipStream_t s, uint32_t const* sub_offset = nullptr);
+constexpr unsigned WIN_MAX = 10;   // candidate windows per lane: 9 rows, the own row split
+inline size_t windowed_build_lds(seg_params const& sp) {
+    return (size_t(sp.lds_cap) + 3) * 8 + size_t(3) * (sp.R + 2) * ((sp.S + 2) * SUB_BINS + 1) * 4 +
+           size_t(WIN_MAX) * sp.threads * 4;
+}
 
 // ---------------------------------------------------------------------------
 // pair forces (force.hip)

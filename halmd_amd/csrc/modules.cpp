@@ -176,6 +176,54 @@ void particle::settle_velocity() {
     pending_half_dt = 0;
 }
 
+void particle::download_parts(dbuf<float4> const& b, float* out) {
+    host_tmp_.resize(2 * size_t(n));
+    b.download(host_tmp_.data(), host_tmp_.size(), stream);
+    sync();
+    for (size_t i = 0; i < n; ++i) {
+        float4 const h = host_tmp_[2 * i], l = host_tmp_[2 * i + 1];
+        float* const o = out + 6 * i;
+        o[0] = h.x; o[1] = l.x; o[2] = h.y; o[3] = l.y; o[4] = h.z; o[5] = l.z;
+    }
+}
+
+void particle::upload_parts(dbuf<float4>& b, float const* in) {
+    // (the fourth words, species and mass, stay as they are)
+    host_tmp_.resize(2 * size_t(n));
+    b.download(host_tmp_.data(), host_tmp_.size(), stream);
+    sync();
+    for (size_t i = 0; i < n; ++i) {
+        float4& h = host_tmp_[2 * i];
+        float4& l = host_tmp_[2 * i + 1];
+        float const* const v = in + 6 * i;
+        h.x = v[0]; l.x = v[1]; h.y = v[2]; l.y = v[3]; h.z = v[4]; l.z = v[5];
+        l.w = 0;
+    }
+    b.upload(host_tmp_.data(), host_tmp_.size(), stream);
+    sync();
+}
+
+void particle::get_position_parts(float* parts) {
+    observe();
+    download_parts(pos, parts);
+}
+void particle::set_position_parts(float const* parts) {
+    settle_velocity();
+    upload_parts(pos, parts);
+    requantize();
+    positions_changed();
+}
+void particle::get_velocity_parts(float* parts) {
+    observe();
+    settle_velocity();
+    download_parts(vel, parts);
+}
+void particle::set_velocity_parts(float const* parts) {
+    pending_half_dt = 0;   // overwritten
+    upload_parts(vel, parts);
+    velocities_changed();
+}
+
 void particle::set_position(double const* r) {
     settle_velocity();
     upload_hilo(pos, r, true);
@@ -664,16 +712,32 @@ void binning::update(bool need_cell_pos) {
     }
 }
 
-void binning::reorder_particles() {
-    update(false);
+void binning::reorder_particles(bool sub_bins) {
     hipStream_t s = p.stream;
-    timer.start(s);
-    p.rearrange(vals);
+    if (!sub_bins) {
+        update(false);
+        timer.start(s);
+        p.rearrange(vals);
+    } else {
+        // (cell, x sub-bin) order: a stable sort by the finer key keeps the
+        // cells contiguous and cuts each into SUB_BINS slabs along x
+        if (order_sub_offset.size() != (size_t(ncells) << SUB_BITS) + 1)
+            order_sub_offset.resize((size_t(ncells) << SUB_BITS) + 1);
+        timer.start(s);
+        launch_compute_cell(p.posq, p.n, grid, keys, vals, s, SUB_BITS);
+        unsigned key_bits = SUB_BITS;
+        while ((1ull << (key_bits - SUB_BITS)) < ncells) ++key_bits;
+        radix_sort(keys, vals, p.n, key_bits, ws, s);
+        launch_cell_offsets(keys, p.n, ncells, offset, s, SUB_BITS);
+        p.rearrange(vals);
+        launch_cell_offsets(keys, p.n, ncells << SUB_BITS, order_sub_offset, s, 0);
+    }
     launch_iota(vals, p.n, s);        // sorted order is now memory order
     HIP_CHECK(hipMemcpyAsync(order_offset.get(), offset.get(), order_offset.bytes(), hipMemcpyDeviceToDevice, s));
     timer.stop(s);
     // keys and offsets describe the new order as well
     version = p.position_version;
+    sub_order = sub_bins;
 }
 
 void binning::cell_lists(unsigned* begin_out, unsigned* end_out, unsigned* index_out) {
@@ -863,6 +927,13 @@ void neighbour::init(double const* rc, unsigned nt1, unsigned nt2) {
         size_t const static_lds = SEG_NR * (SEG_MAX + 3) * 4 + 2048;
         require(size_t(lds) > static_lds + 27 * 16, "neighbour: device LDS too small");
         lds_limit = std::min<size_t>((size_t(lds) - static_lds) / sizeof(uint4), SEG_MAX_RECORDS);
+        lds_bytes = size_t(lds) - static_lds;
+        // candidate windows from the (cell, x sub-bin) order: cubic boxes
+        // with 4 or more cells along x, on request (HALMD_LIST_WINDOWS=1;
+        // the same lists, but measured no faster than the plain 3-cell scan
+        // at N = 2^20, with a third sort pass in the binning)
+        char const* ew = std::getenv("HALMD_LIST_WINDOWS");
+        windows = params.iso && bin->grid.ncell[0] >= 4 && ew && std::atoi(ew) != 0;
         // about `target` particles per segment (per 256-thread workgroup) at
         // the mean cell occupancy (pair_trunc::tune refines the width by
         // measurement); HALMD_SEG_TARGET overrides the target
@@ -1010,12 +1081,15 @@ void neighbour::update() {
         ++rebuilds;
         return;
     }
-    // memory order := cell order, then the segment lists
-    bin->reorder_particles();
+    // memory order := cell order (with x sub-bins for the windowed build),
+    // then the segment lists
+    bin->reorder_particles(windows);
     while (true) {
         launch_fill_u32(d_maxcount, 0u, 2, s);
         timer.start(s);
-        launch_build_neighbours_staged(p.posq, bin->order_offset, sp, params, cap, list16, count, d_maxcount, s);
+        bool const win = windows && windowed_build_lds(sp) <= lds_bytes;
+        launch_build_neighbours_staged(p.posq, bin->order_offset, sp, params, cap, list16, count, d_maxcount, s,
+                                       win ? bin->order_sub_offset.get() : nullptr);
         timer.stop(s);
         HIP_CHECK(hipMemcpyAsync(h_maxcount.get(), d_maxcount.get(), 8, hipMemcpyDeviceToHost, s));
         p.sync();
@@ -1042,9 +1116,13 @@ void neighbour::update() {
             list16.resize(size_t(round_up(p.n, WAVE)) * cap);
             continue;
         }
-        // the force kernel stages exactly the largest union
+        // the force kernel stages exactly the largest union; the next build
+        // stages the largest one of this build with a margin (a smaller
+        // workgroup footprint, more workgroups per CU; a union that outgrows
+        // it is caught above and rebuilt)
         sp_force = sp;
         sp_force.lds_cap = std::max(1u, u);
+        sp.lds_cap = unsigned(std::min<size_t>(lds_limit, round_up(size_t(u) + u / 16 + 64, 64)));
         break;
     }
     ++rebuilds;
@@ -1122,6 +1200,7 @@ pair_trunc::pair_trunc(potential const& pot_, particle& p1_, particle& p2_, box 
             "pair_trunc: the neighbour lists belong to other particle instances");
     params = pot.device_params(b);
     HIP_CHECK(hipEventCreateWithFlags(&p2_ready_, hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&p1_done_, hipEventDisableTiming));
     p.forces.push_back(this);
     p.force_valid = p.aux_valid = false;
 }
@@ -1131,6 +1210,7 @@ pair_trunc::~pair_trunc() {
     v.erase(std::remove(v.begin(), v.end(), static_cast<force_module*>(this)), v.end());
     p.force_valid = p.aux_valid = false;
     if (p2_ready_) (void)hipEventDestroy(p2_ready_);
+    if (p1_done_) (void)hipEventDestroy(p1_done_);
 }
 
 bool pair_trunc::inputs_changed() const { return &p != &p2 && p2.position_version != p2_version_; }
@@ -1150,7 +1230,10 @@ pair_full::pair_full(potential const& pot_, particle& p1_, particle& p2_, box co
             "pair_full: potential shape != nspecies of the second instance");
     params = pot.device_params(b, true);
     partial_.resize(std::max<size_t>(1, pair_full_partial_floats(p1.n, p2.n)));
-    if (&p1 != &p2) HIP_CHECK(hipEventCreateWithFlags(&p2_ready_, hipEventDisableTiming));
+    if (&p1 != &p2) {
+        HIP_CHECK(hipEventCreateWithFlags(&p2_ready_, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&p1_done_, hipEventDisableTiming));
+    }
     p1.forces.push_back(this);
     p1.force_valid = p1.aux_valid = false;
 }
@@ -1160,6 +1243,7 @@ pair_full::~pair_full() {
     v.erase(std::remove(v.begin(), v.end(), static_cast<force_module*>(this)), v.end());
     p1.force_valid = p1.aux_valid = false;
     if (p2_ready_) (void)hipEventDestroy(p2_ready_);
+    if (p1_done_) (void)hipEventDestroy(p1_done_);
 }
 
 void pair_full::prepare() {}
@@ -1180,6 +1264,12 @@ void pair_full::apply(bool force_zero) {
     launch_pair_full(p1.posq, p1.n, p2.posq, p2.n, &p1 == &p2, params, aux, force_zero, w, p1.force, p1.stress,
                      p1.n_pad, partial_, p1.stream);
     timer.stop(p1.stream);
+    if (&p1 != &p2) {
+        // the second instance's stream must not move its positions before this
+        // kernel has read them
+        HIP_CHECK(hipEventRecord(p1_done_, p1.stream));
+        HIP_CHECK(hipStreamWaitEvent(p2.stream, p1_done_, 0));
+    }
 }
 
 unsigned pair_trunc::tune() {
@@ -1251,7 +1341,14 @@ bool pair_trunc::launch(bool force_zero, uint32_t const* abort_slots, uint32_t a
     else
         launch_pair_force(p.posq, p.n, nb.list, nb.count, nb.cap, params, aux, force_zero, w, p.force, p.stress,
                           p.n_pad, p.stream, fuse, p2.posq);
-    return timer.stop(p.stream);
+    bool const done = timer.stop(p.stream);
+    if (!same) {
+        // the second instance's stream must not move, rearrange or rebin its
+        // particles before this kernel has read their positions
+        HIP_CHECK(hipEventRecord(p1_done_, p.stream));
+        HIP_CHECK(hipStreamWaitEvent(p2.stream, p1_done_, 0));
+    }
+    return done;
 }
 
 bool pair_trunc::integrate_fused(verlet& v) {

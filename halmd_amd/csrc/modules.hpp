@@ -106,6 +106,11 @@ public:
     void get_position(double* r);
     void set_velocity(double const* v);
     void get_velocity(double* v);
+    // float-float words as stored (hi, lo per component), memory order
+    void get_position_parts(float* parts);
+    void set_position_parts(float const* parts);
+    void get_velocity_parts(float* parts);
+    void set_velocity_parts(float const* parts);
     void set_species(unsigned const* s);
     void get_species(unsigned* s);
     void set_mass(double const* m);
@@ -129,6 +134,8 @@ public:
     void settle_velocity();
     float take_pending_kick() { float const h = pending_half_dt; pending_half_dt = 0; return h; }
     // recompute the fixed-point snapshot after a host-side change of pos
+    void download_parts(dbuf<float4> const& b, float* out);
+    void upload_parts(dbuf<float4>& b, float const* in);
     void requantize();
     // bind the periodic box (done by every module constructed from particle
     // and box); a particle belongs to one box
@@ -221,8 +228,12 @@ public:
     // cell keys, permutation and offsets for the current positions; the
     // compact records cell_pos only if need_cell_pos
     void update(bool need_cell_pos = true);
-    // rearrange the particles into cell order (stable); offsets stay valid
-    void reorder_particles();
+    // rearrange the particles into cell order (stable); offsets stay valid.
+    // With sub_bins, into (cell, x sub-bin) order, and order_sub_offset
+    // holds the offsets of the (cell, sub-bin) ranges
+    void reorder_particles(bool sub_bins = false);
+    dbuf<uint32_t> order_sub_offset;   // (ncells << SUB_BITS) + 1 entries
+    bool sub_order = false;            // the last reorder used sub-bins
     void cell_lists(unsigned* begin_out, unsigned* end_out, unsigned* index_out);
     kernel_timer timer;
 };
@@ -305,6 +316,8 @@ public:
     bool staged = false;
     seg_params sp{}, sp_force{};      // build staging / force staging (lds_cap = largest union)
     size_t lds_limit = 0;             // records of dynamic LDS a build workgroup may use
+    size_t lds_bytes = 0;             // dynamic LDS bytes a build workgroup may use
+    bool windows = false;             // windowed list build over the (cell, x sub-bin) order
     dbuf<uint16_t> list16;
     void plan_segments(unsigned S, unsigned R);
     // segments of S cells by R rows, workgroups of `threads`, with the staging
@@ -385,6 +398,7 @@ private:
     dbuf<float> partial_;
     uint64_t p2_version_ = 0;
     hipEvent_t p2_ready_ = nullptr;
+    hipEvent_t p1_done_ = nullptr;   // this kernel read p2 (p2's stream waits)
 };
 
 class pair_trunc : public force_module {
@@ -422,6 +436,7 @@ private:
                 fused_integrate const* fuse = nullptr);
     uint64_t p2_version_ = 0;
     hipEvent_t p2_ready_ = nullptr;
+    hipEvent_t p1_done_ = nullptr;   // this kernel read p2 (p2's stream waits)
 };
 
 class integrator {

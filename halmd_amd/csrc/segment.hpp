@@ -72,7 +72,7 @@ __device__ __forceinline__ void put_rec(uint2* s, unsigned k, uint4 const r) { s
 
 // Called by the whole workgroup (at least 64 threads).  Fills st (and, if
 // bound is given, the LDS start of each of the w + 2 cells of every union row:
-// bound[ur * SEG_BOUND + k], k = 0 .. w + 2), then, if U + 1 <= sp.lds_cap,
+// bound[ur * bstride + k], k = 0 .. w + 2; bstride >= w + 3), then, if U + 1 <= sp.lds_cap,
 // copies the union into s_rec[0 .. U) (in the format of Rec) and a sentinel
 // record into s_rec[U]:
 // fixed-point y and z half a box from the centre of the segment's rows and
@@ -87,7 +87,7 @@ __device__ __forceinline__ void put_rec(uint2* s, unsigned k, uint4 const r) { s
 template <class Rec>
 __device__ __forceinline__ unsigned stage_union(seg_params const& sp, unsigned seg, uint32_t const* __restrict__ off,
                                                 uint4 const* __restrict__ posq, Rec* s_rec, seg_stage& st,
-                                                unsigned* bound) {
+                                                unsigned* bound, unsigned bstride = SEG_BOUND) {
     unsigned const tid = threadIdx.x, lane = tid % WAVE;
     if (tid < WAVE) {
         seg_place const q = seg_locate(sp, seg);
@@ -120,7 +120,7 @@ __device__ __forceinline__ unsigned stage_union(seg_params const& sp, unsigned s
         // cells x0 - 1 .. x0 + w (periodic) of union row tid, in LDS order
         unsigned const nx = sp.grid.ncell[0];
         unsigned const row = seg_row_start(sp, q, tid);
-        unsigned* const bd = bound + tid * SEG_BOUND;
+        unsigned* const bd = bound + tid * bstride;
         unsigned b = st.base[tid];
         bd[0] = b;
         for (unsigned k = 0; k < q.w + 2; ++k) {

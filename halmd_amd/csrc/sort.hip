@@ -177,21 +177,69 @@ __global__ void fill_u32_kernel(uint32_t* out, uint32_t value, size_t n) {
     if (i < n) out[i] = value;
 }
 
-void scan_level(uint32_t* data, size_t m, scan_workspace& ws, unsigned level, hipStream_t s) {
+// block sums of level l live at partial + level_offset[l]; nlevel entries
+struct scan_view {
+    uint32_t* partial;
+    size_t const* level_offset;
+    size_t nlevel;
+};
+
+void scan_level(uint32_t* data, size_t m, scan_view const& v, unsigned level, hipStream_t s) {
     unsigned const nb = div_up(m, SCAN_TILE);
     if (nb == 1) {
         scan_apply_kernel<<<1, SCAN_THREADS, 0, s>>>(data, m, nullptr);
         return;
     }
-    if (level + 1 >= ws.level_offset.size() || ws.level_offset[level + 1] - ws.level_offset[level] < nb)
+    if (level + 1 >= v.nlevel || v.level_offset[level + 1] - v.level_offset[level] < nb)
         throw error(HALMD_ERR_LOGIC, "scan workspace too small");
-    uint32_t* partial = ws.partial.get() + ws.level_offset[level];
+    uint32_t* partial = v.partial + v.level_offset[level];
     scan_reduce_kernel<<<nb, SCAN_THREADS, 0, s>>>(data, m, partial);
-    scan_level(partial, nb, ws, level + 1, s);
+    scan_level(partial, nb, v, level + 1, s);
     scan_apply_kernel<<<nb, SCAN_THREADS, 0, s>>>(data, m, partial);
 }
 
+// per-level block-sum offsets of a scan of n values; returns the level count
+// (offset[nlevel - 1] = the words of block sums in all)
+size_t scan_levels(size_t n, size_t* offset, size_t max_levels) {
+    size_t k = 0, m = n;
+    offset[k++] = 0;
+    while (true) {
+        size_t const nb = div_up(m, SCAN_TILE);
+        if (nb <= 1) break;
+        require(k < max_levels, "scan: too many levels");
+        offset[k] = offset[k - 1] + nb;
+        ++k;
+        m = nb;
+    }
+    return k;
+}
+
+// the sort proper, on caller-provided buffers: keys_alt and vals_alt of n
+// words, counts of div_up(n, RS_TILE) * RS_BUCKETS words and the block sums
+// of their scan
+void radix_sort_on(uint32_t* keys, uint32_t* vals, size_t n, unsigned key_bits, uint32_t* keys_alt,
+                   uint32_t* vals_alt, uint32_t* counts, scan_view const& scan, hipStream_t s) {
+    unsigned const passes = div_up(key_bits, 8);
+    unsigned const nb = div_up(n, RS_TILE);
+    uint32_t *kin = keys, *vin = vals, *kout = keys_alt, *vout = vals_alt;
+    for (unsigned p = 0; p < passes; ++p) {
+        unsigned const shift = 8 * p;
+        rs_histogram_kernel<<<nb, RS_THREADS, 0, s>>>(kin, n, shift, counts, nb);
+        scan_level(counts, size_t(nb) * RS_BUCKETS, scan, 0, s);
+        rs_scatter_kernel<<<nb, RS_THREADS, 0, s>>>(kin, vin, kout, vout, n, shift, counts, nb);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    if (passes % 2) {
+        HIP_CHECK(hipMemcpyAsync(keys, kin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(vals, vin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+
 } // namespace
+
 
 void scan_workspace::reserve(size_t n) {
     std::vector<size_t> off{0};
@@ -213,7 +261,7 @@ void scan_workspace::reserve(size_t n) {
 void scan_exclusive(uint32_t* data, size_t m, scan_workspace& ws, hipStream_t s) {
     if (m == 0) return;
     ws.reserve(m);
-    scan_level(data, m, ws, 0, s);
+    scan_level(data, m, scan_view{ws.partial.get(), ws.level_offset.data(), ws.level_offset.size()}, 0, s);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -232,22 +280,8 @@ void radix_sort(uint32_t* keys, uint32_t* vals, size_t n, unsigned key_bits, rad
     if (n <= 1 || key_bits == 0) return;
     require(key_bits <= 32, "radix_sort: key_bits > 32");
     ws.reserve(n);
-    unsigned const passes = div_up(key_bits, 8);
-    unsigned const nb = div_up(n, RS_TILE);
-    uint32_t *kin = keys, *vin = vals, *kout = ws.keys_alt, *vout = ws.vals_alt;
-    for (unsigned p = 0; p < passes; ++p) {
-        unsigned const shift = 8 * p;
-        rs_histogram_kernel<<<nb, RS_THREADS, 0, s>>>(kin, n, shift, ws.counts, nb);
-        scan_exclusive(ws.counts, size_t(nb) * RS_BUCKETS, ws.scan, s);
-        rs_scatter_kernel<<<nb, RS_THREADS, 0, s>>>(kin, vin, kout, vout, n, shift, ws.counts, nb);
-        std::swap(kin, kout);
-        std::swap(vin, vout);
-    }
-    if (passes % 2) {
-        HIP_CHECK(hipMemcpyAsync(keys, kin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-        HIP_CHECK(hipMemcpyAsync(vals, vin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-    }
-    HIP_CHECK(hipGetLastError());
+    radix_sort_on(keys, vals, n, key_bits, ws.keys_alt, ws.vals_alt, ws.counts,
+                  scan_view{ws.scan.partial.get(), ws.scan.level_offset.data(), ws.scan.level_offset.size()}, s);
 }
 
 void launch_iota(uint32_t* out, size_t n, hipStream_t s) {

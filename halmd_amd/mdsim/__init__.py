@@ -44,6 +44,27 @@ class box(Handle):
     def lowest_corner(self):
         return -self.length / 2
 
+    @property
+    def edges(self):
+        return np.diag(self.length)
+
+    def writer(self, file, location):
+        """lua/halmd/mdsim/box.lua.in:138-170 (box fixed in time): group
+        location/box with attributes dimension and boundary, dataset edges."""
+        loc = list(location) + ["box"]
+        file.write_attribute_int(loc, "dimension", 3)
+        file.write_attribute_strings(loc, "boundary", ["periodic"] * 3)
+        w = file.writer(location=list(location), mode="truncate")
+        w.on_write(lambda: self.edges, ["box", "edges"])
+        w.write()
+        return w
+
+    @staticmethod
+    def reader(file, location):
+        """lua/halmd/mdsim/box.lua.in:190-215: the edge vectors stored at
+        location/box/edges."""
+        return file.read_dataset(list(location) + ["box", "edges"])
+
 
 class _data:
     """particle.data[name] get/set in memory order (particle.lua.in data table)."""
@@ -74,6 +95,11 @@ class _data:
             out = np.zeros(n)
             call("halmd_mdsim_particle_get_potential_energy", p.handle, dptr(out))
             return out
+        if name in ("position_parts", "velocity_parts"):
+            # the float-float words (hi, lo) per component, exact
+            out = np.zeros((n, 3, 2), dtype=np.float32)
+            call(f"halmd_mdsim_particle_get_{name}", p.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+            return out
         if name == "position_fixed":
             out = np.zeros((n, 3), dtype=np.uint32)
             call("halmd_mdsim_particle_get_position_fixed", p.handle, out.ctypes.data_as(ctypes.c_void_p))
@@ -90,6 +116,9 @@ class _data:
         if name in ("position", "velocity"):
             a = f64(value, (n, 3))
             call(f"halmd_mdsim_particle_set_{name}", p.handle, dptr(a))
+        elif name in ("position_parts", "velocity_parts"):
+            a = np.ascontiguousarray(value, dtype=np.float32).reshape(n, 3, 2)
+            call(f"halmd_mdsim_particle_set_{name}", p.handle, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
         elif name == "image":
             a = np.ascontiguousarray(value, dtype=np.int32).reshape(n, 3)
             call("halmd_mdsim_particle_set_image", p.handle, iptr(a))
@@ -141,6 +170,14 @@ class particle(Handle):
     def ordered(self, name):
         """data[name] in particle-ID order (reverse_id maps ID -> memory index)."""
         return self.data[name][self.data["reverse_id"]]
+
+    def reset_order(self):
+        """Rearrange the particles into particle-ID order (memory index =
+        ID), as a freshly created instance has them: neighbour lists and cell
+        order are rebuilt at the next force evaluation.  With the float-float
+        state restored exactly, a run continued from here and a run restarted
+        from that state then proceed bit for bit alike."""
+        self.rearrange(self.data["reverse_id"])
 
 
 class binning(Handle):
@@ -366,6 +403,16 @@ class core(Handle):
         v = ctypes.c_uint64()
         call("halmd_mdsim_core_step", self.handle, ctypes.byref(v))
         return v.value
+
+    @step.setter
+    def step(self, value):
+        call("halmd_mdsim_core_set_step", self.handle, int(value))
+
+    @property
+    def time(self):
+        """Simulation time of the current step (halmd/mdsim/clock.hpp): the
+        step count times the integrator's timestep."""
+        return self.step * (self.integrator.timestep if self.integrator is not None else 0.0)
 
 
 from . import potentials, forces, integrators, sorts, positions, velocities  # noqa: E402,F401

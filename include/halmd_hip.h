@@ -100,6 +100,15 @@ HALMD_API int halmd_mdsim_particle_set_position(halmd_particle* p, const double*
 HALMD_API int halmd_mdsim_particle_get_position(halmd_particle* p, double* r);
 HALMD_API int halmd_mdsim_particle_set_velocity(halmd_particle* p, const double* v);
 HALMD_API int halmd_mdsim_particle_get_velocity(halmd_particle* p, double* v);
+/* The float-float words themselves, memory order, (hi, lo) per component:
+ * parts[6 i + 2 k] = hi, parts[6 i + 2 k + 1] = lo.  The reference exposes
+ * only the hi word of its double-single arrays (particle_array_gpu.cpp:
+ * 174-201); these carry the state exactly, for a restart that continues
+ * bit for bit (hi + lo rounded to double can drop low bits of lo). */
+HALMD_API int halmd_mdsim_particle_get_position_parts(halmd_particle* p, float* parts);
+HALMD_API int halmd_mdsim_particle_set_position_parts(halmd_particle* p, const float* parts);
+HALMD_API int halmd_mdsim_particle_get_velocity_parts(halmd_particle* p, float* parts);
+HALMD_API int halmd_mdsim_particle_set_velocity_parts(halmd_particle* p, const float* parts);
 HALMD_API int halmd_mdsim_particle_set_species(halmd_particle* p, const unsigned* species);
 HALMD_API int halmd_mdsim_particle_get_species(halmd_particle* p, unsigned* species);
 HALMD_API int halmd_mdsim_particle_set_mass(halmd_particle* p, const double* mass);
@@ -330,6 +339,9 @@ HALMD_API int halmd_mdsim_core_add_integrator(halmd_core* c, halmd_integrator* i
 HALMD_API int halmd_mdsim_core_set_sort(halmd_core* c, halmd_sort* s, unsigned interval);
 HALMD_API int halmd_mdsim_core_mdstep(halmd_core* c);
 HALMD_API int halmd_mdsim_core_step(const halmd_core* c, uint64_t* step);
+/* the step counter of a run restarted at a saved step (halmd/mdsim/clock.hpp
+ * advance(): the sort schedule counts from it) */
+HALMD_API int halmd_mdsim_core_set_step(halmd_core* c, uint64_t step);
 
 /* ---- observables.thermodynamics: halmd/observables/gpu/thermodynamics.cpp:67-191,
  *      halmd/observables/thermodynamics.hpp:80-90 (mean values per particle) -- */

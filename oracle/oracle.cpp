@@ -851,6 +851,43 @@ long oracle_cutoff_band(int n, double const* pos, int const* species, double con
     return count;
 }
 
+// Per-particle sum of the pair-force magnitudes, scale[i] = sum_j |fval_ij| r_ij
+// over the pairs within the cutoff.  An FP32 force sum carries rounding
+// errors in proportion to its terms, not to its result: this is the scale of
+// the error a correct FP32 evaluation may have on particle i.
+void oracle_pair_force_scale(int n, double const* pos, int const* species, double const* L, void const* p,
+                             double* scale)
+{
+    auto const& pot = *static_cast<potential_t const*>(p);
+    box_t box(L);
+    std::fill(scale, scale + n, 0.0);
+    binning_t b(box, pot.r_cut_max(), 1e-9);
+    auto visit = [&](unsigned i, unsigned j) {
+        double r[3];
+        for (int k = 0; k < 3; ++k) r[k] = pos[3 * i + k] - pos[3 * j + k];
+        box.reduce_periodic(r);
+        double const rr = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        if (!pot.within_range(rr, species[i], species[j])) return;
+        double fval, u;
+        pot.eval(rr, species[i], species[j], fval, u);
+        double const f = std::abs(fval) * std::sqrt(rr);
+        scale[i] += f;
+        scale[j] += f;
+    };
+    if (std::min({b.ncell[0], b.ncell[1], b.ncell[2]}) < 3) {
+        for (int i = 0; i < n; ++i)
+            for (int j = i + 1; j < n; ++j) visit(unsigned(i), unsigned(j));
+        return;
+    }
+    std::vector<double> ps(pos, pos + 3 * size_t(n));
+    std::vector<int> sp(species, species + n);
+    b.update(ps, n);
+    neighbour_t nb(pot, 1e-9);
+    nb.update(b, box, ps, sp, n);
+    for (int i = 0; i < n; ++i)
+        for (unsigned j : nb.list[i]) visit(unsigned(i), j);
+}
+
 // ===========================================================================
 // FP32 device rules (bit-exact references)
 // ===========================================================================

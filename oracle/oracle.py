@@ -52,6 +52,7 @@ def lib():
             "oracle_pair_force": (None, [ctypes.c_int, _d, _i, _d, _vp, ctypes.c_double, ctypes.c_double, _d, _d, _d]),
             "oracle_pair_force_brute": (None, [ctypes.c_int, _d, _i, _d, _vp, _d, _d, _d]),
             "oracle_cutoff_band": (ctypes.c_long, [ctypes.c_int, _d, _i, _d, _vp, ctypes.c_double, _d]),
+            "oracle_pair_force_scale": (None, [ctypes.c_int, _d, _i, _d, _vp, _d]),
             "oracle_fixed_point": (None, [ctypes.c_int, _d, _d, _u]),
             "oracle_fixed_cell_index": (None, [ctypes.c_int, _u, _u, _u]),
             "oracle_fixed_neighbour_list": (ctypes.c_int, [ctypes.c_int, _u, _u, _u, _f, ctypes.c_int, _f, ctypes.c_int, _u, _u]),
@@ -214,6 +215,16 @@ def cutoff_band(pos, species, L, pot, band=1e-6):
     jump = np.zeros(len(pos))
     cnt = lib().oracle_cutoff_band(len(pos), _p(pos, _d), _p(sp, _i), _p(L, _d), pot.h, float(band), _p(jump, _d))
     return cnt, jump
+
+
+def pair_force_scale(pos, species, L, pot):
+    """Per-particle sum of the pair-force magnitudes within the cutoff."""
+    pos = _dd(pos)
+    sp = np.ascontiguousarray(species, dtype=np.int32)
+    L = _dd(L)
+    out = np.zeros(len(pos))
+    lib().oracle_pair_force_scale(len(pos), _p(pos, _d), _p(sp, _i), _p(L, _d), pot.h, _p(out, _d))
+    return out
 
 
 def fixed_point(pos, L):

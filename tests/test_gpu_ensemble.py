@@ -75,6 +75,7 @@ def test_each_system_same_alone_and_in_the_mix(oracle):
             assert np.max(np.abs((r_e + im_e * L) - (r_a + im_a * L))) < 1e-4
             assert np.max(np.abs(v_e - v_a)) < 1e-3
             assert e.rebuilds(k) == w.neighbour.rebuilds
+        else:
             # the same device code and the same rebuild rule: bitwise
             assert np.array_equal(r_e, r_a)
             assert np.array_equal(v_e, v_a)

@@ -231,6 +231,34 @@ def test_force_small_box_is_brute_force(oracle):
     assert np.allclose(u, 0.5 * ub, rtol=1e-15) and np.allclose(s, 0.5 * sb, rtol=1e-15)
 
 
+def test_pair_force_scale(oracle):
+    # closed form on three particles, then a brute-force sweep on a mixture
+    L = np.full(3, 20.0)
+    pot = oracle.Potential(oracle.POT_LJ, "shifted", 1.0, 1.0, 2.5)
+    pos = np.array([[0.0, 0, 0], [1.1, 0, 0], [0, 2.0, 0]])
+
+    def mag(r):
+        return abs(48 * r ** -13 - 24 * r ** -7)
+    d12 = np.hypot(1.1, 2.0)
+    s = oracle.pair_force_scale(pos, np.zeros(3, np.int32), L, pot)
+    assert np.allclose(s, [mag(1.1) + mag(2.0), mag(1.1) + mag(d12), mag(2.0) + mag(d12)], rtol=1e-13)
+    pos, L, species, _ = _liquid(oracle, 1372, 1.2, 2, species_frac=0.2)
+    pot = _ka_potential(oracle)
+    s = oracle.pair_force_scale(pos, species, L, pot)
+    rc = np.array([[2.5, 2.0], [2.0, 2.2]])
+    eps = np.array([[1.0, 1.5], [1.5, 0.5]])
+    sig = np.array([[1.0, 0.8], [0.8, 0.88]])
+    d = pos[:, None, :] - pos[None, :, :]
+    d -= L * np.round(d / L)
+    r = np.sqrt(np.sum(d * d, axis=-1))
+    np.fill_diagonal(r, np.inf)
+    e, sg = eps[species][:, species], sig[species][:, species]
+    x = sg / r
+    f = np.abs(24 * e / r * (2 * x ** 12 - x ** 6))
+    f[r >= rc[species][:, species]] = 0
+    assert np.allclose(s, f.sum(axis=1), rtol=1e-12)
+
+
 def test_cutoff_band_pairs(oracle):
     # closed form: two particles at r_c (1 +- 4e-7) are in a 1e-6 band with
     # |F(r_c)| = |48 r^-13 - 24 r^-7| each; at r_c (1 - 4e-6) they are not
