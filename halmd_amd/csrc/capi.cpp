@@ -128,6 +128,16 @@ HALMD_API int halmd_algorithm_radix_sort(uint32_t* keys, uint32_t* values, size_
     });
 }
 
+HALMD_API int halmd_algorithm_radix_sort_noalloc(uint32_t* keys, uint32_t* values, size_t n, unsigned key_bits) {
+    static_assert(HALMD_RADIX_SORT_NOALLOC_MAX == NOALLOC_SORT_MAX, "the header states the kernel's limit");
+    return guard([&] {
+        nonnull(keys, "keys");
+        nonnull(values, "values");
+        radix_sort_noalloc(keys, values, n, key_bits, nullptr);
+        HIP_CHECK(hipStreamSynchronize(nullptr));
+    });
+}
+
 HALMD_API int halmd_algorithm_scan_exclusive(uint32_t* data, size_t n) {
     return guard([&] {
         nonnull(data, "data");

@@ -29,6 +29,10 @@ struct radix_sort_workspace {
 // stable LSD radix sort of (key, value) pairs by the low key_bits bits of key,
 // in place; 8-bit digits, ceil(key_bits / 8) passes
 void radix_sort(uint32_t* keys, uint32_t* vals, size_t n, unsigned key_bits, radix_sort_workspace& ws, hipStream_t s);
+// the same sort on a file-scope device scratch array (no allocation): at most
+// NOALLOC_SORT_MAX elements; calls that overlap on one device race
+constexpr size_t NOALLOC_SORT_MAX = size_t(1) << 21;
+void radix_sort_noalloc(uint32_t* keys, uint32_t* vals, size_t n, unsigned key_bits, hipStream_t s);
 
 void launch_iota(uint32_t* out, size_t n, hipStream_t s);
 void launch_fill_u32(uint32_t* out, uint32_t value, size_t n, hipStream_t s);

@@ -11,6 +11,8 @@
 // per call.
 #include "kernels.hpp"
 
+#include <mutex>
+
 namespace halmd {
 namespace {
 
@@ -240,6 +242,39 @@ void radix_sort_on(uint32_t* keys, uint32_t* vals, size_t n, unsigned key_bits, 
 
 } // namespace
 
+// ---------------------------------------------------------------------------
+// scratch of the allocation-free sort: keys_alt, vals_alt, counts and the
+// block sums of their scan for up to NOALLOC_SORT_MAX elements (16.5 MiB).
+// Each device holds its own copy from the library's load to the process end;
+// every call on a device shares it, so calls that overlap on one device race.
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_BLOCKS = NOALLOC_SORT_MAX / RS_TILE;
+constexpr size_t NOALLOC_COUNTS = NOALLOC_BLOCKS * RS_BUCKETS;
+constexpr size_t NOALLOC_PARTIAL = 256;   // block sums of a scan of NOALLOC_COUNTS words (64 + 1)
+constexpr size_t NOALLOC_WORDS = 2 * NOALLOC_SORT_MAX + NOALLOC_COUNTS + NOALLOC_PARTIAL;
+static_assert(NOALLOC_SORT_MAX % RS_TILE == 0, "whole tiles");
+
+static __device__ __align__(256) uint32_t scratch[NOALLOC_WORDS];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex scratch_mutex;
+static uint32_t* scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// the device address of this device's scratch (the symbol's host-side name is
+// no device address), looked up once per device with the device current
+static uint32_t* scratch_on_current_device() {
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    require(dev >= 0 && dev < NOALLOC_MAX_DEVICES, "radix_sort_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        require(ptr != nullptr, "radix_sort_noalloc: no device address for the scratch");
+        scratch_ptr[dev] = static_cast<uint32_t*>(ptr);
+    }
+    return scratch_ptr[dev];
+}
 
 void scan_workspace::reserve(size_t n) {
     std::vector<size_t> off{0};
@@ -282,6 +317,23 @@ void radix_sort(uint32_t* keys, uint32_t* vals, size_t n, unsigned key_bits, rad
     ws.reserve(n);
     radix_sort_on(keys, vals, n, key_bits, ws.keys_alt, ws.vals_alt, ws.counts,
                   scan_view{ws.scan.partial.get(), ws.scan.level_offset.data(), ws.scan.level_offset.size()}, s);
+}
+
+void radix_sort_noalloc(uint32_t* keys, uint32_t* vals, size_t n, unsigned key_bits, hipStream_t s) {
+    if (n > NOALLOC_SORT_MAX)
+        throw error(HALMD_ERR_CAPACITY, "radix_sort_noalloc: more elements than the scratch holds (2^21)");
+    require(key_bits <= 32, "radix_sort: key_bits > 32");
+    uint32_t* const base = scratch_on_current_device();
+    if (n <= 1 || key_bits == 0) return;
+    size_t level_offset[8];
+    size_t const nb = div_up(n, RS_TILE);
+    size_t const nlevel = scan_levels(nb * RS_BUCKETS, level_offset, 8);
+    require(level_offset[nlevel - 1] <= NOALLOC_PARTIAL, "radix_sort_noalloc: scan block sums exceed the scratch");
+    uint32_t* const keys_alt = base;
+    uint32_t* const vals_alt = base + NOALLOC_SORT_MAX;
+    uint32_t* const counts = base + 2 * NOALLOC_SORT_MAX;
+    uint32_t* const partial = counts + NOALLOC_COUNTS;
+    radix_sort_on(keys, vals, n, key_bits, keys_alt, vals_alt, counts, scan_view{partial, level_offset, nlevel}, s);
 }
 
 void launch_iota(uint32_t* out, size_t n, hipStream_t s) {

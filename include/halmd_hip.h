@@ -78,6 +78,15 @@ HALMD_API int halmd_device_copy(void* dst, const void* src, size_t bytes);
 HALMD_API int halmd_algorithm_radix_sort(uint32_t* keys, uint32_t* values, size_t n, unsigned key_bits);
 /* exclusive prefix sum in place; halmd/algorithm/gpu/scan.hpp:37-107 */
 HALMD_API int halmd_algorithm_scan_exclusive(uint32_t* data, size_t n);
+/* halmd_algorithm_radix_sort without any device allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * scratch (16.5 MiB) is a file-scope device array that each GPU holds until
+ * the process ends.  At most HALMD_RADIX_SORT_NOALLOC_MAX elements
+ * (HALMD_ERR_CAPACITY beyond).  Every call on a GPU shares that GPU's scratch:
+ * calls that overlap on one GPU race.  Same result as halmd_algorithm_radix_sort. */
+#define HALMD_RADIX_SORT_NOALLOC_MAX ((size_t)1 << 21)
+HALMD_API int halmd_algorithm_radix_sort_noalloc(uint32_t* keys, uint32_t* values, size_t n, unsigned key_bits);
 
 /* ---- mdsim.box: halmd/mdsim/box.cpp:37-62, lua/halmd/mdsim/box.lua.in ---- */
 HALMD_API int halmd_mdsim_box_create(const double* length, halmd_box** out);
