@@ -10,7 +10,13 @@ is bracketed by device synchronisations.
 
 Prints one JSON line with the throughput (particle-steps/s), the HBM roofline
 of the dominant kernel (the pair-force kernel) measured with HIP events on its
-stream, and the FP64 CPU oracle timed on a bounded sample on this host.
+stream, and the FP64 CPU oracle timed on a bounded sample on this host.  The
+same line carries, as extra keys, the metric's second half and context:
+"batched" (256 seeded systems stepped together for 5000 steps against the
+plain per-job path), "serving" (a Poisson job stream through the
+continuous-batching server against the plain per-job path run first-in
+first-out on the same arrivals, both measured), and "reference_benchmarks"
+(the reference's published LJ N=64,000 and Kob-Andersen N=256,000 setups).
 """
 import argparse
 import json
@@ -81,6 +87,18 @@ def kernel_times(w, steps):
     return out
 
 
+def calibration_valu(blocks=2048, iterations=8192):
+    """The VALU rate probe: a known number of FP32 FMA wave instructions with
+    no memory traffic; its hardware-counter reading and duration calibrate
+    the VALU issue fraction of the other kernels of the same profile."""
+    import ctypes
+    from halmd_amd import _lib
+    n, ms = ctypes.c_uint64(), ctypes.c_double()
+    _lib.call("halmd_device_valu_probe", blocks, iterations, ctypes.byref(n), ctypes.byref(ms))
+    log(f"calibration VALU probe: {n.value} FMA wave instructions in {ms.value:.3f} ms "
+        f"({n.value / (ms.value * 1e-3):.3e} per second)")
+
+
 def calibration_copy(nbytes):
     import ctypes
     from halmd_amd import _lib
@@ -113,44 +131,52 @@ def counter_summary(kernel):
     return out
 
 
-def cpu_baseline(w, budget_s):
-    """FP64 oracle (host backend restatement), one thread, same initial state."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, steps):
+    """FP64 oracle (host backend restatement), one thread, same initial state,
+    `steps` NVE steps after the setup (the reference host backend is
+    single-threaded: doc/benchmarks.rst.in:17-27)."""
     from oracle import oracle as orc
     pot = orc.Potential(orc.POT_LJ, "shifted", 1.0, 1.0, R_CUT)
-    t0 = time.perf_counter()
     sim = orc.Simulation(w.position, w.velocity, w.species, np.ones(N), w.length, pot, SKIN, TIMESTEP)
     sim.time(0)                     # initial binning, lists and forces
-    setup = time.perf_counter() - t0
-    dt1 = sim.time(2)
-    steps = max(2, int((budget_s - setup) / max(dt1 / 2, 1e-6)))
-    steps = min(steps, 200)
-    dt = sim.time(steps)
-    total_steps = steps + 2
-    secs = dt + dt1
-    return {"value": N * total_steps / secs, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"FP64 oracle (host-backend restatement, single thread), same LJ N={N} state, "
-                      f"{total_steps} NVE steps after setup ({sim.rebuilds()} neighbour-list builds incl. setup), "
-                      f"{secs:.1f} s"}
+    builds0 = sim.rebuilds()
+    secs = sim.time(steps)
+    return {"value": N * steps / secs, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 oracle (host-backend restatement, single thread) on {cpu_model()}, same LJ N={N} "
+                      f"state, {steps} NVE steps after setup ({sim.rebuilds() - builds0} neighbour-list builds), "
+                      f"{secs:.1f} s",
+            "cpu_model": cpu_model()}
 
 
-def batched(args):
-    """Batched ensembles: 256 seeded systems (N log-uniform in [500, 32000], rho
-    in [0.7, 1.2], T in [0.5, 2.0], LJ / WCA / Morse), NVE dt = 0.003, stepped
-    together; against the plain per-job path (each system alone through the
-    per-system modules, one after another) on every 16th system."""
+def batched_run(systems, steps, warmup, seed, plain_steps=1000):
+    """Batched ensembles: `systems` seeded systems (N log-uniform in [500,
+    32000], rho in [0.7, 1.2], T in [0.5, 2.0], LJ / WCA / Morse), NVE dt =
+    0.003, stepped together for `steps` steps; against the plain per-job path
+    (each system alone through the per-system modules, one after another) on
+    every 16th system.  Returns the figures as a dict."""
     from halmd_amd import workloads
-    specs = workloads.batched_specs(args.systems, seed=args.seed)
+    specs = workloads.batched_specs(systems, seed=seed)
     ntot = sum(s["n"] for s in specs)
     t0 = time.perf_counter()
     e = workloads.ensemble_from_specs(specs, timestep=0.003)
     log(f"ensemble of {len(specs)} systems, {ntot} particles, setup {time.perf_counter() - t0:.1f} s")
-    e.step(args.warmup)
-    ms = e.step_timed(args.steps)
-    value = ntot * args.steps / (ms / 1e3)
-    log(f"batched: {ms / args.steps:.3f} ms/step, {value:.4e} particle-steps/s")
+    e.step(warmup)
     pot0, kin0 = e.energies()
+    ms = e.step_timed(steps)
+    pot1, kin1 = e.energies()
+    value = ntot * steps / (ms / 1e3)
+    log(f"batched: {ms / steps:.3f} ms/step, {value:.4e} particle-steps/s")
     sample = specs[::16]
-    plain_steps = max(10, args.steps // 5)
     psum, tsum = 0.0, 0.0
     for sp in sample:
         w = workloads.single_from_spec(sp, timestep=0.003)
@@ -159,30 +185,40 @@ def batched(args):
         psum += sp["n"] * plain_steps
         tsum += t / 1e3
     plain = psum / tsum
-    log(f"plain per-job path on {len(sample)} systems: {plain:.4e} particle-steps/s")
+    log(f"plain per-job path on {len(sample)} systems: {plain:.4e} particle-steps/s ({value / plain:.1f}x)")
+    drift = np.abs((pot1 + kin1) - (pot0 + kin0)) / (steps * 0.003)
+    return {
+        "workload": f"{len(specs)} independent NVE systems, N log-uniform in [500, 32000] (total {ntot}), "
+                    "rho U[0.7, 1.2], T U[0.5, 2.0], LJ rc 2.5 / WCA / Morse rc 2.5, dt 0.003, skin 0.3",
+        "systems": len(specs),
+        "particles": ntot,
+        "steps": steps,
+        "ms_per_step": ms / steps,
+        "particle_steps_per_s": value,
+        "plain_per_job_particle_steps_per_s": plain,
+        "plain_sample": f"every 16th system ({len(sample)}), {plain_steps} timed steps each, one after another",
+        "speedup_vs_plain": value / plain,
+        "energy_drift_per_particle_per_tau_max": float(drift.max()),
+    }
+
+
+def batched(args):
+    """--workload batched: the batched figures as the JSON line."""
+    b = batched_run(args.systems, args.steps, args.warmup, args.seed)
     out = {
         "metric": "aggregate particle-steps/s, batched ensemble of 256 systems vs plain per-job GPU path",
-        "value": value,
+        "value": b["particle_steps_per_s"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms / args.steps,
+        "ms_per_step": b["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {
-            "workload": f"{len(specs)} independent NVE systems, N log-uniform in [500, 32000] (total {ntot}), "
-                        "rho U[0.7, 1.2], T U[0.5, 2.0], LJ rc 2.5 / WCA / Morse rc 2.5, dt 0.003, skin 0.3",
-            "systems": len(specs),
-            "particles": ntot,
-            "plain_per_job_particle_steps_per_s": plain,
-            "plain_sample": f"every 16th system ({len(sample)}), {plain_steps} timed steps each, sequential",
-            "speedup_vs_plain": value / plain,
-            "en_tot_per_particle_range": [float(np.min(pot0 + kin0)), float(np.max(pot0 + kin0))],
-        },
+        "config": b,
     }
     print(json.dumps(out), flush=True)
 
@@ -228,26 +264,30 @@ def tiny(args):
     print(json.dumps(out), flush=True)
 
 
-def serving(args):
+def serving_run(rate, duration, seed, step_scale, tick=50, max_jobs=256, budget_gib=128):
     """Served job stream: Poisson arrivals of LJ / Kob-Andersen / WCA jobs (N
     log-normal, median 2e4, clipped to [1e3, 2e5]; rho in [0.8, 1.2]; T in
-    [0.45, 1.5]; steps log-uniform in [1e3, 1e5] times --step-scale), each with
-    blocking-scheme MSD and ISF at 8 wavenumbers, through one server in real
-    time; against the plain per-job path replayed first-in first-out with
-    service times measured on a sample of the stream's jobs."""
+    [0.45, 1.5]; steps log-uniform in [1e3, 1e5] times step_scale), each with
+    blocking-scheme MSD and ISF at 8 wavenumbers.  First through one server in
+    real time, then through the plain per-job path: the same jobs with the same
+    correlators, one at a time in arrival order, each built, run and read out
+    through the per-system modules before the next starts, arrivals replayed in
+    real time.  Both latencies are measured.  Returns the figures as a dict."""
     from collections import deque
-    from halmd_amd import workloads
+    from halmd_amd import observables, workloads
+    from halmd_amd.observables.dynamics import blocking_scheme
+    from halmd_amd.observables.utility import wavevector
     from halmd_amd.serve import server
-    stream = workloads.serving_stream(rate=args.rate, duration=args.duration, seed=args.seed,
-                                      step_scale=args.step_scale)
+    stream = workloads.serving_stream(rate=rate, duration=duration, seed=seed, step_scale=step_scale)
     dt = 0.003
     k = list(workloads.SERVING_WAVENUMBERS)
     corr = dict(block_size=10, shift=2, separation=10, wavenumber=k, tolerance=0.01, max_count=7)
     t0 = time.perf_counter()
     inputs = [workloads.system_inputs(j["n"], j["density"], j["temperature"], j["seed"]) for j in stream]
-    log(f"{len(stream)} jobs over {args.duration:.0f} s, inputs ready in {time.perf_counter() - t0:.1f} s")
-    s = server(budget_bytes=args.budget_gib << 30, max_jobs=args.max_jobs, tick_steps=args.tick)
-    offered = sum(j["n"] * j["steps"] for j in stream) / args.duration
+    log(f"{len(stream)} jobs over {duration:.0f} s, inputs ready in {time.perf_counter() - t0:.1f} s")
+    s = server(budget_bytes=budget_gib << 30, max_jobs=max_jobs, tick_steps=tick)
+    work = sum(j["n"] * (-(-j["steps"] // tick) * tick) for j in stream)
+    offered = work / duration
     pending = deque(range(len(stream)))
     ids = {}
     start = s.now()
@@ -260,98 +300,149 @@ def serving(args):
             L, r0, v0, _ = inputs[q]
             ids[q] = s.submit(r0, v0, L[0], j["steps"], potential=j["potential"], timestep=dt,
                               species=workloads.spec_species(j), arrival=start + j["arrival"], **corr)
-            inputs[q] = None
         st = s.stats()
         if not st["active"] and not st["queued"]:
             time.sleep(max(0.0, stream[pending[0]]["arrival"] - (s.now() - start)))
             continue
         s.tick()
         ticks += 1
-        if ticks % 50 == 0:
-            log(f"t = {s.now() - start:6.1f} s: {st['active']} running, {st['queued']} queued, "
+        if ticks % 100 == 0:
+            log(f"served t = {s.now() - start:6.1f} s: {st['active']} running, {st['queued']} queued, "
                 f"{len(ids)} submitted of {len(stream)}")
     end = s.now()
     lat = np.array([s.info(ids[q])["done"] - (start + stream[q]["arrival"]) for q in range(len(stream))])
     wait = np.array([s.info(ids[q])["admitted"] - (start + stream[q]["arrival"]) for q in range(len(stream))])
     psteps = s.stats()["particle_steps"]
     phases = s.phases()
-    log("tick phases (s): " + ", ".join(f"{k} {v:.2f}" for k, v in phases.items()))
-    value = psteps / (end - start)
-    log(f"served: {value:.4e} particle-steps/s over {end - start:.1f} s; latency p50 {np.percentile(lat, 50):.2f} s, "
+    for q in range(len(stream)):
+        s.release(ids[q])
+    served = psteps / (end - start)
+    log(f"served: {served:.4e} particle-steps/s over {end - start:.1f} s; latency p50 {np.percentile(lat, 50):.2f} s, "
         f"p99 {np.percentile(lat, 99):.2f} s; offered {offered:.3e} particle-steps/s")
+    del s
 
-    # plain per-job path: one job at a time through the per-system modules
-    # (with the same correlators); time per step fitted as a + b N on a sample
-    from halmd_amd import observables
-    from halmd_amd.observables.dynamics import blocking_scheme
-    from halmd_amd.observables.utility import wavevector
-    rng = np.random.default_rng(args.seed)
-    sample = sorted(rng.choice(len(stream), size=min(8, len(stream)), replace=False),
-                    key=lambda q: stream[q]["n"])
-    ns, ts, setups = [], [], []
-    for q in sample:
-        j = stream[q]
-        c0 = time.perf_counter()
+    # plain per-job path, first in first out on the same arrivals
+    plat = np.zeros(len(stream))
+    pstart = time.perf_counter()
+    for q, j in enumerate(stream):
+        now = time.perf_counter() - pstart
+        if now < j["arrival"]:
+            time.sleep(j["arrival"] - now)
+        steps = -(-j["steps"] // tick) * tick   # whole ticks, as the server runs them
         w = workloads.single_from_spec(j, timestep=dt)
-        steps = -(-j["steps"] // args.tick) * args.tick   # whole ticks, as the server runs them
-        nsteps = min(steps, 300)
-        bs = blocking_scheme(w.box, w.particle, timestep=dt, max_lag=steps * dt, resolution=args.tick * dt,
+        bs = blocking_scheme(w.box, w.particle, timestep=dt, max_lag=steps * dt, resolution=tick * dt,
                              block_size=10, shift=2, separation=10)
         bs.correlate_msd()
         bs.correlate_isf(observables.density_mode(w.particle, wavevector(w.box, k, tolerance=0.01, max_count=7)))
         w.sampler.on_sample_blocking_scheme(bs)
-        w.sampler.run(1)
-        setups.append(time.perf_counter() - c0)
-        ms = w.sampler.run_timed(nsteps)
-        ns.append(j["n"])
-        ts.append(ms / 1e3 / nsteps)
-    b, a = np.polyfit(ns, ts, 1)
-    setup = float(np.median(setups))
-    service = [setup + -(-j["steps"] // args.tick) * args.tick * max(a + b * j["n"], 0.0) for j in stream]
-    fin, plat = 0.0, []
-    for j, sv in zip(stream, service):
-        fin = max(fin, j["arrival"]) + sv
-        plat.append(fin - j["arrival"])
-    plat = np.array(plat)
-    plain = sum(j["n"] * j["steps"] for j in stream) / fin
-    log(f"plain per-job path (FIFO replay): {plain:.4e} particle-steps/s; latency p50 "
-        f"{np.percentile(plat, 50):.2f} s, p99 {np.percentile(plat, 99):.2f} s")
+        w.sampler.run(steps)
+        bs.finalise()
+        bs.result("msd")
+        bs.result("isf")
+        plat[q] = time.perf_counter() - pstart - j["arrival"]
+        if q % 100 == 99:
+            log(f"plain path: {q + 1} of {len(stream)} jobs done at {time.perf_counter() - pstart:.1f} s")
+    pend = time.perf_counter() - pstart
+    plain = work / pend
+    log(f"plain per-job path (first in first out, measured): {plain:.4e} particle-steps/s over {pend:.1f} s; "
+        f"latency p50 {np.percentile(plat, 50):.2f} s, p99 {np.percentile(plat, 99):.2f} s")
+    return {
+        "workload": f"Poisson {rate:g} jobs/s for {duration:g} s ({len(stream)} jobs): LJ / KA / WCA, "
+                    "N log-normal (median 2e4, log-sigma 0.75, clipped [1e3, 2e5]), rho U[0.8, 1.2], "
+                    f"T U[0.45, 1.5], steps log-uniform [1e3, 1e5] x {step_scale:g} (whole ticks of {tick}), "
+                    f"dt {dt}, MSD + ISF at 8 wavenumbers (blocking scheme, block 10)",
+        "jobs": len(stream),
+        "offered_particle_steps_per_s": offered,
+        "particle_steps_per_s": served,
+        "latency_p50_s": float(np.percentile(lat, 50)),
+        "latency_p99_s": float(np.percentile(lat, 99)),
+        "queue_wait_p50_s": float(np.percentile(wait, 50)),
+        "queue_wait_p99_s": float(np.percentile(wait, 99)),
+        "makespan_s": end - start,
+        "ticks": ticks,
+        "tick_steps": tick,
+        "tick_phase_seconds": phases,
+        "plain_per_job_particle_steps_per_s": plain,
+        "plain_latency_p50_s": float(np.percentile(plat, 50)),
+        "plain_latency_p99_s": float(np.percentile(plat, 99)),
+        "plain_makespan_s": pend,
+        "plain_method": "the same jobs and correlators one at a time in arrival order through the per-system "
+                        "modules (build, run, read out), arrivals replayed in real time; measured",
+        "speedup_vs_plain": served / plain,
+    }
+
+
+def serving(args):
+    """--workload serving: the serving figures as the JSON line."""
+    r = serving_run(args.rate, args.duration, args.seed, args.step_scale, args.tick, args.max_jobs,
+                    args.budget_gib)
     out = {
         "metric": "served job stream: aggregate particle-steps/s and p50/p99 job latency vs plain per-job GPU path",
-        "value": value,
+        "value": r["particle_steps_per_s"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
-        "steps": ticks,
+        "steps": r["ticks"],
         "warmup": 0,
-        "ms_per_step": (end - start) * 1e3 / max(ticks, 1),
+        "ms_per_step": r["makespan_s"] * 1e3 / max(r["ticks"], 1),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {
-            "workload": f"Poisson {args.rate:g} jobs/s for {args.duration:g} s ({len(stream)} jobs): LJ / KA / WCA, "
-                        "N log-normal (median 2e4, log-sigma 0.75, clipped [1e3, 2e5]), rho U[0.8, 1.2], "
-                        f"T U[0.45, 1.5], steps log-uniform [1e3, 1e5] x {args.step_scale:g}, dt {dt}, "
-                        "MSD + ISF at 8 wavenumbers (blocking scheme, block 10)",
-            "tick_steps": args.tick,
-            "max_jobs": args.max_jobs,
-            "budget_gib": args.budget_gib,
-            "offered_particle_steps_per_s": offered,
-            "latency_p50_s": float(np.percentile(lat, 50)),
-            "latency_p99_s": float(np.percentile(lat, 99)),
-            "queue_wait_p50_s": float(np.percentile(wait, 50)),
-            "queue_wait_p99_s": float(np.percentile(wait, 99)),
-            "makespan_s": end - start,
-            "tick_phase_seconds": phases,
-            "plain_per_job_particle_steps_per_s": plain,
-            "plain_latency_p50_s": float(np.percentile(plat, 50)),
-            "plain_latency_p99_s": float(np.percentile(plat, 99)),
-            "plain_method": f"FIFO replay of the same arrivals; time per step a + b N fitted on {len(sample)} "
-                            f"sampled jobs ({min(ns)}..{max(ns)} particles, up to 300 timed steps each, same "
-                            f"correlators), median setup {setup:.3f} s per job",
-            "speedup_vs_plain": value / plain,
-        },
+        "config": r,
+    }
+    print(json.dumps(out), flush=True)
+
+
+# the reference's published setups (doc/benchmarks.rst.in): H100 double-single
+REFERENCE_SETUPS = {
+    "lj64k": dict(n=64000, density=0.4, r_cut=3.0, skin=0.7, timestep=0.002, h100_ns=3.4,
+                  source="doc/benchmarks.rst.in:17-29"),
+    "ka256k": dict(n=256000, density=1.2, r_cut=2.5, skin=0.3, timestep=0.001, h100_ns=1.79, occupancy=0.7,
+                   source="doc/benchmarks.rst.in:62-79"),
+}
+
+
+def reference_setup_run(name, steps, warmup, seed):
+    """One of the reference's published benchmark setups, NVE, from an fcc
+    start at T = 1.2 (LJ) or 0.7 (Kob-Andersen): ns per step and particle
+    beside the reference's H100 figure (context; the setups differ from the
+    workloads of the metric)."""
+    from halmd_amd import workloads
+    c = REFERENCE_SETUPS[name]
+    if name == "lj64k":
+        w = workloads.lennard_jones(c["n"], density=c["density"], temperature=1.2, r_cut=c["r_cut"], skin=c["skin"],
+                                    timestep=c["timestep"], seed=seed)
+    else:
+        w = workloads.kob_andersen(c["n"], density=c["density"], temperature=0.7, skin=c["skin"],
+                                   timestep=c["timestep"], seed=seed)
+    w.force.tune()
+    w.sampler.run(warmup)
+    ms = w.sampler.run_timed(steps)
+    ns = ms / steps * 1e6 / c["n"]
+    log(f"{name}: {ns:.3f} ns per step and particle (H100 double-single: {c['h100_ns']} ns)")
+    return {"particles": c["n"], "steps": steps, "ns_per_step_and_particle": ns,
+            "particle_steps_per_s": c["n"] * steps / (ms / 1e3), "h100_ns_per_step_and_particle": c["h100_ns"],
+            "ratio_to_h100": c["h100_ns"] / ns, "source": c["source"],
+            "setup": {k: v for k, v in c.items() if k not in ("h100_ns", "source")}}
+
+
+def reference_setup(args):
+    r = reference_setup_run(args.workload, args.steps, args.warmup, args.seed)
+    out = {
+        "metric": f"particle-steps/s, reference benchmark setup {args.workload}",
+        "value": r["particle_steps_per_s"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ns_per_step_and_particle"] * r["particles"] * 1e-6,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": r["ratio_to_h100"],
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": r,
     }
     print(json.dumps(out), flush=True)
 
@@ -362,18 +453,25 @@ def main():
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--warmup", type=int, default=400)
     ap.add_argument("--profile-steps", type=int, default=500)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-steps", type=int, default=200,
+                    help="NVE steps of the FP64 CPU baseline (one thread) after its setup")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="LJ workload only: leave out the batched, serving and reference-setup figures")
+    ap.add_argument("--batched-steps", type=int, default=5000, help="steps of the batched ensemble figure")
     ap.add_argument("--no-tune", action="store_true", help="keep the default cell-segment shape")
     ap.add_argument("--segments", default=None,
                     help="cell-segment shape 'cells,rows,threads' instead of autotuning (profiling runs)")
-    ap.add_argument("--workload", default="lj", choices=["lj", "batched", "serving", "tiny"],
+    ap.add_argument("--workload", default="lj", choices=["lj", "batched", "serving", "tiny", "lj64k", "ka256k"],
                     help="lj: LJ fluid N = 2^20 (default); batched: ensemble of independent systems; "
                          "serving: Poisson job stream through the continuous-batching server; "
-                         "tiny: Kob-Andersen N = 4096 with all-pairs neighbour lists")
+                         "tiny: Kob-Andersen N = 4096 with all-pairs neighbour lists; lj64k, ka256k: the "
+                         "reference's published benchmark setups")
     ap.add_argument("--rate", type=float, default=50.0, help="serving: job arrivals per second")
-    ap.add_argument("--duration", type=float, default=10.0, help="serving: seconds of arrivals")
-    ap.add_argument("--step-scale", type=float, default=1.0 / 30, help="serving: factor on the job step counts")
+    ap.add_argument("--duration", type=float, default=6.0, help="serving: seconds of arrivals")
+    ap.add_argument("--step-scale", type=float, default=0.1,
+                    help="serving: factor on the job step counts (0.1 keeps the stream above what one GPU "
+                         "serves, as the full-length stream is)")
     ap.add_argument("--tick", type=int, default=50, help="serving: steps per tick")
     ap.add_argument("--max-jobs", type=int, default=256, help="serving: running-job cap")
     ap.add_argument("--budget-gib", type=int, default=128, help="serving: device memory budget for jobs")
@@ -394,9 +492,12 @@ def main():
         return serving(args)
     if args.workload == "tiny":
         return tiny(args)
+    if args.workload in REFERENCE_SETUPS:
+        return reference_setup(args)
     arch, cus, mem = halmd_amd.device_info()
     if args.calibration_copy_mib:
         calibration_copy(args.calibration_copy_mib << 20)
+        calibration_valu()
     log(f"device {arch}, {cus} CUs, {mem / 2**30:.0f} GiB; building LJ N={N}")
     t0 = time.perf_counter()
     w = build_workload(args.seed)
@@ -441,14 +542,48 @@ def main():
     traffic = prof.get("traffic_bytes_per_launch")
     kernels = {k: {"launches": int(n), "avg_ms": (ms_ / n if n else None)} for k, (ms_, n) in kt.items()}
     log(f"force kernel {f_avg:.4f} ms/launch, {achieved:.0f} GB/s algorithmic; n_nb = {n_nb:.2f}")
+    nb_ms, nb_n = kt["neighbour"]
+    nb_avg = nb_ms / nb_n if nb_n else None
+    nb_bytes = N * (24 + 13 + 4 * (n_nb + 1))
+    vf = prof.get("valu_issue_frac")
+    limiter = ("valu" if vf is not None and vf > achieved / HBM_PEAK_GBS else "hbm")
+    cell_segments = list(w.neighbour.segments)
+    rebuilds = int(w.neighbour.rebuilds)
+    nwave = int(len(w.wavevector.value))
     thermo = w.thermodynamics
     en_tot = thermo.en_tot()
+    # the Hilbert sort (every 100 steps) and the density modes (every 1000
+    # steps) amortised per step, when the timed window holds fewer steps
+    import halmd_amd._lib as _hl
+    _hl.call("halmd_device_synchronize")
+    t0 = time.perf_counter()
+    for _ in range(3):
+        w.density_mode.acquire()
+    modes_ms = (time.perf_counter() - t0) / 3 * 1e3
+    sort_ms = kt["sort"][0] / max(kt["sort"][1], 1)
+    amortised = ms / args.steps
+    if args.steps < SORT_INTERVAL:
+        amortised += sort_ms / SORT_INTERVAL
+    if args.steps < MODES_INTERVAL:
+        amortised += modes_ms / MODES_INTERVAL
 
     cpu = None
     if not args.no_cpu_baseline:
-        log("CPU baseline (FP64 oracle, 1 thread)")
-        cpu = cpu_baseline(w, args.cpu_seconds)
+        log(f"CPU baseline (FP64 oracle, 1 thread, {args.cpu_steps} steps)")
+        cpu = cpu_baseline(w, args.cpu_steps)
         cpu["host_cpus"] = os.cpu_count()
+
+    extras = {}
+    if not args.no_extras:
+        del w
+        log("batched ensemble figure")
+        extras["batched"] = batched_run(args.systems, args.batched_steps, 100, args.seed)
+        log("served stream figure")
+        extras["serving"] = serving_run(args.rate, args.duration, args.seed, args.step_scale, args.tick,
+                                        args.max_jobs, args.budget_gib)
+        log("reference benchmark setups")
+        extras["reference_benchmarks"] = {name: reference_setup_run(name, 2000, 200, args.seed)
+                                          for name in REFERENCE_SETUPS}
 
     out = {
         "metric": "particle-steps/s, LJ N=1M",
@@ -468,15 +603,22 @@ def main():
                         "skin=0.3, NVE dt=0.005, Hilbert sort every 100 steps, density modes every 1000 steps",
             "nparticle": N,
             "precision": "FP32 pair forces, float-float positions/velocities, FP64 reductions",
-            "wavevectors": int(len(w.wavevector.value)),
+            "wavevectors": nwave,
             "neighbours_mean": n_nb,
-            "neighbour_rebuilds": int(w.neighbour.rebuilds),
-            "cell_segments": list(w.neighbour.segments),
+            "neighbour_rebuilds": rebuilds,
+            "cell_segments": cell_segments,
             "en_tot_per_particle_end": en_tot,
             "device": arch,
+            "ms_per_step_amortised": amortised,
+            "amortised": f"Hilbert sort {sort_ms:.3f} ms / {SORT_INTERVAL} steps, density modes {modes_ms:.3f} ms "
+                         f"/ {MODES_INTERVAL} steps, added where the timed window holds fewer steps",
         },
         "roofline": {
+            # the roofline is HBM (no MFMA on this path); what limits the kernel
+            # is read from its counters: VALU issue against the measured rate
+            # of the VALU probe, or HBM
             "bound": "hbm",
+            "limiter": limiter,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -485,13 +627,24 @@ def main():
             "kernel": kname,
             "bytes_per_launch": force_bytes,
             "avg_ms": f_avg,
-            # from the counter summary: fraction of the VALU issue rate
-            # (wave64 instruction = 4 cycles on 1024 SIMDs) the kernel used
+            # from the counter summary: VALU wave instructions per second of
+            # the kernel over those of the VALU probe in the same profile
             "valu_issue_frac": prof.get("valu_issue_frac"),
             "counters": os.path.relpath(TRAFFIC_PROFILE, ROOT) if prof else None,
+            "counter_shape": prof.get("segments"),
+        },
+        "neighbour": {
+            # the list build, per SURVEY-fixed algorithmic bytes: r 24 B, cell
+            # slots ~13 B, list words (n_nb + 1) 4 B per particle
+            "avg_ms": nb_avg,
+            "bytes_per_launch": nb_bytes,
+            "achieved": nb_bytes / (nb_avg * 1e-3) / 1e9 if nb_avg else None,
+            "frac": nb_bytes / (nb_avg * 1e-3) / 1e9 / HBM_PEAK_GBS if nb_avg else None,
+            "unit": "GB/s",
         },
         "kernels": kernels,
         "cpu_baseline": cpu,
+        **extras,
     }
     print(json.dumps(out), flush=True)
 

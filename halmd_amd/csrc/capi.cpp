@@ -116,6 +116,26 @@ HALMD_API int halmd_device_copy(void* dst, const void* src, size_t bytes) {
     });
 }
 
+HALMD_API int halmd_device_valu_probe(unsigned blocks, unsigned iterations, uint64_t* instructions, double* ms) {
+    return guard([&] {
+        require(blocks > 0 && iterations > 0, "valu_probe: blocks and iterations must be positive");
+        dbuf<float> out(blocks);
+        hipEvent_t e[2];
+        HIP_CHECK(hipEventCreate(&e[0]));
+        HIP_CHECK(hipEventCreate(&e[1]));
+        HIP_CHECK(hipEventRecord(e[0], nullptr));
+        uint64_t const n = launch_valu_probe(out.get(), blocks, iterations, nullptr);
+        HIP_CHECK(hipEventRecord(e[1], nullptr));
+        HIP_CHECK(hipEventSynchronize(e[1]));
+        float t = 0;
+        HIP_CHECK(hipEventElapsedTime(&t, e[0], e[1]));
+        (void)hipEventDestroy(e[0]);
+        (void)hipEventDestroy(e[1]);
+        if (instructions) *instructions = n;
+        if (ms) *ms = t;
+    });
+}
+
 // ---- algorithm ---------------------------------------------------------------
 HALMD_API int halmd_algorithm_radix_sort(uint32_t* keys, uint32_t* values, size_t n, unsigned key_bits) {
     return guard([&] {

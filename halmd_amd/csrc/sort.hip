@@ -174,6 +174,26 @@ __global__ __launch_bounds__(256) void copy_kernel(uint4 const* __restrict__ src
     if (i < n) dst[i] = src[i];
 }
 
+// VALU issue-rate probe: each lane runs `iterations` rounds of VALU_PROBE_FMA
+// independent FP32 fused multiply-adds, with no memory traffic but one
+// guarded store; the wave instruction count is known, so the counts and
+// durations of other kernels can be set against the rate a pure VALU stream
+// reaches on this device
+constexpr int VALU_PROBE_FMA = 16;
+__global__ __launch_bounds__(256) void valu_probe_kernel(float* out, unsigned iterations, float a, float b) {
+    float x[VALU_PROBE_FMA];
+#pragma unroll
+    for (int k = 0; k < VALU_PROBE_FMA; ++k) x[k] = float(threadIdx.x) * 1e-3f + float(k);
+    for (unsigned i = 0; i < iterations; ++i) {
+#pragma unroll
+        for (int k = 0; k < VALU_PROBE_FMA; ++k) x[k] = __builtin_fmaf(x[k], a, b);
+    }
+    float sum = 0;
+#pragma unroll
+    for (int k = 0; k < VALU_PROBE_FMA; ++k) sum += x[k];
+    if (sum == -1.0f) out[blockIdx.x] = sum;   // keeps the chains live (never taken for a, b in (0, 1))
+}
+
 __global__ void fill_u32_kernel(uint32_t* out, uint32_t value, size_t n) {
     size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) out[i] = value;
@@ -346,6 +366,12 @@ void launch_copy(void const* src, void* dst, size_t bytes, hipStream_t s) {
     if (!n) return;
     copy_kernel<<<div_up(n, 256), 256, 0, s>>>(static_cast<uint4 const*>(src), static_cast<uint4*>(dst), n);
     HIP_CHECK(hipGetLastError());
+}
+
+uint64_t launch_valu_probe(float* out, unsigned blocks, unsigned iterations, hipStream_t s) {
+    valu_probe_kernel<<<blocks, 256, 0, s>>>(out, iterations, 0.999f, 0.001f);
+    HIP_CHECK(hipGetLastError());
+    return uint64_t(blocks) * (256 / WAVE) * iterations * VALU_PROBE_FMA;
 }
 
 void launch_fill_u32(uint32_t* out, uint32_t value, size_t n, hipStream_t s) {

@@ -71,6 +71,11 @@ HALMD_API int halmd_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 /* streaming device-to-device copy kernel (bytes a multiple of 16): the
  * reference traffic for calibrating hardware memory counters */
 HALMD_API int halmd_device_copy(void* dst, const void* src, size_t bytes);
+/* VALU rate probe: `blocks` workgroups of 256 threads, each lane `iterations`
+ * rounds of 16 independent FP32 FMAs (no memory traffic); returns the FMA
+ * wave-instruction count and the duration (HIP events): the reference rate
+ * for calibrating hardware VALU counters */
+HALMD_API int halmd_device_valu_probe(unsigned blocks, unsigned iterations, uint64_t* instructions, double* ms);
 
 /* ---- algorithm (kernel seam; device pointers, synchronous) -------------- */
 /* stable sort of (key, value) by the low key_bits bits of key;
