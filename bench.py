@@ -416,7 +416,7 @@ def reference_setup_run(name, steps, warmup, seed):
     else:
         w = workloads.kob_andersen(c["n"], density=c["density"], temperature=0.7, skin=c["skin"],
                                    timestep=c["timestep"], seed=seed)
-    w.force.tune()
+    w.core.tune()
     w.sampler.run(warmup)
     ms = w.sampler.run_timed(steps)
     ns = ms / steps * 1e6 / c["n"]
@@ -505,7 +505,7 @@ def main():
         w.neighbour.set_segments(*[int(x) for x in args.segments.split(",")])
         log(f"cell segments set to {w.neighbour.segments} (cells, rows, workgroup size)")
     elif not args.no_tune:
-        cells = w.force.tune()
+        w.core.tune()
         log(f"autotuned cell segments: {w.neighbour.segments} (cells, rows, workgroup size)")
     log(f"setup {time.perf_counter() - t0:.1f} s; warmup {args.warmup} steps")
     w.sampler.run(args.warmup)

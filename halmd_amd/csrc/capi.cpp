@@ -674,6 +674,9 @@ HALMD_API int halmd_mdsim_core_step(const halmd_core* c, uint64_t* step) {
 HALMD_API int halmd_mdsim_core_set_step(halmd_core* c, uint64_t step) {
     return guard([&] { as<core>(c)->step = step; });
 }
+HALMD_API int halmd_mdsim_core_tune(halmd_core* c, unsigned steps) {
+    return guard([&] { as<core>(c)->tune_segments(steps); });
+}
 
 // ---- observables ------------------------------------------------------------------------------
 HALMD_API int halmd_observables_phase_space_create(halmd_particle* p, const halmd_box* b, halmd_phase_space** out) {

@@ -522,6 +522,12 @@ public:
     uint64_t step = 0;
     bool wired = false;
     void mdstep();
+    // choose the cell-segment shape of the particle's pair force by timing
+    // `steps` whole MD steps per candidate shape (the step the run will
+    // take, fused or not), each from the same snapshot of the state; the
+    // state is restored at the end (plain NVE Verlet with one pair force over
+    // cell-ordered lists)
+    void tune_segments(unsigned steps);
 };
 
 // ---------------------------------------------------------------------------

@@ -408,6 +408,12 @@ class core(Handle):
     def step(self, value):
         call("halmd_mdsim_core_set_step", self.handle, int(value))
 
+    def tune(self, steps=40):
+        """Choose the cell-segment shape of the pair force by timing `steps`
+        MD steps per candidate shape from one snapshot of the state (restored
+        afterwards): the project's autotuner of the step the run takes."""
+        call("halmd_mdsim_core_tune", self.handle, int(steps))
+
     @property
     def time(self):
         """Simulation time of the current step (halmd/mdsim/clock.hpp): the

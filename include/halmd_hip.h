@@ -356,6 +356,12 @@ HALMD_API int halmd_mdsim_core_step(const halmd_core* c, uint64_t* step);
 /* the step counter of a run restarted at a saved step (halmd/mdsim/clock.hpp
  * advance(): the sort schedule counts from it) */
 HALMD_API int halmd_mdsim_core_set_step(halmd_core* c, uint64_t step);
+/* The project's autotuner (no counterpart in the reference, whose sizing is
+ * fixed: binning.cpp:81-101): the cell-segment shape of the particle's pair
+ * force, chosen by timing `steps` MD steps of each candidate shape from one
+ * snapshot of the state, which is restored at the end (plain NVE Verlet,
+ * one pair force over cell-ordered lists) */
+HALMD_API int halmd_mdsim_core_tune(halmd_core* c, unsigned steps);
 
 /* ---- observables.thermodynamics: halmd/observables/gpu/thermodynamics.cpp:67-191,
  *      halmd/observables/thermodynamics.hpp:80-90 (mean values per particle) -- */

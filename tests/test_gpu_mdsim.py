@@ -160,6 +160,31 @@ def test_autotune_keeps_parity(oracle):
     assert np.array_equal(entries, olists)
 
 
+def test_step_tuner_restores_the_state(oracle):
+    # core.tune times MD steps of each candidate shape from a snapshot: the
+    # state, step counter and physics afterwards are those before tuning
+    w = workloads.lennard_jones(13500, density=0.8442, temperature=1.0, jitter=0.1, seed=37)
+    w.sampler.run(15)
+    step = w.core.step
+    r0, v0, im0 = (w.particle.ordered(k) for k in ("position", "velocity", "image"))
+    e0 = w.thermodynamics.en_tot()
+    w.core.tune(steps=8)
+    S, R, T = w.neighbour.segments
+    assert S >= 1 and R >= 1 and T in (128, 192, 256)
+    assert w.core.step == step
+    for k, a in (("position", r0), ("velocity", v0), ("image", im0)):
+        assert np.array_equal(w.particle.ordered(k), a), k
+    # (the energy sums follow the lists of the new shape: FP32 summation order)
+    assert w.thermodynamics.en_tot() == pytest.approx(e0, rel=1e-7)
+    # the run continues: forces against FP64 and the rules of the lists
+    f = w.particle.ordered("force")
+    pot = oracle.Potential(oracle.POT_LJ, "shifted", 1.0, 1.0, 2.5)
+    fref, _, _ = oracle.pair_force(r0, w.particle.ordered("species"), w.length, pot, skin=0.3)
+    assert _force_error(f, fref) < FORCE_TOL
+    w.sampler.run(30)
+    assert abs(w.thermodynamics.en_tot() - e0) < 2e-3
+
+
 @pytest.mark.parametrize("n,rho", [(500, 1.2), (256, 0.8)])
 def test_from_particle_small_box(oracle, n, rho):
     # boxes under 3 cells of rc + skin: all-pairs lists (ascending j, the
