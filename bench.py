@@ -126,7 +126,9 @@ def counter_summary(kernel):
             k, v = line[2:].split(":", 1)
             try:
                 out[k.strip()] = float(v.split()[0])
-            except (ValueError, IndexError):
+            except ValueError:
+                out[k.strip()] = v.strip()
+            except IndexError:
                 pass
     return out
 
