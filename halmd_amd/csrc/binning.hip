@@ -401,7 +401,7 @@ __device__ __forceinline__ void staged_build_segment(
     uint4 const* __restrict__ posq, uint32_t const* __restrict__ off, seg_params const& sp, unsigned seg,
     neighbour_params const& p, unsigned cap, uint16_t* __restrict__ list, uint32_t* __restrict__ count,
     uint32_t* __restrict__ stats, uint4* s_lds, seg_stage& st, unsigned* s_bound, uint32_t* s_rr,
-    unsigned bstride = SEG_BOUND)
+    unsigned bstride = SEG_BOUND, uint32_t* overflow = nullptr)
 {
     using rec_t = typename std::conditional<ISO, uint2, uint4>::type;
     rec_t* const s_rec = reinterpret_cast<rec_t*>(s_lds);
@@ -409,7 +409,10 @@ __device__ __forceinline__ void staged_build_segment(
         s_rr[threadIdx.x] = ISO ? p.rr16[threadIdx.x] : __float_as_uint(p.rr_cut_skin[threadIdx.x]);
     unsigned const U = stage_union(sp, seg, off, posq, s_rec, st, s_bound, bstride);
     if (threadIdx.x == 0) atomicMax(&stats[1], U + 1);
-    if (U + 1 > sp.lds_cap) return;   // uniform: the caller enlarges the staging and rebuilds
+    if (U + 1 > sp.lds_cap) {   // uniform: the caller enlarges the staging and rebuilds
+        if (overflow && threadIdx.x == 0) atomicOr(overflow, ENS_SKIP_OVERFLOW);
+        return;
+    }
 
     unsigned const nx = p.grid.ncell[0];
     uint32_t const rr0 = ISO ? p.rr16[0] : __float_as_uint(p.rr_cut_skin[0]);
@@ -462,6 +465,7 @@ __device__ __forceinline__ void staged_build_segment(
     }
     for (int o = WAVE / 2; o > 0; o >>= 1) cmax = max(cmax, (unsigned)__shfl_xor((int)cmax, o, WAVE));
     if ((threadIdx.x % WAVE) == 0 && cmax > 0) atomicMax(&stats[0], cmax);
+    if (overflow && (threadIdx.x % WAVE) == 0 && cmax > cap) atomicOr(overflow, ENS_SKIP_OVERFLOW);
 }
 
 template <bool MULTI, bool ISO>
@@ -777,20 +781,15 @@ __global__ __launch_bounds__(SEG_THREADS_MAX) void ens_build_kernel(
                                       list + d.list_off, count + d.base, stats + 2 * sys, s_rec, st, s_bound, s_rr);
 }
 
-// all-pairs lists of small systems: workgroup (x, y) builds the lists of
-// particles y ENS_SMALL_P .. of system rsmall[x], with the whole system staged
-// in LDS; one wave per particle, lane l testing j = base + l, a ballot giving
-// each hit its rank (entries are particle indices of the system, ascending)
-__global__ __launch_bounds__(256) void ens_build_small_kernel(uint4 const* __restrict__ posq,
-                                                              ens_desc const* __restrict__ descs,
-                                                              uint32_t const* __restrict__ rsmall,
-                                                              uint16_t* __restrict__ list,
-                                                              uint32_t* __restrict__ count,
-                                                              uint32_t* __restrict__ stats) {
-    extern __shared__ uint4 s_rec[];
-    unsigned const sys = rsmall[blockIdx.x];
-    ens_desc const& d = descs[sys];
-    unsigned const i0 = blockIdx.y * ENS_SMALL_P;
+// all-pairs lists of small systems: the workgroup builds the lists of
+// particles y ENS_SMALL_P .. of system sys, with the whole system staged in
+// LDS; one wave per particle, lane l testing j = base + l, a ballot giving each
+// hit its rank (entries are particle indices of the system, ascending)
+__device__ __forceinline__ void ens_build_small_group(uint4 const* __restrict__ posq, ens_desc const& d, unsigned sys,
+                                                      unsigned y, uint16_t* __restrict__ list,
+                                                      uint32_t* __restrict__ count, uint32_t* __restrict__ stats,
+                                                      uint4* s_rec, uint32_t* overflow) {
+    unsigned const i0 = y * ENS_SMALL_P;
     if (i0 >= d.n) return;   // the whole workgroup
     unsigned const n = d.n, cap = d.cap;
     for (unsigned t = threadIdx.x; t < n; t += blockDim.x) s_rec[t] = posq[d.base + t];
@@ -819,7 +818,447 @@ __global__ __launch_bounds__(256) void ens_build_small_kernel(uint4 const* __res
         cmax = max(cmax, c);
     }
     if (lane == 0 && cmax > 0) atomicMax(&stats[2 * sys], cmax);
-    if (blockIdx.y == 0 && threadIdx.x == 0) atomicMax(&stats[2 * sys + 1], n + 1);
+    if (overflow && lane == 0 && cmax > cap) atomicOr(overflow, ENS_SKIP_OVERFLOW);
+    if (y == 0 && threadIdx.x == 0) atomicMax(&stats[2 * sys + 1], n + 1);
+}
+
+// workgroup (x, y): system rsmall[x], group y
+__global__ __launch_bounds__(256) void ens_build_small_kernel(uint4 const* __restrict__ posq,
+                                                              ens_desc const* __restrict__ descs,
+                                                              uint32_t const* __restrict__ rsmall,
+                                                              uint16_t* __restrict__ list,
+                                                              uint32_t* __restrict__ count,
+                                                              uint32_t* __restrict__ stats) {
+    extern __shared__ uint4 s_rec[];
+    unsigned const sys = rsmall[blockIdx.x];
+    ens_build_small_group(posq, descs[sys], sys, blockIdx.y, list, count, stats, s_rec, nullptr);
+}
+
+// --- ensembles: rebuilds decided on the device ---------------------------
+
+// k with pre[k] <= g < pre[k + 1], for g < pre[m] (pre ascending, pre[0] = 0)
+__device__ __forceinline__ unsigned work_find(uint32_t const* __restrict__ pre, unsigned m, uint32_t g) {
+    unsigned lo = 0, hi = m;   // pre[lo] <= g < pre[hi]
+    while (hi - lo > 1) {
+        unsigned const mid = (lo + hi) / 2;
+        if (pre[mid] <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// exclusive scans of V values over the workgroup (blockDim.x a multiple of
+// 64, at most 1024): e[k] the prefix of v[k], t[k] its total.  s_tmp: 17 V
+// words of LDS.  Two barriers (and one more before s_tmp is reused).
+template <int V>
+__device__ __forceinline__ void block_exclusive_scan_n(uint32_t const (&v)[V], uint32_t (&e)[V], uint32_t (&t)[V],
+                                                       uint32_t* s_tmp) {
+    unsigned const lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+    uint32_t x[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        x[k] = v[k];
+        for (unsigned o = 1; o < WAVE; o <<= 1) {
+            uint32_t const y = __shfl_up(x[k], o, WAVE);
+            if (lane >= o) x[k] += y;
+        }
+    }
+    if (lane == WAVE - 1)
+#pragma unroll
+        for (int k = 0; k < V; ++k) s_tmp[wid * V + k] = x[k];
+    __syncthreads();
+    if (threadIdx.x < unsigned(V)) {   // value k: exclusive scan over the waves
+        uint32_t acc = 0;
+        for (unsigned w = 0; w < nw; ++w) {
+            uint32_t const y = s_tmp[w * V + threadIdx.x];
+            s_tmp[w * V + threadIdx.x] = acc;
+            acc += y;
+        }
+        s_tmp[16 * V + threadIdx.x] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        e[k] = s_tmp[wid * V + k] + x[k] - v[k];
+        t[k] = s_tmp[16 * V + k];
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp, uint32_t& total) {
+    uint32_t const vv[1] = {v};
+    uint32_t e[1], t[1];
+    block_exclusive_scan_n<1>(vv, e, t, s_tmp);
+    total = t[0];
+    return e[0];
+}
+
+// one workgroup of 1024 threads
+__global__ __launch_bounds__(1024) void ens_decide_kernel(ens_desc const* __restrict__ descs, unsigned nsys,
+                                                          uint32_t const* __restrict__ skip,
+                                                          uint32_t* __restrict__ disp, uint32_t* __restrict__ steps,
+                                                          uint32_t* __restrict__ rebuilds,
+                                                          uint32_t* __restrict__ stats, ens_work w) {
+    constexpr int V = 8;   // cell systems: flag, cells, blocks, segments, cell blocks; all-pairs: flag, blocks, groups
+    __shared__ uint32_t s_tmp[17 * V];
+    uint32_t carry[V] = {};
+    for (unsigned base = 0; base < nsys; base += blockDim.x) {
+        unsigned const s = base + threadIdx.x;
+        uint32_t v[V] = {};
+        if (s < nsys) {
+            uint32_t* const slots = disp + size_t(s) * ENS_DISP_WORDS;
+            uint32_t m = 0;
+#pragma unroll
+            for (unsigned k = 0; k < ENS_DISP_SLOTS; ++k) m = max(m, slots[k * ENS_DISP_STRIDE]);
+#pragma unroll
+            for (unsigned k = 0; k < ENS_DISP_SLOTS; ++k) slots[k * ENS_DISP_STRIDE] = 0u;
+            ens_desc const& d = descs[s];
+            if (!skip[s]) {
+                ++steps[s];
+                if (m > d.keep_bits) {   // bits of non-negative floats order as the floats
+                    ++rebuilds[s];
+                    stats[2 * s] = 0u;
+                    stats[2 * s + 1] = 0u;
+                    unsigned const nblk = (d.n + ENS_BLOCK - 1) / ENS_BLOCK;
+                    if (d.small) {
+                        v[5] = 1; v[6] = nblk; v[7] = (d.n + ENS_SMALL_P - 1) / ENS_SMALL_P;
+                    } else {
+                        v[0] = 1; v[1] = d.ncells; v[2] = nblk; v[3] = d.sp.nseg;
+                        v[4] = (d.ncells + ENS_CELL_BLOCK - 1) / ENS_CELL_BLOCK;
+                    }
+                }
+            }
+        }
+        uint32_t e[V], t[V];
+        block_exclusive_scan_n<V>(v, e, t, s_tmp);
+        if (v[0]) {
+            unsigned const r = carry[0] + e[0];
+            w.lsys[r] = s;
+            w.lcell[r] = carry[1] + e[1];
+            w.lblk[r] = carry[2] + e[2];
+            w.lseg[r] = carry[3] + e[3];
+            w.lcb[r] = carry[4] + e[4];
+        }
+        if (v[5]) {
+            unsigned const r = carry[5] + e[5];
+            w.ssys[r] = s;
+            w.sblk[r] = carry[6] + e[6];
+            w.sgrp[r] = carry[7] + e[7];
+        }
+#pragma unroll
+        for (int k = 0; k < V; ++k) carry[k] += t[k];
+    }
+    if (threadIdx.x == 0) {
+        w.head[0] = carry[0];
+        w.head[1] = carry[5];
+        w.lcell[carry[0]] = carry[1];
+        w.lblk[carry[0]] = carry[2];
+        w.lseg[carry[0]] = carry[3];
+        w.lcb[carry[0]] = carry[4];
+        w.sblk[carry[5]] = carry[6];
+        w.sgrp[carry[5]] = carry[7];
+    }
+}
+
+__device__ __forceinline__ void cell_coords(unsigned c, unsigned nc, unsigned& x, unsigned& y, unsigned& z) {
+    x = c % nc;
+    unsigned const r = c / nc;
+    y = r % nc;
+    z = r / nc;
+}
+
+// a - b on a ring of nc >= 3 cells, as -1, 0, +1 (or 2 when farther)
+__device__ __forceinline__ int ring_step(unsigned a, unsigned b, unsigned nc) {
+    int d = int(a) - int(b);
+    if (d > 1) d -= int(nc);
+    if (d < -1) d += int(nc);
+    return (d >= -1 && d <= 1) ? d : 2;
+}
+
+__device__ __forceinline__ unsigned ring_add(unsigned a, int e, unsigned nc) {
+    int const r = int(a) + e;
+    return unsigned(r < 0 ? r + int(nc) : (r >= int(nc) ? r - int(nc) : r));
+}
+
+// the offsets -1, 0, +1 along one ring in the ascending order of a + e
+__device__ __forceinline__ void ascending_steps(unsigned a, unsigned nc, int o[3]) {
+    if (a == 0) { o[0] = 0; o[1] = 1; o[2] = -1; }
+    else if (a == nc - 1) { o[0] = 1; o[1] = -1; o[2] = 0; }
+    else { o[0] = -1; o[1] = 0; o[2] = 1; }
+}
+
+__device__ __forceinline__ unsigned step_code(int ex, int ey, int ez) { return unsigned((ez + 1) * 9 + (ey + 1) * 3 + (ex + 1)); }
+
+// waves over runs of consecutive old cells of the flagged cell systems
+__global__ __launch_bounds__(256) void ens_rank_kernel(uint4 const* __restrict__ posq,
+                                                       uint32_t const* __restrict__ offsets,
+                                                       ens_desc const* __restrict__ descs, uint32_t* skip,
+                                                       ens_work w, uint32_t* __restrict__ keys,
+                                                       uint32_t* __restrict__ vals, uint32_t* __restrict__ cnt27) {
+    unsigned const nl = w.head[0];
+    uint32_t const total = w.lcell[nl];
+    unsigned const lane = threadIdx.x % WAVE;
+    unsigned const waves = gridDim.x * (blockDim.x / WAVE);
+    unsigned const wave = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    uint64_t const below = (uint64_t(1) << lane) - 1u;
+    uint32_t const run = (total + waves - 1) / waves;
+    uint32_t const g0 = min(total, wave * run), g1 = min(total, g0 + run);
+    if (g0 >= g1) return;   // the whole wave
+    unsigned k = work_find(w.lcell, nl, g0);
+    for (uint32_t g = g0; g < g1; ++g) {   // wave-uniform
+        while (g >= w.lcell[k + 1]) ++k;
+        unsigned const s = w.lsys[k];
+        ens_desc const& d = descs[s];
+        unsigned const cp = g - w.lcell[k], nc = d.sp.grid.ncell[0];
+        unsigned xo, yo, zo;
+        cell_coords(cp, nc, xo, yo, zo);
+        uint32_t const* const off = offsets + d.cell_base;
+        unsigned const b = off[cp], e = off[cp + 1];
+        unsigned cnt = 0;   // lane l < 27: particles so far with code l
+        bool far = false;
+        for (unsigned i0 = b; i0 < e; i0 += WAVE) {
+            unsigned const i = i0 + lane;
+            bool const act = i < e;
+            unsigned code = 31u, c = 0;
+            if (act) {
+                uint4 const q = posq[d.base + i];
+                unsigned const x = cell_component(q.x, nc), y = cell_component(q.y, nc), z = cell_component(q.z, nc);
+                c = (z * nc + y) * nc + x;
+                int const ex = ring_step(xo, x, nc), ey = ring_step(yo, y, nc), ez = ring_step(zo, z, nc);
+                if (ex == 2 || ey == 2 || ez == 2)
+                    far = true;
+                else
+                    code = step_code(ex, ey, ez);
+            }
+            uint64_t peers = ~uint64_t(0);
+#pragma unroll
+            for (unsigned bit = 0; bit < 5; ++bit) {
+                bool const on = (code >> bit) & 1u;
+                uint64_t const m = __ballot(on);
+                peers &= on ? m : ~m;
+            }
+            unsigned const rank = unsigned(__popcll(peers & below));
+            unsigned const base = unsigned(__shfl(int(cnt), int(code & 31u), WAVE));
+            if (act && code < 27u) {
+                keys[d.base + i] = c | (code << ENS_CELL_KEY_BITS);
+                vals[d.base + i] = base + rank;
+            }
+            for (unsigned l = 0; l < 27; ++l) {
+                uint64_t const m = __ballot(code == l);
+                if (lane == l) cnt += unsigned(__popcll(m));
+            }
+        }
+        if (__ballot(far) != 0 && lane == 0) atomicOr(&skip[s], ENS_SKIP_FAR);
+        if (lane < 27) cnt27[(size_t(d.cell_base) + cp) * 27 + lane] = cnt;
+    }
+}
+
+// one thread per new cell of the flagged cell systems (workgroups of
+// ENS_CELL_BLOCK cells of one system): the particles that reach cell c from
+// each of its 27 source cells, in ascending source index, as running sums
+// (base27), their total scanned within the block (offsets_t) and the block
+// sums (bsum, one per block of the work table)
+__global__ __launch_bounds__(ENS_CELL_BLOCK) void ens_cell_count_kernel(
+    ens_desc const* __restrict__ descs, uint32_t const* skip, ens_work w, uint32_t const* __restrict__ cnt27,
+    uint32_t* __restrict__ base27, uint32_t* __restrict__ offsets_t, uint32_t* __restrict__ bsum) {
+    __shared__ uint32_t s_tmp[17];
+    unsigned const nl = w.head[0];
+    uint32_t const total = w.lcb[nl];
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {   // workgroup-uniform
+        unsigned const k = work_find(w.lcb, nl, g);
+        unsigned const s = w.lsys[k];
+        if (skip[s]) continue;   // (no writer of skip runs during this kernel: uniform)
+        ens_desc const& d = descs[s];
+        unsigned const C = d.ncells, nc = d.sp.grid.ncell[0];
+        unsigned const c = (g - w.lcb[k]) * ENS_CELL_BLOCK + threadIdx.x;
+        uint32_t v = 0;
+        if (c < C) {
+            uint32_t const* const cnt = cnt27 + size_t(d.cell_base) * 27;
+            uint32_t* const out = base27 + (size_t(d.cell_base) + c) * 27;
+            unsigned x, y, z;
+            cell_coords(c, nc, x, y, z);
+            int ox[3], oy[3], oz[3];
+            ascending_steps(x, nc, ox);
+            ascending_steps(y, nc, oy);
+            ascending_steps(z, nc, oz);
+            uint32_t n27[27];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b)
+#pragma unroll
+                    for (int e = 0; e < 3; ++e) {
+                        unsigned const src =
+                            (ring_add(z, oz[a], nc) * nc + ring_add(y, oy[b], nc)) * nc + ring_add(x, ox[e], nc);
+                        n27[a * 9 + b * 3 + e] = cnt[size_t(src) * 27 + step_code(ox[e], oy[b], oz[a])];
+                    }
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b)
+#pragma unroll
+                    for (int e = 0; e < 3; ++e) {
+                        out[step_code(ox[e], oy[b], oz[a])] = v;
+                        v += n27[a * 9 + b * 3 + e];
+                    }
+        }
+        uint32_t tot;
+        uint32_t const ex = block_exclusive_scan(v, s_tmp, tot);
+        if (c < C) offsets_t[d.cell_base + c] = ex;
+        if (threadIdx.x == 0) bsum[g] = tot;
+    }
+}
+
+// the block sums of the earlier blocks of the same system added; the last
+// block also writes offsets_t[n_cells] = n
+__global__ __launch_bounds__(ENS_CELL_BLOCK) void ens_cell_carry_kernel(ens_desc const* __restrict__ descs,
+                                                                        uint32_t const* skip, ens_work w,
+                                                                        uint32_t const* __restrict__ bsum,
+                                                                        uint32_t* __restrict__ offsets_t) {
+    __shared__ uint32_t s_tmp[17];
+    unsigned const nl = w.head[0];
+    uint32_t const total = w.lcb[nl];
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {   // workgroup-uniform
+        unsigned const k = work_find(w.lcb, nl, g);
+        unsigned const s = w.lsys[k];
+        if (skip[s]) continue;
+        ens_desc const& d = descs[s];
+        unsigned const cb = g - w.lcb[k], C = d.ncells;
+        uint32_t part = 0;
+        for (unsigned j = threadIdx.x; j < cb; j += blockDim.x) part += bsum[w.lcb[k] + j];
+        uint32_t carry;
+        (void)block_exclusive_scan(part, s_tmp, carry);   // carry: the sum over the workgroup
+        unsigned const c = cb * ENS_CELL_BLOCK + threadIdx.x;
+        if (c < C) offsets_t[d.cell_base + c] += carry;
+        if (threadIdx.x == 0 && (cb + 1) * ENS_CELL_BLOCK >= C) offsets_t[d.cell_base + C] = carry + bsum[g];
+    }
+}
+
+// each particle of the flagged cell systems to its slot in the new order
+__global__ __launch_bounds__(256) void ens_scatter_kernel(
+    float4 const* __restrict__ pos, float4 const* __restrict__ vel, int4 const* __restrict__ image,
+    uint32_t const* __restrict__ id, uint4 const* __restrict__ posq, float4* __restrict__ pos_t,
+    float4* __restrict__ vel_t, int4* __restrict__ image_t, uint32_t* __restrict__ id_t, uint4* __restrict__ posq_t,
+    ens_desc const* __restrict__ descs, uint32_t const* skip, ens_work w, uint32_t const* __restrict__ keys,
+    uint32_t const* __restrict__ vals, uint32_t const* __restrict__ base27, uint32_t const* __restrict__ offsets_t) {
+    unsigned const nl = w.head[0];
+    uint32_t const total = w.lblk[nl];
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {   // workgroup-uniform
+        unsigned const k = work_find(w.lblk, nl, g);
+        unsigned const s = w.lsys[k];
+        if (skip[s]) continue;
+        ens_desc const& d = descs[s];
+        unsigned const li = (g - w.lblk[k]) * ENS_BLOCK + threadIdx.x;
+        if (li >= d.n) continue;
+        uint32_t const key = keys[d.base + li];
+        unsigned const c = key & ((1u << ENS_CELL_KEY_BITS) - 1u), code = key >> ENS_CELL_KEY_BITS;
+        size_t const cc = size_t(d.cell_base) + c;
+        size_t const dst = d.base + offsets_t[cc] + base27[cc * 27 + code] + vals[d.base + li];
+        size_t const src = d.base + li;
+        pos_t[2 * dst] = pos[2 * src];
+        pos_t[2 * dst + 1] = pos[2 * src + 1];
+        vel_t[2 * dst] = vel[2 * src];
+        vel_t[2 * dst + 1] = vel[2 * src + 1];
+        image_t[dst] = image[src];
+        id_t[dst] = id[src];
+        posq_t[dst] = posq[src];
+    }
+}
+
+// twins back into place, r0 := new positions (hi words), new cell offsets
+__global__ __launch_bounds__(256) void ens_copyback_flagged_kernel(
+    float4* __restrict__ pos, float4* __restrict__ vel, int4* __restrict__ image, uint32_t* __restrict__ id,
+    uint4* __restrict__ posq, float4* __restrict__ r0, float4 const* __restrict__ pos_t,
+    float4 const* __restrict__ vel_t, int4 const* __restrict__ image_t, uint32_t const* __restrict__ id_t,
+    uint4 const* __restrict__ posq_t, uint32_t* __restrict__ offsets, uint32_t const* __restrict__ offsets_t,
+    ens_desc const* __restrict__ descs, uint32_t const* skip, ens_work w) {
+    unsigned const nl = w.head[0];
+    uint32_t const total = w.lblk[nl];
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {   // workgroup-uniform
+        unsigned const k = work_find(w.lblk, nl, g);
+        unsigned const s = w.lsys[k];
+        if (skip[s]) continue;
+        ens_desc const& d = descs[s];
+        unsigned const lb = g - w.lblk[k], nblk = (d.n + ENS_BLOCK - 1) / ENS_BLOCK;
+        // this block's share of the n_cells + 1 offsets
+        size_t const C1 = size_t(d.ncells) + 1;
+        size_t const c_lo = C1 * lb / nblk, c_hi = C1 * (lb + 1) / nblk;
+        for (size_t c = c_lo + threadIdx.x; c < c_hi; c += blockDim.x)
+            offsets[d.cell_base + c] = offsets_t[d.cell_base + c];
+        unsigned const li = lb * ENS_BLOCK + threadIdx.x;
+        if (li >= d.n) continue;
+        size_t const i = d.base + li;
+        float4 const h = pos_t[2 * i];
+        pos[2 * i] = h;
+        pos[2 * i + 1] = pos_t[2 * i + 1];
+        vel[2 * i] = vel_t[2 * i];
+        vel[2 * i + 1] = vel_t[2 * i + 1];
+        image[i] = image_t[i];
+        id[i] = id_t[i];
+        posq[i] = posq_t[i];
+        r0[i] = h;
+    }
+}
+
+// the staged list build over the flagged segments
+template <bool MULTI>
+__global__ __launch_bounds__(SEG_THREADS_MAX) void ens_build_flagged_kernel(
+    uint4 const* __restrict__ posq, uint32_t const* __restrict__ offsets, ens_desc const* __restrict__ descs,
+    uint32_t* skip, ens_work w, uint16_t* __restrict__ list, uint32_t* __restrict__ count,
+    uint32_t* __restrict__ stats)
+{
+    extern __shared__ uint4 s_rec[];
+    __shared__ seg_stage st;
+    __shared__ unsigned s_bound[SEG_NR * SEG_BOUND];
+    __shared__ uint32_t s_rr[MAX_TYPES * MAX_TYPES];
+    __shared__ uint32_t s_skip;
+    unsigned const nl = w.head[0];
+    uint32_t const total = w.lseg[nl];
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {   // workgroup-uniform
+        unsigned const k = work_find(w.lseg, nl, g);
+        unsigned const s = w.lsys[k];
+        // (other workgroups of this launch may set the system's overflow bit:
+        // one read, shared, keeps the branch uniform)
+        if (threadIdx.x == 0) s_skip = skip[s];
+        __syncthreads();
+        bool const skipped = s_skip != 0;
+        if (!skipped) {
+            ens_desc const& d = descs[s];
+            staged_build_segment<MULTI, true>(posq + d.base, offsets + d.cell_base, d.sp, g - w.lseg[k], d.np, d.cap,
+                                              list + d.list_off, count + d.base, stats + 2 * s, s_rec, st, s_bound,
+                                              s_rr, SEG_BOUND, &skip[s]);
+        }
+        __syncthreads();   // the next segment reuses the LDS
+    }
+}
+
+// r0 := positions of the flagged all-pairs systems
+__global__ __launch_bounds__(256) void ens_r0_flagged_kernel(float4 const* __restrict__ pos, float4* __restrict__ r0,
+                                                             ens_desc const* __restrict__ descs, ens_work w) {
+    unsigned const ns = w.head[1];
+    uint32_t const total = w.sblk[ns];
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+        unsigned const k = work_find(w.sblk, ns, g);
+        ens_desc const& d = descs[w.ssys[k]];
+        unsigned const li = (g - w.sblk[k]) * ENS_BLOCK + threadIdx.x;
+        if (li < d.n) r0[d.base + li] = pos[2 * size_t(d.base + li)];
+    }
+}
+
+__global__ __launch_bounds__(256) void ens_build_small_flagged_kernel(uint4 const* __restrict__ posq,
+                                                                      ens_desc const* __restrict__ descs,
+                                                                      uint32_t* skip, ens_work w,
+                                                                      uint16_t* __restrict__ list,
+                                                                      uint32_t* __restrict__ count,
+                                                                      uint32_t* __restrict__ stats) {
+    extern __shared__ uint4 s_rec[];
+    unsigned const ns = w.head[1];
+    uint32_t const total = w.sgrp[ns];
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {   // workgroup-uniform
+        unsigned const k = work_find(w.sgrp, ns, g);
+        unsigned const s = w.ssys[k];
+        ens_build_small_group(posq, descs[s], s, g - w.sgrp[k], list, count, stats, s_rec, &skip[s]);
+        __syncthreads();   // the next group restages the LDS
+    }
 }
 
 } // namespace
@@ -963,6 +1402,66 @@ void launch_ens_build(uint4 const* posq, uint32_t const* offsets, ens_desc const
     else
         ens_build_kernel<false><<<nrseg, threads, lds, s>>>(posq, offsets, descs, rseg, list, count, stats);
     HIP_CHECK(hipGetLastError());
+}
+
+ens_work ens_work_at(uint32_t* words, size_t nsys) {
+    ens_work w;
+    w.head = words;
+    uint32_t* p = words + 16;
+    size_t const m = nsys + 1;
+    w.lsys = p; p += m;
+    w.lcell = p; p += m;
+    w.lblk = p; p += m;
+    w.lseg = p; p += m;
+    w.lcb = p; p += m;
+    w.ssys = p; p += m;
+    w.sblk = p; p += m;
+    w.sgrp = p;
+    return w;
+}
+
+void launch_ens_decide(ens_desc const* descs, unsigned nsys, uint32_t const* skip, uint32_t* disp, uint32_t* steps,
+                       uint32_t* rebuilds, uint32_t* stats, ens_work w, hipStream_t s) {
+    ens_decide_kernel<<<1, 1024, 0, s>>>(descs, nsys, skip, disp, steps, rebuilds, stats, w);
+    HIP_CHECK(hipGetLastError());
+}
+
+// fixed grids (the work is on the device): 4 workgroups per CU
+constexpr unsigned ENS_WORK_GRID = 1024;
+
+void launch_ens_rebuild_device(ens_rebuild_args const& a, hipStream_t s) {
+    ens_rank_kernel<<<ENS_WORK_GRID, 256, 0, s>>>(a.posq, a.offsets, a.descs, a.skip, a.w, a.keys, a.vals, a.cnt27);
+    HIP_CHECK(hipGetLastError());
+    ens_cell_count_kernel<<<ENS_WORK_GRID, ENS_CELL_BLOCK, 0, s>>>(a.descs, a.skip, a.w, a.cnt27, a.base27,
+                                                                  a.offsets_t, a.bsum);
+    HIP_CHECK(hipGetLastError());
+    ens_cell_carry_kernel<<<ENS_WORK_GRID, ENS_CELL_BLOCK, 0, s>>>(a.descs, a.skip, a.w, a.bsum, a.offsets_t);
+    HIP_CHECK(hipGetLastError());
+    ens_scatter_kernel<<<ENS_WORK_GRID, 256, 0, s>>>(a.pos, a.vel, a.image, a.id, a.posq, a.pos_t, a.vel_t,
+                                                    a.image_t, a.id_t, a.posq_t, a.descs, a.skip, a.w, a.keys,
+                                                    a.vals, a.base27, a.offsets_t);
+    HIP_CHECK(hipGetLastError());
+    ens_copyback_flagged_kernel<<<ENS_WORK_GRID, 256, 0, s>>>(a.pos, a.vel, a.image, a.id, a.posq, a.r0, a.pos_t,
+                                                             a.vel_t, a.image_t, a.id_t, a.posq_t, a.offsets,
+                                                             a.offsets_t, a.descs, a.skip, a.w);
+    HIP_CHECK(hipGetLastError());
+    if (a.lds_records) {
+        size_t const lds = size_t(a.lds_records) * sizeof(uint2);   // cubic boxes: packed records
+        if (a.multi)
+            ens_build_flagged_kernel<true><<<ENS_WORK_GRID, a.threads, lds, s>>>(a.posq, a.offsets, a.descs, a.skip,
+                                                                                a.w, a.list, a.count, a.stats);
+        else
+            ens_build_flagged_kernel<false><<<ENS_WORK_GRID, a.threads, lds, s>>>(a.posq, a.offsets, a.descs, a.skip,
+                                                                                 a.w, a.list, a.count, a.stats);
+        HIP_CHECK(hipGetLastError());
+    }
+    if (a.small_max_n) {
+        ens_r0_flagged_kernel<<<ENS_WORK_GRID, 256, 0, s>>>(a.pos, a.r0, a.descs, a.w);
+        HIP_CHECK(hipGetLastError());
+        ens_build_small_flagged_kernel<<<ENS_WORK_GRID, 256, size_t(a.small_max_n) * sizeof(uint4), s>>>(
+            a.posq, a.descs, a.skip, a.w, a.list, a.count, a.stats);
+        HIP_CHECK(hipGetLastError());
+    }
 }
 
 void launch_ens_build_small(uint4 const* posq, ens_desc const* descs, uint32_t const* rsmall, unsigned nsmall,

@@ -958,6 +958,22 @@ HALMD_API int halmd_ensemble_retire(halmd_ensemble* e, unsigned system) {
 HALMD_API int halmd_ensemble_apply(halmd_ensemble* e) {
     return guard([&] { as<ensemble>(e)->apply(); });
 }
+HALMD_API int halmd_ensemble_set_device_steps(halmd_ensemble* e, int on) {
+    return guard([&] { as<ensemble>(e)->set_device_steps(on != 0); });
+}
+HALMD_API int halmd_ensemble_recoveries(const halmd_ensemble* e, uint64_t* count) {
+    return guard([&] {
+        require(count != nullptr, "ensemble: null output");
+        *count = as<ensemble>(e)->recoveries();
+    });
+}
+HALMD_API int halmd_ensemble_set_list_capacity(halmd_ensemble* e, unsigned system, unsigned capacity) {
+    return guard([&] {
+        auto* x = as<ensemble>(e);
+        x->set_list_capacity(system, capacity);
+        HIP_CHECK(hipStreamSynchronize(x->stream));
+    });
+}
 HALMD_API int halmd_ensemble_step(halmd_ensemble* e, uint64_t steps) {
     return guard([&] {
         auto* x = as<ensemble>(e);

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 
 namespace halmd {
 
@@ -181,6 +182,16 @@ void ensemble::prepare(sys_host& h) {
     d.ncells = h.ncells;
     d.small = h.small;
     d.skin_half_sq = float(h.skin * h.skin / 4);
+    {
+        // the largest float m2 with sqrt(double(m2)) <= skin / 2: the host rule
+        // of step_host, as bits (non-negative floats order as their bits)
+        double const half = h.skin / 2;
+        auto keeps = [half](float m2) { return std::sqrt(double(m2)) <= half; };
+        float f = float(half * half);
+        while (keeps(f) && f < std::numeric_limits<float>::max()) f = std::nextafter(f, HUGE_VALF);
+        while (!keeps(f) && f > 0.0f) f = std::nextafter(f, 0.0f);
+        std::memcpy(&d.keep_bits, &f, 4);
+    }
     d.sp.grid.ncell[0] = d.sp.grid.ncell[1] = d.sp.grid.ncell[2] = h.ncell;
     d.sp.S = h.S;
     d.sp.R = h.R;
@@ -396,14 +407,26 @@ void ensemble::apply() {
     std::swap(posq_, posq_t_); std::swap(image_, image_t_); std::swap(id_, id_t_); std::swap(count_, count_t_);
     std::swap(list_, list_t_); std::swap(offsets_, offsets_t_);
 
-    // scratch sized for the new layout (the twins serve the reorder)
+    // scratch sized for the new layout (the twins serve the reorder, and the
+    // offsets twin receives the new cell offsets of a device-decided rebuild)
     grow(pos_t_, 2 * N); grow(vel_t_, 2 * N); grow(posq_t_, N); grow(image_t_, N); grow(id_t_, N);
+    grow(offsets_t_, cells_total_);
     grow(aux_force_, N); grow(stress_, 6 * N); grow(keys_, N); grow(vals_, N);
     ws_.reserve(std::max<size_t>(N, 1));
     size_t const ns = std::max<size_t>(sys_.size(), 1);
-    grow(disp2_, ns * DISP_STRIDE); grow(stats_, 2 * ns); grow(thermo_, 4 * ns);
-    if (h_disp2_.size() < ns) {
-        h_disp2_.resize(ns + ns / 4); h_stats_.resize(2 * (ns + ns / 4)); h_thermo_.resize(4 * (ns + ns / 4));
+    grow(disp2_, ns * ENS_DISP_WORDS); grow(stats_, 2 * ns); grow(thermo_, 4 * ns);
+    grow(skip_, ns); grow(dsteps_, ns); grow(drebuilds_, ns); grow(work_, ens_work_words(ns));
+    grow(cnt27_, 27 * std::max<size_t>(cells_total_, 1));
+    grow(base27_, 27 * std::max<size_t>(cells_total_, 1));
+    grow(bsum_, cells_total_ / ENS_CELL_BLOCK + ns + 1);   // blocks of ENS_CELL_BLOCK cells, per system
+    // the device-decided step expects cleared displacement slots, skip words and counts
+    launch_fill_u32(disp2_, 0u, ns * ENS_DISP_WORDS, stream);
+    launch_fill_u32(skip_, 0u, ns, stream);
+    launch_fill_u32(dsteps_, 0u, ns, stream);
+    launch_fill_u32(drebuilds_, 0u, ns, stream);
+    if (h_disp2_.size() < ns * ENS_DISP_SLOTS) {
+        size_t const m = ns + ns / 4;
+        h_disp2_.resize(m * ENS_DISP_SLOTS); h_stats_.resize(2 * m); h_thermo_.resize(4 * m); h_tick_.resize(3 * m);
     }
     {
         std::vector<uint16_t> bs(std::max<unsigned>(nblocks_, 1), 0);
@@ -443,12 +466,12 @@ void ensemble::forces_of(std::vector<unsigned> const& R) {
         size_t const a = off[2 * k], b = off[2 * k + 1], e = off[2 * k + 2];
         launch_ens_force(c.kind, c.trunc, false, c.multi, posq_, offsets_, descs_, fsub_tables_.get() + a,
                          unsigned(b - a), threads_, c.lds_records, fsub_tables_.get() + b, unsigned(e - b),
-                         c.small_max_n, list_, count_, force_, nullptr, total_, stream);
+                         c.small_max_n, list_, count_, force_, nullptr, total_, nullptr, stream);
     }
     HIP_CHECK(hipStreamSynchronize(stream));   // the table upload reads host memory
 }
 
-void ensemble::rebuild(std::vector<unsigned> const& R0) {
+void ensemble::rebuild(std::vector<unsigned> const& R0, bool count) {
     std::vector<unsigned> R = R0;
     while (true) {
         std::vector<unsigned> large, small;
@@ -528,25 +551,7 @@ void ensemble::rebuild(std::vector<unsigned> const& R0) {
             }
         }
         if (relayout) {
-            // new list offsets; the lists of the systems not being rebuilt
-            // move unchanged, those of R are built again
-            std::vector<size_t> old_off(sys_.size());
-            for (size_t s = 0; s < sys_.size(); ++s) old_off[s] = sys_[s]->list_off;
-            list_words_ = layout_lists();
-            grow(list_t_, list_words_);
-            std::vector<char> in_r(sys_.size(), 0);
-            for (unsigned s : R) in_r[s] = 1;
-            std::vector<ens_move> ml;
-            for (size_t s = 0; s < sys_.size(); ++s) {
-                auto const& h = *sys_[s];
-                if (!in_r[s]) ml.push_back({old_off[s], h.list_off, size_t(round_up(h.n, WAVE)) * h.cap});
-            }
-            grow(moves_, ml.size());
-            if (!ml.empty()) moves_.upload(ml.data(), ml.size(), stream);
-            launch_ens_move(u4(list_t_), cu4(list_), moves_.get(), unsigned(ml.size()), 1, 8, stream);
-            HIP_CHECK(hipStreamSynchronize(stream));
-            std::swap(list_, list_t_);
-            upload_descs();
+            relayout_lists(R);
             continue;
         }
         if (redesc) {
@@ -556,7 +561,8 @@ void ensemble::rebuild(std::vector<unsigned> const& R0) {
         }
         break;
     }
-    for (unsigned s : R0) ++sys_[s]->rebuilds;
+    if (count)
+        for (unsigned s : R0) ++sys_[s]->rebuilds;
     // the force kernels stage at most the largest staging of their class
     for (auto& c : classes_) {
         c.lds_records = 64;
@@ -564,17 +570,69 @@ void ensemble::rebuild(std::vector<unsigned> const& R0) {
     }
 }
 
-void ensemble::forces(bool aux, float4* force, float* stress) {
+// new list offsets at the present capacities; the lists of the systems not
+// in R move unchanged, those of R are to be built again
+void ensemble::relayout_lists(std::vector<unsigned> const& R) {
+    std::vector<size_t> old_off(sys_.size());
+    for (size_t s = 0; s < sys_.size(); ++s) old_off[s] = sys_[s]->list_off;
+    list_words_ = layout_lists();
+    grow(list_t_, list_words_);
+    std::vector<char> in_r(sys_.size(), 0);
+    for (unsigned s : R) in_r[s] = 1;
+    std::vector<ens_move> ml;
+    for (size_t s = 0; s < sys_.size(); ++s) {
+        auto const& h = *sys_[s];
+        if (!in_r[s]) ml.push_back({old_off[s], h.list_off, size_t(round_up(h.n, WAVE)) * h.cap});
+    }
+    grow(moves_, ml.size());
+    if (!ml.empty()) moves_.upload(ml.data(), ml.size(), stream);
+    launch_ens_move(u4(list_t_), cu4(list_), moves_.get(), unsigned(ml.size()), 1, 8, stream);
+    HIP_CHECK(hipStreamSynchronize(stream));
+    std::swap(list_, list_t_);
+    upload_descs();
+}
+
+void ensemble::set_list_capacity(unsigned s, unsigned cap) {
+    require(committed_ && s < sys_.size() && sys_[s]->on_device, "ensemble: system index out of range");
+    apply();
+    settle();
+    auto& h = *sys_[s];
+    if (cap == 0) {
+        std::vector<uint32_t> c(h.n);
+        HIP_CHECK(hipMemcpyAsync(c.data(), count_.get() + h.base, h.n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                 stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        cap = c.empty() ? 1u : *std::max_element(c.begin(), c.end());
+    }
+    h.cap = unsigned(round_up(std::max(cap, 1u), NB16_BLOCK));
+    if (h.small) h.cap = std::min(h.cap, unsigned(round_up(h.n, NB16_BLOCK)));
+    relayout_lists({s});
+    // lists built again at the new capacity (counted as the rebuild it is: r0
+    // is reset, and the particles may be rearranged, so the force follows;
+    // the pending kick was applied above with the force of the old order)
+    rebuild({s});
+    forces_of({s});
+}
+
+void ensemble::forces(bool aux, float4* force, float* stress, uint32_t const* skip) {
     for (auto& c : classes_)
         launch_ens_force(c.kind, c.trunc, aux, c.multi, posq_, offsets_, descs_, fclass_tables_.get() + c.seg_off,
                          unsigned(c.seg.size()), threads_, c.lds_records, fclass_tables_.get() + c.small_off,
-                         unsigned(c.small.size()), c.small_max_n, list_, count_, force, stress, total_, stream);
+                         unsigned(c.small.size()), c.small_max_n, list_, count_, force, stress, total_, skip,
+                         stream);
 }
 
 void ensemble::settle() {
     if (!pending_) return;
     launch_ens_settle(vel_, force_, descs_, block_sys_, nblocks_, stream);
     pending_ = false;
+}
+
+bool ensemble::device_steps() const {
+    if (!device_steps_) return false;
+    for (auto const& h : sys_)
+        if (!h->small && (h->ncells >= (1u << ENS_CELL_KEY_BITS) || h->ncell < 3)) return false;
+    return true;
 }
 
 void ensemble::step(unsigned steps) {
@@ -584,21 +642,31 @@ void ensemble::step(unsigned steps) {
         steps_done += steps;
         return;
     }
+    if (device_steps())
+        step_device(steps);
+    else
+        step_host(steps);
+}
+
+// the host decides each step's rebuilds from the displacement maxima it reads back
+void ensemble::step_host(unsigned steps) {
     std::vector<unsigned> R;
+    size_t const ns = sys_.size();
     for (unsigned k = 0; k < steps; ++k) {
-        // one word per system, a cache line apart
-        launch_fill_u32(disp2_, 0u, sys_.size() * DISP_STRIDE, stream);
+        launch_fill_u32(disp2_, 0u, ns * ENS_DISP_WORDS, stream);
         launch_ens_integrate(pos_, vel_, force_, image_, posq_, r0_, descs_, block_sys_, nblocks_, pending_, disp2_,
-                             stream);
+                             nullptr, stream);
         pending_ = false;
-        HIP_CHECK(hipMemcpy2DAsync(h_disp2_.get(), sizeof(uint32_t), disp2_.get(), DISP_STRIDE * sizeof(uint32_t),
-                                   sizeof(uint32_t), sys_.size(), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipMemcpy2DAsync(h_disp2_.get(), sizeof(uint32_t), disp2_.get(),
+                                   ENS_DISP_STRIDE * sizeof(uint32_t), sizeof(uint32_t), ns * ENS_DISP_SLOTS,
+                                   hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
         R.clear();
-        for (size_t s = 0; s < sys_.size(); ++s) {
+        for (size_t s = 0; s < ns; ++s) {
             // the single-system rule: rebuild when max |r - r0| > skin / 2
+            uint32_t bits = 0;
+            for (unsigned j = 0; j < ENS_DISP_SLOTS; ++j) bits = std::max(bits, h_disp2_[s * ENS_DISP_SLOTS + j]);
             float d2;
-            uint32_t const bits = h_disp2_[s];
             std::memcpy(&d2, &bits, 4);
             if (std::sqrt(double(d2)) > sys_[s]->skin / 2) R.push_back(unsigned(s));
         }
@@ -607,6 +675,103 @@ void ensemble::step(unsigned steps) {
         pending_ = true;   // the finalize kick, folded into the next integrate
         ++steps_done;
     }
+    launch_fill_u32(disp2_, 0u, ns * ENS_DISP_WORDS, stream);   // as the device-decided step leaves them
+}
+
+// Device-decided steps, in chunks: each chunk is one stretch of launches with
+// no host round trip, then one readback.  A system that stopped in the chunk
+// (ENS_SKIP_OVERFLOW, ENS_SKIP_FAR) is rebuilt on the host, its force
+// evaluated, and it runs the steps it missed with every other system held;
+// its trajectory is the one it would have had without stopping.
+void ensemble::step_device(unsigned steps) {
+    constexpr unsigned CHUNK = 64;
+    size_t const ns = sys_.size();
+    std::vector<uint32_t> left(ns), done, skip;
+    std::vector<char> hold(ns);
+    for (unsigned k0 = 0; k0 < steps; k0 += CHUNK) {
+        unsigned const K = std::min(CHUNK, steps - k0);
+        std::fill(left.begin(), left.end(), K);
+        bool all = true;
+        while (true) {
+            unsigned m = 0;
+            for (size_t s = 0; s < ns; ++s) {
+                hold[s] = left[s] == 0;
+                if (left[s]) m = m ? std::min(m, left[s]) : left[s];
+            }
+            if (!m) break;
+            tick(m, all ? nullptr : &hold, done, skip);
+            all = false;
+            std::vector<unsigned> R;
+            for (size_t s = 0; s < ns; ++s) {
+                if (hold[s]) continue;
+                left[s] -= std::min(left[s], done[s]);
+                if (skip[s] & (ENS_SKIP_OVERFLOW | ENS_SKIP_FAR)) R.push_back(unsigned(s));
+            }
+            if (!R.empty()) {
+                // at the end of their last completed step, lists to be rebuilt
+                // (the device counted that rebuild), force not evaluated
+                recoveries_ += R.size();
+                rebuild(R, false);
+                forces_of(R);
+            }
+        }
+        steps_done += K;
+    }
+}
+
+void ensemble::tick(unsigned steps, std::vector<char> const* hold, std::vector<uint32_t>& done,
+                    std::vector<uint32_t>& skip) {
+    size_t const ns = sys_.size();
+    if (hold) {
+        for (size_t s = 0; s < ns; ++s) h_tick_[s] = (*hold)[s] ? ENS_SKIP_HOLD : 0u;
+        HIP_CHECK(hipMemcpyAsync(skip_.get(), h_tick_.get(), ns * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    }
+    unsigned build_lds = 0, small_max = 0;
+    bool multi = false;
+    for (auto const& h : sys_) {
+        if (h->small) {
+            small_max = std::max(small_max, h->n);
+        } else {
+            build_lds = std::max(build_lds, h->lds_cap);
+            multi = multi || h->ntype > 1;
+        }
+    }
+    // every buffer the device-decided rebuild writes, at the sizes its kernels index
+    size_t const N = total_;
+    require(pos_t_.size() >= 2 * N && vel_t_.size() >= 2 * N && posq_t_.size() >= N && image_t_.size() >= N &&
+                id_t_.size() >= N && keys_.size() >= N && vals_.size() >= N && r0_.size() >= N &&
+                offsets_.size() >= cells_total_ && offsets_t_.size() >= cells_total_ &&
+                cnt27_.size() >= 27 * cells_total_ && base27_.size() >= 27 * cells_total_ &&
+                bsum_.size() >= cells_total_ / ENS_CELL_BLOCK + ns + 1 &&
+                disp2_.size() >= ns * ENS_DISP_WORDS && skip_.size() >= ns &&
+                dsteps_.size() >= ns && drebuilds_.size() >= ns && stats_.size() >= 2 * ns &&
+                work_.size() >= ens_work_words(ns) && h_tick_.size() >= 3 * ns,
+            "ensemble: device step buffers not sized for the layout");
+    ens_work const w = ens_work_at(work_.get(), ns);
+    for (unsigned k = 0; k < steps; ++k) {
+        launch_ens_integrate(pos_, vel_, force_, image_, posq_, r0_, descs_, block_sys_, nblocks_, pending_, disp2_,
+                             skip_, stream);
+        pending_ = false;
+        launch_ens_decide(descs_, unsigned(ns), skip_, disp2_, dsteps_, drebuilds_, stats_, w, stream);
+        ens_rebuild_args const a{pos_, vel_, image_, id_, posq_, r0_, pos_t_, vel_t_, image_t_, id_t_, posq_t_,
+                                 offsets_, offsets_t_, keys_, vals_, cnt27_, base27_, bsum_, descs_, w, skip_,
+                                 threads_, build_lds, multi, small_max, list_, count_, stats_};
+        launch_ens_rebuild_device(a, stream);
+        forces(false, force_, nullptr, skip_);
+        pending_ = true;
+    }
+    // one readback for the whole stretch
+    uint32_t* const h = h_tick_.get();
+    HIP_CHECK(hipMemcpyAsync(h, skip_.get(), ns * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipMemcpyAsync(h + ns, dsteps_.get(), ns * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipMemcpyAsync(h + 2 * ns, drebuilds_.get(), ns * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    launch_fill_u32(skip_, 0u, ns, stream);
+    launch_fill_u32(dsteps_, 0u, ns, stream);
+    launch_fill_u32(drebuilds_, 0u, ns, stream);
+    HIP_CHECK(hipStreamSynchronize(stream));
+    skip.assign(h, h + ns);
+    done.assign(h + ns, h + 2 * ns);
+    for (size_t s = 0; s < ns; ++s) sys_[s]->rebuilds += h[2 * ns + s];
 }
 
 void ensemble::thermo(double* out) {

@@ -2,13 +2,15 @@
 //
 // All systems live in shared device arrays (each starting on a 256-particle
 // boundary) with one descriptor each.  A step is one integrate launch for
-// every system, one 4-byte-per-system readback of the displacement maxima, a
-// batched rebuild of the systems whose displacement passed half the skin
-// (one launch per rebuild stage for all of them), and one force launch per
-// (potential, truncation, species) class.  The per-lane arithmetic is the
-// single-system path's own device code (staged_force_segment,
-// staged_build_segment, integrate_particle), so a system gives the same
-// result alone and in the mix.
+// every system, the rebuild of the systems whose displacement passed half the
+// skin (one launch per rebuild stage for all of them), and one force launch
+// per (potential, truncation, species) class.  The rebuild decision is taken
+// on the device (a decide kernel writes the work tables the rebuild kernels
+// stride over), so a stretch of steps runs with no host round trip; the host
+// reads back once per stretch.  The per-lane arithmetic is the single-system
+// path's own device code (staged_force_segment, staged_build_segment,
+// integrate_particle), so a system gives the same result alone and in the
+// mix.
 //
 // Membership changes between steps (continuous batching): systems added after
 // the first apply() and systems retired are placed by the next apply(), which
@@ -51,6 +53,18 @@ public:
     void apply();
     void commit() { apply(); }
     void step(unsigned steps);
+    // rebuilds decided on the device (no host round trip per step) when every
+    // system allows it; false: the host decides after each step
+    void set_device_steps(bool on) { device_steps_ = on; }
+    bool device_steps() const;
+    // systems whose device-decided step stopped (list or staging capacity
+    // exceeded, or a particle moved past the 26 cells around its old cell)
+    // and were rebuilt on the host and caught up
+    uint64_t recoveries() const { return recoveries_; }
+    // list capacity of a placed system, its lists rebuilt at once (0: the
+    // largest list it holds now, rounded up to a whole block); later
+    // rebuilds enlarge it again when a list does not fit
+    void set_list_capacity(unsigned sys, unsigned cap);
     // per system: {sum m v^2, sum en_pot, sum tr stress, n} (FP64)
     void thermo(double* out);
     // state of one system in particle-ID order: positions (in the box),
@@ -82,7 +96,8 @@ public:
     struct sys_host;
 private:
     std::vector<std::unique_ptr<sys_host>> sys_;
-    bool committed_ = false, pending_ = false;
+    bool committed_ = false, pending_ = false, device_steps_ = true;
+    uint64_t recoveries_ = 0;
     uint64_t next_uid_ = 1;
     unsigned total_ = 0, nblocks_ = 0, threads_ = SEG_THREADS_MAX;
     size_t list_words_ = 0, cells_total_ = 0;
@@ -93,6 +108,10 @@ private:
     dbuf<uint4> posq_, posq_t_;
     dbuf<int4> image_, image_t_;
     dbuf<uint32_t> id_, id_t_, count_, count_t_, offsets_, offsets_t_, disp2_, stats_, keys_, vals_, tables_;
+    // device-decided steps: per-system skip words, step and rebuild counts of
+    // the current tick, the rebuild work tables, 27 counts per cell
+    dbuf<uint32_t> skip_, dsteps_, drebuilds_, work_, cnt27_, base27_, bsum_;
+    hbuf<uint32_t> h_tick_;
     dbuf<float> stress_;
     dbuf<uint16_t> list_, list_t_, block_sys_;
     dbuf<ens_move> moves_;
@@ -121,8 +140,15 @@ private:
     size_t layout_lists();
     void upload_descs();
     void build_classes();
-    void rebuild(std::vector<unsigned> const& R);
-    void forces(bool aux, float4* force, float* stress);
+    void rebuild(std::vector<unsigned> const& R, bool count = true);
+    void relayout_lists(std::vector<unsigned> const& R);
+    void step_host(unsigned steps);
+    void step_device(unsigned steps);
+    // `steps` device-decided steps of the systems with hold[s] == 0 (hold
+    // null: all); per system the steps it completed and its skip word
+    void tick(unsigned steps, std::vector<char> const* hold, std::vector<uint32_t>& done,
+              std::vector<uint32_t>& skip);
+    void forces(bool aux, float4* force, float* stress, uint32_t const* skip = nullptr);
     void forces_of(std::vector<unsigned> const& R);   // plain forces of the listed systems only
     void settle();
 };

@@ -609,13 +609,14 @@ template <int KIND, int TRUNC, bool AUX, bool MULTI>
 __global__ __launch_bounds__(SEG_THREADS_MAX) void ens_force_kernel(
     uint4 const* __restrict__ posq, uint32_t const* __restrict__ offsets, ens_desc const* __restrict__ descs,
     uint32_t const* __restrict__ fseg, uint16_t const* __restrict__ list, uint32_t const* __restrict__ count,
-    float4* __restrict__ force, float* __restrict__ stress, unsigned stride)
+    float4* __restrict__ force, float* __restrict__ stress, unsigned stride, uint32_t const* skip)
 {
     extern __shared__ uint4 s_rec[];
     __shared__ seg_stage st;
     __shared__ pair_param s_pair[MAX_TYPES * MAX_TYPES];
     uint32_t const e = fseg[blockIdx.x];
     unsigned const sys = e >> 16, seg = e & 0xffffu;
+    if (skip && skip[sys]) return;   // the whole workgroup
     ens_desc const& d = descs[sys];
     staged_force_segment<KIND, TRUNC, AUX, true, MULTI>(
         posq + d.base, offsets + d.cell_base, d.sp, seg, list + d.list_off, count + d.base, d.cap, d.P, true, 0.5f,
@@ -631,11 +632,12 @@ template <int KIND, int TRUNC, bool AUX>
 __global__ __launch_bounds__(256) void ens_force_small_kernel(
     uint4 const* __restrict__ posq, ens_desc const* __restrict__ descs, uint32_t const* __restrict__ fsmall,
     uint16_t const* __restrict__ list, uint32_t const* __restrict__ count, float4* __restrict__ force,
-    float* __restrict__ stress, unsigned stride)
+    float* __restrict__ stress, unsigned stride, uint32_t const* skip)
 {
     constexpr unsigned T = NB16_BLOCK;   // lanes per particle: lane u reads slot u of each list block
     static_assert(256 / T == ENS_SMALL_P, "one particle per group of T lanes");
     extern __shared__ uint4 s_rec[];
+    if (skip && skip[fsmall[blockIdx.x]]) return;   // the whole workgroup
     ens_desc const& d = descs[fsmall[blockIdx.x]];
     unsigned const i0 = blockIdx.y * ENS_SMALL_P;
     if (i0 >= d.n) return;   // the whole workgroup
@@ -1064,62 +1066,65 @@ void launch_pair_force_staged(uint4 const* posq, uint32_t const* offset, seg_par
 }
 
 namespace {
+struct ens_force_args {
+    bool multi;
+    uint4 const* posq;
+    uint32_t const* offsets;
+    ens_desc const* descs;
+    uint32_t const* fseg;
+    unsigned nfseg, threads, lds_records;
+    uint32_t const* fsmall;
+    unsigned nsmall, small_max_n;
+    uint16_t const* list;
+    uint32_t const* count;
+    float4* force;
+    float* stress;
+    unsigned stride;
+    uint32_t const* skip;
+    hipStream_t s;
+};
+
 template <int KIND, int TRUNC, bool AUX>
-void ens_force_go(bool multi, uint4 const* posq, uint32_t const* offsets, ens_desc const* descs,
-                  uint32_t const* fseg, unsigned nfseg, unsigned threads, unsigned lds_records,
-                  uint32_t const* fsmall, unsigned nsmall, unsigned small_max_n, uint16_t const* list,
-                  uint32_t const* count, float4* force, float* stress, unsigned stride, hipStream_t s) {
-    size_t const lds = size_t(lds_records) * sizeof(uint4);
+void ens_force_go(ens_force_args const& a) {
+    size_t const lds = size_t(a.lds_records) * sizeof(uint4);
+    unsigned nfseg = a.nfseg;
     if (nfseg) {
         if constexpr (KIND == POTENTIAL_LENNARD_JONES) {
-            if (multi) {
-                ens_force_kernel<KIND, TRUNC, AUX, true><<<nfseg, threads, lds, s>>>(posq, offsets, descs, fseg, list,
-                                                                                    count, force, stress, stride);
+            if (a.multi) {
+                ens_force_kernel<KIND, TRUNC, AUX, true><<<nfseg, a.threads, lds, a.s>>>(
+                    a.posq, a.offsets, a.descs, a.fseg, a.list, a.count, a.force, a.stress, a.stride, a.skip);
                 nfseg = 0;
             }
         } else {
-            if (multi) throw error(HALMD_ERR_INVALID_ARGUMENT, "ensemble: several species need Lennard-Jones");
+            if (a.multi) throw error(HALMD_ERR_INVALID_ARGUMENT, "ensemble: several species need Lennard-Jones");
         }
         if (nfseg)
-            ens_force_kernel<KIND, TRUNC, AUX, false><<<nfseg, threads, lds, s>>>(posq, offsets, descs, fseg, list,
-                                                                                 count, force, stress, stride);
+            ens_force_kernel<KIND, TRUNC, AUX, false><<<nfseg, a.threads, lds, a.s>>>(
+                a.posq, a.offsets, a.descs, a.fseg, a.list, a.count, a.force, a.stress, a.stride, a.skip);
     }
-    if (nsmall)
+    if (a.nsmall)
         ens_force_small_kernel<KIND, TRUNC, AUX>
-            <<<dim3(nsmall, div_up(small_max_n, ENS_SMALL_P)), 256, size_t(small_max_n) * sizeof(uint4), s>>>(
-                posq, descs, fsmall, list, count, force, stress, stride);
+            <<<dim3(a.nsmall, div_up(a.small_max_n, ENS_SMALL_P)), 256, size_t(a.small_max_n) * sizeof(uint4), a.s>>>(
+                a.posq, a.descs, a.fsmall, a.list, a.count, a.force, a.stress, a.stride, a.skip);
 }
 
 template <int KIND, int TRUNC>
-void ens_force_aux(bool aux, bool multi, uint4 const* posq, uint32_t const* offsets, ens_desc const* descs,
-                   uint32_t const* fseg, unsigned nfseg, unsigned threads, unsigned lds_records,
-                   uint32_t const* fsmall, unsigned nsmall, unsigned small_max_n, uint16_t const* list,
-                   uint32_t const* count, float4* force, float* stress, unsigned stride, hipStream_t s) {
+void ens_force_aux(bool aux, ens_force_args const& a) {
     if (aux)
-        ens_force_go<KIND, TRUNC, true>(multi, posq, offsets, descs, fseg, nfseg, threads, lds_records, fsmall,
-                                        nsmall, small_max_n, list, count, force, stress, stride, s);
+        ens_force_go<KIND, TRUNC, true>(a);
     else
-        ens_force_go<KIND, TRUNC, false>(multi, posq, offsets, descs, fseg, nfseg, threads, lds_records, fsmall,
-                                         nsmall, small_max_n, list, count, force, stress, stride, s);
+        ens_force_go<KIND, TRUNC, false>(a);
 }
 
 template <int KIND>
-void ens_force_trunc(int trunc, bool aux, bool multi, uint4 const* posq, uint32_t const* offsets,
-                     ens_desc const* descs, uint32_t const* fseg, unsigned nfseg, unsigned threads,
-                     unsigned lds_records, uint32_t const* fsmall, unsigned nsmall, unsigned small_max_n,
-                     uint16_t const* list, uint32_t const* count, float4* force, float* stress, unsigned stride,
-                     hipStream_t s) {
-#define HALMD_ENS_TRUNC(T)                                                                                         \
-    ens_force_aux<KIND, T>(aux, multi, posq, offsets, descs, fseg, nfseg, threads, lds_records, fsmall, nsmall,   \
-                           small_max_n, list, count, force, stress, stride, s)
+void ens_force_trunc(int trunc, bool aux, ens_force_args const& a) {
     switch (trunc) {
-    case TRUNC_SHARP: HALMD_ENS_TRUNC(TRUNC_SHARP); break;
-    case TRUNC_SHIFTED: HALMD_ENS_TRUNC(TRUNC_SHIFTED); break;
-    case TRUNC_FORCE_SHIFTED: HALMD_ENS_TRUNC(TRUNC_FORCE_SHIFTED); break;
-    case TRUNC_SMOOTH_R4: HALMD_ENS_TRUNC(TRUNC_SMOOTH_R4); break;
+    case TRUNC_SHARP: ens_force_aux<KIND, TRUNC_SHARP>(aux, a); break;
+    case TRUNC_SHIFTED: ens_force_aux<KIND, TRUNC_SHIFTED>(aux, a); break;
+    case TRUNC_FORCE_SHIFTED: ens_force_aux<KIND, TRUNC_FORCE_SHIFTED>(aux, a); break;
+    case TRUNC_SMOOTH_R4: ens_force_aux<KIND, TRUNC_SMOOTH_R4>(aux, a); break;
     default: throw error(HALMD_ERR_INVALID_ARGUMENT, "unknown truncation");
     }
-#undef HALMD_ENS_TRUNC
 }
 } // namespace
 
@@ -1127,15 +1132,14 @@ void launch_ens_force(int kind, int trunc, bool aux, bool multi, uint4 const* po
                       ens_desc const* descs, uint32_t const* fseg, unsigned nfseg, unsigned threads,
                       unsigned lds_records, uint32_t const* fsmall, unsigned nsmall, unsigned small_max_n,
                       uint16_t const* list, uint32_t const* count, float4* force, float* stress, unsigned stride,
-                      hipStream_t s) {
+                      uint32_t const* skip, hipStream_t s) {
     if (!nfseg && !nsmall) return;
+    ens_force_args const a{multi, posq,  offsets, descs,  fseg,   nfseg, threads, lds_records, fsmall,
+                           nsmall, small_max_n, list, count, force, stress, stride, skip, s};
     if (kind == POTENTIAL_LENNARD_JONES)
-        ens_force_trunc<POTENTIAL_LENNARD_JONES>(trunc, aux, multi, posq, offsets, descs, fseg, nfseg, threads,
-                                                 lds_records, fsmall, nsmall, small_max_n, list, count, force,
-                                                 stress, stride, s);
+        ens_force_trunc<POTENTIAL_LENNARD_JONES>(trunc, aux, a);
     else if (kind == POTENTIAL_MORSE)
-        ens_force_trunc<POTENTIAL_MORSE>(trunc, aux, multi, posq, offsets, descs, fseg, nfseg, threads, lds_records,
-                                         fsmall, nsmall, small_max_n, list, count, force, stress, stride, s);
+        ens_force_trunc<POTENTIAL_MORSE>(trunc, aux, a);
     else
         throw error(HALMD_ERR_INVALID_ARGUMENT, "unknown potential");
     HIP_CHECK(hipGetLastError());

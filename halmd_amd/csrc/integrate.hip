@@ -135,20 +135,22 @@ __global__ __launch_bounds__(256) void andersen_finalize_kernel(float4* __restri
 }
 
 // ensembles: block b serves system block_sys[b] (systems start on block
-// boundaries); one displacement maximum per system
+// boundaries); one displacement maximum per system, spread over its slots
 __global__ __launch_bounds__(ENS_BLOCK) void ens_integrate_kernel(
     float4* __restrict__ pos, float4* __restrict__ vel, float4 const* __restrict__ force, int4* __restrict__ image,
     uint4* __restrict__ posq, float4 const* __restrict__ r0, ens_desc const* __restrict__ descs,
-    uint16_t const* __restrict__ block_sys, bool pending, uint32_t* __restrict__ disp2)
+    uint16_t const* __restrict__ block_sys, bool pending, uint32_t* __restrict__ disp, uint32_t const* skip)
 {
     unsigned const s = block_sys[blockIdx.x];
+    if (skip && skip[s]) return;   // the whole workgroup (one system per block)
     ens_desc const& d = descs[s];
     unsigned const li = (blockIdx.x - d.blk_base) * ENS_BLOCK + threadIdx.x;
     float d2 = 0.0f;
     if (li < d.n)
         d2 = integrate_particle(pos, vel, force, image, posq, d.base + li, d.ip, pending ? d.ip.half_dt : 0.0f,
                                 nullptr, r0);
-    wave_max_to(d2, disp2 + size_t(s) * DISP_STRIDE);
+    unsigned const slot = ((blockIdx.x * ENS_BLOCK + threadIdx.x) / WAVE) % ENS_DISP_SLOTS;
+    wave_max_to(d2, disp + size_t(s) * ENS_DISP_WORDS + slot * ENS_DISP_STRIDE);
 }
 
 // apply the deferred finalize kick of every system (before velocities are read)
@@ -223,10 +225,10 @@ void launch_andersen_finalize(float4* vel, float4 const* force, uint32_t const* 
 
 void launch_ens_integrate(float4* pos, float4* vel, float4 const* force, int4* image, uint4* posq,
                           float4 const* r0, ens_desc const* descs, uint16_t const* block_sys, unsigned nblocks,
-                          bool pending, uint32_t* disp2, hipStream_t s) {
+                          bool pending, uint32_t* disp, uint32_t const* skip, hipStream_t s) {
     if (!nblocks) return;
     ens_integrate_kernel<<<nblocks, ENS_BLOCK, 0, s>>>(pos, vel, force, image, posq, r0, descs, block_sys, pending,
-                                                       disp2);
+                                                       disp, skip);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -365,7 +365,7 @@ struct ens_desc {
     unsigned cell_base, ncells;
     unsigned small;                          // all-pairs lists (fewer than 3 cells per dimension)
     unsigned cap;                            // list entries per particle
-    unsigned pad_;
+    uint32_t keep_bits;                      // rebuild when the displacement maximum's float bits exceed it
     unsigned long long list_off;
     float skin_half_sq;                      // rebuild when |r - r0|^2 exceeds it
     seg_params sp;
@@ -373,12 +373,24 @@ struct ens_desc {
     integrate_params ip;
     force_params P;
 };
+// displacement maxima: ENS_DISP_SLOTS words per system, ENS_DISP_STRIDE words
+// apart (a wave folds its maximum into slot (global wave index) mod
+// ENS_DISP_SLOTS, so that the waves of a large system spread their atomics);
+// the maximum over a system's slots is its displacement maximum
+constexpr unsigned ENS_DISP_SLOTS = 16;
+constexpr unsigned ENS_DISP_STRIDE = 16;
+constexpr unsigned ENS_DISP_WORDS = ENS_DISP_SLOTS * ENS_DISP_STRIDE;   // per system
+// per-system state words of the device-decided step (skip[s]): a system with
+// any bit set is left as it is by every kernel of the step
+constexpr uint32_t ENS_SKIP_OVERFLOW = 1;   // its rebuild overflowed the list or staging capacity
+constexpr uint32_t ENS_SKIP_FAR = 2;        // a particle left the 27 cells around its old cell
+constexpr uint32_t ENS_SKIP_HOLD = 4;       // held by the host (catching up other systems)
 // step part 1: deferred kick (if pending), kick, drift, snapshot; per-system
-// maximum displacement (float bits) into disp2[s * DISP_STRIDE] (one cache
-// line per system)
+// displacement maximum (float bits) into the slots at disp + s ENS_DISP_WORDS;
+// systems with skip[s] set (skip may be null) are left as they are
 void launch_ens_integrate(float4* pos, float4* vel, float4 const* force, int4* image, uint4* posq,
                           float4 const* r0, ens_desc const* descs, uint16_t const* block_sys, unsigned nblocks,
-                          bool pending, uint32_t* disp2, hipStream_t s);
+                          bool pending, uint32_t* disp, uint32_t const* skip, hipStream_t s);
 // the deferred finalize kick of every system, applied now
 void launch_ens_settle(float4* vel, float4 const* force, ens_desc const* descs, uint16_t const* block_sys,
                        unsigned nblocks, hipStream_t s);
@@ -426,7 +438,66 @@ void launch_ens_force(int kind, int trunc, bool aux, bool multi, uint4 const* po
                       ens_desc const* descs, uint32_t const* fseg, unsigned nfseg, unsigned threads,
                       unsigned lds_records, uint32_t const* fsmall, unsigned nsmall, unsigned small_max_n,
                       uint16_t const* list, uint32_t const* count, float4* force, float* stress, unsigned stride,
-                      hipStream_t s);
+                      uint32_t const* skip, hipStream_t s);
+
+// Device-decided rebuilds (a step with no host round trip).  After the
+// integrate launch, ens_decide flags each system whose displacement maximum
+// exceeds its keep_bits and writes the work tables of the step's rebuild: the
+// flagged cell systems in ascending index with exclusive prefix sums of their
+// cells, 256-particle blocks and segments, and the flagged all-pairs systems
+// with prefix sums of their blocks and ENS_SMALL_P-particle groups.  The
+// rebuild kernels stride over those tables with fixed grids.
+struct ens_work {
+    uint32_t* head;          // [0] flagged cell systems, [1] flagged all-pairs systems
+    uint32_t *lsys, *lcell, *lblk, *lseg, *lcb;   // cell systems (prefixes have one more entry)
+    uint32_t *ssys, *sblk, *sgrp;                 // all-pairs systems
+};
+constexpr size_t ens_work_words(size_t nsys) { return 16 + 8 * (nsys + 1); }
+constexpr unsigned ENS_CELL_BLOCK = 256;   // cells per workgroup of the new-offset scan
+ens_work ens_work_at(uint32_t* words, size_t nsys);
+// flags, work tables, per-system step and rebuild counts (+1 each), the
+// displacement slots cleared, stats of flagged systems cleared
+void launch_ens_decide(ens_desc const* descs, unsigned nsys, uint32_t const* skip, uint32_t* disp,
+                       uint32_t* steps, uint32_t* rebuilds, uint32_t* stats, ens_work w, hipStream_t s);
+// Rebuild of the flagged cell systems.  The particles of a system are in the
+// order of its previous rebuild (cells ascending, offsets[] describe it), and
+// none moves farther than into one of the 26 cells around its old one (one
+// that does sets ENS_SKIP_FAR, and the host rebuilds that system).  The stable
+// order by new cell is then a merge: cell c takes, from each of the 27 cells c
+// + e in ascending index, the particles that moved by -e, in memory order.
+//  - rank: waves over runs of old cells; keys[i] = new cell | code(e) << 27,
+//    vals[i] = rank of i among the particles of its old cell with that code,
+//    cnt27[27 c' + code] = their number
+//  - cell_count: one thread per new cell; base27[27 c + code] = particles
+//    that reach c from sources below c + e, offsets_t = the exclusive scan of
+//    the counts within each block of ENS_CELL_BLOCK cells, bsum = block sums
+//  - cell_carry: the sums of the earlier blocks added (offsets_t final)
+//  - scatter: each particle to its new slot (into the twin arrays)
+//  - copyback: twins back, r0 := new positions, offsets_t back
+//  - build: the staged list build over the flagged segments
+// Afterwards the all-pairs systems: r0 and their lists.
+struct ens_rebuild_args {
+    float4 *pos, *vel;
+    int4* image;
+    uint32_t* id;
+    uint4* posq;
+    float4* r0;
+    float4 *pos_t, *vel_t;
+    int4* image_t;
+    uint32_t* id_t;
+    uint4* posq_t;
+    uint32_t *offsets, *offsets_t, *keys, *vals, *cnt27, *base27, *bsum;
+    ens_desc const* descs;
+    ens_work w;
+    uint32_t* skip;
+    unsigned threads, lds_records;   // list build (lds_records 0: no cell systems)
+    bool multi;
+    unsigned small_max_n;            // 0: no all-pairs systems
+    uint16_t* list;
+    uint32_t *count, *stats;
+};
+void launch_ens_rebuild_device(ens_rebuild_args const& a, hipStream_t s);
+constexpr unsigned ENS_CELL_KEY_BITS = 27;   // cells per system below 2^27
 // per-system FP64 sums, one workgroup per listed system: out[4 s] = sum m v^2
 // (velocities after the pending kick, when pending), [1] = sum en_pot, [2] =
 // sum of the stress traces (both from aux_force / stress when given), [3] = n

@@ -123,3 +123,18 @@ class ensemble(Handle):
         c = ctypes.c_uint64()
         call("halmd_ensemble_rebuilds", self.handle, int(k), ctypes.byref(c))
         return c.value
+
+    def set_device_steps(self, on=True):
+        """Rebuild decisions on the device (default) or on the host after each step."""
+        call("halmd_ensemble_set_device_steps", self.handle, int(bool(on)))
+
+    @property
+    def recoveries(self):
+        """Systems rebuilt on the host after a device-decided stretch stopped them."""
+        c = ctypes.c_uint64()
+        call("halmd_ensemble_recoveries", self.handle, ctypes.byref(c))
+        return c.value
+
+    def set_list_capacity(self, k, capacity=0):
+        """List capacity of system k (0: its largest present list); lists rebuilt at once."""
+        call("halmd_ensemble_set_list_capacity", self.handle, int(k), int(capacity))

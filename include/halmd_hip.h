@@ -496,6 +496,20 @@ HALMD_API int halmd_ensemble_state(halmd_ensemble* e, unsigned system, double* p
  * (particle-ID order, n x 3) */
 HALMD_API int halmd_ensemble_force(halmd_ensemble* e, unsigned system, double* force);
 HALMD_API int halmd_ensemble_rebuilds(const halmd_ensemble* e, unsigned system, uint64_t* count);
+/* rebuild decisions on the device (1, the default: a stretch of steps with no
+ * host round trip) or on the host after each step (0); the same trajectories.
+ * No counterpart in the reference, whose step checks the displacement on the
+ * host (halmd/mdsim/gpu/max_displacement.cpp:96). */
+HALMD_API int halmd_ensemble_set_device_steps(halmd_ensemble* e, int on);
+/* systems that stopped in a device-decided stretch (a list or the staging
+ * outgrew its capacity, or a particle moved past the cells around its old
+ * one) and were rebuilt on the host and caught up */
+HALMD_API int halmd_ensemble_recoveries(const halmd_ensemble* e, uint64_t* count);
+/* list capacity of one system (entries per particle, rounded up to 16; 0: its
+ * largest present list); its lists are rebuilt at once, and later rebuilds
+ * enlarge the capacity again when a list does not fit (the reference sizes
+ * lists from the occupancy, from_binning.cpp:96-119) */
+HALMD_API int halmd_ensemble_set_list_capacity(halmd_ensemble* e, unsigned system, unsigned capacity);
 
 /* ---- serving: continuous batching of MD jobs over one ensemble -----------
  * No counterpart in the reference (one system per process,

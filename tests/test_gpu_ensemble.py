@@ -152,3 +152,69 @@ def test_ensemble_overflow_regrows_lists():
     r_e, v_e, _ = e.state(0)
     r_a, v_a, _, _ = _alone(SPECS[0], 5)
     assert np.array_equal(r_e, r_a) and np.array_equal(v_e, v_a)
+
+
+# ----------------------------------------------------------------------------
+# rebuilds decided on the device: the same trajectories as the host decision
+# ----------------------------------------------------------------------------
+def _device_and_host(specs, steps, prepare=None):
+    out = []
+    for device in (True, False):
+        e = workloads.ensemble_from_specs(specs, timestep=DT)
+        e.set_device_steps(device)
+        if prepare:
+            prepare(e)
+        e.step(steps)
+        out.append(e)
+    return out
+
+
+def _same_trajectories(ed, eh, count):
+    for k in range(count):
+        for a, b in zip(ed.state(k), eh.state(k)):
+            assert np.all(np.isfinite(a)) and np.array_equal(a, b), k
+        assert ed.rebuilds(k) == eh.rebuilds(k), k
+
+
+def test_device_decided_steps_match_host_decided_steps():
+    # chunks of 64 steps and a remainder
+    ed, eh = _device_and_host(SPECS, 150)
+    print("rebuilds:", [ed.rebuilds(k) for k in range(len(SPECS))], "recoveries:", ed.recoveries)
+    _same_trajectories(ed, eh, len(SPECS))
+
+
+def test_tight_list_capacity_recovers_on_the_host():
+    # capacities at the lattice's list length: the melting fluid outgrows them
+    # in device-decided stretches, which stop those systems, rebuild them on
+    # the host with more room and let them catch up
+    def tight(e):
+        for k in range(len(SPECS)):
+            e.set_list_capacity(k, 0)
+    ed, eh = _device_and_host(SPECS, 200, tight)
+    print("recoveries:", ed.recoveries)
+    assert ed.recoveries > 0
+    _same_trajectories(ed, eh, len(SPECS))
+
+
+def test_particle_past_the_neighbouring_cells_recovers():
+    # one particle of a dilute gas crosses several cells per step: the
+    # device's merge of the old cell order cannot place it, the system stops
+    # and is rebuilt (full sort) on the host; the other system runs on
+    from halmd_amd.ensemble import ensemble
+    runs = []
+    for device in (True, False):
+        e = ensemble()
+        L, r0, v0, _ = workloads.system_inputs(500, 0.0002, 1.0, seed=5)
+        v0[0] = [1100.0, 530.0, 270.0]
+        e.add(r0, v0, L[0], potential="wca", timestep=DT)
+        sp = SPECS[0]
+        L, r0, v0, _ = workloads.system_inputs(sp["n"], sp["density"], sp["temperature"], sp["seed"])
+        e.add(r0, v0, L[0], potential=sp["potential"], timestep=DT)
+        e.commit()
+        e.set_device_steps(device)
+        e.step(30)
+        runs.append(e)
+    ed, eh = runs
+    print("recoveries:", ed.recoveries, "rebuilds:", ed.rebuilds(0), ed.rebuilds(1))
+    assert ed.recoveries > 0 and ed.rebuilds(0) >= 25
+    _same_trajectories(ed, eh, 2)
