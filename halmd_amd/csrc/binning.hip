@@ -897,7 +897,8 @@ __global__ __launch_bounds__(1024) void ens_decide_kernel(ens_desc const* __rest
                                                           uint32_t const* __restrict__ skip,
                                                           uint32_t* __restrict__ disp, uint32_t* __restrict__ steps,
                                                           uint32_t* __restrict__ rebuilds,
-                                                          uint32_t* __restrict__ stats, ens_work w) {
+                                                          uint32_t* __restrict__ stats, ens_work w,
+                                                          uint32_t* __restrict__ flags) {
     constexpr int V = 8;   // cell systems: flag, cells, blocks, segments, cell blocks; all-pairs: flag, blocks, groups
     __shared__ uint32_t s_tmp[17 * V];
     uint32_t carry[V] = {};
@@ -912,10 +913,12 @@ __global__ __launch_bounds__(1024) void ens_decide_kernel(ens_desc const* __rest
 #pragma unroll
             for (unsigned k = 0; k < ENS_DISP_SLOTS; ++k) slots[k * ENS_DISP_STRIDE] = 0u;
             ens_desc const& d = descs[s];
+            uint32_t flag = 0;
             if (!skip[s]) {
-                ++steps[s];
-                if (m > d.keep_bits) {   // bits of non-negative floats order as the floats
-                    ++rebuilds[s];
+                atomicAdd(&steps[s], 1u);   // (no result: nothing waits for it)
+                if (m > d.keep_bits) {      // bits of non-negative floats order as the floats
+                    flag = 1;
+                    atomicAdd(&rebuilds[s], 1u);
                     stats[2 * s] = 0u;
                     stats[2 * s + 1] = 0u;
                     unsigned const nblk = (d.n + ENS_BLOCK - 1) / ENS_BLOCK;
@@ -927,6 +930,7 @@ __global__ __launch_bounds__(1024) void ens_decide_kernel(ens_desc const* __rest
                     }
                 }
             }
+            flags[s] = flag;
         }
         uint32_t e[V], t[V];
         block_exclusive_scan_n<V>(v, e, t, s_tmp);
@@ -1421,8 +1425,8 @@ ens_work ens_work_at(uint32_t* words, size_t nsys) {
 }
 
 void launch_ens_decide(ens_desc const* descs, unsigned nsys, uint32_t const* skip, uint32_t* disp, uint32_t* steps,
-                       uint32_t* rebuilds, uint32_t* stats, ens_work w, hipStream_t s) {
-    ens_decide_kernel<<<1, 1024, 0, s>>>(descs, nsys, skip, disp, steps, rebuilds, stats, w);
+                       uint32_t* rebuilds, uint32_t* stats, ens_work w, uint32_t* flags, hipStream_t s) {
+    ens_decide_kernel<<<1, 1024, 0, s>>>(descs, nsys, skip, disp, steps, rebuilds, stats, w, flags);
     HIP_CHECK(hipGetLastError());
 }
 

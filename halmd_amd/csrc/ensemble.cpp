@@ -47,9 +47,19 @@ template <typename T>
 uint4 const* cu4(dbuf<T> const& b) { return reinterpret_cast<uint4 const*>(b.get()); }
 } // namespace
 
-ensemble::ensemble() { HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)); }
+ensemble::ensemble() {
+    HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIP_CHECK(hipStreamCreateWithFlags(&rebuild_stream_, hipStreamNonBlocking));
+    HIP_CHECK(hipEventCreateWithFlags(&decided_, hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&rebuilt_, hipEventDisableTiming));
+}
 
-ensemble::~ensemble() { (void)hipStreamDestroy(stream); }
+ensemble::~ensemble() {
+    (void)hipEventDestroy(rebuilt_);
+    (void)hipEventDestroy(decided_);
+    (void)hipStreamDestroy(rebuild_stream_);
+    (void)hipStreamDestroy(stream);
+}
 
 uint64_t ensemble::rebuilds(unsigned s) const {
     require(s < sys_.size(), "ensemble: system index out of range");
@@ -415,7 +425,7 @@ void ensemble::apply() {
     ws_.reserve(std::max<size_t>(N, 1));
     size_t const ns = std::max<size_t>(sys_.size(), 1);
     grow(disp2_, ns * ENS_DISP_WORDS); grow(stats_, 2 * ns); grow(thermo_, 4 * ns);
-    grow(skip_, ns); grow(dsteps_, ns); grow(drebuilds_, ns); grow(work_, ens_work_words(ns));
+    grow(skip_, ns); grow(dsteps_, ns); grow(drebuilds_, ns); grow(work_, ens_work_words(ns)); grow(flags_, ns);
     grow(cnt27_, 27 * std::max<size_t>(cells_total_, 1));
     grow(base27_, 27 * std::max<size_t>(cells_total_, 1));
     grow(bsum_, cells_total_ / ENS_CELL_BLOCK + ns + 1);   // blocks of ENS_CELL_BLOCK cells, per system
@@ -466,7 +476,7 @@ void ensemble::forces_of(std::vector<unsigned> const& R) {
         size_t const a = off[2 * k], b = off[2 * k + 1], e = off[2 * k + 2];
         launch_ens_force(c.kind, c.trunc, false, c.multi, posq_, offsets_, descs_, fsub_tables_.get() + a,
                          unsigned(b - a), threads_, c.lds_records, fsub_tables_.get() + b, unsigned(e - b),
-                         c.small_max_n, list_, count_, force_, nullptr, total_, nullptr, stream);
+                         c.small_max_n, list_, count_, force_, nullptr, total_, nullptr, nullptr, stream);
     }
     HIP_CHECK(hipStreamSynchronize(stream));   // the table upload reads host memory
 }
@@ -614,11 +624,11 @@ void ensemble::set_list_capacity(unsigned s, unsigned cap) {
     forces_of({s});
 }
 
-void ensemble::forces(bool aux, float4* force, float* stress, uint32_t const* skip) {
+void ensemble::forces(bool aux, float4* force, float* stress, uint32_t const* skip, ens_fuse const* fuse) {
     for (auto& c : classes_)
         launch_ens_force(c.kind, c.trunc, aux, c.multi, posq_, offsets_, descs_, fclass_tables_.get() + c.seg_off,
                          unsigned(c.seg.size()), threads_, c.lds_records, fclass_tables_.get() + c.small_off,
-                         unsigned(c.small.size()), c.small_max_n, list_, count_, force, stress, total_, skip,
+                         unsigned(c.small.size()), c.small_max_n, list_, count_, force, stress, total_, skip, fuse,
                          stream);
 }
 
@@ -745,21 +755,43 @@ void ensemble::tick(unsigned steps, std::vector<char> const* hold, std::vector<u
                 bsum_.size() >= cells_total_ / ENS_CELL_BLOCK + ns + 1 &&
                 disp2_.size() >= ns * ENS_DISP_WORDS && skip_.size() >= ns &&
                 dsteps_.size() >= ns && drebuilds_.size() >= ns && stats_.size() >= 2 * ns &&
-                work_.size() >= ens_work_words(ns) && h_tick_.size() >= 3 * ns,
+                work_.size() >= ens_work_words(ns) && flags_.size() >= ns && h_tick_.size() >= 3 * ns,
             "ensemble: device step buffers not sized for the layout");
     ens_work const w = ens_work_at(work_.get(), ns);
+    // Step 1 integrates with the force in memory; steps 2 .. evaluate the
+    // force of their positions in the kernel that integrates (the twin
+    // snapshot receives the new positions and is swapped in), in two passes:
+    // first the systems the previous step did not rebuild, while that rebuild
+    // runs on the rebuild stream, then, once it is done, the rebuilt ones.
+    // The stretch ends with the force of the final positions in memory, its
+    // finalize kick pending.  (The two streams touch disjoint systems: the
+    // rebuild reorders only flagged systems, the first pass integrates only
+    // the others; the work tables are rewritten only after the rebuild that
+    // reads them has been waited for.)
     for (unsigned k = 0; k < steps; ++k) {
-        launch_ens_integrate(pos_, vel_, force_, image_, posq_, r0_, descs_, block_sys_, nblocks_, pending_, disp2_,
-                             skip_, stream);
-        pending_ = false;
-        launch_ens_decide(descs_, unsigned(ns), skip_, disp2_, dsteps_, drebuilds_, stats_, w, stream);
+        if (k == 0) {
+            launch_ens_integrate(pos_, vel_, force_, image_, posq_, r0_, descs_, block_sys_, nblocks_, pending_,
+                                 disp2_, skip_, stream);
+        } else {
+            ens_fuse E{pos_, vel_, image_, posq_t_, r0_, disp2_, true, 1, flags_};
+            forces(false, nullptr, nullptr, skip_, &E);
+            HIP_CHECK(hipStreamWaitEvent(stream, rebuilt_, 0));
+            E.pass = 2;
+            forces(false, nullptr, nullptr, skip_, &E);
+            std::swap(posq_, posq_t_);
+        }
+        launch_ens_decide(descs_, unsigned(ns), skip_, disp2_, dsteps_, drebuilds_, stats_, w, flags_, stream);
+        HIP_CHECK(hipEventRecord(decided_, stream));
+        HIP_CHECK(hipStreamWaitEvent(rebuild_stream_, decided_, 0));
         ens_rebuild_args const a{pos_, vel_, image_, id_, posq_, r0_, pos_t_, vel_t_, image_t_, id_t_, posq_t_,
                                  offsets_, offsets_t_, keys_, vals_, cnt27_, base27_, bsum_, descs_, w, skip_,
                                  threads_, build_lds, multi, small_max, list_, count_, stats_};
-        launch_ens_rebuild_device(a, stream);
-        forces(false, force_, nullptr, skip_);
-        pending_ = true;
+        launch_ens_rebuild_device(a, rebuild_stream_);
+        HIP_CHECK(hipEventRecord(rebuilt_, rebuild_stream_));
     }
+    HIP_CHECK(hipStreamWaitEvent(stream, rebuilt_, 0));
+    forces(false, force_, nullptr, skip_);
+    pending_ = true;
     // one readback for the whole stretch
     uint32_t* const h = h_tick_.get();
     HIP_CHECK(hipMemcpyAsync(h, skip_.get(), ns * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));

@@ -110,7 +110,11 @@ private:
     dbuf<uint32_t> id_, id_t_, count_, count_t_, offsets_, offsets_t_, disp2_, stats_, keys_, vals_, tables_;
     // device-decided steps: per-system skip words, step and rebuild counts of
     // the current tick, the rebuild work tables, 27 counts per cell
-    dbuf<uint32_t> skip_, dsteps_, drebuilds_, work_, cnt27_, base27_, bsum_;
+    dbuf<uint32_t> skip_, dsteps_, drebuilds_, work_, cnt27_, base27_, bsum_, flags_;
+    // the rebuild of a step runs on its own stream beside the next step's
+    // force of the systems it does not rebuild
+    hipStream_t rebuild_stream_ = nullptr;
+    hipEvent_t decided_ = nullptr, rebuilt_ = nullptr;
     hbuf<uint32_t> h_tick_;
     dbuf<float> stress_;
     dbuf<uint16_t> list_, list_t_, block_sys_;
@@ -148,7 +152,8 @@ private:
     // null: all); per system the steps it completed and its skip word
     void tick(unsigned steps, std::vector<char> const* hold, std::vector<uint32_t>& done,
               std::vector<uint32_t>& skip);
-    void forces(bool aux, float4* force, float* stress, uint32_t const* skip = nullptr);
+    void forces(bool aux, float4* force, float* stress, uint32_t const* skip = nullptr,
+                ens_fuse const* fuse = nullptr);
     void forces_of(std::vector<unsigned> const& R);   // plain forces of the listed systems only
     void settle();
 };

@@ -366,7 +366,7 @@ constexpr unsigned SPLIT_MAX = 8;
 // FUSE (no aux): each particle is integrated (fused_integrate) as soon as its
 // force is final, instead of having its force written.
 template <int KIND, int TRUNC, bool AUX, bool ISO, bool MULTI, bool FUSE = false>
-__device__ __forceinline__ void staged_force_segment(
+__device__ __forceinline__ float staged_force_segment(
     uint4 const* __restrict__ posq, uint32_t const* __restrict__ off, seg_params const& sp, unsigned seg,
     uint16_t const* __restrict__ list, uint32_t const* __restrict__ count, unsigned cap, force_params const& P,
     bool force_zero, float aux_weight, float4* __restrict__ force, float* __restrict__ stress, unsigned stride,
@@ -570,11 +570,9 @@ __device__ __forceinline__ void staged_force_segment(
             for (int c = 0; c < 6; ++c) stress[c * size_t(stride) + i] = sc[c];
         }
     }
-    if (FUSE) {
-        // every lane of every wave reaches this point (the loop is wave-uniform)
-        wave_max_to(d2max, disp_slot(F->disp_write));
-        if (blockIdx.x == 0 && threadIdx.x < DISP_SLOTS) F->disp_clear[threadIdx.x * DISP_STRIDE] = 0u;
-    }
+    // FUSE: the lane's largest squared displacement; every lane of every wave
+    // returns here (the loop is wave-uniform)
+    return d2max;
 }
 // Speculative launch: abort_slots holds the per-wave displacement maxima of
 // the step (DISP_SLOTS words DISP_STRIDE apart, float bits of squared
@@ -597,30 +595,68 @@ __global__ __launch_bounds__(SEG_THREADS_MAX) void pair_force_staged_kernel(
     unsigned const seg = seg_of_block(sp, blockIdx.x);   // workgroup 0 has segment 0
     if (seg >= sp.nseg) return;   // the whole workgroup
     if (displacement_aborts(abort_slots, abort_bits)) return;   // the whole workgroup
-    staged_force_segment<KIND, TRUNC, AUX, ISO, MULTI, FUSE>(posq, off, sp, seg, list, count, cap, P, force_zero,
-                                                             aux_weight, force, stress, stride, s_rec, st, s_pair,
-                                                             &F);
+    float const d2 = staged_force_segment<KIND, TRUNC, AUX, ISO, MULTI, FUSE>(
+        posq, off, sp, seg, list, count, cap, P, force_zero, aux_weight, force, stress, stride, s_rec, st, s_pair, &F);
+    if (FUSE) {
+        wave_max_to(d2, disp_slot(F.disp_write));
+        if (blockIdx.x == 0 && threadIdx.x < DISP_SLOTS) F.disp_clear[threadIdx.x * DISP_STRIDE] = 0u;
+    }
 }
 
 
+// the snapshot words of the particles segment `seg` owns, copied from posq
+// to posq_out (a system that a fused step leaves as it is)
+__device__ __forceinline__ void seg_copy_own(seg_params const& sp, unsigned seg, uint32_t const* __restrict__ off,
+                                             uint4 const* __restrict__ posq, uint4* __restrict__ posq_out) {
+    seg_place const q = seg_locate(sp, seg);
+    for (unsigned j = 0; j < q.h; ++j) {
+        unsigned b, n;
+        seg_own_row(sp, q, j, off, b, n);
+        for (unsigned t = threadIdx.x; t < n; t += blockDim.x) posq_out[b + t] = posq[b + t];
+    }
+}
+
+__device__ __forceinline__ uint32_t* ens_disp_slot(ens_fuse const& E, unsigned sys) {
+    unsigned const slot = ((blockIdx.x * blockDim.x + threadIdx.x) / WAVE) % ENS_DISP_SLOTS;
+    return E.disp + size_t(sys) * ENS_DISP_WORDS + slot * ENS_DISP_STRIDE;
+}
+
 // ensembles: one workgroup per (system, segment) of a (kind, truncation)
-// class; the per-system parameters come from the descriptor table
-template <int KIND, int TRUNC, bool AUX, bool MULTI>
+// class; the per-system parameters come from the descriptor table.  FUSE: the
+// Verlet step of each particle with its force (after the pending finalize
+// kick), the new snapshot into E.posq_out, the displacement maximum into the
+// system's slots; a skipped system's snapshot is copied over unchanged.
+template <int KIND, int TRUNC, bool AUX, bool MULTI, bool FUSE>
 __global__ __launch_bounds__(SEG_THREADS_MAX) void ens_force_kernel(
     uint4 const* __restrict__ posq, uint32_t const* __restrict__ offsets, ens_desc const* __restrict__ descs,
     uint32_t const* __restrict__ fseg, uint16_t const* __restrict__ list, uint32_t const* __restrict__ count,
-    float4* __restrict__ force, float* __restrict__ stress, unsigned stride, uint32_t const* skip)
+    float4* __restrict__ force, float* __restrict__ stress, unsigned stride, uint32_t const* skip, ens_fuse E)
 {
     extern __shared__ uint4 s_rec[];
     __shared__ seg_stage st;
     __shared__ pair_param s_pair[MAX_TYPES * MAX_TYPES];
     uint32_t const e = fseg[blockIdx.x];
     unsigned const sys = e >> 16, seg = e & 0xffffu;
-    if (skip && skip[sys]) return;   // the whole workgroup
+    if (FUSE && E.pass && (E.flags[sys] != 0) != (E.pass == 2)) return;   // the other pass's system
     ens_desc const& d = descs[sys];
-    staged_force_segment<KIND, TRUNC, AUX, true, MULTI>(
+    if (skip && skip[sys]) {   // the whole workgroup
+        if (FUSE) seg_copy_own(d.sp, seg, offsets + d.cell_base, posq + d.base, E.posq_out + d.base);
+        return;
+    }
+    fused_integrate F{};
+    if (FUSE) {
+        F.pos = E.pos + 2 * size_t(d.base);
+        F.vel = E.vel + 2 * size_t(d.base);
+        F.image = E.image + d.base;
+        F.posq_out = E.posq_out + d.base;
+        F.r0 = E.r0 + d.base;
+        F.ip = d.ip;
+        F.pre_half_dt = E.pending ? d.ip.half_dt : 0.0f;
+    }
+    float const d2 = staged_force_segment<KIND, TRUNC, AUX, true, MULTI, FUSE>(
         posq + d.base, offsets + d.cell_base, d.sp, seg, list + d.list_off, count + d.base, d.cap, d.P, true, 0.5f,
-        force + d.base, AUX ? stress + d.base : nullptr, stride, s_rec, st, s_pair);
+        force + d.base, AUX ? stress + d.base : nullptr, stride, s_rec, st, s_pair, &F);
+    if (FUSE) wave_max_to(d2, ens_disp_slot(E, sys));
 }
 
 // all-pairs systems: workgroup (x, y) takes particles y ENS_SMALL_P .. of
@@ -628,19 +664,25 @@ __global__ __launch_bounds__(SEG_THREADS_MAX) void ens_force_kernel(
 // particle (lane u the entries k = u mod 16, then a butterfly over the 16
 // lanes); list entries are particle indices (the arithmetic of the cubic-box
 // path)
-template <int KIND, int TRUNC, bool AUX>
+template <int KIND, int TRUNC, bool AUX, bool FUSE>
 __global__ __launch_bounds__(256) void ens_force_small_kernel(
     uint4 const* __restrict__ posq, ens_desc const* __restrict__ descs, uint32_t const* __restrict__ fsmall,
     uint16_t const* __restrict__ list, uint32_t const* __restrict__ count, float4* __restrict__ force,
-    float* __restrict__ stress, unsigned stride, uint32_t const* skip)
+    float* __restrict__ stress, unsigned stride, uint32_t const* skip, ens_fuse E)
 {
     constexpr unsigned T = NB16_BLOCK;   // lanes per particle: lane u reads slot u of each list block
     static_assert(256 / T == ENS_SMALL_P, "one particle per group of T lanes");
     extern __shared__ uint4 s_rec[];
-    if (skip && skip[fsmall[blockIdx.x]]) return;   // the whole workgroup
-    ens_desc const& d = descs[fsmall[blockIdx.x]];
+    unsigned const sys = fsmall[blockIdx.x];
+    if (FUSE && E.pass && (E.flags[sys] != 0) != (E.pass == 2)) return;   // the other pass's system
+    ens_desc const& d = descs[sys];
     unsigned const i0 = blockIdx.y * ENS_SMALL_P;
     if (i0 >= d.n) return;   // the whole workgroup
+    if (skip && skip[sys]) {   // the whole workgroup
+        if (FUSE && threadIdx.x < min(ENS_SMALL_P, d.n - i0))
+            E.posq_out[d.base + i0 + threadIdx.x] = posq[d.base + i0 + threadIdx.x];
+        return;
+    }
     for (unsigned t = threadIdx.x; t < d.n; t += blockDim.x) s_rec[t] = posq[d.base + t];
     __syncthreads();
     force_params const& P = d.P;
@@ -683,10 +725,20 @@ __global__ __launch_bounds__(256) void ens_force_small_kernel(
             s3 += __shfl_xor(s3, o, T); s4 += __shfl_xor(s4, o, T); s5 += __shfl_xor(s5, o, T);
         }
     }
-    if (!live || u != 0) return;
     float const qh = P.q[0], ql = P.q_lo[0];
-    force[d.base + i] = make_float4(separation_scale(fx, qh, ql), separation_scale(fy, qh, ql),
-                                    separation_scale(fz, qh, ql), AUX ? 0.5f * en : 0.0f);
+    float4 const f = make_float4(separation_scale(fx, qh, ql), separation_scale(fy, qh, ql),
+                                 separation_scale(fz, qh, ql), AUX ? 0.5f * en : 0.0f);
+    if (FUSE) {
+        float d2 = 0.0f;
+        if (live && u == 0)
+            d2 = integrate_particle_f(E.pos + 2 * size_t(d.base), E.vel + 2 * size_t(d.base), f, E.image + d.base,
+                                      E.posq_out + d.base, i, d.ip, E.pending ? d.ip.half_dt : 0.0f, nullptr,
+                                      E.r0 + d.base);
+        wave_max_to(d2, ens_disp_slot(E, sys));   // every lane of the wave
+        return;
+    }
+    if (!live || u != 0) return;
+    force[d.base + i] = f;
     if (AUX) {
         float const sv[6] = {s0, s1, s2, s3, s4, s5};
         for (int c = 0; c < 6; ++c)
@@ -1081,39 +1133,45 @@ struct ens_force_args {
     float* stress;
     unsigned stride;
     uint32_t const* skip;
+    ens_fuse const* fuse;
     hipStream_t s;
 };
 
-template <int KIND, int TRUNC, bool AUX>
+template <int KIND, int TRUNC, bool AUX, bool FUSE>
 void ens_force_go(ens_force_args const& a) {
     size_t const lds = size_t(a.lds_records) * sizeof(uint4);
+    ens_fuse const E = a.fuse ? *a.fuse : ens_fuse{};
     unsigned nfseg = a.nfseg;
     if (nfseg) {
         if constexpr (KIND == POTENTIAL_LENNARD_JONES) {
             if (a.multi) {
-                ens_force_kernel<KIND, TRUNC, AUX, true><<<nfseg, a.threads, lds, a.s>>>(
-                    a.posq, a.offsets, a.descs, a.fseg, a.list, a.count, a.force, a.stress, a.stride, a.skip);
+                ens_force_kernel<KIND, TRUNC, AUX, true, FUSE><<<nfseg, a.threads, lds, a.s>>>(
+                    a.posq, a.offsets, a.descs, a.fseg, a.list, a.count, a.force, a.stress, a.stride, a.skip, E);
                 nfseg = 0;
             }
         } else {
             if (a.multi) throw error(HALMD_ERR_INVALID_ARGUMENT, "ensemble: several species need Lennard-Jones");
         }
         if (nfseg)
-            ens_force_kernel<KIND, TRUNC, AUX, false><<<nfseg, a.threads, lds, a.s>>>(
-                a.posq, a.offsets, a.descs, a.fseg, a.list, a.count, a.force, a.stress, a.stride, a.skip);
+            ens_force_kernel<KIND, TRUNC, AUX, false, FUSE><<<nfseg, a.threads, lds, a.s>>>(
+                a.posq, a.offsets, a.descs, a.fseg, a.list, a.count, a.force, a.stress, a.stride, a.skip, E);
     }
     if (a.nsmall)
-        ens_force_small_kernel<KIND, TRUNC, AUX>
+        ens_force_small_kernel<KIND, TRUNC, AUX, FUSE>
             <<<dim3(a.nsmall, div_up(a.small_max_n, ENS_SMALL_P)), 256, size_t(a.small_max_n) * sizeof(uint4), a.s>>>(
-                a.posq, a.descs, a.fsmall, a.list, a.count, a.force, a.stress, a.stride, a.skip);
+                a.posq, a.descs, a.fsmall, a.list, a.count, a.force, a.stress, a.stride, a.skip, E);
 }
 
 template <int KIND, int TRUNC>
 void ens_force_aux(bool aux, ens_force_args const& a) {
-    if (aux)
-        ens_force_go<KIND, TRUNC, true>(a);
-    else
-        ens_force_go<KIND, TRUNC, false>(a);
+    if (a.fuse) {
+        if (aux) throw error(HALMD_ERR_INVALID_ARGUMENT, "ensemble: the fused step has no aux path");
+        ens_force_go<KIND, TRUNC, false, true>(a);
+    } else if (aux) {
+        ens_force_go<KIND, TRUNC, true, false>(a);
+    } else {
+        ens_force_go<KIND, TRUNC, false, false>(a);
+    }
 }
 
 template <int KIND>
@@ -1132,10 +1190,10 @@ void launch_ens_force(int kind, int trunc, bool aux, bool multi, uint4 const* po
                       ens_desc const* descs, uint32_t const* fseg, unsigned nfseg, unsigned threads,
                       unsigned lds_records, uint32_t const* fsmall, unsigned nsmall, unsigned small_max_n,
                       uint16_t const* list, uint32_t const* count, float4* force, float* stress, unsigned stride,
-                      uint32_t const* skip, hipStream_t s) {
+                      uint32_t const* skip, ens_fuse const* fuse, hipStream_t s) {
     if (!nfseg && !nsmall) return;
     ens_force_args const a{multi, posq,  offsets, descs,  fseg,   nfseg, threads, lds_records, fsmall,
-                           nsmall, small_max_n, list, count, force, stress, stride, skip, s};
+                           nsmall, small_max_n, list, count, force, stress, stride, skip, fuse, s};
     if (kind == POTENTIAL_LENNARD_JONES)
         ens_force_trunc<POTENTIAL_LENNARD_JONES>(trunc, aux, a);
     else if (kind == POTENTIAL_MORSE)

@@ -434,11 +434,28 @@ void launch_ens_build_small(uint4 const* posq, ens_desc const* descs, uint32_t c
 // forces of one (kind, truncation, species) class: fseg (s << 16) | segment,
 // fsmall the all-pairs systems; aux: energy in force.w and stress (stride =
 // total); multi: pair parameters by species (Lennard-Jones, cell systems)
+// fuse (optional, no aux): instead of writing the force, each particle makes
+// its Verlet step with it (the pending finalize kick first when pending), the
+// new snapshot goes to posq_out and the displacement maximum to the system's
+// slots at disp; skipped systems have their snapshot copied to posq_out
+struct ens_fuse {
+    float4 *pos, *vel;
+    int4* image;
+    uint4* posq_out;
+    float4 const* r0;
+    uint32_t* disp;
+    bool pending;
+    // pass 1: only the systems with flags[s] == 0, pass 2: those with
+    // flags[s] == 1 (0: all); the two passes of a step together take each
+    // system once
+    int pass;
+    uint32_t const* flags;
+};
 void launch_ens_force(int kind, int trunc, bool aux, bool multi, uint4 const* posq, uint32_t const* offsets,
                       ens_desc const* descs, uint32_t const* fseg, unsigned nfseg, unsigned threads,
                       unsigned lds_records, uint32_t const* fsmall, unsigned nsmall, unsigned small_max_n,
                       uint16_t const* list, uint32_t const* count, float4* force, float* stress, unsigned stride,
-                      uint32_t const* skip, hipStream_t s);
+                      uint32_t const* skip, ens_fuse const* fuse, hipStream_t s);
 
 // Device-decided rebuilds (a step with no host round trip).  After the
 // integrate launch, ens_decide flags each system whose displacement maximum
@@ -455,10 +472,12 @@ struct ens_work {
 constexpr size_t ens_work_words(size_t nsys) { return 16 + 8 * (nsys + 1); }
 constexpr unsigned ENS_CELL_BLOCK = 256;   // cells per workgroup of the new-offset scan
 ens_work ens_work_at(uint32_t* words, size_t nsys);
-// flags, work tables, per-system step and rebuild counts (+1 each), the
-// displacement slots cleared, stats of flagged systems cleared
+// work tables, per-system step and rebuild counts (+1 each), flags[s] = 1
+// for a system to rebuild (else 0), the displacement slots cleared, stats of
+// flagged systems cleared
 void launch_ens_decide(ens_desc const* descs, unsigned nsys, uint32_t const* skip, uint32_t* disp,
-                       uint32_t* steps, uint32_t* rebuilds, uint32_t* stats, ens_work w, hipStream_t s);
+                       uint32_t* steps, uint32_t* rebuilds, uint32_t* stats, ens_work w, uint32_t* flags,
+                       hipStream_t s);
 // Rebuild of the flagged cell systems.  The particles of a system are in the
 // order of its previous rebuild (cells ascending, offsets[] describe it), and
 // none moves farther than into one of the 26 cells around its old one (one
