@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 
@@ -48,6 +49,9 @@ uint4 const* cu4(dbuf<T> const& b) { return reinterpret_cast<uint4 const*>(b.get
 } // namespace
 
 ensemble::ensemble() {
+    // particles per cell segment (one 256-thread workgroup of the list build
+    // and the force); HALMD_SEG_TARGET overrides, as for single systems
+    if (char const* e = std::getenv("HALMD_SEG_TARGET")) seg_target_ = std::max(1.0, std::atof(e));
     HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIP_CHECK(hipStreamCreateWithFlags(&rebuild_stream_, hipStreamNonBlocking));
     HIP_CHECK(hipEventCreateWithFlags(&decided_, hipEventDisableTiming));
@@ -176,7 +180,7 @@ void ensemble::prepare(sys_host& h) {
         double const occ = double(h.n) / h.ncells;
         cell_grid g;
         g.ncell[0] = g.ncell[1] = g.ncell[2] = h.ncell;
-        segment_shape(occ, g, 200.0, h.S, h.R);
+        segment_shape(occ, g, seg_target_, h.S, h.R);
         h.nsx = unsigned(div_up(h.ncell, h.S));
         h.nsy = unsigned(div_up(h.ncell, h.R));
         h.nseg = h.nsx * h.nsy * h.ncell;
@@ -624,12 +628,13 @@ void ensemble::set_list_capacity(unsigned s, unsigned cap) {
     forces_of({s});
 }
 
-void ensemble::forces(bool aux, float4* force, float* stress, uint32_t const* skip, ens_fuse const* fuse) {
+void ensemble::forces(bool aux, float4* force, float* stress, uint32_t const* skip, ens_fuse const* fuse,
+                      hipStream_t s) {
     for (auto& c : classes_)
         launch_ens_force(c.kind, c.trunc, aux, c.multi, posq_, offsets_, descs_, fclass_tables_.get() + c.seg_off,
                          unsigned(c.seg.size()), threads_, c.lds_records, fclass_tables_.get() + c.small_off,
                          unsigned(c.small.size()), c.small_max_n, list_, count_, force, stress, total_, skip, fuse,
-                         stream);
+                         s ? s : stream);
 }
 
 void ensemble::settle() {
@@ -761,13 +766,13 @@ void ensemble::tick(unsigned steps, std::vector<char> const* hold, std::vector<u
     // Step 1 integrates with the force in memory; steps 2 .. evaluate the
     // force of their positions in the kernel that integrates (the twin
     // snapshot receives the new positions and is swapped in), in two passes:
-    // first the systems the previous step did not rebuild, while that rebuild
-    // runs on the rebuild stream, then, once it is done, the rebuilt ones.
-    // The stretch ends with the force of the final positions in memory, its
-    // finalize kick pending.  (The two streams touch disjoint systems: the
-    // rebuild reorders only flagged systems, the first pass integrates only
-    // the others; the work tables are rewritten only after the rebuild that
-    // reads them has been waited for.)
+    // on the main stream the systems the previous step did not rebuild, on the
+    // rebuild stream, after that rebuild, the rebuilt ones.  The stretch ends
+    // with the force of the final positions in memory, its finalize kick
+    // pending.  (The two streams touch disjoint systems: the rebuild and the
+    // second pass only flagged ones, the first pass only the others; the work
+    // tables and flags are rewritten only after the rebuild stream's work of
+    // the step has been waited for.)
     for (unsigned k = 0; k < steps; ++k) {
         if (k == 0) {
             launch_ens_integrate(pos_, vel_, force_, image_, posq_, r0_, descs_, block_sys_, nblocks_, pending_,
@@ -775,9 +780,10 @@ void ensemble::tick(unsigned steps, std::vector<char> const* hold, std::vector<u
         } else {
             ens_fuse E{pos_, vel_, image_, posq_t_, r0_, disp2_, true, 1, flags_};
             forces(false, nullptr, nullptr, skip_, &E);
-            HIP_CHECK(hipStreamWaitEvent(stream, rebuilt_, 0));
             E.pass = 2;
-            forces(false, nullptr, nullptr, skip_, &E);
+            forces(false, nullptr, nullptr, skip_, &E, rebuild_stream_);
+            HIP_CHECK(hipEventRecord(rebuilt_, rebuild_stream_));
+            HIP_CHECK(hipStreamWaitEvent(stream, rebuilt_, 0));
             std::swap(posq_, posq_t_);
         }
         launch_ens_decide(descs_, unsigned(ns), skip_, disp2_, dsteps_, drebuilds_, stats_, w, flags_, stream);

@@ -97,6 +97,7 @@ public:
 private:
     std::vector<std::unique_ptr<sys_host>> sys_;
     bool committed_ = false, pending_ = false, device_steps_ = true;
+    double seg_target_ = 200;
     uint64_t recoveries_ = 0;
     uint64_t next_uid_ = 1;
     unsigned total_ = 0, nblocks_ = 0, threads_ = SEG_THREADS_MAX;
@@ -153,7 +154,7 @@ private:
     void tick(unsigned steps, std::vector<char> const* hold, std::vector<uint32_t>& done,
               std::vector<uint32_t>& skip);
     void forces(bool aux, float4* force, float* stress, uint32_t const* skip = nullptr,
-                ens_fuse const* fuse = nullptr);
+                ens_fuse const* fuse = nullptr, hipStream_t s = nullptr);
     void forces_of(std::vector<unsigned> const& R);   // plain forces of the listed systems only
     void settle();
 };
