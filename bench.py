@@ -169,8 +169,16 @@ def batched_run(systems, steps, warmup, seed, plain_steps=1000):
     from halmd_amd import workloads
     specs = workloads.batched_specs(systems, seed=seed)
     ntot = sum(s["n"] for s in specs)
+    # the autotuner picks cell size (skin), segment shape and workgroup size
+    # by timing the same systems from their start
     t0 = time.perf_counter()
-    e = workloads.ensemble_from_specs(specs, timestep=0.003)
+    best, table = workloads.tune_ensemble(specs, timestep=0.003)
+    skin, segments = best["skin"], best["segments"]
+    log(f"ensemble autotuner ({len(table)} candidates, {time.perf_counter() - t0:.1f} s): skin {skin}, "
+        f"{segments[0]} particles per segment, {segments[1]} threads per workgroup "
+        f"({best['ms_per_step']:.3f} ms/step over 128 steps)")
+    t0 = time.perf_counter()
+    e = workloads.ensemble_from_specs(specs, timestep=0.003, skin=skin, segments=segments)
     log(f"ensemble of {len(specs)} systems, {ntot} particles, setup {time.perf_counter() - t0:.1f} s")
     e.step(warmup)
     pot0, kin0 = e.energies()
@@ -181,7 +189,7 @@ def batched_run(systems, steps, warmup, seed, plain_steps=1000):
     sample = specs[::16]
     psum, tsum = 0.0, 0.0
     for sp in sample:
-        w = workloads.single_from_spec(sp, timestep=0.003)
+        w = workloads.single_from_spec(sp, timestep=0.003, skin=skin)
         w.sampler.run(5)
         t = w.sampler.run_timed(plain_steps)
         psum += sp["n"] * plain_steps
@@ -191,7 +199,12 @@ def batched_run(systems, steps, warmup, seed, plain_steps=1000):
     drift = np.abs((pot1 + kin1) - (pot0 + kin0)) / (steps * 0.003)
     return {
         "workload": f"{len(specs)} independent NVE systems, N log-uniform in [500, 32000] (total {ntot}), "
-                    "rho U[0.7, 1.2], T U[0.5, 2.0], LJ rc 2.5 / WCA / Morse rc 2.5, dt 0.003, skin 0.3",
+                    f"rho U[0.7, 1.2], T U[0.5, 2.0], LJ rc 2.5 / WCA / Morse rc 2.5, dt 0.003, skin {skin} "
+                    "(autotuned)",
+        "autotuned": {"skin": skin, "segment_particles": segments[0], "workgroup": segments[1],
+                      "candidates": [[t["skin"], t["segments"][0], t["segments"][1],
+                                      None if t["ms_per_step"] is None else round(t["ms_per_step"], 4)]
+                                     for t in table]},
         "systems": len(specs),
         "particles": ntot,
         "steps": steps,

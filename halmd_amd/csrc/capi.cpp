@@ -967,6 +967,9 @@ HALMD_API int halmd_ensemble_recoveries(const halmd_ensemble* e, uint64_t* count
         *count = as<ensemble>(e)->recoveries();
     });
 }
+HALMD_API int halmd_ensemble_set_segments(halmd_ensemble* e, double target, unsigned threads) {
+    return guard([&] { as<ensemble>(e)->set_segments(target, threads); });
+}
 HALMD_API int halmd_ensemble_set_list_capacity(halmd_ensemble* e, unsigned system, unsigned capacity) {
     return guard([&] {
         auto* x = as<ensemble>(e);

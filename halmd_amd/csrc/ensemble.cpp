@@ -52,6 +52,9 @@ ensemble::ensemble() {
     // particles per cell segment (one 256-thread workgroup of the list build
     // and the force); HALMD_SEG_TARGET overrides, as for single systems
     if (char const* e = std::getenv("HALMD_SEG_TARGET")) seg_target_ = std::max(1.0, std::atof(e));
+    // stretches replayed from hipGraphs on request (HALMD_ENSEMBLE_GRAPHS=1):
+    // measured 3% slower than eager launches on the 256-system workload
+    if (char const* e = std::getenv("HALMD_ENSEMBLE_GRAPHS")) use_graphs_ = std::atoi(e) != 0;
     HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIP_CHECK(hipStreamCreateWithFlags(&rebuild_stream_, hipStreamNonBlocking));
     HIP_CHECK(hipEventCreateWithFlags(&decided_, hipEventDisableTiming));
@@ -59,6 +62,7 @@ ensemble::ensemble() {
 }
 
 ensemble::~ensemble() {
+    for (auto& g : graphs_) (void)hipGraphExecDestroy(g.exec);
     (void)hipEventDestroy(rebuilt_);
     (void)hipEventDestroy(decided_);
     (void)hipStreamDestroy(rebuild_stream_);
@@ -628,6 +632,14 @@ void ensemble::set_list_capacity(unsigned s, unsigned cap) {
     forces_of({s});
 }
 
+void ensemble::set_segments(double target, unsigned threads) {
+    require(!committed_, "ensemble: segment shapes are set before the first placement");
+    require(target >= 1 && threads % WAVE == 0 && threads >= WAVE && threads <= SEG_THREADS_MAX,
+            "ensemble: invalid segment target or workgroup size");
+    seg_target_ = target;
+    threads_ = threads;
+}
+
 void ensemble::forces(bool aux, float4* force, float* stress, uint32_t const* skip, ens_fuse const* fuse,
                       hipStream_t s) {
     for (auto& c : classes_)
@@ -699,12 +711,11 @@ void ensemble::step_host(unsigned steps) {
 // evaluated, and it runs the steps it missed with every other system held;
 // its trajectory is the one it would have had without stopping.
 void ensemble::step_device(unsigned steps) {
-    constexpr unsigned CHUNK = 64;
     size_t const ns = sys_.size();
     std::vector<uint32_t> left(ns), done, skip;
     std::vector<char> hold(ns);
-    for (unsigned k0 = 0; k0 < steps; k0 += CHUNK) {
-        unsigned const K = std::min(CHUNK, steps - k0);
+    for (unsigned k0 = 0; k0 < steps; k0 += STRETCH) {
+        unsigned const K = std::min(STRETCH, steps - k0);
         std::fill(left.begin(), left.end(), K);
         bool all = true;
         while (true) {
@@ -734,34 +745,8 @@ void ensemble::step_device(unsigned steps) {
     }
 }
 
-void ensemble::tick(unsigned steps, std::vector<char> const* hold, std::vector<uint32_t>& done,
-                    std::vector<uint32_t>& skip) {
+void ensemble::enqueue_stretch(unsigned steps, unsigned build_lds, bool multi, unsigned small_max) {
     size_t const ns = sys_.size();
-    if (hold) {
-        for (size_t s = 0; s < ns; ++s) h_tick_[s] = (*hold)[s] ? ENS_SKIP_HOLD : 0u;
-        HIP_CHECK(hipMemcpyAsync(skip_.get(), h_tick_.get(), ns * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
-    }
-    unsigned build_lds = 0, small_max = 0;
-    bool multi = false;
-    for (auto const& h : sys_) {
-        if (h->small) {
-            small_max = std::max(small_max, h->n);
-        } else {
-            build_lds = std::max(build_lds, h->lds_cap);
-            multi = multi || h->ntype > 1;
-        }
-    }
-    // every buffer the device-decided rebuild writes, at the sizes its kernels index
-    size_t const N = total_;
-    require(pos_t_.size() >= 2 * N && vel_t_.size() >= 2 * N && posq_t_.size() >= N && image_t_.size() >= N &&
-                id_t_.size() >= N && keys_.size() >= N && vals_.size() >= N && r0_.size() >= N &&
-                offsets_.size() >= cells_total_ && offsets_t_.size() >= cells_total_ &&
-                cnt27_.size() >= 27 * cells_total_ && base27_.size() >= 27 * cells_total_ &&
-                bsum_.size() >= cells_total_ / ENS_CELL_BLOCK + ns + 1 &&
-                disp2_.size() >= ns * ENS_DISP_WORDS && skip_.size() >= ns &&
-                dsteps_.size() >= ns && drebuilds_.size() >= ns && stats_.size() >= 2 * ns &&
-                work_.size() >= ens_work_words(ns) && flags_.size() >= ns && h_tick_.size() >= 3 * ns,
-            "ensemble: device step buffers not sized for the layout");
     ens_work const w = ens_work_at(work_.get(), ns);
     // Step 1 integrates with the force in memory; steps 2 .. evaluate the
     // force of their positions in the kernel that integrates (the twin
@@ -798,6 +783,89 @@ void ensemble::tick(unsigned steps, std::vector<char> const* hold, std::vector<u
     HIP_CHECK(hipStreamWaitEvent(stream, rebuilt_, 0));
     forces(false, force_, nullptr, skip_);
     pending_ = true;
+}
+
+// every value the launches of a stretch take from the host
+std::vector<uint64_t> ensemble::stretch_key(unsigned steps, unsigned build_lds, bool multi, unsigned small_max) const {
+    auto u = [](void const* p) { return uint64_t(reinterpret_cast<uintptr_t>(p)); };
+    std::vector<uint64_t> k = {steps, pending_, build_lds, multi, small_max, sys_.size(), total_, nblocks_, threads_,
+                               u(pos_.get()), u(vel_.get()), u(force_.get()), u(r0_.get()), u(pos_t_.get()),
+                               u(vel_t_.get()), u(posq_.get()), u(posq_t_.get()), u(image_.get()), u(image_t_.get()),
+                               u(id_.get()), u(id_t_.get()), u(count_.get()), u(offsets_.get()), u(offsets_t_.get()),
+                               u(disp2_.get()), u(stats_.get()), u(keys_.get()), u(vals_.get()), u(skip_.get()),
+                               u(dsteps_.get()), u(drebuilds_.get()), u(work_.get()), u(cnt27_.get()),
+                               u(base27_.get()), u(bsum_.get()), u(flags_.get()), u(list_.get()),
+                               u(block_sys_.get()), u(descs_.get()), u(fclass_tables_.get())};
+    for (auto const& c : classes_) {
+        k.insert(k.end(), {uint64_t(c.kind), uint64_t(c.trunc), uint64_t(c.multi), c.seg_off, c.seg.size(),
+                           c.small_off, c.small.size(), c.lds_records, c.small_max_n});
+    }
+    return k;
+}
+
+void ensemble::tick(unsigned steps, std::vector<char> const* hold, std::vector<uint32_t>& done,
+                    std::vector<uint32_t>& skip) {
+    size_t const ns = sys_.size();
+    if (hold) {
+        for (size_t s = 0; s < ns; ++s) h_tick_[s] = (*hold)[s] ? ENS_SKIP_HOLD : 0u;
+        HIP_CHECK(hipMemcpyAsync(skip_.get(), h_tick_.get(), ns * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    }
+    unsigned build_lds = 0, small_max = 0;
+    bool multi = false;
+    for (auto const& h : sys_) {
+        if (h->small) {
+            small_max = std::max(small_max, h->n);
+        } else {
+            build_lds = std::max(build_lds, h->lds_cap);
+            multi = multi || h->ntype > 1;
+        }
+    }
+    // every buffer the device-decided rebuild writes, at the sizes its kernels index
+    size_t const N = total_;
+    require(pos_t_.size() >= 2 * N && vel_t_.size() >= 2 * N && posq_t_.size() >= N && image_t_.size() >= N &&
+                id_t_.size() >= N && keys_.size() >= N && vals_.size() >= N && r0_.size() >= N &&
+                offsets_.size() >= cells_total_ && offsets_t_.size() >= cells_total_ &&
+                cnt27_.size() >= 27 * cells_total_ && base27_.size() >= 27 * cells_total_ &&
+                bsum_.size() >= cells_total_ / ENS_CELL_BLOCK + ns + 1 &&
+                disp2_.size() >= ns * ENS_DISP_WORDS && skip_.size() >= ns &&
+                dsteps_.size() >= ns && drebuilds_.size() >= ns && stats_.size() >= 2 * ns &&
+                work_.size() >= ens_work_words(ns) && flags_.size() >= ns && h_tick_.size() >= 3 * ns,
+            "ensemble: device step buffers not sized for the layout");
+    // a stretch of the usual length runs from a cached hipGraph of its
+    // launches (captured on first use; the key holds every value the
+    // launches bake in), any other length eagerly
+    if (use_graphs_ && steps == STRETCH) {
+        std::vector<uint64_t> const key = stretch_key(steps, build_lds, multi, small_max);
+        auto it = std::find_if(graphs_.begin(), graphs_.end(), [&](stretch_graph const& g) { return g.key == key; });
+        if (it != graphs_.end()) {
+            HIP_CHECK(hipGraphLaunch(it->exec, stream));
+            if ((steps - 1) % 2) std::swap(posq_, posq_t_);   // as enqueue_stretch leaves them
+            pending_ = true;
+        } else {
+            hipGraph_t g = nullptr;
+            HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+            try {
+                enqueue_stretch(steps, build_lds, multi, small_max);
+            } catch (...) {
+                (void)hipStreamEndCapture(stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIP_CHECK(hipStreamEndCapture(stream, &g));
+            hipGraphExec_t exec = nullptr;
+            hipError_t const e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIP_CHECK(e);
+            if (graphs_.size() >= 4) {
+                (void)hipGraphExecDestroy(graphs_.front().exec);
+                graphs_.erase(graphs_.begin());
+            }
+            graphs_.push_back({key, exec});
+            HIP_CHECK(hipGraphLaunch(exec, stream));
+        }
+    } else {
+        enqueue_stretch(steps, build_lds, multi, small_max);
+    }
     // one readback for the whole stretch
     uint32_t* const h = h_tick_.get();
     HIP_CHECK(hipMemcpyAsync(h, skip_.get(), ns * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));

@@ -65,6 +65,9 @@ public:
     // largest list it holds now, rounded up to a whole block); later
     // rebuilds enlarge it again when a list does not fit
     void set_list_capacity(unsigned sys, unsigned cap);
+    // particles per cell segment and workgroup size of the list build and
+    // force kernels (before the first placement)
+    void set_segments(double target, unsigned threads);
     // per system: {sum m v^2, sum en_pot, sum tr stress, n} (FP64)
     void thermo(double* out);
     // state of one system in particle-ID order: positions (in the box),
@@ -153,6 +156,15 @@ private:
     // null: all); per system the steps it completed and its skip word
     void tick(unsigned steps, std::vector<char> const* hold, std::vector<uint32_t>& done,
               std::vector<uint32_t>& skip);
+    void enqueue_stretch(unsigned steps, unsigned build_lds, bool multi, unsigned small_max);
+    std::vector<uint64_t> stretch_key(unsigned steps, unsigned build_lds, bool multi, unsigned small_max) const;
+    static constexpr unsigned STRETCH = 64;   // device-decided steps between readbacks
+    struct stretch_graph {
+        std::vector<uint64_t> key;
+        hipGraphExec_t exec;
+    };
+    std::vector<stretch_graph> graphs_;
+    bool use_graphs_ = false;
     void forces(bool aux, float4* force, float* stress, uint32_t const* skip = nullptr,
                 ens_fuse const* fuse = nullptr, hipStream_t s = nullptr);
     void forces_of(std::vector<unsigned> const& R);   // plain forces of the listed systems only

@@ -135,6 +135,10 @@ class ensemble(Handle):
         call("halmd_ensemble_recoveries", self.handle, ctypes.byref(c))
         return c.value
 
+    def set_segments(self, target=200.0, threads=256):
+        """About `target` particles per cell segment, `threads` per workgroup (before commit)."""
+        call("halmd_ensemble_set_segments", self.handle, float(target), int(threads))
+
     def set_list_capacity(self, k, capacity=0):
         """List capacity of system k (0: its largest present list); lists rebuilt at once."""
         call("halmd_ensemble_set_list_capacity", self.handle, int(k), int(capacity))

@@ -135,16 +135,43 @@ def single_from_spec(spec, timestep=0.003, skin=0.3):
                  seed=spec["seed"], species=spec_species(spec))
 
 
-def ensemble_from_specs(specs, timestep=0.003, skin=0.3):
-    """All systems of `specs` in one committed ensemble (inputs as fluid() makes them)."""
+def ensemble_from_specs(specs, timestep=0.003, skin=0.3, segments=None):
+    """All systems of `specs` in one committed ensemble (inputs as fluid() makes
+    them); segments: (target particles per segment, workgroup size)."""
     from .ensemble import ensemble
     e = ensemble()
+    if segments is not None:
+        e.set_segments(*segments)
     for sp in specs:
         L, r0, v0, _ = system_inputs(sp["n"], sp["density"], sp["temperature"], sp["seed"])
         e.add(r0, v0, L[0], potential=sp["potential"], truncation=sp.get("truncation", "shifted"),
               smoothing=sp.get("smoothing", 0.005), timestep=timestep, skin=skin, species=spec_species(sp))
     e.commit()
     return e
+
+
+def tune_ensemble(specs, timestep=0.003, targets=(150, 200, 260), threads=(128, 256), skins=(0.25, 0.3, 0.35),
+                  warmup=32, steps=128):
+    """Autotuner of the batched path: cell size (through the skin: cells are
+    rc + skin wide), segment shape (particles per segment) and workgroup size,
+    each candidate timed over `steps` device-decided steps of the systems of
+    `specs` from their start.  Returns (best, table): best = dict(skin,
+    segments=(target, threads), ms_per_step); table = every candidate's
+    ms per step (None where a staging does not fit)."""
+    from ._lib import HalmdError
+    table = []
+    for skin in skins:
+        for tg in targets:
+            for th in threads:
+                try:
+                    e = ensemble_from_specs(specs, timestep=timestep, skin=skin, segments=(tg, th))
+                    e.step(warmup)
+                    ms = e.step_timed(steps) / steps
+                except HalmdError:
+                    ms = None   # e.g. a cell neighbourhood beyond the staging limit
+                table.append(dict(skin=skin, segments=(tg, th), ms_per_step=ms))
+    best = min((t for t in table if t["ms_per_step"] is not None), key=lambda t: t["ms_per_step"])
+    return best, table
 
 
 # ---------------------------------------------------------------------------

@@ -510,6 +510,11 @@ HALMD_API int halmd_ensemble_recoveries(const halmd_ensemble* e, uint64_t* count
  * enlarge the capacity again when a list does not fit (the reference sizes
  * lists from the occupancy, from_binning.cpp:96-119) */
 HALMD_API int halmd_ensemble_set_list_capacity(halmd_ensemble* e, unsigned system, unsigned capacity);
+/* cell segments of the list build and force kernels: about `target` particles
+ * per segment, `threads` per workgroup (a multiple of 64, at most 256); before
+ * the first commit.  No counterpart in the reference (its launch shapes are
+ * fixed, halmd/mdsim/gpu/neighbours/from_binning.cpp:96-119); for tuners. */
+HALMD_API int halmd_ensemble_set_segments(halmd_ensemble* e, double target, unsigned threads);
 
 /* ---- serving: continuous batching of MD jobs over one ensemble -----------
  * No counterpart in the reference (one system per process,

@@ -177,10 +177,25 @@ def _same_trajectories(ed, eh, count):
 
 
 def test_device_decided_steps_match_host_decided_steps():
-    # chunks of 64 steps and a remainder
+    # stretches of 64 steps and a remainder
     ed, eh = _device_and_host(SPECS, 150)
     print("rebuilds:", [ed.rebuilds(k) for k in range(len(SPECS))], "recoveries:", ed.recoveries)
     _same_trajectories(ed, eh, len(SPECS))
+
+
+def test_stretches_replayed_from_graphs_match_host_decided_steps(monkeypatch):
+    # HALMD_ENSEMBLE_GRAPHS=1: each 64-step stretch captured once per key
+    # (snapshot parity, pending kick) and then replayed; 5 stretches replay
+    # two of the captured graphs, with the program's readbacks between them
+    monkeypatch.setenv("HALMD_ENSEMBLE_GRAPHS", "1")
+    ed = workloads.ensemble_from_specs(SPECS, timestep=DT)
+    monkeypatch.delenv("HALMD_ENSEMBLE_GRAPHS")
+    eh = workloads.ensemble_from_specs(SPECS, timestep=DT)
+    eh.set_device_steps(False)
+    for chunk in (128, 192):
+        ed.step(chunk)
+        eh.step(chunk)
+        _same_trajectories(ed, eh, len(SPECS))
 
 
 def test_tight_list_capacity_recovers_on_the_host():
@@ -218,3 +233,34 @@ def test_particle_past_the_neighbouring_cells_recovers():
     print("recoveries:", ed.recoveries, "rebuilds:", ed.rebuilds(0), ed.rebuilds(1))
     assert ed.recoveries > 0 and ed.rebuilds(0) >= 25
     _same_trajectories(ed, eh, 2)
+
+
+def test_segment_shapes_change_only_the_rounding():
+    # the workgroup size does not enter the arithmetic: bitwise the same;
+    # the particles per segment set how the last wave of a segment splits its
+    # lists (the order of the FP32 sums), so another segment size gives the
+    # same lists and forces that agree to rounding
+    ref = workloads.ensemble_from_specs(SPECS, timestep=DT)
+    e = workloads.ensemble_from_specs(SPECS, timestep=DT, segments=(200, 128))
+    other = workloads.ensemble_from_specs(SPECS, timestep=DT, segments=(120, 256))
+    # (from the lattice the forces cancel; after 20 steps they do not, and the
+    # two runs have drifted apart only by the rounding of those steps)
+    ref.step(20)
+    other.step(20)
+    for k in range(len(SPECS)):
+        f0, f1 = ref.force(k), other.force(k)
+        rms = np.sqrt(np.mean(np.sum(f0 ** 2, axis=1)))
+        assert np.max(np.abs(f1 - f0)) < 1e-5 * rms, k
+        assert np.max(np.abs(ref.state(k)[0] - other.state(k)[0])) < 1e-5, k
+    e.step(20)
+    _same_trajectories(e, ref, len(SPECS))
+    ref.step(130)
+    e.step(130)
+    _same_trajectories(e, ref, len(SPECS))
+
+
+def test_ensemble_autotuner_returns_a_measured_choice():
+    best, table = workloads.tune_ensemble(SPECS[:3], timestep=DT, targets=(150, 200), threads=(256,),
+                                          skins=(0.3,), warmup=4, steps=16)
+    assert len(table) == 2 and all(t["ms_per_step"] > 0 for t in table)
+    assert best["ms_per_step"] == min(t["ms_per_step"] for t in table)
