@@ -378,17 +378,38 @@ __device__ __forceinline__ void scan16(nb16_state& s, unsigned t, unsigned const
     }
     for (; t < e; ++t) test16<MULTI, SELF>(s, s_rec[t], t, p, s_rr, rr0);
 }
+// cubic box, packed records: the hits of up to 32 candidates gathered in a
+// bit mask with no branch, then appended in ascending candidate order (one
+// store instruction per round of the wave's remaining hits, instead of one
+// per candidate)
+template <bool MULTI, bool SELF>
+__device__ __forceinline__ uint32_t hit16(nb16_state const& s, uint2 const rj, unsigned slot, uint32_t const* s_rr,
+                                          uint32_t rr0, unsigned bit) {
+    bool const inside = pass16<MULTI>(s, rj, s_rr, rr0) && (!SELF || slot != s.own);
+    return inside ? (1u << bit) : 0u;
+}
+
 template <bool MULTI, bool SELF>
 __device__ __forceinline__ void scan16(nb16_state& s, unsigned t, unsigned const e, uint2 const* s_rec,
                                        neighbour_params const&, uint32_t const* s_rr, uint32_t rr0) {
-    for (; t + 4 <= e; t += 4) {
-        uint2 const r0 = s_rec[t], r1 = s_rec[t + 1], r2 = s_rec[t + 2], r3 = s_rec[t + 3];
-        test16p<MULTI, SELF>(s, r0, t, s_rr, rr0);
-        test16p<MULTI, SELF>(s, r1, t + 1, s_rr, rr0);
-        test16p<MULTI, SELF>(s, r2, t + 2, s_rr, rr0);
-        test16p<MULTI, SELF>(s, r3, t + 3, s_rr, rr0);
+    while (t < e) {
+        unsigned const n = min(e - t, 32u);
+        uint32_t m = 0;
+        unsigned k = 0;
+        for (; k + 4 <= n; k += 4) {
+            uint2 const r0 = s_rec[t + k], r1 = s_rec[t + k + 1], r2 = s_rec[t + k + 2], r3 = s_rec[t + k + 3];
+            m |= hit16<MULTI, SELF>(s, r0, t + k, s_rr, rr0, k);
+            m |= hit16<MULTI, SELF>(s, r1, t + k + 1, s_rr, rr0, k + 1);
+            m |= hit16<MULTI, SELF>(s, r2, t + k + 2, s_rr, rr0, k + 2);
+            m |= hit16<MULTI, SELF>(s, r3, t + k + 3, s_rr, rr0, k + 3);
+        }
+        for (; k < n; ++k) m |= hit16<MULTI, SELF>(s, s_rec[t + k], t + k, s_rr, rr0, k);
+        while (m) {
+            append16(s, (t + unsigned(__builtin_ctz(m))) * 16u);
+            m &= m - 1u;
+        }
+        t += n;
     }
-    for (; t < e; ++t) test16p<MULTI, SELF>(s, s_rec[t], t, s_rr, rr0);
 }
 
 // One cell segment of the staged list build (the whole workgroup calls it);
