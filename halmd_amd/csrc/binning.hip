@@ -378,35 +378,41 @@ __device__ __forceinline__ void scan16(nb16_state& s, unsigned t, unsigned const
     }
     for (; t < e; ++t) test16<MULTI, SELF>(s, s_rec[t], t, p, s_rr, rr0);
 }
-// cubic box, packed records: the hits of up to 32 candidates gathered in a
-// bit mask with no branch, then appended in ascending candidate order (one
-// store instruction per round of the wave's remaining hits, instead of one
-// per candidate)
+// cubic box, packed records: the hits of up to 64 candidates gathered in a
+// bit mask with no branch (m = 2 m + hit), then appended in ascending
+// candidate order from the top bit: one store instruction per round of the
+// wave's remaining hits instead of one per candidate, and most rows (3 cells)
+// in one chunk.  Candidates go in groups of 4; a group may read up to 3
+// records past the row's end, which it masks out (the launches give every
+// workgroup at least 4 records of LDS past its staging, the sentinel's slot
+// and the cell-start table after it included).
 template <bool MULTI, bool SELF>
-__device__ __forceinline__ uint32_t hit16(nb16_state const& s, uint2 const rj, unsigned slot, uint32_t const* s_rr,
-                                          uint32_t rr0, unsigned bit) {
-    bool const inside = pass16<MULTI>(s, rj, s_rr, rr0) && (!SELF || slot != s.own);
-    return inside ? (1u << bit) : 0u;
+__device__ __forceinline__ uint64_t push16(uint64_t m, nb16_state const& s, uint2 const rj, unsigned slot, bool valid,
+                                           uint32_t const* s_rr, uint32_t rr0) {
+    // (bitwise: every part evaluated, no branch)
+    uint32_t const hit = uint32_t(pass16<MULTI>(s, rj, s_rr, rr0)) & uint32_t(valid) & uint32_t(!SELF || slot != s.own);
+    return (m << 1) | hit;
 }
 
 template <bool MULTI, bool SELF>
 __device__ __forceinline__ void scan16(nb16_state& s, unsigned t, unsigned const e, uint2 const* s_rec,
                                        neighbour_params const&, uint32_t const* s_rr, uint32_t rr0) {
     while (t < e) {
-        unsigned const n = min(e - t, 32u);
-        uint32_t m = 0;
-        unsigned k = 0;
-        for (; k + 4 <= n; k += 4) {
+        unsigned const n = min(e - t, 64u);
+        unsigned const g = (n + 3u) & ~3u;   // candidates pushed, n rounded up to whole groups
+        uint64_t m = 0;
+        for (unsigned k = 0; k < n; k += 4) {
             uint2 const r0 = s_rec[t + k], r1 = s_rec[t + k + 1], r2 = s_rec[t + k + 2], r3 = s_rec[t + k + 3];
-            m |= hit16<MULTI, SELF>(s, r0, t + k, s_rr, rr0, k);
-            m |= hit16<MULTI, SELF>(s, r1, t + k + 1, s_rr, rr0, k + 1);
-            m |= hit16<MULTI, SELF>(s, r2, t + k + 2, s_rr, rr0, k + 2);
-            m |= hit16<MULTI, SELF>(s, r3, t + k + 3, s_rr, rr0, k + 3);
+            m = push16<MULTI, SELF>(m, s, r0, t + k, true, s_rr, rr0);
+            m = push16<MULTI, SELF>(m, s, r1, t + k + 1, k + 1 < n, s_rr, rr0);
+            m = push16<MULTI, SELF>(m, s, r2, t + k + 2, k + 2 < n, s_rr, rr0);
+            m = push16<MULTI, SELF>(m, s, r3, t + k + 3, k + 3 < n, s_rr, rr0);
         }
-        for (; k < n; ++k) m |= hit16<MULTI, SELF>(s, s_rec[t + k], t + k, s_rr, rr0, k);
+        unsigned const top = t + g - 1;   // the candidate of bit 0
         while (m) {
-            append16(s, (t + unsigned(__builtin_ctz(m))) * 16u);
-            m &= m - 1u;
+            unsigned const b = 63u - unsigned(__builtin_clzll(m));
+            append16(s, (top - b) * 16u);
+            m ^= uint64_t(1) << b;
         }
         t += n;
     }
@@ -1421,7 +1427,8 @@ void launch_ens_build(uint4 const* posq, uint32_t const* offsets, ens_desc const
                       unsigned nrseg, unsigned threads, unsigned lds_records, bool multi, uint16_t* list,
                       uint32_t* count, uint32_t* stats, hipStream_t s) {
     if (!nrseg) return;
-    size_t const lds = size_t(lds_records) * sizeof(uint2);   // cubic boxes: packed records
+    // cubic boxes: packed records, and 4 records of slack for the scan's last group
+    size_t const lds = (size_t(lds_records) + 4) * sizeof(uint2);
     if (multi)
         ens_build_kernel<true><<<nrseg, threads, lds, s>>>(posq, offsets, descs, rseg, list, count, stats);
     else
@@ -1471,7 +1478,8 @@ void launch_ens_rebuild_device(ens_rebuild_args const& a, hipStream_t s) {
                                                              a.offsets_t, a.descs, a.skip, a.w);
     HIP_CHECK(hipGetLastError());
     if (a.lds_records) {
-        size_t const lds = size_t(a.lds_records) * sizeof(uint2);   // cubic boxes: packed records
+        // cubic boxes: packed records, and 4 records of slack for the scan's last group
+        size_t const lds = (size_t(a.lds_records) + 4) * sizeof(uint2);
         if (a.multi)
             ens_build_flagged_kernel<true><<<ENS_WORK_GRID, a.threads, lds, s>>>(a.posq, a.offsets, a.descs, a.skip,
                                                                                 a.w, a.list, a.count, a.stats);
