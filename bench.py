@@ -160,7 +160,7 @@ def cpu_baseline(w, steps):
             "cpu_model": cpu_model()}
 
 
-def batched_run(systems, steps, warmup, seed, plain_steps=1000):
+def batched_run(systems, steps, warmup, seed, plain_steps=1000, pinned=None):
     """Batched ensembles: `systems` seeded systems (N log-uniform in [500,
     32000], rho in [0.7, 1.2], T in [0.5, 2.0], LJ / WCA / Morse), NVE dt =
     0.003, stepped together for `steps` steps; against the plain per-job path
@@ -172,11 +172,15 @@ def batched_run(systems, steps, warmup, seed, plain_steps=1000):
     # the autotuner picks cell size (skin), segment shape and workgroup size
     # by timing the same systems from their start
     t0 = time.perf_counter()
-    best, table = workloads.tune_ensemble(specs, timestep=0.003)
-    skin, segments = best["skin"], best["segments"]
-    log(f"ensemble autotuner ({len(table)} candidates, {time.perf_counter() - t0:.1f} s): skin {skin}, "
-        f"{segments[0]} particles per segment, {segments[1]} threads per workgroup "
-        f"({best['ms_per_step']:.3f} ms/step over 128 steps)")
+    if pinned:   # (skin, particles per segment, workgroup size), e.g. for profiling
+        skin, segments, table = pinned[0], (pinned[1], int(pinned[2])), []
+        log(f"ensemble shape pinned: skin {skin}, {segments[0]} particles per segment, {segments[1]} threads")
+    else:
+        best, table = workloads.tune_ensemble(specs, timestep=0.003)
+        skin, segments = best["skin"], best["segments"]
+        log(f"ensemble autotuner ({len(table)} candidates, {time.perf_counter() - t0:.1f} s): skin {skin}, "
+            f"{segments[0]} particles per segment, {segments[1]} threads per workgroup "
+            f"({best['ms_per_step']:.3f} ms/step over 128 steps)")
     t0 = time.perf_counter()
     e = workloads.ensemble_from_specs(specs, timestep=0.003, skin=skin, segments=segments)
     log(f"ensemble of {len(specs)} systems, {ntot} particles, setup {time.perf_counter() - t0:.1f} s")
@@ -186,21 +190,22 @@ def batched_run(systems, steps, warmup, seed, plain_steps=1000):
     pot1, kin1 = e.energies()
     value = ntot * steps / (ms / 1e3)
     log(f"batched: {ms / steps:.3f} ms/step, {value:.4e} particle-steps/s")
-    sample = specs[::16]
-    psum, tsum = 0.0, 0.0
+    sample = specs[::16] if plain_steps else []
+    psum, tsum = 0.0, 1e-300
     for sp in sample:
         w = workloads.single_from_spec(sp, timestep=0.003, skin=skin)
         w.sampler.run(5)
         t = w.sampler.run_timed(plain_steps)
         psum += sp["n"] * plain_steps
         tsum += t / 1e3
-    plain = psum / tsum
-    log(f"plain per-job path on {len(sample)} systems: {plain:.4e} particle-steps/s ({value / plain:.1f}x)")
+    plain = psum / tsum if sample else None
+    if sample:
+        log(f"plain per-job path on {len(sample)} systems: {plain:.4e} particle-steps/s ({value / plain:.1f}x)")
     drift = np.abs((pot1 + kin1) - (pot0 + kin0)) / (steps * 0.003)
     return {
         "workload": f"{len(specs)} independent NVE systems, N log-uniform in [500, 32000] (total {ntot}), "
                     f"rho U[0.7, 1.2], T U[0.5, 2.0], LJ rc 2.5 / WCA / Morse rc 2.5, dt 0.003, skin {skin} "
-                    "(autotuned)",
+                    + ("(pinned)" if pinned else "(autotuned)"),
         "autotuned": {"skin": skin, "segment_particles": segments[0], "workgroup": segments[1],
                       "candidates": [[t["skin"], t["segments"][0], t["segments"][1],
                                       None if t["ms_per_step"] is None else round(t["ms_per_step"], 4)]
@@ -212,14 +217,15 @@ def batched_run(systems, steps, warmup, seed, plain_steps=1000):
         "particle_steps_per_s": value,
         "plain_per_job_particle_steps_per_s": plain,
         "plain_sample": f"every 16th system ({len(sample)}), {plain_steps} timed steps each, one after another",
-        "speedup_vs_plain": value / plain,
+        "speedup_vs_plain": value / plain if plain else None,
         "energy_drift_per_particle_per_tau_max": float(drift.max()),
     }
 
 
 def batched(args):
     """--workload batched: the batched figures as the JSON line."""
-    b = batched_run(args.systems, args.steps, args.warmup, args.seed)
+    pinned = [float(x) for x in args.pin_ensemble.split(",")] if args.pin_ensemble else None
+    b = batched_run(args.systems, args.steps, args.warmup, args.seed, plain_steps=args.plain_steps, pinned=pinned)
     out = {
         "metric": "aggregate particle-steps/s, batched ensemble of 256 systems vs plain per-job GPU path",
         "value": b["particle_steps_per_s"],
@@ -491,6 +497,10 @@ def main():
     ap.add_argument("--max-jobs", type=int, default=256, help="serving: running-job cap")
     ap.add_argument("--budget-gib", type=int, default=128, help="serving: device memory budget for jobs")
     ap.add_argument("--systems", type=int, default=256, help="systems of the batched workload")
+    ap.add_argument("--pin-ensemble", default=None,
+                    help="batched: SKIN,PARTICLES_PER_SEGMENT,THREADS instead of the autotuner's choice")
+    ap.add_argument("--plain-steps", type=int, default=1000,
+                    help="batched: steps of each sampled plain per-job run (0: none)")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="run one copy kernel of this size first (hardware-counter calibration)")
