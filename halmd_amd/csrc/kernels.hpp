@@ -38,6 +38,8 @@ void launch_iota(uint32_t* out, size_t n, hipStream_t s);
 void launch_fill_u32(uint32_t* out, uint32_t value, size_t n, hipStream_t s);
 // copy of bytes / 16 16-byte words (a multiple of 16 bytes)
 void launch_copy(void const* src, void* dst, size_t bytes, hipStream_t s);
+// n words (a kernel rather than a DMA copy: shorter latency for small arrays)
+void launch_copy_u32(uint32_t const* src, uint32_t* dst, size_t n, hipStream_t s);
 // VALU rate probe (blocks workgroups of 256 threads); returns the number of
 // FP32 FMA wave instructions it issues
 uint64_t launch_valu_probe(float* out, unsigned blocks, unsigned iterations, hipStream_t s);

@@ -735,7 +735,7 @@ void binning::reorder_particles(bool sub_bins) {
         launch_cell_offsets(keys, p.n, ncells << SUB_BITS, order_sub_offset, s, 0);
     }
     launch_iota(vals, p.n, s);        // sorted order is now memory order
-    HIP_CHECK(hipMemcpyAsync(order_offset.get(), offset.get(), order_offset.bytes(), hipMemcpyDeviceToDevice, s));
+    launch_copy_u32(offset.get(), order_offset.get(), order_offset.size(), s);
     timer.stop(s);
     // keys and offsets describe the new order as well
     version = p.position_version;

@@ -72,20 +72,27 @@ __global__ __launch_bounds__(DM_THREADS) void density_mode_kernel(float4 const* 
     }
 }
 
-// out[2k + c] = sum over blocks of partials for wavevector k
-__global__ void density_mode_final_kernel(double const* __restrict__ partials, unsigned nq, unsigned nblocks,
-                                          double* __restrict__ out) {
-    unsigned const k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nq) return;
+// out[2k + c] = sum over blocks of partials for wavevector k: one wave per
+// wavevector, lane l summing blocks l, l + 64, ..., then a butterfly (a fixed
+// order)
+__global__ __launch_bounds__(WAVE) void density_mode_final_kernel(double const* __restrict__ partials,
+                                                                  unsigned nblocks, double* __restrict__ out) {
+    unsigned const k = blockIdx.x;
     unsigned const chunk = k / DM_KCHUNK, kk = k % DM_KCHUNK;
     double a = 0, b = 0;
-    for (unsigned blk = 0; blk < nblocks; ++blk) {
+    for (unsigned blk = threadIdx.x; blk < nblocks; blk += WAVE) {
         double const* p = partials + (size_t(chunk) * nblocks + blk) * 2 * DM_KCHUNK;
         a += p[2 * kk];
         b += p[2 * kk + 1];
     }
-    out[2 * size_t(k)] = a;
-    out[2 * size_t(k) + 1] = b;
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, WAVE);
+        b += __shfl_xor(b, off, WAVE);
+    }
+    if (threadIdx.x == 0) {
+        out[2 * size_t(k)] = a;
+        out[2 * size_t(k) + 1] = b;
+    }
 }
 
 // velocity sample in particle-ID order (hi + lo in FP64)
@@ -159,9 +166,11 @@ __global__ __launch_bounds__(REDUCE_THREADS) void tcf_partials_kernel(double con
 void launch_density_mode(float4 const* pos, unsigned n, double const* q, unsigned nq, double* partials, double* out,
                          hipStream_t s) {
     if (!nq) return;
-    dim3 grid(DM_BLOCKS, div_up(nq, DM_KCHUNK));
+    // no more particle blocks than the particles fill (at most DM_BLOCKS)
+    unsigned const nblk = std::max(1u, std::min(DM_BLOCKS, div_up(n, DM_THREADS)));
+    dim3 grid(nblk, div_up(nq, DM_KCHUNK));
     density_mode_kernel<<<grid, DM_THREADS, 0, s>>>(pos, n, q, nq, partials);
-    density_mode_final_kernel<<<div_up(nq, 64), 64, 0, s>>>(partials, nq, DM_BLOCKS, out);
+    density_mode_final_kernel<<<nq, WAVE, 0, s>>>(partials, nblk, out);
     HIP_CHECK(hipGetLastError());
 }
 

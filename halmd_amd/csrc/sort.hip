@@ -162,6 +162,20 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(uint32_t const* 
     }
 }
 
+__global__ void copy_u32x2_kernel(uint32_t const* __restrict__ a, uint32_t const* __restrict__ b,
+                                  uint32_t* __restrict__ a_out, uint32_t* __restrict__ b_out, size_t n) {
+    size_t const i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) {
+        a_out[i] = a[i];
+        b_out[i] = b[i];
+    }
+}
+
+__global__ void copy_u32_kernel(uint32_t const* __restrict__ a, uint32_t* __restrict__ a_out, size_t n) {
+    size_t const i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) a_out[i] = a[i];
+}
+
 __global__ void iota_kernel(uint32_t* out, size_t n) {
     size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) out[i] = uint32_t(i);
@@ -252,9 +266,8 @@ void radix_sort_on(uint32_t* keys, uint32_t* vals, size_t n, unsigned key_bits, 
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
-    if (passes % 2) {
-        HIP_CHECK(hipMemcpyAsync(keys, kin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-        HIP_CHECK(hipMemcpyAsync(vals, vin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    if (passes % 2) {   // back into keys, vals (a kernel: a DMA copy has several microseconds of latency)
+        copy_u32x2_kernel<<<div_up(n, 256), 256, 0, s>>>(kin, vin, keys, vals, n);
     }
     HIP_CHECK(hipGetLastError());
 }
@@ -365,6 +378,12 @@ void launch_copy(void const* src, void* dst, size_t bytes, hipStream_t s) {
     size_t const n = bytes / sizeof(uint4);
     if (!n) return;
     copy_kernel<<<div_up(n, 256), 256, 0, s>>>(static_cast<uint4 const*>(src), static_cast<uint4*>(dst), n);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_copy_u32(uint32_t const* src, uint32_t* dst, size_t n, hipStream_t s) {
+    if (!n) return;
+    copy_u32_kernel<<<div_up(n, 256), 256, 0, s>>>(src, dst, n);
     HIP_CHECK(hipGetLastError());
 }
 
