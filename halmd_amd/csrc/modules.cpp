@@ -1796,9 +1796,14 @@ wavevector::wavevector(std::vector<double> const& k, double const* L, double tol
     size_t full = 0;   // shells holding max_count vectors: once all are, no triple adds any
     while (idx[2] <= msm[2] && full < nk) {
         unsigned hkl[3] = {idx[0], idx[1] - idx[0], idx[2] - idx[1]};
-        if (std::gcd(std::gcd(hkl[0], hkl[1]), hkl[2]) == 1) {
-            double const r0[3] = {unit[0] * hkl[0], unit[1] * hkl[1], unit[2] * hkl[2]};
-            double const r0n = std::sqrt(r0[0] * r0[0] + r0[1] * r0[1] + r0[2] * r0[2]);
+        double const r0[3] = {unit[0] * hkl[0], unit[1] * hkl[1], unit[2] * hkl[2]};
+        double const r0q = r0[0] * r0[0] + r0[1] * r0[1] + r0[2] * r0[2];
+        // a triple with idx[1] < idx[0] or idx[2] < idx[1] wraps to a huge
+        // component; either way only lengths up to r_max can contribute
+        // (nn >= 1 needs r0n <= k (1 + tolerance)), so the gcd is skipped
+        if (idx[1] >= idx[0] && idx[2] >= idx[1] && r0q <= r_max * r_max * (1 + 1e-12) &&
+            std::gcd(std::gcd(hkl[0], hkl[1]), hkl[2]) == 1) {
+            double const r0n = std::sqrt(r0q);
             for (size_t i = 0; i < nk; ++i) {
                 if (cnt[i] >= max_count) continue;
                 unsigned const nn = unsigned(std::floor(k[i] / r0n + 0.5));

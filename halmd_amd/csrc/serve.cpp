@@ -183,7 +183,9 @@ server::server(size_t budget_bytes, unsigned max_jobs, unsigned tick_steps)
 {
     require(max_jobs_ >= 1, "server: at least one running job");
     require(tick_ >= 1, "server: a tick is at least one step");
-    worker_ = std::thread([this] { prepare_loop(); });
+    // several: a burst of arrivals is admitted within a tick or two, and a
+    // job whose shells are not ready builds them inside the tick
+    for (int k = 0; k < 4; ++k) workers_.emplace_back([this] { prepare_loop(); });
 }
 
 server::~server() {
@@ -192,7 +194,7 @@ server::~server() {
         stop_ = true;
     }
     cv_.notify_all();
-    worker_.join();
+    for (auto& w : workers_) w.join();
 }
 
 static std::unique_ptr<wavevector> make_shells(std::vector<double> const& k, double length, double tolerance,

@@ -191,7 +191,7 @@ private:
     std::map<uint64_t, std::unique_ptr<wavevector>> ready_;     // finished
     std::map<uint64_t, bool> taken_;                            // admitted while being prepared
     bool stop_ = false;
-    std::thread worker_;
+    std::vector<std::thread> workers_;   // host threads preparing queued jobs' wavevector shells
     void prepare_loop();
     std::unique_ptr<wavevector> take_shells(job& j);
     size_t job_footprint(job_request const& r) const;
