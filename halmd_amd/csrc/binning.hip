@@ -31,6 +31,9 @@
 // the placeholder terminator, and no memset of the list is needed.
 #include "segment.hpp"
 
+#include <cstring>
+#include <vector>
+
 namespace halmd {
 namespace {
 
@@ -990,11 +993,15 @@ __global__ __launch_bounds__(1024) void ens_decide_kernel(ens_desc const* __rest
     }
 }
 
-__device__ __forceinline__ void cell_coords(unsigned c, unsigned nc, unsigned& x, unsigned& y, unsigned& z) {
-    x = c % nc;
-    unsigned const r = c / nc;
-    y = r % nc;
-    z = r / nc;
+__device__ __forceinline__ void cell_coords(unsigned c, cell_grid const& g, unsigned& x, unsigned& y, unsigned& z) {
+    x = c % g.ncell[0];
+    unsigned const r = c / g.ncell[0];
+    y = r % g.ncell[1];
+    z = r / g.ncell[1];
+}
+
+__device__ __forceinline__ unsigned cell_index(unsigned x, unsigned y, unsigned z, cell_grid const& g) {
+    return (z * g.ncell[1] + y) * g.ncell[0] + x;
 }
 
 // a - b on a ring of nc >= 3 cells, as -1, 0, +1 (or 2 when farther)
@@ -1039,9 +1046,10 @@ __global__ __launch_bounds__(256) void ens_rank_kernel(uint4 const* __restrict__
         while (g >= w.lcell[k + 1]) ++k;
         unsigned const s = w.lsys[k];
         ens_desc const& d = descs[s];
-        unsigned const cp = g - w.lcell[k], nc = d.sp.grid.ncell[0];
+        cell_grid const& cg = d.sp.grid;
+        unsigned const cp = g - w.lcell[k];
         unsigned xo, yo, zo;
-        cell_coords(cp, nc, xo, yo, zo);
+        cell_coords(cp, cg, xo, yo, zo);
         uint32_t const* const off = offsets + d.cell_base;
         unsigned const b = off[cp], e = off[cp + 1];
         unsigned cnt = 0;   // lane l < 27: particles so far with code l
@@ -1052,9 +1060,11 @@ __global__ __launch_bounds__(256) void ens_rank_kernel(uint4 const* __restrict__
             unsigned code = 31u, c = 0;
             if (act) {
                 uint4 const q = posq[d.base + i];
-                unsigned const x = cell_component(q.x, nc), y = cell_component(q.y, nc), z = cell_component(q.z, nc);
-                c = (z * nc + y) * nc + x;
-                int const ex = ring_step(xo, x, nc), ey = ring_step(yo, y, nc), ez = ring_step(zo, z, nc);
+                unsigned const x = cell_component(q.x, cg.ncell[0]), y = cell_component(q.y, cg.ncell[1]),
+                               z = cell_component(q.z, cg.ncell[2]);
+                c = cell_index(x, y, z, cg);
+                int const ex = ring_step(xo, x, cg.ncell[0]), ey = ring_step(yo, y, cg.ncell[1]),
+                          ez = ring_step(zo, z, cg.ncell[2]);
                 if (ex == 2 || ey == 2 || ez == 2)
                     far = true;
                 else
@@ -1099,18 +1109,19 @@ __global__ __launch_bounds__(ENS_CELL_BLOCK) void ens_cell_count_kernel(
         unsigned const s = w.lsys[k];
         if (skip[s]) continue;   // (no writer of skip runs during this kernel: uniform)
         ens_desc const& d = descs[s];
-        unsigned const C = d.ncells, nc = d.sp.grid.ncell[0];
+        cell_grid const& cg = d.sp.grid;
+        unsigned const C = d.ncells;
         unsigned const c = (g - w.lcb[k]) * ENS_CELL_BLOCK + threadIdx.x;
         uint32_t v = 0;
         if (c < C) {
             uint32_t const* const cnt = cnt27 + size_t(d.cell_base) * 27;
             uint32_t* const out = base27 + (size_t(d.cell_base) + c) * 27;
             unsigned x, y, z;
-            cell_coords(c, nc, x, y, z);
+            cell_coords(c, cg, x, y, z);
             int ox[3], oy[3], oz[3];
-            ascending_steps(x, nc, ox);
-            ascending_steps(y, nc, oy);
-            ascending_steps(z, nc, oz);
+            ascending_steps(x, cg.ncell[0], ox);
+            ascending_steps(y, cg.ncell[1], oy);
+            ascending_steps(z, cg.ncell[2], oz);
             uint32_t n27[27];
 #pragma unroll
             for (int a = 0; a < 3; ++a)
@@ -1118,8 +1129,8 @@ __global__ __launch_bounds__(ENS_CELL_BLOCK) void ens_cell_count_kernel(
                 for (int b = 0; b < 3; ++b)
 #pragma unroll
                     for (int e = 0; e < 3; ++e) {
-                        unsigned const src =
-                            (ring_add(z, oz[a], nc) * nc + ring_add(y, oy[b], nc)) * nc + ring_add(x, ox[e], nc);
+                        unsigned const src = cell_index(ring_add(x, ox[e], cg.ncell[0]), ring_add(y, oy[b], cg.ncell[1]),
+                                                        ring_add(z, oz[a], cg.ncell[2]), cg);
                         n27[a * 9 + b * 3 + e] = cnt[size_t(src) * 27 + step_code(ox[e], oy[b], oz[a])];
                     }
 #pragma unroll
@@ -1192,6 +1203,30 @@ __global__ __launch_bounds__(256) void ens_scatter_kernel(
         image_t[dst] = image[src];
         id_t[dst] = id[src];
         posq_t[dst] = posq[src];
+    }
+}
+
+// the same slots as a permutation: perm[slot] = li (a gather list, for a
+// particle instance that rearranges itself)
+__global__ __launch_bounds__(256) void ens_order_kernel(ens_desc const* __restrict__ descs, uint32_t const* skip,
+                                                        ens_work w, uint32_t const* __restrict__ keys,
+                                                        uint32_t const* __restrict__ vals,
+                                                        uint32_t const* __restrict__ base27,
+                                                        uint32_t const* __restrict__ offsets_t,
+                                                        uint32_t* __restrict__ perm) {
+    unsigned const nl = w.head[0];
+    uint32_t const total = w.lblk[nl];
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {   // workgroup-uniform
+        unsigned const k = work_find(w.lblk, nl, g);
+        unsigned const s = w.lsys[k];
+        if (skip[s]) continue;
+        ens_desc const& d = descs[s];
+        unsigned const li = (g - w.lblk[k]) * ENS_BLOCK + threadIdx.x;
+        if (li >= d.n) continue;
+        uint32_t const key = keys[d.base + li];
+        unsigned const c = key & ((1u << ENS_CELL_KEY_BITS) - 1u), code = key >> ENS_CELL_KEY_BITS;
+        size_t const cc = size_t(d.cell_base) + c;
+        perm[d.base + offsets_t[cc] + base27[cc * 27 + code] + vals[d.base + li]] = li;
     }
 }
 
@@ -1495,6 +1530,46 @@ void launch_ens_rebuild_device(ens_rebuild_args const& a, hipStream_t s) {
             a.posq, a.descs, a.skip, a.w, a.list, a.count, a.stats);
         HIP_CHECK(hipGetLastError());
     }
+}
+
+void merge_tables_upload(cell_grid const& g, unsigned n, merge_scratch& m, hipStream_t s) {
+    m.n = n;
+    m.ncells = g.ncell[0] * g.ncell[1] * g.ncell[2];
+    ens_desc d;
+    std::memset(&d, 0, sizeof(d));
+    d.n = n;
+    d.ncells = g.ncell[0] * g.ncell[1] * g.ncell[2];
+    d.sp.grid = g;
+    std::vector<uint32_t> words(ens_work_words(1), 0u);
+    ens_work const w = ens_work_at(words.data(), 1);
+    w.head[0] = 1;                    // one cell system, flagged
+    w.lsys[0] = 0;
+    w.lcell[1] = d.ncells;
+    w.lblk[1] = div_up(n, ENS_BLOCK);
+    w.lcb[1] = div_up(d.ncells, ENS_CELL_BLOCK);
+    HIP_CHECK(hipMemcpyAsync(m.desc, &d, sizeof(d), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(m.work, words.data(), words.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));   // the host copies are released
+}
+
+void launch_merge_cell_order(uint4 const* posq, uint32_t const* old_offsets, uint32_t* new_offsets, uint32_t* perm,
+                             merge_scratch const& m, hipStream_t s) {
+    ens_work const w = ens_work_at(m.work, 1);
+    // grids sized to this one system (the kernels stride over any grid): a
+    // wave per old cell, a workgroup per block of cells or particles
+    unsigned const rank_grid = std::max(1u, std::min(div_up(m.ncells, 256 / WAVE), 65535u));
+    unsigned const cell_grid_n = std::max(1u, std::min(div_up(m.ncells, ENS_CELL_BLOCK), 65535u));
+    unsigned const blk_grid = std::max(1u, std::min(div_up(m.n, ENS_BLOCK), 65535u));
+    launch_fill_u32(m.flag, 0u, 1, s);
+    ens_rank_kernel<<<rank_grid, 256, 0, s>>>(posq, old_offsets, m.desc, m.flag, w, m.keys, m.rank, m.cnt27);
+    HIP_CHECK(hipGetLastError());
+    ens_cell_count_kernel<<<cell_grid_n, ENS_CELL_BLOCK, 0, s>>>(m.desc, m.flag, w, m.cnt27, m.base27, new_offsets,
+                                                                m.bsum);
+    HIP_CHECK(hipGetLastError());
+    ens_cell_carry_kernel<<<cell_grid_n, ENS_CELL_BLOCK, 0, s>>>(m.desc, m.flag, w, m.bsum, new_offsets);
+    HIP_CHECK(hipGetLastError());
+    ens_order_kernel<<<blk_grid, 256, 0, s>>>(m.desc, m.flag, w, m.keys, m.rank, m.base27, new_offsets, perm);
+    HIP_CHECK(hipGetLastError());
 }
 
 void launch_ens_build_small(uint4 const* posq, ens_desc const* descs, uint32_t const* rsmall, unsigned nsmall,

@@ -472,6 +472,15 @@ HALMD_API int halmd_mdsim_neighbours_from_binning_set_segments(halmd_neighbour* 
 HALMD_API int halmd_mdsim_neighbours_from_binning_rebuilds(const halmd_neighbour* nb, uint64_t* count) {
     return guard([&] { *count = as<neighbour>(const_cast<halmd_neighbour*>(nb))->rebuild_count(); });
 }
+HALMD_API int halmd_mdsim_neighbours_from_binning_reorders(const halmd_neighbour* nb, uint64_t* merged,
+                                                          uint64_t* sorted) {
+    return guard([&] {
+        auto const* n = as<neighbour>(const_cast<halmd_neighbour*>(nb));
+        require(merged && sorted, "neighbour: null output");
+        *merged = n->bin ? n->bin->merges : 0;
+        *sorted = n->bin ? n->bin->sorts : 0;
+    });
+}
 HALMD_API int halmd_mdsim_neighbours_from_binning_on_prepend_update_sort(halmd_neighbour* hnb, halmd_sort* hs) {
     return guard([&] {
         auto* nb = as<neighbour>(hnb);

@@ -519,6 +519,25 @@ struct ens_rebuild_args {
 };
 void launch_ens_rebuild_device(ens_rebuild_args const& a, hipStream_t s);
 constexpr unsigned ENS_CELL_KEY_BITS = 27;   // cells per system below 2^27
+
+// The same merge for one particle instance: the stable cell order of its
+// present positions from the cell order of its previous binning (old_offsets
+// describe the memory order; every particle has moved at most into one of the
+// 26 cells around its old one).  Writes new_offsets (n_cells + 1 entries) and
+// perm (the gather list of the new order); flag receives ENS_SKIP_FAR if a
+// particle moved farther (then new_offsets and perm are not written).  The
+// tables come from merge_tables_upload for this grid and n.
+struct merge_scratch {
+    ens_desc* desc;         // 1 descriptor
+    uint32_t* work;         // ens_work_words(1) words
+    uint32_t *keys, *rank;  // n words each
+    uint32_t *cnt27, *base27, *bsum;   // 27 n_cells, 27 n_cells, n_cells / ENS_CELL_BLOCK + 2 words
+    uint32_t* flag;         // 1 word
+    unsigned n = 0, ncells = 0;   // set by merge_tables_upload
+};
+void merge_tables_upload(cell_grid const& g, unsigned n, merge_scratch& m, hipStream_t s);
+void launch_merge_cell_order(uint4 const* posq, uint32_t const* old_offsets, uint32_t* new_offsets, uint32_t* perm,
+                             merge_scratch const& m, hipStream_t s);
 // per-system FP64 sums, one workgroup per listed system: out[4 s] = sum m v^2
 // (velocities after the pending kick, when pending), [1] = sum en_pot, [2] =
 // sum of the stress traces (both from aux_force / stress when given), [3] = n

@@ -694,6 +694,7 @@ binning::binning(particle& p_, box const& b_, double const* r_cut, unsigned nt1,
     order_offset.resize(size_t(ncells) + 1);
     cell_pos.resize(p.n);
     ws.reserve(p.n);
+    if (char const* e = std::getenv("HALMD_BINNING_MERGE")) merge_enabled = std::atoi(e) != 0;
 }
 
 void binning::update(bool need_cell_pos) {
@@ -714,12 +715,54 @@ void binning::update(bool need_cell_pos) {
     }
 }
 
-void binning::reorder_particles(bool sub_bins) {
+void binning::reorder_particles(bool sub_bins, double displacement_bound) {
     hipStream_t s = p.stream;
-    if (!sub_bins) {
+    double cell_width = std::numeric_limits<double>::max();
+    bool three = true;
+    for (int k = 0; k < 3; ++k) {
+        cell_width = std::min(cell_width, b.L[k] / grid.ncell[k]);
+        three = three && grid.ncell[k] >= 3;
+    }
+    if (merge_flag_pending) {
+        // the last merge's check (the stream has passed it: the caller
+        // synchronised after its list build)
+        require(h_merge_flag[0] == 0, "binning: a particle moved past the cells around its old cell");
+        merge_flag_pending = false;
+    }
+    bool const merge = merge_enabled && !sub_bins && !sub_order && three && p.n > 0 &&
+                       reordered_version == p.order_version &&
+                       displacement_bound >= 0 && displacement_bound < 0.5 * cell_width &&
+                       ncells < (1u << ENS_CELL_KEY_BITS);
+    if (merge) {
+        // the memory order is the cell order of the last reorder, and no
+        // particle left the 27 cells around its old one: merge, no sort
+        if (merge_desc.size() == 0) {
+            merge_desc.resize(1);
+            merge_work.resize(ens_work_words(1));
+            merge_rank.resize(p.n);
+            merge_cnt27.resize(27 * size_t(ncells));
+            merge_base27.resize(27 * size_t(ncells));
+            merge_bsum.resize(ncells / ENS_CELL_BLOCK + 2);
+            merge_flag.resize(1);
+            h_merge_flag.resize(1);
+            merge_scratch m{merge_desc, merge_work, keys, merge_rank, merge_cnt27, merge_base27, merge_bsum,
+                            merge_flag};
+            merge_tables_upload(grid, p.n, m, s);
+        }
+        merge_scratch m{merge_desc, merge_work, keys, merge_rank, merge_cnt27, merge_base27, merge_bsum, merge_flag};
+        m.n = p.n;
+        m.ncells = ncells;
+        timer.start(s);
+        launch_merge_cell_order(p.posq, order_offset, offset, vals, m, s);
+        HIP_CHECK(hipMemcpyAsync(h_merge_flag.get(), merge_flag.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        merge_flag_pending = true;
+        p.rearrange(vals);
+        ++merges;
+    } else if (!sub_bins) {
         update(false);
         timer.start(s);
         p.rearrange(vals);
+        ++sorts;
     } else {
         // (cell, x sub-bin) order: a stable sort by the finer key keeps the
         // cells contiguous and cuts each into SUB_BINS slabs along x
@@ -737,9 +780,10 @@ void binning::reorder_particles(bool sub_bins) {
     launch_iota(vals, p.n, s);        // sorted order is now memory order
     launch_copy_u32(offset.get(), order_offset.get(), order_offset.size(), s);
     timer.stop(s);
-    // keys and offsets describe the new order as well
+    // offsets describe the new order as well (the keys only after a sort)
     version = p.position_version;
     sub_order = sub_bins;
+    reordered_version = p.order_version;
 }
 
 void binning::cell_lists(unsigned* begin_out, unsigned* end_out, unsigned* index_out) {
@@ -1045,15 +1089,19 @@ void neighbour::require_current() {
         return;
     }
     if (built && built_order == p.order_version && disp.compute() <= r_skin / 2) return;
+    // the largest displacement since the last build (and the reorder that
+    // went with it), when it is known for the present positions
+    double const bound =
+        built && built_order == p.order_version && disp.computed_version == p.position_version ? disp.value : -1;
     for (auto& f : on_prepend_update) f();
-    update();
+    update(bound);
     disp.zero();
     built_order = p.order_version;
     built = true;
     for (auto& f : on_append_update) f();
 }
 
-void neighbour::update() {
+void neighbour::update(double displacement_bound) {
     hipStream_t s = p.stream;
     if (!staged) {
         if (bin2) bin2->update(true);
@@ -1085,7 +1133,7 @@ void neighbour::update() {
     }
     // memory order := cell order (with x sub-bins for the windowed build),
     // then the segment lists
-    bin->reorder_particles(windows);
+    bin->reorder_particles(windows, displacement_bound);
     while (true) {
         launch_fill_u32(d_maxcount, 0u, 2, s);
         timer.start(s);

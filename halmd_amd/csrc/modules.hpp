@@ -231,9 +231,21 @@ public:
     // rearrange the particles into cell order (stable); offsets stay valid.
     // With sub_bins, into (cell, x sub-bin) order, and order_sub_offset
     // holds the offsets of the (cell, sub-bin) ranges
-    void reorder_particles(bool sub_bins = false);
+    // displacement_bound: an upper bound of every particle's displacement
+    // since the last reorder (as its caller's rebuild check measured it; a
+    // negative value: unknown).  Below half the narrowest cell width, with no
+    // rearrangement since, the new order is merged from the old one
+    // (launch_merge_cell_order) instead of sorted.
+    void reorder_particles(bool sub_bins = false, double displacement_bound = -1);
+    uint64_t merges = 0, sorts = 0;    // reorders of each kind
+    bool merge_enabled = true;
     dbuf<uint32_t> order_sub_offset;   // (ncells << SUB_BITS) + 1 entries
     bool sub_order = false;            // the last reorder used sub-bins
+    uint64_t reordered_version = ~uint64_t(0);   // particle order version this binning's last reorder left
+    dbuf<ens_desc> merge_desc;
+    dbuf<uint32_t> merge_work, merge_rank, merge_cnt27, merge_base27, merge_bsum, merge_flag;
+    hbuf<uint32_t> h_merge_flag;
+    bool merge_flag_pending = false;   // h_merge_flag holds the last merge's flag once the stream reaches it
     void cell_lists(unsigned* begin_out, unsigned* end_out, unsigned* index_out);
     kernel_timer timer;
 };
@@ -347,7 +359,9 @@ public:
     // them) that keep the lists: sqrt(double(m2)) <= skin / 2 exactly when
     // bits(m2) <= keep_bits
     uint32_t keep_bits = 0;
-    void update();
+    // displacement_bound: the largest displacement since the last build when
+    // known (else negative), for the binning's choice of merge or sort
+    void update(double displacement_bound = -1);
     void get_lists(unsigned* counts, unsigned* entries, unsigned stride);
     kernel_timer timer;
 private:

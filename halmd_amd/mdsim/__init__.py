@@ -352,6 +352,13 @@ class neighbour(Handle):
         call("halmd_mdsim_neighbours_from_binning_rebuilds", self.handle, ctypes.byref(v))
         return v.value
 
+    @property
+    def reorders(self):
+        """(merged, sorted): how the rebuilds put the particles into cell order."""
+        m, s = ctypes.c_uint64(), ctypes.c_uint64()
+        call("halmd_mdsim_neighbours_from_binning_reorders", self.handle, ctypes.byref(m), ctypes.byref(s))
+        return m.value, s.value
+
     def counts(self):
         out = np.zeros(self.particle.nparticle, dtype=np.uint32)
         call("halmd_mdsim_neighbours_from_binning_counts", self.handle, uptr(out))

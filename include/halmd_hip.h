@@ -264,6 +264,13 @@ HALMD_API int halmd_mdsim_neighbours_from_binning_set_segments(halmd_neighbour* 
 /* number of neighbours of each particle (memory order) */
 HALMD_API int halmd_mdsim_neighbours_from_binning_counts(halmd_neighbour* nb, unsigned* counts);
 HALMD_API int halmd_mdsim_neighbours_from_binning_rebuilds(const halmd_neighbour* nb, uint64_t* count);
+/* how the rebuilds put the particles into cell order: merged from the order of
+ * the previous rebuild (no particle left the cells around its old one, as the
+ * displacement check bounds it) or sorted (the reference always sorts,
+ * halmd/mdsim/gpu/binning.cpp:124-149); HALMD_BINNING_MERGE=0 in the
+ * environment of a new binning always sorts */
+HALMD_API int halmd_mdsim_neighbours_from_binning_reorders(const halmd_neighbour* nb, uint64_t* merged,
+                                                          uint64_t* sorted);
 /* neighbour:on_prepend_update(sort.order) (lua/halmd/mdsim/neighbour.lua.in:289-295) */
 HALMD_API int halmd_mdsim_neighbours_from_binning_on_prepend_update_sort(halmd_neighbour* nb, halmd_sort* sort);
 
