@@ -574,19 +574,19 @@ __device__ __forceinline__ float staged_force_segment(
     // returns here (the loop is wave-uniform)
     return d2max;
 }
-// Speculative launch: abort_slots holds the per-wave displacement maxima of
+// Speculative launch: chk.slots holds the per-wave displacement maxima of
 // the step (DISP_SLOTS words DISP_STRIDE apart, float bits of squared
-// displacements); when their maximum exceeds abort_bits the lists are stale and
+// displacements); when their maximum exceeds chk.bits the lists are stale and
 // every wave returns before touching memory (the caller rebuilds and launches
 // again).  Each wave reads the 64 words itself, so the decision is uniform
-// without a barrier.
+// without a barrier; workgroup 0 publishes the maximum to the host (chk.publish).
 
 template <int KIND, int TRUNC, bool AUX, bool ISO, bool MULTI, bool FUSE>
 __global__ __launch_bounds__(SEG_THREADS_MAX) void pair_force_staged_kernel(
     uint4 const* __restrict__ posq, uint32_t const* __restrict__ off, seg_params sp,
     uint16_t const* __restrict__ list, uint32_t const* __restrict__ count, unsigned cap, force_params P,
     bool force_zero, float aux_weight, float4* __restrict__ force, float* __restrict__ stress, unsigned stride,
-    uint32_t const* __restrict__ abort_slots, uint32_t abort_bits, fused_integrate F)
+    displacement_check chk, fused_integrate F)
 {
     extern __shared__ uint4 s_rec[];
     __shared__ seg_stage st;
@@ -594,7 +594,7 @@ __global__ __launch_bounds__(SEG_THREADS_MAX) void pair_force_staged_kernel(
 
     unsigned const seg = seg_of_block(sp, blockIdx.x);   // workgroup 0 has segment 0
     if (seg >= sp.nseg) return;   // the whole workgroup
-    if (displacement_aborts(abort_slots, abort_bits)) return;   // the whole workgroup
+    if (displacement_aborts(chk.slots, chk.bits, chk.publish, chk.seq)) return;   // the whole workgroup
     float const d2 = staged_force_segment<KIND, TRUNC, AUX, ISO, MULTI, FUSE>(
         posq, off, sp, seg, list, count, cap, P, force_zero, aux_weight, force, stress, stride, s_rec, st, s_pair, &F);
     if (FUSE) {
@@ -822,8 +822,7 @@ struct staged_launch {
     float4* force;
     float* stress;
     unsigned stride;
-    uint32_t const* abort_slots;
-    uint32_t abort_bits;
+    displacement_check chk;
     fused_integrate const* fuse;
     hipStream_t s;
 };
@@ -836,13 +835,13 @@ void staged_go(staged_launch const& a) {
         if (a.fuse) {
             pair_force_staged_kernel<KIND, TRUNC, false, ISO, MULTI, true><<<grid, a.sp.threads, lds, a.s>>>(
                 a.posq, a.off, a.sp, a.list, a.count, a.cap, a.p, true, a.w, a.force, a.stress, a.stride,
-                a.abort_slots, a.abort_bits, *a.fuse);
+                a.chk, *a.fuse);
             return;
         }
     }
     pair_force_staged_kernel<KIND, TRUNC, AUX, ISO, MULTI, false><<<grid, a.sp.threads, lds, a.s>>>(
         a.posq, a.off, a.sp, a.list, a.count, a.cap, a.p, a.force_zero, a.w, a.force, a.stress, a.stride,
-        a.abort_slots, a.abort_bits, fused_integrate{});
+        a.chk, fused_integrate{});
 }
 
 template <int KIND, int TRUNC>
@@ -1101,11 +1100,12 @@ void launch_pair_full(uint4 const* posq1, unsigned n1, uint4 const* posq2, unsig
 void launch_pair_force_staged(uint4 const* posq, uint32_t const* offset, seg_params const& sp, uint16_t const* list,
                               uint32_t const* count, unsigned cap, force_params const& p, bool aux, bool force_zero,
                               float aux_weight, float4* force, float* stress, unsigned stride, hipStream_t s,
-                              uint32_t const* abort_slots, uint32_t abort_bits, fused_integrate const* fuse) {
+                              displacement_check const* check, fused_integrate const* fuse) {
     if (!sp.nseg) return;
     if (fuse && aux) throw error(HALMD_ERR_INVALID_ARGUMENT, "pair force: the fused step has no aux path");
-    staged_launch const a{posq,       offset, sp,     list,   count,       cap,        p,    aux, force_zero,
-                          aux_weight, force,  stress, stride, abort_slots, abort_bits, fuse, s};
+    displacement_check const chk = check ? *check : displacement_check{};
+    staged_launch const a{posq,  offset, sp,     list,   count, cap, p,    aux, force_zero, aux_weight,
+                          force, stress, stride, chk,    fuse,  s};
     if (p.kind == POTENTIAL_LENNARD_JONES)
         staged_trunc<POTENTIAL_LENNARD_JONES>(a);
     else if (p.kind == POTENTIAL_MORSE)

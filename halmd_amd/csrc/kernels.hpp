@@ -315,15 +315,26 @@ struct fused_integrate {
     float pre_half_dt;
 };
 
+// A speculative launch's check: it does nothing when the maximum of the
+// DISP_SLOTS displacement words at slots exceeds bits (its lists are stale).
+// Given publish, the launch also stores (seq << 32) | that maximum into
+// publish[seq % PUBLISH_RING] (host-mapped memory) as it starts, so the host
+// learns the displacement without a copy behind the stream.
+struct displacement_check {
+    uint32_t const* slots;
+    uint32_t bits;
+    unsigned long long* publish;
+    uint32_t seq;
+};
+constexpr unsigned PUBLISH_RING = 4;
+
 // the same over 16-bit slot lists of cell segments (particles in cell order);
-// abort_slots (optional): a speculative launch that does nothing when the
-// maximum of the DISP_SLOTS displacement words there exceeds abort_bits;
-// fuse (optional, no aux): integrate each particle instead of writing its force
+// check (optional): a speculative launch; fuse (optional, no aux): integrate
+// each particle instead of writing its force
 void launch_pair_force_staged(uint4 const* posq, uint32_t const* offset, seg_params const& sp, uint16_t const* list,
                               uint32_t const* count, unsigned cap, force_params const& p, bool aux, bool force_zero,
                               float aux_weight, float4* force, float* stress, unsigned stride, hipStream_t s,
-                              uint32_t const* abort_slots = nullptr, uint32_t abort_bits = 0,
-                              fused_integrate const* fuse = nullptr);
+                              displacement_check const* check = nullptr, fused_integrate const* fuse = nullptr);
 
 // ---------------------------------------------------------------------------
 // integrators (integrate.hip)

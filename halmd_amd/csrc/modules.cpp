@@ -1,6 +1,7 @@
 // C++ module layer (see modules.hpp for the reference module each class mirrors).
 #include "modules.hpp"
 
+#include <chrono>
 #include <typeinfo>
 
 #include <algorithm>
@@ -803,17 +804,18 @@ max_displacement::max_displacement(particle& p_, box const& b_) : p(p_), b(b_) {
     r0.resize(p.n);
     d_max2.resize(3 * DISP_SET);
     h_max2.resize(DISP_SLOTS);
-    HIP_CHECK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
-    HIP_CHECK(hipEventCreateWithFlags(&ready_, hipEventDisableTiming));
-    HIP_CHECK(hipEventCreateWithFlags(&copied_, hipEventDisableTiming));
+    HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_pub_), PUBLISH_RING * sizeof(unsigned long long),
+                            hipHostMallocCoherent | hipHostMallocMapped));
+    for (unsigned k = 0; k < PUBLISH_RING; ++k) h_pub_[k] = 0;
+    HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&d_pub_), h_pub_, 0));
     zero();
 }
 
 max_displacement::~max_displacement() {
-    if (copied_) (void)hipEventSynchronize(copied_);
-    if (ready_) (void)hipEventDestroy(ready_);
-    if (copied_) (void)hipEventDestroy(copied_);
-    if (side_) (void)hipStreamDestroy(side_);
+    if (h_pub_) {
+        (void)hipDeviceSynchronize();   // a launch still queued may store into the words
+        (void)hipHostFree(h_pub_);
+    }
 }
 
 void max_displacement::zero() {
@@ -852,7 +854,10 @@ double max_displacement::compute() {
 
 double max_displacement::finish(uint32_t const* words) {
     // the words hold float bits of non-negative values: integer order is float order
-    uint32_t const bits = *std::max_element(words, words + DISP_SLOTS);
+    return finish_bits(*std::max_element(words, words + DISP_SLOTS));
+}
+
+double max_displacement::finish_bits(uint32_t bits) {
     float m2;
     std::memcpy(&m2, &bits, 4);
     value = std::sqrt(double(m2));
@@ -864,19 +869,36 @@ bool max_displacement::speculation_ready() const {
     return computed_version != p.position_version && tracked_version == p.position_version;
 }
 
-void max_displacement::begin_readback() {
-    // the copy waits for the work already queued on the particle stream (the
-    // integrate kernel that wrote the words), not for what follows it
-    HIP_CHECK(hipEventRecord(ready_, p.stream));
-    HIP_CHECK(hipStreamWaitEvent(side_, ready_, 0));
-    HIP_CHECK(hipMemcpy2DAsync(h_max2.get(), sizeof(uint32_t), current_slots(), DISP_STRIDE * sizeof(uint32_t),
-                               sizeof(uint32_t), DISP_SLOTS, hipMemcpyDeviceToHost, side_));
-    HIP_CHECK(hipEventRecord(copied_, side_));
+void max_displacement::begin_readback(displacement_check& chk, uint32_t keep_bits) {
+    chk.slots = current_slots();
+    chk.bits = keep_bits;
+    chk.publish = d_pub_;
+    chk.seq = pub_next_;
+    pub_wait_ = pub_next_;
+    if (++pub_next_ == 0) pub_next_ = 1;
 }
 
 double max_displacement::finish_readback() {
-    HIP_CHECK(hipEventSynchronize(copied_));
-    return finish(h_max2.get());
+    // the launch that stores the word is on the particle stream; it runs (or
+    // the stream reports an error) in finite time, so the poll is bounded
+    uint32_t const seq = pub_wait_;
+    volatile unsigned long long const* w = h_pub_ + seq % PUBLISH_RING;
+    auto const t0 = std::chrono::steady_clock::now();
+    for (unsigned k = 0;; ++k) {
+        unsigned long long const v = *w;
+        if (uint32_t(v >> 32) == seq) return finish_bits(uint32_t(v));
+        if ((k & 1023u) != 1023u) continue;
+        hipError_t const q = hipStreamQuery(p.stream);
+        if (q == hipSuccess) {
+            // the stream has drained: the store is visible by now or never comes
+            unsigned long long const u = *w;
+            if (uint32_t(u >> 32) == seq) return finish_bits(uint32_t(u));
+            throw error(HALMD_ERR_LOGIC, "max_displacement: no launch stored the displacement of the readback");
+        }
+        if (q != hipErrorNotReady) HIP_CHECK(q);
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(60))
+            throw error(HALMD_ERR_HIP, "max_displacement: the displacement readback did not arrive in 60 s");
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1376,11 +1398,10 @@ unsigned pair_trunc::tune() {
 
 void pair_trunc::apply(bool force_zero) {
     nb.require_current();
-    launch(force_zero, nullptr, 0);
+    launch(force_zero);
 }
 
-bool pair_trunc::launch(bool force_zero, uint32_t const* abort_slots, uint32_t abort_bits,
-                        fused_integrate const* fuse) {
+bool pair_trunc::launch(bool force_zero, displacement_check const* check, fused_integrate const* fuse) {
     bool const aux = p.aux_enabled && !fuse;   // the fused step evaluates the plain force
     // a particle interacting with itself counts each pair twice (pair_trunc.hpp:291-294)
     bool const same = &p == &p2;
@@ -1394,7 +1415,7 @@ bool pair_trunc::launch(bool force_zero, uint32_t const* abort_slots, uint32_t a
     timer.start(p.stream);
     if (nb.staged)
         launch_pair_force_staged(p.posq, nb.bin->order_offset, nb.sp_force, nb.list16, nb.count, nb.cap, params, aux,
-                                 force_zero, w, p.force, p.stress, p.n_pad, p.stream, abort_slots, abort_bits, fuse);
+                                 force_zero, w, p.force, p.stress, p.n_pad, p.stream, check, fuse);
     else
         launch_pair_force(p.posq, p.n, nb.list, nb.count, nb.cap, params, aux, force_zero, w, p.force, p.stress,
                           p.n_pad, p.stream, fuse, p2.posq);
@@ -1429,16 +1450,16 @@ bool pair_trunc::integrate_fused(verlet& v) {
         F.pre_half_dt = pre;
         return F;
     };
-    uint32_t const* const cur = d.current_slots();   // before begin_update moves it
-    if (spec) d.begin_readback();
+    displacement_check chk{};
+    if (spec) d.begin_readback(chk, nb.keep_bits);   // before begin_update moves the slots
     fused_integrate F = args();
-    bool const timed = launch(true, spec ? cur : nullptr, nb.keep_bits, &F);   // timed as the force
+    bool const timed = launch(true, spec ? &chk : nullptr, &F);   // timed as the force
     if (spec && d.finish_readback() > nb.r_skin / 2) {
         // the launch found its lists stale and did nothing
         if (timed) timer.drop_last();
         nb.require_current();
         F = args();
-        launch(true, nullptr, 0, &F);
+        launch(true, nullptr, &F);
     }
     std::swap(p.posq, p.posq_next);
     p.pending_half_dt = 0;
@@ -1451,14 +1472,16 @@ bool pair_trunc::integrate_fused(verlet& v) {
 
 bool pair_trunc::apply_speculative() {
     if (!nb.staged || !nb.current_but_displacement() || !nb.disp.speculation_ready()) return false;
-    nb.disp.begin_readback();
-    // the kernel reads the same displacement words and does nothing when they
-    // exceed keep_bits, exactly when the readback below asks for a rebuild
-    bool const timed = launch(true, nb.disp.current_slots(), nb.keep_bits);
+    // the kernel reads the displacement words, stores their maximum for the
+    // readback below and does nothing when they exceed keep_bits, exactly when
+    // the readback asks for a rebuild
+    displacement_check chk{};
+    nb.disp.begin_readback(chk, nb.keep_bits);
+    bool const timed = launch(true, &chk);
     if (nb.disp.finish_readback() > nb.r_skin / 2) {
         if (timed) timer.drop_last();   // that launch did no work
         nb.require_current();           // the displacement is known: rebuilds
-        launch(true, nullptr, 0);
+        launch(true);
     }
     return true;
 }

@@ -281,12 +281,22 @@ public:
     // keeps the same state as compute().
     bool speculation_ready() const;
     uint32_t const* current_slots() const { return d_max2.get() + (updates + 2) % 3 * DISP_SET; }
-    void begin_readback();
+    // Asynchronous form of compute() for positions whose maxima the integrate
+    // kernel wrote (speculation_ready): begin_readback fills the check of the
+    // staged force launch that follows on the particle stream (the words of
+    // current_slots(), a new sequence number and the host-mapped words); that
+    // launch stores their maximum there as it starts, and finish_readback polls
+    // for it and keeps the same state as compute().  No event, copy or second
+    // stream stands between the launches.
+    void begin_readback(displacement_check& chk, uint32_t keep_bits);
     double finish_readback();
 private:
-    hipStream_t side_ = nullptr;
-    hipEvent_t ready_ = nullptr, copied_ = nullptr;
+    unsigned long long* h_pub_ = nullptr;   // PUBLISH_RING words, hipHostMalloc'd (coherent, mapped)
+    unsigned long long* d_pub_ = nullptr;   // the same words, as the device addresses them
+    uint32_t pub_next_ = 1;                 // sequence number of the next readback (never 0)
+    uint32_t pub_wait_ = 0;                 // the number finish_readback polls for
     double finish(uint32_t const* words);
+    double finish_bits(uint32_t bits);
 };
 
 class hilbert;
@@ -446,8 +456,7 @@ public:
     unsigned tune();
     kernel_timer timer;
 private:
-    bool launch(bool force_zero, uint32_t const* abort_slots, uint32_t abort_bits,
-                fused_integrate const* fuse = nullptr);
+    bool launch(bool force_zero, displacement_check const* check = nullptr, fused_integrate const* fuse = nullptr);
     uint64_t p2_version_ = 0;
     hipEvent_t p2_ready_ = nullptr;
     hipEvent_t p1_done_ = nullptr;   // this kernel read p2 (p2's stream waits)

@@ -40,12 +40,18 @@ __device__ __forceinline__ void seg_own(seg_stage const& st, unsigned t, unsigne
 
 // the largest of the DISP_SLOTS displacement words at slots exceeds bits
 // (false without slots).  Each wave reads the 64 words itself, so the
-// decision is uniform without a barrier.
-__device__ __forceinline__ bool displacement_aborts(uint32_t const* __restrict__ slots, uint32_t bits) {
+// decision is uniform without a barrier.  publish (optional): lane 0 of
+// workgroup 0 stores (seq << 32) | the largest word into publish[seq %
+// PUBLISH_RING], host-mapped memory, by a vector store at system scope.
+__device__ __forceinline__ bool displacement_aborts(uint32_t const* __restrict__ slots, uint32_t bits,
+                                                    unsigned long long* publish = nullptr, uint32_t seq = 0) {
     if (!slots) return false;
     static_assert(DISP_SLOTS == WAVE, "one displacement slot per lane");
     unsigned m = slots[(threadIdx.x % WAVE) * DISP_STRIDE];
     for (int o = WAVE / 2; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o, WAVE));
+    if (publish && blockIdx.x == 0 && threadIdx.x == 0)
+        __hip_atomic_store(publish + seq % PUBLISH_RING, (static_cast<unsigned long long>(seq) << 32) | m,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     return m > bits;
 }
 

@@ -587,15 +587,20 @@ def test_device_decided_rebuild_after_host_position_change(oracle):
     assert _force_error(w.particle.data["force"], fref) < FORCE_TOL
 
 
-@pytest.mark.parametrize("kind", ["lj", "ka", "ka_all_pairs"])
+@pytest.mark.parametrize("kind", ["lj", "ka", "ka_all_pairs", "lj_256k"])
 def test_fused_step_is_bitwise_identical(kind):
     # the force kernel that integrates each particle as it finishes it gives
     # the trajectory of separate force and integrate kernels bit for bit,
-    # across rebuilds, Hilbert sorts and samples taken in between
+    # across rebuilds, Hilbert sorts and samples taken in between; the fused
+    # launches publish their displacement maximum to the host (the last of
+    # about a thousand workgroups at N = 256,000), the separate ones are read
+    # back by a copy, and both must ask for the same rebuilds
     runs = []
     for fused in (True, False):
         if kind == "lj":
             w = workloads.lennard_jones(6912, density=0.8442, temperature=1.5, seed=82, sort_interval=40)
+        elif kind == "lj_256k":
+            w = workloads.lennard_jones(256000, density=0.8442, temperature=1.5, seed=85, sort_interval=100)
         elif kind == "ka_all_pairs":
             # all-pairs lists (split-lane force kernel, rebuilds decided on the device)
             w = workloads.kob_andersen(3000, density=1.2, temperature=0.9, seed=84, binning=False)
