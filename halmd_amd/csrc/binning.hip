@@ -381,41 +381,63 @@ __device__ __forceinline__ void scan16(nb16_state& s, unsigned t, unsigned const
     }
     for (; t < e; ++t) test16<MULTI, SELF>(s, s_rec[t], t, p, s_rr, rr0);
 }
-// cubic box, packed records: the hits of up to 64 candidates gathered in a
-// bit mask with no branch (m = 2 m + hit), then appended in ascending
-// candidate order from the top bit: one store instruction per round of the
-// wave's remaining hits instead of one per candidate, and most rows (3 cells)
-// in one chunk.  Candidates go in groups of 4; a group may read up to 3
-// records past the row's end, which it masks out (the launches give every
-// workgroup at least 4 records of LDS past its staging, the sentinel's slot
-// and the cell-start table after it included).
-template <bool MULTI, bool SELF>
-__device__ __forceinline__ uint64_t push16(uint64_t m, nb16_state const& s, uint2 const rj, unsigned slot, bool valid,
-                                           uint32_t const* s_rr, uint32_t rr0) {
-    // (bitwise: every part evaluated, no branch)
-    uint32_t const hit = uint32_t(pass16<MULTI>(s, rj, s_rr, rr0)) & uint32_t(valid) & uint32_t(!SELF || slot != s.own);
-    return (m << 1) | hit;
+// cubic box, packed records: the hits of up to 32 candidates gathered in a
+// bit mask with no branch, then appended in ascending candidate order: one
+// store instruction per round of the wave's remaining hits instead of one per
+// candidate.  Each candidate costs its test and one v_addc_co_u32 (m = 2 m +
+// the lane's bit of the compare's lane mask, the mask as carry-in).
+// Candidates go in groups of 4; a group may read up to 3 records past the
+// row's end (the launches give every workgroup at least 4 records of LDS past
+// its staging, the sentinel's slot and the cell-start table after it
+// included), whose bits one mask per chunk clears, as it clears the
+// particle's own bit.
+
+// m = 2 m + (this lane's bit of lanes)
+__device__ __forceinline__ uint32_t shift_in(uint32_t m, uint64_t lanes) {
+    uint32_t r;
+    uint64_t carry;
+    asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(r), "=s"(carry) : "v"(m), "s"(lanes));
+    return r;
+}
+
+// the lanes whose candidate rj passes the test of pass16 (compare lane mask)
+template <bool MULTI>
+__device__ __forceinline__ uint64_t pass16_lanes(nb16_state const& s, uint2 const rj, uint32_t const* s_rr,
+                                                 uint32_t rr0) {
+    ushort2v const dxy = __builtin_bit_cast(ushort2v, s.ri.x) - __builtin_bit_cast(ushort2v, rj.x);
+    int const dz = short(s.ri.y - rj.y);
+    short2v const d = __builtin_bit_cast(short2v, dxy);
+    uint32_t const rq = uint32_t(__builtin_amdgcn_sdot2(d, d, dz * dz, false));
+    uint32_t const thr = MULTI ? s_rr[s.ti + (rj.y >> 16)] : rr0;
+    return __builtin_amdgcn_uicmp(rq, thr, 37);   // 37: unsigned less or equal
 }
 
 template <bool MULTI, bool SELF>
 __device__ __forceinline__ void scan16(nb16_state& s, unsigned t, unsigned const e, uint2 const* s_rec,
                                        neighbour_params const&, uint32_t const* s_rr, uint32_t rr0) {
     while (t < e) {
-        unsigned const n = min(e - t, 64u);
-        unsigned const g = (n + 3u) & ~3u;   // candidates pushed, n rounded up to whole groups
-        uint64_t m = 0;
-        for (unsigned k = 0; k < n; k += 4) {
+        unsigned const n = min(e - t, 32u);
+        uint32_t m = 0;
+        unsigned k = 0;
+        for (; k < n; k += 4) {
             uint2 const r0 = s_rec[t + k], r1 = s_rec[t + k + 1], r2 = s_rec[t + k + 2], r3 = s_rec[t + k + 3];
-            m = push16<MULTI, SELF>(m, s, r0, t + k, true, s_rr, rr0);
-            m = push16<MULTI, SELF>(m, s, r1, t + k + 1, k + 1 < n, s_rr, rr0);
-            m = push16<MULTI, SELF>(m, s, r2, t + k + 2, k + 2 < n, s_rr, rr0);
-            m = push16<MULTI, SELF>(m, s, r3, t + k + 3, k + 3 < n, s_rr, rr0);
+            m = shift_in(m, pass16_lanes<MULTI>(s, r0, s_rr, rr0));
+            m = shift_in(m, pass16_lanes<MULTI>(s, r1, s_rr, rr0));
+            m = shift_in(m, pass16_lanes<MULTI>(s, r2, s_rr, rr0));
+            m = shift_in(m, pass16_lanes<MULTI>(s, r3, s_rr, rr0));
         }
-        unsigned const top = t + g - 1;   // the candidate of bit 0
+        // k (n rounded up to whole groups) bits shifted in: candidate t + c
+        // at bit k - 1 - c, reversed at bit 32 - k + c = c - base (mod 2^32)
+        m = __builtin_bitreverse32(m);
+        unsigned const base = t + k - 32u;
+        m &= ~0u >> (k - n);   // candidates t + n .. t + k - 1 (k - n <= 3)
+        if (SELF) {
+            unsigned const ob = s.own - base;   // below 32 only inside the chunk or its empty low bits
+            m &= ob < 32u ? ~(1u << ob) : ~0u;
+        }
         while (m) {
-            unsigned const b = 63u - unsigned(__builtin_clzll(m));
-            append16(s, (top - b) * 16u);
-            m ^= uint64_t(1) << b;
+            append16(s, (base + unsigned(__builtin_ctz(m))) * 16u);
+            m &= m - 1u;
         }
         t += n;
     }
