@@ -1049,14 +1049,18 @@ __device__ __forceinline__ void ascending_steps(unsigned a, unsigned nc, int o[3
 __device__ __forceinline__ unsigned step_code(int ex, int ey, int ez) { return unsigned((ez + 1) * 9 + (ey + 1) * 3 + (ex + 1)); }
 
 // waves over runs of consecutive old cells of the flagged cell systems
+// (workgroups of 256 threads: s_cnt holds one row per wave)
 __global__ __launch_bounds__(256) void ens_rank_kernel(uint4 const* __restrict__ posq,
                                                        uint32_t const* __restrict__ offsets,
                                                        ens_desc const* __restrict__ descs, uint32_t* skip,
                                                        ens_work w, uint32_t* __restrict__ keys,
                                                        uint32_t* __restrict__ vals, uint32_t* __restrict__ cnt27) {
+    // per wave: particles so far of each of the 27 codes in the current cell
+    __shared__ uint32_t s_cnt[256 / WAVE][32];
     unsigned const nl = w.head[0];
     uint32_t const total = w.lcell[nl];
     unsigned const lane = threadIdx.x % WAVE;
+    uint32_t* const wc = s_cnt[threadIdx.x / WAVE];
     unsigned const waves = gridDim.x * (blockDim.x / WAVE);
     unsigned const wave = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
     uint64_t const below = (uint64_t(1) << lane) - 1u;
@@ -1074,7 +1078,8 @@ __global__ __launch_bounds__(256) void ens_rank_kernel(uint4 const* __restrict__
         cell_coords(cp, cg, xo, yo, zo);
         uint32_t const* const off = offsets + d.cell_base;
         unsigned const b = off[cp], e = off[cp + 1];
-        unsigned cnt = 0;   // lane l < 27: particles so far with code l
+        if (lane < 32) wc[lane] = 0u;
+        __builtin_amdgcn_wave_barrier();
         bool far = false;
         for (unsigned i0 = b; i0 < e; i0 += WAVE) {
             unsigned const i = i0 + lane;
@@ -1100,18 +1105,19 @@ __global__ __launch_bounds__(256) void ens_rank_kernel(uint4 const* __restrict__
                 peers &= on ? m : ~m;
             }
             unsigned const rank = unsigned(__popcll(peers & below));
-            unsigned const base = unsigned(__shfl(int(cnt), int(code & 31u), WAVE));
             if (act && code < 27u) {
+                // the wave's LDS instructions run in order: every lane reads
+                // its code's count before the lowest lane of each code adds
+                // the chunk's particles of that code
+                unsigned const base = wc[code];
                 keys[d.base + i] = c | (code << ENS_CELL_KEY_BITS);
                 vals[d.base + i] = base + rank;
+                if (rank == 0) wc[code] = base + unsigned(__popcll(peers));
             }
-            for (unsigned l = 0; l < 27; ++l) {
-                uint64_t const m = __ballot(code == l);
-                if (lane == l) cnt += unsigned(__popcll(m));
-            }
+            __builtin_amdgcn_wave_barrier();
         }
         if (__ballot(far) != 0 && lane == 0) atomicOr(&skip[s], ENS_SKIP_FAR);
-        if (lane < 27) cnt27[(size_t(d.cell_base) + cp) * 27 + lane] = cnt;
+        if (lane < 27) cnt27[(size_t(d.cell_base) + cp) * 27 + lane] = wc[lane];
     }
 }
 

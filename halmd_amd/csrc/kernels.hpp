@@ -40,6 +40,13 @@ void launch_fill_u32(uint32_t* out, uint32_t value, size_t n, hipStream_t s);
 void launch_copy(void const* src, void* dst, size_t bytes, hipStream_t s);
 // n words (a kernel rather than a DMA copy: shorter latency for small arrays)
 void launch_copy_u32(uint32_t const* src, uint32_t* dst, size_t n, hipStream_t s);
+// dst[k] := (seq << 32) | *src.word[k] for k < n (MAILBOX_WORDS at most; a
+// null source reads as 0), stored at system scope into host-mapped memory
+constexpr unsigned MAILBOX_WORDS = 4;
+struct mailbox_sources {
+    uint32_t const* word[MAILBOX_WORDS];
+};
+void launch_post_words(mailbox_sources const& src, unsigned n, unsigned long long* dst, uint32_t seq, hipStream_t s);
 // VALU rate probe (blocks workgroups of 256 threads); returns the number of
 // FP32 FMA wave instructions it issues
 uint64_t launch_valu_probe(float* out, unsigned blocks, unsigned iterations, hipStream_t s);

@@ -725,10 +725,10 @@ void binning::reorder_particles(bool sub_bins, double displacement_bound) {
         three = three && grid.ncell[k] >= 3;
     }
     if (merge_flag_pending) {
-        // the last merge's check (the stream has passed it: the caller
-        // synchronised after its list build)
-        require(h_merge_flag[0] == 0, "binning: a particle moved past the cells around its old cell");
-        merge_flag_pending = false;
+        // the last merge's check, when no list build read it
+        HIP_CHECK(hipMemcpyAsync(h_merge_flag.get(), merge_flag.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        p.sync();
+        merge_checked(h_merge_flag[0]);
     }
     bool const merge = merge_enabled && !sub_bins && !sub_order && three && p.n > 0 &&
                        reordered_version == p.order_version &&
@@ -755,8 +755,7 @@ void binning::reorder_particles(bool sub_bins, double displacement_bound) {
         m.ncells = ncells;
         timer.start(s);
         launch_merge_cell_order(p.posq, order_offset, offset, vals, m, s);
-        HIP_CHECK(hipMemcpyAsync(h_merge_flag.get(), merge_flag.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        merge_flag_pending = true;
+        merge_flag_pending = true;   // read by the list build that follows (merge_check_word)
         p.rearrange(vals);
         ++merges;
     } else if (!sub_bins) {
@@ -787,6 +786,14 @@ void binning::reorder_particles(bool sub_bins, double displacement_bound) {
     reordered_version = p.order_version;
 }
 
+uint32_t const* binning::merge_check_word() const { return merge_flag_pending ? merge_flag.get() : nullptr; }
+
+void binning::merge_checked(uint32_t flag) {
+    if (!merge_flag_pending) return;
+    merge_flag_pending = false;
+    require(flag == 0, "binning: a particle moved past the cells around its old cell");
+}
+
 void binning::cell_lists(unsigned* begin_out, unsigned* end_out, unsigned* index_out) {
     update();
     std::vector<uint32_t> off(size_t(ncells) + 1);
@@ -799,6 +806,50 @@ void binning::cell_lists(unsigned* begin_out, unsigned* end_out, unsigned* index
 // ---------------------------------------------------------------------------
 // max_displacement (halmd/mdsim/gpu/max_displacement.cpp:60-106)
 // ---------------------------------------------------------------------------
+mailbox::mailbox() {
+    HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_), MAILBOX_WORDS * sizeof(unsigned long long),
+                            hipHostMallocCoherent | hipHostMallocMapped));
+    for (unsigned k = 0; k < MAILBOX_WORDS; ++k) h_[k] = 0;
+    HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&d_), h_, 0));
+}
+
+mailbox::~mailbox() {
+    if (h_) {
+        (void)hipDeviceSynchronize();   // a post still queued may store into the words
+        (void)hipHostFree(h_);
+    }
+}
+
+uint32_t mailbox::post(mailbox_sources const& src, unsigned n, hipStream_t s) {
+    uint32_t const seq = next_;
+    if (++next_ == 0) next_ = 1;
+    launch_post_words(src, n, d_, seq, s);
+    return seq;
+}
+
+void mailbox::wait(uint32_t seq, unsigned n, uint32_t* out, hipStream_t s) {
+    volatile unsigned long long const* w = h_;
+    auto const ready = [&]() {
+        for (unsigned k = 0; k < n; ++k)
+            if (uint32_t(w[k] >> 32) != seq) return false;
+        return true;
+    };
+    auto const t0 = std::chrono::steady_clock::now();
+    for (unsigned k = 0; !ready(); ++k) {
+        if ((k & 1023u) != 1023u) continue;
+        hipError_t const q = hipStreamQuery(s);
+        if (q == hipSuccess) {
+            // the stream has drained: the stores are visible by now or never come
+            if (ready()) break;
+            throw error(HALMD_ERR_LOGIC, "mailbox: the stream drained without the posted words");
+        }
+        if (q != hipErrorNotReady) HIP_CHECK(q);
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(60))
+            throw error(HALMD_ERR_HIP, "mailbox: the posted words did not arrive in 60 s");
+    }
+    for (unsigned k = 0; k < n; ++k) out[k] = uint32_t(w[k]);
+}
+
 max_displacement::max_displacement(particle& p_, box const& b_) : p(p_), b(b_) {
     p.attach_box(b);
     r0.resize(p.n);
@@ -1163,9 +1214,13 @@ void neighbour::update(double displacement_bound) {
         launch_build_neighbours_staged(p.posq, bin->order_offset, sp, params, cap, list16, count, d_maxcount, s,
                                        win ? bin->order_sub_offset.get() : nullptr);
         timer.stop(s);
-        HIP_CHECK(hipMemcpyAsync(h_maxcount.get(), d_maxcount.get(), 8, hipMemcpyDeviceToHost, s));
-        p.sync();
-        unsigned const m = h_maxcount[0], u = h_maxcount[1];
+        // the largest count, the largest union and the merge check of the
+        // binning (0 when it sorted), polled as they are stored
+        mailbox_sources const src{{d_maxcount.get(), d_maxcount.get() + 1, bin->merge_check_word(), nullptr}};
+        uint32_t got[3];
+        build_box.wait(build_box.post(src, 3, s), 3, got, s);
+        bin->merge_checked(got[2]);
+        unsigned const m = got[0], u = got[1];
         if (u > sp.lds_cap) {
             // a union outgrew the staging: enlarge it, else narrow the segments
             if (u + u / 8 <= lds_limit) {

@@ -245,9 +245,34 @@ public:
     dbuf<ens_desc> merge_desc;
     dbuf<uint32_t> merge_work, merge_rank, merge_cnt27, merge_base27, merge_bsum, merge_flag;
     hbuf<uint32_t> h_merge_flag;
-    bool merge_flag_pending = false;   // h_merge_flag holds the last merge's flag once the stream reaches it
+    bool merge_flag_pending = false;   // the last merge's flag is still to be checked
+    // the device word of the pending merge check (null: none pending); the
+    // caller that reads it after the merge hands its value to merge_checked,
+    // which throws when a particle left the 27 cells around its old one
+    uint32_t const* merge_check_word() const;
+    void merge_checked(uint32_t flag);
     void cell_lists(unsigned* begin_out, unsigned* end_out, unsigned* index_out);
     kernel_timer timer;
+};
+
+// Host-mapped words that a one-thread-per-word kernel on a stream fills with
+// small device results, each tagged with a sequence number, for the host to
+// poll: no copy behind the stream and no stream synchronisation.
+class mailbox {
+public:
+    mailbox();
+    ~mailbox();
+    mailbox(mailbox const&) = delete;
+    mailbox& operator=(mailbox const&) = delete;
+    // enqueue on s the store of the n source words; returns their number
+    uint32_t post(mailbox_sources const& src, unsigned n, hipStream_t s);
+    // poll until the n words carry seq, into out (an error when s reports
+    // one, or drains without the store)
+    void wait(uint32_t seq, unsigned n, uint32_t* out, hipStream_t s);
+private:
+    unsigned long long* h_ = nullptr;   // MAILBOX_WORDS words, hipHostMalloc'd (coherent, mapped)
+    unsigned long long* d_ = nullptr;   // the same words, as the device addresses them
+    uint32_t next_ = 1;                 // never 0
 };
 
 class max_displacement {
@@ -334,6 +359,7 @@ public:
     unsigned cap;
     dbuf<uint32_t> list, count, d_maxcount;
     hbuf<uint32_t> h_maxcount;
+    mailbox build_box;   // the staged build's counts (and the binning's merge check)
     // cell-segment lists (this module keeps the particles in cell order)
     bool staged = false;
     seg_params sp{}, sp_force{};      // build staging / force staging (lds_cap = largest union)

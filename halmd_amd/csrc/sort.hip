@@ -381,6 +381,21 @@ void launch_copy(void const* src, void* dst, size_t bytes, hipStream_t s) {
     HIP_CHECK(hipGetLastError());
 }
 
+__global__ void post_words_kernel(mailbox_sources src, unsigned n, unsigned long long* dst, uint32_t seq) {
+    unsigned const k = threadIdx.x;
+    if (k >= n) return;
+    uint32_t const v = src.word[k] ? *src.word[k] : 0u;
+    __hip_atomic_store(dst + k, (static_cast<unsigned long long>(seq) << 32) | v, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+void launch_post_words(mailbox_sources const& src, unsigned n, unsigned long long* dst, uint32_t seq,
+                       hipStream_t s) {
+    if (n > MAILBOX_WORDS) throw error(HALMD_ERR_INVALID_ARGUMENT, "mailbox: too many words");
+    post_words_kernel<<<1, WAVE, 0, s>>>(src, n, dst, seq);
+    HIP_CHECK(hipGetLastError());
+}
+
 void launch_copy_u32(uint32_t const* src, uint32_t* dst, size_t n, hipStream_t s) {
     if (!n) return;
     copy_u32_kernel<<<div_up(n, 256), 256, 0, s>>>(src, dst, n);
