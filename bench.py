@@ -19,6 +19,7 @@ first-out on the same arrivals, both measured), and "reference_benchmarks"
 (the reference's published LJ N=64,000 and Kob-Andersen N=256,000 setups).
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -600,7 +601,9 @@ def main():
 
     extras = {}
     if not args.no_extras:
-        del w
+        # the LJ modules go first (their device memory and streams)
+        del w, thermo, kt
+        gc.collect()
         log("batched ensemble figure")
         extras["batched"] = batched_run(args.systems, args.batched_steps, 100, args.seed)
         log("served stream figure")
