@@ -176,6 +176,16 @@ __global__ void copy_u32_kernel(uint32_t const* __restrict__ a, uint32_t* __rest
     if (i < n) a_out[i] = a[i];
 }
 
+// word k of a mailbox: (seq << 32) | *src.word[k] (0 for a null source), one
+// system-scope store into host-mapped memory
+__global__ void post_words_kernel(mailbox_sources src, unsigned n, unsigned long long* dst, uint32_t seq) {
+    unsigned const k = threadIdx.x;
+    if (k >= n) return;
+    uint32_t const v = src.word[k] ? *src.word[k] : 0u;
+    __hip_atomic_store(dst + k, (static_cast<unsigned long long>(seq) << 32) | v, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __global__ void iota_kernel(uint32_t* out, size_t n) {
     size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) out[i] = uint32_t(i);
@@ -379,14 +389,6 @@ void launch_copy(void const* src, void* dst, size_t bytes, hipStream_t s) {
     if (!n) return;
     copy_kernel<<<div_up(n, 256), 256, 0, s>>>(static_cast<uint4 const*>(src), static_cast<uint4*>(dst), n);
     HIP_CHECK(hipGetLastError());
-}
-
-__global__ void post_words_kernel(mailbox_sources src, unsigned n, unsigned long long* dst, uint32_t seq) {
-    unsigned const k = threadIdx.x;
-    if (k >= n) return;
-    uint32_t const v = src.word[k] ? *src.word[k] : 0u;
-    __hip_atomic_store(dst + k, (static_cast<unsigned long long>(seq) << 32) | v, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 void launch_post_words(mailbox_sources const& src, unsigned n, unsigned long long* dst, uint32_t seq,

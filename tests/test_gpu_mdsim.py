@@ -520,15 +520,19 @@ def test_device_rebuild_leaves_displacement_current():
     assert hits >= 3
 
 
-def test_speculative_force_is_bitwise_identical():
+@pytest.mark.parametrize("n, steps", [(6912, 300), (256000, 150)])
+def test_speculative_force_is_bitwise_identical(n, steps):
     # the force launched before the displacement check comes back (and
     # skipped on the device when the lists are stale) gives the trajectory of
-    # the waiting schedule bit for bit, across many list rebuilds
+    # the waiting schedule bit for bit, across many list rebuilds; the
+    # speculative launches store the displacement maximum into host-mapped
+    # memory (workgroup 0 of about a thousand at N = 256,000), the waiting
+    # schedule copies the words after the stream
     runs = []
     for spec in (True, False):
-        w = workloads.lennard_jones(6912, density=0.8442, temperature=1.5, seed=81, timestep=0.005)
+        w = workloads.lennard_jones(n, density=0.8442, temperature=1.5, seed=81, timestep=0.005)
         w.particle.set_speculative_force(spec)
-        w.sampler.run(300)
+        w.sampler.run(steps)
         runs.append((w.particle.ordered("position"), w.particle.ordered("velocity"),
                      w.thermodynamics.en_pot(), w.neighbour.rebuilds))
     assert runs[0][3] == runs[1][3] and runs[0][3] >= 10
