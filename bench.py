@@ -490,10 +490,10 @@ def main():
                          "tiny: Kob-Andersen N = 4096 with all-pairs neighbour lists; lj64k, ka256k: the "
                          "reference's published benchmark setups")
     ap.add_argument("--rate", type=float, default=50.0, help="serving: job arrivals per second")
-    ap.add_argument("--duration", type=float, default=6.0, help="serving: seconds of arrivals")
-    ap.add_argument("--step-scale", type=float, default=0.1,
-                    help="serving: factor on the job step counts (0.1 keeps the stream above what one GPU "
-                         "serves, as the full-length stream is)")
+    ap.add_argument("--duration", type=float, default=4.0, help="serving: seconds of arrivals")
+    ap.add_argument("--step-scale", type=float, default=0.2,
+                    help="serving: factor on the job step counts (0.2 keeps the offered load, about 5e9 "
+                         "particle-steps/s, above what one GPU serves, as the full-length stream's is)")
     ap.add_argument("--tick", type=int, default=50, help="serving: steps per tick")
     ap.add_argument("--max-jobs", type=int, default=256, help="serving: running-job cap")
     ap.add_argument("--budget-gib", type=int, default=128, help="serving: device memory budget for jobs")
