@@ -91,12 +91,11 @@ __device__ double sum_partials(double const* __restrict__ partials, unsigned nq,
     return t;
 }
 
+// workgroup q sums quantity q (sum_partials: the same order in any grid)
 __global__ __launch_bounds__(REDUCE_THREADS) void reduce_final_kernel(double const* __restrict__ partials, unsigned nq,
                                                                       double* __restrict__ out) {
-    for (unsigned q = 0; q < nq; ++q) {
-        double const t = sum_partials(partials, nq, q);
-        if (threadIdx.x == 0) out[q] = t;
-    }
+    double const t = sum_partials(partials, nq, blockIdx.x);
+    if (threadIdx.x == 0) out[blockIdx.x] = t;
 }
 
 // Nose-Hoover chain of length 2, half step (Martyna et al., J. Chem. Phys. 97,
@@ -139,7 +138,8 @@ void launch_thermo_partials(float4 const* vel, float4 const* force, float const*
 }
 
 void launch_reduce_final(double const* partials, unsigned nq, double* out, hipStream_t s) {
-    reduce_final_kernel<<<1, REDUCE_THREADS, 0, s>>>(partials, nq, out);
+    if (!nq) return;
+    reduce_final_kernel<<<nq, REDUCE_THREADS, 0, s>>>(partials, nq, out);
     HIP_CHECK(hipGetLastError());
 }
 
