@@ -608,6 +608,33 @@ void launch_density_mode(float4 const* pos, unsigned n, double const* q, unsigne
 // unwrapped positions r + image L in ID order, FP64
 void launch_sample_position(float4 const* pos, int4 const* image, uint32_t const* id, unsigned n, double const* L,
                             double* out, hipStream_t s);
+// The same two samples of many systems in one launch each.  Task t covers the
+// virtual blocks [vb0, vb0 + its block count) of its launch (ascending vb0,
+// the first at 0) and the density modes' final waves [vk0, vk0 + nq); each
+// virtual block computes exactly what the block of the one-system launch with
+// those indices computes, so the results are the same bit for bit.
+struct position_task {
+    float4 const* pos;
+    int4 const* image;
+    uint32_t const* id;
+    double* out;
+    double L[3];
+    unsigned n, vb0;   // blocks: ceil(n / 256)
+};
+struct density_task {
+    float4 const* pos;
+    double const* q;
+    double* partials;   // launch_density_mode's partials (its nblk blocks)
+    double* out;        // 2 nq
+    unsigned n, nq;
+    unsigned nblk;      // particle blocks, as launch_density_mode chooses them for n
+    unsigned vb0, vk0;  // blocks: nblk * ceil(nq / DM_KCHUNK); final waves: nq
+};
+unsigned density_mode_blocks(unsigned n);   // nblk of launch_density_mode
+// tasks: device tables; vblocks / vwaves: the totals of the counts above
+void launch_sample_tasks(position_task const* pos_tasks, unsigned npos, unsigned pos_vblocks,
+                         density_task const* dm_tasks, unsigned ndm, unsigned dm_vblocks, unsigned dm_vwaves,
+                         hipStream_t s);
 // per-block partials of sum |r2 - r1|^2 and sum |r2 - r1|^4 over n particles,
 // written to partials[block * nq + qoff + {0, 1}]
 void launch_msd_partials(double const* r1, double const* r2, unsigned n, double* partials, unsigned nq, unsigned qoff,

@@ -20,12 +20,13 @@ namespace {
 
 constexpr unsigned DM_THREADS = 256;
 
-__global__ __launch_bounds__(DM_THREADS) void density_mode_kernel(float4 const* __restrict__ pos, unsigned n,
-                                                                  double const* __restrict__ q, unsigned nq,
-                                                                  double* __restrict__ partials) {
+// block (blk, chunk) of a grid of nblk particle blocks by wavevector chunks
+__device__ __forceinline__ void density_mode_block(float4 const* __restrict__ pos, unsigned n,
+                                                   double const* __restrict__ q, unsigned nq,
+                                                   double* __restrict__ partials, unsigned blk, unsigned chunk,
+                                                   unsigned nblk) {
     __shared__ double s_q[DM_KCHUNK][3];
     __shared__ double s_red[DM_THREADS / WAVE][2 * DM_KCHUNK];
-    unsigned const chunk = blockIdx.y;
     if (threadIdx.x < 3 * DM_KCHUNK) {
         unsigned const kk = threadIdx.x / 3, c = threadIdx.x % 3;
         unsigned const k = chunk * DM_KCHUNK + kk;
@@ -38,7 +39,7 @@ __global__ __launch_bounds__(DM_THREADS) void density_mode_kernel(float4 const* 
 #pragma unroll
     for (unsigned kk = 0; kk < DM_KCHUNK; ++kk) { re[kk] = 0; im[kk] = 0; }
 
-    for (unsigned i = blockIdx.x * DM_THREADS + threadIdx.x; i < n; i += gridDim.x * DM_THREADS) {
+    for (unsigned i = blk * DM_THREADS + threadIdx.x; i < n; i += nblk * DM_THREADS) {
         float4 const hi = pos[2 * size_t(i)], lo = pos[2 * size_t(i) + 1];
         double const x = double(hi.x) + double(lo.x);
         double const y = double(hi.y) + double(lo.y);
@@ -68,16 +69,21 @@ __global__ __launch_bounds__(DM_THREADS) void density_mode_kernel(float4 const* 
     if (threadIdx.x < 2 * DM_KCHUNK) {
         double t = 0;
         for (unsigned w = 0; w < DM_THREADS / WAVE; ++w) t += s_red[w][threadIdx.x];
-        partials[(size_t(chunk) * gridDim.x + blockIdx.x) * 2 * DM_KCHUNK + threadIdx.x] = t;
+        partials[(size_t(chunk) * nblk + blk) * 2 * DM_KCHUNK + threadIdx.x] = t;
     }
+}
+
+__global__ __launch_bounds__(DM_THREADS) void density_mode_kernel(float4 const* __restrict__ pos, unsigned n,
+                                                                  double const* __restrict__ q, unsigned nq,
+                                                                  double* __restrict__ partials) {
+    density_mode_block(pos, n, q, nq, partials, blockIdx.x, blockIdx.y, gridDim.x);
 }
 
 // out[2k + c] = sum over blocks of partials for wavevector k: one wave per
 // wavevector, lane l summing blocks l, l + 64, ..., then a butterfly (a fixed
 // order)
-__global__ __launch_bounds__(WAVE) void density_mode_final_kernel(double const* __restrict__ partials,
-                                                                  unsigned nblocks, double* __restrict__ out) {
-    unsigned const k = blockIdx.x;
+__device__ __forceinline__ void density_mode_final_wave(double const* __restrict__ partials, unsigned nblocks,
+                                                        double* __restrict__ out, unsigned k) {
     unsigned const chunk = k / DM_KCHUNK, kk = k % DM_KCHUNK;
     double a = 0, b = 0;
     for (unsigned blk = threadIdx.x; blk < nblocks; blk += WAVE) {
@@ -95,6 +101,11 @@ __global__ __launch_bounds__(WAVE) void density_mode_final_kernel(double const* 
     }
 }
 
+__global__ __launch_bounds__(WAVE) void density_mode_final_kernel(double const* __restrict__ partials,
+                                                                  unsigned nblocks, double* __restrict__ out) {
+    density_mode_final_wave(partials, nblocks, out, blockIdx.x);
+}
+
 // velocity sample in particle-ID order (hi + lo in FP64)
 __global__ void sample_velocity_kernel(float4 const* __restrict__ vel, uint32_t const* __restrict__ id, unsigned n,
                                        double* __restrict__ out) {
@@ -107,10 +118,9 @@ __global__ void sample_velocity_kernel(float4 const* __restrict__ vel, uint32_t 
     out[o + 2] = double(hi.z) + double(lo.z);
 }
 
-__global__ void sample_position_kernel(float4 const* __restrict__ pos, int4 const* __restrict__ image,
-                                       uint32_t const* __restrict__ id, unsigned n, double Lx, double Ly, double Lz,
-                                       double* __restrict__ out) {
-    unsigned const i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void sample_position_at(float4 const* __restrict__ pos, int4 const* __restrict__ image,
+                                                   uint32_t const* __restrict__ id, unsigned n, double Lx, double Ly,
+                                                   double Lz, double* __restrict__ out, unsigned i) {
     if (i >= n) return;
     float4 const hi = pos[2 * size_t(i)], lo = pos[2 * size_t(i) + 1];
     int4 const im = image[i];
@@ -118,6 +128,49 @@ __global__ void sample_position_kernel(float4 const* __restrict__ pos, int4 cons
     out[o] = double(hi.x) + double(lo.x) + double(im.x) * Lx;
     out[o + 1] = double(hi.y) + double(lo.y) + double(im.y) * Ly;
     out[o + 2] = double(hi.z) + double(lo.z) + double(im.z) * Lz;
+}
+
+__global__ void sample_position_kernel(float4 const* __restrict__ pos, int4 const* __restrict__ image,
+                                       uint32_t const* __restrict__ id, unsigned n, double Lx, double Ly, double Lz,
+                                       double* __restrict__ out) {
+    sample_position_at(pos, image, id, n, Lx, Ly, Lz, out, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// the task whose range [first, next task's first) holds v (firsts ascending,
+// the first 0); uniform within a workgroup
+template <typename T, typename F>
+__device__ __forceinline__ unsigned task_of(T const* __restrict__ tasks, unsigned ntask, unsigned v, F first) {
+    unsigned lo = 0, hi = ntask;   // first(tasks[lo]) <= v
+    while (hi - lo > 1) {
+        unsigned const mid = (lo + hi) / 2;
+        if (first(tasks[mid]) <= v) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// virtual block v of the position samples of every task
+__global__ __launch_bounds__(256) void sample_position_tasks_kernel(position_task const* __restrict__ tasks,
+                                                                    unsigned ntask) {
+    unsigned const v = blockIdx.x;
+    position_task const& t = tasks[task_of(tasks, ntask, v, [](position_task const& x) { return x.vb0; })];
+    sample_position_at(t.pos, t.image, t.id, t.n, t.L[0], t.L[1], t.L[2], t.out, (v - t.vb0) * 256 + threadIdx.x);
+}
+
+// virtual block v of the density modes of every task: block (blk, chunk) of
+// the task's nblk x ceil(nq / DM_KCHUNK) grid, blk fastest
+__global__ __launch_bounds__(DM_THREADS) void density_mode_tasks_kernel(density_task const* __restrict__ tasks,
+                                                                        unsigned ntask) {
+    unsigned const v = blockIdx.x;
+    density_task const& t = tasks[task_of(tasks, ntask, v, [](density_task const& x) { return x.vb0; })];
+    unsigned const local = v - t.vb0;
+    density_mode_block(t.pos, t.n, t.q, t.nq, t.partials, local % t.nblk, local / t.nblk, t.nblk);
+}
+
+__global__ __launch_bounds__(WAVE) void density_mode_final_tasks_kernel(density_task const* __restrict__ tasks,
+                                                                        unsigned ntask) {
+    unsigned const v = blockIdx.x;
+    density_task const& t = tasks[task_of(tasks, ntask, v, [](density_task const& x) { return x.vk0; })];
+    density_mode_final_wave(t.partials, t.nblk, t.out, v - t.vk0);
 }
 
 // block row y: the per-particle value x of the pair (r1, r2.p[y]) summed as
@@ -163,11 +216,15 @@ __global__ __launch_bounds__(REDUCE_THREADS) void tcf_partials_kernel(double con
 
 } // namespace
 
+unsigned density_mode_blocks(unsigned n) {
+    // no more particle blocks than the particles fill (at most DM_BLOCKS)
+    return std::max(1u, std::min(DM_BLOCKS, div_up(n, DM_THREADS)));
+}
+
 void launch_density_mode(float4 const* pos, unsigned n, double const* q, unsigned nq, double* partials, double* out,
                          hipStream_t s) {
     if (!nq) return;
-    // no more particle blocks than the particles fill (at most DM_BLOCKS)
-    unsigned const nblk = std::max(1u, std::min(DM_BLOCKS, div_up(n, DM_THREADS)));
+    unsigned const nblk = density_mode_blocks(n);
     dim3 grid(nblk, div_up(nq, DM_KCHUNK));
     density_mode_kernel<<<grid, DM_THREADS, 0, s>>>(pos, n, q, nq, partials);
     density_mode_final_kernel<<<nq, WAVE, 0, s>>>(partials, nblk, out);
@@ -179,6 +236,21 @@ void launch_sample_position(float4 const* pos, int4 const* image, uint32_t const
     if (!n) return;
     sample_position_kernel<<<div_up(n, 256), 256, 0, s>>>(pos, image, id, n, L[0], L[1], L[2], out);
     HIP_CHECK(hipGetLastError());
+}
+
+void launch_sample_tasks(position_task const* pos_tasks, unsigned npos, unsigned pos_vblocks,
+                         density_task const* dm_tasks, unsigned ndm, unsigned dm_vblocks, unsigned dm_vwaves,
+                         hipStream_t s) {
+    if (npos && pos_vblocks) {
+        sample_position_tasks_kernel<<<pos_vblocks, 256, 0, s>>>(pos_tasks, npos);
+        HIP_CHECK(hipGetLastError());
+    }
+    if (ndm && dm_vblocks) {
+        density_mode_tasks_kernel<<<dm_vblocks, DM_THREADS, 0, s>>>(dm_tasks, ndm);
+        HIP_CHECK(hipGetLastError());
+        density_mode_final_tasks_kernel<<<dm_vwaves, WAVE, 0, s>>>(dm_tasks, ndm);
+        HIP_CHECK(hipGetLastError());
+    }
 }
 
 void launch_msd_partials(double const* r1, double const* r2, unsigned n, double* partials, unsigned nq, unsigned qoff,

@@ -40,7 +40,6 @@ job_correlator::job_correlator(unsigned n, double length, double dt, uint64_t re
         q_ = dcache_.take(std::max(3u * nq_, 3u));
         q_->upload(wv->q.data(), wv->q.size(), s_);
         dm_partials_ = dcache_.take(size_t(div_up(nq_, DM_KCHUNK)) * DM_BLOCKS * 2 * DM_KCHUNK);
-        dm_out_ = dcache_.take(2 * size_t(nq_));
         isf.resize(interval.size() * block_size_ * wv->shell.size());
         HIP_CHECK(hipStreamSynchronize(s_));
     }
@@ -61,7 +60,7 @@ job_correlator::~job_correlator() {
     for (auto& b : rho_pool_) hcache_.give(std::move(b));
     pos_pool_.clear();
     rho_pool_.clear();
-    for (auto* b : {&q_, &dm_partials_, &dm_out_, &msd_partials_, &msd_out_}) dcache_.give(std::move(*b));
+    for (auto* b : {&q_, &dm_partials_, &msd_partials_, &msd_out_}) dcache_.give(std::move(*b));
     hcache_.give(std::move(h_msd_));
 }
 
@@ -87,7 +86,7 @@ job_correlator::rho_ptr job_correlator::take_rho() {
     return rho_ptr(b, [this](hbuf<double>* p) { rho_pool_.emplace_back(p); });
 }
 
-void job_correlator::sample(uint64_t step, ensemble::view const& v) {
+void job_correlator::sample(uint64_t step, ensemble::view const& v, sample_batch& batch) {
     require(v.n == n_, "correlator: particle count changed");
     pos_ptr ps;
     rho_ptr rs;
@@ -95,22 +94,121 @@ void job_correlator::sample(uint64_t step, ensemble::view const& v) {
         if (step < origin[i] || (step - origin[i]) % interval[i] != 0) continue;
         if (!ps) {
             ps = take_pos();
-            launch_sample_position(v.pos, v.image, v.id, n_, L_, *ps, s_);
+            batch.add_position(v, L_, ps->get());
         }
         pos_blocks_[i].push_back(ps);
         if (pos_blocks_[i].size() > block_size_) pos_blocks_[i].erase(pos_blocks_[i].begin());
         if (nq_) {
             if (!rs) {
                 rs = take_rho();
-                launch_density_mode(v.pos, n_, *q_, nq_, *dm_partials_, *dm_out_, s_);
-                HIP_CHECK(hipMemcpyAsync(rs->get(), dm_out_->get(), 2 * nq_ * sizeof(double), hipMemcpyDeviceToHost,
-                                         s_));
+                batch.add_density(v.pos, n_, *q_, nq_, *dm_partials_, rs->get());
             }
             rho_blocks_[i].push_back(rs);
             if (rho_blocks_[i].size() > block_size_) rho_blocks_[i].erase(rho_blocks_[i].begin());
         }
-        if (pos_blocks_[i].size() == block_size_) process(i);
+        if (pos_blocks_[i].size() == block_size_) due_.push_back(i);
     }
+}
+
+void job_correlator::process_due() {
+    for (unsigned i : due_) process(i);
+    due_.clear();
+}
+
+// ---------------------------------------------------------------------------
+// sample_batch
+// ---------------------------------------------------------------------------
+void sample_batch::add_position(ensemble::view const& v, double const L[3], double* out) {
+    position_task t{};
+    t.pos = v.pos;
+    t.image = v.image;
+    t.id = v.id;
+    t.out = out;
+    for (int k = 0; k < 3; ++k) t.L[k] = L[k];
+    t.n = v.n;
+    pos_.push_back(t);
+}
+
+void sample_batch::add_density(float4 const* pos, unsigned n, double const* q, unsigned nq, double* partials,
+                               double* host_out) {
+    if (!nq) return;
+    density_task t{};
+    t.pos = pos;
+    t.q = q;
+    t.partials = partials;
+    t.n = n;
+    t.nq = nq;
+    t.nblk = density_mode_blocks(n);
+    dm_.push_back(t);
+    dm_host_.push_back(host_out);
+}
+
+void sample_batch::launch(hipStream_t s) {
+    require(!launched_, "sample batch: launched twice before finish");
+    if (pos_.empty() && dm_.empty()) return;
+    // the virtual-block ranges and the density modes' places in the staging
+    // buffers
+    unsigned pos_vb = 0, dm_vb = 0, dm_vk = 0;
+    for (auto& t : pos_) {
+        t.vb0 = pos_vb;
+        pos_vb += div_up(t.n, 256);
+    }
+    out_words_ = 0;
+    dm_off_.clear();
+    for (auto const& t : dm_) {
+        dm_off_.push_back(out_words_);
+        out_words_ += 2 * size_t(t.nq);
+    }
+    if (d_out_.size() < out_words_) {
+        d_out_.resize(out_words_);
+        h_out_.resize(out_words_);
+    }
+    for (size_t k = 0; k < dm_.size(); ++k) {
+        auto& t = dm_[k];
+        t.out = d_out_.get() + dm_off_[k];
+        t.vb0 = dm_vb;
+        t.vk0 = dm_vk;
+        dm_vb += t.nblk * div_up(t.nq, DM_KCHUNK);
+        dm_vk += t.nq;
+    }
+    // tables through page-locked memory (the previous batch's copies are done:
+    // finish() ran after a synchronisation)
+    if (h_pos_.size() < pos_.size()) {
+        h_pos_.resize(pos_.size());
+        d_pos_.resize(pos_.size());
+    }
+    if (h_dm_.size() < dm_.size()) {
+        h_dm_.resize(dm_.size());
+        d_dm_.resize(dm_.size());
+    }
+    if (!pos_.empty()) {
+        std::memcpy(h_pos_.get(), pos_.data(), pos_.size() * sizeof(position_task));
+        d_pos_.upload(h_pos_.get(), pos_.size(), s);
+    }
+    if (!dm_.empty()) {
+        std::memcpy(h_dm_.get(), dm_.data(), dm_.size() * sizeof(density_task));
+        d_dm_.upload(h_dm_.get(), dm_.size(), s);
+    }
+    launch_sample_tasks(d_pos_.get(), unsigned(pos_.size()), pos_vb, d_dm_.get(), unsigned(dm_.size()), dm_vb, dm_vk,
+                        s);
+    if (out_words_) d_out_.download(h_out_.get(), out_words_, s);
+    launched_ = true;
+}
+
+void sample_batch::finish() {
+    if (launched_)
+        for (size_t k = 0; k < dm_.size(); ++k)
+            std::memcpy(dm_host_[k], h_out_.get() + dm_off_[k], 2 * size_t(dm_[k].nq) * sizeof(double));
+    clear();
+}
+
+void sample_batch::clear() {
+    pos_.clear();
+    dm_.clear();
+    dm_host_.clear();
+    dm_off_.clear();
+    out_words_ = 0;
+    launched_ = false;
 }
 
 void job_correlator::process(unsigned level) {
@@ -315,6 +413,7 @@ ensemble::view server::view_of(job const& j) const {
 }
 
 unsigned server::tick() {
+    batch_.clear();   // (requests of a tick that an error ended)
     double t_mark = now();
     auto phase = [&](int k) {
         HIP_CHECK(hipStreamSynchronize(ens.stream));
@@ -362,18 +461,26 @@ unsigned server::tick() {
         fresh.push_back(&j);
     }
     ens.apply();
-    for (job* j : fresh) j->corr->sample(0, view_of(*j));
+    // every job's samples of one point together (sample_batch), then the
+    // levels they filled
+    for (job* j : fresh) j->corr->sample(0, view_of(*j), batch_);
+    batch_.launch(ens.stream);
+    for (job* j : fresh) j->corr->process_due();
     phase(0);
+    batch_.finish();
     if (!active_.empty()) {
         ens.step(tick_);
         phase(1);
         for (job* j : active_) {
             j->step += tick_;
             psteps_ += uint64_t(tick_) * j->req.n;
-            j->corr->sample(j->step, view_of(*j));
+            j->corr->sample(j->step, view_of(*j), batch_);
         }
+        batch_.launch(ens.stream);
+        for (job* j : active_) j->corr->process_due();
     }
     HIP_CHECK(hipStreamSynchronize(ens.stream));
+    batch_.finish();
     for (job* j : active_) j->corr->collect();
     phase(2);
 

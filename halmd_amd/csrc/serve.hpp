@@ -58,6 +58,38 @@ using host_cache = buffer_cache<hbuf<double>>;
 // per wavevector) is copied once to a pinned host buffer.  Processing a level
 // enqueues one MSD reduction over all its lags; collect() folds the results
 // into FP64 accumulators after a stream synchronisation.
+// The position samples and density modes that the jobs' correlators take at
+// one point of a tick, launched together (launch_sample_tasks): a table
+// upload, two or three launches and one copy back, instead of a few launches
+// and a copy per job.  Same results as the one-system launches, bit for bit.
+class sample_batch {
+public:
+    // requests; nothing runs until launch()
+    void add_position(ensemble::view const& v, double const L[3], double* out);
+    void add_density(float4 const* pos, unsigned n, double const* q, unsigned nq, double* partials, double* host_out);
+    // enqueue the whole batch on s
+    void launch(hipStream_t s);
+    // once s has been synchronised after launch(): the density modes into
+    // their host buffers; the batch is empty again
+    void finish();
+    // drop requests that were never launched
+    void clear();
+
+private:
+    std::vector<position_task> pos_;
+    std::vector<density_task> dm_;
+    std::vector<double*> dm_host_;   // host destination of each density task
+    std::vector<size_t> dm_off_;     // its words in the staging buffers
+    hbuf<position_task> h_pos_;
+    hbuf<density_task> h_dm_;
+    dbuf<position_task> d_pos_;
+    dbuf<density_task> d_dm_;
+    dbuf<double> d_out_;
+    hbuf<double> h_out_;
+    size_t out_words_ = 0;
+    bool launched_ = false;
+};
+
 class job_correlator {
 public:
     // the parameter checks of the constructor (blocking_scheme.cpp:52-99)
@@ -68,8 +100,11 @@ public:
                    unsigned block_size, unsigned shift, unsigned separation, std::unique_ptr<wavevector> shells,
                    hipStream_t stream, device_cache& dcache, host_cache& hcache);
     ~job_correlator();
-    // enqueue the sample of `step` (levels due at it) from the system's arrays
-    void sample(uint64_t step, ensemble::view const& v);
+    // the sample of `step` (levels due at it) from the system's arrays, as
+    // requests in batch; levels whose block fills are processed by
+    // process_due(), which the caller calls once the batch is launched
+    void sample(uint64_t step, ensemble::view const& v, sample_batch& batch);
+    void process_due();
     // enqueue the processing of every level that still holds samples; false
     // when none does
     bool flush();
@@ -94,7 +129,8 @@ private:
     host_cache& hcache_;
     std::vector<uint64_t> origin;
     std::unique_ptr<wavevector> wv;
-    std::unique_ptr<dbuf<double>> q_, dm_partials_, dm_out_, msd_partials_, msd_out_;
+    std::unique_ptr<dbuf<double>> q_, dm_partials_, msd_partials_, msd_out_;
+    std::vector<unsigned> due_;   // levels whose block filled at the last sample
     std::unique_ptr<hbuf<double>> h_msd_;
     // pools of sample buffers: a buffer returns to its pool when nothing holds
     // it; a later sample may reuse a device buffer on the same stream, and a
@@ -170,6 +206,7 @@ public:
 private:
     device_cache dcache_;   // (declared before the jobs: their correlators return buffers here)
     host_cache hcache_;
+    sample_batch batch_;    // the correlators' samples of one point of a tick
     size_t budget_, used_ = 0;
     unsigned max_jobs_, tick_;
     std::chrono::steady_clock::time_point t0_;
