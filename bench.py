@@ -161,6 +161,13 @@ def cpu_baseline(w, steps):
             "cpu_model": cpu_model()}
 
 
+def skin_text(skin):
+    """'skin 0.3', or 'skins lj 0.3 / morse 0.3 / wca 0.5' for a skin per potential."""
+    if isinstance(skin, dict):
+        return "skins " + " / ".join(f"{k} {v}" for k, v in sorted(skin.items()))
+    return f"skin {skin}"
+
+
 def batched_run(systems, steps, warmup, seed, plain_steps=1000, pinned=None):
     """Batched ensembles: `systems` seeded systems (N log-uniform in [500,
     32000], rho in [0.7, 1.2], T in [0.5, 2.0], LJ / WCA / Morse), NVE dt =
@@ -179,8 +186,8 @@ def batched_run(systems, steps, warmup, seed, plain_steps=1000, pinned=None):
     else:
         best, table = workloads.tune_ensemble(specs, timestep=0.003)
         skin, segments = best["skin"], best["segments"]
-        log(f"ensemble autotuner ({len(table)} candidates, {time.perf_counter() - t0:.1f} s): skin {skin}, "
-            f"{segments[0]} particles per segment, {segments[1]} threads per workgroup "
+        log(f"ensemble autotuner ({len(table)} candidates, {time.perf_counter() - t0:.1f} s): "
+            f"{skin_text(skin)}, {segments[0]} particles per segment, {segments[1]} threads per workgroup "
             f"({best['ms_per_step']:.3f} ms/step over 128 steps)")
     t0 = time.perf_counter()
     e = workloads.ensemble_from_specs(specs, timestep=0.003, skin=skin, segments=segments)
@@ -205,7 +212,7 @@ def batched_run(systems, steps, warmup, seed, plain_steps=1000, pinned=None):
     drift = np.abs((pot1 + kin1) - (pot0 + kin0)) / (steps * 0.003)
     return {
         "workload": f"{len(specs)} independent NVE systems, N log-uniform in [500, 32000] (total {ntot}), "
-                    f"rho U[0.7, 1.2], T U[0.5, 2.0], LJ rc 2.5 / WCA / Morse rc 2.5, dt 0.003, skin {skin} "
+                    f"rho U[0.7, 1.2], T U[0.5, 2.0], LJ rc 2.5 / WCA / Morse rc 2.5, dt 0.003, {skin_text(skin)} "
                     + ("(pinned)" if pinned else "(autotuned)"),
         "autotuned": {"skin": skin, "segment_particles": segments[0], "workgroup": segments[1],
                       "candidates": [[t["skin"], t["segments"][0], t["segments"][1],

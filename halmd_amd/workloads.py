@@ -128,16 +128,24 @@ def spec_species(spec):
     return ka_species(spec["n"], spec["seed"]) if spec["potential"] == "ka" else None
 
 
+def spec_skin(skin, spec):
+    """The skin of `spec`'s system: `skin` itself, or skin[potential name] for a
+    mapping of potential names to skins."""
+    return skin[spec["potential"]] if isinstance(skin, dict) else skin
+
+
 def single_from_spec(spec, timestep=0.003, skin=0.3):
-    """The system of `spec` alone, through the per-system modules."""
+    """The system of `spec` alone, through the per-system modules (skin: a
+    number, or a mapping of potential names to skins)."""
     pot = spec_potential(spec["potential"], spec.get("truncation", "shifted"), spec.get("smoothing", 0.005))
-    return fluid(spec["n"], spec["density"], spec["temperature"], pot, skin=skin, timestep=timestep,
-                 seed=spec["seed"], species=spec_species(spec))
+    return fluid(spec["n"], spec["density"], spec["temperature"], pot, skin=spec_skin(skin, spec),
+                 timestep=timestep, seed=spec["seed"], species=spec_species(spec))
 
 
 def ensemble_from_specs(specs, timestep=0.003, skin=0.3, segments=None):
     """All systems of `specs` in one committed ensemble (inputs as fluid() makes
-    them); segments: (target particles per segment, workgroup size)."""
+    them); skin: a number, or a mapping of potential names to skins;
+    segments: (target particles per segment, workgroup size)."""
     from .ensemble import ensemble
     e = ensemble()
     if segments is not None:
@@ -145,32 +153,53 @@ def ensemble_from_specs(specs, timestep=0.003, skin=0.3, segments=None):
     for sp in specs:
         L, r0, v0, _ = system_inputs(sp["n"], sp["density"], sp["temperature"], sp["seed"])
         e.add(r0, v0, L[0], potential=sp["potential"], truncation=sp.get("truncation", "shifted"),
-              smoothing=sp.get("smoothing", 0.005), timestep=timestep, skin=skin, species=spec_species(sp))
+              smoothing=sp.get("smoothing", 0.005), timestep=timestep, skin=spec_skin(skin, sp),
+              species=spec_species(sp))
     e.commit()
     return e
 
 
 def tune_ensemble(specs, timestep=0.003, targets=(150, 200, 260), threads=(128, 256), skins=(0.25, 0.3, 0.35),
-                  warmup=32, steps=128):
+                  class_skins=(0.2, 0.3, 0.5, 0.7), warmup=32, steps=128, margin=0.02):
     """Autotuner of the batched path: cell size (through the skin: cells are
     rc + skin wide), segment shape (particles per segment) and workgroup size,
     each candidate timed over `steps` device-decided steps of the systems of
-    `specs` from their start.  Returns (best, table): best = dict(skin,
-    segments=(target, threads), ms_per_step); table = every candidate's
-    ms per step (None where a staging does not fit)."""
+    `specs` from their start.  After the grid over (skin, target, threads),
+    each potential's systems in turn try the skins of `class_skins` with the
+    others fixed (a potential with a short cutoff rebuilds less often with a
+    wider skin at little pair cost); a change is kept when the whole ensemble
+    is faster by `margin`.  Returns (best, table): best = dict(skin (a mapping
+    of potential names to skins when class_skins is not empty), segments =
+    (target, threads), ms_per_step); table = every candidate's ms per step
+    (None where a staging does not fit)."""
     from ._lib import HalmdError
+
+    def trial(skin, segments):
+        try:
+            e = ensemble_from_specs(specs, timestep=timestep, skin=skin, segments=segments)
+            e.step(warmup)
+            return e.step_timed(steps) / steps
+        except HalmdError:
+            return None   # e.g. a cell neighbourhood beyond the staging limit
+
     table = []
     for skin in skins:
         for tg in targets:
             for th in threads:
-                try:
-                    e = ensemble_from_specs(specs, timestep=timestep, skin=skin, segments=(tg, th))
-                    e.step(warmup)
-                    ms = e.step_timed(steps) / steps
-                except HalmdError:
-                    ms = None   # e.g. a cell neighbourhood beyond the staging limit
-                table.append(dict(skin=skin, segments=(tg, th), ms_per_step=ms))
-    best = min((t for t in table if t["ms_per_step"] is not None), key=lambda t: t["ms_per_step"])
+                table.append(dict(skin=skin, segments=(tg, th), ms_per_step=trial(skin, (tg, th))))
+    best = dict(min((t for t in table if t["ms_per_step"] is not None), key=lambda t: t["ms_per_step"]))
+    if class_skins:
+        names = sorted({sp["potential"] for sp in specs})
+        best["skin"] = {name: best["skin"] for name in names}
+        for name in names:
+            for alt in class_skins:
+                if alt == best["skin"][name]:
+                    continue
+                skin = dict(best["skin"], **{name: alt})
+                ms = trial(skin, best["segments"])
+                table.append(dict(skin=skin, segments=best["segments"], ms_per_step=ms))
+                if ms is not None and ms < best["ms_per_step"] * (1 - margin):
+                    best = dict(skin=skin, segments=best["segments"], ms_per_step=ms)
     return best, table
 
 

@@ -261,6 +261,21 @@ def test_segment_shapes_change_only_the_rounding():
 
 def test_ensemble_autotuner_returns_a_measured_choice():
     best, table = workloads.tune_ensemble(SPECS[:3], timestep=DT, targets=(150, 200), threads=(256,),
-                                          skins=(0.3,), warmup=4, steps=16)
+                                          skins=(0.3,), class_skins=(), warmup=4, steps=16)
     assert len(table) == 2 and all(t["ms_per_step"] > 0 for t in table)
     assert best["ms_per_step"] == min(t["ms_per_step"] for t in table)
+
+
+def test_ensemble_autotuner_skins_per_potential():
+    # the per-potential pass: every trial is measured, the choice is one of
+    # them, and its skins are those of the potentials present
+    specs = SPECS[:3]
+    best, table = workloads.tune_ensemble(specs, timestep=DT, targets=(200,), threads=(256,), skins=(0.3,),
+                                          class_skins=(0.3, 0.5), warmup=4, steps=16)
+    names = {sp["potential"] for sp in specs}
+    assert isinstance(best["skin"], dict) and set(best["skin"]) == names
+    assert all(t["ms_per_step"] > 0 for t in table)
+    assert len(table) == 1 + len(names)   # the grid's one point, then one alternative per potential
+    # kept: the grid's skin for every potential, or a measured alternative
+    assert best["skin"] == {name: 0.3 for name in names} or \
+        any(t["skin"] == best["skin"] and t["ms_per_step"] == best["ms_per_step"] for t in table[1:])
