@@ -669,7 +669,7 @@ def main():
             "counter_shape": prof.get("segments"),
         },
         "neighbour": {
-            # the list build, per SURVEY-fixed algorithmic bytes: r 24 B, cell
+            # the list build against its fixed algorithmic bytes: r 24 B, cell
             # slots ~13 B, list words (n_nb + 1) 4 B per particle
             "avg_ms": nb_avg,
             "bytes_per_launch": nb_bytes,

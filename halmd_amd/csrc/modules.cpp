@@ -932,7 +932,9 @@ void max_displacement::begin_readback(displacement_check& chk, uint32_t keep_bit
 double max_displacement::finish_readback() {
     // the launch that stores the word is on the particle stream; it runs (or
     // the stream reports an error) in finite time, so the poll is bounded
+    require(pub_wait_ != 0, "max_displacement: finish_readback without begin_readback");
     uint32_t const seq = pub_wait_;
+    pub_wait_ = 0;
     volatile unsigned long long const* w = h_pub_ + seq % PUBLISH_RING;
     auto const t0 = std::chrono::steady_clock::now();
     for (unsigned k = 0;; ++k) {
