@@ -1048,42 +1048,50 @@ __device__ __forceinline__ void ascending_steps(unsigned a, unsigned nc, int o[3
 
 __device__ __forceinline__ unsigned step_code(int ex, int ey, int ez) { return unsigned((ez + 1) * 9 + (ey + 1) * 3 + (ex + 1)); }
 
-// waves over runs of consecutive old cells of the flagged cell systems
-// (workgroups of 256 threads: s_cnt holds one row per wave)
+// waves over runs of consecutive old cells of the flagged cell systems, a
+// cell per half-wave (32 lanes: most cells hold fewer particles; a cell with
+// more takes more chunks); workgroups of 256 threads: s_cnt holds one row per
+// half-wave
 __global__ __launch_bounds__(256) void ens_rank_kernel(uint4 const* __restrict__ posq,
                                                        uint32_t const* __restrict__ offsets,
                                                        ens_desc const* __restrict__ descs, uint32_t* skip,
                                                        ens_work w, uint32_t* __restrict__ keys,
                                                        uint32_t* __restrict__ vals, uint32_t* __restrict__ cnt27) {
-    // per wave: particles so far of each of the 27 codes in the current cell
-    __shared__ uint32_t s_cnt[256 / WAVE][32];
+    // per half-wave: particles so far of each of the 27 codes in its cell
+    __shared__ uint32_t s_cnt[256 / WAVE][2][32];
     unsigned const nl = w.head[0];
     uint32_t const total = w.lcell[nl];
-    unsigned const lane = threadIdx.x % WAVE;
-    uint32_t* const wc = s_cnt[threadIdx.x / WAVE];
+    unsigned const lane = threadIdx.x % WAVE, half = lane / 32, hl = lane % 32;
+    uint32_t* const wc = s_cnt[threadIdx.x / WAVE][half];
     unsigned const waves = gridDim.x * (blockDim.x / WAVE);
     unsigned const wave = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
     uint64_t const below = (uint64_t(1) << lane) - 1u;
-    uint32_t const run = (total + waves - 1) / waves;
+    uint64_t const mine = half ? ~uint64_t(0) << 32 : ~uint64_t(0) >> 32;
+    // an even number of cells per wave: a pair at a time
+    uint32_t const run = ((total + waves - 1) / waves + 1u) & ~1u;
     uint32_t const g0 = min(total, wave * run), g1 = min(total, g0 + run);
     if (g0 >= g1) return;   // the whole wave
-    unsigned k = work_find(w.lcell, nl, g0);
-    for (uint32_t g = g0; g < g1; ++g) {   // wave-uniform
-        while (g >= w.lcell[k + 1]) ++k;
+    unsigned k = work_find(w.lcell, nl, g0);   // (per lane from here: the halves may be in two systems)
+    for (uint32_t gp = g0; gp < g1; gp += 2) {   // wave-uniform
+        uint32_t const g = gp + half;
+        bool const has = g < g1;
+        if (has)
+            while (g >= w.lcell[k + 1]) ++k;
         unsigned const s = w.lsys[k];
         ens_desc const& d = descs[s];
         cell_grid const& cg = d.sp.grid;
-        unsigned const cp = g - w.lcell[k];
+        unsigned const cp = has ? g - w.lcell[k] : 0u;
         unsigned xo, yo, zo;
         cell_coords(cp, cg, xo, yo, zo);
         uint32_t const* const off = offsets + d.cell_base;
-        unsigned const b = off[cp], e = off[cp + 1];
-        if (lane < 32) wc[lane] = 0u;
+        unsigned const b = has ? off[cp] : 0u, len = has ? off[cp + 1] - b : 0u;
+        unsigned const lmax = max(len, unsigned(__shfl_xor(int(len), 32, WAVE)));   // the longer cell of the pair
+        wc[hl] = 0u;
         __builtin_amdgcn_wave_barrier();
         bool far = false;
-        for (unsigned i0 = b; i0 < e; i0 += WAVE) {
-            unsigned const i = i0 + lane;
-            bool const act = i < e;
+        for (unsigned i0 = 0; i0 < lmax; i0 += 32) {
+            unsigned const i = b + i0 + hl;
+            bool const act = i0 + hl < len;
             unsigned code = 31u, c = 0;
             if (act) {
                 uint4 const q = posq[d.base + i];
@@ -1097,7 +1105,7 @@ __global__ __launch_bounds__(256) void ens_rank_kernel(uint4 const* __restrict__
                 else
                     code = step_code(ex, ey, ez);
             }
-            uint64_t peers = ~uint64_t(0);
+            uint64_t peers = mine;   // lanes of this half with the same code
 #pragma unroll
             for (unsigned bit = 0; bit < 5; ++bit) {
                 bool const on = (code >> bit) & 1u;
@@ -1116,8 +1124,9 @@ __global__ __launch_bounds__(256) void ens_rank_kernel(uint4 const* __restrict__
             }
             __builtin_amdgcn_wave_barrier();
         }
-        if (__ballot(far) != 0 && lane == 0) atomicOr(&skip[s], ENS_SKIP_FAR);
-        if (lane < 27) cnt27[(size_t(d.cell_base) + cp) * 27 + lane] = wc[lane];
+        if ((__ballot(far) & mine) != 0 && hl == 0) atomicOr(&skip[s], ENS_SKIP_FAR);
+        if (has && hl < 27) cnt27[(size_t(d.cell_base) + cp) * 27 + hl] = wc[hl];
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -1584,8 +1593,8 @@ void launch_merge_cell_order(uint4 const* posq, uint32_t const* old_offsets, uin
                              merge_scratch const& m, hipStream_t s) {
     ens_work const w = ens_work_at(m.work, 1);
     // grids sized to this one system (the kernels stride over any grid): a
-    // wave per old cell, a workgroup per block of cells or particles
-    unsigned const rank_grid = std::max(1u, std::min(div_up(m.ncells, 256 / WAVE), 65535u));
+    // wave per pair of old cells, a workgroup per block of cells or particles
+    unsigned const rank_grid = std::max(1u, std::min(div_up(m.ncells, 2 * (256 / WAVE)), 65535u));
     unsigned const cell_grid_n = std::max(1u, std::min(div_up(m.ncells, ENS_CELL_BLOCK), 65535u));
     unsigned const blk_grid = std::max(1u, std::min(div_up(m.n, ENS_BLOCK), 65535u));
     launch_fill_u32(m.flag, 0u, 1, s);
