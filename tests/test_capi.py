@@ -193,3 +193,15 @@ def test_serving_stream_statistics():
     assert {j["potential"] for j in jobs} == {"lj", "ka", "wca"}
     scaled = serving_stream(rate=50.0, duration=5.0, seed=7, step_scale=0.01)
     assert all(1 <= j["steps"] <= 1000 for j in scaled)
+
+
+def test_skin_per_potential():
+    # a skin for all systems, or one per potential name (the ensemble
+    # autotuner's result); the specs of the batched mix draw all three names
+    from halmd_amd.workloads import batched_specs, spec_skin
+    specs = batched_specs(count=32, seed=5)
+    assert {sp["potential"] for sp in specs} == {"lj", "wca", "morse"}
+    skins = {"lj": 0.35, "wca": 0.5, "morse": 0.3}
+    for sp in specs:
+        assert spec_skin(0.3, sp) == 0.3
+        assert spec_skin(skins, sp) == skins[sp["potential"]]
