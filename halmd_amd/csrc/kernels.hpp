@@ -35,6 +35,8 @@ constexpr size_t NOALLOC_SORT_MAX = size_t(1) << 21;
 void radix_sort_noalloc(uint32_t* keys, uint32_t* vals, size_t n, unsigned key_bits, hipStream_t s);
 
 void launch_iota(uint32_t* out, size_t n, hipStream_t s);
+// out[i] := i for i < n and dst[k] := src[k] for k < m, in one launch
+void launch_iota_copy_u32(uint32_t* out, size_t n, uint32_t const* src, uint32_t* dst, size_t m, hipStream_t s);
 void launch_fill_u32(uint32_t* out, uint32_t value, size_t n, hipStream_t s);
 // copy of bytes / 16 16-byte words (a multiple of 16 bytes)
 void launch_copy(void const* src, void* dst, size_t bytes, hipStream_t s);
@@ -592,9 +594,10 @@ void launch_mv2_partials(float4 const* vel, unsigned n, double* partials, hipStr
 // ---------------------------------------------------------------------------
 void launch_hilbert_map(float4 const* pos, unsigned n, float const* L, unsigned depth, uint32_t* code,
                         uint32_t* index, hipStream_t s);
+// rid (optional): rid[id] := the particle's new index (the reverse map)
 void launch_rearrange(float4 const* pos_in, float4 const* vel_in, int4 const* image_in, uint32_t const* id_in,
                       uint4 const* posq_in, uint32_t const* perm, unsigned n, float4* pos_out, float4* vel_out,
-                      int4* image_out, uint32_t* id_out, uint4* posq_out, hipStream_t s);
+                      int4* image_out, uint32_t* id_out, uint4* posq_out, hipStream_t s, uint32_t* rid = nullptr);
 void launch_reverse_id(uint32_t const* id, unsigned n, uint32_t* rid, hipStream_t s);
 
 // ---------------------------------------------------------------------------

@@ -424,13 +424,13 @@ void particle::rearrange(uint32_t const* perm) {
     // evaluated in the new order); one whose force is in memory is applied,
     // since the force array is not permuted
     if (force_valid) settle_velocity();
-    launch_rearrange(pos, vel, image, id, posq, perm, n, pos_alt, vel_alt, image_alt, id_alt, posq_alt, stream);
+    launch_rearrange(pos, vel, image, id, posq, perm, n, pos_alt, vel_alt, image_alt, id_alt, posq_alt, stream,
+                     reverse_id);
     std::swap(posq, posq_alt);
     std::swap(pos, pos_alt);
     std::swap(vel, vel_alt);
     std::swap(image, image_alt);
     std::swap(id, id_alt);
-    launch_reverse_id(id, n, reverse_id, stream);
     ++order_version;
     positions_changed();
     velocities_changed();
@@ -777,8 +777,8 @@ void binning::reorder_particles(bool sub_bins, double displacement_bound) {
         p.rearrange(vals);
         launch_cell_offsets(keys, p.n, ncells << SUB_BITS, order_sub_offset, s, 0);
     }
-    launch_iota(vals, p.n, s);        // sorted order is now memory order
-    launch_copy_u32(offset.get(), order_offset.get(), order_offset.size(), s);
+    // sorted order is now memory order; the offsets describe it
+    launch_iota_copy_u32(vals, p.n, offset.get(), order_offset.get(), order_offset.size(), s);
     timer.stop(s);
     // offsets describe the new order as well (the keys only after a sort)
     version = p.position_version;

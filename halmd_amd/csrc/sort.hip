@@ -186,6 +186,13 @@ __global__ void post_words_kernel(mailbox_sources src, unsigned n, unsigned long
                        __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+__global__ void iota_copy_kernel(uint32_t* __restrict__ out, size_t n, uint32_t const* __restrict__ src,
+                                 uint32_t* __restrict__ dst, size_t m) {
+    size_t const i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = uint32_t(i);
+    if (i < m) dst[i] = src[i];
+}
+
 __global__ void iota_kernel(uint32_t* out, size_t n) {
     size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) out[i] = uint32_t(i);
@@ -377,6 +384,13 @@ void radix_sort_noalloc(uint32_t* keys, uint32_t* vals, size_t n, unsigned key_b
     uint32_t* const counts = base + 2 * NOALLOC_SORT_MAX;
     uint32_t* const partial = counts + NOALLOC_COUNTS;
     radix_sort_on(keys, vals, n, key_bits, keys_alt, vals_alt, counts, scan_view{partial, level_offset, nlevel}, s);
+}
+
+void launch_iota_copy_u32(uint32_t* out, size_t n, uint32_t const* src, uint32_t* dst, size_t m, hipStream_t s) {
+    size_t const t = std::max(n, m);
+    if (!t) return;
+    iota_copy_kernel<<<div_up(t, 256), 256, 0, s>>>(out, n, src, dst, m);
+    HIP_CHECK(hipGetLastError());
 }
 
 void launch_iota(uint32_t* out, size_t n, hipStream_t s) {
