@@ -71,12 +71,15 @@ __global__ void rearrange_kernel(float4 const* __restrict__ pos_in, float4 const
                                  uint4 const* __restrict__ posq_in, uint32_t const* __restrict__ perm, unsigned n,
                                  float4* __restrict__ pos_out, float4* __restrict__ vel_out,
                                  int4* __restrict__ image_out, uint32_t* __restrict__ id_out,
-                                 uint4* __restrict__ posq_out, uint32_t* __restrict__ rid) {
+                                 uint4* __restrict__ posq_out, uint32_t* __restrict__ rid,
+                                 float4* __restrict__ r0) {
     unsigned const i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     size_t const j = perm[i];
     if (rid) rid[id_in[j]] = i;   // the reverse map of the new order
-    pos_out[2 * size_t(i)] = pos_in[2 * j];
+    float4 const hi = pos_in[2 * j];
+    if (r0) r0[i] = hi;
+    pos_out[2 * size_t(i)] = hi;
     pos_out[2 * size_t(i) + 1] = pos_in[2 * j + 1];
     vel_out[2 * size_t(i)] = vel_in[2 * j];
     vel_out[2 * size_t(i) + 1] = vel_in[2 * j + 1];
@@ -101,10 +104,11 @@ void launch_hilbert_map(float4 const* pos, unsigned n, float const* L, unsigned 
 
 void launch_rearrange(float4 const* pos_in, float4 const* vel_in, int4 const* image_in, uint32_t const* id_in,
                       uint4 const* posq_in, uint32_t const* perm, unsigned n, float4* pos_out, float4* vel_out,
-                      int4* image_out, uint32_t* id_out, uint4* posq_out, hipStream_t s, uint32_t* rid) {
+                      int4* image_out, uint32_t* id_out, uint4* posq_out, hipStream_t s, uint32_t* rid,
+                      float4* r0) {
     if (!n) return;
     rearrange_kernel<<<div_up(n, 256), 256, 0, s>>>(pos_in, vel_in, image_in, id_in, posq_in, perm, n, pos_out,
-                                                    vel_out, image_out, id_out, posq_out, rid);
+                                                    vel_out, image_out, id_out, posq_out, rid, r0);
     HIP_CHECK(hipGetLastError());
 }
 

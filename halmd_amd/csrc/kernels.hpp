@@ -594,10 +594,12 @@ void launch_mv2_partials(float4 const* vel, unsigned n, double* partials, hipStr
 // ---------------------------------------------------------------------------
 void launch_hilbert_map(float4 const* pos, unsigned n, float const* L, unsigned depth, uint32_t* code,
                         uint32_t* index, hipStream_t s);
-// rid (optional): rid[id] := the particle's new index (the reverse map)
+// rid (optional): rid[id] := the particle's new index (the reverse map);
+// r0 (optional): r0[i] := the hi word of the new position i
 void launch_rearrange(float4 const* pos_in, float4 const* vel_in, int4 const* image_in, uint32_t const* id_in,
                       uint4 const* posq_in, uint32_t const* perm, unsigned n, float4* pos_out, float4* vel_out,
-                      int4* image_out, uint32_t* id_out, uint4* posq_out, hipStream_t s, uint32_t* rid = nullptr);
+                      int4* image_out, uint32_t* id_out, uint4* posq_out, hipStream_t s, uint32_t* rid = nullptr,
+                      float4* r0 = nullptr);
 void launch_reverse_id(uint32_t const* id, unsigned n, uint32_t* rid, hipStream_t s);
 
 // ---------------------------------------------------------------------------

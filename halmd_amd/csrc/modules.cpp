@@ -419,13 +419,13 @@ void particle::evaluate_forces() {
     }
 }
 
-void particle::rearrange(uint32_t const* perm) {
+void particle::rearrange(uint32_t const* perm, float4* r0) {
     // a kick owed with a force not yet evaluated stays owed (the force will be
     // evaluated in the new order); one whose force is in memory is applied,
     // since the force array is not permuted
     if (force_valid) settle_velocity();
     launch_rearrange(pos, vel, image, id, posq, perm, n, pos_alt, vel_alt, image_alt, id_alt, posq_alt, stream,
-                     reverse_id);
+                     reverse_id, r0);
     std::swap(posq, posq_alt);
     std::swap(pos, pos_alt);
     std::swap(vel, vel_alt);
@@ -716,7 +716,7 @@ void binning::update(bool need_cell_pos) {
     }
 }
 
-void binning::reorder_particles(bool sub_bins, double displacement_bound) {
+void binning::reorder_particles(bool sub_bins, double displacement_bound, float4* r0) {
     hipStream_t s = p.stream;
     double cell_width = std::numeric_limits<double>::max();
     bool three = true;
@@ -756,12 +756,12 @@ void binning::reorder_particles(bool sub_bins, double displacement_bound) {
         timer.start(s);
         launch_merge_cell_order(p.posq, order_offset, offset, vals, m, s);
         merge_flag_pending = true;   // read by the list build that follows (merge_check_word)
-        p.rearrange(vals);
+        p.rearrange(vals, r0);
         ++merges;
     } else if (!sub_bins) {
         update(false);
         timer.start(s);
-        p.rearrange(vals);
+        p.rearrange(vals, r0);
         ++sorts;
     } else {
         // (cell, x sub-bin) order: a stable sort by the finer key keeps the
@@ -774,7 +774,7 @@ void binning::reorder_particles(bool sub_bins, double displacement_bound) {
         while ((1ull << (key_bits - SUB_BITS)) < ncells) ++key_bits;
         radix_sort(keys, vals, p.n, key_bits, ws, s);
         launch_cell_offsets(keys, p.n, ncells, offset, s, SUB_BITS);
-        p.rearrange(vals);
+        p.rearrange(vals, r0);
         launch_cell_offsets(keys, p.n, ncells << SUB_BITS, order_sub_offset, s, 0);
     }
     // sorted order is now memory order; the offsets describe it
@@ -869,8 +869,8 @@ max_displacement::~max_displacement() {
     }
 }
 
-void max_displacement::zero() {
-    launch_copy_hi(p.pos, r0, p.n, p.stream);
+void max_displacement::zero(bool r0_current) {
+    if (!r0_current) launch_copy_hi(p.pos, r0, p.n, p.stream);
     launch_fill_u32(d_max2, 0u, 3 * DISP_SET, p.stream);
     updates = 0;
     tracked_version = p.position_version;
@@ -1170,7 +1170,7 @@ void neighbour::require_current() {
         built && built_order == p.order_version && disp.computed_version == p.position_version ? disp.value : -1;
     for (auto& f : on_prepend_update) f();
     update(bound);
-    disp.zero();
+    disp.zero(staged);   // (the staged update's rearrangement wrote r0)
     built_order = p.order_version;
     built = true;
     for (auto& f : on_append_update) f();
@@ -1208,7 +1208,7 @@ void neighbour::update(double displacement_bound) {
     }
     // memory order := cell order (with x sub-bins for the windowed build),
     // then the segment lists
-    bin->reorder_particles(windows, displacement_bound);
+    bin->reorder_particles(windows, displacement_bound, disp.r0);
     while (true) {
         launch_fill_u32(d_maxcount, 0u, 2, s);
         timer.start(s);

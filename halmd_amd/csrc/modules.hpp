@@ -166,7 +166,9 @@ public:
     // used)
     bool hold_force = false;
     // reorder by a device permutation perm[new] = old
-    void rearrange(uint32_t const* perm);
+    // r0 (optional): also receives the new order's positions (hi words), the
+    // reference positions of a displacement tracker reset at the same point
+    void rearrange(uint32_t const* perm, float4* r0 = nullptr);
     void sync();
 
 private:
@@ -236,7 +238,8 @@ public:
     // negative value: unknown).  Below half the narrowest cell width, with no
     // rearrangement since, the new order is merged from the old one
     // (launch_merge_cell_order) instead of sorted.
-    void reorder_particles(bool sub_bins = false, double displacement_bound = -1);
+    // r0 (optional): passed to particle::rearrange
+    void reorder_particles(bool sub_bins = false, double displacement_bound = -1, float4* r0 = nullptr);
     uint64_t merges = 0, sorts = 0;    // reorders of each kind
     bool merge_enabled = true;
     dbuf<uint32_t> order_sub_offset;   // (ncells << SUB_BITS) + 1 entries
@@ -294,7 +297,10 @@ public:
     uint64_t tracked_version = 0;     // position version whose maxima the last update wrote
     uint64_t computed_version = 0;
     double value = 0;
-    void zero();
+    // reference positions := the current ones and maxima cleared; with
+    // r0_current the reference positions were already written (a
+    // rearrangement that wrote them)
+    void zero(bool r0_current = false);
     double compute();
     // the set the next position update writes and the set it zeroes; marks
     // the resulting position version as tracked
