@@ -293,7 +293,24 @@ def tiny(args):
     print(json.dumps(out), flush=True)
 
 
-def serving_run(rate, duration, seed, step_scale, tick=50, max_jobs=256, budget_gib=128):
+# the skin of each served job by potential, the same on the plain path: the
+# ensemble autotuner's per-potential choice on the batched mix (WCA 0.7, LJ
+# 0.35); Kob-Andersen keeps 0.3
+SERVING_SKINS = {"lj": 0.35, "ka": 0.3, "wca": 0.7}
+
+
+def parse_skins(text):
+    """'lj:0.35,ka:0.3,wca:0.7' -> the mapping (None: the defaults)."""
+    if not text:
+        return None
+    out = dict(SERVING_SKINS)
+    for part in text.split(","):
+        name, value = part.split(":")
+        out[name.strip()] = float(value)
+    return out
+
+
+def serving_run(rate, duration, seed, step_scale, tick=50, max_jobs=256, budget_gib=128, skins=None):
     """Served job stream: Poisson arrivals of LJ / Kob-Andersen / WCA jobs (N
     log-normal, median 2e4, clipped to [1e3, 2e5]; rho in [0.8, 1.2]; T in
     [0.45, 1.5]; steps log-uniform in [1e3, 1e5] times step_scale), each with
@@ -308,6 +325,7 @@ def serving_run(rate, duration, seed, step_scale, tick=50, max_jobs=256, budget_
     from halmd_amd.observables.utility import wavevector
     from halmd_amd.serve import server
     stream = workloads.serving_stream(rate=rate, duration=duration, seed=seed, step_scale=step_scale)
+    skins = dict(SERVING_SKINS if skins is None else skins)   # per potential, the same on both paths
     dt = 0.003
     k = list(workloads.SERVING_WAVENUMBERS)
     corr = dict(block_size=10, shift=2, separation=10, wavenumber=k, tolerance=0.01, max_count=7)
@@ -328,7 +346,8 @@ def serving_run(rate, duration, seed, step_scale, tick=50, max_jobs=256, budget_
             j = stream[q]
             L, r0, v0, _ = inputs[q]
             ids[q] = s.submit(r0, v0, L[0], j["steps"], potential=j["potential"], timestep=dt,
-                              species=workloads.spec_species(j), arrival=start + j["arrival"], **corr)
+                              skin=skins[j["potential"]], species=workloads.spec_species(j),
+                              arrival=start + j["arrival"], **corr)
         st = s.stats()
         if not st["active"] and not st["queued"]:
             time.sleep(max(0.0, stream[pending[0]]["arrival"] - (s.now() - start)))
@@ -358,7 +377,7 @@ def serving_run(rate, duration, seed, step_scale, tick=50, max_jobs=256, budget_
         if now < j["arrival"]:
             time.sleep(j["arrival"] - now)
         steps = -(-j["steps"] // tick) * tick   # whole ticks, as the server runs them
-        w = workloads.single_from_spec(j, timestep=dt)
+        w = workloads.single_from_spec(j, timestep=dt, skin=skins)
         bs = blocking_scheme(w.box, w.particle, timestep=dt, max_lag=steps * dt, resolution=tick * dt,
                              block_size=10, shift=2, separation=10)
         bs.correlate_msd()
@@ -379,7 +398,7 @@ def serving_run(rate, duration, seed, step_scale, tick=50, max_jobs=256, budget_
         "workload": f"Poisson {rate:g} jobs/s for {duration:g} s ({len(stream)} jobs): LJ / KA / WCA, "
                     "N log-normal (median 2e4, log-sigma 0.75, clipped [1e3, 2e5]), rho U[0.8, 1.2], "
                     f"T U[0.45, 1.5], steps log-uniform [1e3, 1e5] x {step_scale:g} (whole ticks of {tick}), "
-                    f"dt {dt}, MSD + ISF at 8 wavenumbers (blocking scheme, block 10)",
+                    f"dt {dt}, {skin_text(skins)}, MSD + ISF at 8 wavenumbers (blocking scheme, block 10)",
         "jobs": len(stream),
         "offered_particle_steps_per_s": offered,
         "particle_steps_per_s": served,
@@ -404,7 +423,7 @@ def serving_run(rate, duration, seed, step_scale, tick=50, max_jobs=256, budget_
 def serving(args):
     """--workload serving: the serving figures as the JSON line."""
     r = serving_run(args.rate, args.duration, args.seed, args.step_scale, args.tick, args.max_jobs,
-                    args.budget_gib)
+                    args.budget_gib, parse_skins(args.serving_skins))
     out = {
         "metric": "served job stream: aggregate particle-steps/s and p50/p99 job latency vs plain per-job GPU path",
         "value": r["particle_steps_per_s"],
@@ -502,6 +521,8 @@ def main():
                     help="serving: factor on the job step counts (0.2 keeps the offered load, about 5e9 "
                          "particle-steps/s, above what one GPU serves, as the full-length stream's is)")
     ap.add_argument("--tick", type=int, default=50, help="serving: steps per tick")
+    ap.add_argument("--serving-skins", default=None,
+                    help="serving: skin per potential, e.g. 'lj:0.35,ka:0.3,wca:0.7' (both paths)")
     ap.add_argument("--max-jobs", type=int, default=256, help="serving: running-job cap")
     ap.add_argument("--budget-gib", type=int, default=128, help="serving: device memory budget for jobs")
     ap.add_argument("--systems", type=int, default=256, help="systems of the batched workload")
@@ -615,7 +636,7 @@ def main():
         extras["batched"] = batched_run(args.systems, args.batched_steps, 100, args.seed)
         log("served stream figure")
         extras["serving"] = serving_run(args.rate, args.duration, args.seed, args.step_scale, args.tick,
-                                        args.max_jobs, args.budget_gib)
+                                        args.max_jobs, args.budget_gib, parse_skins(args.serving_skins))
         log("reference benchmark setups")
         extras["reference_benchmarks"] = {name: reference_setup_run(name, 2000, 200, args.seed)
                                           for name in REFERENCE_SETUPS}
