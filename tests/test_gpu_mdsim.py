@@ -61,14 +61,22 @@ def test_cell_lists_bit_exact(oracle, n, rho):
     assert np.array_equal(begin, np.concatenate([[0], np.cumsum(counts)[:-1]]))
 
 
-@pytest.mark.parametrize("kind", ["lj", "ka"])
+@pytest.mark.parametrize("kind", ["lj", "ka", "lj_3cells", "lj_4cells"])
 def test_neighbour_lists_bit_exact(oracle, kind):
     if kind == "lj":
         w = workloads.lennard_jones(6912, density=0.8442, jitter=0.15, seed=11)
-    else:
+    elif kind == "ka":
         w = workloads.kob_andersen(8788, density=1.2, jitter=0.15, seed=12)
+    else:
+        # the smallest boxes on cell segments: 3 cells a side (every cell is
+        # in every neighbourhood, and the 16-bit minimum image picks the
+        # nearest image across the whole box) and 4
+        n = 864 if kind == "lj_3cells" else 1372
+        w = workloads.lennard_jones(n, density=0.8442, jitter=0.15, seed=13)
     w.sampler.run(37)    # positions written by the integrate kernel
     ncell = w.neighbour.binning.ncell
+    if kind.endswith("cells"):
+        assert tuple(ncell) == (int(kind[3]),) * 3
     rcs = w.potential.r_cut + w.skin
     # rebuild from the current positions (which reorders the particles into
     # cell order), then compare with the rule on them in that memory order
@@ -284,6 +292,24 @@ def test_lj_forces_match_fp64(oracle, truncation):
     assert np.max(np.abs(u - uref)) < 1e-5 * np.max(np.abs(uref))
     assert abs(tr - trref) / abs(trref) < VIRIAL_TOL * widen
     assert np.max(np.abs(s - sref)) < 1e-5 * np.max(np.abs(sref))
+
+
+@pytest.mark.parametrize("n,ncell", [(864, 3), (1372, 4)])
+def test_lj_forces_smallest_cell_boxes_match_fp64(oracle, n, ncell):
+    # 3 and 4 cells of rc + skin a side on cell segments: the minimum image
+    # of the fixed-point differences across the whole box, in the list build
+    # and in the force kernel
+    w = workloads.lennard_jones(n, density=0.8442, temperature=1.2, jitter=0.05, seed=22)
+    assert tuple(w.neighbour.binning.ncell) == (ncell,) * 3
+    pos, species = _thermalised(w)
+    pot = oracle.Potential(oracle.POT_LJ, "shifted", 1.0, 1.0, 2.5, h=0.005)
+    fref, uref, sref = oracle.pair_force(pos, species, w.length, pot, skin=0.3)
+    f = w.particle.data["force"]
+    print(f"N={n}: max|dF|/rms = {_force_error(f, fref):.3e}, median rel = {_median_rel(f, fref):.3e}")
+    assert _force_error(f, fref) < FORCE_TOL
+    assert _median_rel(f, fref) < MEDIAN_TOL
+    u = w.particle.data["potential_energy"]
+    assert abs(u.sum() - uref.sum()) / abs(uref.sum()) < EN_POT_TOL
 
 
 def test_ka_forces_match_fp64(oracle):
