@@ -427,8 +427,16 @@ __device__ __forceinline__ float staged_force_segment(
                 dz = float(Dz);
                 float const rq = dx * dx + dy * dy + dz * dz;
                 inside = rq < pp.rr_cut_q;
-                float const r = rcp_nr(rq);
-                float const sr2 = __builtin_fmaf(r, pp.p_lo[0], r * pp.p_lo[1]);   // (sigma / r)^2
+                // (sigma / r)^2 = c / rq with c = sigma^2 / q^2: c w (1 + e), w the
+                // hardware reciprocal of rq and e = 1 - rq w its residual (the
+                // Newton step of rcp_nr folded into the scaling: four
+                // instructions for five), c as the float nearest it (the same
+                // potential with sigma^2 scaled by a fixed factor within 2^-24
+                // of 1)
+                float const w = __builtin_amdgcn_rcpf(rq);
+                float const e = __builtin_fmaf(-rq, w, 1.0f);
+                float const cw = w * pp.p_lo[0];
+                float const sr2 = __builtin_fmaf(cw, e, cw);
                 float const x2 = sr2 * sr2;
                 float const x4 = x2 * x2;
                 float const h = __builtin_fmaf(x2, sr2, -0.5f);                    // (sigma / r)^6 - 1/2

@@ -261,7 +261,8 @@ enum truncation_kind : int { TRUNC_SHARP = 0, TRUNC_SHIFTED = 1, TRUNC_FORCE_SHI
 struct pair_param {
     float p[4];        // LJ: {48 eps / sigma^2, 4 eps, sigma^2, -}; Morse: {eps/A, 1/(sigma B), r_min_sigma/B, A};
                        // power law / Mie: {c_m, c_n, sigma^2, -}
-    float p_lo[4];     // LJ, cubic boxes: sigma^2 / q^2 as a double word in p_lo[0], p_lo[1];
+    float p_lo[4];     // LJ, cubic boxes: sigma^2 / q^2 as a double word in p_lo[0], p_lo[1] (the
+                       // kernels take the nearest float, p_lo[0]);
                        // power law / Mie: the indices {m, n} (exact small integers)
     float rr_cut, r_cut, en_cut, force_cut;
     float rr_cut_q;    // rr_cut in fixed-point units squared, rr_cut / q^2 (cubic boxes)
